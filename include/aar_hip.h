@@ -1,0 +1,125 @@
+/*
+ * aar_hip.h -- extension entry points of libaar_hip (no MPI dependency).
+ *
+ * The reference's solvers take host arrays and a POISSON descriptor
+ * (AAR.h, PGR.h, PL2R.h, CG.h in this directory keep those signatures).
+ * The entry points below are their device-resident counterparts: every
+ * vector argument is a DEVICE pointer into HBM (hipMalloc'd or a framework
+ * tensor), laid out x fastest (idx = i + j*nx + k*nx*ny), and the whole
+ * solve runs on the current GPU on the library stream (aar_hip_set_stream).
+ * Each is the GPU form of the reference routine named in its comment; the
+ * screened term sigma, the FP32-refined solver and the diagnostics have no
+ * reference counterpart and say so.
+ *
+ * All functions return 0 on success or an AAR_HIP_ERR_* code; the text of
+ * the last error is returned by aar_hip_last_error().
+ */
+#ifndef AAR_HIP_H
+#define AAR_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AAR_HIP_OK 0
+#define AAR_HIP_ERR_ARG 1        /* invalid argument (sizes, m, fdn, null pointers) */
+#define AAR_HIP_ERR_HIP 2        /* HIP runtime error (allocation, launch) */
+#define AAR_HIP_ERR_SVD 3        /* SVD did not converge in 250 sweeps (tools.c:270 exits) */
+#define AAR_HIP_ERR_CALLBACK 4   /* operator/preconditioner callback is not libaar_hip's */
+#define AAR_HIP_ERR_NONFINITE 5  /* residual became NaN */
+
+/* Grid and operator: y = a*Lap_FD(2*fdn)(x) + sigma*x, periodic.
+ * fdn must be 6 (the 12th-order, radius-6 stencil of MPI_REAL/main.c:35).
+ * sigma = 0 is the reference operator (system.c:189-205); sigma != 0 is
+ * the screened operator (extension).  The Jacobi diagonal is
+ * a*(3*coeff[0]) + sigma (AAR.c:94 for sigma = 0). */
+typedef struct {
+    int nx, ny, nz;
+    int fdn;
+    double a;
+    double sigma;
+} aar_hip_grid;
+
+typedef struct {
+    int iterations;         /* AAR/PGR/PL2R: iter-1 (AAR.c:153); CG: iter_count (CG.c:159) */
+    double relres;          /* last checked ||b - A x|| / ||b|| */
+    double rhs_norm;        /* ||b|| */
+    int converged;          /* relres <= tol at exit */
+    int printed_converged;  /* the reference's "converged!" condition (AAR.c:152) */
+    int printed_warning;    /* the reference's "exceeded maximum iterations" condition */
+    int status;             /* 0 or an AAR_HIP_ERR_* code */
+    int hist_len;           /* entries available from aar_hip_history() */
+    double solve_ms;        /* GPU time of the solve (hipEvents on the library stream) */
+    double bytes;           /* algorithmic HBM bytes moved by the solve */
+    int outer_steps;        /* FP32-refined solver: FP64 residual-refinement steps */
+    int inner_iterations;   /* FP32-refined solver: total FP32 AAR iterations */
+} aar_hip_stats;
+
+/* system.c:189-205 -- y = a*Lap(x) + sigma*x (device pointers, FP64) */
+int aar_hip_matvec(const aar_hip_grid *g, const double *x, double *y);
+/* same in FP32 (the arithmetic of the refined solver's inner sweeps) */
+int aar_hip_matvec_f32(const aar_hip_grid *g, const float *x, float *y);
+
+/* AAR.c:37-177 -- x: initial guess in, solution out */
+int aar_hip_aar(const aar_hip_grid *g, double *x, const double *b, double omega, double beta,
+                int m, int p, int max_iter, double tol, int precond, aar_hip_stats *st);
+
+/* AAR.c:37-177 in FP32 (normal equations and the m x m solve stay FP64) */
+int aar_hip_aar_f32(const aar_hip_grid *g, float *x, const float *b, double omega, double beta,
+                    int m, int p, int max_iter, double tol, int precond, aar_hip_stats *st);
+
+/* PGR.c:35-173 (l2 = 0) and PL2R.c:35-172 (l2 = 1) */
+int aar_hip_pgr(const aar_hip_grid *g, double *x, const double *b, double omega, int m, int p,
+                int max_iter, double tol, int precond, int l2, aar_hip_stats *st);
+
+/* CG.c:33-175 */
+int aar_hip_cg(const aar_hip_grid *g, double *x, const double *b, int max_iter, double tol,
+               int precond, aar_hip_stats *st);
+
+/* Extension (no reference counterpart): FP64 residual refinement around
+ * FP32 AAR.  Outer: r = b - A x in FP64, stop at ||r|| <= tol*||b||;
+ * inner: FP32 AAR on A e = r from e = 0 to relative tolerance
+ * max(inner_tol, 0.5*tol*||b||/||r||); x += e in FP64. */
+int aar_hip_aar_refined(const aar_hip_grid *g, double *x, const double *b, double omega,
+                        double beta, int m, int p, int max_outer, int max_inner, double tol,
+                        double inner_tol, int precond, aar_hip_stats *st);
+
+/* tools.c:52-105 on the GPU: x = pinv(A) b as the reference defines it,
+ * A row-major m x m (A[i*m+j] = A[i][j]), m <= 16.  HOST pointers.
+ * w_out, v_out may be NULL. */
+int aar_hip_pinv(const double *A, const double *b, double *x, int m, double *w_out, double *v_out);
+
+/* Residual history of the last solve: ||b - A x_k|| for every iterate
+ * (AAR: entry k is the residual of the iterate entering iteration k+1;
+ * PGR/PL2R: of the Richardson iterate before projection; CG: of r_k).
+ * Copies min(cap, hist_len) entries to host memory; returns the count. */
+int aar_hip_history(double *out, int cap);
+
+/* Statistics of the last solve started by any entry point, including the
+ * reference-signature ones in AAR.h/PGR.h/PL2R.h/CG.h. */
+int aar_hip_last_stats(aar_hip_stats *st);
+
+/* Stream for all work (a hipStream_t); NULL restores the library's own. */
+int aar_hip_set_stream(void *stream);
+
+/* Stencil launch shape: lz planes per block (0 = automatic), py rows per
+ * thread (2 or 4).  Applies to subsequent calls. */
+int aar_hip_set_tuning(int lz, int py);
+
+/* Per-kernel GPU timing with hipEvents (off by default).  Kernel names:
+ * "stencil_matvec", "stencil_aar_rich", "stencil_resid", "stencil_pgr_rich",
+ * "stencil_cg_q", "stencil_cg_resid", "gram", "ls", "aar_update",
+ * "pgr_update", "cg_update". */
+int aar_hip_profile(int enable);
+int aar_hip_profile_reset(void);
+int aar_hip_profile_query(const char *name, int *count, double *total_ms, double *bytes);
+
+/* Release cached device workspace. */
+int aar_hip_release(void);
+
+const char *aar_hip_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

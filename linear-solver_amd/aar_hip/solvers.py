@@ -1,0 +1,183 @@
+"""Python mirror of the reference solver interface over libaar_hip.
+
+Names and argument meaning follow MPI_REAL (AAR.h:18-22, PGR.h:18-22,
+PL2R.h:18-22, CG.h:18-21, system.h:82): ``x`` is the initial guess and is
+overwritten with the solution, ``rhs``/``b`` is the right-hand side, ``a``
+scales the Laplacian (y = a*Lap(x) + sigma*x), Jacobi preconditioning with
+the constant diagonal a*(3*coeff[0]) + sigma unless ``precond=False``
+(the reference's Precondition == NULL).  Vectors are torch tensors on the
+GPU in the reference layout (x fastest; a C-ordered (nz, ny, nx) tensor or
+its flat view).  The grid is given as ``shape = (nx, ny, nz)``.
+
+Each solver returns the statistics the reference prints (iterations,
+relative residual, converged / warning conditions) plus GPU time and
+algorithmic HBM bytes.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import Grid, Stats, check
+
+
+def _L():
+    return _lib.load()
+
+
+def _bind_stream():
+    import torch
+    _L().aar_hip_set_stream(ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+
+
+def _dev(t, dtype, name):
+    import torch
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError(f"{name} must be a CUDA/HIP tensor")
+    if t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _grid(shape, a, sigma=0.0, fdn=6):
+    nx, ny, nz = (int(s) for s in shape)
+    return Grid(nx, ny, nz, fdn, float(a), float(sigma))
+
+
+def _check_len(t, shape, name):
+    n = int(np.prod(shape))
+    if t.numel() != n:
+        raise ValueError(f"{name} has {t.numel()} elements, grid {tuple(shape)} needs {n}")
+
+
+def Lap_Vec_mult(x, shape, a, sigma=0.0, out=None):
+    """system.c:80-213 -- y = a*Lap(x) (+ sigma*x), FP64 or FP32."""
+    import torch
+    _check_len(x, shape, "x")
+    y = torch.empty_like(x) if out is None else out
+    g = _grid(shape, a, sigma)
+    _bind_stream()
+    if x.dtype == torch.float64:
+        check(_L().aar_hip_matvec(ctypes.byref(g), _dev(x, torch.float64, "x"),
+                                  _dev(y, torch.float64, "y")))
+    elif x.dtype == torch.float32:
+        check(_L().aar_hip_matvec_f32(ctypes.byref(g), _dev(x, torch.float32, "x"),
+                                      _dev(y, torch.float32, "y")))
+    else:
+        raise TypeError("x must be float64 or float32")
+    return y
+
+
+def AAR(x, rhs, shape, a, omega, beta, m, p, max_iter, tol, sigma=0.0, precond=True):
+    """AAR.c:37-177 (FP64 tensors), or the same iteration in FP32 (FP32 tensors;
+    normal equations and the m x m solve stay FP64)."""
+    import torch
+    _check_len(x, shape, "x")
+    _check_len(rhs, shape, "rhs")
+    g = _grid(shape, a, sigma)
+    st = Stats()
+    _bind_stream()
+    if x.dtype == torch.float64:
+        check(_L().aar_hip_aar(ctypes.byref(g), _dev(x, torch.float64, "x"),
+                               _dev(rhs, torch.float64, "rhs"), omega, beta, m, p, max_iter, tol,
+                               int(bool(precond)), ctypes.byref(st)))
+    else:
+        check(_L().aar_hip_aar_f32(ctypes.byref(g), _dev(x, torch.float32, "x"),
+                                   _dev(rhs, torch.float32, "rhs"), omega, beta, m, p, max_iter,
+                                   tol, int(bool(precond)), ctypes.byref(st)))
+    return st.as_dict()
+
+
+def PGR(x, rhs, shape, a, omega, m, p, max_iter, tol, sigma=0.0, precond=True, _l2=False):
+    """PGR.c:35-173."""
+    import torch
+    _check_len(x, shape, "x")
+    _check_len(rhs, shape, "rhs")
+    g = _grid(shape, a, sigma)
+    st = Stats()
+    _bind_stream()
+    check(_L().aar_hip_pgr(ctypes.byref(g), _dev(x, torch.float64, "x"),
+                           _dev(rhs, torch.float64, "rhs"), omega, m, p, max_iter, tol,
+                           int(bool(precond)), int(bool(_l2)), ctypes.byref(st)))
+    return st.as_dict()
+
+
+def PL2R(x, rhs, shape, a, omega, m, p, max_iter, tol, sigma=0.0, precond=True):
+    """PL2R.c:35-172."""
+    return PGR(x, rhs, shape, a, omega, m, p, max_iter, tol, sigma, precond, _l2=True)
+
+
+def CG(x, b, shape, a, max_iter, tol, sigma=0.0, precond=True):
+    """CG.c:33-175."""
+    import torch
+    _check_len(x, shape, "x")
+    _check_len(b, shape, "b")
+    g = _grid(shape, a, sigma)
+    st = Stats()
+    _bind_stream()
+    check(_L().aar_hip_cg(ctypes.byref(g), _dev(x, torch.float64, "x"), _dev(b, torch.float64, "b"),
+                          max_iter, tol, int(bool(precond)), ctypes.byref(st)))
+    return st.as_dict()
+
+
+def AAR_refined(x, rhs, shape, a, omega, beta, m, p, tol, inner_tol=1e-5, max_outer=30,
+                max_inner=100000, sigma=0.0, precond=True):
+    """FP64 residual refinement around FP32 AAR (no reference counterpart)."""
+    import torch
+    _check_len(x, shape, "x")
+    _check_len(rhs, shape, "rhs")
+    g = _grid(shape, a, sigma)
+    st = Stats()
+    _bind_stream()
+    check(_L().aar_hip_aar_refined(ctypes.byref(g), _dev(x, torch.float64, "x"),
+                                   _dev(rhs, torch.float64, "rhs"), omega, beta, m, p, max_outer,
+                                   max_inner, tol, inner_tol, int(bool(precond)), ctypes.byref(st)))
+    return st.as_dict()
+
+
+def pinv(A, b):
+    """tools.c:52-105 on the GPU: x = V diag(1/w) V^T b, returns (x, w, V)."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    m = A.shape[0]
+    x = np.zeros(m)
+    w = np.zeros(m)
+    V = np.zeros((m, m))
+    dp = lambda arr: arr.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    check(_L().aar_hip_pinv(dp(A), dp(b), dp(x), m, dp(w), dp(V)))
+    return x, w, V
+
+
+def history(cap=1 << 20):
+    """Residual history ||b - A x_k|| of the last solve (see aar_hip.h)."""
+    buf = np.zeros(cap)
+    n = _L().aar_hip_history(buf.ctypes.data_as(ctypes.c_void_p), cap)
+    return buf[:n].copy()
+
+
+def set_tuning(lz=0, py=4):
+    check(_L().aar_hip_set_tuning(int(lz), int(py)))
+
+
+def profile(enable=True):
+    _L().aar_hip_profile(int(bool(enable)))
+
+
+def profile_reset():
+    _L().aar_hip_profile_reset()
+
+
+def profile_query(name):
+    c = ctypes.c_int()
+    ms = ctypes.c_double()
+    by = ctypes.c_double()
+    _L().aar_hip_profile_query(name.encode(), ctypes.byref(c), ctypes.byref(ms), ctypes.byref(by))
+    return {"count": c.value, "ms": ms.value, "bytes": by.value}
+
+
+def release():
+    _L().aar_hip_release()

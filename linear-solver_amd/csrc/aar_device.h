@@ -1,0 +1,139 @@
+// aar_device.h -- device helpers shared by the kernels of libaar_hip.
+//
+// Reductions are deterministic: every block reduces its own values in a
+// fixed butterfly/tree order into a per-work-item slot, and the last block
+// to finish sums the slots in a fixed order.  Results therefore do not
+// depend on block scheduling, and repeated solves are bit-reproducible.
+#pragma once
+
+#include "aar_common.h"
+
+namespace aar {
+
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Reduce NACC per-thread accumulators over the block; thread k < NACC
+// writes dst[k].  `scratch` must hold NACC * (BLOCK/64) doubles.
+template <int NACC>
+__device__ inline void block_reduce_to(const double (&acc)[NACC], double *dst, double *scratch)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    constexpr int NW = BLOCK / 64;
+#pragma unroll
+    for (int k = 0; k < NACC; k++) {
+        double v = wave_sum(acc[k]);
+        if (lane == 0) scratch[k * NW + wv] = v;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < NACC; k += BLOCK) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) s += scratch[k * NW + w];
+        dst[k] = s;
+    }
+}
+
+// Signal completion; returns true in exactly one block (the last one),
+// after which all other blocks' global writes are visible to it.
+__device__ inline bool last_block_arrive(unsigned *counter, unsigned total)
+{
+    __shared__ int am_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned prev = atomicAdd(counter, 1u);
+        am_last = (prev == total - 1);
+    }
+    __syncthreads();
+    if (am_last) __threadfence();
+    return am_last != 0;
+}
+
+// Sum partials[item * nacc + k] over items in a fixed order; result in
+// out[k] (shared or global), valid for thread 0 after return.
+__device__ inline void final_sum(const double *partials, int nitems, int nacc, double *out, double *scratch)
+{
+    for (int k = 0; k < nacc; k++) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < nitems; i += BLOCK) s += partials[(size_t)i * nacc + k];
+        s = wave_sum(s);
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        if (lane == 0) scratch[wv] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int w = 0; w < BLOCK / 64; w++) t += scratch[w];
+            out[k] = t;
+        }
+        __syncthreads();
+    }
+}
+
+// Finalisers (thread 0 of the last block).
+__device__ inline void finalize(const FinArgs &f, Ctrl *c, const double *sums, int nacc)
+{
+    switch (f.fin) {
+    case FIN_SUMS:
+        for (int k = 0; k < nacc && k < 8; k++) c->red[k] = sums[k];
+        break;
+    case FIN_NORM_B: {
+        const double nb = sqrt(sums[0]);
+        c->rhs_norm = nb;
+        c->tol_abs = f.tol * nb;          // AAR.c:79 (tol *= rhs_norm)
+        // AAR.c:76 starts relres at tol+1 (unscaled); the loop does not run
+        // when that is already within the scaled tolerance.
+        if (f.check && !(f.tol + 1.0 > c->tol_abs)) {
+            c->done = 1;
+            c->final_iter = 0;
+            c->relres = f.tol + 1.0;
+        }
+        break;
+    }
+    case FIN_AAR_RES:
+    case FIN_PGR_RES: {
+        const double r = sqrt(sums[0]);
+        if (f.hist && f.it - 1 >= 0 && f.it - 1 < f.hist_cap) f.hist[f.it - 1] = r;
+        if (f.check) {
+            c->relres = r;
+            if (!(r == r)) {            // NaN residual: stop, flag it
+                c->status = 2;
+                c->done = 1;
+                c->final_iter = f.it - 1;
+            } else if (r <= c->tol_abs) {
+                c->done = 1;
+                c->final_iter = f.it - 1;
+            }
+        }
+        break;
+    }
+    case FIN_CG_ALPHA:
+        c->alpha = c->delta_new / sums[0];   // CG.c:100-104
+        break;
+    case FIN_CG_BETA: {
+        c->delta_old = c->delta_new;         // CG.c:131
+        c->delta_new = sums[0];              // CG.c:134
+        const double rr = sqrt(sums[1]);     // CG.c:137
+        c->relres = rr;
+        c->beta = c->delta_new / c->delta_old;  // CG.c:148
+        if (f.hist && f.it >= 0 && f.it < f.hist_cap) f.hist[f.it] = rr;
+        if (!(rr == rr)) {
+            c->status = 2;
+            c->done = 1;
+            c->final_iter = f.it;
+        } else if (!(rr > c->tol_abs)) {
+            c->done = 1;
+            c->final_iter = f.it;
+        }
+        break;
+    }
+    default:
+        break;
+    }
+}
+
+}  // namespace aar

@@ -1,0 +1,89 @@
+// aar_solver.h -- host-side context and solver entry points (internal).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "aar_common.h"
+#include "aar_hip.h"
+
+namespace aar {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct ProfEntry {
+    int count = 0;
+    double ms = 0.0;
+    double bytes = 0.0;
+};
+
+struct PendingEv {
+    std::string name;
+    hipEvent_t a = nullptr, b = nullptr;
+    double bytes = 0.0;
+};
+
+constexpr int NPOLL = 64;
+
+// One per process: the library is single-GPU and, like the reference,
+// not re-entrant across threads.
+struct Context {
+    bool init = false;
+    hipStream_t own = nullptr, stream = nullptr;
+    Ctrl *ctrl = nullptr;        // [2]: 0 outer / plain solves, 1 inner FP32 solves
+    Ctrl *ctrl_h = nullptr;      // pinned host mirror [2]
+    int *poll_h = nullptr;       // pinned polling slots
+    hipEvent_t poll_ev[NPOLL] = {};
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    unsigned *counters = nullptr;   // last-block counters (64)
+    DevBuf ring64x, ring64f, ring32x, ring32f;
+    DevBuf hist0, hist1, part0, part1;
+    DevBuf vec32a, vec32b, vec64a, vec64b;
+    std::string err;
+    int lz = 0, py = 4;
+    bool prof = false;
+    std::map<std::string, ProfEntry> profile;
+    std::vector<PendingEv> pending;
+    std::vector<hipEvent_t> evpool;
+    aar_hip_stats last = {};
+    std::vector<double> hist_host;
+    double bytes_acc = 0.0;
+};
+
+struct Op {
+    int nx = 0, ny = 0, nz = 0;
+    long n = 0;
+    double w[R + 1] = {};
+    double sigma = 0.0;
+    double diag = 1.0;
+};
+
+Context &ctx();
+int context_init();
+int fail(int code, const std::string &msg);
+void lap_coefficient(double *c, int fdn);
+int make_op(const aar_hip_grid *g, const double *coeff, Op &op);
+
+template <typename T>
+int aar_solve(const Op &op, T *x, const T *b, double omega, double beta, int m, int p, int max_iter,
+              double tol, int precond, int ci, aar_hip_stats *st, int keep_history);
+int pgr_solve(const Op &op, double *x, const double *b, double omega, int m, int p, int max_iter,
+              double tol, int precond, int l2, aar_hip_stats *st);
+int cg_solve(const Op &op, double *x, const double *b, int max_iter, double tol, int precond,
+             aar_hip_stats *st);
+template <typename T>
+int matvec(const Op &op, const T *x, T *y);
+int aar_refined(const Op &op, double *x, const double *b, double omega, double beta, int m, int p,
+                int max_outer, int max_inner, double tol, double inner_tol, int precond,
+                aar_hip_stats *st);
+int pinv_host(const double *A, const double *b, double *x, int m, double *w_out, double *v_out);
+void release_all();
+void prof_flush();
+
+}  // namespace aar

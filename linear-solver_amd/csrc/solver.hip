@@ -1,0 +1,943 @@
+// solver.hip -- host drivers of the device-resident solvers (libaar_hip).
+//
+// Storage (per solve, cached across calls):
+//   X ring: m+1 slots of N -- iterates x_k in slot k % (m+1)
+//   F ring: m+1 slots of N -- AAR: preconditioned residuals f_k;
+//                             PGR/PL2R: A x_k
+//   b, and a control block (Ctrl) + residual history in device memory.
+// A difference column of the reference (DX[c] = x - x_old, DF[c] = f -
+// f_old, AAR.c:96-102, PGR.c:105-109) is the difference of two ring slots
+// and is formed on the fly, so a Richardson step writes 2 vectors instead
+// of the reference's 4 (x, f, DX[c], DF[c]) plus 2 copies.
+//
+// Control: the host enqueues iterations following the reference's static
+// schedule (Anderson / projection every p-th iteration); convergence is
+// decided on the GPU by the last block of the reducing kernel, which sets
+// ctrl->done; every later kernel returns at once.  The host polls `done`
+// with a lag of two Anderson cycles, so the GPU never waits for the host.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "aar_common.h"
+#include "aar_kernels.h"
+#include "aar_solver.h"
+
+namespace aar {
+
+static Context g_ctx;
+Context &ctx() { return g_ctx; }
+
+static hipError_t ensure(DevBuf &b, size_t bytes)
+{
+    if (b.p && b.bytes >= bytes) return hipSuccess;
+    if (b.p) {
+        hipError_t e = hipFree(b.p);
+        b.p = nullptr;
+        b.bytes = 0;
+        if (e != hipSuccess) return e;
+    }
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e == hipSuccess) b.bytes = bytes;
+    return e;
+}
+
+int fail(int code, const std::string &msg)
+{
+    ctx().err = msg;
+    return code;
+}
+
+#define HC(x)                                                                         \
+    do {                                                                              \
+        hipError_t e_ = (x);                                                          \
+        if (e_ != hipSuccess)                                                         \
+            return fail(AAR_HIP_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int context_init()
+{
+    Context &C = ctx();
+    if (C.init) return 0;
+    HC(hipStreamCreateWithFlags(&C.own, hipStreamNonBlocking));
+    if (!C.stream) C.stream = C.own;
+    HC(hipMalloc((void **)&C.ctrl, 2 * sizeof(Ctrl)));
+    HC(hipHostMalloc((void **)&C.ctrl_h, 2 * sizeof(Ctrl), hipHostMallocDefault));
+    HC(hipHostMalloc((void **)&C.poll_h, NPOLL * sizeof(int), hipHostMallocDefault));
+    HC(hipMalloc((void **)&C.counters, 64 * sizeof(unsigned)));
+    HC(hipMemset(C.counters, 0, 64 * sizeof(unsigned)));
+    for (int i = 0; i < NPOLL; i++) HC(hipEventCreateWithFlags(&C.poll_ev[i], hipEventDisableTiming));
+    HC(hipEventCreate(&C.t0));
+    HC(hipEventCreate(&C.t1));
+    C.init = true;
+    return 0;
+}
+
+void lap_coefficient(double *c, int fdn)
+{
+    // system.c:480-498: c0 = sum_p -2/p^2; c_p = 2 (-1)^(p+1) (prod ratio) / p^2
+    double acc = 0.0;
+    for (int p = 1; p <= fdn; p++) acc += -(2.0 / (p * p));
+    c[0] = acc;
+    for (int p = 1; p <= fdn; p++) {
+        double num = 1.0, den = 1.0;
+        for (int i = fdn - p + 1; i <= fdn; i++) num *= i;
+        for (int i = fdn + 1; i <= fdn + p; i++) den *= i;
+        const double ratio = num / den;
+        const double sgn = (p % 2 == 1) ? 1.0 : -1.0;
+        c[p] = 2.0 * sgn * ratio / (p * p);
+    }
+}
+
+int make_op(const aar_hip_grid *g, const double *coeff, Op &op)
+{
+    if (!g) return fail(AAR_HIP_ERR_ARG, "null grid");
+    if (g->fdn != R) return fail(AAR_HIP_ERR_ARG, "only fdn = 6 (FD12) is built");
+    if (g->nx < 1 || g->ny < 1 || g->nz < 1) return fail(AAR_HIP_ERR_ARG, "grid sizes must be positive");
+    if ((double)g->nx * g->ny * g->nz > 2147483647.0) return fail(AAR_HIP_ERR_ARG, "grid too large");
+    double c[R + 1];
+    if (coeff) for (int o = 0; o <= R; o++) c[o] = coeff[o];
+    else lap_coefficient(c, R);
+    op.nx = g->nx;
+    op.ny = g->ny;
+    op.nz = g->nz;
+    op.n = (long)g->nx * g->ny * g->nz;
+    for (int o = 0; o <= R; o++) op.w[o] = g->a * c[o];   // system.c:102-104
+    op.sigma = g->sigma;
+    op.diag = g->a * (3 * c[0]);                           // AAR.c:94
+    if (g->sigma != 0.0) op.diag += g->sigma;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers
+// ---------------------------------------------------------------------------
+static int auto_lz(const Op &op, int py)
+{
+    const int ntx = (op.nx + TX - 1) / TX, nty = (op.ny + TYT * py - 1) / (TYT * py);
+    const int cols = ntx * nty;
+    int nzc = std::max(1, 2048 / std::max(1, cols));
+    int lz = (op.nz + nzc - 1) / nzc;
+    lz = std::max(lz, std::min(op.nz, 16));
+    return lz;
+}
+
+template <typename T>
+SArgs<T> base_sargs(const Op &op)
+{
+    Context &C = ctx();
+    SArgs<T> a;
+    a.nx = op.nx;
+    a.ny = op.ny;
+    a.nz = op.nz;
+    a.lz = C.lz > 0 ? C.lz : auto_lz(op, C.py);
+    for (int o = 0; o <= R; o++) a.w[o] = op.w[o];
+    a.sigma = op.sigma;
+    a.diag = op.diag;
+    return a;
+}
+
+static int nblocks_for(long n)
+{
+    long nb = (n + 4095) / 4096;
+    return (int)std::max(1L, std::min(nb, 2048L));
+}
+
+void prof_begin(const char *name)
+{
+    Context &C = ctx();
+    if (!C.prof) return;
+    PendingEv ev;
+    ev.name = name;
+    if (C.evpool.size() >= 2) {
+        ev.a = C.evpool.back();
+        C.evpool.pop_back();
+        ev.b = C.evpool.back();
+        C.evpool.pop_back();
+    } else {
+        (void)hipEventCreate(&ev.a);
+        (void)hipEventCreate(&ev.b);
+    }
+    (void)hipEventRecord(ev.a, C.stream);
+    C.pending.push_back(ev);
+}
+
+void prof_end(double bytes)
+{
+    Context &C = ctx();
+    if (!C.prof || C.pending.empty()) return;
+    PendingEv &ev = C.pending.back();
+    ev.bytes = bytes;
+    (void)hipEventRecord(ev.b, C.stream);
+}
+
+void prof_flush()
+{
+    Context &C = ctx();
+    for (auto &ev : C.pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ev.a, ev.b) == hipSuccess) {
+            ProfEntry &e = C.profile[ev.name];
+            e.count++;
+            e.ms += ms;
+            e.bytes += ev.bytes;
+        }
+        C.evpool.push_back(ev.a);
+        C.evpool.push_back(ev.b);
+    }
+    C.pending.clear();
+}
+
+static const char *smode_name(SMode m)
+{
+    switch (m) {
+    case SMode::MATVEC: return "stencil_matvec";
+    case SMode::AAR_RICH: return "stencil_aar_rich";
+    case SMode::RESID: return "stencil_resid";
+    case SMode::PGR_RICH: return "stencil_pgr_rich";
+    case SMode::CG_Q: return "stencil_cg_q";
+    case SMode::CG_RESID: return "stencil_cg_resid";
+    }
+    return "stencil";
+}
+
+template <typename T>
+int run_stencil(SMode mode, SArgs<T> a, double bytes)
+{
+    Context &C = ctx();
+    prof_begin(smode_name(mode));
+    hipError_t e = launch_stencil<T>(mode, a, C.py, C.stream);
+    prof_end(bytes);
+    if (e != hipSuccess) return fail(AAR_HIP_ERR_HIP, std::string("stencil launch: ") + hipGetErrorString(e));
+    C.bytes_acc += bytes;
+    return 0;
+}
+
+template <typename T>
+int stencil_partials_needed(const Op &op)
+{
+    SArgs<T> a = base_sargs<T>(op);
+    return stencil_items<T>(a, ctx().py) * 2;
+}
+
+// poll ring: record ctrl->done after a cycle; true once done is observed
+static int poll_record(Ctrl *ctrl, int cyc)
+{
+    Context &C = ctx();
+    const int s = cyc % NPOLL;
+    HC(hipMemcpyAsync(&C.poll_h[s], &ctrl->done, sizeof(int), hipMemcpyDeviceToHost, C.stream));
+    HC(hipEventRecord(C.poll_ev[s], C.stream));
+    return 0;
+}
+
+static bool poll_check(int cyc)
+{
+    Context &C = ctx();
+    if (cyc < 0) return false;
+    const int s = cyc % NPOLL;
+    if (hipEventSynchronize(C.poll_ev[s]) != hipSuccess) return true;
+    return C.poll_h[s] != 0;
+}
+
+static int finish_ctrl(Ctrl *ctrl, Ctrl *host)
+{
+    Context &C = ctx();
+    HC(hipMemcpyAsync(host, ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, C.stream));
+    HC(hipStreamSynchronize(C.stream));
+    return 0;
+}
+
+static void fill_stats(aar_hip_stats *st, const Ctrl &c, int iterations, int loop_iter, int max_iter,
+                       double ms, double bytes, int hist_len)
+{
+    aar_hip_stats s;
+    std::memset(&s, 0, sizeof(s));
+    s.iterations = iterations;
+    s.rhs_norm = c.rhs_norm;
+    s.relres = c.rhs_norm > 0 ? c.relres / c.rhs_norm : c.relres;
+    s.converged = c.relres <= c.tol_abs;
+    s.printed_converged = (loop_iter < max_iter && c.relres <= c.tol_abs);
+    s.printed_warning = (loop_iter >= max_iter);
+    s.status = c.status == 1 ? AAR_HIP_ERR_SVD : (c.status == 2 ? AAR_HIP_ERR_NONFINITE : 0);
+    s.hist_len = hist_len;
+    s.solve_ms = ms;
+    s.bytes = bytes;
+    ctx().last = s;
+    if (st) *st = s;
+}
+
+static int store_history(const double *hist_dev, int len)
+{
+    Context &C = ctx();
+    C.hist_host.assign(std::max(len, 0), 0.0);
+    if (len > 0) HC(hipMemcpy(C.hist_host.data(), hist_dev, sizeof(double) * len, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// AAR (AAR.c:37-177), FP64 or FP32
+// ---------------------------------------------------------------------------
+template <typename T>
+int aar_solve(const Op &op, T *x, const T *b, double omega, double beta, int m, int p, int max_iter,
+              double tol, int precond, int ci, aar_hip_stats *st, int keep_history)
+{
+    Context &C = ctx();
+    if (int rc = context_init()) return rc;
+    if (!x || !b) return fail(AAR_HIP_ERR_ARG, "null vector");
+    if (m < 1 || m > MAXM) return fail(AAR_HIP_ERR_ARG, "history m must be in 1..8");
+    if (p < 1) return fail(AAR_HIP_ERR_ARG, "p must be >= 1");
+    if (max_iter < 0) return fail(AAR_HIP_ERR_ARG, "max_iter must be >= 0");
+    const long n = op.n;
+    const int ns = m + 1;
+    DevBuf &RX = (sizeof(T) == 8) ? C.ring64x : C.ring32x;
+    DevBuf &RF = (sizeof(T) == 8) ? C.ring64f : C.ring32f;
+    HC(ensure(RX, sizeof(T) * n * ns));
+    HC(ensure(RF, sizeof(T) * n * ns));
+    DevBuf &HB = ci == 0 ? C.hist0 : C.hist1;
+    HC(ensure(HB, sizeof(double) * (size_t)(max_iter + 4)));
+    const int nb = nblocks_for(n);
+    const size_t need = std::max<size_t>((size_t)stencil_partials_needed<T>(op), (size_t)nb * 80);
+    DevBuf &PB = ci == 0 ? C.part0 : C.part1;
+    HC(ensure(PB, sizeof(double) * need));
+    T *X = (T *)RX.p, *F = (T *)RF.p;
+    double *hist = (double *)HB.p, *part = (double *)PB.p;
+    Ctrl *ctrl = C.ctrl + ci;
+    unsigned *cnt = C.counters + 8 * ci;
+    auto slot = [&](long it) { return (int)(it % ns); };
+    const double s_b = (double)sizeof(T) * n;
+    C.bytes_acc = 0.0;
+
+    HC(launch_zero_ctrl(ctrl, C.stream));
+    HC(hipEventRecord(C.t0, C.stream));
+    // x_1 = x0 into its slot
+    HC(hipMemcpyAsync(X + (size_t)slot(1) * n, x, sizeof(T) * n, hipMemcpyDeviceToDevice, C.stream));
+    C.bytes_acc += 2 * s_b;
+    // ||b|| and the scaled tolerance (AAR.c:78-79)
+    {
+        DotArgs<T> d;
+        d.n = n;
+        d.v = b;
+        d.nblocks = nb;
+        d.partials = part;
+        d.counter = cnt + 0;
+        d.ctrl = ctrl;
+        d.fin.fin = FIN_NORM_B;
+        d.fin.tol = tol;
+        d.fin.check = 1;
+        prof_begin("dot");
+        HC(launch_dot<T>(d, C.stream));
+        prof_end(s_b);
+    }
+    C.bytes_acc += s_b;
+
+    int cyc = 0;
+    int it = 1;
+    bool stopped = false;
+    for (; it <= max_iter && !stopped; it++) {
+        const bool anderson = (it % p == 0) && (it > 1);            // AAR.c:109
+        const int prev = it - 1;
+        const bool check = prev >= 1 && (prev % p == 0) && (prev > 1);
+        SArgs<T> a = base_sargs<T>(op);
+        a.omega = omega;
+        a.precond = precond;
+        a.in0 = X + (size_t)slot(it) * n;
+        a.bvec = b;
+        a.out0 = F + (size_t)slot(it) * n;
+        a.out1 = X + (size_t)slot(it + 1) * n;
+        a.write_x = anderson ? 0 : 1;
+        a.partials = part;
+        a.counter = cnt + 1;
+        a.ctrl = ctrl;
+        a.fin.fin = FIN_AAR_RES;
+        a.fin.it = it;
+        a.fin.check = check;
+        a.fin.hist = hist;
+        a.fin.hist_cap = max_iter + 4;
+        if (int rc = run_stencil<T>(SMode::AAR_RICH, a, s_b * (anderson ? 3 : 4))) return rc;
+        if (anderson) {
+            GramArgs<T> g;
+            UpdArgs<T> u;
+            int nslots_read = 0;
+            bool seen[MAXM + 1] = {false};
+            seen[slot(it)] = true;
+            nslots_read = 1;
+            for (int j = 0; j < m; j++) {
+                const int off = (int)(((it - 2 - j) % m + m) % m);
+                const long ij = it - off;                            // AAR.c:97: col = (iter-2) % m
+                const int ok = ij >= 2;
+                g.valid[j] = u.valid[j] = ok;
+                g.sa[j] = u.sa[j] = ok ? slot(ij) : 0;
+                g.sb[j] = u.sb[j] = ok ? slot(ij - 1) : 0;
+                if (ok) {
+                    if (!seen[slot(ij)]) { seen[slot(ij)] = true; nslots_read++; }
+                    if (!seen[slot(ij - 1)]) { seen[slot(ij - 1)] = true; nslots_read++; }
+                }
+            }
+            g.n = n;
+            g.m = m;
+            g.kind = 0;
+            g.F = F;
+            g.ld = n;
+            g.fslot = slot(it);
+            g.nblocks = nb;
+            g.partials = part;
+            g.counter = cnt + 2;
+            g.ctrl = ctrl;
+            prof_begin("gram");
+            HC(launch_gram<T>(g, C.stream));
+            prof_end(s_b * nslots_read);
+            prof_begin("ls");
+            HC(launch_ls(ctrl, m, 0, C.stream));
+            prof_end(0);
+            u.n = n;
+            u.m = m;
+            u.beta = beta;
+            u.X = X;
+            u.F = F;
+            u.ld = n;
+            u.xslot = slot(it);
+            u.fslot = slot(it);
+            u.oslot = slot(it + 1);
+            u.ctrl = ctrl;
+            prof_begin("aar_update");
+            HC(launch_aar_update<T>(u, C.stream));
+            prof_end(s_b * (2 * nslots_read + 1));
+            C.bytes_acc += s_b * (3 * nslots_read + 1);
+            if (int rc = poll_record(ctrl, cyc)) return rc;
+            if (poll_check(cyc - 2)) stopped = true;
+            cyc++;
+        }
+    }
+    if (!stopped) {
+        // the residual of the last iterate: the reference computes it inside
+        // the last iteration (AAR.c:119-123 / 137-139) and checks it when
+        // that iteration was an Anderson step
+        const int last = std::min(it, max_iter + 1);
+        const int prev = last - 1;
+        const bool check = prev >= 1 && (prev % p == 0) && (prev > 1);
+        SArgs<T> a = base_sargs<T>(op);
+        a.omega = omega;
+        a.precond = precond;
+        a.in0 = X + (size_t)slot(last) * n;
+        a.bvec = b;
+        a.out0 = F + (size_t)slot(last) * n;
+        a.write_x = 0;
+        a.partials = part;
+        a.counter = cnt + 1;
+        a.ctrl = ctrl;
+        a.fin.fin = FIN_AAR_RES;
+        a.fin.it = last;
+        a.fin.check = check;
+        a.fin.hist = hist;
+        a.fin.hist_cap = max_iter + 4;
+        if (int rc = run_stencil<T>(SMode::AAR_RICH, a, s_b * 3)) return rc;
+    }
+    HC(hipEventRecord(C.t1, C.stream));
+    Ctrl &hc = C.ctrl_h[ci];
+    if (int rc = finish_ctrl(ctrl, &hc)) return rc;
+    float ms = 0.f;
+    HC(hipEventElapsedTime(&ms, C.t0, C.t1));
+    prof_flush();
+    int iterations, loop_iter;
+    if (hc.done) {
+        iterations = hc.final_iter;
+        loop_iter = hc.final_iter + 1;
+    } else {
+        iterations = max_iter;
+        loop_iter = max_iter + 1;
+    }
+    // solution: iterate following the last iteration
+    HC(hipMemcpyAsync(x, X + (size_t)slot(iterations + 1) * n, sizeof(T) * n, hipMemcpyDeviceToDevice,
+                      C.stream));
+    HC(hipStreamSynchronize(C.stream));
+    C.bytes_acc += 2 * s_b;
+    const int hl = std::min(iterations + 1, max_iter + 4);
+    if (keep_history)
+        if (int rc = store_history(hist, hl)) return rc;
+    fill_stats(st, hc, iterations, loop_iter, max_iter, ms, C.bytes_acc, hl);
+    if (hc.status == 1) return fail(AAR_HIP_ERR_SVD, "no convergence in 250 svd iterations");
+    if (hc.status == 2) return fail(AAR_HIP_ERR_NONFINITE, "residual is not finite");
+    return 0;
+}
+
+template int aar_solve<double>(const Op &, double *, const double *, double, double, int, int, int, double,
+                               int, int, aar_hip_stats *, int);
+template int aar_solve<float>(const Op &, float *, const float *, double, double, int, int, int, double,
+                              int, int, aar_hip_stats *, int);
+
+// ---------------------------------------------------------------------------
+// PGR / PL2R (PGR.c:35-218, PL2R.c:35-218), FP64
+// ---------------------------------------------------------------------------
+int pgr_solve(const Op &op, double *x, const double *b, double omega, int m, int p, int max_iter,
+              double tol, int precond, int l2, aar_hip_stats *st)
+{
+    Context &C = ctx();
+    if (int rc = context_init()) return rc;
+    if (!x || !b) return fail(AAR_HIP_ERR_ARG, "null vector");
+    if (m < 1 || m > MAXM) return fail(AAR_HIP_ERR_ARG, "history m must be in 1..8");
+    if (p < 1) return fail(AAR_HIP_ERR_ARG, "p must be >= 1");
+    if (max_iter < 0) return fail(AAR_HIP_ERR_ARG, "max_iter must be >= 0");
+    const long n = op.n;
+    const int ns = m + 1;
+    HC(ensure(C.ring64x, sizeof(double) * n * ns));
+    HC(ensure(C.ring64f, sizeof(double) * n * ns));
+    HC(ensure(C.hist0, sizeof(double) * (size_t)(max_iter + 4)));
+    const int nb = nblocks_for(n);
+    const size_t need = std::max<size_t>((size_t)stencil_partials_needed<double>(op), (size_t)nb * 80);
+    HC(ensure(C.part0, sizeof(double) * need));
+    double *X = (double *)C.ring64x.p, *Y = (double *)C.ring64f.p;
+    double *hist = (double *)C.hist0.p, *part = (double *)C.part0.p;
+    Ctrl *ctrl = C.ctrl;
+    unsigned *cnt = C.counters;
+    auto slot = [&](long t) { return (int)(t % ns); };
+    const double s_b = 8.0 * n;
+    C.bytes_acc = 0.0;
+
+    HC(launch_zero_ctrl(ctrl, C.stream));
+    HC(hipEventRecord(C.t0, C.stream));
+    HC(hipMemcpyAsync(X + (size_t)slot(1) * n, x, sizeof(double) * n, hipMemcpyDeviceToDevice, C.stream));
+    {
+        DotArgs<double> d;
+        d.n = n;
+        d.v = b;
+        d.nblocks = nb;
+        d.partials = part;
+        d.counter = cnt + 0;
+        d.ctrl = ctrl;
+        d.fin.fin = FIN_NORM_B;
+        d.fin.tol = tol;
+        d.fin.check = 1;
+        HC(launch_dot<double>(d, C.stream));
+    }
+    // Ax_1 and ||b - A x_1|| (PGR.c:79-81)
+    {
+        SArgs<double> a = base_sargs<double>(op);
+        a.in0 = X + (size_t)slot(1) * n;
+        a.bvec = b;
+        a.out0 = Y + (size_t)slot(1) * n;
+        a.partials = part;
+        a.counter = cnt + 1;
+        a.ctrl = ctrl;
+        a.fin.fin = FIN_AAR_RES;
+        a.fin.it = 1;
+        a.fin.check = 0;
+        a.fin.hist = hist;
+        a.fin.hist_cap = max_iter + 4;
+        if (int rc = run_stencil<double>(SMode::MATVEC, a, s_b * 3)) return rc;
+    }
+    int cyc = 0;
+    int t = 1;
+    bool stopped = false;
+    for (; t <= max_iter && !stopped; t++) {
+        SArgs<double> a = base_sargs<double>(op);
+        a.omega = omega;
+        a.precond = precond;
+        a.in0 = X + (size_t)slot(t) * n;
+        a.in1 = b;
+        a.in2 = Y + (size_t)slot(t) * n;
+        a.bvec = b;
+        a.out0 = X + (size_t)slot(t + 1) * n;
+        a.out1 = Y + (size_t)slot(t + 1) * n;
+        a.trace = 1;
+        a.partials = part;
+        a.counter = cnt + 1;
+        a.ctrl = ctrl;
+        a.fin.fin = FIN_AAR_RES;
+        a.fin.it = t + 1;
+        a.fin.check = 0;
+        a.fin.hist = hist;
+        a.fin.hist_cap = max_iter + 4;
+        if (int rc = run_stencil<double>(SMode::PGR_RICH, a, s_b * 5)) return rc;
+        if (t % p == 0) {                                           // PGR.c:111
+            GramArgs<double> g;
+            PUpdArgs<double> u;
+            int nsl = 1;
+            bool seen[MAXM + 1] = {false};
+            seen[slot(t + 1)] = true;
+            for (int j = 0; j < m; j++) {
+                // column j holds iteration tj with (tj-2+m) % m == j (PGR.c:105)
+                const int off = (int)(((t - j - 2) % m + m) % m);
+                const long tj = t - off;
+                const int ok = tj >= 1;
+                g.valid[j] = u.valid[j] = ok;
+                g.sa[j] = u.sa[j] = ok ? slot(tj + 1) : 0;
+                g.sb[j] = u.sb[j] = ok ? slot(tj) : 0;
+                if (ok) {
+                    if (!seen[slot(tj + 1)]) { seen[slot(tj + 1)] = true; nsl++; }
+                    if (!seen[slot(tj)]) { seen[slot(tj)] = true; nsl++; }
+                }
+            }
+            g.n = n;
+            g.m = m;
+            g.kind = l2 ? 2 : 1;
+            g.F = X;
+            g.Y = Y;
+            g.b = b;
+            g.ld = n;
+            g.fslot = slot(t + 1);
+            g.nblocks = nb;
+            g.partials = part;
+            g.counter = cnt + 2;
+            g.ctrl = ctrl;
+            prof_begin("gram");
+            HC(launch_gram<double>(g, C.stream));
+            prof_end(s_b * ((l2 ? 1 : 2) * nsl + 1));
+            prof_begin("ls");
+            HC(launch_ls(ctrl, m, 0, C.stream));
+            prof_end(0);
+            u.n = n;
+            u.m = m;
+            u.X = X;
+            u.Y = Y;
+            u.b = b;
+            u.ld = n;
+            u.cur = slot(t + 1);
+            u.nblocks = nb;
+            u.partials = part;
+            u.counter = cnt + 3;
+            u.ctrl = ctrl;
+            u.fin.fin = FIN_PGR_RES;
+            u.fin.it = t + 1;
+            u.fin.check = 1;
+            u.fin.hist = nullptr;
+            prof_begin("pgr_update");
+            HC(launch_pgr_update<double>(u, C.stream));
+            prof_end(s_b * (2 * nsl + 1 + 2));
+            C.bytes_acc += s_b * ((l2 ? 1 : 2) * nsl + 1) + s_b * (2 * nsl + 3);
+            if (int rc = poll_record(ctrl, cyc)) return rc;
+            if (poll_check(cyc - 2)) stopped = true;
+            cyc++;
+        }
+    }
+    HC(hipEventRecord(C.t1, C.stream));
+    Ctrl &hc = C.ctrl_h[0];
+    if (int rc = finish_ctrl(ctrl, &hc)) return rc;
+    float ms = 0.f;
+    HC(hipEventElapsedTime(&ms, C.t0, C.t1));
+    prof_flush();
+    int iterations, loop_iter;
+    if (hc.done) {
+        iterations = hc.final_iter;
+        loop_iter = hc.final_iter + 1;
+    } else {
+        iterations = max_iter;
+        loop_iter = max_iter + 1;
+    }
+    HC(hipMemcpyAsync(x, X + (size_t)slot(iterations + 1) * n, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                      C.stream));
+    HC(hipStreamSynchronize(C.stream));
+    const int hl = std::min(iterations + 1, max_iter + 4);
+    if (int rc = store_history(hist, hl)) return rc;
+    fill_stats(st, hc, iterations, loop_iter, max_iter, ms, C.bytes_acc, hl);
+    if (hc.status == 1) return fail(AAR_HIP_ERR_SVD, "no convergence in 250 svd iterations");
+    if (hc.status == 2) return fail(AAR_HIP_ERR_NONFINITE, "residual is not finite");
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// CG (CG.c:33-175), FP64
+// ---------------------------------------------------------------------------
+int cg_solve(const Op &op, double *x, const double *b, int max_iter, double tol, int precond,
+             aar_hip_stats *st)
+{
+    Context &C = ctx();
+    if (int rc = context_init()) return rc;
+    if (!x || !b) return fail(AAR_HIP_ERR_ARG, "null vector");
+    if (max_iter < 0) return fail(AAR_HIP_ERR_ARG, "max_iter must be >= 0");
+    const long n = op.n;
+    // r, d[2], q
+    HC(ensure(C.ring64x, sizeof(double) * n * 4));
+    HC(ensure(C.hist0, sizeof(double) * (size_t)(max_iter + 4)));
+    const int nb = nblocks_for(n);
+    const size_t need = std::max<size_t>((size_t)stencil_partials_needed<double>(op), (size_t)nb * 80);
+    HC(ensure(C.part0, sizeof(double) * need));
+    double *r = (double *)C.ring64x.p, *dd[2] = {r + n, r + 2 * n}, *q = r + 3 * n;
+    double *hist = (double *)C.hist0.p, *part = (double *)C.part0.p;
+    Ctrl *ctrl = C.ctrl;
+    unsigned *cnt = C.counters;
+    const double s_b = 8.0 * n;
+    C.bytes_acc = 0.0;
+
+    HC(launch_zero_ctrl(ctrl, C.stream));
+    HC(hipEventRecord(C.t0, C.stream));
+    {
+        DotArgs<double> d;
+        d.n = n;
+        d.v = b;
+        d.nblocks = nb;
+        d.partials = part;
+        d.counter = cnt + 0;
+        d.ctrl = ctrl;
+        d.fin.fin = FIN_NORM_B;
+        d.fin.tol = tol;
+        HC(launch_dot<double>(d, C.stream));
+    }
+    // r = b - A x; d = r / diag; delta = r.d; relres = ||r|| (CG.c:65-84)
+    {
+        SArgs<double> a = base_sargs<double>(op);
+        a.precond = precond;
+        a.in0 = x;
+        a.bvec = b;
+        a.out0 = r;
+        a.out1 = dd[0];
+        a.partials = part;
+        a.counter = cnt + 1;
+        a.ctrl = ctrl;
+        a.fin.fin = FIN_CG_BETA;
+        a.fin.it = 0;
+        a.fin.hist = hist;
+        a.fin.hist_cap = max_iter + 4;
+        if (int rc = run_stencil<double>(SMode::CG_RESID, a, s_b * 4)) return rc;
+    }
+    int k = 0;
+    bool stopped = false;
+    int cyc = 0;
+    for (; k < max_iter && !stopped; k++) {
+        const int cur = (k + 1) & 1, old = k & 1;
+        {
+            SArgs<double> a = base_sargs<double>(op);
+            a.precond = precond;
+            a.cg_first = (k == 0);
+            a.cg_beta = &ctrl->beta;
+            a.in0 = r;
+            a.in1 = dd[old];
+            a.out0 = dd[cur];
+            a.out1 = q;
+            a.partials = part;
+            a.counter = cnt + 2;
+            a.ctrl = ctrl;
+            a.fin.fin = FIN_CG_ALPHA;
+            if (int rc = run_stencil<double>(SMode::CG_Q, a, s_b * 4)) return rc;
+        }
+        const bool restart = (k % 50) == 0;                          // CG.c:109
+        {
+            CgUpdArgs u;
+            u.n = n;
+            u.x = x;
+            u.r = r;
+            u.d = dd[cur];
+            u.q = q;
+            u.diag = op.diag;
+            u.precond = precond;
+            u.update_r = restart ? 0 : 1;
+            u.nblocks = nb;
+            u.partials = part;
+            u.counter = cnt + 3;
+            u.ctrl = ctrl;
+            u.fin.fin = FIN_CG_BETA;
+            u.fin.it = k + 1;
+            u.fin.hist = hist;
+            u.fin.hist_cap = max_iter + 4;
+            prof_begin("cg_update");
+            HC(launch_cg_update(u, C.stream));
+            prof_end(s_b * (restart ? 3 : 6));
+            C.bytes_acc += s_b * (restart ? 3 : 6);
+        }
+        if (restart) {
+            SArgs<double> a = base_sargs<double>(op);
+            a.precond = precond;
+            a.in0 = x;
+            a.bvec = b;
+            a.out0 = r;
+            a.partials = part;
+            a.counter = cnt + 1;
+            a.ctrl = ctrl;
+            a.fin.fin = FIN_CG_BETA;
+            a.fin.it = k + 1;
+            a.fin.hist = hist;
+            a.fin.hist_cap = max_iter + 4;
+            if (int rc = run_stencil<double>(SMode::CG_RESID, a, s_b * 3)) return rc;
+        }
+        if ((k + 1) % 8 == 0) {
+            if (int rc = poll_record(ctrl, cyc)) return rc;
+            if (poll_check(cyc - 2)) stopped = true;
+            cyc++;
+        }
+    }
+    HC(hipEventRecord(C.t1, C.stream));
+    Ctrl &hc = C.ctrl_h[0];
+    if (int rc = finish_ctrl(ctrl, &hc)) return rc;
+    float ms = 0.f;
+    HC(hipEventElapsedTime(&ms, C.t0, C.t1));
+    prof_flush();
+    const int iterations = hc.done ? hc.final_iter : max_iter;
+    const int hl = std::min(iterations + 1, max_iter + 4);
+    if (int rc = store_history(hist, hl)) return rc;
+    fill_stats(st, hc, iterations, iterations, max_iter, ms, C.bytes_acc, hl);
+    if (hc.status == 2) return fail(AAR_HIP_ERR_NONFINITE, "residual is not finite");
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// matvec
+// ---------------------------------------------------------------------------
+template <typename T>
+int matvec(const Op &op, const T *x, T *y)
+{
+    Context &C = ctx();
+    if (int rc = context_init()) return rc;
+    if (!x || !y) return fail(AAR_HIP_ERR_ARG, "null vector");
+    SArgs<T> a = base_sargs<T>(op);
+    a.in0 = x;
+    a.out0 = y;
+    if (int rc = run_stencil<T>(SMode::MATVEC, a, 2.0 * sizeof(T) * op.n)) return rc;
+    HC(hipStreamSynchronize(C.stream));
+    prof_flush();
+    return 0;
+}
+template int matvec<double>(const Op &, const double *, double *);
+template int matvec<float>(const Op &, const float *, float *);
+
+// ---------------------------------------------------------------------------
+// FP64 residual refinement around FP32 AAR (extension)
+// ---------------------------------------------------------------------------
+int aar_refined(const Op &op, double *x, const double *b, double omega, double beta, int m, int p,
+                int max_outer, int max_inner, double tol, double inner_tol, int precond,
+                aar_hip_stats *st)
+{
+    Context &C = ctx();
+    if (int rc = context_init()) return rc;
+    if (!x || !b) return fail(AAR_HIP_ERR_ARG, "null vector");
+    const long n = op.n;
+    HC(ensure(C.vec32a, sizeof(float) * n));   // r (FP32 RHS of the correction)
+    HC(ensure(C.vec32b, sizeof(float) * n));   // e (FP32 correction)
+    HC(ensure(C.hist0, sizeof(double) * (size_t)(max_outer + 4)));
+    const int nb = nblocks_for(n);
+    const size_t need = std::max<size_t>((size_t)stencil_partials_needed<double>(op), (size_t)nb * 80);
+    HC(ensure(C.part0, sizeof(double) * need));
+    float *r32 = (float *)C.vec32a.p, *e32 = (float *)C.vec32b.p;
+    double *part = (double *)C.part0.p;
+    Ctrl *ctrl = C.ctrl;
+    unsigned *cnt = C.counters;
+    const double s_b = 8.0 * n;
+    Op op32 = op;
+    std::vector<double> hist_out;
+    double bytes = 0.0, ms_total = 0.0;
+    int inner_total = 0, outer = 0;
+    const bool prof_was = C.prof;
+
+    hipEvent_t e0, e1;
+    HC(hipEventCreate(&e0));
+    HC(hipEventCreate(&e1));
+    HC(launch_zero_ctrl(ctrl, C.stream));
+    HC(hipEventRecord(e0, C.stream));
+    {
+        DotArgs<double> d;
+        d.n = n;
+        d.v = b;
+        d.nblocks = nb;
+        d.partials = part;
+        d.counter = cnt + 0;
+        d.ctrl = ctrl;
+        d.fin.fin = FIN_NORM_B;
+        d.fin.tol = tol;
+        HC(launch_dot<double>(d, C.stream));
+        bytes += s_b;
+    }
+    double rhs_norm = 0.0, rel = 0.0;
+    int status = 0;
+    for (outer = 0;; outer++) {
+        SArgs<double> a = base_sargs<double>(op);
+        a.in0 = x;
+        a.bvec = b;
+        a.out32 = r32;
+        a.partials = part;
+        a.counter = cnt + 1;
+        a.ctrl = ctrl;
+        a.fin.fin = FIN_SUMS;
+        if (int rc = run_stencil<double>(SMode::RESID, a, s_b * 2 + 4.0 * n)) return rc;
+        bytes += s_b * 2 + 4.0 * n;
+        Ctrl &hc = C.ctrl_h[0];
+        if (int rc = finish_ctrl(ctrl, &hc)) return rc;
+        rhs_norm = hc.rhs_norm;
+        const double rn = std::sqrt(hc.red[0]);
+        rel = rhs_norm > 0 ? rn / rhs_norm : rn;
+        hist_out.push_back(rn);
+        if (!(rel == rel)) { status = AAR_HIP_ERR_NONFINITE; break; }
+        if (rel <= tol || outer >= max_outer) break;
+        const double tin = std::max(inner_tol, 0.5 * tol / rel);
+        HC(launch_fill_f32(e32, 0.0f, n, C.stream));
+        aar_hip_stats ist;
+        int rc = aar_solve<float>(op32, e32, r32, omega, beta, m, p, max_inner, tin, precond, 1, &ist, 0);
+        if (rc) return rc;
+        inner_total += ist.iterations;
+        bytes += ist.bytes + 4.0 * n;
+        HC(launch_axpy_f64_f32(x, e32, n, C.stream));
+        bytes += s_b * 2 + 4.0 * n;
+    }
+    HC(hipEventRecord(e1, C.stream));
+    HC(hipStreamSynchronize(C.stream));
+    float ms = 0.f;
+    HC(hipEventElapsedTime(&ms, e0, e1));
+    ms_total = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)prof_was;
+    C.hist_host = hist_out;
+    aar_hip_stats s;
+    std::memset(&s, 0, sizeof(s));
+    s.iterations = inner_total;
+    s.inner_iterations = inner_total;
+    s.outer_steps = outer;
+    s.relres = rel;
+    s.rhs_norm = rhs_norm;
+    s.converged = rel <= tol;
+    s.printed_converged = s.converged;
+    s.status = status;
+    s.hist_len = (int)hist_out.size();
+    s.solve_ms = ms_total;
+    s.bytes = bytes;
+    C.last = s;
+    if (st) *st = s;
+    if (status) return fail(status, "residual is not finite");
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// pinv on device (host matrices)
+// ---------------------------------------------------------------------------
+int pinv_host(const double *A, const double *b, double *x, int m, double *w_out, double *v_out)
+{
+    Context &C = ctx();
+    if (int rc = context_init()) return rc;
+    if (m < 1 || m > MAXM_LS) return fail(AAR_HIP_ERR_ARG, "pinv: m must be in 1..16");
+    Ctrl &hc = C.ctrl_h[1];
+    std::memset(&hc, 0, sizeof(Ctrl));
+    // allredvec layout: A[i][j] at G[j*m + i], b at G[m*m + j]
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j < m; j++) hc.G[j * m + i] = A[i * m + j];
+    for (int j = 0; j < m; j++) hc.G[m * m + j] = b[j];
+    Ctrl *dc = C.ctrl + 1;
+    HC(hipMemcpyAsync(dc, &hc, sizeof(Ctrl), hipMemcpyHostToDevice, C.stream));
+    HC(launch_ls(dc, m, 0, C.stream));
+    HC(hipMemcpyAsync(&hc, dc, sizeof(Ctrl), hipMemcpyDeviceToHost, C.stream));
+    HC(hipStreamSynchronize(C.stream));
+    for (int j = 0; j < m; j++) x[j] = hc.s[j];
+    if (w_out)
+        for (int j = 0; j < m; j++) w_out[j] = hc.w[j];
+    if (v_out)
+        for (int j = 0; j < m * m; j++) v_out[j] = hc.V[j];
+    if (hc.status == 1) return fail(AAR_HIP_ERR_SVD, "no convergence in 250 svd iterations");
+    return 0;
+}
+
+void release_all()
+{
+    Context &C = ctx();
+    DevBuf *bufs[] = {&C.ring64x, &C.ring64f, &C.ring32x, &C.ring32f, &C.hist0, &C.hist1,
+                      &C.part0, &C.part1, &C.vec32a, &C.vec32b, &C.vec64a, &C.vec64b};
+    for (DevBuf *b : bufs) {
+        if (b->p) (void)hipFree(b->p);
+        b->p = nullptr;
+        b->bytes = 0;
+    }
+}
+
+}  // namespace aar

@@ -1,0 +1,356 @@
+// stencil.hip -- the FD12 (radius-6, 37-point) periodic star stencil and
+// the kernels that fuse it with the solver sweeps around it.
+//
+// Reference operator: MPI_REAL/system.c:189-205 (1-rank branch of
+// Lap_Vec_mult): y = (x*3)*w0; for o = 1..6:
+//   y += (x[i-o] + x[i+o] + x[j-o] + x[j+o] + x[k-o] + x[k+o]) * w_o,
+// w_o = a*coeff_lap[o].  The same operation order is used here with
+// contraction off, so a stencil result is bit-identical to the reference.
+//
+// Design (gfx950): a block of 256 threads = 4 wavefronts owns a 64 x TY
+// column of the grid (64 = one wavefront across x, TY = 4*PY rows, PY
+// rows per thread) over a z-chunk of `lz` planes and marches it in z.
+// * z-neighbours come from a per-thread register queue of 13 planes, so
+//   every plane of the column is read from HBM once per chunk;
+// * x/y-neighbours come from a double-buffered LDS tile of the current
+//   plane with a 6-wide halo (corners are not needed by a star stencil);
+//   the halo is fetched one plane ahead, one barrier per plane;
+// * block ids are mapped XCD-aware onto tiles so that y-adjacent tiles
+//   (whose halos overlap) run on the same XCD and share its L2.
+// The epilogue of each mode fuses the pointwise work of the solver step
+// that consumes A*x, so that step costs one pass over the grid.
+#include "aar_common.h"
+#include "aar_device.h"
+#include "aar_kernels.h"
+
+namespace aar {
+
+template <typename T> struct Arith;
+template <> struct Arith<double> {
+    using W = double;
+    __device__ static inline double add(double a, double b) { return __dadd_rn(a, b); }
+    __device__ static inline double sub(double a, double b) { return __dsub_rn(a, b); }
+    __device__ static inline double mul(double a, double b) { return __dmul_rn(a, b); }
+    __device__ static inline double div(double a, double b) { return a / b; }
+    __device__ static inline double sq(double a) { return a * a; }
+};
+template <> struct Arith<float> {
+    using W = float;
+    __device__ static inline float add(float a, float b) { return __fadd_rn(a, b); }
+    __device__ static inline float sub(float a, float b) { return __fsub_rn(a, b); }
+    __device__ static inline float mul(float a, float b) { return __fmul_rn(a, b); }
+    __device__ static inline float div(float a, float b) { return __fdiv_rn(a, b); }
+    __device__ static inline double sq(float a) { return (double)a * (double)a; }
+};
+
+template <SMode M> struct ModeInfo;
+template <> struct ModeInfo<SMode::MATVEC> { static constexpr int NF = 1, NE = 1, NACC = 1; };
+template <> struct ModeInfo<SMode::AAR_RICH> { static constexpr int NF = 1, NE = 1, NACC = 1; };
+template <> struct ModeInfo<SMode::RESID> { static constexpr int NF = 1, NE = 1, NACC = 1; };
+template <> struct ModeInfo<SMode::PGR_RICH> { static constexpr int NF = 3, NE = 1, NACC = 1; };
+template <> struct ModeInfo<SMode::CG_Q> { static constexpr int NF = 2, NE = 0, NACC = 1; };
+template <> struct ModeInfo<SMode::CG_RESID> { static constexpr int NF = 1, NE = 1, NACC = 2; };
+
+__device__ inline int modp(int i, int n) { i %= n; return i < 0 ? i + n : i; }
+
+template <typename T, int PY, SMode MODE>
+__global__ __launch_bounds__(BLOCK) void stencil_kernel(const SArgs<T> a)
+{
+    using A = Arith<T>;
+    constexpr int NF = ModeInfo<MODE>::NF;
+    constexpr int NE = ModeInfo<MODE>::NE;
+    constexpr int NACC = ModeInfo<MODE>::NACC;
+    constexpr int TY = TYT * PY;
+    constexpr int LH = TY + 2 * R;
+    constexpr int LW = TX + 2 * R;
+    constexpr int NXH = (2 * R * TY + BLOCK - 1) / BLOCK;
+    constexpr int QN = 2 * R + 1;
+
+    __shared__ T lds[2][LH][LW];
+    __shared__ double red_scratch[8 * (BLOCK / 64)];
+    __shared__ double red_out[8];
+
+    if (a.ctrl && a.ctrl->done) return;
+
+    const int item = xcd_item(blockIdx.x, a.nitems);
+    const int itx = item % a.ntx;
+    const int rest = item / a.ntx;
+    const int ity = rest % a.nty;
+    const int izc = rest / a.nty;
+    const int x0 = itx * TX, y0 = ity * TY;
+    const int z0 = izc * a.lz;
+    const int z1 = min(z0 + a.lz, a.nz);
+    const int tid = threadIdx.x, tx = tid & 63, ty = tid >> 6;
+    const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = a.nxy;
+
+    // in-plane offsets of this thread's points, halo cells and LDS slots
+    const int cx = modp(x0 + tx, nx);
+    int c_off[PY];
+    bool c_ok[PY];
+#pragma unroll
+    for (int p = 0; p < PY; p++) {
+        const int y = y0 + ty * PY + p;
+        c_off[p] = cx + modp(y, ny) * nx;
+        c_ok[p] = (x0 + tx < nx) && (y < ny);
+    }
+    int yh_off[3], yh_row[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int hr = ty + TYT * k;                       // 0 .. 2R-1
+        const int gy = hr < R ? y0 - R + hr : y0 + TY + (hr - R);
+        yh_off[k] = cx + modp(gy, ny) * nx;
+        yh_row[k] = hr < R ? hr : TY + hr;
+    }
+    int xh_off[NXH], xh_row[NXH], xh_col[NXH];
+    bool xh_ok[NXH];
+#pragma unroll
+    for (int k = 0; k < NXH; k++) {
+        const int c = tid + BLOCK * k;
+        xh_ok[k] = c < 2 * R * TY;
+        const int r = xh_ok[k] ? c / (2 * R) : 0, cc = xh_ok[k] ? c % (2 * R) : 0;
+        const int gx = cc < R ? x0 - R + cc : x0 + TX + (cc - R);
+        xh_off[k] = modp(gx, nx) + modp(y0 + r, ny) * nx;
+        xh_row[k] = R + r;
+        xh_col[k] = cc < R ? cc : TX + cc;
+    }
+
+    const T wt0 = (T)a.w[0];
+    T wt[R + 1];
+#pragma unroll
+    for (int o = 0; o <= R; o++) wt[o] = (T)a.w[o];
+    const T sig = (T)a.sigma, dg = (T)a.diag, om = (T)a.omega;
+    const T cgb = (MODE == SMode::CG_Q && a.cg_beta) ? (T)(*a.cg_beta) : (T)0;
+
+    auto load_raw = [&](int off, T (&v)[NF]) {
+        v[0] = a.in0[off];
+        if constexpr (NF > 1) v[1] = a.in1[off];
+        if constexpr (NF > 2) v[2] = a.in2[off];
+    };
+    // stencil operand u at a point from its loaded fields
+    auto make_u = [&](const T (&v)[NF]) -> T {
+        if constexpr (MODE == SMode::PGR_RICH) {
+            // PGR.c:92-98: f = (rhs - Ax) / diag; x_new = x + omega*f
+            T f = A::sub(v[1], v[2]);
+            if (a.precond) f = A::div(f, dg);
+            return A::add(v[0], A::mul(om, f));
+        } else if constexpr (MODE == SMode::CG_Q) {
+            // CG.c:128-150: s = r/diag; d = s + beta*d  (first pass: d as given)
+            if (a.cg_first) return v[1];
+            T s = v[0];
+            if (a.precond) s = A::div(s, dg);
+            return A::add(s, A::mul(cgb, v[1]));
+        } else {
+            return v[0];
+        }
+    };
+
+    T q[PY][QN];
+    T qn[PY][NF];
+    T hy[3][NF], hx[NXH][NF];
+    T en[PY][NE > 0 ? NE : 1] = {};
+
+    // prologue: queue planes z0-R .. z0+R-1, then prefetch plane z0+R and
+    // the halo / epilogue inputs of plane z0
+#pragma unroll
+    for (int s = 0; s < 2 * R; s++) {
+        const int zo = modp(z0 - R + s, nz) * nxy;
+#pragma unroll
+        for (int p = 0; p < PY; p++) {
+            T v[NF];
+            load_raw(c_off[p] + zo, v);
+            q[p][s] = make_u(v);
+        }
+    }
+    {
+        const int zq = modp(z0 + R, nz) * nxy, zc = modp(z0, nz) * nxy;
+#pragma unroll
+        for (int p = 0; p < PY; p++) load_raw(c_off[p] + zq, qn[p]);
+#pragma unroll
+        for (int k = 0; k < 3; k++) load_raw(yh_off[k] + zc, hy[k]);
+#pragma unroll
+        for (int k = 0; k < NXH; k++)
+            if (xh_ok[k]) load_raw(xh_off[k] + zc, hx[k]);
+        if constexpr (NE > 0) {
+            if (a.bvec) {
+#pragma unroll
+                for (int p = 0; p < PY; p++) en[p][0] = a.bvec[c_off[p] + zc];
+            }
+        }
+    }
+
+    double acc[NACC];
+#pragma unroll
+    for (int k = 0; k < NACC; k++) acc[k] = 0.0;
+
+    for (int z = z0; z < z1; z++) {
+        const int buf = (z - z0) & 1;
+        // queue top <- plane z+R; LDS <- plane z (centre + halo)
+#pragma unroll
+        for (int p = 0; p < PY; p++) {
+            q[p][2 * R] = make_u(qn[p]);
+            lds[buf][R + ty * PY + p][R + tx] = q[p][R];
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) lds[buf][yh_row[k]][R + tx] = make_u(hy[k]);
+#pragma unroll
+        for (int k = 0; k < NXH; k++)
+            if (xh_ok[k]) lds[buf][xh_row[k]][xh_col[k]] = make_u(hx[k]);
+        T ev[PY][NE > 0 ? NE : 1];
+        if constexpr (NE > 0) {
+#pragma unroll
+            for (int p = 0; p < PY; p++) ev[p][0] = en[p][0];
+        }
+        // prefetch plane z+1 (halo, epilogue inputs) and z+R+1 (queue)
+        if (z + 1 < z1) {
+            const int zq = modp(z + 1 + R, nz) * nxy, zc = modp(z + 1, nz) * nxy;
+#pragma unroll
+            for (int p = 0; p < PY; p++) load_raw(c_off[p] + zq, qn[p]);
+#pragma unroll
+            for (int k = 0; k < 3; k++) load_raw(yh_off[k] + zc, hy[k]);
+#pragma unroll
+            for (int k = 0; k < NXH; k++)
+                if (xh_ok[k]) load_raw(xh_off[k] + zc, hx[k]);
+            if constexpr (NE > 0) {
+                if (a.bvec) {
+#pragma unroll
+                    for (int p = 0; p < PY; p++) en[p][0] = a.bvec[c_off[p] + zc];
+                }
+            }
+        }
+        __syncthreads();
+
+        const int zoff = z * nxy;
+#pragma unroll
+        for (int p = 0; p < PY; p++) {
+            const int lr = R + ty * PY + p, lc = R + tx;
+            const T u = q[p][R];
+            // system.c:195-201 order: centre, then o = 1..6 with
+            // (x-, x+, y-, y+, z-, z+) summed left to right
+            T y = A::mul(A::mul(u, (T)3), wt0);
+#pragma unroll
+            for (int o = 1; o <= R; o++) {
+                T nb = A::add(lds[buf][lr][lc - o], lds[buf][lr][lc + o]);
+                nb = A::add(nb, lds[buf][lr - o][lc]);
+                nb = A::add(nb, lds[buf][lr + o][lc]);
+                nb = A::add(nb, q[p][R - o]);
+                nb = A::add(nb, q[p][R + o]);
+                y = A::add(y, A::mul(nb, wt[o]));
+            }
+            if (a.sigma != 0.0) y = A::add(y, A::mul(sig, u));
+            if (!c_ok[p]) continue;
+            const int g = c_off[p] + zoff;
+            if constexpr (MODE == SMode::MATVEC) {
+                // optional residual norm of b - A x (PGR.c:79-81 start-up)
+                if (a.out0) a.out0[g] = y;
+                if (a.bvec) acc[0] += A::sq(A::sub(ev[p][0], y));
+            } else if constexpr (MODE == SMode::AAR_RICH) {
+                // AAR.c:80-82/93-94/134-139: f = (rhs - A x)/diag; x_new = x + omega*f
+                const T r = A::sub(ev[p][0], y);
+                acc[0] += A::sq(r);
+                T f = r;
+                if (a.precond) f = A::div(f, dg);
+                a.out0[g] = f;
+                if (a.write_x) a.out1[g] = A::add(u, A::mul(om, f));
+            } else if constexpr (MODE == SMode::RESID) {
+                const T r = A::sub(ev[p][0], y);
+                acc[0] += A::sq(r);
+                if (a.out32) a.out32[g] = (float)r;
+                if (a.out0) a.out0[g] = r;
+            } else if constexpr (MODE == SMode::PGR_RICH) {
+                // PGR.c:95-103: x_old=x; x += omega*f; Ax = A x; f = rhs - Ax
+                a.out0[g] = u;
+                a.out1[g] = y;
+                if (a.trace) acc[0] += A::sq(A::sub(ev[p][0], y));
+            } else if constexpr (MODE == SMode::CG_Q) {
+                // CG.c:94-100: q = A d; alpha partial d.q
+                a.out0[g] = u;
+                a.out1[g] = y;
+                acc[0] += (double)u * (double)y;
+            } else if constexpr (MODE == SMode::CG_RESID) {
+                // CG.c:65-84 / 112-119: r = b - A x; s = r/diag; r.s, r.r
+                const T r = A::sub(ev[p][0], y);
+                a.out0[g] = r;
+                T s = r;
+                if (a.precond) s = A::div(s, dg);
+                if (a.out1) a.out1[g] = s;
+                acc[0] += (double)r * (double)s;
+                acc[1] += A::sq(r);
+            }
+        }
+        // shift the queue one plane
+#pragma unroll
+        for (int p = 0; p < PY; p++)
+#pragma unroll
+            for (int s = 0; s < 2 * R; s++) q[p][s] = q[p][s + 1];
+    }
+
+    if (a.partials) {
+        block_reduce_to<NACC>(acc, a.partials + (size_t)item * NACC, red_scratch);
+        if (last_block_arrive(a.counter, gridDim.x)) {
+            final_sum(a.partials, a.nitems, NACC, red_out, red_scratch);
+            if (tid == 0) {
+                finalize(a.fin, a.ctrl, red_out, NACC);
+                *a.counter = 0u;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host-side launch
+// ---------------------------------------------------------------------------
+
+template <typename T>
+void stencil_geometry(SArgs<T> &a, int py)
+{
+    const int ty = TYT * py;
+    a.nxy = a.nx * a.ny;
+    a.ntx = (a.nx + TX - 1) / TX;
+    a.nty = (a.ny + ty - 1) / ty;
+    if (a.lz <= 0) a.lz = a.nz;
+    if (a.lz > a.nz) a.lz = a.nz;
+    a.nzc = (a.nz + a.lz - 1) / a.lz;
+    a.nitems = a.ntx * a.nty * a.nzc;
+}
+
+template <typename T, SMode M>
+hipError_t launch_stencil_t(SArgs<T> a, int py, hipStream_t st)
+{
+    stencil_geometry(a, py);
+    dim3 grid(a.nitems), block(BLOCK);
+    switch (py) {
+    case 2: hipLaunchKernelGGL((stencil_kernel<T, 2, M>), grid, block, 0, st, a); break;
+    case 4: hipLaunchKernelGGL((stencil_kernel<T, 4, M>), grid, block, 0, st, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_stencil(SMode m, const SArgs<T> &a, int py, hipStream_t st)
+{
+    switch (m) {
+    case SMode::MATVEC: return launch_stencil_t<T, SMode::MATVEC>(a, py, st);
+    case SMode::AAR_RICH: return launch_stencil_t<T, SMode::AAR_RICH>(a, py, st);
+    case SMode::RESID: return launch_stencil_t<T, SMode::RESID>(a, py, st);
+    case SMode::PGR_RICH: return launch_stencil_t<T, SMode::PGR_RICH>(a, py, st);
+    case SMode::CG_Q: return launch_stencil_t<T, SMode::CG_Q>(a, py, st);
+    case SMode::CG_RESID: return launch_stencil_t<T, SMode::CG_RESID>(a, py, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <typename T>
+int stencil_items(const SArgs<T> &a0, int py)
+{
+    SArgs<T> a = a0;
+    stencil_geometry(a, py);
+    return a.nitems;
+}
+
+template hipError_t launch_stencil<double>(SMode, const SArgs<double> &, int, hipStream_t);
+template hipError_t launch_stencil<float>(SMode, const SArgs<float> &, int, hipStream_t);
+template int stencil_items<double>(const SArgs<double> &, int);
+template int stencil_items<float>(const SArgs<float> &, int);
+
+}  // namespace aar

@@ -1,0 +1,742 @@
+/*
+ * aar_oracle.c -- CPU restatement of the reference MPI_REAL solver path
+ * (one rank, no MPI).  TEST INFRASTRUCTURE: the parity checker for the HIP
+ * product, never a part of it.  See aar_oracle.h for the contract.
+ *
+ * Every routine keeps the reference's floating-point operation order
+ * (left-to-right sums, no fused multiply-add, division where the
+ * reference divides) so that, built with gcc -O3 -funroll-loops
+ * -ffp-contract=off, its results are bit-identical to the reference's on
+ * one rank.  Citations are file:line in the reference tree (MPI_REAL/).
+ */
+#include "aar_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+static double now_seconds(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+/* ------------------------------------------------------------------ */
+/* Finite-difference weights: system.c:480-498.                        */
+/* c0 = sum_p -2/p^2; c_p = 2 (-1)^(p+1) [prod_{FDn-p+1..FDn} i /      */
+/* prod_{FDn+1..FDn+p} i] / p^2, evaluated in that order.              */
+/* ------------------------------------------------------------------ */
+void orc_lap_coefficient(double *coeff, int fdn)
+{
+    double acc = 0.0;
+    for (int p = 1; p <= fdn; p++)
+        acc += -(2.0 / (p * p));
+    coeff[0] = acc;
+    for (int p = 1; p <= fdn; p++) {
+        double num = 1.0, den = 1.0;
+        for (int i = fdn - p + 1; i <= fdn; i++) num *= i;
+        for (int i = fdn + 1; i <= fdn + p; i++) den *= i;
+        double ratio = num / den;
+        double sgn = (p % 2 == 1) ? 1.0 : -1.0;   /* pow(-1, p+1), exact */
+        coeff[p] = 2.0 * sgn * ratio / (p * p);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Stencil: system.c:189-205.  y = (x*3)*w0, then for o = 1..FDn       */
+/* y += (x[i-o] + x[i+o] + x[j-o] + x[j+o] + x[k-o] + x[k+o]) * w_o    */
+/* with w_o = a*coeff[o] and periodic wrap; x fastest (system.c:82).   */
+/* ------------------------------------------------------------------ */
+static inline int wrap(int i, int n) { i %= n; return i < 0 ? i + n : i; }
+
+void orc_matvec(int nx, int ny, int nz, int fdn, const double *coeff,
+                double a, double sigma, const double *x, double *y)
+{
+    double w[32];
+    for (int o = 0; o <= fdn; o++) w[o] = a * coeff[o];
+    const long sxy = (long)nx * ny;
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                const long c = i + (long)j * nx + k * sxy;
+                double acc = x[c] * 3 * w[0];
+                for (int o = 1; o <= fdn; o++) {
+                    double nb = x[wrap(i - o, nx) + (long)j * nx + k * sxy];
+                    nb = nb + x[wrap(i + o, nx) + (long)j * nx + k * sxy];
+                    nb = nb + x[i + (long)wrap(j - o, ny) * nx + k * sxy];
+                    nb = nb + x[i + (long)wrap(j + o, ny) * nx + k * sxy];
+                    nb = nb + x[i + (long)j * nx + wrap(k - o, nz) * sxy];
+                    nb = nb + x[i + (long)j * nx + wrap(k + o, nz) * sxy];
+                    acc += nb * w[o];
+                }
+                y[c] = acc;
+            }
+    if (sigma != 0.0) {
+        const long n = sxy * nz;
+        for (long c = 0; c < n; c++) y[c] = y[c] + sigma * x[c];
+    }
+}
+
+/* tools.c:18-28 and tools.c:35-43: plain sequential sums. */
+double orc_norm2(const double *v, long n)
+{
+    double s = 0.0;
+    for (long k = 0; k < n; k++) s = s + v[k] * v[k];
+    return sqrt(s);
+}
+
+double orc_dot(const double *u, const double *v, long n)
+{
+    double s = 0.0;
+    for (long k = 0; k < n; k++) s = s + u[k] * v[k];
+    return s;
+}
+
+/* ------------------------------------------------------------------ */
+/* SVD and pseudo-inverse: tools.c:52-352.                             */
+/* Golub-Reinsch: Householder bidiagonalisation, accumulation of the   */
+/* right transforms into V, implicit-shift QR on the bidiagonal (at    */
+/* most 250 sweeps per singular value).  Only w and V are formed: the  */
+/* reference overwrites U with V before use (tools.c:324-329), and no  */
+/* U-only step feeds w, the superdiagonal or V.  The left Householder  */
+/* step applies its reflector to columns 1..n-1 (tools.c:137), which   */
+/* re-scales the current reflector before the trailing columns are    */
+/* updated; that order is kept because it moves the result by an ulp.  */
+/* ------------------------------------------------------------------ */
+double orc_pythag(double a, double b)
+{
+    double absa = fabs(a), absb = fabs(b);
+    if (absa > absb) return absa * sqrt(1.0 + absb * absb / (absa * absa));
+    return absb == 0.0 ? 0.0 : absb * sqrt(1.0 + absa * absa / (absb * absb));
+}
+
+static inline double sign_of(double mag, double s) { return s >= 0.0 ? fabs(mag) : -fabs(mag); }
+static inline double dmax(double a, double b) { return a > b ? a : b; }
+
+/* a: n*n row-major (overwritten); w: n; v: n*n row-major.  Returns 0/1. */
+static int gr_svd_wv(double *a, int n, double *w, double *v)
+{
+#define A(r, c) a[(r) * n + (c)]
+#define V(r, c) v[(r) * n + (c)]
+    const int m = n;
+    double *rv1 = (double *)calloc((size_t)n, sizeof(double));
+    double g = 0.0, scale = 0.0, anorm = 0.0, s, f, h;
+    int l = 0;
+
+    for (int i = 0; i < n; i++) {
+        l = i + 1;
+        rv1[i] = scale * g;
+        g = s = scale = 0.0;
+        if (i < m) {
+            for (int k = i; k < m; k++) scale += fabs(A(k, i));
+            if (scale != 0.0) {
+                for (int k = i; k < m; k++) {
+                    A(k, i) /= scale;
+                    s += A(k, i) * A(k, i);
+                }
+                f = A(i, i);
+                g = -sign_of(sqrt(s), f);
+                h = f * g - s;
+                A(i, i) = f - g;
+                for (int j = 1; j < n; j++) {          /* tools.c:137 starts at 1 */
+                    double t = 0.0;
+                    for (int k = i; k < m; k++) t += A(k, i) * A(k, j);
+                    double fac = t / h;
+                    for (int k = i; k < m; k++) A(k, j) += fac * A(k, i);
+                }
+                for (int k = i; k < m; k++) A(k, i) *= scale;
+            }
+        }
+        w[i] = scale * g;
+        g = s = scale = 0.0;
+        if (i <= m - 1 && i != n - 1) {
+            for (int k = l; k < n; k++) scale += fabs(A(i, k));
+            if (scale != 0.0) {
+                for (int k = l; k < n; k++) {
+                    A(i, k) /= scale;
+                    s += A(i, k) * A(i, k);
+                }
+                f = A(i, l);
+                g = -sign_of(sqrt(s), f);
+                h = f * g - s;
+                A(i, l) = f - g;
+                for (int k = l; k < n; k++) rv1[k] = A(i, k) / h;
+                for (int j = l; j < m; j++) {
+                    double t = 0.0;
+                    for (int k = l; k < n; k++) t += A(j, k) * A(i, k);
+                    for (int k = l; k < n; k++) A(j, k) += t * rv1[k];
+                }
+                for (int k = l; k < n; k++) A(i, k) *= scale;
+            }
+        }
+        anorm = dmax(anorm, fabs(w[i]) + fabs(rv1[i]));
+    }
+
+    /* right-hand transforms -> V */
+    for (int i = n - 1; i >= 0; i--) {
+        if (i < n - 1) {
+            if (g != 0.0) {
+                for (int j = l; j < n; j++) V(j, i) = (A(i, j) / A(i, l)) / g;
+                for (int j = l; j < n; j++) {
+                    double t = 0.0;
+                    for (int k = l; k < n; k++) t += A(i, k) * V(k, j);
+                    for (int k = l; k < n; k++) V(k, j) += t * V(k, i);
+                }
+            }
+            for (int j = l; j < n; j++) V(i, j) = V(j, i) = 0.0;
+        }
+        V(i, i) = 1.0;
+        g = rv1[i];
+        l = i;
+    }
+
+    /* implicit-shift QR on the bidiagonal (w, rv1) */
+    int status = 0;
+    for (int k = n - 1; k >= 0; k--) {
+        for (int its = 0; its <= 250; its++) {
+            int cancel = 1, nm = 0;
+            for (l = k; l >= 0; l--) {
+                nm = l - 1;
+                if ((double)(fabs(rv1[l]) + anorm) == anorm) { cancel = 0; break; }
+                if ((double)(fabs(w[nm]) + anorm) == anorm) break;
+            }
+            if (cancel) {
+                double c = 0.0;
+                s = 1.0;
+                for (int i = l; i <= k; i++) {
+                    f = s * rv1[i];
+                    rv1[i] = c * rv1[i];
+                    if ((double)(fabs(f) + anorm) == anorm) break;
+                    g = w[i];
+                    h = orc_pythag(f, g);
+                    w[i] = h;
+                    h = 1.0 / h;
+                    c = g * h;
+                    s = -f * h;
+                }
+            }
+            double z = w[k];
+            if (l == k) {
+                if (z < 0.0) {
+                    w[k] = -z;
+                    for (int j = 0; j < n; j++) V(j, k) = -V(j, k);
+                }
+                break;
+            }
+            if (its == 250) { status = 1; break; }
+            double x = w[l];
+            nm = k - 1;
+            double y = w[nm];
+            g = rv1[nm];
+            h = rv1[k];
+            f = ((y - z) * (y + z) + (g - h) * (g + h)) / (2.0 * h * y);
+            g = orc_pythag(f, 1.0);
+            f = ((x - z) * (x + z) + h * ((y / (f + sign_of(g, f))) - h)) / x;
+            double c = 1.0;
+            s = 1.0;
+            for (int j = l; j <= nm; j++) {
+                int i = j + 1;
+                g = rv1[i];
+                y = w[i];
+                h = s * g;
+                g = c * g;
+                z = orc_pythag(f, h);
+                rv1[j] = z;
+                c = f / z;
+                s = h / z;
+                f = x * c + g * s;
+                g = g * c - x * s;
+                h = y * s;
+                y *= c;
+                for (int jj = 0; jj < n; jj++) {
+                    double vx = V(jj, j), vz = V(jj, i);
+                    V(jj, j) = vx * c + vz * s;
+                    V(jj, i) = vz * c - vx * s;
+                }
+                z = orc_pythag(f, h);
+                w[j] = z;
+                if (z != 0.0) {
+                    z = 1.0 / z;
+                    c = f * z;
+                    s = h * z;
+                }
+                f = c * g + s * y;
+                x = c * y - s * g;
+            }
+            rv1[l] = 0.0;
+            rv1[k] = f;
+            w[k] = x;
+        }
+        if (status) break;
+    }
+    free(rv1);
+
+    /* tools.c:331-335: zero w_j < n * w_max * 2.22044605e-16 */
+    double wmax = 0.0;
+    for (int j = 0; j < n; j++) if (w[j] > wmax) wmax = w[j];
+    double wmin = n * wmax * (2.22044605e-16);
+    for (int j = 0; j < n; j++) if (w[j] < wmin) w[j] = 0.0;
+    return status;
+#undef A
+#undef V
+}
+
+int orc_pinv_times_vec(const double *Ain, const double *b, double *x, int m,
+                       double *w_out, double *v_out)
+{
+    double *a = (double *)malloc(sizeof(double) * m * m);
+    double *v = (double *)calloc((size_t)m * m, sizeof(double));
+    double *w = (double *)calloc((size_t)m, sizeof(double));
+    double *tmp = (double *)calloc((size_t)m, sizeof(double));
+    memcpy(a, Ain, sizeof(double) * m * m);
+    int st = gr_svd_wv(a, m, w, v);
+    /* tools.c:76-94: x = V diag(1/w) V^T b, with U replaced by V */
+    for (int j = 0; j < m; j++) {
+        double s = 0.0;
+        if (w[j] != 0.0) {
+            for (int i = 0; i < m; i++) s += v[i * m + j] * b[i];
+            s /= w[j];
+        }
+        tmp[j] = s;
+    }
+    for (int j = 0; j < m; j++) {
+        double s = 0.0;
+        for (int jj = 0; jj < m; jj++) s += v[j * m + jj] * tmp[jj];
+        x[j] = s;
+    }
+    if (w_out) memcpy(w_out, w, sizeof(double) * m);
+    if (v_out) memcpy(v_out, v, sizeof(double) * m * m);
+    free(a); free(v); free(w); free(tmp);
+    return st;
+}
+
+/* ------------------------------------------------------------------ */
+/* AAR.c:185-232 AndersonExtrapolation (one rank: the allreduce of     */
+/* AAR.c:212 is the identity).                                         */
+/* ------------------------------------------------------------------ */
+int orc_anderson_extrapolation(const double *DX, const double *DF, const double *f,
+                               double beta, int m, long n, double *am,
+                               double *G_out, double *s_out)
+{
+    double *red = (double *)calloc((size_t)(m * m + m), sizeof(double));
+    double *G = (double *)calloc((size_t)m * m, sizeof(double));
+    double *g = (double *)calloc((size_t)m, sizeof(double));
+    double *s = (double *)calloc((size_t)m, sizeof(double));
+    for (int j = 0; j < m; j++)
+        for (int i = 0; i <= j; i++) {
+            double t = orc_dot(DF + (long)i * n, DF + (long)j * n, n);
+            red[j * m + i] = t;
+            red[i * m + j] = t;
+        }
+    for (int j = 0; j < m; j++) red[m * m + j] = orc_dot(DF + (long)j * n, f, n);
+    /* FtF[i][j] = red[j*m + i]; Ftf[j] = red[m*m + j] (AAR.c:214-221) */
+    for (int j = 0; j < m; j++)
+        for (int i = 0; i < m; i++) G[i * m + j] = red[j * m + i];
+    for (int j = 0; j < m; j++) g[j] = red[m * m + j];
+    int st = orc_pinv_times_vec(G, g, s, m, NULL, NULL);
+    for (long k = 0; k < n; k++) {
+        double t = 0.0;
+        for (int j = 0; j < m; j++)
+            t = t + (DX[(long)j * n + k] + beta * DF[(long)j * n + k]) * s[j];
+        am[k] = t;
+    }
+    if (G_out) memcpy(G_out, red, sizeof(double) * (m * m + m));
+    if (s_out) memcpy(s_out, s, sizeof(double) * m);
+    free(red); free(G); free(g); free(s);
+    return st;
+}
+
+static void finish_stats(orc_stats *st, int iters, double relres, double rhs_norm,
+                         double tol_abs, int loop_iter, int max_iter, int trace_len,
+                         double secs, int status)
+{
+    if (!st) return;
+    st->iterations = iters;
+    st->relres = relres / rhs_norm;
+    st->rhs_norm = rhs_norm;
+    st->converged = relres <= tol_abs;
+    st->printed_converged = (loop_iter < max_iter && relres <= tol_abs);
+    st->printed_warning = (loop_iter >= max_iter);
+    st->trace_len = trace_len;
+    st->seconds = secs;
+    st->status = status;
+}
+
+/* ------------------------------------------------------------------ */
+/* AAR.c:37-177.                                                       */
+/* ------------------------------------------------------------------ */
+int orc_aar(int nx, int ny, int nz, int fdn, double a, double sigma,
+            double *x, const double *rhs, double omega, double beta,
+            int m, int p, int max_iter, double tol, int precond,
+            double *trace, int trace_cap, orc_stats *st)
+{
+    const long n = (long)nx * ny * nz;
+    double coeff[32] = {0};
+    orc_lap_coefficient(coeff, fdn);
+    double diag = a * (3 * coeff[0]);
+    if (sigma != 0.0) diag += sigma;
+
+    double *f = (double *)calloc((size_t)n, sizeof(double));
+    double *x_old = (double *)calloc((size_t)n, sizeof(double));
+    double *f_old = (double *)calloc((size_t)n, sizeof(double));
+    double *am = (double *)calloc((size_t)n, sizeof(double));
+    double *DX = (double *)calloc((size_t)n * m, sizeof(double));
+    double *DF = (double *)calloc((size_t)n * m, sizeof(double));
+    int tlen = 0, status = 0;
+
+    int iter = 1;
+    double relres = tol + 1;                 /* AAR.c:76, before tol is scaled */
+    double rhs_norm = orc_norm2(rhs, n);
+    tol *= rhs_norm;
+    orc_matvec(nx, ny, nz, fdn, coeff, a, sigma, x, f);
+    for (long i = 0; i < n; i++) f[i] = rhs[i] - f[i];
+    if (trace && tlen < trace_cap) trace[tlen++] = orc_norm2(f, n);
+
+    double t0 = now_seconds();
+    while (relres > tol && iter <= max_iter) {
+        if (precond)
+            for (long i = 0; i < n; i++) f[i] /= diag;
+        if (iter > 1) {
+            const int col = (iter - 2) % m;
+            double *dx = DX + (long)col * n, *df = DF + (long)col * n;
+            for (long i = 0; i < n; i++) {
+                dx[i] = x[i] - x_old[i];
+                df[i] = f[i] - f_old[i];
+            }
+        }
+        memcpy(x_old, x, sizeof(double) * n);
+        memcpy(f_old, f, sizeof(double) * n);
+        double rn = -1.0;
+        if (iter % p == 0 && iter > 1) {
+            if (orc_anderson_extrapolation(DX, DF, f, beta, m, n, am, NULL, NULL)) { status = 1; break; }
+            for (long i = 0; i < n; i++) x[i] = x_old[i] + beta * f[i] - am[i];
+            orc_matvec(nx, ny, nz, fdn, coeff, a, sigma, x, f);
+            for (long i = 0; i < n; i++) f[i] = rhs[i] - f[i];
+            relres = orc_norm2(f, n);
+            rn = relres;
+        } else {
+            for (long i = 0; i < n; i++) x[i] = x_old[i] + omega * f[i];
+            orc_matvec(nx, ny, nz, fdn, coeff, a, sigma, x, f);
+            for (long i = 0; i < n; i++) f[i] = rhs[i] - f[i];
+        }
+        if (trace && tlen < trace_cap) trace[tlen++] = rn >= 0.0 ? rn : orc_norm2(f, n);
+        iter = iter + 1;
+    }
+    double t1 = now_seconds();
+    finish_stats(st, iter - 1, relres, rhs_norm, tol, iter, max_iter, tlen, t1 - t0, status);
+    free(f); free(x_old); free(f_old); free(am); free(DX); free(DF);
+    return status;
+}
+
+/* ------------------------------------------------------------------ */
+/* PGR.c:35-218 and PL2R.c:35-218.                                     */
+/* ------------------------------------------------------------------ */
+int orc_pgr(int nx, int ny, int nz, int fdn, double a, double sigma,
+            double *x, const double *rhs, double omega, int m, int p,
+            int max_iter, double tol, int precond, int l2,
+            double *trace, int trace_cap, orc_stats *st)
+{
+    const long n = (long)nx * ny * nz;
+    double coeff[32] = {0};
+    orc_lap_coefficient(coeff, fdn);
+    double diag = a * (3 * coeff[0]);
+    if (sigma != 0.0) diag += sigma;
+
+    double *Ax = (double *)calloc((size_t)n, sizeof(double));
+    double *Ax_old = (double *)calloc((size_t)n, sizeof(double));
+    double *f = (double *)calloc((size_t)n, sizeof(double));
+    double *x_old = (double *)calloc((size_t)n, sizeof(double));
+    double *DX = (double *)calloc((size_t)n * m, sizeof(double));
+    double *DF = (double *)calloc((size_t)n * m, sizeof(double));
+    double *red = (double *)calloc((size_t)(m * m + m), sizeof(double));
+    double *G = (double *)calloc((size_t)m * m, sizeof(double));
+    double *g = (double *)calloc((size_t)m, sizeof(double));
+    double *s = (double *)calloc((size_t)m, sizeof(double));
+    int tlen = 0, status = 0;
+
+    int iter = 1;
+    double relres = tol + 1;
+    double rhs_norm = orc_norm2(rhs, n);
+    tol *= rhs_norm;
+    orc_matvec(nx, ny, nz, fdn, coeff, a, sigma, x, Ax);
+    for (long i = 0; i < n; i++) f[i] = rhs[i] - Ax[i];
+    if (trace && tlen < trace_cap) trace[tlen++] = orc_norm2(f, n);
+
+    double t0 = now_seconds();
+    while (relres > tol && iter <= max_iter) {
+        if (precond)
+            for (long i = 0; i < n; i++) f[i] /= diag;
+        for (long i = 0; i < n; i++) {
+            x_old[i] = x[i];
+            Ax_old[i] = Ax[i];
+            x[i] += (omega * f[i]);
+        }
+        orc_matvec(nx, ny, nz, fdn, coeff, a, sigma, x, Ax);
+        for (long i = 0; i < n; i++) f[i] = rhs[i] - Ax[i];
+        if (trace && tlen < trace_cap) trace[tlen++] = orc_norm2(f, n);
+
+        const int col = (iter - 2 + m) % m;
+        double *dx = DX + (long)col * n, *df = DF + (long)col * n;
+        for (long i = 0; i < n; i++) {
+            dx[i] = x[i] - x_old[i];
+            df[i] = Ax[i] - Ax_old[i];
+        }
+        if (iter % p == 0) {
+            /* PGR.c:181-218 (DX^T DF, DX^T f) or PL2R.c:180-218 (DF^T DF, DF^T f) */
+            if (!l2) {
+                for (int j = 0; j < m; j++)
+                    for (int i = 0; i < m; i++)
+                        red[j * m + i] = orc_dot(DX + (long)i * n, DF + (long)j * n, n);
+                for (int j = 0; j < m; j++) red[m * m + j] = orc_dot(DX + (long)j * n, f, n);
+            } else {
+                for (int j = 0; j < m; j++)
+                    for (int i = 0; i <= j; i++) {
+                        double t = orc_dot(DF + (long)i * n, DF + (long)j * n, n);
+                        red[j * m + i] = t;
+                        red[i * m + j] = t;
+                    }
+                for (int j = 0; j < m; j++) red[m * m + j] = orc_dot(DF + (long)j * n, f, n);
+            }
+            for (int j = 0; j < m; j++)
+                for (int i = 0; i < m; i++) G[i * m + j] = red[j * m + i];
+            for (int j = 0; j < m; j++) g[j] = red[m * m + j];
+            if (orc_pinv_times_vec(G, g, s, m, NULL, NULL)) { status = 1; break; }
+            for (int i = 0; i < m; i++) {
+                const double *dxi = DX + (long)i * n, *dfi = DF + (long)i * n;
+                for (long j = 0; j < n; j++) {
+                    x[j] = x[j] + dxi[j] * s[i];
+                    Ax[j] = Ax[j] + dfi[j] * s[i];
+                }
+            }
+            for (long i = 0; i < n; i++) {
+                dx[i] = x[i] - x_old[i];
+                df[i] = Ax[i] - Ax_old[i];
+            }
+        }
+        for (long i = 0; i < n; i++) f[i] = rhs[i] - Ax[i];
+        if (iter % p == 0) relres = orc_norm2(f, n);
+        iter = iter + 1;
+    }
+    double t1 = now_seconds();
+    finish_stats(st, iter - 1, relres, rhs_norm, tol, iter, max_iter, tlen, t1 - t0, status);
+    free(Ax); free(Ax_old); free(f); free(x_old); free(DX); free(DF);
+    free(red); free(G); free(g); free(s);
+    return status;
+}
+
+/* ------------------------------------------------------------------ */
+/* CG.c:33-175.                                                        */
+/* ------------------------------------------------------------------ */
+int orc_cg(int nx, int ny, int nz, int fdn, double a, double sigma,
+           double *x, const double *b, int max_iter, double tol, int precond,
+           double *trace, int trace_cap, orc_stats *st)
+{
+    const long n = (long)nx * ny * nz;
+    double coeff[32] = {0};
+    orc_lap_coefficient(coeff, fdn);
+    double diag = a * (3 * coeff[0]);
+    if (sigma != 0.0) diag += sigma;
+    double *r = (double *)malloc(sizeof(double) * n);
+    double *d = (double *)malloc(sizeof(double) * n);
+    double *s = (double *)malloc(sizeof(double) * n);
+    double *q = (double *)malloc(sizeof(double) * n);
+    int tlen = 0;
+
+    double b_2norm = orc_norm2(b, n);
+    orc_matvec(nx, ny, nz, fdn, coeff, a, sigma, x, r);
+    for (long i = 0; i < n; i++) {
+        r[i] = b[i] - r[i];
+        d[i] = r[i];
+    }
+    if (precond)
+        for (long i = 0; i < n; i++) d[i] /= diag;
+    double delta_new = orc_dot(r, d, n), delta_old;
+    double relres = precond ? orc_norm2(r, n) : sqrt(delta_new);
+    if (trace && tlen < trace_cap) trace[tlen++] = orc_norm2(r, n);
+    tol *= b_2norm;
+
+    int iter_count = 0;
+    double t0 = now_seconds();
+    while (iter_count < max_iter && relres > tol) {
+        orc_matvec(nx, ny, nz, fdn, coeff, a, sigma, d, q);
+        double alpha = orc_dot(d, q, n);
+        alpha = delta_new / alpha;
+        for (long i = 0; i < n; i++) x[i] = x[i] + alpha * d[i];
+        if ((iter_count % 50) == 0) {
+            orc_matvec(nx, ny, nz, fdn, coeff, a, sigma, x, r);
+            for (long i = 0; i < n; i++) {
+                r[i] = b[i] - r[i];
+                s[i] = r[i];
+            }
+            if (trace && tlen < trace_cap) trace[tlen++] = orc_norm2(r, n);
+        } else {
+            for (long i = 0; i < n; i++) {
+                r[i] = r[i] - alpha * q[i];
+                s[i] = r[i];
+            }
+        }
+        if (precond)
+            for (long i = 0; i < n; i++) s[i] /= diag;
+        delta_old = delta_new;
+        delta_new = orc_dot(r, s, n);
+        relres = precond ? orc_norm2(r, n) : sqrt(delta_new);
+        double beta = delta_new / delta_old;
+        for (long i = 0; i < n; i++) d[i] = s[i] + beta * d[i];
+        iter_count++;
+    }
+    double t1 = now_seconds();
+    finish_stats(st, iter_count, relres, b_2norm, tol, iter_count, max_iter, tlen, t1 - t0, 0);
+    free(r); free(d); free(s); free(q);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Complex Helmholtz (extension).  Operator order per point:           */
+/*   lap = a*Lap(u) per component in the order of system.c:189-205,   */
+/*   y_re = lap_re + (c*u_re - eta*u_im), y_im = lap_im + (c*u_im +    */
+/*   eta*u_re).  Jacobi divides by d = (a*(3*c0_lap) + c) + i*eta with */
+/*   f = ((r_re d_re + r_im d_im) + i (r_im d_re - r_re d_im)) / |d|^2.*/
+/* ------------------------------------------------------------------ */
+void orc_matvec_complex(int nx, int ny, int nz, int fdn, const double *coeff,
+                        double a, const double *c, double c0, double eta,
+                        const double *u, double *y)
+{
+    const long n = (long)nx * ny * nz;
+    double *ur = (double *)malloc(sizeof(double) * n);
+    double *ui = (double *)malloc(sizeof(double) * n);
+    double *yr = (double *)malloc(sizeof(double) * n);
+    double *yi = (double *)malloc(sizeof(double) * n);
+    for (long k = 0; k < n; k++) { ur[k] = u[2 * k]; ui[k] = u[2 * k + 1]; }
+    orc_matvec(nx, ny, nz, fdn, coeff, a, 0.0, ur, yr);
+    orc_matvec(nx, ny, nz, fdn, coeff, a, 0.0, ui, yi);
+    for (long k = 0; k < n; k++) {
+        double ck = c ? c[k] : c0;
+        y[2 * k] = yr[k] + (ck * ur[k] - eta * ui[k]);
+        y[2 * k + 1] = yi[k] + (ck * ui[k] + eta * ur[k]);
+    }
+    free(ur); free(ui); free(yr); free(yi);
+}
+
+static double cnorm2(const double *v, long n)
+{
+    double s = 0.0;
+    for (long k = 0; k < n; k++) s = s + (v[2 * k] * v[2 * k] + v[2 * k + 1] * v[2 * k + 1]);
+    return sqrt(s);
+}
+
+/* <u, v> = sum conj(u) v */
+static void cdot(const double *u, const double *v, long n, double *re, double *im)
+{
+    double sr = 0.0, si = 0.0;
+    for (long k = 0; k < n; k++) {
+        sr = sr + (u[2 * k] * v[2 * k] + u[2 * k + 1] * v[2 * k + 1]);
+        si = si + (u[2 * k] * v[2 * k + 1] - u[2 * k + 1] * v[2 * k]);
+    }
+    *re = sr;
+    *im = si;
+}
+
+int orc_aar_complex(int nx, int ny, int nz, int fdn, double a,
+                    const double *c, double c0, double eta,
+                    double *x, const double *rhs, double omega, double beta,
+                    int m, int p, int max_iter, double tol,
+                    double *trace, int trace_cap, orc_stats *st)
+{
+    const long n = (long)nx * ny * nz;
+    const long n2 = 2 * n;
+    double coeff[32] = {0};
+    orc_lap_coefficient(coeff, fdn);
+    const double dlap = a * (3 * coeff[0]);
+    double *f = (double *)calloc((size_t)n2, sizeof(double));
+    double *x_old = (double *)calloc((size_t)n2, sizeof(double));
+    double *f_old = (double *)calloc((size_t)n2, sizeof(double));
+    double *DX = (double *)calloc((size_t)n2 * m, sizeof(double));
+    double *DF = (double *)calloc((size_t)n2 * m, sizeof(double));
+    const int m2 = 2 * m;
+    double *E = (double *)calloc((size_t)m2 * m2, sizeof(double));
+    double *e = (double *)calloc((size_t)m2, sizeof(double));
+    double *se = (double *)calloc((size_t)m2, sizeof(double));
+    double *Gr = (double *)calloc((size_t)m * m, sizeof(double));
+    double *Gi = (double *)calloc((size_t)m * m, sizeof(double));
+    int tlen = 0, status = 0;
+
+    int iter = 1;
+    double relres = tol + 1;
+    double rhs_norm = cnorm2(rhs, n);
+    tol *= rhs_norm;
+    orc_matvec_complex(nx, ny, nz, fdn, coeff, a, c, c0, eta, x, f);
+    for (long i = 0; i < n2; i++) f[i] = rhs[i] - f[i];
+    if (trace && tlen < trace_cap) trace[tlen++] = cnorm2(f, n);
+
+    double t0 = now_seconds();
+    while (relres > tol && iter <= max_iter) {
+        for (long k = 0; k < n; k++) {
+            const double dr = dlap + (c ? c[k] : c0), di = eta;
+            const double den = dr * dr + di * di;
+            const double rr = f[2 * k], ri = f[2 * k + 1];
+            f[2 * k] = (rr * dr + ri * di) / den;
+            f[2 * k + 1] = (ri * dr - rr * di) / den;
+        }
+        if (iter > 1) {
+            const int col = (iter - 2) % m;
+            double *dx = DX + (long)col * n2, *df = DF + (long)col * n2;
+            for (long i = 0; i < n2; i++) {
+                dx[i] = x[i] - x_old[i];
+                df[i] = f[i] - f_old[i];
+            }
+        }
+        memcpy(x_old, x, sizeof(double) * n2);
+        memcpy(f_old, f, sizeof(double) * n2);
+        double rn = -1.0;
+        if (iter % p == 0 && iter > 1) {
+            /* Hermitian Gram G_ij = <DF_i, DF_j>, g_i = <DF_i, f>
+             * (PETSC_COMPLEX/AAR_Complex.c:213-236), solved through the real
+             * embedding [Gr -Gi; Gi Gr][sr; si] = [gr; gi]. */
+            for (int j = 0; j < m; j++)
+                for (int i = 0; i <= j; i++) {
+                    double re, im;
+                    cdot(DF + (long)i * n2, DF + (long)j * n2, n, &re, &im);
+                    Gr[i * m + j] = re; Gi[i * m + j] = im;
+                    Gr[j * m + i] = re; Gi[j * m + i] = -im;
+                }
+            for (int i = 0; i < m; i++) cdot(DF + (long)i * n2, f, n, &e[i], &e[m + i]);
+            for (int i = 0; i < m; i++)
+                for (int j = 0; j < m; j++) {
+                    E[i * m2 + j] = Gr[i * m + j];
+                    E[i * m2 + m + j] = -Gi[i * m + j];
+                    E[(m + i) * m2 + j] = Gi[i * m + j];
+                    E[(m + i) * m2 + m + j] = Gr[i * m + j];
+                }
+            if (orc_pinv_times_vec(E, e, se, m2, NULL, NULL)) { status = 1; break; }
+            for (long k = 0; k < n; k++) {
+                double tr = 0.0, ti = 0.0;
+                for (int j = 0; j < m; j++) {
+                    const double *dx = DX + (long)j * n2, *df = DF + (long)j * n2;
+                    const double vr = dx[2 * k] + beta * df[2 * k];
+                    const double vi = dx[2 * k + 1] + beta * df[2 * k + 1];
+                    tr = tr + (vr * se[j] - vi * se[m + j]);
+                    ti = ti + (vr * se[m + j] + vi * se[j]);
+                }
+                x[2 * k] = x_old[2 * k] + beta * f[2 * k] - tr;
+                x[2 * k + 1] = x_old[2 * k + 1] + beta * f[2 * k + 1] - ti;
+            }
+            orc_matvec_complex(nx, ny, nz, fdn, coeff, a, c, c0, eta, x, f);
+            for (long i = 0; i < n2; i++) f[i] = rhs[i] - f[i];
+            relres = cnorm2(f, n);
+            rn = relres;
+        } else {
+            for (long i = 0; i < n2; i++) x[i] = x_old[i] + omega * f[i];
+            orc_matvec_complex(nx, ny, nz, fdn, coeff, a, c, c0, eta, x, f);
+            for (long i = 0; i < n2; i++) f[i] = rhs[i] - f[i];
+        }
+        if (trace && tlen < trace_cap) trace[tlen++] = rn >= 0.0 ? rn : cnorm2(f, n);
+        iter = iter + 1;
+    }
+    double t1 = now_seconds();
+    finish_stats(st, iter - 1, relres, rhs_norm, tol, iter, max_iter, tlen, t1 - t0, status);
+    free(f); free(x_old); free(f_old); free(DX); free(DF);
+    free(E); free(e); free(se); free(Gr); free(Gi);
+    return status;
+}

@@ -1,0 +1,103 @@
+/*
+ * aar_oracle.h -- CPU restatement of the reference MPI_REAL solver path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the
+ * HIP product in linear-solver_amd/.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load it.  It is never linked into
+ * libaar_hip and is never a fallback for it.
+ *
+ * Each function restates one reference routine (cited file:line, paths
+ * relative to the reference tree) on one rank, with the reference's
+ * floating-point operation order kept so that results are bit-identical
+ * to the reference built with gcc -O3 -funroll-loops on x86-64.
+ *
+ * Extensions with no reference counterpart (sigma != 0 screened operator,
+ * the residual trace, the complex Helmholtz solver) are marked as such.
+ */
+#ifndef AAR_ORACLE_H
+#define AAR_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int iterations;     /* as printed by the reference: iter-1 (AAR/PGR/PL2R), iter_count (CG) */
+    double relres;      /* last checked ||b-Ax|| / ||b|| */
+    double rhs_norm;    /* ||b|| */
+    int converged;      /* relres <= tol*||b|| at exit */
+    int printed_converged; /* the reference's "converged!" line condition */
+    int printed_warning;   /* the reference's "exceeded maximum iterations" condition */
+    int trace_len;      /* number of entries written to trace */
+    double seconds;     /* wall time of the iteration loop */
+    int status;         /* 0 ok, 1 SVD failed to converge */
+} orc_stats;
+
+/* system.c:480-498 Lap_coefficient */
+void orc_lap_coefficient(double *coeff, int fdn);
+
+/* system.c:189-205 (1-rank branch of Lap_Vec_mult): y = a*Lap(x), periodic.
+ * sigma != 0 adds sigma*x afterwards (screened operator; no reference
+ * counterpart, applied the way a user operator callback would). */
+void orc_matvec(int nx, int ny, int nz, int fdn, const double *coeff,
+                double a, double sigma, const double *x, double *y);
+
+/* tools.c:18-43 Vector2Norm / VectorDotProduct (sequential sums) */
+double orc_norm2(const double *v, long n);
+double orc_dot(const double *u, const double *v, long n);
+
+/* tools.c:52-105 PseudoInverseTimesVec, tools.c:113-337 SingularValueDecomp.
+ * A is m*m row-major (A[i*m+j] is the reference's A[i][j]).  Returns 0, or
+ * 1 when the QR sweep exceeds 250 iterations (the reference exits(1)).
+ * w_out, v_out (optional) receive the truncated singular values and V. */
+int orc_pinv_times_vec(const double *A, const double *b, double *x, int m,
+                       double *w_out, double *v_out);
+double orc_pythag(double a, double b);
+
+/* AAR.c:185-232 AndersonExtrapolation.  DX, DF are m columns of length n
+ * (column j at DX + j*n).  G_out (m*m + m, optional) receives allredvec. */
+int orc_anderson_extrapolation(const double *DX, const double *DF, const double *f,
+                               double beta, int m, long n, double *am,
+                               double *G_out, double *s_out);
+
+/* AAR.c:37-177.  precond: 1 = Jacobi with diag = a*(3*coeff[0]) + sigma,
+ * 0 = none (the reference's Precondition == NULL).  trace (optional,
+ * trace_cap entries) receives ||rhs - A x|| each time the operator is
+ * applied to the iterate (initial residual first). */
+int orc_aar(int nx, int ny, int nz, int fdn, double a, double sigma,
+            double *x, const double *rhs, double omega, double beta,
+            int m, int p, int max_iter, double tol, int precond,
+            double *trace, int trace_cap, orc_stats *st);
+
+/* PGR.c:35-218 (l2 = 0) and PL2R.c:35-218 (l2 = 1). */
+int orc_pgr(int nx, int ny, int nz, int fdn, double a, double sigma,
+            double *x, const double *rhs, double omega, int m, int p,
+            int max_iter, double tol, int precond, int l2,
+            double *trace, int trace_cap, orc_stats *st);
+
+/* CG.c:33-175.  trace receives ||b - A x|| at each explicit residual
+ * evaluation (start, and every 50 iterations). */
+int orc_cg(int nx, int ny, int nz, int fdn, double a, double sigma,
+           double *x, const double *b, int max_iter, double tol, int precond,
+           double *trace, int trace_cap, orc_stats *st);
+
+/* Complex Helmholtz AAR (no MPI_REAL counterpart; semantics of
+ * PETSC_COMPLEX/AAR_Complex.c:103-246 with a Jacobi preconditioner):
+ * A u = a*Lap(u) + (c(x) + i*eta) u, complex128 interleaved (re, im).
+ * c may be NULL (c = c0 everywhere).  The Anderson LS uses the Hermitian
+ * Gram DF^H DF and DF^H f solved through the real 2m x 2m embedding with
+ * the pinv of tools.c.  Parity with PETSc/zgelsd is unpinned. */
+int orc_aar_complex(int nx, int ny, int nz, int fdn, double a,
+                    const double *c, double c0, double eta,
+                    double *u, const double *rhs, double omega, double beta,
+                    int m, int p, int max_iter, double tol,
+                    double *trace, int trace_cap, orc_stats *st);
+
+void orc_matvec_complex(int nx, int ny, int nz, int fdn, const double *coeff,
+                        double a, const double *c, double c0, double eta,
+                        const double *u, double *y);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,169 @@
+"""Write the golden fixtures of tests/golden/ from the REAL reference solvers.
+
+Runs oracle/_ref/ref_harness (the reference MPI_REAL sources compiled by
+oracle/Makefile, driven through their public entry points) on seeded
+inputs and stores inputs' checksums, iteration counts, printed relative
+residuals, full-precision residual traces and solutions (full vectors for
+small grids, norms/sums/strided samples for 48^3).  Also stores
+pseudo-inverse and Anderson-step known-answer vectors from tools.c /
+AAR.c, the reference's own RHS (system.c:425-471), and the 1-rank vs
+8-rank spread (the rounding-noise floor of the reference itself).
+
+    python tests/golden/make_golden.py
+
+Only runs where the reference tree and MPICH exist (this build container).
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+sys.path.insert(0, os.path.join(REPO, "linear-solver_amd"))
+
+from oracle import oracle as O  # noqa: E402
+from aar_hip import problems as P  # noqa: E402
+
+SOLVERS = ("aar", "pgr", "pl2r", "cg")
+
+
+def case_inputs(c):
+    shape = tuple(c["shape"])
+    n = int(np.prod(shape))
+    if c["rhs"] == "uniform":
+        b = P.rhs_uniform(shape, c["seed"])
+    elif c["rhs"] == "normal":
+        b = P.rhs_normal(shape, c["seed"])
+    elif c["rhs"] == "zero":
+        b = np.zeros(n)
+    elif c["rhs"] == "reference":
+        b = np.load(os.path.join(HERE, f"refrhs_{shape[0]}.npy"))
+    else:
+        raise ValueError(c["rhs"])
+    x0 = np.full(n, float(c.get("x0", 0.0)))
+    return shape, b, x0
+
+
+def cases():
+    base = dict(omega=0.6, beta=0.6, m=7, p=6, tol=1e-6, max_iter=1000, sigma=0.0, precond=1,
+                rhs="uniform", x0=0.0)
+    out = {}
+    for s in (16, 24, 48):
+        out[f"c1_{s}"] = dict(base, shape=[s, s, s], a=P.poisson_a(0.5), seed=1001)
+    out["screened_24"] = dict(base, shape=[24, 24, 24], a=P.poisson_a(0.5), sigma=0.5, seed=1003,
+                              rhs="normal")
+    out["aniso_20x12x16"] = dict(base, shape=[20, 12, 16], a=P.poisson_a(0.5), seed=1006)
+    out["maxiter13_16"] = dict(base, shape=[16, 16, 16], a=P.poisson_a(0.5), seed=1001, max_iter=13)
+    out["maxiter12_16"] = dict(base, shape=[16, 16, 16], a=P.poisson_a(0.5), seed=1001, max_iter=12)
+    out["p1_16"] = dict(base, shape=[16, 16, 16], a=P.poisson_a(0.5), seed=1001, p=1, max_iter=200)
+    out["m1_16"] = dict(base, shape=[16, 16, 16], a=P.poisson_a(0.5), seed=1001, m=1)
+    out["m3p5_16"] = dict(base, shape=[16, 16, 16], a=P.poisson_a(0.5), seed=1001, m=3, p=5)
+    out["noprec_16"] = dict(base, shape=[16, 16, 16], a=P.poisson_a(0.5), seed=1001, precond=0,
+                            omega=0.015, beta=0.015, max_iter=600)
+    out["zero_rhs_16"] = dict(base, shape=[16, 16, 16], a=P.poisson_a(0.5), seed=0, rhs="zero")
+    # the reference driver's own setting: its RHS, x0 = 1, a = -1/(4 pi)
+    # (MPI_REAL/main.c:63-68), with the C1 solver parameters
+    out["refdrv_48"] = dict(base, shape=[48, 48, 48], a=-1.0 / (4 * math.pi), seed=0,
+                            rhs="reference", x0=1.0)
+    out["refdrv_16"] = dict(base, shape=[16, 16, 16], a=-1.0 / (4 * math.pi), seed=0,
+                            rhs="reference", x0=1.0)
+    return out
+
+
+def sample_idx(n):
+    return np.arange(0, n, 97, dtype=np.int64)
+
+
+def main():
+    if not O.ref_available():
+        raise SystemExit("oracle/_ref/ref_harness is not built (make -C oracle ref)")
+    meta = {"cases": {}, "kats": {}}
+    # the reference's own RHS (glibc srand/rand), 1 rank
+    for s in (16, 48):
+        rr = O.ref_init_rhs((s, s, s))
+        np.save(os.path.join(HERE, f"refrhs_{s}.npy"), rr)
+    rr8 = O.ref_init_rhs((48, 48, 48), ranks=8)
+    meta["refrhs_48_rank8_equal"] = bool(np.array_equal(rr8, np.load(os.path.join(HERE, "refrhs_48.npy"))))
+
+    arrays = {}
+    for name, c in cases().items():
+        shape, b, x0 = case_inputs(c)
+        n = b.size
+        entry = {"params": c, "rhs_sum": float(np.sum(b)), "rhs_norm": float(np.linalg.norm(b)),
+                 "solvers": {}}
+        for sv in SOLVERS:
+            x, d = O.ref_solve(sv, shape, b, x0, c["a"], c["sigma"], c["omega"], c["beta"], c["m"],
+                               c["p"], c["max_iter"], c["tol"], bool(c["precond"]))
+            r = {"iterations": d["iterations"], "relres_printed": d["relres_printed"],
+                 "converged_line": d["converged_line"], "warning_line": d["warning_line"],
+                 "x_norm": float(np.linalg.norm(x)), "x_sum": float(np.sum(x)),
+                 "trace_len": int(d["trace"].size)}
+            arrays[f"{name}__{sv}__trace"] = d["trace"]
+            if n <= 24 ** 3:
+                arrays[f"{name}__{sv}__x"] = x
+            else:
+                arrays[f"{name}__{sv}__xs"] = x[sample_idx(n)]
+            if name == "c1_48":
+                x8, d8 = O.ref_solve(sv, shape, b, x0, c["a"], c["sigma"], c["omega"], c["beta"],
+                                     c["m"], c["p"], c["max_iter"], c["tol"], bool(c["precond"]),
+                                     ranks=8)
+                r["rank8_iterations"] = d8["iterations"]
+                r["rank8_rel_diff"] = float(np.linalg.norm(x8 - x) / np.linalg.norm(x))
+            entry["solvers"][sv] = r
+            print(name, sv, r["iterations"], r.get("rank8_rel_diff", ""), flush=True)
+        meta["cases"][name] = entry
+
+    # pinv known answers (tools.c:52-352): SPD, rank-deficient, non-symmetric
+    rng = np.random.default_rng(2024)
+    kats = []
+    for m in (1, 2, 3, 5, 7, 8, 13, 16):
+        F = rng.standard_normal((50, m))
+        kats.append(("spd", F.T @ F))
+        if m >= 3:
+            G = F.T @ F
+            G[m - 1, :] = 0.0
+            G[:, m - 1] = 0.0
+            kats.append(("zero_col", G))
+            F2 = F.copy()
+            F2[:, 1] = F2[:, 0] * (1 + 1e-9)
+            kats.append(("near_singular", F2.T @ F2))
+        kats.append(("nonsym", rng.standard_normal((m, m)) + m * np.eye(m)))
+    kat_A, kat_b, kat_x, kat_m = [], [], [], []
+    for kind, A in kats:
+        m = A.shape[0]
+        b = rng.standard_normal(m)
+        x = O.ref_pinv(A, b)
+        kat_A.append(A.reshape(-1))
+        kat_b.append(b)
+        kat_x.append(x)
+        kat_m.append(m)
+        meta["kats"].setdefault("pinv_kinds", []).append(kind)
+    arrays["pinv_m"] = np.array(kat_m, dtype=np.int64)
+    arrays["pinv_A"] = np.concatenate(kat_A)
+    arrays["pinv_b"] = np.concatenate(kat_b)
+    arrays["pinv_x"] = np.concatenate(kat_x)
+
+    # Anderson step known answer (AAR.c:185-232), m = 7, N = 4096
+    m, n = 7, 4096
+    DX = rng.standard_normal((m, n))
+    DF = rng.standard_normal((m, n))
+    DF[5:] = 0.0                           # two never-written columns
+    f = rng.standard_normal(n)
+    am, red = O.ref_anderson(DX, DF, f, 0.6)
+    arrays["and_DX"], arrays["and_DF"], arrays["and_f"] = DX, DF, f
+    arrays["and_am"], arrays["and_red"] = am, red
+
+    np.savez_compressed(os.path.join(HERE, "golden.npz"), **arrays)
+    with open(os.path.join(HERE, "golden.json"), "w") as fp:
+        json.dump(meta, fp, indent=1, sort_keys=True)
+    print("wrote", os.path.join(HERE, "golden.npz"))
+
+
+if __name__ == "__main__":
+    main()

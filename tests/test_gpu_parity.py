@@ -1,0 +1,217 @@
+"""GPU parity of libaar_hip against the oracle and the reference's fixtures.
+
+Bar (north star): iteration counts equal to the reference's; solution
+relative difference <= 1e-8 (AAR, PGR, PL2R) at the reference tolerance;
+CG within 1e-7 (the reference itself moves its CG solution by 6e-9 between
+1 and 8 ranks).  The stencil and the pseudo-inverse are bit-exact.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from aar_hip import problems as P
+from test_oracle import SMALL_CASES, case_inputs, fft_solve, run_oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"aar": 1e-8, "pgr": 1e-8, "pl2r": 1e-8, "cg": 1e-7}
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def dev(v, dtype=None):
+    import torch
+    t = torch.from_numpy(np.ascontiguousarray(v)).to("cuda")
+    return t if dtype is None else t.to(dtype)
+
+
+@pytest.mark.parametrize("shape", [(48, 48, 48), (20, 12, 16), (8, 6, 7), (64, 64, 64), (130, 70, 33),
+                                   (24, 100, 13)])
+@pytest.mark.parametrize("sigma", [0.0, 0.5])
+def test_stencil_bitwise(gpu, oracle, shape, sigma):
+    say("stencil", shape, sigma)
+    x = np.random.default_rng(sum(shape)).standard_normal(int(np.prod(shape)))
+    y = gpu.Lap_Vec_mult(dev(x), shape, -4.0, sigma).cpu().numpy()
+    yo = oracle.matvec(x, shape, -4.0, sigma)
+    assert np.array_equal(y, yo)
+
+
+@pytest.mark.parametrize("lz,py", [(0, 4), (0, 2), (7, 4), (1, 2), (512, 4)])
+def test_stencil_tuning_bitwise(gpu, oracle, lz, py):
+    say("stencil tuning", lz, py)
+    shape = (72, 40, 36)
+    x = np.random.default_rng(7).standard_normal(int(np.prod(shape)))
+    gpu.set_tuning(lz, py)
+    try:
+        y = gpu.Lap_Vec_mult(dev(x), shape, -16.0).cpu().numpy()
+    finally:
+        gpu.set_tuning(0, 4)
+    assert np.array_equal(y, oracle.matvec(x, shape, -16.0))
+
+
+def test_stencil_f32(gpu, oracle):
+    import torch
+    shape = (40, 40, 40)
+    x = np.random.default_rng(8).standard_normal(40 ** 3)
+    y32 = gpu.Lap_Vec_mult(dev(x, torch.float32), shape, -4.0).cpu().numpy().astype(np.float64)
+    y = oracle.matvec(x.astype(np.float32).astype(np.float64), shape, -4.0)
+    assert np.linalg.norm(y32 - y) / np.linalg.norm(y) < 1e-6
+
+
+def test_pinv_kat_bitwise(gpu, golden):
+    _, g = golden
+    ms, A, b, xg = g["pinv_m"], g["pinv_A"], g["pinv_b"], g["pinv_x"]
+    ia = ib = 0
+    for m in ms:
+        say("pinv m", m)
+        x, _, _ = gpu.pinv(A[ia:ia + m * m].reshape(m, m), b[ib:ib + m])
+        assert np.array_equal(x, xg[ib:ib + m]), f"m={m}"
+        ia += m * m
+        ib += m
+
+
+def _gpu_solve(gpu, solver, c, shape, b, x0):
+    xd = dev(x0)
+    bd = dev(b)
+    kw = dict(sigma=c["sigma"], precond=bool(c["precond"]))
+    if solver == "aar":
+        st = gpu.AAR(xd, bd, shape, c["a"], c["omega"], c["beta"], c["m"], c["p"], c["max_iter"],
+                     c["tol"], **kw)
+    elif solver == "pgr":
+        st = gpu.PGR(xd, bd, shape, c["a"], c["omega"], c["m"], c["p"], c["max_iter"], c["tol"], **kw)
+    elif solver == "pl2r":
+        st = gpu.PL2R(xd, bd, shape, c["a"], c["omega"], c["m"], c["p"], c["max_iter"], c["tol"], **kw)
+    else:
+        st = gpu.CG(xd, bd, shape, c["a"], c["max_iter"], c["tol"], **kw)
+    return xd.cpu().numpy(), st
+
+
+@pytest.mark.parametrize("name", SMALL_CASES + ["c1_48", "refdrv_48"])
+@pytest.mark.parametrize("solver", ["aar", "pgr", "pl2r", "cg"])
+def test_solver_parity(gpu, oracle, golden, name, solver):
+    meta, g = golden
+    c, shape, b, x0 = case_inputs(meta, name)
+    say("parity", name, solver)
+    x, st = _gpu_solve(gpu, solver, c, shape, b, x0)
+    ref = meta["cases"][name]["solvers"][solver]
+    assert st["iterations"] == ref["iterations"]
+    assert bool(st["printed_converged"]) == ref["converged_line"]
+    assert bool(st["printed_warning"]) == ref["warning_line"]
+    if f"{name}__{solver}__x" in g:
+        xr = g[f"{name}__{solver}__x"]
+    else:
+        xr, _ = run_oracle(oracle, solver, c, shape, b, x0)
+        assert np.array_equal(xr[::97], g[f"{name}__{solver}__xs"])
+    nr = np.linalg.norm(xr)
+    if nr == 0.0:
+        assert np.linalg.norm(x) == 0.0
+    else:
+        assert np.linalg.norm(x - xr) / nr <= TOL[solver]
+    # residual histories agree with the reference's (rounding-level drift only)
+    h = gpu.history()
+    tr = g[f"{name}__{solver}__trace"]
+    if solver == "cg":
+        h = np.concatenate([h[:1], h[1::50]])
+    k = min(len(h), len(tr))
+    assert k == len(tr)
+    if k and tr[0] > 0:
+        # rounding-level drift: relative to each entry, or to ||r_0|| once the
+        # residual itself is tiny
+        assert np.all(np.abs(h[:k] - tr[:k]) <= 1e-6 * tr[:k] + 1e-10 * tr[0])
+
+
+def test_aar_fp32(gpu):
+    import torch
+    shape = (32, 32, 32)
+    b = P.rhs_uniform(shape, 5)
+    x = torch.zeros(b.size, dtype=torch.float32, device="cuda")
+    st = gpu.AAR(x, dev(b, torch.float32), shape, P.poisson_a(0.5), 0.6, 0.6, 7, 6, 1000, 1e-5)
+    assert st["converged"] and st["iterations"] % 6 == 0
+    xf = fft_solve(b, shape, P.poisson_a(0.5), __import__("oracle").oracle.lap_coefficient(6))
+    xx = x.cpu().numpy().astype(np.float64)
+    assert np.linalg.norm(xx - xf) / np.linalg.norm(xf) < 1e-3
+
+
+@pytest.mark.parametrize("sigma", [0.0, 0.5])
+def test_refined_vs_fp64(gpu, oracle, sigma):
+    shape = (48, 48, 48)
+    a = P.poisson_a(0.5)
+    b = P.rhs_uniform(shape, 1003) if sigma == 0.0 else P.rhs_normal(shape, 1003)
+    say("refined", sigma)
+    xr = dev(np.zeros(b.size))
+    st = gpu.AAR_refined(xr, dev(b), shape, a, 0.6, 0.6, 7, 6, 1e-10, inner_tol=1e-5, sigma=sigma)
+    assert st["converged"] and st["relres"] <= 1e-10
+    x64 = dev(np.zeros(b.size))
+    st64 = gpu.AAR(x64, dev(b), shape, a, 0.6, 0.6, 7, 6, 5000, 1e-10, sigma=sigma)
+    assert st64["converged"]
+    u, v = xr.cpu().numpy(), x64.cpu().numpy()
+    if sigma == 0.0:                  # singular periodic problem: compare modulo the mean
+        u = u - u.mean()
+        v = v - v.mean()
+    assert np.linalg.norm(u - v) / np.linalg.norm(v) <= 1e-9
+
+
+def test_large_residual_properties(gpu):
+    """256^3 FP64 AAR: the true residual of the returned x (by the bit-exact
+    GPU stencil) meets the tolerance; mean(x) stays at mean(x0) = 0."""
+    import torch
+    shape = (256, 256, 256)
+    a = P.poisson_a(0.25)
+    say("large 256^3")
+    b = P.rhs_uniform_torch(shape, 1002)
+    x = torch.zeros_like(b)
+    st = gpu.AAR(x, b, shape, a, 0.6, 0.6, 7, 6, 20000, 1e-6)
+    assert st["converged"] and st["iterations"] % 6 == 0
+    r = b - gpu.Lap_Vec_mult(x, shape, a)
+    rel = float(torch.linalg.norm(r) / torch.linalg.norm(b))
+    assert rel <= 1e-6 and abs(rel - st["relres"]) < 1e-9
+    assert abs(float(x.mean())) < 1e-10 * float(x.abs().max())
+
+
+def test_dropin_reference_driver(golden):
+    """The reference's main.c, compiled unchanged against include/ and linked
+    with libaar_hip (oracle/_ref/main_on_libaar), reports the reference's
+    iteration counts for its own configuration (48^3, glibc RHS, x0 = 1,
+    a = -1/(4 pi))."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    exe = os.path.join(os.path.dirname(__file__), "..", "oracle", "_ref", "main_on_libaar")
+    if not os.path.exists(exe):
+        pytest.skip("drop-in demo not built")
+    meta, _ = golden
+    tmp = os.path.join(os.path.dirname(__file__), "..", "gpurun_out")
+    os.makedirs(tmp, exist_ok=True)
+    out = subprocess.run([os.path.abspath(exe), "1e-6", "7", "6", "0.6", "0.6"], capture_output=True,
+                         text=True, timeout=300, cwd=tmp).stdout
+    say(out[-1200:])
+    ref = meta["cases"]["refdrv_48"]["solvers"]
+    import re
+    for sv in ("AAR", "PGR", "PL2R", "CG"):
+        m = re.search(sv + r" converged!:\s+Iterations = (\d+)", out)
+        assert m, f"{sv} line missing"
+        assert int(m.group(1)) == ref[sv.lower()]["iterations"]
+
+
+def test_foreign_callback_rejected(gpu):
+    """A user operator callback is not executed on the host: AAR() refuses
+    it and leaves x unchanged (the reference would run it on the CPU)."""
+    import ctypes
+    from aar_hip import _lib
+    L = _lib.load()
+    CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
+                          ctypes.c_int)
+    called = []
+    cb = CB(lambda *a: called.append(1))
+    x = (ctypes.c_double * 8)(*([1.0] * 8))
+    b = (ctypes.c_double * 8)(*([0.5] * 8))
+    L.AAR.argtypes = [ctypes.c_void_p, CB, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p,
+                      ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                      ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+    L.AAR(None, cb, None, -4.0, x, b, 0.6, 0.6, 7, 6, 10, 1e-6, 8, 0)
+    assert not called and list(x) == [1.0] * 8
+    assert b"foreign" in L.aar_hip_last_error()

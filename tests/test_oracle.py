@@ -1,0 +1,329 @@
+"""Pins of the CPU oracle (oracle/aar_oracle.c) -- CPU only.
+
+The oracle is the parity checker for the GPU path, so it is itself pinned:
+* bit-for-bit against fixtures written by the real reference solvers
+  (tests/golden, made by tests/golden/make_golden.py from oracle/_ref);
+* against what the mathematics fixes: the closed-form FD12 weights, the
+  Fourier symbol of the periodic stencil, a dense brute-force operator,
+  numpy's SVD for the pseudo-inverse, and the FFT spectral solution of the
+  periodic system;
+* and each check is shown to catch a planted mistake (tests at the end).
+"""
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from aar_hip import problems as P
+
+C12 = [Fraction(-5369, 1800), Fraction(12, 7), Fraction(-15, 56), Fraction(10, 189),
+       Fraction(-1, 112), Fraction(2, 1925), Fraction(-1, 16632)]
+
+
+def fd_weights_exact(fdn):
+    """Central 2nd-derivative weights of order 2*fdn from the closed form
+    c_p = 2 (-1)^(p+1) (fdn!)^2 / ((fdn-p)! (fdn+p)!) / p^2, c0 = -2 sum 1/p^2."""
+    c = [Fraction(0)] * (fdn + 1)
+    for p in range(1, fdn + 1):
+        c[p] = Fraction(2 * (-1) ** (p + 1) * math.factorial(fdn) ** 2,
+                        math.factorial(fdn - p) * math.factorial(fdn + p) * p * p)
+        c[0] -= Fraction(2, p * p)
+    return c
+
+
+def symbol(shape, a, coeff, sigma=0.0):
+    """Eigenvalues of y = a*Lap(x) + sigma x on the periodic grid, shaped (nz, ny, nx)."""
+    nx, ny, nz = shape
+    def s1(n):
+        th = 2 * np.pi * np.arange(n) / n
+        return coeff[0] + 2 * sum(coeff[p] * np.cos(p * th) for p in range(1, len(coeff)))
+    lam = s1(nz)[:, None, None] + s1(ny)[None, :, None] + s1(nx)[None, None, :]
+    return a * lam + sigma
+
+
+def fft_solve(b, shape, a, coeff, sigma=0.0, mean=0.0):
+    nx, ny, nz = shape
+    lam = symbol(shape, a, coeff, sigma)
+    bh = np.fft.fftn(b.reshape(nz, ny, nx))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        xh = np.where(np.abs(lam) > 1e-300, bh / lam, 0.0)
+    if sigma == 0.0:
+        xh[0, 0, 0] = mean * b.size
+    return np.real(np.fft.ifftn(xh)).reshape(-1)
+
+
+def dense_operator(shape, a, coeff):
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    M = np.zeros((n, n))
+    w = [a * c for c in coeff]
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                r = i + j * nx + k * nx * ny
+                M[r, r] += 3 * w[0]
+                for o in range(1, len(coeff)):
+                    for (ii, jj, kk) in ((i - o, j, k), (i + o, j, k), (i, j - o, k), (i, j + o, k),
+                                         (i, j, k - o), (i, j, k + o)):
+                        M[r, (ii % nx) + (jj % ny) * nx + (kk % nz) * nx * ny] += w[o]
+    return M
+
+
+# ---------------------------------------------------------------------------
+# operator
+# ---------------------------------------------------------------------------
+
+def test_fd12_weights_closed_form(oracle):
+    c = oracle.lap_coefficient(6)
+    for p in range(7):
+        assert c[p] == pytest.approx(float(C12[p]), rel=2e-16, abs=0)
+        assert C12[p] == fd_weights_exact(6)[p]
+    assert abs(c[0] + 2 * c[1:].sum()) < 1e-14          # zero row sum
+
+
+@pytest.mark.parametrize("fdn,exact", [(1, [-2, 1]), (2, [Fraction(-5, 2), Fraction(4, 3), Fraction(-1, 12)])])
+def test_low_order_weights(oracle, fdn, exact):
+    c = oracle.lap_coefficient(fdn)
+    assert np.allclose(c, [float(e) for e in exact], rtol=1e-15, atol=0)
+
+
+def test_matvec_fourier_symbol(oracle):
+    shape = (16, 12, 10)
+    nx, ny, nz = shape
+    coeff = oracle.lap_coefficient(6)
+    a = -3.7
+    k = np.arange(nz)[:, None, None]
+    j = np.arange(ny)[None, :, None]
+    i = np.arange(nx)[None, None, :]
+    lam = symbol(shape, a, coeff)
+    for (mx, my, mz) in [(1, 0, 0), (0, 2, 0), (0, 0, 3), (3, 5, 4), (8, 6, 5)]:
+        x = np.cos(2 * np.pi * (mx * i / nx + my * j / ny + mz * k / nz)).reshape(-1)
+        y = oracle.matvec(x, shape, a)
+        assert np.allclose(y, lam[mz, my, mx] * x, rtol=0, atol=1e-12 * np.abs(lam).max())
+
+
+def test_matvec_dense_bruteforce(oracle):
+    shape = (8, 6, 7)                     # sides below 2*FDn+1: wraps overlap, as in the reference
+    coeff = oracle.lap_coefficient(6)
+    M = dense_operator(shape, -4.0, coeff)
+    x = np.random.default_rng(3).standard_normal(M.shape[0])
+    assert np.allclose(oracle.matvec(x, shape, -4.0), M @ x, rtol=0, atol=1e-11)
+    assert np.allclose(M, M.T)                             # symmetric
+    assert np.allclose(M.sum(axis=1), 0.0, atol=1e-12)     # zero row sums
+
+
+def test_matvec_screened(oracle):
+    shape = (12, 12, 12)
+    x = np.random.default_rng(4).standard_normal(12 ** 3)
+    y0 = oracle.matvec(x, shape, -4.0)
+    y1 = oracle.matvec(x, shape, -4.0, sigma=0.5)
+    assert np.allclose(y1, y0 + 0.5 * x, rtol=0, atol=1e-13)
+
+
+# ---------------------------------------------------------------------------
+# pseudo-inverse
+# ---------------------------------------------------------------------------
+
+def numpy_pinv_ref(A, b):
+    """tools.c semantics from numpy: V diag(1/w) V^T b with w < n w_max eps zeroed."""
+    U, w, Vt = np.linalg.svd(A)
+    n = A.shape[0]
+    w = w.copy()
+    w[w < n * w.max() * 2.22044605e-16] = 0.0
+    inv = np.where(w > 0, 1.0 / np.where(w > 0, w, 1.0), 0.0)
+    V = Vt.T
+    return V @ (inv * (V.T @ b)), w
+
+
+@pytest.mark.parametrize("m", [1, 2, 4, 7, 8, 13])
+def test_pinv_matches_numpy_svd(oracle, m):
+    rng = np.random.default_rng(10 + m)
+    F = rng.standard_normal((40, m))
+    A = F.T @ F
+    b = rng.standard_normal(m)
+    x, w, V = oracle.pinv_times_vec(A, b)
+    xr, wr = numpy_pinv_ref(A, b)
+    assert np.allclose(np.sort(w), np.sort(wr), rtol=1e-12, atol=0)
+    assert np.allclose(x, xr, rtol=1e-9, atol=1e-12)
+    assert np.allclose(x, np.linalg.solve(A, b), rtol=1e-8)
+    assert np.allclose(V.T @ V, np.eye(m), atol=1e-12)
+
+
+def test_pinv_rank_deficient(oracle):
+    rng = np.random.default_rng(11)
+    F = rng.standard_normal((40, 7))
+    F[:, 5:] = 0.0                                   # never-written history columns
+    A = F.T @ F
+    b = F.T @ rng.standard_normal(40)
+    x, w, _ = oracle.pinv_times_vec(A, b)
+    assert (w == 0).sum() == 2
+    assert np.allclose(x, np.linalg.pinv(A) @ b, rtol=1e-9, atol=1e-12)
+    assert np.allclose(x[5:], 0.0, atol=1e-14)
+
+
+def test_pinv_nonsymmetric_uses_v_only(oracle):
+    """The reference replaces U by V (tools.c:324-329): for a non-symmetric
+    matrix the result is (A^T A)^(-1/2) b, not pinv(A) b."""
+    rng = np.random.default_rng(12)
+    A = rng.standard_normal((5, 5)) + 5 * np.eye(5)
+    b = rng.standard_normal(5)
+    x, _, _ = oracle.pinv_times_vec(A, b)
+    ev, Q = np.linalg.eigh(A.T @ A)
+    assert np.allclose(x, Q @ ((Q.T @ b) / np.sqrt(ev)), rtol=1e-10)
+    assert not np.allclose(x, np.linalg.solve(A, b), rtol=1e-3)
+
+
+def test_pinv_golden_bitwise(oracle, golden):
+    meta, g = golden
+    ms, A, b, xg = g["pinv_m"], g["pinv_A"], g["pinv_b"], g["pinv_x"]
+    ia = ib = 0
+    for m in ms:
+        x, _, _ = oracle.pinv_times_vec(A[ia:ia + m * m].reshape(m, m), b[ib:ib + m])
+        assert np.array_equal(x, xg[ib:ib + m]), f"m={m}"
+        ia += m * m
+        ib += m
+
+
+def test_anderson_golden_bitwise(oracle, golden):
+    _, g = golden
+    am, red, _ = oracle.anderson_extrapolation(g["and_DX"], g["and_DF"], g["and_f"], 0.6)
+    assert np.array_equal(am, g["and_am"])
+    assert np.array_equal(red, g["and_red"])
+
+
+# ---------------------------------------------------------------------------
+# solvers vs the reference's own outputs
+# ---------------------------------------------------------------------------
+
+def case_inputs(meta, name):
+    c = meta["cases"][name]["params"]
+    shape = tuple(c["shape"])
+    n = int(np.prod(shape))
+    if c["rhs"] == "uniform":
+        b = P.rhs_uniform(shape, c["seed"])
+    elif c["rhs"] == "normal":
+        b = P.rhs_normal(shape, c["seed"])
+    elif c["rhs"] == "zero":
+        b = np.zeros(n)
+    else:
+        import os
+        from conftest import GOLDEN_DIR
+        b = np.load(os.path.join(GOLDEN_DIR, f"refrhs_{shape[0]}.npy"))
+    x0 = np.full(n, float(c["x0"]))
+    return c, shape, b, x0
+
+
+def run_oracle(O, solver, c, shape, b, x0):
+    if solver == "aar":
+        return O.aar(x0, b, shape, c["a"], c["omega"], c["beta"], c["m"], c["p"], c["max_iter"],
+                     c["tol"], sigma=c["sigma"], precond=bool(c["precond"]))
+    if solver == "cg":
+        return O.cg(x0, b, shape, c["a"], c["max_iter"], c["tol"], sigma=c["sigma"],
+                    precond=bool(c["precond"]))
+    return O.pgr(x0, b, shape, c["a"], c["omega"], c["m"], c["p"], c["max_iter"], c["tol"],
+                 sigma=c["sigma"], precond=bool(c["precond"]), l2=(solver == "pl2r"))
+
+
+SMALL_CASES = ["c1_16", "c1_24", "screened_24", "aniso_20x12x16", "maxiter13_16", "maxiter12_16",
+               "p1_16", "m1_16", "m3p5_16", "noprec_16", "zero_rhs_16", "refdrv_16"]
+
+
+@pytest.mark.parametrize("name", SMALL_CASES + ["c1_48", "refdrv_48"])
+@pytest.mark.parametrize("solver", ["aar", "pgr", "pl2r", "cg"])
+def test_oracle_bitwise_vs_reference(oracle, golden, name, solver):
+    meta, g = golden
+    c, shape, b, x0 = case_inputs(meta, name)
+    assert float(np.sum(b)) == meta["cases"][name]["rhs_sum"]
+    x, d = run_oracle(oracle, solver, c, shape, b, x0)
+    ref = meta["cases"][name]["solvers"][solver]
+    assert d["iterations"] == ref["iterations"]
+    assert bool(d["printed_converged"]) == ref["converged_line"]
+    assert bool(d["printed_warning"]) == ref["warning_line"]
+    assert np.array_equal(d["trace"], g[f"{name}__{solver}__trace"])
+    if f"{name}__{solver}__x" in g:
+        assert np.array_equal(x, g[f"{name}__{solver}__x"])
+    else:
+        assert np.array_equal(x[::97], g[f"{name}__{solver}__xs"])
+        assert float(np.linalg.norm(x)) == ref["x_norm"]
+    if ref["relres_printed"] is not None and np.isfinite(ref["relres_printed"]):
+        assert float(f"{d['relres']:g}") == ref["relres_printed"]
+
+
+# ---------------------------------------------------------------------------
+# closed forms and invariants of the converged solutions
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("solver", ["aar", "pgr", "pl2r", "cg"])
+@pytest.mark.parametrize("sigma", [0.0, 0.5])
+def test_oracle_solution_vs_fft(oracle, solver, sigma):
+    shape = (16, 16, 16)
+    a = P.poisson_a(0.5)
+    b = P.rhs_uniform(shape, 77) if sigma == 0.0 else P.rhs_normal(shape, 78)
+    x0 = np.zeros(b.size)
+    c = dict(a=a, omega=0.6, beta=0.6, m=7, p=6, max_iter=5000, tol=1e-11, sigma=sigma, precond=1)
+    x, d = run_oracle(oracle, solver, c, shape, b, x0)
+    xf = fft_solve(b, shape, a, oracle.lap_coefficient(6), sigma, mean=0.0)
+    assert d["converged"]
+    assert np.linalg.norm(x - xf) / np.linalg.norm(xf) < 1e-8
+    if solver in ("aar", "pgr", "pl2r"):
+        assert d["iterations"] % 6 == 0          # residual checked on projection steps only
+    if sigma == 0.0:
+        assert abs(x.mean()) < 1e-12 * np.abs(x).max()   # mean(x) = mean(x0) for periodic Poisson
+
+
+# ---------------------------------------------------------------------------
+# each check catches a planted mistake (mutations of the oracle's arithmetic,
+# emulated in numpy on the CPU)
+# ---------------------------------------------------------------------------
+
+def _numpy_matvec(x, shape, a, coeff, drop=None, sign_flip=None):
+    nx, ny, nz = shape
+    X = x.reshape(nz, ny, nx)
+    w = [a * c for c in coeff]
+    y = X * 3 * w[0]
+    for o in range(1, len(coeff)):
+        terms = [np.roll(X, o, 2), np.roll(X, -o, 2), np.roll(X, o, 1), np.roll(X, -o, 1),
+                 np.roll(X, o, 0), np.roll(X, -o, 0)]
+        if drop is not None and o == drop[0]:
+            terms[drop[1]] = 0 * terms[drop[1]]
+        s = sum(terms)
+        y = y + s * (-w[o] if sign_flip == o else w[o])
+    return y.reshape(-1)
+
+
+def test_symbol_check_catches_dropped_or_flipped_term(oracle):
+    shape = (16, 12, 10)
+    coeff = oracle.lap_coefficient(6)
+    x = np.cos(2 * np.pi * 3 * np.arange(16) / 16)[None, None, :] * np.ones((10, 12, 1))
+    x = x.reshape(-1)
+    good = _numpy_matvec(x, shape, -3.7, coeff)
+    assert np.allclose(good, oracle.matvec(x, shape, -3.7), atol=1e-12)
+    lam = symbol(shape, -3.7, coeff)[0, 0, 3]
+    for bad in (_numpy_matvec(x, shape, -3.7, coeff, drop=(2, 1)),
+                _numpy_matvec(x, shape, -3.7, coeff, sign_flip=4)):
+        assert not np.allclose(bad, lam * x, atol=1e-9)
+
+
+def test_pinv_check_catches_transpose(oracle):
+    rng = np.random.default_rng(5)
+    F = rng.standard_normal((30, 6))
+    A = F.T @ F
+    b = rng.standard_normal(6)
+    xr, _ = numpy_pinv_ref(A, b)
+    U, w, Vt = np.linalg.svd(A)
+    wrong = Vt @ ((U.T @ b) / w)                 # V^T in place of V
+    assert not np.allclose(wrong, xr, rtol=1e-6)
+
+
+def test_generator_numpy_vs_torch_bits():
+    import torch
+    u_np = P.uniform01(5000, 1234, 3)
+    u_t = P.uniform01_torch(5000, 1234, 3, device="cpu").numpy()
+    assert np.array_equal(u_np, u_t)
+    assert 0.0 <= u_np.min() and u_np.max() < 1.0
+    assert abs(u_np.mean() - 0.5) < 0.02
+    assert not np.array_equal(u_np, P.uniform01(5000, 1234, 4))
+    b = P.rhs_uniform((8, 8, 8), 1)
+    assert abs(b.mean()) < 1e-16
+    del torch
