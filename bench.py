@@ -1,0 +1,258 @@
+"""Benchmark: AAR solves/s on the 512^3 FD12 periodic Poisson system.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512]
+
+A step is one AAR solve (MPI_REAL/AAR.c semantics) from x0 = 0 to
+||b - A x|| / ||b|| <= 1e-6 on the 512^3 grid, h = 0.25 (a = -16),
+b ~ U[-0.5, 0.5] (seed 1002, mean removed), omega = beta = 0.6, m = 7,
+p = 6, with b and x already resident in HBM.  `value` is FP64 AAR solves/s;
+the FP64-residual-refined FP32 solve of the same system to the same
+tolerance, the stencil matvec bandwidth, the roofline of the dominant
+kernel (the fused stencil + Richardson sweep, timed with HIP events on the
+library's stream) and a CPU baseline (the reference solvers on the host's
+cores, bounded sample) are reported on the same JSON line.
+
+--mode kernels runs a short profiling workload (a few matvecs, one plain
+copy of known size, 24 AAR iterations) for rocprofv3.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "linear-solver_amd"))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--mode", default="bench", choices=["bench", "kernels"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=256)
+    ap.add_argument("--cpu-iters", type=int, default=12)
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n_cpu, iters, gpu_iterations, n_gpu):
+    """Time the reference AAR (oracle/_ref, MPI ranks on the host cores) on
+    a bounded sample: n_cpu^3 for `iters` iterations; scale the per-iteration
+    time by the point ratio and the GPU solve's iteration count."""
+    import numpy as np
+    from aar_hip import problems as P
+    from oracle import oracle as O
+    shape = (n_cpu,) * 3
+    b = P.rhs_uniform(shape, 1002)
+    x0 = np.zeros(b.size)
+    a = P.poisson_a(0.25)
+    cores = min(16, os.cpu_count() or 1)
+    secs, kind, used = None, None, 1
+    if O.ref_available():
+        for ranks in ([cores, 1] if cores > 1 else [1]):
+            try:
+                _, d = O.ref_solve("aar", shape, b, x0, a, 0.0, 0.6, 0.6, 7, 6, iters, 1e-30, True,
+                                   ranks=ranks, timeout=600)
+                secs, kind, used = d["seconds"], "reference", ranks
+                break
+            except Exception as e:  # noqa: BLE001
+                log(f"cpu baseline: reference with {ranks} ranks failed: {e}")
+    if secs is None:
+        t0 = time.time()
+        O.aar(x0, b, shape, a, 0.6, 0.6, 7, 6, iters, 1e-30, trace=False)
+        secs, kind, used = time.time() - t0, "port", 1
+    per_iter = secs / iters * (n_gpu / n_cpu) ** 3
+    solve_s = per_iter * gpu_iterations
+    return {
+        "value": 1.0 / solve_s,
+        "unit": "solves/s",
+        "cores": used,
+        "kind": kind,
+        "sample": (f"AAR on {n_cpu}^3 for {iters} iterations ({iters // 6} Anderson cycles) in "
+                   f"{secs:.2f} s; per-iteration time x {(n_gpu / n_cpu) ** 3:.0f} (points of "
+                   f"{n_gpu}^3) x {gpu_iterations} iterations (the GPU solve's count)"),
+        "cpu": platform.processor() or platform.machine(),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "libaar_hip runs on one GPU: the whole grid is resident in "
+                                   "one GPU's HBM; multi-GPU runs are not part of this build",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import aar_hip
+    from aar_hip import problems as P
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(0)
+    n = args.n
+    shape = (n, n, n)
+    N = n ** 3
+    a = P.poisson_a(0.25)
+    omega = beta = 0.6
+    m, p, max_iter = 7, 6, 20000
+    b = P.rhs_uniform_torch(shape, 1002, device=dev)
+    x = torch.zeros_like(b)
+    torch.cuda.synchronize()
+
+    if args.mode == "kernels":
+        y = torch.empty_like(b)
+        for _ in range(5):
+            aar_hip.Lap_Vec_mult(b, shape, a, out=y)
+        y.copy_(b)                                     # plain copy of 2*8*N bytes (counter scale)
+        aar_hip.AAR(x, b, shape, a, omega, beta, m, p, 24, 1e-30)
+        torch.cuda.synchronize()
+        log("kernels workload done")
+        return
+
+    def solve64():
+        x.zero_()
+        return aar_hip.AAR(x, b, shape, a, omega, beta, m, p, max_iter, args.tol)
+
+    def solve_ref():
+        # FP32 AAR stalls near 3e-5 on this grid (FP32-stored corrections carry
+        # rounding noise amplified by |A|), so inner solves stop at 1e-4
+        x.zero_()
+        return aar_hip.AAR_refined(x, b, shape, a, omega, beta, m, p, args.tol, inner_tol=1e-4,
+                                   max_inner=4000)
+
+    # warm-up (allocations, code objects)
+    for _ in range(args.warmup):
+        st64 = solve64()
+        log(f"warmup fp64: {st64['iterations']} its relres {st64['relres']:.3e} "
+            f"{st64['solve_ms']:.0f} ms")
+    str_ = solve_ref()
+    log(f"warmup refined: {str_['inner_iterations']} inner its / {str_['outer_steps']} outer, relres "
+        f"{str_['relres']:.3e} {str_['solve_ms']:.0f} ms")
+
+    # timed FP64 solves
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = []
+    for k in range(args.steps):
+        stats.append(solve64())
+        log(f"step {k}: fp64 {stats[-1]['iterations']} its {stats[-1]['solve_ms']:.0f} ms")
+    torch.cuda.synchronize()
+    t64 = (time.perf_counter() - t0) / args.steps
+    st64 = stats[-1]
+    assert st64["converged"], st64
+    # true residual of the returned x, by the (bit-exact) device stencil
+    r = b - aar_hip.Lap_Vec_mult(x, shape, a)
+    relres_true = float(torch.linalg.norm(r) / torch.linalg.norm(b))
+    x64 = x.clone()
+
+    # timed FP32-refined solve (one: it is the slower path on this grid)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    str_ = solve_ref()
+    torch.cuda.synchronize()
+    tref = time.perf_counter() - t0
+    log(f"refined: {str_['inner_iterations']} inner its {tref * 1e3:.0f} ms")
+    r = b - aar_hip.Lap_Vec_mult(x, shape, a)
+    relres_ref = float(torch.linalg.norm(r) / torch.linalg.norm(b))
+    u = x - x.mean()
+    v = x64 - x64.mean()
+    diff_ref = float(torch.linalg.norm(u - v) / torch.linalg.norm(v))
+    del r, u, v, x64
+
+    # roofline of the dominant kernel: one profiled FP64 solve, HIP events
+    # around every launch of the fused stencil sweep on the library stream
+    aar_hip.profile(True)
+    aar_hip.profile_reset()
+    solve64()
+    aar_hip.profile(False)
+    kern = {k: aar_hip.profile_query(k) for k in
+            ("stencil_aar_rich", "gram", "ls", "aar_update", "dot")}
+    kr = kern["stencil_aar_rich"]
+    avg_ms = kr["ms"] / kr["count"]
+    bytes_per_launch = kr["bytes"] / kr["count"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+
+    # stencil matvec alone: y = a Lap x, 2*8*N algorithmic bytes
+    y = torch.empty_like(b)
+    aar_hip.profile(True)
+    aar_hip.profile_reset()
+    for _ in range(10):
+        aar_hip.Lap_Vec_mult(b, shape, a, out=y)
+    aar_hip.profile(False)
+    mv = aar_hip.profile_query("stencil_matvec")
+    mv_gbs = mv["bytes"] / mv["ms"] / 1e6
+    del y
+
+    traffic = None
+    tf = os.path.join(REPO, "profiles", "traffic_stencil_aar_rich_512.json")
+    if os.path.exists(tf) and n == 512:
+        with open(tf) as fp:
+            traffic = json.load(fp).get("bytes_per_launch")
+
+    log(f"roofline: stencil_aar_rich {achieved:.0f} GB/s; matvec {mv_gbs:.0f} GB/s")
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(args.cpu_n, args.cpu_iters, st64["iterations"], n)
+        except Exception as e:  # noqa: BLE001
+            log(f"cpu baseline failed: {e}")
+
+    out = {
+        "metric": "AAR solves/s at 512^3 to rel-res 1e-6: FP32-refined vs FP64; matvec HBM GB/s",
+        "value": 1.0 / t64,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t64 * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded U[-0.5,0.5] RHS, mean removed)",
+        "config": {"workload": f"FD12 periodic Poisson {n}^3, h=0.25, AAR m=7 p=6 omega=beta=0.6, "
+                               f"x0=0, tol {args.tol:g}", "grid": [n, n, n], "points": N},
+        "iterations": st64["iterations"],
+        "relres": st64["relres"],
+        "relres_true": relres_true,
+        "fp32_refined": {
+            "solves_per_s": 1.0 / tref, "ms_per_step": tref * 1e3,
+            "speedup_vs_fp64": t64 / tref, "inner_iterations": str_["inner_iterations"],
+            "outer_steps": str_["outer_steps"], "relres_true": relres_ref,
+            "rel_diff_vs_fp64_mod_mean": diff_ref, "dtype": "f32 inner, f64 residual/solution",
+        },
+        "matvec": {"GBps": mv_gbs, "frac_of_peak": mv_gbs / HBM_PEAK_GBS,
+                   "avg_ms": mv["ms"] / mv["count"], "bytes_per_launch": 16.0 * N},
+        "kernels": {k: {"count": v["count"], "avg_ms": (v["ms"] / v["count"]) if v["count"] else 0.0,
+                        "GBps": (v["bytes"] / v["ms"] / 1e6) if v["ms"] > 0 else 0.0}
+                    for k, v in kern.items()},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "stencil_aar_rich (FD12 stencil fused with the Jacobi-Richardson "
+                               "update; algorithmic 4x8 B/pt Richardson, 3x8 B/pt on Anderson "
+                               "iterations)",
+                     "avg_ms": avg_ms, "bytes_per_launch": bytes_per_launch},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -99,12 +99,24 @@ int aar_hip_history(double *out, int cap);
  * reference-signature ones in AAR.h/PGR.h/PL2R.h/CG.h. */
 int aar_hip_last_stats(aar_hip_stats *st);
 
-/* Stream for all work (a hipStream_t); NULL restores the library's own. */
+/* Stream for all work (a hipStream_t; NULL is the default stream).
+ * Without this call the library uses a stream of its own that is ordered
+ * with the default stream. */
 int aar_hip_set_stream(void *stream);
 
-/* Stencil launch shape: lz planes per block (0 = automatic), py rows per
- * thread (2 or 4).  Applies to subsequent calls. */
-int aar_hip_set_tuning(int lz, int py);
+/* Stencil launch shape: lz planes per block (0 = automatic), wy
+ * wavefronts per block stacked in y (4 or 8), py rows per thread (2 or 4);
+ * a block owns a 64 x (wy*py) column.  Applies to subsequent calls. */
+int aar_hip_set_tuning(int lz, int wy, int py);
+
+/* Tuning options of the history sweeps (apply to subsequent solves):
+ *   "ring_pad"    extra elements per ring slot, a multiple of 32 (default
+ *                 512), so that slots whose size is a large power of two
+ *                 do not alias onto the same HBM channels;
+ *   "hist_blocks" grid of the reducing sweeps (0 = automatic, <= 2048);
+ *   "two_pass"    1 (default): reductions finish in a one-block second
+ *                 kernel; 0: in the last block of the reducing kernel. */
+int aar_hip_set_option(const char *name, long value);
 
 /* Per-kernel GPU timing with hipEvents (off by default).  Kernel names:
  * "stencil_matvec", "stencil_aar_rich", "stencil_resid", "stencil_pgr_rich",

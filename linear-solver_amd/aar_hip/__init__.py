@@ -2,5 +2,5 @@
 PL2R; CG as comparator) for the FD12 periodic Poisson family, over the C ABI
 of libaar_hip.so.  See solvers.py for the reference-shaped interface."""
 from .solvers import (AAR, CG, PGR, PL2R, AAR_refined, Lap_Vec_mult, history, pinv, profile,  # noqa: F401
-                      profile_query, profile_reset, release, set_tuning)
+                      profile_query, profile_reset, release, set_option, set_tuning)
 from . import problems  # noqa: F401

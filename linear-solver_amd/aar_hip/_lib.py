@@ -51,7 +51,7 @@ REFERENCE_SYMBOLS = [
 EXTENSION_SYMBOLS = [
     "aar_hip_matvec", "aar_hip_matvec_f32", "aar_hip_aar", "aar_hip_aar_f32", "aar_hip_pgr",
     "aar_hip_cg", "aar_hip_aar_refined", "aar_hip_pinv", "aar_hip_history", "aar_hip_last_stats",
-    "aar_hip_set_stream", "aar_hip_set_tuning", "aar_hip_profile", "aar_hip_profile_reset",
+    "aar_hip_set_stream", "aar_hip_set_tuning", "aar_hip_set_option", "aar_hip_profile", "aar_hip_profile_reset",
     "aar_hip_profile_query", "aar_hip_release", "aar_hip_last_error",
 ]
 
@@ -88,7 +88,8 @@ def load(require_gpu=True):
         "aar_hip_history": [P, I],
         "aar_hip_last_stats": [S],
         "aar_hip_set_stream": [P],
-        "aar_hip_set_tuning": [I, I],
+        "aar_hip_set_tuning": [I, I, I],
+        "aar_hip_set_option": [ctypes.c_char_p, ctypes.c_long],
         "aar_hip_profile": [I],
         "aar_hip_profile_reset": [],
         "aar_hip_profile_query": [ctypes.c_char_p, ctypes.POINTER(I), ctypes.POINTER(D),

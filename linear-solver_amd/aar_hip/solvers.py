@@ -159,8 +159,14 @@ def history(cap=1 << 20):
     return buf[:n].copy()
 
 
-def set_tuning(lz=0, py=4):
-    check(_L().aar_hip_set_tuning(int(lz), int(py)))
+def set_tuning(lz=0, wy=4, py=4):
+    """Stencil launch shape (see aar_hip_set_tuning in include/aar_hip.h)."""
+    check(_L().aar_hip_set_tuning(int(lz), int(wy), int(py)))
+
+
+def set_option(name, value):
+    """Tuning option of the history sweeps (see aar_hip_set_option)."""
+    check(_L().aar_hip_set_option(name.encode(), int(value)))
 
 
 def profile(enable=True):

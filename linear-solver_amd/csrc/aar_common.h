@@ -13,8 +13,7 @@ namespace aar {
 
 constexpr int R = 6;                 // stencil radius: FD12 (FDn = 6, MPI_REAL/main.c:35)
 constexpr int TX = 64;               // tile width in x: one wavefront
-constexpr int TYT = 4;               // wavefronts per block, stacked in y
-constexpr int BLOCK = TX * TYT;      // 256 threads
+constexpr int BLOCK = 256;           // threads per block of the streaming (history/dot) kernels
 constexpr int MAXM = 8;              // largest Anderson history the fused Gram kernels hold in registers
 constexpr int MAXM_LS = 16;          // largest LS system (complex path embeds m -> 2m)
 constexpr int NXCD = 8;              // MI355X: 8 XCDs, blocks dispatched round-robin

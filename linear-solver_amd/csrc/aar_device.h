@@ -18,19 +18,19 @@ __device__ inline double wave_sum(double v)
 }
 
 // Reduce NACC per-thread accumulators over the block; thread k < NACC
-// writes dst[k].  `scratch` must hold NACC * (BLOCK/64) doubles.
-template <int NACC>
+// writes dst[k].  `scratch` must hold NACC * (NT/64) doubles.
+template <int NACC, int NT = BLOCK>
 __device__ inline void block_reduce_to(const double (&acc)[NACC], double *dst, double *scratch)
 {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    constexpr int NW = BLOCK / 64;
+    constexpr int NW = NT / 64;
 #pragma unroll
     for (int k = 0; k < NACC; k++) {
         double v = wave_sum(acc[k]);
         if (lane == 0) scratch[k * NW + wv] = v;
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < NACC; k += BLOCK) {
+    for (int k = threadIdx.x; k < NACC; k += NT) {
         double s = 0.0;
 #pragma unroll
         for (int w = 0; w < NW; w++) s += scratch[k * NW + w];
@@ -56,18 +56,19 @@ __device__ inline bool last_block_arrive(unsigned *counter, unsigned total)
 
 // Sum partials[item * nacc + k] over items in a fixed order; result in
 // out[k] (shared or global), valid for thread 0 after return.
+template <int NT = BLOCK>
 __device__ inline void final_sum(const double *partials, int nitems, int nacc, double *out, double *scratch)
 {
     for (int k = 0; k < nacc; k++) {
         double s = 0.0;
-        for (int i = threadIdx.x; i < nitems; i += BLOCK) s += partials[(size_t)i * nacc + k];
+        for (int i = threadIdx.x; i < nitems; i += NT) s += partials[(size_t)i * nacc + k];
         s = wave_sum(s);
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
         if (lane == 0) scratch[wv] = s;
         __syncthreads();
         if (threadIdx.x == 0) {
             double t = 0.0;
-            for (int w = 0; w < BLOCK / 64; w++) t += scratch[w];
+            for (int w = 0; w < NT / 64; w++) t += scratch[w];
             out[k] = t;
         }
         __syncthreads();

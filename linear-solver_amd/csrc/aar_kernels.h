@@ -12,6 +12,7 @@ enum class SMode : int {
     PGR_RICH = 3,  // u = x + omega (b - Ax)/diag; Au = A u        (PGR.c:92-103)
     CG_Q = 4,      // d = s + beta d (s = r/diag); q = A d; d.q   (CG.c:94-100,148-150)
     CG_RESID = 5,  // r = b - A x; r.s, r.r                        (CG.c:65-84,112-137)
+    MATVEC_RES = 6,  // y = A x and ||b - A x||^2                  (PGR.c:79-81)
 };
 
 template <typename T>
@@ -38,32 +39,40 @@ struct SArgs {
     FinArgs fin = {};
 };
 
+// wy: wavefronts per block stacked in y (4 or 8); py: rows per thread (2 or 4)
 template <typename T>
-hipError_t launch_stencil(SMode m, const SArgs<T> &a, int py, hipStream_t st);
+hipError_t launch_stencil(SMode m, const SArgs<T> &a, int wy, int py, hipStream_t st);
 template <typename T>
-int stencil_items(const SArgs<T> &a, int py);
+int stencil_items(const SArgs<T> &a, int wy, int py);
 
 // ---------------------------------------------------------------------------
 // history sweeps
 // ---------------------------------------------------------------------------
 
-// One Anderson / projection normal-equation build over ring buffers.
-//  kind 0 (AAR, AAR.c:192-212):    DF_j = F[a_j] - F[b_j]; G = DF^T DF (sym), g = DF^T f
-//  kind 1 (PGR, PGR.c:188-206):    DX_j = X[a_j]-X[b_j], DF_j = Y[a_j]-Y[b_j]; G = DX^T DF, g = DX^T f
-//  kind 2 (PL2R, PL2R.c:187-206):  DF_j = Y[a_j]-Y[b_j]; G = DF^T DF (sym), g = DF^T f
-// For kinds 1/2 f = b - Y[fslot] (unpreconditioned residual).
-// Columns j with valid[j] == 0 are zero (never written yet, AAR.c:64-65).
+// History columns are indexed by AGE: the ring slot of age k holds the
+// iterate (or residual) k steps before the newest, and the difference
+// column of age k is slot(k) - slot(k+1) -- the reference's
+// DX = x - x_old, DF = f - f_old (AAR.c:96-102, PGR.c:105-109).  Columns
+// of age < nvalid exist (a prefix; the rest are the reference's never
+// written zero columns, AAR.c:64-65).  The reference stores column of age
+// k at index (it - 2 - k) mod m (AAR.c:97; PGR.c:105 with it = t), which
+// is how results are laid out for the m x m solve.  In DIRECT mode (the
+// stand-alone Anderson/Galerkin/L2 entry points) slot k holds column k
+// itself and the layout is the identity.
 template <typename T>
 struct GramArgs {
     long n = 0;
     int m = 0;
-    int kind = 0;
-    const T *F = nullptr;     // ring base: slot s at F + s*ld (kind 0: f; kinds 1/2: X)
-    const T *Y = nullptr;     // ring base of A x (kinds 1/2)
-    const T *b = nullptr;     // rhs (kinds 1/2)
+    int kind = 0;             // 0 AAR (DF^T DF), 1 PGR (DX^T DF), 2 PL2R (DF^T DF of A x)
+    int direct = 0;
+    const T *F = nullptr;     // kind 0: residual ring; kinds 1/2: iterate ring
+    const T *Y = nullptr;     // kinds 1/2: ring of A x
+    const T *b = nullptr;     // kinds 1/2: rhs, f = b - Y[fslot]
     long ld = 0;
-    int fslot = 0;            // slot of f (kind 0) or of the current Ax (kinds 1/2)
-    int sa[MAXM] = {}, sb[MAXM] = {}, valid[MAXM] = {};
+    int slot[MAXM + 1] = {};  // ring slot of age k
+    int fslot = 0;            // slot of f (kind 0) or of A x (kinds 1/2)
+    int nvalid = 0;
+    int it = 0;
     int nblocks = 0;
     double *partials = nullptr;
     unsigned *counter = nullptr;
@@ -78,23 +87,26 @@ hipError_t launch_gram(const GramArgs<T> &a, hipStream_t st);
 // ctrl->G (allredvec layout), writes ctrl->s, ctrl->w.
 hipError_t launch_ls(Ctrl *ctrl, int m, int complex_embed, hipStream_t st);
 
-// AAR.c:225-231 + 116-117:  x_new = (x_k + beta f_k) - sum_j (DX_j + beta DF_j) s_j
+// AAR.c:225-231 + 116-117:  x_new = (x_k + beta f_k) - sum_k (DX_k + beta DF_k) s_col(k)
 template <typename T>
 struct UpdArgs {
     long n = 0;
     int m = 0;
+    int direct = 0;
     double beta = 0.0;
     const T *X = nullptr, *F = nullptr;
     long ld = 0;
+    int slot[MAXM + 1] = {};
+    int nvalid = 0;
+    int it = 0;
     int xslot = 0, fslot = 0, oslot = 0;
-    int sa[MAXM] = {}, sb[MAXM] = {}, valid[MAXM] = {};
     const Ctrl *ctrl = nullptr;
 };
 template <typename T>
 hipError_t launch_aar_update(const UpdArgs<T> &a, hipStream_t st);
 
-// PGR.c:117-130: x += sum_i DX_i s_i; Ax += sum_i DF_i s_i (in place at
-// slot `cur`), ||b - Ax||^2 -> FIN_PGR_RES.
+// PGR.c:117-130: x += sum DX s; Ax += sum DF s (in place at slot `cur`,
+// the age-0 slot), ||b - Ax||^2 -> FIN_PGR_RES.
 template <typename T>
 struct PUpdArgs {
     long n = 0;
@@ -102,8 +114,9 @@ struct PUpdArgs {
     T *X = nullptr, *Y = nullptr;
     const T *b = nullptr;
     long ld = 0;
-    int cur = 0;
-    int sa[MAXM] = {}, sb[MAXM] = {}, valid[MAXM] = {};
+    int slot[MAXM + 1] = {};
+    int nvalid = 0;
+    int it = 0;
     int nblocks = 0;
     double *partials = nullptr;
     unsigned *counter = nullptr;
@@ -142,6 +155,13 @@ struct CgUpdArgs {
     FinArgs fin = {};
 };
 hipError_t launch_cg_update(const CgUpdArgs &a, hipStream_t st);
+
+// Reducing kernels run the finaliser in their last block when given a
+// counter, or only write per-block partials when counter == nullptr; then
+// launch_finish (or, for the Gram, launch_gram itself) runs a one-block
+// second pass.
+hipError_t launch_finish(const double *partials, int nitems, int nacc, const FinArgs &fin, Ctrl *ctrl,
+                         hipStream_t st);
 
 // small pointwise helpers
 hipError_t launch_copy_f64(double *dst, const double *src, long n, const Ctrl *ctrl, hipStream_t st);

@@ -36,6 +36,7 @@ constexpr int NPOLL = 64;
 struct Context {
     bool init = false;
     hipStream_t own = nullptr, stream = nullptr;
+    bool stream_set = false;     // aar_hip_set_stream() chose `stream` (may be the null stream)
     Ctrl *ctrl = nullptr;        // [2]: 0 outer / plain solves, 1 inner FP32 solves
     Ctrl *ctrl_h = nullptr;      // pinned host mirror [2]
     int *poll_h = nullptr;       // pinned polling slots
@@ -46,7 +47,10 @@ struct Context {
     DevBuf hist0, hist1, part0, part1;
     DevBuf vec32a, vec32b, vec64a, vec64b;
     std::string err;
-    int lz = 0, py = 4;
+    int lz = 0, wy = 8, py = 4;
+    long ring_pad = 512;           // extra elements per ring slot (multiple of 32)
+    int hist_blocks = 0;           // grid of the reducing history kernels (0: automatic)
+    bool two_pass = true;          // reductions finish in a one-block second kernel
     bool prof = false;
     std::map<std::string, ProfEntry> profile;
     std::vector<PendingEv> pending;
@@ -72,7 +76,7 @@ int make_op(const aar_hip_grid *g, const double *coeff, Op &op);
 
 template <typename T>
 int aar_solve(const Op &op, T *x, const T *b, double omega, double beta, int m, int p, int max_iter,
-              double tol, int precond, int ci, aar_hip_stats *st, int keep_history);
+              double tol, int precond, int ci, aar_hip_stats *st, int keep_history, int ref_start = 1);
 int pgr_solve(const Op &op, double *x, const double *b, double omega, int m, int p, int max_iter,
               double tol, int precond, int l2, aar_hip_stats *st);
 int cg_solve(const Op &op, double *x, const double *b, int max_iter, double tol, int precond,

@@ -153,9 +153,10 @@ void host_solve(const char *name, POISSON *sys, aar_op_cb opcb, aar_pc_cb pccb, 
     report(name, st);
 }
 
-// host matrices -> device ring slots for the stand-alone Anderson /
-// Galerkin / L2 steps: slot 0 holds zeros, slot j+1 column j, so the
-// difference kernels see (column - 0), which is the column exactly.
+// host matrices -> device slots for the stand-alone Anderson / Galerkin /
+// L2 steps (DIRECT mode of the history kernels): slot j+1 holds column j,
+// slot m+1 of F holds f, slot 0 holds zeros (the update's base point and,
+// for PGR/PL2R, the A x of f = b - A x with b = f).
 struct StepBufs {
     double *X = nullptr, *F = nullptr;
 };
@@ -211,11 +212,9 @@ int step_gram_ls(int kind, const StepBufs &sb, int m, int n)
     g.m = m;
     g.kind = kind;
     g.ld = n;
-    for (int j = 0; j < m; j++) {
-        g.valid[j] = 1;
-        g.sa[j] = j + 1;
-        g.sb[j] = 0;
-    }
+    g.direct = 1;
+    g.nvalid = m;
+    for (int j = 0; j < m; j++) g.slot[j] = j + 1;
     g.nblocks = nb;
     g.partials = (double *)part.p;
     g.counter = C.counters + 32;
@@ -609,11 +608,9 @@ void AndersonExtrapolation(double **DX, double **DF, double *f, double beta, int
     u.X = sb.X;
     u.F = sb.F;
     u.ld = N;
-    for (int j = 0; j < m; j++) {
-        u.valid[j] = 1;
-        u.sa[j] = j + 1;
-        u.sb[j] = 0;
-    }
+    u.direct = 1;
+    u.nvalid = m;
+    for (int j = 0; j < m; j++) u.slot[j] = j + 1;
     u.xslot = 0;
     u.fslot = 0;
     u.oslot = 0;   // written after all reads of slot 0 in the same pass (pointwise)
@@ -739,16 +736,37 @@ int aar_hip_set_stream(void *stream)
 {
     Context &C = ctx();
     if (int rc = context_init()) return rc;
-    C.stream = stream ? (hipStream_t)stream : C.own;
+    C.stream = (hipStream_t)stream;
+    C.stream_set = true;
     return 0;
 }
 
-int aar_hip_set_tuning(int lz, int py)
+int aar_hip_set_tuning(int lz, int wy, int py)
 {
     if (py != 2 && py != 4) return fail(AAR_HIP_ERR_ARG, "py must be 2 or 4");
+    if (wy != 4 && wy != 8) return fail(AAR_HIP_ERR_ARG, "wy must be 4 or 8");
     if (lz < 0) return fail(AAR_HIP_ERR_ARG, "lz must be >= 0");
     ctx().lz = lz;
+    ctx().wy = wy;
     ctx().py = py;
+    return 0;
+}
+
+int aar_hip_set_option(const char *name, long value)
+{
+    Context &C = ctx();
+    const std::string k = name ? name : "";
+    if (k == "ring_pad") {
+        if (value < 0 || value % 32) return fail(AAR_HIP_ERR_ARG, "ring_pad must be a non-negative multiple of 32");
+        C.ring_pad = value;
+    } else if (k == "hist_blocks") {
+        if (value < 0 || value > 16384) return fail(AAR_HIP_ERR_ARG, "hist_blocks must be in 0..16384");
+        C.hist_blocks = (int)value;
+    } else if (k == "two_pass") {
+        C.two_pass = value != 0;
+    } else {
+        return fail(AAR_HIP_ERR_ARG, "unknown option " + k);
+    }
     return 0;
 }
 

@@ -15,33 +15,65 @@
 
 namespace aar {
 
-template <typename T> __device__ inline T tsub(T a, T b) { return a - b; }
+template <typename T> struct Vec2;
+template <> struct Vec2<double> { using type = double2; };
+template <> struct Vec2<float> { using type = float2; };
+
+// reference column index of the difference column of age k (AAR.c:97,
+// PGR.c:105); identity in DIRECT mode
+__device__ inline int ref_col(int it, int k, int m, int direct)
+{
+    return direct ? k : ((it - 2 - k) % m + m) % m;
+}
+
+// reference allredvec layout (AAR.c:192-212, PGR.c:188-206): [j*m + i] =
+// col_i . col_j, [m*m + j] = col_j . f; missing columns are zero
+template <typename T, int MM, int KIND>
+__device__ inline void gram_layout(const GramArgs<T> &a, const double *sums)
+{
+    constexpr int NG = (KIND == 1) ? MM * MM : MM * (MM + 1) / 2;
+    const int m = a.m, nv = a.nvalid, dir = a.direct;
+    double *G = a.ctrl->G;
+    for (int k = 0; k < m * m + m; k++) G[k] = 0.0;
+    if constexpr (KIND == 1) {
+        for (int j = 0; j < nv; j++)
+            for (int q = 0; q < nv; q++)
+                G[ref_col(a.it, j, m, dir) * m + ref_col(a.it, q, m, dir)] = sums[j * MM + q];
+    } else {
+        for (int j = 0; j < nv; j++)
+            for (int q = 0; q <= j; q++) {
+                const double v = sums[j * (j + 1) / 2 + q];
+                const int cj = ref_col(a.it, j, m, dir), cq = ref_col(a.it, q, m, dir);
+                G[cj * m + cq] = v;
+                G[cq * m + cj] = v;
+            }
+    }
+    for (int j = 0; j < nv; j++) G[m * m + ref_col(a.it, j, m, dir)] = sums[NG + j];
+}
 
 // ---------------------------------------------------------------------------
-// normal equations, one pass
+// normal equations, one pass over the m+1 ring slots
 // ---------------------------------------------------------------------------
 template <typename T, int MM, int KIND>
 __global__ __launch_bounds__(BLOCK) void gram_kernel(const GramArgs<T> a)
 {
     constexpr int NG = (KIND == 1) ? MM * MM : MM * (MM + 1) / 2;
     constexpr int NA = NG + MM;
+    constexpr int NS = MM + 1;
     __shared__ double scratch[NA * (BLOCK / 64)];
     __shared__ double sums[NA];
     if (a.ctrl->done) return;
 
-    const int m = a.m;
-    const T *pa[MM], *pb[MM], *qa[MM], *qb[MM];
-    bool ok[MM];
+    const int m = a.m, nv = a.nvalid, dir = a.direct;
+    const T *D = (KIND == 0) ? a.F : a.Y;     // ring of the DF columns
+    const T *pd[NS], *px[NS];
+    bool use[NS];
 #pragma unroll
-    for (int j = 0; j < MM; j++) {
-        ok[j] = (j < m) && a.valid[j];
-        const int sa = ok[j] ? a.sa[j] : 0, sb = ok[j] ? a.sb[j] : 0;
-        pa[j] = a.F + (size_t)sa * a.ld;
-        pb[j] = a.F + (size_t)sb * a.ld;
-        if constexpr (KIND != 0) {
-            qa[j] = a.Y + (size_t)sa * a.ld;
-            qb[j] = a.Y + (size_t)sb * a.ld;
-        }
+    for (int k = 0; k < NS; k++) {
+        use[k] = dir ? (k < nv) : (k <= nv);
+        const size_t off = (size_t)(use[k] ? a.slot[k] : 0) * a.ld;
+        pd[k] = D + off;
+        px[k] = a.F + off;
     }
     const T *pf = (KIND == 0 ? a.F : a.Y) + (size_t)a.fslot * a.ld;
 
@@ -49,27 +81,17 @@ __global__ __launch_bounds__(BLOCK) void gram_kernel(const GramArgs<T> a)
 #pragma unroll
     for (int k = 0; k < NA; k++) acc[k] = 0.0;
 
-    const long chunk = (a.n + a.nblocks - 1) / a.nblocks;
-    const long beg = (long)blockIdx.x * chunk;
-    const long end = min(a.n, beg + chunk);
-    for (long i = beg + threadIdx.x; i < end; i += BLOCK) {
-        T fv;
-        if constexpr (KIND == 0) fv = pf[i];
-        else fv = tsub(a.b[i], pf[i]);                     // PGR.c:102-103: f = rhs - Ax
+    // one point: DF (and DX) columns by age, then all products in FP64
+    auto point = [&](T fv, const T (&vd)[NS], const T (&vx)[NS]) {
         double d[MM], e[MM];
 #pragma unroll
-        for (int j = 0; j < MM; j++) {
-            d[j] = 0.0;
-            e[j] = 0.0;
-            if (ok[j]) {
-                if constexpr (KIND == 0) {
-                    d[j] = (double)tsub(pa[j][i], pb[j][i]);   // DF = f - f_old (AAR.c:100)
-                } else if constexpr (KIND == 1) {
-                    e[j] = (double)tsub(pa[j][i], pb[j][i]);   // DX = x - x_old (PGR.c:107)
-                    d[j] = (double)tsub(qa[j][i], qb[j][i]);   // DF = Ax - Ax_old (PGR.c:108)
-                } else {
-                    d[j] = (double)tsub(qa[j][i], qb[j][i]);
-                }
+        for (int k = 0; k < MM; k++) {
+            d[k] = 0.0;
+            e[k] = 0.0;
+            if (k < nv) {
+                d[k] = dir ? (double)vd[k] : (double)(vd[k] - vd[k + 1]);   // DF = f - f_old
+                if constexpr (KIND == 1)
+                    e[k] = dir ? (double)vx[k] : (double)(vx[k] - vx[k + 1]);   // DX = x - x_old
             }
         }
         const double fd = (double)fv;
@@ -78,58 +100,111 @@ __global__ __launch_bounds__(BLOCK) void gram_kernel(const GramArgs<T> a)
             for (int j = 0; j < MM; j++) {
 #pragma unroll
                 for (int q = 0; q < MM; q++) acc[j * MM + q] += e[q] * d[j];   // DX_q . DF_j
-                acc[NG + j] += e[j] * fd;
+                acc[NG + j] += e[j] * fd;                                      // DX_j . f
             }
         } else {
             int t = 0;
 #pragma unroll
             for (int j = 0; j < MM; j++) {
 #pragma unroll
-                for (int q = 0; q <= j; q++) acc[t++] += d[q] * d[j];
-                acc[NG + j] += d[j] * fd;
+                for (int q = 0; q <= j; q++) acc[t++] += d[q] * d[j];          // DF_q . DF_j
+                acc[NG + j] += d[j] * fd;                                      // DF_j . f
             }
+        }
+    };
+
+    // grid-stride (a fixed point-to-thread assignment, so the reduction
+    // order is fixed); two consecutive points per 16-byte load when N is
+    // even (every slot is then 16-byte aligned)
+    const long n = a.n;
+    const long stride = (long)gridDim.x * BLOCK;
+    if ((n & 1) == 0) {
+        using V2 = typename Vec2<T>::type;
+        const long np = n >> 1;
+        for (long ip = (long)blockIdx.x * BLOCK + threadIdx.x; ip < np; ip += stride) {
+            const long i = 2 * ip;
+            T vd0[NS], vd1[NS], vx0[NS], vx1[NS];
+#pragma unroll
+            for (int k = 0; k < NS; k++) {
+                vd0[k] = vd1[k] = vx0[k] = vx1[k] = (T)0;
+                if (use[k]) {
+                    const V2 v = *reinterpret_cast<const V2 *>(pd[k] + i);
+                    vd0[k] = v.x;
+                    vd1[k] = v.y;
+                    if constexpr (KIND == 1) {
+                        const V2 w = *reinterpret_cast<const V2 *>(px[k] + i);
+                        vx0[k] = w.x;
+                        vx1[k] = w.y;
+                    }
+                }
+            }
+            const V2 fv = *reinterpret_cast<const V2 *>(pf + i);
+            T f0 = fv.x, f1 = fv.y;
+            if constexpr (KIND != 0) {
+                const V2 bv = *reinterpret_cast<const V2 *>(a.b + i);
+                f0 = bv.x - fv.x;                                             // PGR.c:102-103
+                f1 = bv.y - fv.y;
+            }
+            point(f0, vd0, vx0);
+            point(f1, vd1, vx1);
+        }
+    } else {
+        for (long i = (long)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) {
+            T vd[NS], vx[NS];
+#pragma unroll
+            for (int k = 0; k < NS; k++) {
+                vd[k] = use[k] ? pd[k][i] : (T)0;
+                vx[k] = (KIND == 1 && use[k]) ? px[k][i] : (T)0;
+            }
+            T f = pf[i];
+            if constexpr (KIND != 0) f = a.b[i] - f;
+            point(f, vd, vx);
         }
     }
 
     block_reduce_to<NA>(acc, a.partials + (size_t)blockIdx.x * NA, scratch);
+    if (!a.counter) return;                     // two-pass: gram_finish_kernel follows
     if (last_block_arrive(a.counter, gridDim.x)) {
-        // fixed-order sum over blocks, one entry per thread group
-        for (int k = threadIdx.x; k < NA; k += BLOCK) {
-            double s = 0.0;
-            for (int bidx = 0; bidx < a.nblocks; bidx++) s += a.partials[(size_t)bidx * NA + k];
-            sums[k] = s;
-        }
-        __syncthreads();
+        // all threads, fixed order: strided partial sums then a fixed tree
+        final_sum(a.partials, a.nblocks, NA, sums, scratch);
         if (threadIdx.x == 0) {
-            // reference allredvec layout (AAR.c:192-212, PGR.c:188-206)
-            double *G = a.ctrl->G;
-            if constexpr (KIND == 1) {
-                for (int j = 0; j < m; j++)
-                    for (int q = 0; q < m; q++) G[j * m + q] = sums[j * MM + q];
-            } else {
-                for (int j = 0; j < m; j++)
-                    for (int q = 0; q <= j; q++) {
-                        const double v = sums[j * (j + 1) / 2 + q];
-                        G[j * m + q] = v;
-                        G[q * m + j] = v;
-                    }
-            }
-            for (int j = 0; j < m; j++) G[m * m + j] = sums[NG + j];
+            gram_layout<T, MM, KIND>(a, sums);
             *a.counter = 0u;
         }
     }
 }
 
+// second pass of the normal-equation build: one block sums the per-block
+// partials in a fixed order and lays the result out for the m x m solve
+template <typename T, int MM, int KIND>
+__global__ __launch_bounds__(BLOCK) void gram_finish_kernel(const GramArgs<T> a)
+{
+    constexpr int NG = (KIND == 1) ? MM * MM : MM * (MM + 1) / 2;
+    constexpr int NA = NG + MM;
+    __shared__ double scratch[BLOCK / 64];
+    __shared__ double sums[NA];
+    if (a.ctrl->done) return;
+    final_sum(a.partials, a.nblocks, NA, sums, scratch);
+    if (threadIdx.x == 0) gram_layout<T, MM, KIND>(a, sums);
+}
+
 template <typename T>
 hipError_t launch_gram(const GramArgs<T> &a, hipStream_t st)
 {
-    if (a.m < 1 || a.m > MAXM) return hipErrorInvalidValue;
+    if (a.m < 1 || a.m > MAXM || a.nvalid < 0 || a.nvalid > a.m) return hipErrorInvalidValue;
     dim3 grid(a.nblocks), block(BLOCK);
     switch (a.kind) {
     case 0: hipLaunchKernelGGL((gram_kernel<T, MAXM, 0>), grid, block, 0, st, a); break;
     case 1: hipLaunchKernelGGL((gram_kernel<T, MAXM, 1>), grid, block, 0, st, a); break;
     case 2: hipLaunchKernelGGL((gram_kernel<T, MAXM, 2>), grid, block, 0, st, a); break;
     default: return hipErrorInvalidValue;
+    }
+    if (!a.counter) {
+        switch (a.kind) {
+        case 0: hipLaunchKernelGGL((gram_finish_kernel<T, MAXM, 0>), dim3(1), block, 0, st, a); break;
+        case 1: hipLaunchKernelGGL((gram_finish_kernel<T, MAXM, 1>), dim3(1), block, 0, st, a); break;
+        default: hipLaunchKernelGGL((gram_finish_kernel<T, MAXM, 2>), dim3(1), block, 0, st, a); break;
+        }
     }
     return hipGetLastError();
 }
@@ -146,7 +221,7 @@ template hipError_t launch_gram<float>(const GramArgs<float> &, hipStream_t);
 // by V (tools.c:324-329) and no U-only step feeds w, the superdiagonal or
 // V.  The left reflector is applied to columns 1..n-1 (tools.c:137).
 // ---------------------------------------------------------------------------
-__device__ inline double d_pythag(double a, double b)
+__device__ __attribute__((always_inline)) inline double d_pythag(double a, double b)
 {
     const double absa = fabs(a), absb = fabs(b);
     if (absa > absb) return absa * sqrt(1.0 + absb * absb / (absa * absa));
@@ -154,8 +229,143 @@ __device__ inline double d_pythag(double a, double b)
 }
 __device__ inline double d_sign(double mag, double s) { return s >= 0.0 ? fabs(mag) : -fabs(mag); }
 
+// One singular value of the implicit-shift QR phase (tools.c:254-350),
+// k a compile-time constant so that every array index below is static and
+// w, rv1 and V stay in registers.
+template <int M, int k>
+__device__ __attribute__((always_inline)) inline void gr_qr_step(double (&w)[M], double (&rv1)[M], double (&v)[M][M], const double anorm,
+                                  const int n, int &status)
+{
+    double g = 0.0, s = 0.0, f = 0.0, h = 0.0;
+    if (k < n && !status) {
+        for (int its = 0; its <= 250; its++) {
+            int flag = 1, l = 0;
+            bool found = false;
+    #pragma unroll
+            for (int ll = k; ll >= 0; ll--) {
+                if (!found) {
+                    if ((double)(fabs(rv1[ll]) + anorm) == anorm) {
+                        flag = 0;
+                        l = ll;
+                        found = true;
+                    } else if (ll >= 1 && (double)(fabs(w[ll >= 1 ? ll - 1 : 0]) + anorm) == anorm) {
+                        l = ll;
+                        found = true;
+                    }
+                }
+            }
+            if (flag) {
+                double c = 0.0;
+                s = 1.0;
+                bool stop = false;
+    #pragma unroll
+                for (int i = 0; i <= k; i++) {
+                    if (i >= l && !stop) {
+                        f = s * rv1[i];
+                        rv1[i] = c * rv1[i];
+                        if ((double)(fabs(f) + anorm) == anorm) {
+                            stop = true;
+                        } else {
+                            g = w[i];
+                            h = d_pythag(f, g);
+                            w[i] = h;
+                            h = 1.0 / h;
+                            c = g * h;
+                            s = -f * h;
+                        }
+                    }
+                }
+            }
+            double z = w[k];
+            if (l == k) {
+                if (z < 0.0) {
+                    w[k] = -z;
+    #pragma unroll
+                    for (int j = 0; j < M; j++)
+                        if (j < n) v[j][k] = -v[j][k];
+                }
+                break;
+            }
+            if (its == 250) {
+                status = 1;
+                break;
+            }
+            double x = 0.0;
+    #pragma unroll
+            for (int ll = 0; ll <= k; ll++)
+                if (ll == l) x = w[ll];
+            const int nm = k - 1;
+            double y = w[nm >= 0 ? nm : 0];
+            g = rv1[nm >= 0 ? nm : 0];
+            h = rv1[k];
+            f = ((y - z) * (y + z) + (g - h) * (g + h)) / (2.0 * h * y);
+            g = d_pythag(f, 1.0);
+            f = ((x - z) * (x + z) + h * ((y / (f + d_sign(g, f))) - h)) / x;
+            double c = 1.0;
+            s = 1.0;
+    #pragma unroll
+            for (int j = 0; j < k; j++) {
+                if (j >= l) {
+                    const int i = j + 1;
+                    g = rv1[i];
+                    y = w[i];
+                    h = s * g;
+                    g = c * g;
+                    z = d_pythag(f, h);
+                    rv1[j] = z;
+                    c = f / z;
+                    s = h / z;
+                    f = x * c + g * s;
+                    g = g * c - x * s;
+                    h = y * s;
+                    y *= c;
+    #pragma unroll
+                    for (int jj = 0; jj < M; jj++) {
+                        if (jj < n) {
+                            const double vx = v[jj][j], vz = v[jj][i];
+                            v[jj][j] = vx * c + vz * s;
+                            v[jj][i] = vz * c - vx * s;
+                        }
+                    }
+                    z = d_pythag(f, h);
+                    w[j] = z;
+                    if (z != 0.0) {
+                        z = 1.0 / z;
+                        c = f * z;
+                        s = h * z;
+                    }
+                    f = c * g + s * y;
+                    x = c * y - s * g;
+                }
+            }
+    #pragma unroll
+            for (int ll = 0; ll <= k; ll++)
+                if (ll == l) rv1[ll] = 0.0;
+            rv1[k] = f;
+            w[k] = x;
+        }
+    }
+}
+
+template <int M, int k>
+struct GrQr {
+    __device__ __attribute__((always_inline)) static inline void run(double (&w)[M], double (&rv1)[M], double (&v)[M][M], const double anorm,
+                                      const int n, int &status)
+    {
+        gr_qr_step<M, k>(w, rv1, v, anorm, n, status);
+        GrQr<M, k - 1>::run(w, rv1, v, anorm, n, status);
+    }
+};
 template <int M>
-__device__ int gr_pinv(double (&a)[M][M], const int n, const double (&bv)[M], double (&xv)[M],
+struct GrQr<M, -1> {
+    __device__ static inline void run(double (&)[M], double (&)[M], double (&)[M][M], const double, const int,
+                                      int &)
+    {
+    }
+};
+
+template <int M>
+__device__ __attribute__((always_inline)) inline int gr_pinv(double (&a)[M][M], const int n, const double (&bv)[M], double (&xv)[M],
                        double (&w)[M], double (&v)[M][M])
 {
     double rv1[M];
@@ -285,119 +495,9 @@ __device__ int gr_pinv(double (&a)[M][M], const int n, const double (&bv)[M], do
         }
     }
 
-    // implicit-shift QR on the bidiagonal
+    // implicit-shift QR on the bidiagonal, k = n-1 .. 0
     int status = 0;
-#pragma unroll
-    for (int k = M - 1; k >= 0; k--) {
-        if (k < n && !status) {
-            for (int its = 0; its <= 250; its++) {
-                int flag = 1, l = 0;
-                bool found = false;
-#pragma unroll
-                for (int ll = k; ll >= 0; ll--) {
-                    if (!found) {
-                        if ((double)(fabs(rv1[ll]) + anorm) == anorm) {
-                            flag = 0;
-                            l = ll;
-                            found = true;
-                        } else if (ll >= 1 && (double)(fabs(w[ll >= 1 ? ll - 1 : 0]) + anorm) == anorm) {
-                            l = ll;
-                            found = true;
-                        }
-                    }
-                }
-                if (flag) {
-                    double c = 0.0;
-                    s = 1.0;
-                    bool stop = false;
-#pragma unroll
-                    for (int i = 0; i <= k; i++) {
-                        if (i >= l && !stop) {
-                            f = s * rv1[i];
-                            rv1[i] = c * rv1[i];
-                            if ((double)(fabs(f) + anorm) == anorm) {
-                                stop = true;
-                            } else {
-                                g = w[i];
-                                h = d_pythag(f, g);
-                                w[i] = h;
-                                h = 1.0 / h;
-                                c = g * h;
-                                s = -f * h;
-                            }
-                        }
-                    }
-                }
-                double z = w[k];
-                if (l == k) {
-                    if (z < 0.0) {
-                        w[k] = -z;
-#pragma unroll
-                        for (int j = 0; j < M; j++)
-                            if (j < n) v[j][k] = -v[j][k];
-                    }
-                    break;
-                }
-                if (its == 250) {
-                    status = 1;
-                    break;
-                }
-                double x = 0.0;
-#pragma unroll
-                for (int ll = 0; ll <= k; ll++)
-                    if (ll == l) x = w[ll];
-                const int nm = k - 1;
-                double y = w[nm >= 0 ? nm : 0];
-                g = rv1[nm >= 0 ? nm : 0];
-                h = rv1[k];
-                f = ((y - z) * (y + z) + (g - h) * (g + h)) / (2.0 * h * y);
-                g = d_pythag(f, 1.0);
-                f = ((x - z) * (x + z) + h * ((y / (f + d_sign(g, f))) - h)) / x;
-                double c = 1.0;
-                s = 1.0;
-#pragma unroll
-                for (int j = 0; j < k; j++) {
-                    if (j >= l) {
-                        const int i = j + 1;
-                        g = rv1[i];
-                        y = w[i];
-                        h = s * g;
-                        g = c * g;
-                        z = d_pythag(f, h);
-                        rv1[j] = z;
-                        c = f / z;
-                        s = h / z;
-                        f = x * c + g * s;
-                        g = g * c - x * s;
-                        h = y * s;
-                        y *= c;
-#pragma unroll
-                        for (int jj = 0; jj < M; jj++) {
-                            if (jj < n) {
-                                const double vx = v[jj][j], vz = v[jj][i];
-                                v[jj][j] = vx * c + vz * s;
-                                v[jj][i] = vz * c - vx * s;
-                            }
-                        }
-                        z = d_pythag(f, h);
-                        w[j] = z;
-                        if (z != 0.0) {
-                            z = 1.0 / z;
-                            c = f * z;
-                            s = h * z;
-                        }
-                        f = c * g + s * y;
-                        x = c * y - s * g;
-                    }
-                }
-#pragma unroll
-                for (int ll = 0; ll <= k; ll++)
-                    if (ll == l) rv1[ll] = 0.0;
-                rv1[k] = f;
-                w[k] = x;
-            }
-        }
-    }
+    GrQr<M, M - 1>::run(w, rv1, v, anorm, n, status);
 
     // truncation (tools.c:331-335) and x = V diag(1/w) V^T b (tools.c:76-94)
     double wmax = 0.0;
@@ -470,54 +570,102 @@ hipError_t launch_ls(Ctrl *ctrl, int n, int /*complex_embed*/, hipStream_t st)
 }
 
 // ---------------------------------------------------------------------------
-// Anderson update (AAR.c:225-231 then AAR.c:116-117)
+// Anderson update (AAR.c:225-231 then AAR.c:116-117):
+//   x_new = (x_k + beta f_k) - sum_k (DX_k + beta DF_k) s_col(k)
+// The m-term sum runs in age order (the reference sums in column order;
+// the two differ by summation order only).
 // ---------------------------------------------------------------------------
 template <typename T, int MM>
 __global__ __launch_bounds__(BLOCK) void aar_update_kernel(const UpdArgs<T> a)
 {
+    constexpr int NS = MM + 1;
     if (a.ctrl->done) return;
-    const T *xa[MM], *xb[MM], *fa[MM], *fb[MM];
-    bool ok[MM];
+    const int nv = a.nvalid, dir = a.direct;
+    const T *px[NS], *pf[NS];
+    bool use[NS];
     double sv[MM];
 #pragma unroll
-    for (int j = 0; j < MM; j++) {
-        ok[j] = (j < a.m) && a.valid[j];
-        const int sa = ok[j] ? a.sa[j] : 0, sb = ok[j] ? a.sb[j] : 0;
-        xa[j] = a.X + (size_t)sa * a.ld;
-        xb[j] = a.X + (size_t)sb * a.ld;
-        fa[j] = a.F + (size_t)sa * a.ld;
-        fb[j] = a.F + (size_t)sb * a.ld;
-        sv[j] = j < a.m ? a.ctrl->s[j] : 0.0;
+    for (int k = 0; k < NS; k++) {
+        use[k] = dir ? (k < nv) : (k <= nv);
+        const size_t off = (size_t)(use[k] ? a.slot[k] : 0) * a.ld;
+        px[k] = a.X + off;
+        pf[k] = a.F + off;
     }
-    const T *xk = a.X + (size_t)a.xslot * a.ld;
-    const T *fk = a.F + (size_t)a.fslot * a.ld;
+#pragma unroll
+    for (int k = 0; k < MM; k++) sv[k] = k < nv ? a.ctrl->s[ref_col(a.it, k, a.m, dir)] : 0.0;
+    const T *xk = dir ? a.X + (size_t)a.xslot * a.ld : px[0];
+    const T *fk = dir ? a.F + (size_t)a.fslot * a.ld : pf[0];
     T *xo = const_cast<T *>(a.X) + (size_t)a.oslot * a.ld;
     const T beta = (T)a.beta;
-    const long stride = (long)gridDim.x * BLOCK;
-    for (long i = (long)blockIdx.x * BLOCK + threadIdx.x; i < a.n; i += stride) {
+
+    auto point = [&](T xb, T fb, const T (&vx)[NS], const T (&vf)[NS]) -> T {
         double am = 0.0;
 #pragma unroll
-        for (int j = 0; j < MM; j++) {
-            if (j < a.m) {
-                T dx = (T)0, df = (T)0;
-                if (ok[j]) {
-                    dx = xa[j][i] - xb[j][i];
-                    df = fa[j][i] - fb[j][i];
-                }
-                am = am + (double)(dx + beta * df) * sv[j];
+        for (int k = 0; k < MM; k++) {
+            if (k < nv) {
+                const T dx = dir ? vx[k] : vx[k] - vx[k + 1];
+                const T df = dir ? vf[k] : vf[k] - vf[k + 1];
+                am = am + (double)(dx + beta * df) * sv[k];
             }
         }
-        const T xb0 = xk[i] + beta * fk[i];
-        xo[i] = (T)((double)xb0 - am);
+        const T x0 = xb + beta * fb;
+        return (T)((double)x0 - am);
+    };
+
+    const long n = a.n;
+    const long stride = (long)gridDim.x * BLOCK;
+    if ((n & 1) == 0) {
+        using V2 = typename Vec2<T>::type;
+        const long np = n >> 1;
+        for (long ip = (long)blockIdx.x * BLOCK + threadIdx.x; ip < np; ip += stride) {
+            const long i = 2 * ip;
+            T vx0[NS], vx1[NS], vf0[NS], vf1[NS];
+#pragma unroll
+            for (int k = 0; k < NS; k++) {
+                vx0[k] = vx1[k] = vf0[k] = vf1[k] = (T)0;
+                if (use[k]) {
+                    const V2 u = *reinterpret_cast<const V2 *>(px[k] + i);
+                    const V2 w = *reinterpret_cast<const V2 *>(pf[k] + i);
+                    vx0[k] = u.x;
+                    vx1[k] = u.y;
+                    vf0[k] = w.x;
+                    vf1[k] = w.y;
+                }
+            }
+            V2 xb, fb;
+            if (dir) {
+                xb = *reinterpret_cast<const V2 *>(xk + i);
+                fb = *reinterpret_cast<const V2 *>(fk + i);
+            } else {
+                xb.x = vx0[0];
+                xb.y = vx1[0];
+                fb.x = vf0[0];
+                fb.y = vf1[0];
+            }
+            V2 o;
+            o.x = point(xb.x, fb.x, vx0, vf0);
+            o.y = point(xb.y, fb.y, vx1, vf1);
+            *reinterpret_cast<V2 *>(xo + i) = o;
+        }
+    } else {
+        for (long i = (long)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) {
+            T vx[NS], vf[NS];
+#pragma unroll
+            for (int k = 0; k < NS; k++) {
+                vx[k] = use[k] ? px[k][i] : (T)0;
+                vf[k] = use[k] ? pf[k][i] : (T)0;
+            }
+            xo[i] = point(xk[i], fk[i], vx, vf);
+        }
     }
 }
 
 template <typename T>
 hipError_t launch_aar_update(const UpdArgs<T> &a, hipStream_t st)
 {
-    if (a.m < 1 || a.m > MAXM) return hipErrorInvalidValue;
-    long nb = (a.n + BLOCK - 1) / BLOCK;
-    if (nb > 4096) nb = 4096;
+    if (a.m < 1 || a.m > MAXM || a.nvalid < 0 || a.nvalid > a.m) return hipErrorInvalidValue;
+    long nb = (a.n / 2 + BLOCK - 1) / BLOCK;
+    nb = nb < 1 ? 1 : (nb > 4096 ? 4096 : nb);
     hipLaunchKernelGGL((aar_update_kernel<T, MAXM>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a);
     return hipGetLastError();
 }
@@ -525,57 +673,93 @@ template hipError_t launch_aar_update<double>(const UpdArgs<double> &, hipStream
 template hipError_t launch_aar_update<float>(const UpdArgs<float> &, hipStream_t);
 
 // ---------------------------------------------------------------------------
-// PGR / PL2R projection update (PGR.c:117-133)
+// PGR / PL2R projection update (PGR.c:117-133): x += sum DX s; Ax += sum DF s
+// in place at the age-0 slot; ||b - Ax||^2 for the convergence check.
 // ---------------------------------------------------------------------------
 template <typename T, int MM>
 __global__ __launch_bounds__(BLOCK) void pgr_update_kernel(const PUpdArgs<T> a)
 {
+    constexpr int NS = MM + 1;
     __shared__ double scratch[BLOCK / 64];
     __shared__ double sums[1];
     if (a.ctrl->done) return;
-    const T *xa[MM], *xb[MM], *ya[MM], *yb[MM];
-    bool ok[MM];
+    const int nv = a.nvalid;
+    const T *px[NS], *py[NS];
+    bool use[NS];
     T sv[MM];
 #pragma unroll
-    for (int j = 0; j < MM; j++) {
-        ok[j] = (j < a.m) && a.valid[j];
-        const int sa = ok[j] ? a.sa[j] : 0, sb = ok[j] ? a.sb[j] : 0;
-        xa[j] = a.X + (size_t)sa * a.ld;
-        xb[j] = a.X + (size_t)sb * a.ld;
-        ya[j] = a.Y + (size_t)sa * a.ld;
-        yb[j] = a.Y + (size_t)sb * a.ld;
-        sv[j] = j < a.m ? (T)a.ctrl->s[j] : (T)0;
+    for (int k = 0; k < NS; k++) {
+        use[k] = k <= nv;
+        const size_t off = (size_t)(use[k] ? a.slot[k] : 0) * a.ld;
+        px[k] = a.X + off;
+        py[k] = a.Y + off;
     }
-    T *xc = a.X + (size_t)a.cur * a.ld;
-    T *yc = a.Y + (size_t)a.cur * a.ld;
+#pragma unroll
+    for (int k = 0; k < MM; k++) sv[k] = k < nv ? (T)a.ctrl->s[ref_col(a.it, k, a.m, 0)] : (T)0;
+    T *xc = a.X + (size_t)a.slot[0] * a.ld;
+    T *yc = a.Y + (size_t)a.slot[0] * a.ld;
     double acc[1] = {0.0};
-    const long chunk = (a.n + a.nblocks - 1) / a.nblocks;
-    const long beg = (long)blockIdx.x * chunk, end = min(a.n, beg + chunk);
-    for (long i = beg + threadIdx.x; i < end; i += BLOCK) {
-        T dx[MM], dy[MM];
+
+    auto point = [&](const T (&vx)[NS], const T (&vy)[NS], T &xv, T &yv) {
+        xv = vx[0];
+        yv = vy[0];
 #pragma unroll
-        for (int j = 0; j < MM; j++) {
-            dx[j] = (T)0;
-            dy[j] = (T)0;
-            if (ok[j]) {
-                dx[j] = xa[j][i] - xb[j][i];
-                dy[j] = ya[j][i] - yb[j][i];
+        for (int k = 0; k < MM; k++) {
+            if (k < nv) {
+                xv = xv + (vx[k] - vx[k + 1]) * sv[k];
+                yv = yv + (vy[k] - vy[k + 1]) * sv[k];
             }
         }
-        T xv = xc[i], yv = yc[i];
+    };
+
+    const long n = a.n;
+    const long stride = (long)gridDim.x * BLOCK;
+    if ((n & 1) == 0) {
+        using V2 = typename Vec2<T>::type;
+        const long np = n >> 1;
+        for (long ip = (long)blockIdx.x * BLOCK + threadIdx.x; ip < np; ip += stride) {
+            const long i = 2 * ip;
+            T vx0[NS], vx1[NS], vy0[NS], vy1[NS];
 #pragma unroll
-        for (int j = 0; j < MM; j++) {
-            if (j < a.m) {
-                xv = xv + dx[j] * sv[j];
-                yv = yv + dy[j] * sv[j];
+            for (int k = 0; k < NS; k++) {
+                vx0[k] = vx1[k] = vy0[k] = vy1[k] = (T)0;
+                if (use[k]) {
+                    const V2 u = *reinterpret_cast<const V2 *>(px[k] + i);
+                    const V2 w = *reinterpret_cast<const V2 *>(py[k] + i);
+                    vx0[k] = u.x;
+                    vx1[k] = u.y;
+                    vy0[k] = w.x;
+                    vy1[k] = w.y;
+                }
             }
+            V2 xo, yo;
+            point(vx0, vy0, xo.x, yo.x);
+            point(vx1, vy1, xo.y, yo.y);
+            *reinterpret_cast<V2 *>(xc + i) = xo;
+            *reinterpret_cast<V2 *>(yc + i) = yo;
+            const V2 bv = *reinterpret_cast<const V2 *>(a.b + i);
+            const T r0 = bv.x - yo.x, r1 = bv.y - yo.y;            // PGR.c:129-130
+            acc[0] += (double)r0 * (double)r0;
+            acc[0] += (double)r1 * (double)r1;
         }
-        xc[i] = xv;
-        yc[i] = yv;
-        const T r = a.b[i] - yv;                 // PGR.c:129-130
-        acc[0] += (double)r * (double)r;
+    } else {
+        for (long i = (long)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) {
+            T vx[NS], vy[NS];
+#pragma unroll
+            for (int k = 0; k < NS; k++) {
+                vx[k] = use[k] ? px[k][i] : (T)0;
+                vy[k] = use[k] ? py[k][i] : (T)0;
+            }
+            T xv, yv;
+            point(vx, vy, xv, yv);
+            xc[i] = xv;
+            yc[i] = yv;
+            const T r = a.b[i] - yv;
+            acc[0] += (double)r * (double)r;
+        }
     }
     block_reduce_to<1>(acc, a.partials + blockIdx.x, scratch);
+    if (!a.counter) return;                     // two-pass: finish_kernel follows
     if (last_block_arrive(a.counter, gridDim.x)) {
         final_sum(a.partials, a.nblocks, 1, sums, scratch);
         if (threadIdx.x == 0) {
@@ -588,7 +772,7 @@ __global__ __launch_bounds__(BLOCK) void pgr_update_kernel(const PUpdArgs<T> a)
 template <typename T>
 hipError_t launch_pgr_update(const PUpdArgs<T> &a, hipStream_t st)
 {
-    if (a.m < 1 || a.m > MAXM) return hipErrorInvalidValue;
+    if (a.m < 1 || a.m > MAXM || a.nvalid < 0 || a.nvalid > a.m) return hipErrorInvalidValue;
     hipLaunchKernelGGL((pgr_update_kernel<T, MAXM>), dim3(a.nblocks), dim3(BLOCK), 0, st, a);
     return hipGetLastError();
 }
@@ -605,11 +789,12 @@ __global__ __launch_bounds__(BLOCK) void dot_kernel(const DotArgs<T> a)
     __shared__ double sums[1];
     if (a.ctrl->done) return;
     double acc[1] = {0.0};
-    const long chunk = (a.n + a.nblocks - 1) / a.nblocks;
-    const long beg = (long)blockIdx.x * chunk, end = min(a.n, beg + chunk);
     const T *w = a.w ? a.w : a.v;
-    for (long i = beg + threadIdx.x; i < end; i += BLOCK) acc[0] += (double)a.v[i] * (double)w[i];
+    const long stride = (long)gridDim.x * BLOCK;
+    for (long i = (long)blockIdx.x * BLOCK + threadIdx.x; i < a.n; i += stride)
+        acc[0] += (double)a.v[i] * (double)w[i];
     block_reduce_to<1>(acc, a.partials + blockIdx.x, scratch);
+    if (!a.counter) return;                     // two-pass: finish_kernel follows
     if (last_block_arrive(a.counter, gridDim.x)) {
         final_sum(a.partials, a.nblocks, 1, sums, scratch);
         if (threadIdx.x == 0) {
@@ -635,9 +820,8 @@ __global__ __launch_bounds__(BLOCK) void cg_update_kernel(const CgUpdArgs a)
     if (a.ctrl->done) return;
     const double alpha = a.ctrl->alpha;
     double acc[2] = {0.0, 0.0};
-    const long chunk = (a.n + a.nblocks - 1) / a.nblocks;
-    const long beg = (long)blockIdx.x * chunk, end = min(a.n, beg + chunk);
-    for (long i = beg + threadIdx.x; i < end; i += BLOCK) {
+    const long stride = (long)gridDim.x * BLOCK;
+    for (long i = (long)blockIdx.x * BLOCK + threadIdx.x; i < a.n; i += stride) {
         a.x[i] = a.x[i] + alpha * a.d[i];            // CG.c:106-107
         if (a.update_r) {
             const double r = a.r[i] - alpha * a.q[i];   // CG.c:121-124
@@ -650,6 +834,7 @@ __global__ __launch_bounds__(BLOCK) void cg_update_kernel(const CgUpdArgs a)
     }
     if (!a.update_r) return;   // restart: a stencil pass recomputes r and reduces
     block_reduce_to<2>(acc, a.partials + (size_t)blockIdx.x * 2, scratch);
+    if (!a.counter) return;                     // two-pass: finish_kernel follows
     if (last_block_arrive(a.counter, gridDim.x)) {
         final_sum(a.partials, a.nblocks, 2, sums, scratch);
         if (threadIdx.x == 0) {
@@ -662,6 +847,29 @@ __global__ __launch_bounds__(BLOCK) void cg_update_kernel(const CgUpdArgs a)
 hipError_t launch_cg_update(const CgUpdArgs &a, hipStream_t st)
 {
     hipLaunchKernelGGL(cg_update_kernel, dim3(a.nblocks), dim3(BLOCK), 0, st, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// second pass of a reduction: one block sums per-block partials in a fixed
+// order and runs the finaliser (no device-scope fence or atomic needed: the
+// kernel boundary orders it after the partials)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BLOCK) void finish_kernel(const double *partials, int nitems, int nacc, FinArgs fin,
+                                                      Ctrl *ctrl)
+{
+    __shared__ double scratch[BLOCK / 64];
+    __shared__ double sums[8];
+    if (ctrl->done) return;
+    final_sum(partials, nitems, nacc, sums, scratch);
+    if (threadIdx.x == 0) finalize(fin, ctrl, sums, nacc);
+}
+
+hipError_t launch_finish(const double *partials, int nitems, int nacc, const FinArgs &fin, Ctrl *ctrl,
+                         hipStream_t st)
+{
+    if (nacc < 1 || nacc > 8) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(BLOCK), 0, st, partials, nitems, nacc, fin, ctrl);
     return hipGetLastError();
 }
 

@@ -66,8 +66,10 @@ int context_init()
 {
     Context &C = ctx();
     if (C.init) return 0;
-    HC(hipStreamCreateWithFlags(&C.own, hipStreamNonBlocking));
-    if (!C.stream) C.stream = C.own;
+    // a blocking stream: ordered with work on the legacy default stream,
+    // which is where framework callers (torch's default stream) put inputs
+    HC(hipStreamCreateWithFlags(&C.own, hipStreamDefault));
+    if (!C.stream_set) C.stream = C.own;
     HC(hipMalloc((void **)&C.ctrl, 2 * sizeof(Ctrl)));
     HC(hipHostMalloc((void **)&C.ctrl_h, 2 * sizeof(Ctrl), hipHostMallocDefault));
     HC(hipHostMalloc((void **)&C.poll_h, NPOLL * sizeof(int), hipHostMallocDefault));
@@ -119,11 +121,12 @@ int make_op(const aar_hip_grid *g, const double *coeff, Op &op)
 // ---------------------------------------------------------------------------
 // launch helpers
 // ---------------------------------------------------------------------------
-static int auto_lz(const Op &op, int py)
+static int auto_lz(const Op &op, int wy, int py)
 {
-    const int ntx = (op.nx + TX - 1) / TX, nty = (op.ny + TYT * py - 1) / (TYT * py);
+    const int ntx = (op.nx + TX - 1) / TX, nty = (op.ny + wy * py - 1) / (wy * py);
     const int cols = ntx * nty;
-    int nzc = std::max(1, 2048 / std::max(1, cols));
+    // long z-marches (about one block per CU) amortise the 2R-plane warm-up
+    int nzc = std::max(1, 256 / std::max(1, cols));
     int lz = (op.nz + nzc - 1) / nzc;
     lz = std::max(lz, std::min(op.nz, 16));
     return lz;
@@ -137,17 +140,28 @@ SArgs<T> base_sargs(const Op &op)
     a.nx = op.nx;
     a.ny = op.ny;
     a.nz = op.nz;
-    a.lz = C.lz > 0 ? C.lz : auto_lz(op, C.py);
+    a.lz = C.lz > 0 ? C.lz : auto_lz(op, C.wy, C.py);
     for (int o = 0; o <= R; o++) a.w[o] = op.w[o];
     a.sigma = op.sigma;
     a.diag = op.diag;
     return a;
 }
 
+// Slot stride of the ring buffers: N rounded up to 256 B plus the padding
+// option "ring_pad" (aar_hip_set_option), so that equal offsets in different slots
+// do not fall on the same HBM channel when N*sizeof(T) is a large power of
+// two (512^3 FP64 slots are exactly 2^30 B apart).
+static long ring_ld(long n, size_t es)
+{
+    const long q = 256 / (long)es;
+    return (n + q - 1) / q * q + ctx().ring_pad;
+}
+
 static int nblocks_for(long n)
 {
+    const long cap = ctx().hist_blocks > 0 ? ctx().hist_blocks : 2048;
     long nb = (n + 4095) / 4096;
-    return (int)std::max(1L, std::min(nb, 2048L));
+    return (int)std::max(1L, std::min(nb, cap));
 }
 
 void prof_begin(const char *name)
@@ -204,27 +218,74 @@ static const char *smode_name(SMode m)
     case SMode::PGR_RICH: return "stencil_pgr_rich";
     case SMode::CG_Q: return "stencil_cg_q";
     case SMode::CG_RESID: return "stencil_cg_resid";
+    case SMode::MATVEC_RES: return "stencil_matvec_res";
     }
     return "stencil";
 }
 
+// Reductions either finish in the last block of the reducing kernel
+// (counter + device-scope fence) or, in two-pass mode, in a one-block
+// second kernel; the choice is a measured tuning option.
 template <typename T>
 int run_stencil(SMode mode, SArgs<T> a, double bytes)
 {
     Context &C = ctx();
+    const bool two = C.two_pass && a.partials;
+    if (two) a.counter = nullptr;
     prof_begin(smode_name(mode));
-    hipError_t e = launch_stencil<T>(mode, a, C.py, C.stream);
+    hipError_t e = launch_stencil<T>(mode, a, C.wy, C.py, C.stream);
     prof_end(bytes);
+    if (two && e == hipSuccess)
+        e = launch_finish(a.partials, stencil_items<T>(a, C.wy, C.py), mode == SMode::CG_RESID ? 2 : 1, a.fin,
+                          a.ctrl, C.stream);
     if (e != hipSuccess) return fail(AAR_HIP_ERR_HIP, std::string("stencil launch: ") + hipGetErrorString(e));
     C.bytes_acc += bytes;
     return 0;
 }
 
 template <typename T>
+static hipError_t run_dot(DotArgs<T> d)
+{
+    Context &C = ctx();
+    const bool two = C.two_pass;
+    if (two) d.counter = nullptr;
+    hipError_t e = launch_dot<T>(d, C.stream);
+    if (two && e == hipSuccess) e = launch_finish(d.partials, d.nblocks, 1, d.fin, d.ctrl, C.stream);
+    return e;
+}
+
+template <typename T>
+static hipError_t run_gram(GramArgs<T> g)
+{
+    if (ctx().two_pass) g.counter = nullptr;
+    return launch_gram<T>(g, ctx().stream);
+}
+
+static hipError_t run_pgr_update(PUpdArgs<double> u)
+{
+    Context &C = ctx();
+    const bool two = C.two_pass;
+    if (two) u.counter = nullptr;
+    hipError_t e = launch_pgr_update<double>(u, C.stream);
+    if (two && e == hipSuccess) e = launch_finish(u.partials, u.nblocks, 1, u.fin, u.ctrl, C.stream);
+    return e;
+}
+
+static hipError_t run_cg_update(CgUpdArgs u)
+{
+    Context &C = ctx();
+    const bool two = C.two_pass && u.update_r;
+    if (two) u.counter = nullptr;
+    hipError_t e = launch_cg_update(u, C.stream);
+    if (two && e == hipSuccess) e = launch_finish(u.partials, u.nblocks, 2, u.fin, u.ctrl, C.stream);
+    return e;
+}
+
+template <typename T>
 int stencil_partials_needed(const Op &op)
 {
     SArgs<T> a = base_sargs<T>(op);
-    return stencil_items<T>(a, ctx().py) * 2;
+    return stencil_items<T>(a, ctx().wy, ctx().py) * 2;
 }
 
 // poll ring: record ctrl->done after a cycle; true once done is observed
@@ -286,7 +347,7 @@ static int store_history(const double *hist_dev, int len)
 // ---------------------------------------------------------------------------
 template <typename T>
 int aar_solve(const Op &op, T *x, const T *b, double omega, double beta, int m, int p, int max_iter,
-              double tol, int precond, int ci, aar_hip_stats *st, int keep_history)
+              double tol, int precond, int ci, aar_hip_stats *st, int keep_history, int ref_start)
 {
     Context &C = ctx();
     if (int rc = context_init()) return rc;
@@ -298,8 +359,9 @@ int aar_solve(const Op &op, T *x, const T *b, double omega, double beta, int m, 
     const int ns = m + 1;
     DevBuf &RX = (sizeof(T) == 8) ? C.ring64x : C.ring32x;
     DevBuf &RF = (sizeof(T) == 8) ? C.ring64f : C.ring32f;
-    HC(ensure(RX, sizeof(T) * n * ns));
-    HC(ensure(RF, sizeof(T) * n * ns));
+    const long ld = ring_ld(n, sizeof(T));
+    HC(ensure(RX, sizeof(T) * ld * ns));
+    HC(ensure(RF, sizeof(T) * ld * ns));
     DevBuf &HB = ci == 0 ? C.hist0 : C.hist1;
     HC(ensure(HB, sizeof(double) * (size_t)(max_iter + 4)));
     const int nb = nblocks_for(n);
@@ -317,7 +379,7 @@ int aar_solve(const Op &op, T *x, const T *b, double omega, double beta, int m, 
     HC(launch_zero_ctrl(ctrl, C.stream));
     HC(hipEventRecord(C.t0, C.stream));
     // x_1 = x0 into its slot
-    HC(hipMemcpyAsync(X + (size_t)slot(1) * n, x, sizeof(T) * n, hipMemcpyDeviceToDevice, C.stream));
+    HC(hipMemcpyAsync(X + (size_t)slot(1) * ld, x, sizeof(T) * n, hipMemcpyDeviceToDevice, C.stream));
     C.bytes_acc += 2 * s_b;
     // ||b|| and the scaled tolerance (AAR.c:78-79)
     {
@@ -330,9 +392,9 @@ int aar_solve(const Op &op, T *x, const T *b, double omega, double beta, int m, 
         d.ctrl = ctrl;
         d.fin.fin = FIN_NORM_B;
         d.fin.tol = tol;
-        d.fin.check = 1;
+        d.fin.check = ref_start;      // the reference's zero-iteration rule (AAR.c:76-79)
         prof_begin("dot");
-        HC(launch_dot<T>(d, C.stream));
+        HC(run_dot<T>(d));
         prof_end(s_b);
     }
     C.bytes_acc += s_b;
@@ -347,10 +409,10 @@ int aar_solve(const Op &op, T *x, const T *b, double omega, double beta, int m, 
         SArgs<T> a = base_sargs<T>(op);
         a.omega = omega;
         a.precond = precond;
-        a.in0 = X + (size_t)slot(it) * n;
+        a.in0 = X + (size_t)slot(it) * ld;
         a.bvec = b;
-        a.out0 = F + (size_t)slot(it) * n;
-        a.out1 = X + (size_t)slot(it + 1) * n;
+        a.out0 = F + (size_t)slot(it) * ld;
+        a.out1 = X + (size_t)slot(it + 1) * ld;
         a.write_x = anderson ? 0 : 1;
         a.partials = part;
         a.counter = cnt + 1;
@@ -362,54 +424,40 @@ int aar_solve(const Op &op, T *x, const T *b, double omega, double beta, int m, 
         a.fin.hist_cap = max_iter + 4;
         if (int rc = run_stencil<T>(SMode::AAR_RICH, a, s_b * (anderson ? 3 : 4))) return rc;
         if (anderson) {
+            // history by age: slot of iterate it-k, k = 0..m; difference
+            // columns of age < nv exist (written from iteration 2 on, AAR.c:96)
             GramArgs<T> g;
             UpdArgs<T> u;
-            int nslots_read = 0;
-            bool seen[MAXM + 1] = {false};
-            seen[slot(it)] = true;
-            nslots_read = 1;
-            for (int j = 0; j < m; j++) {
-                const int off = (int)(((it - 2 - j) % m + m) % m);
-                const long ij = it - off;                            // AAR.c:97: col = (iter-2) % m
-                const int ok = ij >= 2;
-                g.valid[j] = u.valid[j] = ok;
-                g.sa[j] = u.sa[j] = ok ? slot(ij) : 0;
-                g.sb[j] = u.sb[j] = ok ? slot(ij - 1) : 0;
-                if (ok) {
-                    if (!seen[slot(ij)]) { seen[slot(ij)] = true; nslots_read++; }
-                    if (!seen[slot(ij - 1)]) { seen[slot(ij - 1)] = true; nslots_read++; }
-                }
-            }
-            g.n = n;
-            g.m = m;
+            const int nv = (int)std::min<long>(m, it - 1);
+            for (int k = 0; k <= m; k++) g.slot[k] = u.slot[k] = it - k >= 1 ? slot(it - k) : 0;
+            g.n = u.n = n;
+            g.m = u.m = m;
+            g.nvalid = u.nvalid = nv;
+            g.it = u.it = it;
             g.kind = 0;
             g.F = F;
-            g.ld = n;
+            g.ld = ld;
             g.fslot = slot(it);
             g.nblocks = nb;
             g.partials = part;
             g.counter = cnt + 2;
             g.ctrl = ctrl;
             prof_begin("gram");
-            HC(launch_gram<T>(g, C.stream));
-            prof_end(s_b * nslots_read);
+            HC(run_gram<T>(g));
+            prof_end(s_b * (nv + 1));
             prof_begin("ls");
             HC(launch_ls(ctrl, m, 0, C.stream));
             prof_end(0);
-            u.n = n;
-            u.m = m;
             u.beta = beta;
             u.X = X;
             u.F = F;
-            u.ld = n;
-            u.xslot = slot(it);
-            u.fslot = slot(it);
+            u.ld = ld;
             u.oslot = slot(it + 1);
             u.ctrl = ctrl;
             prof_begin("aar_update");
             HC(launch_aar_update<T>(u, C.stream));
-            prof_end(s_b * (2 * nslots_read + 1));
-            C.bytes_acc += s_b * (3 * nslots_read + 1);
+            prof_end(s_b * (2 * (nv + 1) + 1));
+            C.bytes_acc += s_b * (3 * (nv + 1) + 1);
             if (int rc = poll_record(ctrl, cyc)) return rc;
             if (poll_check(cyc - 2)) stopped = true;
             cyc++;
@@ -425,9 +473,9 @@ int aar_solve(const Op &op, T *x, const T *b, double omega, double beta, int m, 
         SArgs<T> a = base_sargs<T>(op);
         a.omega = omega;
         a.precond = precond;
-        a.in0 = X + (size_t)slot(last) * n;
+        a.in0 = X + (size_t)slot(last) * ld;
         a.bvec = b;
-        a.out0 = F + (size_t)slot(last) * n;
+        a.out0 = F + (size_t)slot(last) * ld;
         a.write_x = 0;
         a.partials = part;
         a.counter = cnt + 1;
@@ -454,7 +502,7 @@ int aar_solve(const Op &op, T *x, const T *b, double omega, double beta, int m, 
         loop_iter = max_iter + 1;
     }
     // solution: iterate following the last iteration
-    HC(hipMemcpyAsync(x, X + (size_t)slot(iterations + 1) * n, sizeof(T) * n, hipMemcpyDeviceToDevice,
+    HC(hipMemcpyAsync(x, X + (size_t)slot(iterations + 1) * ld, sizeof(T) * n, hipMemcpyDeviceToDevice,
                       C.stream));
     HC(hipStreamSynchronize(C.stream));
     C.bytes_acc += 2 * s_b;
@@ -468,9 +516,9 @@ int aar_solve(const Op &op, T *x, const T *b, double omega, double beta, int m, 
 }
 
 template int aar_solve<double>(const Op &, double *, const double *, double, double, int, int, int, double,
-                               int, int, aar_hip_stats *, int);
+                               int, int, aar_hip_stats *, int, int);
 template int aar_solve<float>(const Op &, float *, const float *, double, double, int, int, int, double,
-                              int, int, aar_hip_stats *, int);
+                              int, int, aar_hip_stats *, int, int);
 
 // ---------------------------------------------------------------------------
 // PGR / PL2R (PGR.c:35-218, PL2R.c:35-218), FP64
@@ -486,8 +534,9 @@ int pgr_solve(const Op &op, double *x, const double *b, double omega, int m, int
     if (max_iter < 0) return fail(AAR_HIP_ERR_ARG, "max_iter must be >= 0");
     const long n = op.n;
     const int ns = m + 1;
-    HC(ensure(C.ring64x, sizeof(double) * n * ns));
-    HC(ensure(C.ring64f, sizeof(double) * n * ns));
+    const long ld = ring_ld(n, sizeof(double));
+    HC(ensure(C.ring64x, sizeof(double) * ld * ns));
+    HC(ensure(C.ring64f, sizeof(double) * ld * ns));
     HC(ensure(C.hist0, sizeof(double) * (size_t)(max_iter + 4)));
     const int nb = nblocks_for(n);
     const size_t need = std::max<size_t>((size_t)stencil_partials_needed<double>(op), (size_t)nb * 80);
@@ -502,7 +551,7 @@ int pgr_solve(const Op &op, double *x, const double *b, double omega, int m, int
 
     HC(launch_zero_ctrl(ctrl, C.stream));
     HC(hipEventRecord(C.t0, C.stream));
-    HC(hipMemcpyAsync(X + (size_t)slot(1) * n, x, sizeof(double) * n, hipMemcpyDeviceToDevice, C.stream));
+    HC(hipMemcpyAsync(X + (size_t)slot(1) * ld, x, sizeof(double) * n, hipMemcpyDeviceToDevice, C.stream));
     {
         DotArgs<double> d;
         d.n = n;
@@ -514,14 +563,14 @@ int pgr_solve(const Op &op, double *x, const double *b, double omega, int m, int
         d.fin.fin = FIN_NORM_B;
         d.fin.tol = tol;
         d.fin.check = 1;
-        HC(launch_dot<double>(d, C.stream));
+        HC(run_dot<double>(d));
     }
     // Ax_1 and ||b - A x_1|| (PGR.c:79-81)
     {
         SArgs<double> a = base_sargs<double>(op);
-        a.in0 = X + (size_t)slot(1) * n;
+        a.in0 = X + (size_t)slot(1) * ld;
         a.bvec = b;
-        a.out0 = Y + (size_t)slot(1) * n;
+        a.out0 = Y + (size_t)slot(1) * ld;
         a.partials = part;
         a.counter = cnt + 1;
         a.ctrl = ctrl;
@@ -530,7 +579,7 @@ int pgr_solve(const Op &op, double *x, const double *b, double omega, int m, int
         a.fin.check = 0;
         a.fin.hist = hist;
         a.fin.hist_cap = max_iter + 4;
-        if (int rc = run_stencil<double>(SMode::MATVEC, a, s_b * 3)) return rc;
+        if (int rc = run_stencil<double>(SMode::MATVEC_RES, a, s_b * 3)) return rc;
     }
     int cyc = 0;
     int t = 1;
@@ -539,12 +588,12 @@ int pgr_solve(const Op &op, double *x, const double *b, double omega, int m, int
         SArgs<double> a = base_sargs<double>(op);
         a.omega = omega;
         a.precond = precond;
-        a.in0 = X + (size_t)slot(t) * n;
+        a.in0 = X + (size_t)slot(t) * ld;
         a.in1 = b;
-        a.in2 = Y + (size_t)slot(t) * n;
+        a.in2 = Y + (size_t)slot(t) * ld;
         a.bvec = b;
-        a.out0 = X + (size_t)slot(t + 1) * n;
-        a.out1 = Y + (size_t)slot(t + 1) * n;
+        a.out0 = X + (size_t)slot(t + 1) * ld;
+        a.out1 = Y + (size_t)slot(t + 1) * ld;
         a.trace = 1;
         a.partials = part;
         a.counter = cnt + 1;
@@ -556,49 +605,36 @@ int pgr_solve(const Op &op, double *x, const double *b, double omega, int m, int
         a.fin.hist_cap = max_iter + 4;
         if (int rc = run_stencil<double>(SMode::PGR_RICH, a, s_b * 5)) return rc;
         if (t % p == 0) {                                           // PGR.c:111
+            // history by age relative to the newest iterate x_{t+1}: the
+            // column of age k belongs to iteration t-k (PGR.c:105-109)
             GramArgs<double> g;
             PUpdArgs<double> u;
-            int nsl = 1;
-            bool seen[MAXM + 1] = {false};
-            seen[slot(t + 1)] = true;
-            for (int j = 0; j < m; j++) {
-                // column j holds iteration tj with (tj-2+m) % m == j (PGR.c:105)
-                const int off = (int)(((t - j - 2) % m + m) % m);
-                const long tj = t - off;
-                const int ok = tj >= 1;
-                g.valid[j] = u.valid[j] = ok;
-                g.sa[j] = u.sa[j] = ok ? slot(tj + 1) : 0;
-                g.sb[j] = u.sb[j] = ok ? slot(tj) : 0;
-                if (ok) {
-                    if (!seen[slot(tj + 1)]) { seen[slot(tj + 1)] = true; nsl++; }
-                    if (!seen[slot(tj)]) { seen[slot(tj)] = true; nsl++; }
-                }
-            }
-            g.n = n;
-            g.m = m;
+            const int nv = std::min(m, t);
+            for (int k = 0; k <= m; k++) g.slot[k] = u.slot[k] = t + 1 - k >= 1 ? slot(t + 1 - k) : 0;
+            g.n = u.n = n;
+            g.m = u.m = m;
+            g.nvalid = u.nvalid = nv;
+            g.it = u.it = t;
             g.kind = l2 ? 2 : 1;
             g.F = X;
             g.Y = Y;
             g.b = b;
-            g.ld = n;
+            g.ld = ld;
             g.fslot = slot(t + 1);
             g.nblocks = nb;
             g.partials = part;
             g.counter = cnt + 2;
             g.ctrl = ctrl;
             prof_begin("gram");
-            HC(launch_gram<double>(g, C.stream));
-            prof_end(s_b * ((l2 ? 1 : 2) * nsl + 1));
+            HC(run_gram<double>(g));
+            prof_end(s_b * ((l2 ? 1 : 2) * (nv + 1) + 1));
             prof_begin("ls");
             HC(launch_ls(ctrl, m, 0, C.stream));
             prof_end(0);
-            u.n = n;
-            u.m = m;
             u.X = X;
             u.Y = Y;
             u.b = b;
-            u.ld = n;
-            u.cur = slot(t + 1);
+            u.ld = ld;
             u.nblocks = nb;
             u.partials = part;
             u.counter = cnt + 3;
@@ -608,9 +644,9 @@ int pgr_solve(const Op &op, double *x, const double *b, double omega, int m, int
             u.fin.check = 1;
             u.fin.hist = nullptr;
             prof_begin("pgr_update");
-            HC(launch_pgr_update<double>(u, C.stream));
-            prof_end(s_b * (2 * nsl + 1 + 2));
-            C.bytes_acc += s_b * ((l2 ? 1 : 2) * nsl + 1) + s_b * (2 * nsl + 3);
+            HC(run_pgr_update(u));
+            prof_end(s_b * (2 * (nv + 1) + 1 + 2));
+            C.bytes_acc += s_b * ((l2 ? 1 : 2) * (nv + 1) + 1) + s_b * (2 * (nv + 1) + 3);
             if (int rc = poll_record(ctrl, cyc)) return rc;
             if (poll_check(cyc - 2)) stopped = true;
             cyc++;
@@ -630,7 +666,7 @@ int pgr_solve(const Op &op, double *x, const double *b, double omega, int m, int
         iterations = max_iter;
         loop_iter = max_iter + 1;
     }
-    HC(hipMemcpyAsync(x, X + (size_t)slot(iterations + 1) * n, sizeof(double) * n, hipMemcpyDeviceToDevice,
+    HC(hipMemcpyAsync(x, X + (size_t)slot(iterations + 1) * ld, sizeof(double) * n, hipMemcpyDeviceToDevice,
                       C.stream));
     HC(hipStreamSynchronize(C.stream));
     const int hl = std::min(iterations + 1, max_iter + 4);
@@ -652,13 +688,14 @@ int cg_solve(const Op &op, double *x, const double *b, int max_iter, double tol,
     if (!x || !b) return fail(AAR_HIP_ERR_ARG, "null vector");
     if (max_iter < 0) return fail(AAR_HIP_ERR_ARG, "max_iter must be >= 0");
     const long n = op.n;
-    // r, d[2], q
-    HC(ensure(C.ring64x, sizeof(double) * n * 4));
+    // r, d[2], q (padded slots, as the rings)
+    const long ld = ring_ld(n, sizeof(double));
+    HC(ensure(C.ring64x, sizeof(double) * ld * 4));
     HC(ensure(C.hist0, sizeof(double) * (size_t)(max_iter + 4)));
     const int nb = nblocks_for(n);
     const size_t need = std::max<size_t>((size_t)stencil_partials_needed<double>(op), (size_t)nb * 80);
     HC(ensure(C.part0, sizeof(double) * need));
-    double *r = (double *)C.ring64x.p, *dd[2] = {r + n, r + 2 * n}, *q = r + 3 * n;
+    double *r = (double *)C.ring64x.p, *dd[2] = {r + ld, r + 2 * ld}, *q = r + 3 * ld;
     double *hist = (double *)C.hist0.p, *part = (double *)C.part0.p;
     Ctrl *ctrl = C.ctrl;
     unsigned *cnt = C.counters;
@@ -677,7 +714,7 @@ int cg_solve(const Op &op, double *x, const double *b, int max_iter, double tol,
         d.ctrl = ctrl;
         d.fin.fin = FIN_NORM_B;
         d.fin.tol = tol;
-        HC(launch_dot<double>(d, C.stream));
+        HC(run_dot<double>(d));
     }
     // r = b - A x; d = r / diag; delta = r.d; relres = ||r|| (CG.c:65-84)
     {
@@ -736,7 +773,7 @@ int cg_solve(const Op &op, double *x, const double *b, int max_iter, double tol,
             u.fin.hist = hist;
             u.fin.hist_cap = max_iter + 4;
             prof_begin("cg_update");
-            HC(launch_cg_update(u, C.stream));
+            HC(run_cg_update(u));
             prof_end(s_b * (restart ? 3 : 6));
             C.bytes_acc += s_b * (restart ? 3 : 6);
         }
@@ -838,7 +875,7 @@ int aar_refined(const Op &op, double *x, const double *b, double omega, double b
         d.ctrl = ctrl;
         d.fin.fin = FIN_NORM_B;
         d.fin.tol = tol;
-        HC(launch_dot<double>(d, C.stream));
+        HC(run_dot<double>(d));
         bytes += s_b;
     }
     double rhs_norm = 0.0, rel = 0.0;
@@ -865,7 +902,7 @@ int aar_refined(const Op &op, double *x, const double *b, double omega, double b
         const double tin = std::max(inner_tol, 0.5 * tol / rel);
         HC(launch_fill_f32(e32, 0.0f, n, C.stream));
         aar_hip_stats ist;
-        int rc = aar_solve<float>(op32, e32, r32, omega, beta, m, p, max_inner, tin, precond, 1, &ist, 0);
+        int rc = aar_solve<float>(op32, e32, r32, omega, beta, m, p, max_inner, tin, precond, 1, &ist, 0, 0);
         if (rc) return rc;
         inner_total += ist.iterations;
         bytes += ist.bytes + 4.0 * n;

@@ -44,7 +44,8 @@ template <> struct Arith<float> {
 };
 
 template <SMode M> struct ModeInfo;
-template <> struct ModeInfo<SMode::MATVEC> { static constexpr int NF = 1, NE = 1, NACC = 1; };
+template <> struct ModeInfo<SMode::MATVEC> { static constexpr int NF = 1, NE = 0, NACC = 1; };
+template <> struct ModeInfo<SMode::MATVEC_RES> { static constexpr int NF = 1, NE = 1, NACC = 1; };
 template <> struct ModeInfo<SMode::AAR_RICH> { static constexpr int NF = 1, NE = 1, NACC = 1; };
 template <> struct ModeInfo<SMode::RESID> { static constexpr int NF = 1, NE = 1, NACC = 1; };
 template <> struct ModeInfo<SMode::PGR_RICH> { static constexpr int NF = 3, NE = 1, NACC = 1; };
@@ -53,21 +54,23 @@ template <> struct ModeInfo<SMode::CG_RESID> { static constexpr int NF = 1, NE =
 
 __device__ inline int modp(int i, int n) { i %= n; return i < 0 ? i + n : i; }
 
-template <typename T, int PY, SMode MODE>
-__global__ __launch_bounds__(BLOCK) void stencil_kernel(const SArgs<T> a)
+template <typename T, int WY, int PY, SMode MODE>
+__global__ __launch_bounds__(64 * WY) void stencil_kernel(const SArgs<T> a)
 {
     using A = Arith<T>;
+    constexpr int NT = 64 * WY;                       // threads: WY wavefronts stacked in y
     constexpr int NF = ModeInfo<MODE>::NF;
     constexpr int NE = ModeInfo<MODE>::NE;
     constexpr int NACC = ModeInfo<MODE>::NACC;
-    constexpr int TY = TYT * PY;
+    constexpr int TY = WY * PY;
     constexpr int LH = TY + 2 * R;
     constexpr int LW = TX + 2 * R;
-    constexpr int NXH = (2 * R * TY + BLOCK - 1) / BLOCK;
+    constexpr int NYH = (2 * R + WY - 1) / WY;        // y-halo rows per thread row
+    constexpr int NXH = (2 * R * TY + NT - 1) / NT;   // x-halo cells per thread
     constexpr int QN = 2 * R + 1;
 
     __shared__ T lds[2][LH][LW];
-    __shared__ double red_scratch[8 * (BLOCK / 64)];
+    __shared__ double red_scratch[8 * WY];
     __shared__ double red_out[8];
 
     if (a.ctrl && a.ctrl->done) return;
@@ -93,10 +96,13 @@ __global__ __launch_bounds__(BLOCK) void stencil_kernel(const SArgs<T> a)
         c_off[p] = cx + modp(y, ny) * nx;
         c_ok[p] = (x0 + tx < nx) && (y < ny);
     }
-    int yh_off[3], yh_row[3];
+    int yh_off[NYH], yh_row[NYH];
+    bool yh_ok[NYH];
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-        const int hr = ty + TYT * k;                       // 0 .. 2R-1
+    for (int k = 0; k < NYH; k++) {
+        const int hr0 = ty + WY * k;                       // 0 .. 2R-1 when valid
+        yh_ok[k] = hr0 < 2 * R;
+        const int hr = yh_ok[k] ? hr0 : 0;
         const int gy = hr < R ? y0 - R + hr : y0 + TY + (hr - R);
         yh_off[k] = cx + modp(gy, ny) * nx;
         yh_row[k] = hr < R ? hr : TY + hr;
@@ -105,11 +111,13 @@ __global__ __launch_bounds__(BLOCK) void stencil_kernel(const SArgs<T> a)
     bool xh_ok[NXH];
 #pragma unroll
     for (int k = 0; k < NXH; k++) {
-        const int c = tid + BLOCK * k;
+        const int c = tid + NT * k;
         xh_ok[k] = c < 2 * R * TY;
         const int r = xh_ok[k] ? c / (2 * R) : 0, cc = xh_ok[k] ? c % (2 * R) : 0;
         const int gx = cc < R ? x0 - R + cc : x0 + TX + (cc - R);
-        xh_off[k] = modp(gx, nx) + modp(y0 + r, ny) * nx;
+        // cells past the halo load a valid (unused) address so that every
+        // load is unconditional; only their LDS write is predicated
+        xh_off[k] = xh_ok[k] ? modp(gx, nx) + modp(y0 + r, ny) * nx : c_off[0];
         xh_row[k] = R + r;
         xh_col[k] = cc < R ? cc : TX + cc;
     }
@@ -145,12 +153,19 @@ __global__ __launch_bounds__(BLOCK) void stencil_kernel(const SArgs<T> a)
     };
 
     T q[PY][QN];
-    T qn[PY][NF];
-    T hy[3][NF], hx[NXH][NF];
-    T en[PY][NE > 0 ? NE : 1] = {};
+    // prefetch register set: halo + epilogue inputs of one plane and the
+    // queue plane R ahead of it.  Two sets alternate (the plane loop is
+    // unrolled by two) so that a set stays in flight during the barrier and
+    // compute of the previous plane with no register copies of pending loads.
+    struct Pre {
+        T qn[PY][NF];
+        T hy[NYH][NF];
+        T hx[NXH][NF];
+        T en[PY][NE > 0 ? NE : 1];
+    };
+    Pre s0, s1;
 
-    // prologue: queue planes z0-R .. z0+R-1, then prefetch plane z0+R and
-    // the halo / epilogue inputs of plane z0
+    // prologue: queue planes z0-R .. z0+R-1
 #pragma unroll
     for (int s = 0; s < 2 * R; s++) {
         const int zo = modp(z0 - R + s, nz) * nxy;
@@ -161,133 +176,143 @@ __global__ __launch_bounds__(BLOCK) void stencil_kernel(const SArgs<T> a)
             q[p][s] = make_u(v);
         }
     }
-    {
-        const int zq = modp(z0 + R, nz) * nxy, zc = modp(z0, nz) * nxy;
+    // halo / epilogue inputs of plane zp and queue plane zp+R (indices wrap,
+    // so the prefetch past the last plane is a harmless unused load)
+    auto prefetch = [&](int zp, Pre &st) {
+        const int zq = modp(zp + R, nz) * nxy, zc = modp(zp, nz) * nxy;
 #pragma unroll
-        for (int p = 0; p < PY; p++) load_raw(c_off[p] + zq, qn[p]);
+        for (int p = 0; p < PY; p++) load_raw(c_off[p] + zq, st.qn[p]);
 #pragma unroll
-        for (int k = 0; k < 3; k++) load_raw(yh_off[k] + zc, hy[k]);
+        for (int k = 0; k < NYH; k++) load_raw(yh_off[k] + zc, st.hy[k]);
 #pragma unroll
-        for (int k = 0; k < NXH; k++)
-            if (xh_ok[k]) load_raw(xh_off[k] + zc, hx[k]);
+        for (int k = 0; k < NXH; k++) load_raw(xh_off[k] + zc, st.hx[k]);
         if constexpr (NE > 0) {
-            if (a.bvec) {
 #pragma unroll
-                for (int p = 0; p < PY; p++) en[p][0] = a.bvec[c_off[p] + zc];
-            }
+            for (int p = 0; p < PY; p++) st.en[p][0] = a.bvec[c_off[p] + zc];
         }
-    }
+    };
 
     double acc[NACC];
 #pragma unroll
     for (int k = 0; k < NACC; k++) acc[k] = 0.0;
 
-    for (int z = z0; z < z1; z++) {
+    auto plane = [&](const int z, Pre &cur, Pre &nxt) {
         const int buf = (z - z0) & 1;
         // queue top <- plane z+R; LDS <- plane z (centre + halo)
 #pragma unroll
         for (int p = 0; p < PY; p++) {
-            q[p][2 * R] = make_u(qn[p]);
+            q[p][2 * R] = make_u(cur.qn[p]);
             lds[buf][R + ty * PY + p][R + tx] = q[p][R];
         }
 #pragma unroll
-        for (int k = 0; k < 3; k++) lds[buf][yh_row[k]][R + tx] = make_u(hy[k]);
+        for (int k = 0; k < NYH; k++)
+            if (yh_ok[k]) lds[buf][yh_row[k]][R + tx] = make_u(cur.hy[k]);
 #pragma unroll
         for (int k = 0; k < NXH; k++)
-            if (xh_ok[k]) lds[buf][xh_row[k]][xh_col[k]] = make_u(hx[k]);
-        T ev[PY][NE > 0 ? NE : 1];
-        if constexpr (NE > 0) {
-#pragma unroll
-            for (int p = 0; p < PY; p++) ev[p][0] = en[p][0];
-        }
-        // prefetch plane z+1 (halo, epilogue inputs) and z+R+1 (queue)
-        if (z + 1 < z1) {
-            const int zq = modp(z + 1 + R, nz) * nxy, zc = modp(z + 1, nz) * nxy;
-#pragma unroll
-            for (int p = 0; p < PY; p++) load_raw(c_off[p] + zq, qn[p]);
-#pragma unroll
-            for (int k = 0; k < 3; k++) load_raw(yh_off[k] + zc, hy[k]);
-#pragma unroll
-            for (int k = 0; k < NXH; k++)
-                if (xh_ok[k]) load_raw(xh_off[k] + zc, hx[k]);
-            if constexpr (NE > 0) {
-                if (a.bvec) {
-#pragma unroll
-                    for (int p = 0; p < PY; p++) en[p][0] = a.bvec[c_off[p] + zc];
-                }
-            }
-        }
+            if (xh_ok[k]) lds[buf][xh_row[k]][xh_col[k]] = make_u(cur.hx[k]);
+        prefetch(z + 1, nxt);
         __syncthreads();
 
-        const int zoff = z * nxy;
+            const int zoff = z * nxy;
 #pragma unroll
-        for (int p = 0; p < PY; p++) {
-            const int lr = R + ty * PY + p, lc = R + tx;
-            const T u = q[p][R];
-            // system.c:195-201 order: centre, then o = 1..6 with
-            // (x-, x+, y-, y+, z-, z+) summed left to right
-            T y = A::mul(A::mul(u, (T)3), wt0);
+            for (int p = 0; p < PY; p++) {
+                const int lr = R + ty * PY + p, lc = R + tx;
+                const T u = q[p][R];
+                // system.c:195-201 order: centre, then o = 1..6 with
+                // (x-, x+, y-, y+, z-, z+) summed left to right
+                T y;
+                if constexpr (sizeof(T) == 8) {
+                    y = A::mul(A::mul(u, (T)3), wt0);
 #pragma unroll
-            for (int o = 1; o <= R; o++) {
-                T nb = A::add(lds[buf][lr][lc - o], lds[buf][lr][lc + o]);
-                nb = A::add(nb, lds[buf][lr - o][lc]);
-                nb = A::add(nb, lds[buf][lr + o][lc]);
-                nb = A::add(nb, q[p][R - o]);
-                nb = A::add(nb, q[p][R + o]);
-                y = A::add(y, A::mul(nb, wt[o]));
+                    for (int o = 1; o <= R; o++) {
+                        T nb = A::add(lds[buf][lr][lc - o], lds[buf][lr][lc + o]);
+                        nb = A::add(nb, lds[buf][lr - o][lc]);
+                        nb = A::add(nb, lds[buf][lr + o][lc]);
+                        nb = A::add(nb, q[p][R - o]);
+                        nb = A::add(nb, q[p][R + o]);
+                        y = A::add(y, A::mul(nb, wt[o]));
+                    }
+                } else {
+                    // FP32 (no reference counterpart): difference form
+                    // sum_o w_o * sum_nb (x_nb - x_c), i.e. the centre weight taken
+                    // as -2 sum_o w_o.  Row sums are exactly zero and each rounding
+                    // is relative to a local difference, so the smooth modes, whose
+                    // eigenvalues are tiny, keep their accuracy; the reference form
+                    // would shift them by ~eps32 * |w_0| and stall FP32 iterations.
+                    y = (T)0;
+#pragma unroll
+                    for (int o = 1; o <= R; o++) {
+                        T nb = A::add(A::sub(lds[buf][lr][lc - o], u), A::sub(lds[buf][lr][lc + o], u));
+                        nb = A::add(nb, A::sub(lds[buf][lr - o][lc], u));
+                        nb = A::add(nb, A::sub(lds[buf][lr + o][lc], u));
+                        nb = A::add(nb, A::sub(q[p][R - o], u));
+                        nb = A::add(nb, A::sub(q[p][R + o], u));
+                        y = A::add(y, A::mul(nb, wt[o]));
+                    }
+                }
+                if (a.sigma != 0.0) y = A::add(y, A::mul(sig, u));
+                if (!c_ok[p]) continue;
+                const int g = c_off[p] + zoff;
+                if constexpr (MODE == SMode::MATVEC) {
+                    a.out0[g] = y;
+                } else if constexpr (MODE == SMode::MATVEC_RES) {
+                    // y = A x and ||b - A x||^2 (PGR.c:79-81 start-up)
+                    a.out0[g] = y;
+                    acc[0] += A::sq(A::sub(cur.en[p][0], y));
+                } else if constexpr (MODE == SMode::AAR_RICH) {
+                    // AAR.c:80-82/93-94/134-139: f = (rhs - A x)/diag; x_new = x + omega*f
+                    const T r = A::sub(cur.en[p][0], y);
+                    acc[0] += A::sq(r);
+                    T f = r;
+                    if (a.precond) f = A::div(f, dg);
+                    a.out0[g] = f;
+                    if (a.write_x) a.out1[g] = A::add(u, A::mul(om, f));
+                } else if constexpr (MODE == SMode::RESID) {
+                    const T r = A::sub(cur.en[p][0], y);
+                    acc[0] += A::sq(r);
+                    if (a.out32) a.out32[g] = (float)r;
+                    if (a.out0) a.out0[g] = r;
+                } else if constexpr (MODE == SMode::PGR_RICH) {
+                    // PGR.c:95-103: x_old=x; x += omega*f; Ax = A x; f = rhs - Ax
+                    a.out0[g] = u;
+                    a.out1[g] = y;
+                    if (a.trace) acc[0] += A::sq(A::sub(cur.en[p][0], y));
+                } else if constexpr (MODE == SMode::CG_Q) {
+                    // CG.c:94-100: q = A d; alpha partial d.q
+                    a.out0[g] = u;
+                    a.out1[g] = y;
+                    acc[0] += (double)u * (double)y;
+                } else if constexpr (MODE == SMode::CG_RESID) {
+                    // CG.c:65-84 / 112-119: r = b - A x; s = r/diag; r.s, r.r
+                    const T r = A::sub(cur.en[p][0], y);
+                    a.out0[g] = r;
+                    T s = r;
+                    if (a.precond) s = A::div(s, dg);
+                    if (a.out1) a.out1[g] = s;
+                    acc[0] += (double)r * (double)s;
+                    acc[1] += A::sq(r);
+                }
             }
-            if (a.sigma != 0.0) y = A::add(y, A::mul(sig, u));
-            if (!c_ok[p]) continue;
-            const int g = c_off[p] + zoff;
-            if constexpr (MODE == SMode::MATVEC) {
-                // optional residual norm of b - A x (PGR.c:79-81 start-up)
-                if (a.out0) a.out0[g] = y;
-                if (a.bvec) acc[0] += A::sq(A::sub(ev[p][0], y));
-            } else if constexpr (MODE == SMode::AAR_RICH) {
-                // AAR.c:80-82/93-94/134-139: f = (rhs - A x)/diag; x_new = x + omega*f
-                const T r = A::sub(ev[p][0], y);
-                acc[0] += A::sq(r);
-                T f = r;
-                if (a.precond) f = A::div(f, dg);
-                a.out0[g] = f;
-                if (a.write_x) a.out1[g] = A::add(u, A::mul(om, f));
-            } else if constexpr (MODE == SMode::RESID) {
-                const T r = A::sub(ev[p][0], y);
-                acc[0] += A::sq(r);
-                if (a.out32) a.out32[g] = (float)r;
-                if (a.out0) a.out0[g] = r;
-            } else if constexpr (MODE == SMode::PGR_RICH) {
-                // PGR.c:95-103: x_old=x; x += omega*f; Ax = A x; f = rhs - Ax
-                a.out0[g] = u;
-                a.out1[g] = y;
-                if (a.trace) acc[0] += A::sq(A::sub(ev[p][0], y));
-            } else if constexpr (MODE == SMode::CG_Q) {
-                // CG.c:94-100: q = A d; alpha partial d.q
-                a.out0[g] = u;
-                a.out1[g] = y;
-                acc[0] += (double)u * (double)y;
-            } else if constexpr (MODE == SMode::CG_RESID) {
-                // CG.c:65-84 / 112-119: r = b - A x; s = r/diag; r.s, r.r
-                const T r = A::sub(ev[p][0], y);
-                a.out0[g] = r;
-                T s = r;
-                if (a.precond) s = A::div(s, dg);
-                if (a.out1) a.out1[g] = s;
-                acc[0] += (double)r * (double)s;
-                acc[1] += A::sq(r);
-            }
-        }
         // shift the queue one plane
 #pragma unroll
         for (int p = 0; p < PY; p++)
 #pragma unroll
             for (int s = 0; s < 2 * R; s++) q[p][s] = q[p][s + 1];
+    };
+
+    prefetch(z0, s0);
+    int z = z0;
+    for (; z + 1 < z1; z += 2) {
+        plane(z, s0, s1);
+        plane(z + 1, s1, s0);
     }
+    if (z < z1) plane(z, s0, s1);
 
     if (a.partials) {
-        block_reduce_to<NACC>(acc, a.partials + (size_t)item * NACC, red_scratch);
+        block_reduce_to<NACC, NT>(acc, a.partials + (size_t)item * NACC, red_scratch);
+        if (!a.counter) return;                 // two-pass: finish_kernel follows
         if (last_block_arrive(a.counter, gridDim.x)) {
-            final_sum(a.partials, a.nitems, NACC, red_out, red_scratch);
+            final_sum<NT>(a.partials, a.nitems, NACC, red_out, red_scratch);
             if (tid == 0) {
                 finalize(a.fin, a.ctrl, red_out, NACC);
                 *a.counter = 0u;
@@ -301,9 +326,9 @@ __global__ __launch_bounds__(BLOCK) void stencil_kernel(const SArgs<T> a)
 // ---------------------------------------------------------------------------
 
 template <typename T>
-void stencil_geometry(SArgs<T> &a, int py)
+void stencil_geometry(SArgs<T> &a, int wy, int py)
 {
-    const int ty = TYT * py;
+    const int ty = wy * py;
     a.nxy = a.nx * a.ny;
     a.ntx = (a.nx + TX - 1) / TX;
     a.nty = (a.ny + ty - 1) / ty;
@@ -314,43 +339,47 @@ void stencil_geometry(SArgs<T> &a, int py)
 }
 
 template <typename T, SMode M>
-hipError_t launch_stencil_t(SArgs<T> a, int py, hipStream_t st)
+hipError_t launch_stencil_t(SArgs<T> a, int wy, int py, hipStream_t st)
 {
-    stencil_geometry(a, py);
-    dim3 grid(a.nitems), block(BLOCK);
-    switch (py) {
-    case 2: hipLaunchKernelGGL((stencil_kernel<T, 2, M>), grid, block, 0, st, a); break;
-    case 4: hipLaunchKernelGGL((stencil_kernel<T, 4, M>), grid, block, 0, st, a); break;
+    stencil_geometry(a, wy, py);
+    dim3 grid(a.nitems), block(64 * wy);
+    const int key = wy * 10 + py;
+    switch (key) {
+    case 42: hipLaunchKernelGGL((stencil_kernel<T, 4, 2, M>), grid, block, 0, st, a); break;
+    case 44: hipLaunchKernelGGL((stencil_kernel<T, 4, 4, M>), grid, block, 0, st, a); break;
+    case 82: hipLaunchKernelGGL((stencil_kernel<T, 8, 2, M>), grid, block, 0, st, a); break;
+    case 84: hipLaunchKernelGGL((stencil_kernel<T, 8, 4, M>), grid, block, 0, st, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_stencil(SMode m, const SArgs<T> &a, int py, hipStream_t st)
+hipError_t launch_stencil(SMode m, const SArgs<T> &a, int wy, int py, hipStream_t st)
 {
     switch (m) {
-    case SMode::MATVEC: return launch_stencil_t<T, SMode::MATVEC>(a, py, st);
-    case SMode::AAR_RICH: return launch_stencil_t<T, SMode::AAR_RICH>(a, py, st);
-    case SMode::RESID: return launch_stencil_t<T, SMode::RESID>(a, py, st);
-    case SMode::PGR_RICH: return launch_stencil_t<T, SMode::PGR_RICH>(a, py, st);
-    case SMode::CG_Q: return launch_stencil_t<T, SMode::CG_Q>(a, py, st);
-    case SMode::CG_RESID: return launch_stencil_t<T, SMode::CG_RESID>(a, py, st);
+    case SMode::MATVEC: return launch_stencil_t<T, SMode::MATVEC>(a, wy, py, st);
+    case SMode::AAR_RICH: return launch_stencil_t<T, SMode::AAR_RICH>(a, wy, py, st);
+    case SMode::RESID: return launch_stencil_t<T, SMode::RESID>(a, wy, py, st);
+    case SMode::PGR_RICH: return launch_stencil_t<T, SMode::PGR_RICH>(a, wy, py, st);
+    case SMode::CG_Q: return launch_stencil_t<T, SMode::CG_Q>(a, wy, py, st);
+    case SMode::CG_RESID: return launch_stencil_t<T, SMode::CG_RESID>(a, wy, py, st);
+    case SMode::MATVEC_RES: return launch_stencil_t<T, SMode::MATVEC_RES>(a, wy, py, st);
     }
     return hipErrorInvalidValue;
 }
 
 template <typename T>
-int stencil_items(const SArgs<T> &a0, int py)
+int stencil_items(const SArgs<T> &a0, int wy, int py)
 {
     SArgs<T> a = a0;
-    stencil_geometry(a, py);
+    stencil_geometry(a, wy, py);
     return a.nitems;
 }
 
-template hipError_t launch_stencil<double>(SMode, const SArgs<double> &, int, hipStream_t);
-template hipError_t launch_stencil<float>(SMode, const SArgs<float> &, int, hipStream_t);
-template int stencil_items<double>(const SArgs<double> &, int);
-template int stencil_items<float>(const SArgs<float> &, int);
+template hipError_t launch_stencil<double>(SMode, const SArgs<double> &, int, int, hipStream_t);
+template hipError_t launch_stencil<float>(SMode, const SArgs<float> &, int, int, hipStream_t);
+template int stencil_items<double>(const SArgs<double> &, int, int);
+template int stencil_items<float>(const SArgs<float> &, int, int);
 
 }  // namespace aar

@@ -40,16 +40,17 @@ def test_stencil_bitwise(gpu, oracle, shape, sigma):
     assert np.array_equal(y, yo)
 
 
-@pytest.mark.parametrize("lz,py", [(0, 4), (0, 2), (7, 4), (1, 2), (512, 4)])
-def test_stencil_tuning_bitwise(gpu, oracle, lz, py):
-    say("stencil tuning", lz, py)
+@pytest.mark.parametrize("lz,wy,py", [(0, 4, 4), (0, 4, 2), (7, 4, 4), (1, 4, 2), (512, 4, 4),
+                                      (0, 8, 4), (0, 8, 2), (5, 8, 4), (3, 8, 2)])
+def test_stencil_tuning_bitwise(gpu, oracle, lz, wy, py):
+    say("stencil tuning", lz, wy, py)
     shape = (72, 40, 36)
     x = np.random.default_rng(7).standard_normal(int(np.prod(shape)))
-    gpu.set_tuning(lz, py)
+    gpu.set_tuning(lz, wy, py)
     try:
         y = gpu.Lap_Vec_mult(dev(x), shape, -16.0).cpu().numpy()
     finally:
-        gpu.set_tuning(0, 4)
+        gpu.set_tuning()
     assert np.array_equal(y, oracle.matvec(x, shape, -16.0))
 
 
@@ -138,21 +139,30 @@ def test_aar_fp32(gpu):
 
 @pytest.mark.parametrize("sigma", [0.0, 0.5])
 def test_refined_vs_fp64(gpu, oracle, sigma):
+    """FP32-refined AAR to relres 1e-10 is as accurate as FP64 AAR at the
+    same tolerance: both measured against the exact (FFT) solution of the
+    periodic system."""
     shape = (48, 48, 48)
     a = P.poisson_a(0.5)
     b = P.rhs_uniform(shape, 1003) if sigma == 0.0 else P.rhs_normal(shape, 1003)
     say("refined", sigma)
     xr = dev(np.zeros(b.size))
     st = gpu.AAR_refined(xr, dev(b), shape, a, 0.6, 0.6, 7, 6, 1e-10, inner_tol=1e-5, sigma=sigma)
+    say("refined stats", st)
     assert st["converged"] and st["relres"] <= 1e-10
     x64 = dev(np.zeros(b.size))
     st64 = gpu.AAR(x64, dev(b), shape, a, 0.6, 0.6, 7, 6, 5000, 1e-10, sigma=sigma)
     assert st64["converged"]
+    xf = fft_solve(b, shape, a, oracle.lap_coefficient(6), sigma, mean=0.0)
     u, v = xr.cpu().numpy(), x64.cpu().numpy()
     if sigma == 0.0:                  # singular periodic problem: compare modulo the mean
         u = u - u.mean()
         v = v - v.mean()
-    assert np.linalg.norm(u - v) / np.linalg.norm(v) <= 1e-9
+    e_ref = np.linalg.norm(u - xf) / np.linalg.norm(xf)
+    e_64 = np.linalg.norm(v - xf) / np.linalg.norm(xf)
+    say("refined error vs exact", e_ref, "fp64 error vs exact", e_64,
+        "refined vs fp64", np.linalg.norm(u - v) / np.linalg.norm(v))
+    assert e_ref <= max(2 * e_64, 1e-9)
 
 
 def test_large_residual_properties(gpu):
@@ -215,3 +225,40 @@ def test_foreign_callback_rejected(gpu):
     L.AAR(None, cb, None, -4.0, x, b, 0.6, 0.6, 7, 6, 10, 1e-6, 8, 0)
     assert not called and list(x) == [1.0] * 8
     assert b"foreign" in L.aar_hip_last_error()
+
+
+def _ptr_rows(M):
+    import ctypes
+    rows = (ctypes.POINTER(ctypes.c_double) * M.shape[0])()
+    for i in range(M.shape[0]):
+        rows[i] = M[i].ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    return rows
+
+
+def test_anderson_step_reference_abi(gpu, golden):
+    """AndersonExtrapolation (AAR.h:24-25) with host arrays: the Gram
+    (allredvec), the solve and am_vec against the reference's own output."""
+    import ctypes
+    from aar_hip import _lib
+    L = _lib.load()
+    _, g = golden
+    DX = np.ascontiguousarray(g["and_DX"])
+    DF = np.ascontiguousarray(g["and_DF"])
+    f = np.ascontiguousarray(g["and_f"])
+    m, n = DX.shape
+    am = np.zeros(n)
+    red = np.zeros(m * m + m)
+    Ftf = np.zeros(m)
+    sv = np.zeros(m)
+    FtF = np.zeros((m, m))
+    D = ctypes.POINTER(ctypes.c_double)
+    P = ctypes.POINTER(D)
+    L.AndersonExtrapolation.argtypes = [P, P, D, ctypes.c_double, ctypes.c_int, ctypes.c_int, D, P, D, D,
+                                        D, ctypes.c_int]
+    L.AndersonExtrapolation(_ptr_rows(DX), _ptr_rows(DF), f.ctypes.data_as(D), 0.6, m, n,
+                            am.ctypes.data_as(D), _ptr_rows(FtF), red.ctypes.data_as(D),
+                            Ftf.ctypes.data_as(D), sv.ctypes.data_as(D), 0)
+    ref_red = g["and_red"]
+    assert np.allclose(red, ref_red, rtol=1e-13, atol=1e-12 * np.abs(ref_red).max())
+    assert np.array_equal(red[m * m:], Ftf)
+    assert np.allclose(am, g["and_am"], rtol=1e-10, atol=1e-12 * np.abs(g["and_am"]).max())
