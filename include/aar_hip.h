@@ -68,6 +68,15 @@ int aar_hip_aar(const aar_hip_grid *g, double *x, const double *b, double omega,
 int aar_hip_aar_f32(const aar_hip_grid *g, float *x, const float *b, double omega, double beta,
                     int m, int p, int max_iter, double tol, int precond, aar_hip_stats *st);
 
+/* AAR.c:37-177 in mixed precision (extension, no reference counterpart):
+ * x and b FP64; the true residual b - A x of every post-Anderson iterate,
+ * the iterate itself and the m x m normal equations in FP64; the Richardson
+ * sweeps (stencil), the residual history and the Gram reads in FP32.  The
+ * FP64 true residual replaces the FP32 recurrence every p iterations, so
+ * convergence is judged on b - A x in FP64 as in the reference. */
+int aar_hip_aar_mixed(const aar_hip_grid *g, double *x, const double *b, double omega, double beta,
+                      int m, int p, int max_iter, double tol, int precond, aar_hip_stats *st);
+
 /* PGR.c:35-173 (l2 = 0) and PL2R.c:35-172 (l2 = 1) */
 int aar_hip_pgr(const aar_hip_grid *g, double *x, const double *b, double omega, int m, int p,
                 int max_iter, double tol, int precond, int l2, aar_hip_stats *st);
@@ -119,9 +128,9 @@ int aar_hip_set_tuning(int lz, int wy, int py);
 int aar_hip_set_option(const char *name, long value);
 
 /* Per-kernel GPU timing with hipEvents (off by default).  Kernel names:
- * "stencil_matvec", "stencil_aar_rich", "stencil_resid", "stencil_pgr_rich",
- * "stencil_cg_q", "stencil_cg_resid", "gram", "ls", "aar_update",
- * "pgr_update", "cg_update". */
+ * "stencil_matvec", "stencil_aar_true", "stencil_aar_rec", "stencil_resid",
+ * "stencil_pgr_rich", "stencil_cg_q", "stencil_cg_resid", "stencil_matvec_res",
+ * "gram", "ls", "anchor_update", "fold", "dot", "pgr_update", "cg_update". */
 int aar_hip_profile(int enable);
 int aar_hip_profile_reset(void);
 int aar_hip_profile_query(const char *name, int *count, double *total_ms, double *bytes);

@@ -49,7 +49,8 @@ REFERENCE_SYMBOLS = [
     "pythag", "SingularValueDecomp", "PseudoInverseTimesVec", "Vector2Norm", "VectorDotProduct",
 ]
 EXTENSION_SYMBOLS = [
-    "aar_hip_matvec", "aar_hip_matvec_f32", "aar_hip_aar", "aar_hip_aar_f32", "aar_hip_pgr",
+    "aar_hip_matvec", "aar_hip_matvec_f32", "aar_hip_aar", "aar_hip_aar_f32", "aar_hip_aar_mixed",
+    "aar_hip_pgr",
     "aar_hip_cg", "aar_hip_aar_refined", "aar_hip_pinv", "aar_hip_history", "aar_hip_last_stats",
     "aar_hip_set_stream", "aar_hip_set_tuning", "aar_hip_set_option", "aar_hip_profile", "aar_hip_profile_reset",
     "aar_hip_profile_query", "aar_hip_release", "aar_hip_last_error",
@@ -81,6 +82,7 @@ def load(require_gpu=True):
         "aar_hip_matvec_f32": [G, P, P],
         "aar_hip_aar": [G, P, P, D, D, I, I, I, D, I, S],
         "aar_hip_aar_f32": [G, P, P, D, D, I, I, I, D, I, S],
+        "aar_hip_aar_mixed": [G, P, P, D, D, I, I, I, D, I, S],
         "aar_hip_pgr": [G, P, P, D, I, I, I, D, I, I, S],
         "aar_hip_cg": [G, P, P, I, D, I, S],
         "aar_hip_aar_refined": [G, P, P, D, D, I, I, I, I, D, D, I, S],

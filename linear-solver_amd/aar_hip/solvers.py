@@ -92,6 +92,21 @@ def AAR(x, rhs, shape, a, omega, beta, m, p, max_iter, tol, sigma=0.0, precond=T
     return st.as_dict()
 
 
+def AAR_mixed(x, rhs, shape, a, omega, beta, m, p, max_iter, tol, sigma=0.0, precond=True):
+    """AAR.c:37-177 in mixed precision (no reference counterpart): FP64 x, b,
+    true residual and normal equations; FP32 Richardson sweeps and history."""
+    import torch
+    _check_len(x, shape, "x")
+    _check_len(rhs, shape, "rhs")
+    g = _grid(shape, a, sigma)
+    st = Stats()
+    _bind_stream()
+    check(_L().aar_hip_aar_mixed(ctypes.byref(g), _dev(x, torch.float64, "x"),
+                                 _dev(rhs, torch.float64, "rhs"), omega, beta, m, p, max_iter, tol,
+                                 int(bool(precond)), ctypes.byref(st)))
+    return st.as_dict()
+
+
 def PGR(x, rhs, shape, a, omega, m, p, max_iter, tol, sigma=0.0, precond=True, _l2=False):
     """PGR.c:35-173."""
     import torch

@@ -7,12 +7,13 @@ namespace aar {
 
 enum class SMode : int {
     MATVEC = 0,    // y = a*Lap(x) + sigma*x                      (system.c:189-205)
-    AAR_RICH = 1,  // r = b - A x; f = r/diag; x_new = x + omega f (AAR.c:93-94,134-139)
+    AAR_TRUE = 1,  // r = b - A x; g = r/diag (T or FP32 out); ||r||^2 (AAR.c:80-82,119-123)
     RESID = 2,     // r = b - A x (+ optional fp32 copy), ||r||^2
     PGR_RICH = 3,  // u = x + omega (b - Ax)/diag; Au = A u        (PGR.c:92-103)
     CG_Q = 4,      // d = s + beta d (s = r/diag); q = A d; d.q   (CG.c:94-100,148-150)
     CG_RESID = 5,  // r = b - A x; r.s, r.r                        (CG.c:65-84,112-137)
     MATVEC_RES = 6,  // y = A x and ||b - A x||^2                  (PGR.c:79-81)
+    AAR_REC = 7,   // g' = g - (omega/diag) A g: the residual of x + omega g (AAR.c:134-139)
 };
 
 template <typename T>
@@ -21,11 +22,11 @@ struct SArgs {
     int lz = 0;                       // z-chunk length (planes per block)
     int ntx = 0, nty = 0, nzc = 0, nitems = 0;
     double w[R + 1] = {};             // a * coeff_lap[o]
+    double cf[R + 1] = {};            // AAR_REC: g' = cf[0] g + sum_o cf[o] S_o (S_o: the 6 neighbours at o)
     double sigma = 0.0;               // screened term
     double diag = 1.0;                // Jacobi diagonal a*(3*coeff[0]) + sigma
     double omega = 0.0;
     int precond = 1;
-    int write_x = 0;
     int trace = 0;
     int cg_first = 0;
     const T *in0 = nullptr, *in1 = nullptr, *in2 = nullptr;   // stencil operand fields
@@ -104,6 +105,42 @@ struct UpdArgs {
 };
 template <typename T>
 hipError_t launch_aar_update(const UpdArgs<T> &a, hipStream_t st);
+
+// Anderson step of the residual-recurrence AAR (solver.hip).  The iterate
+// is held as an anchor x (updated in place) plus Richardson increments
+// omega*g_j of the stored preconditioned residuals g_j; a term i of the
+// update is ring slot gslot[i] (g_{jlo+i}), a correction term c is
+// cslot[c] of the correction ring (x_{q+1} - x_q of an earlier Anderson
+// step q).  With gamma = ctrl->s (reference column order) and difference
+// column t, j = it - t: dg = g_j - g_{j-1}, dx = omega g_{j-1} or a
+// correction,
+//   corr  = beta g_it - sum_t gamma_t (dx_t + beta dg_t)   (g_it: term ng-1)
+//   x_new = x + omega sum_{xlo <= i < xhi} g_i + corr,   written back to x;
+//   C[cout] = corr when cout >= 0.
+// With ncol = 0 and with_beta = 0 the same kernel folds x += omega sum g_i.
+constexpr int NGMAX = MAXM + 2;
+constexpr int NCMAX = MAXM + 1;
+template <typename Tx, typename Tg>
+struct AncArgs {
+    long n = 0;
+    Tx *x = nullptr;
+    const Tg *G = nullptr;
+    Tg *C = nullptr;
+    long ld = 0;
+    int ng = 0, nc = 0, ncol = 0;
+    int gslot[NGMAX] = {};
+    int xlo = 0, xhi = 0;     // terms of the Richardson increments since the anchor
+    int with_beta = 0;        // beta g_it term (Anderson step)
+    int cslot[NCMAX] = {};
+    int col_ref[MAXM] = {};   // reference column of difference column t
+    int col_j[MAXM] = {};     // term of g_j
+    int col_c[MAXM] = {};     // correction of dx_t, or -1 for omega g_{j-1} (term col_j - 1)
+    double omega = 0.0, beta = 0.0;
+    int cout = -1;
+    const Ctrl *ctrl = nullptr;
+};
+template <typename Tx, typename Tg>
+hipError_t launch_anchor_update(const AncArgs<Tx, Tg> &a, hipStream_t st);
 
 // PGR.c:117-130: x += sum DX s; Ax += sum DF s (in place at slot `cur`,
 // the age-0 slot), ||b - Ax||^2 -> FIN_PGR_RES.

@@ -77,6 +77,8 @@ int make_op(const aar_hip_grid *g, const double *coeff, Op &op);
 template <typename T>
 int aar_solve(const Op &op, T *x, const T *b, double omega, double beta, int m, int p, int max_iter,
               double tol, int precond, int ci, aar_hip_stats *st, int keep_history, int ref_start = 1);
+int aar_mixed(const Op &op, double *x, const double *b, double omega, double beta, int m, int p,
+              int max_iter, double tol, int precond, aar_hip_stats *st);
 int pgr_solve(const Op &op, double *x, const double *b, double omega, int m, int p, int max_iter,
               double tol, int precond, int l2, aar_hip_stats *st);
 int cg_solve(const Op &op, double *x, const double *b, int max_iter, double tol, int precond,

@@ -685,6 +685,14 @@ int aar_hip_aar_f32(const aar_hip_grid *g, float *x, const float *b, double omeg
     return aar_solve<float>(op, x, b, omega, beta, m, p, max_iter, tol, precond, 0, st, 1);
 }
 
+int aar_hip_aar_mixed(const aar_hip_grid *g, double *x, const double *b, double omega, double beta, int m,
+                      int p, int max_iter, double tol, int precond, aar_hip_stats *st)
+{
+    Op op;
+    if (int rc = make_op(g, nullptr, op)) return rc;
+    return aar_mixed(op, x, b, omega, beta, m, p, max_iter, tol, precond, st);
+}
+
 int aar_hip_pgr(const aar_hip_grid *g, double *x, const double *b, double omega, int m, int p, int max_iter,
                 double tol, int precond, int l2, aar_hip_stats *st)
 {

@@ -673,6 +673,131 @@ template hipError_t launch_aar_update<double>(const UpdArgs<double> &, hipStream
 template hipError_t launch_aar_update<float>(const UpdArgs<float> &, hipStream_t);
 
 // ---------------------------------------------------------------------------
+// Anderson step of the residual-recurrence AAR (AAR.c:109-117 with the
+// iterates held as anchor + omega * g increments; see AncArgs).  The term
+// coefficients are computed by every thread from uniform data (kernel
+// arguments, ctrl->s) with static indices only, so they stay in registers
+// without spills; NGM/NCM bound the term counts at compile time.
+// ---------------------------------------------------------------------------
+template <typename Tx, typename Tg, int NGM, int NCM>
+__global__ __launch_bounds__(BLOCK) void anchor_update_kernel(const AncArgs<Tx, Tg> a)
+{
+    if (a.ctrl && a.ctrl->done) return;
+    const int ng = a.ng, nc = a.nc;
+    double cg[NGM], cc[NCM > 0 ? NCM : 1];
+#pragma unroll
+    for (int i = 0; i < NGM; i++) cg[i] = (a.with_beta && i == ng - 1) ? a.beta : 0.0;
+#pragma unroll
+    for (int c = 0; c < (NCM > 0 ? NCM : 1); c++) cc[c] = 0.0;
+    for (int t = 0; t < a.ncol; t++) {
+        const double gam = a.ctrl->s[a.col_ref[t]];
+        const int j = a.col_j[t], cj = a.col_c[t];
+#pragma unroll
+        for (int i = 0; i < NGM; i++) {
+            // dg_t = g_j - g_{j-1}; dx_t = omega g_{j-1} (Richardson step)
+            if (i == j) cg[i] -= a.beta * gam;
+            if (i == j - 1) cg[i] += (cj < 0) ? (a.beta - a.omega) * gam : a.beta * gam;
+        }
+#pragma unroll
+        for (int c = 0; c < NCM; c++)
+            if (c == cj) cc[c] -= gam;       // dx_t = an earlier Anderson correction
+    }
+    const int xlo = a.xlo, xhi = a.xhi;
+    const double om = a.omega;
+    Tg *co = (a.cout >= 0) ? a.C + (size_t)a.cout * a.ld : nullptr;
+
+    const long n = a.n;
+    const long stride = (long)gridDim.x * BLOCK;
+    // pairs when n is even and x is aligned to a pair (ring slots always are)
+    if ((n & 1) == 0 && ((uintptr_t)a.x % (2 * sizeof(Tx))) == 0) {
+        using VX = typename Vec2<Tx>::type;
+        using VG = typename Vec2<Tg>::type;
+        const long np = n >> 1;
+        for (long ip = (long)blockIdx.x * BLOCK + threadIdx.x; ip < np; ip += stride) {
+            const long e = 2 * ip;
+            double sx0 = 0.0, sx1 = 0.0, sc0 = 0.0, sc1 = 0.0;
+            VG v[NGM];
+#pragma unroll
+            for (int i = 0; i < NGM; i++)
+                if (i < ng) v[i] = *reinterpret_cast<const VG *>(a.G + (size_t)a.gslot[i] * a.ld + e);
+            VG w[NCM > 0 ? NCM : 1];
+#pragma unroll
+            for (int c = 0; c < NCM; c++)
+                if (c < nc) w[c] = *reinterpret_cast<const VG *>(a.C + (size_t)a.cslot[c] * a.ld + e);
+            const VX xv = *reinterpret_cast<const VX *>(a.x + e);
+#pragma unroll
+            for (int i = 0; i < NGM; i++)
+                if (i < ng) {
+                    const double g0 = (double)v[i].x, g1 = (double)v[i].y;
+                    if (i >= xlo && i < xhi) {
+                        sx0 += g0;
+                        sx1 += g1;
+                    }
+                    sc0 += cg[i] * g0;
+                    sc1 += cg[i] * g1;
+                }
+#pragma unroll
+            for (int c = 0; c < NCM; c++)
+                if (c < nc) {
+                    sc0 += cc[c] * (double)w[c].x;
+                    sc1 += cc[c] * (double)w[c].y;
+                }
+            VX xo;
+            xo.x = (Tx)((double)xv.x + (om * sx0 + sc0));
+            xo.y = (Tx)((double)xv.y + (om * sx1 + sc1));
+            *reinterpret_cast<VX *>(a.x + e) = xo;
+            if (co) {
+                VG c2;
+                c2.x = (Tg)sc0;
+                c2.y = (Tg)sc1;
+                *reinterpret_cast<VG *>(co + e) = c2;
+            }
+        }
+    } else {
+        for (long e = (long)blockIdx.x * BLOCK + threadIdx.x; e < n; e += stride) {
+            double sx = 0.0, sc = 0.0;
+#pragma unroll
+            for (int i = 0; i < NGM; i++)
+                if (i < ng) {
+                    const double g = (double)a.G[(size_t)a.gslot[i] * a.ld + e];
+                    if (i >= xlo && i < xhi) sx += g;
+                    sc += cg[i] * g;
+                }
+#pragma unroll
+            for (int c = 0; c < NCM; c++)
+                if (c < nc) sc += cc[c] * (double)a.C[(size_t)a.cslot[c] * a.ld + e];
+            a.x[e] = (Tx)((double)a.x[e] + (om * sx + sc));
+            if (co) co[e] = (Tg)sc;
+        }
+    }
+}
+
+template <typename Tx, typename Tg>
+hipError_t launch_anchor_update(const AncArgs<Tx, Tg> &a, hipStream_t st)
+{
+    if (a.ng < 1 || a.ng > NGMAX || a.nc < 0 || a.nc > NCMAX || a.ncol < 0 || a.ncol > MAXM)
+        return hipErrorInvalidValue;
+    if (a.xlo < 0 || a.xhi > a.ng || !a.G || !a.x) return hipErrorInvalidValue;
+    if ((a.nc > 0 || a.cout >= 0) && !a.C) return hipErrorInvalidValue;
+    if (a.ncol > 0 && !a.ctrl) return hipErrorInvalidValue;
+    for (int t = 0; t < a.ncol; t++)
+        if (a.col_j[t] < 1 || a.col_j[t] >= a.ng || a.col_c[t] >= a.nc || a.col_ref[t] < 0 ||
+            a.col_ref[t] >= MAXM)
+            return hipErrorInvalidValue;
+    long nb = (a.n / 2 + BLOCK - 1) / BLOCK;
+    nb = nb < 1 ? 1 : (nb > 4096 ? 4096 : nb);
+    if (a.ng <= 8 && a.nc <= 1)
+        hipLaunchKernelGGL((anchor_update_kernel<Tx, Tg, 8, 1>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a);
+    else
+        hipLaunchKernelGGL((anchor_update_kernel<Tx, Tg, NGMAX, NCMAX>), dim3((unsigned)nb), dim3(BLOCK), 0,
+                           st, a);
+    return hipGetLastError();
+}
+template hipError_t launch_anchor_update<double, double>(const AncArgs<double, double> &, hipStream_t);
+template hipError_t launch_anchor_update<double, float>(const AncArgs<double, float> &, hipStream_t);
+template hipError_t launch_anchor_update<float, float>(const AncArgs<float, float> &, hipStream_t);
+
+// ---------------------------------------------------------------------------
 // PGR / PL2R projection update (PGR.c:117-133): x += sum DX s; Ax += sum DF s
 // in place at the age-0 slot; ||b - Ax||^2 for the convergence check.
 // ---------------------------------------------------------------------------

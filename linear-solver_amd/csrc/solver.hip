@@ -213,7 +213,8 @@ static const char *smode_name(SMode m)
 {
     switch (m) {
     case SMode::MATVEC: return "stencil_matvec";
-    case SMode::AAR_RICH: return "stencil_aar_rich";
+    case SMode::AAR_TRUE: return "stencil_aar_true";
+    case SMode::AAR_REC: return "stencil_aar_rec";
     case SMode::RESID: return "stencil_resid";
     case SMode::PGR_RICH: return "stencil_pgr_rich";
     case SMode::CG_Q: return "stencil_cg_q";
@@ -343,11 +344,31 @@ static int store_history(const double *hist_dev, int len)
 }
 
 // ---------------------------------------------------------------------------
-// AAR (AAR.c:37-177), FP64 or FP32
+// AAR (AAR.c:37-177) by residual recurrence.
+//
+// With the constant Jacobi diagonal d (AAR.c:94), a Richardson step
+// x_{k+1} = x_k + omega g_k (AAR.c:134-135) changes the preconditioned
+// residual g = (b - A x)/d by g_{k+1} = g_k - (omega/d) A g_k.  Between two
+// Anderson steps the solver therefore advances g alone (AAR_REC: read g_k,
+// write g_{k+1}) and keeps x as an anchor: x_k = x_a + omega sum_{a<=j<k} g_j,
+// where x_a is the iterate after the last Anderson step (the caller's x
+// buffer itself).  At an Anderson iteration the Gram of the g ring gives
+// gamma exactly as before (DF columns are g_j - g_{j-1}), DX columns are
+// omega g_{j-1} or, for the step after an earlier Anderson update, the
+// stored correction x_{q+1} - x_q, and one streaming pass writes the new
+// anchor and its correction.  The true residual b - A x of the new anchor
+// (AAR_TRUE) restarts the recurrence and is the residual the reference
+// checks (AAR.c:119-123), so convergence is decided on b - A x exactly
+// where the reference decides it.  Per Anderson cycle (m = 7, p = 6) this
+// moves 33 vectors instead of 49.
+//
+// Tx is the type of x and b; Tg of the g ring and corrections.  <double,
+// double> is FP64 AAR, <float, float> FP32 AAR, <double, float> the mixed
+// solver: FP64 true residual, anchor and normal equations, FP32 sweeps.
 // ---------------------------------------------------------------------------
-template <typename T>
-int aar_solve(const Op &op, T *x, const T *b, double omega, double beta, int m, int p, int max_iter,
-              double tol, int precond, int ci, aar_hip_stats *st, int keep_history, int ref_start)
+template <typename Tx, typename Tg>
+int aar_rec(const Op &op, Tx *x, const Tx *b, double omega, double beta, int m, int p, int max_iter,
+            double tol, int precond, int ci, aar_hip_stats *st, int keep_history, int ref_start)
 {
     Context &C = ctx();
     if (int rc = context_init()) return rc;
@@ -356,34 +377,60 @@ int aar_solve(const Op &op, T *x, const T *b, double omega, double beta, int m, 
     if (p < 1) return fail(AAR_HIP_ERR_ARG, "p must be >= 1");
     if (max_iter < 0) return fail(AAR_HIP_ERR_ARG, "max_iter must be >= 0");
     const long n = op.n;
-    const int ns = m + 1;
-    DevBuf &RX = (sizeof(T) == 8) ? C.ring64x : C.ring32x;
-    DevBuf &RF = (sizeof(T) == 8) ? C.ring64f : C.ring32f;
-    const long ld = ring_ld(n, sizeof(T));
-    HC(ensure(RX, sizeof(T) * ld * ns));
-    HC(ensure(RF, sizeof(T) * ld * ns));
+    // g ring: g_k in slot k % S.  Live at any time: the m+1 residuals of the
+    // next Anderson window and the one being written.
+    const int S = m + 2;
+    // corrections of the Anderson steps inside a window [k-m, k-1]
+    const int NC = (p <= m) ? m / p + 1 : 0;
+    constexpr bool f64g = sizeof(Tg) == 8;
+    DevBuf &RG = f64g ? C.ring64f : C.ring32f;
+    DevBuf &RC = f64g ? C.ring64x : C.ring32x;
+    const long ld = ring_ld(n, sizeof(Tg));
+    HC(ensure(RG, sizeof(Tg) * ld * S));
+    if (NC > 0) HC(ensure(RC, sizeof(Tg) * ld * NC));
     DevBuf &HB = ci == 0 ? C.hist0 : C.hist1;
     HC(ensure(HB, sizeof(double) * (size_t)(max_iter + 4)));
     const int nb = nblocks_for(n);
-    const size_t need = std::max<size_t>((size_t)stencil_partials_needed<T>(op), (size_t)nb * 80);
+    const size_t need = std::max<size_t>((size_t)stencil_partials_needed<Tg>(op), (size_t)nb * 80);
     DevBuf &PB = ci == 0 ? C.part0 : C.part1;
     HC(ensure(PB, sizeof(double) * need));
-    T *X = (T *)RX.p, *F = (T *)RF.p;
+    Tg *G = (Tg *)RG.p;
+    Tg *CR = NC > 0 ? (Tg *)RC.p : nullptr;
     double *hist = (double *)HB.p, *part = (double *)PB.p;
     Ctrl *ctrl = C.ctrl + ci;
     unsigned *cnt = C.counters + 8 * ci;
-    auto slot = [&](long it) { return (int)(it % ns); };
-    const double s_b = (double)sizeof(T) * n;
+    auto gs = [&](long k) { return (int)(k % S); };
+    auto gptr = [&](long k) { return G + (size_t)gs(k) * ld; };
+    auto is_and = [&](long k) { return k > 1 && k % p == 0; };          // AAR.c:109
+    auto next_and = [&](long k) { long q = (k / p + 1) * p; return q > 1 ? q : q + p; };
+    const double sx = (double)sizeof(Tx) * n, sg = (double)sizeof(Tg) * n;
     C.bytes_acc = 0.0;
+
+    // true residual of x into g_k (AAR.c:80-82 / 119-123)
+    auto true_res = [&](long k, int check) -> int {
+        SArgs<Tx> a = base_sargs<Tx>(op);
+        a.omega = omega;
+        a.precond = precond;
+        a.in0 = x;
+        a.bvec = b;
+        if constexpr (sizeof(Tx) == sizeof(Tg)) a.out0 = (Tx *)gptr(k);
+        else a.out32 = (float *)gptr(k);
+        a.partials = part;
+        a.counter = cnt + 1;
+        a.ctrl = ctrl;
+        a.fin.fin = FIN_AAR_RES;
+        a.fin.it = (int)k;
+        a.fin.check = check;
+        a.fin.hist = hist;
+        a.fin.hist_cap = max_iter + 4;
+        return run_stencil<Tx>(SMode::AAR_TRUE, a, 2 * sx + sg);
+    };
 
     HC(launch_zero_ctrl(ctrl, C.stream));
     HC(hipEventRecord(C.t0, C.stream));
-    // x_1 = x0 into its slot
-    HC(hipMemcpyAsync(X + (size_t)slot(1) * ld, x, sizeof(T) * n, hipMemcpyDeviceToDevice, C.stream));
-    C.bytes_acc += 2 * s_b;
     // ||b|| and the scaled tolerance (AAR.c:78-79)
     {
-        DotArgs<T> d;
+        DotArgs<Tx> d;
         d.n = n;
         d.v = b;
         d.nblocks = nb;
@@ -394,98 +441,144 @@ int aar_solve(const Op &op, T *x, const T *b, double omega, double beta, int m, 
         d.fin.tol = tol;
         d.fin.check = ref_start;      // the reference's zero-iteration rule (AAR.c:76-79)
         prof_begin("dot");
-        HC(run_dot<T>(d));
-        prof_end(s_b);
+        HC(run_dot<Tx>(d));
+        prof_end(sx);
     }
-    C.bytes_acc += s_b;
+    C.bytes_acc += sx;
+    if (int rc = true_res(1, 0)) return rc;
 
+    long first = 1;         // x = x_a + omega sum_{first <= j < k} g_j
     int cyc = 0;
-    int it = 1;
+    long it = 1;
     bool stopped = false;
     for (; it <= max_iter && !stopped; it++) {
-        const bool anderson = (it % p == 0) && (it > 1);            // AAR.c:109
-        const int prev = it - 1;
-        const bool check = prev >= 1 && (prev % p == 0) && (prev > 1);
-        SArgs<T> a = base_sargs<T>(op);
-        a.omega = omega;
-        a.precond = precond;
-        a.in0 = X + (size_t)slot(it) * ld;
-        a.bvec = b;
-        a.out0 = F + (size_t)slot(it) * ld;
-        a.out1 = X + (size_t)slot(it + 1) * ld;
-        a.write_x = anderson ? 0 : 1;
-        a.partials = part;
-        a.counter = cnt + 1;
-        a.ctrl = ctrl;
-        a.fin.fin = FIN_AAR_RES;
-        a.fin.it = it;
-        a.fin.check = check;
-        a.fin.hist = hist;
-        a.fin.hist_cap = max_iter + 4;
-        if (int rc = run_stencil<T>(SMode::AAR_RICH, a, s_b * (anderson ? 3 : 4))) return rc;
-        if (anderson) {
-            // history by age: slot of iterate it-k, k = 0..m; difference
-            // columns of age < nv exist (written from iteration 2 on, AAR.c:96)
-            GramArgs<T> g;
-            UpdArgs<T> u;
-            const int nv = (int)std::min<long>(m, it - 1);
-            for (int k = 0; k <= m; k++) g.slot[k] = u.slot[k] = it - k >= 1 ? slot(it - k) : 0;
-            g.n = u.n = n;
-            g.m = u.m = m;
-            g.nvalid = u.nvalid = nv;
-            g.it = u.it = it;
+        if (!is_and(it)) {
+            // Richardson (AAR.c:131-139) on the residual
+            SArgs<Tg> a = base_sargs<Tg>(op);
+            a.omega = omega;
+            a.precond = precond;
+            {
+                const double c = precond ? omega / op.diag : omega;
+                a.cf[0] = 1.0 - c * (3.0 * op.w[0] + op.sigma);
+                for (int o = 1; o <= R; o++) a.cf[o] = -c * op.w[o];
+            }
+            a.in0 = gptr(it);
+            a.out0 = gptr(it + 1);
+            a.trace = 1;
+            a.partials = part;
+            a.counter = cnt + 1;
+            a.ctrl = ctrl;
+            a.fin.fin = FIN_AAR_RES;
+            a.fin.it = (int)it + 1;
+            a.fin.check = 0;
+            a.fin.hist = hist;
+            a.fin.hist_cap = max_iter + 4;
+            if (int rc = run_stencil<Tg>(SMode::AAR_REC, a, 2 * sg)) return rc;
+            // a residual that no Anderson window will read again is folded
+            // into the anchor before its slot is reused (only when p > m+1)
+            if (it < next_and(it) - m) {
+                AncArgs<Tx, Tg> u;
+                u.n = n;
+                u.x = x;
+                u.G = G;
+                u.ld = ld;
+                u.ng = 1;
+                u.gslot[0] = gs(it);
+                u.xlo = 0;
+                u.xhi = 1;
+                u.omega = omega;
+                u.ctrl = ctrl;
+                prof_begin("fold");
+                HC((launch_anchor_update<Tx, Tg>(u, C.stream)));
+                prof_end(2 * sx + sg);
+                C.bytes_acc += 2 * sx + sg;
+                first = it + 1;
+            }
+            continue;
+        }
+        // Anderson step (AAR.c:109-123)
+        const int nv = (int)std::min<long>(m, it - 1);
+        {
+            GramArgs<Tg> g;
+            for (int k = 0; k <= m; k++) g.slot[k] = it - k >= 1 ? gs(it - k) : 0;
+            g.n = n;
+            g.m = m;
+            g.nvalid = nv;
+            g.it = (int)it;
             g.kind = 0;
-            g.F = F;
+            g.F = G;
             g.ld = ld;
-            g.fslot = slot(it);
+            g.fslot = gs(it);
             g.nblocks = nb;
             g.partials = part;
             g.counter = cnt + 2;
             g.ctrl = ctrl;
             prof_begin("gram");
-            HC(run_gram<T>(g));
-            prof_end(s_b * (nv + 1));
-            prof_begin("ls");
-            HC(launch_ls(ctrl, m, 0, C.stream));
-            prof_end(0);
-            u.beta = beta;
-            u.X = X;
-            u.F = F;
-            u.ld = ld;
-            u.oslot = slot(it + 1);
-            u.ctrl = ctrl;
-            prof_begin("aar_update");
-            HC(launch_aar_update<T>(u, C.stream));
-            prof_end(s_b * (2 * (nv + 1) + 1));
-            C.bytes_acc += s_b * (3 * (nv + 1) + 1);
-            if (int rc = poll_record(ctrl, cyc)) return rc;
-            if (poll_check(cyc - 2)) stopped = true;
-            cyc++;
+            HC(run_gram<Tg>(g));
+            prof_end(sg * (nv + 1));
+            C.bytes_acc += sg * (nv + 1);
         }
+        prof_begin("ls");
+        HC(launch_ls(ctrl, m, 0, C.stream));
+        prof_end(0);
+        {
+            AncArgs<Tx, Tg> u;
+            const long jlo = std::min(first, it - nv);
+            u.n = n;
+            u.x = x;
+            u.G = G;
+            u.C = CR;
+            u.ld = ld;
+            u.ng = (int)(it - jlo + 1);
+            for (int i = 0; i < u.ng; i++) u.gslot[i] = gs(jlo + i);
+            u.xlo = (int)(first - jlo);
+            u.xhi = (int)(it - jlo);
+            u.with_beta = 1;
+            u.ncol = nv;
+            u.nc = 0;
+            for (int t = 0; t < nv; t++) {
+                const long j = it - t;
+                u.col_ref[t] = (int)((j - 2) % m);                    // AAR.c:97
+                u.col_j[t] = (int)(j - jlo);
+                u.col_c[t] = -1;
+                if (is_and(j - 1)) {
+                    u.cslot[u.nc] = (int)(((j - 1) / p) % NC);
+                    u.col_c[t] = u.nc++;
+                }
+            }
+            u.cout = NC > 0 ? (int)((it / p) % NC) : -1;
+            u.omega = omega;
+            u.beta = beta;
+            u.ctrl = ctrl;
+            const double by = 2 * sx + sg * (u.ng + u.nc + (u.cout >= 0 ? 1 : 0));
+            prof_begin("anchor_update");
+            HC((launch_anchor_update<Tx, Tg>(u, C.stream)));
+            prof_end(by);
+            C.bytes_acc += by;
+        }
+        first = it + 1;
+        if (int rc = true_res(it + 1, 1)) return rc;
+        if (int rc = poll_record(ctrl, cyc)) return rc;
+        if (poll_check(cyc - 2)) stopped = true;
+        cyc++;
     }
-    if (!stopped) {
-        // the residual of the last iterate: the reference computes it inside
-        // the last iteration (AAR.c:119-123 / 137-139) and checks it when
-        // that iteration was an Anderson step
-        const int last = std::min(it, max_iter + 1);
-        const int prev = last - 1;
-        const bool check = prev >= 1 && (prev % p == 0) && (prev > 1);
-        SArgs<T> a = base_sargs<T>(op);
-        a.omega = omega;
-        a.precond = precond;
-        a.in0 = X + (size_t)slot(last) * ld;
-        a.bvec = b;
-        a.out0 = F + (size_t)slot(last) * ld;
-        a.write_x = 0;
-        a.partials = part;
-        a.counter = cnt + 1;
-        a.ctrl = ctrl;
-        a.fin.fin = FIN_AAR_RES;
-        a.fin.it = last;
-        a.fin.check = check;
-        a.fin.hist = hist;
-        a.fin.hist_cap = max_iter + 4;
-        if (int rc = run_stencil<T>(SMode::AAR_RICH, a, s_b * 3)) return rc;
+    if (!stopped && first <= max_iter) {
+        // ended on Richardson steps: x_{max_iter+1} = x_a + omega sum g_j
+        AncArgs<Tx, Tg> u;
+        u.n = n;
+        u.x = x;
+        u.G = G;
+        u.ld = ld;
+        u.ng = (int)(max_iter - first + 1);
+        for (int i = 0; i < u.ng; i++) u.gslot[i] = gs(first + i);
+        u.xlo = 0;
+        u.xhi = u.ng;
+        u.omega = omega;
+        u.ctrl = ctrl;
+        prof_begin("fold");
+        HC((launch_anchor_update<Tx, Tg>(u, C.stream)));
+        prof_end(2 * sx + sg * u.ng);
+        C.bytes_acc += 2 * sx + sg * u.ng;
     }
     HC(hipEventRecord(C.t1, C.stream));
     Ctrl &hc = C.ctrl_h[ci];
@@ -501,11 +594,6 @@ int aar_solve(const Op &op, T *x, const T *b, double omega, double beta, int m, 
         iterations = max_iter;
         loop_iter = max_iter + 1;
     }
-    // solution: iterate following the last iteration
-    HC(hipMemcpyAsync(x, X + (size_t)slot(iterations + 1) * ld, sizeof(T) * n, hipMemcpyDeviceToDevice,
-                      C.stream));
-    HC(hipStreamSynchronize(C.stream));
-    C.bytes_acc += 2 * s_b;
     const int hl = std::min(iterations + 1, max_iter + 4);
     if (keep_history)
         if (int rc = store_history(hist, hl)) return rc;
@@ -513,6 +601,19 @@ int aar_solve(const Op &op, T *x, const T *b, double omega, double beta, int m, 
     if (hc.status == 1) return fail(AAR_HIP_ERR_SVD, "no convergence in 250 svd iterations");
     if (hc.status == 2) return fail(AAR_HIP_ERR_NONFINITE, "residual is not finite");
     return 0;
+}
+
+template <typename T>
+int aar_solve(const Op &op, T *x, const T *b, double omega, double beta, int m, int p, int max_iter,
+              double tol, int precond, int ci, aar_hip_stats *st, int keep_history, int ref_start)
+{
+    return aar_rec<T, T>(op, x, b, omega, beta, m, p, max_iter, tol, precond, ci, st, keep_history, ref_start);
+}
+
+int aar_mixed(const Op &op, double *x, const double *b, double omega, double beta, int m, int p,
+              int max_iter, double tol, int precond, aar_hip_stats *st)
+{
+    return aar_rec<double, float>(op, x, b, omega, beta, m, p, max_iter, tol, precond, 0, st, 1, 1);
 }
 
 template int aar_solve<double>(const Op &, double *, const double *, double, double, int, int, int, double,

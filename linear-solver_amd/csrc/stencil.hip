@@ -46,7 +46,8 @@ template <> struct Arith<float> {
 template <SMode M> struct ModeInfo;
 template <> struct ModeInfo<SMode::MATVEC> { static constexpr int NF = 1, NE = 0, NACC = 1; };
 template <> struct ModeInfo<SMode::MATVEC_RES> { static constexpr int NF = 1, NE = 1, NACC = 1; };
-template <> struct ModeInfo<SMode::AAR_RICH> { static constexpr int NF = 1, NE = 1, NACC = 1; };
+template <> struct ModeInfo<SMode::AAR_TRUE> { static constexpr int NF = 1, NE = 1, NACC = 1; };
+template <> struct ModeInfo<SMode::AAR_REC> { static constexpr int NF = 1, NE = 0, NACC = 1; };
 template <> struct ModeInfo<SMode::RESID> { static constexpr int NF = 1, NE = 1, NACC = 1; };
 template <> struct ModeInfo<SMode::PGR_RICH> { static constexpr int NF = 3, NE = 1, NACC = 1; };
 template <> struct ModeInfo<SMode::CG_Q> { static constexpr int NF = 2, NE = 0, NACC = 1; };
@@ -127,6 +128,10 @@ __global__ __launch_bounds__(64 * WY) void stencil_kernel(const SArgs<T> a)
 #pragma unroll
     for (int o = 0; o <= R; o++) wt[o] = (T)a.w[o];
     const T sig = (T)a.sigma, dg = (T)a.diag, om = (T)a.omega;
+    T cf[R + 1];
+#pragma unroll
+    for (int o = 0; o <= R; o++) cf[o] = (T)a.cf[o];
+    const double trace_scale = a.precond ? a.diag : 1.0;
     const T cgb = (MODE == SMode::CG_Q && a.cg_beta) ? (T)(*a.cg_beta) : (T)0;
 
     auto load_raw = [&](int off, T (&v)[NF]) {
@@ -218,16 +223,39 @@ __global__ __launch_bounds__(64 * WY) void stencil_kernel(const SArgs<T> a)
             for (int p = 0; p < PY; p++) {
                 const int lr = R + ty * PY + p, lc = R + tx;
                 const T u = q[p][R];
+                // y-neighbour at row offset d: rows of this thread come from
+                // its own registers, the others from the LDS tile (p and d are
+                // compile-time constants after unrolling, so this folds)
+                auto ynb = [&](int d) -> T {
+                    const int pp = p + d;
+                    if (pp >= 0 && pp < PY) return q[pp < 0 ? 0 : (pp < PY ? pp : PY - 1)][R];
+                    return lds[buf][lr + d][lc];
+                };
                 // system.c:195-201 order: centre, then o = 1..6 with
                 // (x-, x+, y-, y+, z-, z+) summed left to right
                 T y;
-                if constexpr (sizeof(T) == 8) {
+                if constexpr (MODE == SMode::AAR_REC) {
+                    // the Richardson step folded into the weights: g' = cf0 g +
+                    // sum_o cf_o S_o, cf0 = 1 - c (3 w0 + sigma), cf_o = -c w_o,
+                    // c = omega/diag (no reference counterpart; the reference
+                    // recomputes b - A x, see the solver)
+                    y = A::mul(u, cf[0]);
+#pragma unroll
+                    for (int o = 1; o <= R; o++) {
+                        T nb = A::add(lds[buf][lr][lc - o], lds[buf][lr][lc + o]);
+                        nb = A::add(nb, ynb(-o));
+                        nb = A::add(nb, ynb(o));
+                        nb = A::add(nb, q[p][R - o]);
+                        nb = A::add(nb, q[p][R + o]);
+                        y = fma(nb, cf[o], y);
+                    }
+                } else if constexpr (sizeof(T) == 8) {
                     y = A::mul(A::mul(u, (T)3), wt0);
 #pragma unroll
                     for (int o = 1; o <= R; o++) {
                         T nb = A::add(lds[buf][lr][lc - o], lds[buf][lr][lc + o]);
-                        nb = A::add(nb, lds[buf][lr - o][lc]);
-                        nb = A::add(nb, lds[buf][lr + o][lc]);
+                        nb = A::add(nb, ynb(-o));
+                        nb = A::add(nb, ynb(o));
                         nb = A::add(nb, q[p][R - o]);
                         nb = A::add(nb, q[p][R + o]);
                         y = A::add(y, A::mul(nb, wt[o]));
@@ -243,54 +271,76 @@ __global__ __launch_bounds__(64 * WY) void stencil_kernel(const SArgs<T> a)
 #pragma unroll
                     for (int o = 1; o <= R; o++) {
                         T nb = A::add(A::sub(lds[buf][lr][lc - o], u), A::sub(lds[buf][lr][lc + o], u));
-                        nb = A::add(nb, A::sub(lds[buf][lr - o][lc], u));
-                        nb = A::add(nb, A::sub(lds[buf][lr + o][lc], u));
+                        nb = A::add(nb, A::sub(ynb(-o), u));
+                        nb = A::add(nb, A::sub(ynb(o), u));
                         nb = A::add(nb, A::sub(q[p][R - o], u));
                         nb = A::add(nb, A::sub(q[p][R + o], u));
                         y = A::add(y, A::mul(nb, wt[o]));
                     }
                 }
-                if (a.sigma != 0.0) y = A::add(y, A::mul(sig, u));
-                if (!c_ok[p]) continue;
+                if (MODE != SMode::AAR_REC && a.sigma != 0.0) y = A::add(y, A::mul(sig, u));
+                // points past the grid edge (partial tiles) computed on wrapped
+                // inputs: only their stores and norm terms are masked, so the
+                // stencil itself runs without branches
+                const bool ok = c_ok[p];
                 const int g = c_off[p] + zoff;
                 if constexpr (MODE == SMode::MATVEC) {
-                    a.out0[g] = y;
+                    if (ok) a.out0[g] = y;
                 } else if constexpr (MODE == SMode::MATVEC_RES) {
                     // y = A x and ||b - A x||^2 (PGR.c:79-81 start-up)
-                    a.out0[g] = y;
-                    acc[0] += A::sq(A::sub(cur.en[p][0], y));
-                } else if constexpr (MODE == SMode::AAR_RICH) {
-                    // AAR.c:80-82/93-94/134-139: f = (rhs - A x)/diag; x_new = x + omega*f
+                    if (ok) a.out0[g] = y;
+                    acc[0] += ok ? A::sq(A::sub(cur.en[p][0], y)) : 0.0;
+                } else if constexpr (MODE == SMode::AAR_TRUE) {
+                    // AAR.c:80-82/93-94/119-123: f = (rhs - A x)/diag, stored in
+                    // T or (mixed precision) rounded to FP32; ||rhs - A x||^2
                     const T r = A::sub(cur.en[p][0], y);
-                    acc[0] += A::sq(r);
+                    acc[0] += ok ? A::sq(r) : 0.0;
                     T f = r;
                     if (a.precond) f = A::div(f, dg);
-                    a.out0[g] = f;
-                    if (a.write_x) a.out1[g] = A::add(u, A::mul(om, f));
+                    if (ok) {
+                        if (a.out32) a.out32[g] = (float)f;
+                        else a.out0[g] = f;
+                    }
+                } else if constexpr (MODE == SMode::AAR_REC) {
+                    // y = g_{k+1} = (rhs - A(x_k + omega g_k))/diag; scale^2 *
+                    // ||g_{k+1}||^2 = ||rhs - A x_{k+1}||^2 for the history
+                    if (ok) a.out0[g] = y;
+                    if (a.trace) {
+                        const double sv = ok ? (double)y * trace_scale : 0.0;
+                        acc[0] = fma(sv, sv, acc[0]);
+                    }
                 } else if constexpr (MODE == SMode::RESID) {
                     const T r = A::sub(cur.en[p][0], y);
-                    acc[0] += A::sq(r);
-                    if (a.out32) a.out32[g] = (float)r;
-                    if (a.out0) a.out0[g] = r;
+                    acc[0] += ok ? A::sq(r) : 0.0;
+                    if (ok) {
+                        if (a.out32) a.out32[g] = (float)r;
+                        if (a.out0) a.out0[g] = r;
+                    }
                 } else if constexpr (MODE == SMode::PGR_RICH) {
                     // PGR.c:95-103: x_old=x; x += omega*f; Ax = A x; f = rhs - Ax
-                    a.out0[g] = u;
-                    a.out1[g] = y;
-                    if (a.trace) acc[0] += A::sq(A::sub(cur.en[p][0], y));
+                    if (ok) {
+                        a.out0[g] = u;
+                        a.out1[g] = y;
+                    }
+                    if (a.trace) acc[0] += ok ? A::sq(A::sub(cur.en[p][0], y)) : 0.0;
                 } else if constexpr (MODE == SMode::CG_Q) {
                     // CG.c:94-100: q = A d; alpha partial d.q
-                    a.out0[g] = u;
-                    a.out1[g] = y;
-                    acc[0] += (double)u * (double)y;
+                    if (ok) {
+                        a.out0[g] = u;
+                        a.out1[g] = y;
+                    }
+                    acc[0] += ok ? (double)u * (double)y : 0.0;
                 } else if constexpr (MODE == SMode::CG_RESID) {
                     // CG.c:65-84 / 112-119: r = b - A x; s = r/diag; r.s, r.r
                     const T r = A::sub(cur.en[p][0], y);
-                    a.out0[g] = r;
                     T s = r;
                     if (a.precond) s = A::div(s, dg);
-                    if (a.out1) a.out1[g] = s;
-                    acc[0] += (double)r * (double)s;
-                    acc[1] += A::sq(r);
+                    if (ok) {
+                        a.out0[g] = r;
+                        if (a.out1) a.out1[g] = s;
+                    }
+                    acc[0] += ok ? (double)r * (double)s : 0.0;
+                    acc[1] += ok ? A::sq(r) : 0.0;
                 }
             }
         // shift the queue one plane
@@ -359,7 +409,8 @@ hipError_t launch_stencil(SMode m, const SArgs<T> &a, int wy, int py, hipStream_
 {
     switch (m) {
     case SMode::MATVEC: return launch_stencil_t<T, SMode::MATVEC>(a, wy, py, st);
-    case SMode::AAR_RICH: return launch_stencil_t<T, SMode::AAR_RICH>(a, wy, py, st);
+    case SMode::AAR_TRUE: return launch_stencil_t<T, SMode::AAR_TRUE>(a, wy, py, st);
+    case SMode::AAR_REC: return launch_stencil_t<T, SMode::AAR_REC>(a, wy, py, st);
     case SMode::RESID: return launch_stencil_t<T, SMode::RESID>(a, wy, py, st);
     case SMode::PGR_RICH: return launch_stencil_t<T, SMode::PGR_RICH>(a, wy, py, st);
     case SMode::CG_Q: return launch_stencil_t<T, SMode::CG_Q>(a, wy, py, st);

@@ -137,6 +137,71 @@ def test_aar_fp32(gpu):
     assert np.linalg.norm(xx - xf) / np.linalg.norm(xf) < 1e-3
 
 
+@pytest.mark.parametrize("name", SMALL_CASES + ["c1_48", "refdrv_48"])
+def test_mixed_parity(gpu, oracle, golden, name):
+    """Mixed-precision AAR (FP32 sweeps and history, FP64 true residual,
+    anchor and normal equations) on the reference's AAR fixtures: the same
+    iteration counts and printed conditions; solution within 1e-8 of the
+    reference's when converged (1e-6 for an iterate stopped at max_iter,
+    whose FP32 sweep noise is not yet damped)."""
+    import torch
+    meta, g = golden
+    c, shape, b, x0 = case_inputs(meta, name)
+    say("mixed parity", name)
+    xd = dev(x0)
+    st = gpu.AAR_mixed(xd, dev(b), shape, c["a"], c["omega"], c["beta"], c["m"], c["p"], c["max_iter"],
+                       c["tol"], sigma=c["sigma"], precond=bool(c["precond"]))
+    ref = meta["cases"][name]["solvers"]["aar"]
+    assert st["iterations"] == ref["iterations"]
+    assert bool(st["printed_converged"]) == ref["converged_line"]
+    assert bool(st["printed_warning"]) == ref["warning_line"]
+    key = f"{name}__aar__x"
+    xr = g[key] if key in g else run_oracle(oracle, "aar", c, shape, b, x0)[0]
+    x = xd.cpu().numpy()
+    nr = np.linalg.norm(xr)
+    if nr == 0.0:
+        assert np.linalg.norm(x) == 0.0
+    else:
+        tol = 1e-8 if ref["converged_line"] else 1e-6
+        assert np.linalg.norm(x - xr) / nr <= tol
+    assert xd.dtype == torch.float64
+
+
+@pytest.mark.parametrize("sigma", [0.0, 0.5])
+def test_mixed_vs_fp64_1e10(gpu, oracle, sigma):
+    """Mixed-precision AAR to relres 1e-10: the true FP64 residual of the
+    returned x meets it, the iteration count is within one Anderson cycle of
+    FP64 AAR's, and the error against the exact (FFT) solution is as small
+    as FP64 AAR's at the same tolerance."""
+    import torch
+    shape = (64, 64, 64)
+    a = P.poisson_a(0.25)
+    b = P.rhs_normal(shape, 2024)
+    if sigma == 0.0:
+        b = b - b.mean()
+    say("mixed 1e-10", sigma)
+    bd = dev(b)
+    xm = torch.zeros_like(bd)
+    st = gpu.AAR_mixed(xm, bd, shape, a, 0.6, 0.6, 7, 6, 20000, 1e-10, sigma=sigma)
+    x64 = torch.zeros_like(bd)
+    st64 = gpu.AAR(x64, bd, shape, a, 0.6, 0.6, 7, 6, 20000, 1e-10, sigma=sigma)
+    say("mixed", st["iterations"], "fp64", st64["iterations"])
+    assert st["converged"] and st64["converged"]
+    assert abs(st["iterations"] - st64["iterations"]) <= 6
+    r = bd - gpu.Lap_Vec_mult(xm, shape, a, sigma)
+    rel = float(torch.linalg.norm(r) / torch.linalg.norm(bd))
+    assert rel <= 1e-10 and abs(rel - st["relres"]) <= 1e-3 * rel
+    xf = fft_solve(b, shape, a, oracle.lap_coefficient(6), sigma, mean=0.0)
+    u, v = xm.cpu().numpy(), x64.cpu().numpy()
+    if sigma == 0.0:
+        u = u - u.mean()
+        v = v - v.mean()
+    e_m = np.linalg.norm(u - xf) / np.linalg.norm(xf)
+    e_64 = np.linalg.norm(v - xf) / np.linalg.norm(xf)
+    say("mixed error vs exact", e_m, "fp64", e_64)
+    assert e_m <= max(2 * e_64, 1e-9)
+
+
 @pytest.mark.parametrize("sigma", [0.0, 0.5])
 def test_refined_vs_fp64(gpu, oracle, sigma):
     """FP32-refined AAR to relres 1e-10 is as accurate as FP64 AAR at the
