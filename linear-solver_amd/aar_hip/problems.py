@@ -134,6 +134,12 @@ def rhs_normal(shape, seed, stream=0):
     return normal(int(np.prod(shape)), seed, stream)
 
 
+def rhs_normal_torch(shape, seed, stream=0, device="cuda"):
+    """Device version of rhs_normal (same uniforms; log/cos of the device
+    library, so equal to the numpy values to rounding, not bitwise)."""
+    return normal_torch(int(np.prod(shape)), seed, stream, device)
+
+
 def helmholtz_coefficient(shape, seed, modes=8, stream=100):
     """c(x) = 1 + 0.5 * (sum of `modes` seeded random Fourier modes,
     normalised to max |.| = 1) on the periodic unit grid."""

@@ -114,39 +114,41 @@ __global__ __launch_bounds__(BLOCK) void gram_kernel(const GramArgs<T> a)
     };
 
     // grid-stride (a fixed point-to-thread assignment, so the reduction
-    // order is fixed); two consecutive points per 16-byte load when N is
-    // even (every slot is then 16-byte aligned)
+    // order is fixed); VW consecutive points per 16-byte load when N is a
+    // multiple of VW (every ring slot is 256-byte aligned).  For the AAR
+    // ring the newest slot is both the age-0 column end and f: one load.
+    constexpr int VW = 16 / sizeof(T);
+    using VT = T __attribute__((ext_vector_type(VW)));
+    const bool f_is_d0 = (KIND == 0) && !dir && a.fslot == a.slot[0];
     const long n = a.n;
     const long stride = (long)gridDim.x * BLOCK;
-    if ((n & 1) == 0) {
-        using V2 = typename Vec2<T>::type;
-        const long np = n >> 1;
-        for (long ip = (long)blockIdx.x * BLOCK + threadIdx.x; ip < np; ip += stride) {
-            const long i = 2 * ip;
-            T vd0[NS], vd1[NS], vx0[NS], vx1[NS];
+    const bool vec_ok = (n % VW) == 0 && (KIND == 0 || ((uintptr_t)a.b % 16) == 0);
+    if (vec_ok) {
+        const long nq = n / VW;
+        for (long iq = (long)blockIdx.x * BLOCK + threadIdx.x; iq < nq; iq += stride) {
+            const long i = VW * iq;
+            VT vd[NS], vx[NS];
 #pragma unroll
             for (int k = 0; k < NS; k++) {
-                vd0[k] = vd1[k] = vx0[k] = vx1[k] = (T)0;
+                vd[k] = (VT)(T)0;
+                vx[k] = (VT)(T)0;
                 if (use[k]) {
-                    const V2 v = *reinterpret_cast<const V2 *>(pd[k] + i);
-                    vd0[k] = v.x;
-                    vd1[k] = v.y;
-                    if constexpr (KIND == 1) {
-                        const V2 w = *reinterpret_cast<const V2 *>(px[k] + i);
-                        vx0[k] = w.x;
-                        vx1[k] = w.y;
-                    }
+                    vd[k] = *reinterpret_cast<const VT *>(pd[k] + i);
+                    if constexpr (KIND == 1) vx[k] = *reinterpret_cast<const VT *>(px[k] + i);
                 }
             }
-            const V2 fv = *reinterpret_cast<const V2 *>(pf + i);
-            T f0 = fv.x, f1 = fv.y;
-            if constexpr (KIND != 0) {
-                const V2 bv = *reinterpret_cast<const V2 *>(a.b + i);
-                f0 = bv.x - fv.x;                                             // PGR.c:102-103
-                f1 = bv.y - fv.y;
+            VT fv = f_is_d0 ? vd[0] : *reinterpret_cast<const VT *>(pf + i);
+            if constexpr (KIND != 0) fv = *reinterpret_cast<const VT *>(a.b + i) - fv;   // PGR.c:102-103
+#pragma unroll
+            for (int e = 0; e < VW; e++) {
+                T sd[NS], sx[NS];
+#pragma unroll
+                for (int k = 0; k < NS; k++) {
+                    sd[k] = vd[k][e];
+                    sx[k] = vx[k][e];
+                }
+                point(fv[e], sd, sx);
             }
-            point(f0, vd0, vx0);
-            point(f1, vd1, vx1);
         }
     } else {
         for (long i = (long)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) {
@@ -193,6 +195,13 @@ hipError_t launch_gram(const GramArgs<T> &a, hipStream_t st)
 {
     if (a.m < 1 || a.m > MAXM || a.nvalid < 0 || a.nvalid > a.m) return hipErrorInvalidValue;
     dim3 grid(a.nblocks), block(BLOCK);
+    // the reference's default history m = 7 gets its own instantiation: 35
+    // accumulators and 8 loads per point instead of MAXM's 44 and 9
+    if (a.kind == 0 && a.m == 7) {
+        hipLaunchKernelGGL((gram_kernel<T, 7, 0>), grid, block, 0, st, a);
+        if (!a.counter) hipLaunchKernelGGL((gram_finish_kernel<T, 7, 0>), dim3(1), block, 0, st, a);
+        return hipGetLastError();
+    }
     switch (a.kind) {
     case 0: hipLaunchKernelGGL((gram_kernel<T, MAXM, 0>), grid, block, 0, st, a); break;
     case 1: hipLaunchKernelGGL((gram_kernel<T, MAXM, 1>), grid, block, 0, st, a); break;
