@@ -121,12 +121,15 @@ int make_op(const aar_hip_grid *g, const double *coeff, Op &op)
 // ---------------------------------------------------------------------------
 // launch helpers
 // ---------------------------------------------------------------------------
-static int auto_lz(const Op &op, int wy, int py)
+static int auto_lz(const Op &op, int wy, int py, size_t es)
 {
     const int ntx = (op.nx + TX - 1) / TX, nty = (op.ny + wy * py - 1) / (wy * py);
     const int cols = ntx * nty;
-    // long z-marches (about one block per CU) amortise the 2R-plane warm-up
-    int nzc = std::max(1, 256 / std::max(1, cols));
+    // long z-marches amortise the 2R-plane warm-up: about one block per CU
+    // for FP64 (its plane tile and halo ring take most of the LDS, so only
+    // one block fits per CU) and two for FP32 (two fit; measured faster)
+    const int target = es == 4 ? 512 : 256;
+    int nzc = std::max(1, target / std::max(1, cols));
     int lz = (op.nz + nzc - 1) / nzc;
     lz = std::max(lz, std::min(op.nz, 16));
     return lz;
@@ -140,7 +143,7 @@ SArgs<T> base_sargs(const Op &op)
     a.nx = op.nx;
     a.ny = op.ny;
     a.nz = op.nz;
-    a.lz = C.lz > 0 ? C.lz : auto_lz(op, C.wy, C.py);
+    a.lz = C.lz > 0 ? C.lz : auto_lz(op, C.wy, C.py, sizeof(T));
     for (int o = 0; o <= R; o++) a.w[o] = op.w[o];
     a.sigma = op.sigma;
     a.diag = op.diag;
@@ -391,7 +394,8 @@ int aar_rec(const Op &op, Tx *x, const Tx *b, double omega, double beta, int m, 
     DevBuf &HB = ci == 0 ? C.hist0 : C.hist1;
     HC(ensure(HB, sizeof(double) * (size_t)(max_iter + 4)));
     const int nb = nblocks_for(n);
-    const size_t need = std::max<size_t>((size_t)stencil_partials_needed<Tg>(op), (size_t)nb * 80);
+    const size_t need = std::max<size_t>(
+        std::max((size_t)stencil_partials_needed<Tg>(op), (size_t)stencil_partials_needed<Tx>(op)), (size_t)nb * 80);
     DevBuf &PB = ci == 0 ? C.part0 : C.part1;
     HC(ensure(PB, sizeof(double) * need));
     Tg *G = (Tg *)RG.p;

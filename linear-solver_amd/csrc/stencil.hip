@@ -13,8 +13,9 @@
 // * z-neighbours come from a per-thread register queue of 13 planes, so
 //   every plane of the column is read from HBM once per chunk;
 // * x/y-neighbours come from a double-buffered LDS tile of the current
-//   plane with a 6-wide halo (corners are not needed by a star stencil);
-//   the halo is fetched one plane ahead, one barrier per plane;
+//   plane with a 6-wide halo (corners are not needed by a star stencil),
+//   one barrier per plane; halo cells are fetched R planes ahead, together
+//   with the queue plane, and parked in a per-thread LDS ring;
 // * block ids are mapped XCD-aware onto tiles so that y-adjacent tiles
 //   (whose halos overlap) run on the same XCD and share its L2.
 // The epilogue of each mode fuses the pointwise work of the solver step
@@ -70,7 +71,14 @@ __global__ __launch_bounds__(64 * WY) void stencil_kernel(const SArgs<T> a)
     constexpr int NXH = (2 * R * TY + NT - 1) / NT;   // x-halo cells per thread
     constexpr int QN = 2 * R + 1;
 
+    constexpr int HR = R + 1;                         // halo ring depth (planes)
+    constexpr int NH = NYH + NXH;                     // halo cells per thread
     __shared__ T lds[2][LH][LW];
+    // Halo cells are fetched R planes ahead, in the same step as the
+    // neighbouring blocks fetch those lines as their own queue planes, so
+    // that they are served from L2; each thread parks its cells here until
+    // their plane is computed (slots private to the thread: no barrier).
+    __shared__ T hring[HR][NH][NT];
     __shared__ double red_scratch[8 * WY];
     __shared__ double red_out[8];
 
@@ -181,16 +189,34 @@ __global__ __launch_bounds__(64 * WY) void stencil_kernel(const SArgs<T> a)
             q[p][s] = make_u(v);
         }
     }
-    // halo / epilogue inputs of plane zp and queue plane zp+R (indices wrap,
-    // so the prefetch past the last plane is a harmless unused load)
+    // halo cells of planes z0 .. z0+R-1 into the ring
+#pragma unroll
+    for (int s = 0; s < R; s++) {
+        const int zo = modp(z0 + s, nz) * nxy;
+#pragma unroll
+        for (int k = 0; k < NYH; k++) {
+            T v[NF];
+            load_raw(yh_off[k] + zo, v);
+            hring[s][k][tid] = make_u(v);
+        }
+#pragma unroll
+        for (int k = 0; k < NXH; k++) {
+            T v[NF];
+            load_raw(xh_off[k] + zo, v);
+            hring[s][NYH + k][tid] = make_u(v);
+        }
+    }
+    // queue plane and halo cells of plane zp+R, epilogue inputs of plane zp
+    // (indices wrap, so the prefetch past the last plane is a harmless
+    // unused load)
     auto prefetch = [&](int zp, Pre &st) {
         const int zq = modp(zp + R, nz) * nxy, zc = modp(zp, nz) * nxy;
 #pragma unroll
         for (int p = 0; p < PY; p++) load_raw(c_off[p] + zq, st.qn[p]);
 #pragma unroll
-        for (int k = 0; k < NYH; k++) load_raw(yh_off[k] + zc, st.hy[k]);
+        for (int k = 0; k < NYH; k++) load_raw(yh_off[k] + zq, st.hy[k]);
 #pragma unroll
-        for (int k = 0; k < NXH; k++) load_raw(xh_off[k] + zc, st.hx[k]);
+        for (int k = 0; k < NXH; k++) load_raw(xh_off[k] + zq, st.hx[k]);
         if constexpr (NE > 0) {
 #pragma unroll
             for (int p = 0; p < PY; p++) st.en[p][0] = a.bvec[c_off[p] + zc];
@@ -209,12 +235,18 @@ __global__ __launch_bounds__(64 * WY) void stencil_kernel(const SArgs<T> a)
             q[p][2 * R] = make_u(cur.qn[p]);
             lds[buf][R + ty * PY + p][R + tx] = q[p][R];
         }
+        // halo of plane z from the ring; park the cells of plane z+R
+        const int rs = (z - z0) % HR, ws = (z - z0 + R) % HR;
 #pragma unroll
         for (int k = 0; k < NYH; k++)
-            if (yh_ok[k]) lds[buf][yh_row[k]][R + tx] = make_u(cur.hy[k]);
+            if (yh_ok[k]) lds[buf][yh_row[k]][R + tx] = hring[rs][k][tid];
 #pragma unroll
         for (int k = 0; k < NXH; k++)
-            if (xh_ok[k]) lds[buf][xh_row[k]][xh_col[k]] = make_u(cur.hx[k]);
+            if (xh_ok[k]) lds[buf][xh_row[k]][xh_col[k]] = hring[rs][NYH + k][tid];
+#pragma unroll
+        for (int k = 0; k < NYH; k++) hring[ws][k][tid] = make_u(cur.hy[k]);
+#pragma unroll
+        for (int k = 0; k < NXH; k++) hring[ws][NYH + k][tid] = make_u(cur.hx[k]);
         prefetch(z + 1, nxt);
         __syncthreads();
 
