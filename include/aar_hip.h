@@ -93,6 +93,36 @@ int aar_hip_aar_refined(const aar_hip_grid *g, double *x, const double *b, doubl
                         double beta, int m, int p, int max_outer, int max_inner, double tol,
                         double inner_tol, int precond, aar_hip_stats *st);
 
+/* Complex Helmholtz (extension, no MPI_REAL counterpart; the complex AAR of
+ * PETSC_COMPLEX/AAR_Complex.c:103-246 with a Jacobi preconditioner):
+ *   A u = a*Lap_FD12(u) + (c(x) + i*eta) u,  periodic,
+ * vectors complex128 interleaved (re, im) in device memory, N = nx*ny*nz
+ * points.  c is a device array of N doubles, or NULL for c = c0
+ * everywhere.  Jacobi diagonal D = (a*3*coeff[0] + c(x)) + i*eta. */
+typedef struct {
+    int nx, ny, nz;
+    int fdn;            /* 6 */
+    double a;
+    double c0;
+    double eta;
+    const double *c;
+} aar_hip_helm;
+
+/* y = A u */
+int aar_hip_helm_matvec(const aar_hip_helm *h, const double *u, double *y);
+
+/* AAR on A u = b (adjoint = 0) or on A^H u = b (adjoint = 1: eta -> -eta),
+ * the control flow of AAR.c:37-177 with the Hermitian Gram DF^H DF, DF^H f
+ * solved through the real 2m x 2m embedding by the pinv of tools.c;
+ * m <= 8.  u: initial guess in, solution out. */
+int aar_hip_helm_aar(const aar_hip_helm *h, double *u, const double *b, double omega, double beta,
+                     int m, int p, int max_iter, double tol, int adjoint, aar_hip_stats *st);
+
+/* Gradient of a loss through u = A(c)^{-1} b with respect to the diagonal
+ * coefficient field c, given the adjoint solution lambda of A^H lambda =
+ * dL/du:  dc[k] = -Re(conj(lambda[k]) u[k]).  (dL/db = lambda.) */
+int aar_hip_helm_grad_c(long n, const double *lambda, const double *u, double *dc);
+
 /* tools.c:52-105 on the GPU: x = pinv(A) b as the reference defines it,
  * A row-major m x m (A[i*m+j] = A[i][j]), m <= 16.  HOST pointers.
  * w_out, v_out may be NULL. */

@@ -4,3 +4,4 @@ of libaar_hip.so.  See solvers.py for the reference-shaped interface."""
 from .solvers import (AAR, CG, PGR, PL2R, AAR_mixed, AAR_refined, Lap_Vec_mult, history, pinv, profile,  # noqa: F401
                       profile_query, profile_reset, release, set_option, set_tuning)
 from . import problems  # noqa: F401
+from .helmholtz import HelmholtzSolve, helm_aar, helm_grad_c, helm_matvec  # noqa: F401

@@ -54,6 +54,7 @@ EXTENSION_SYMBOLS = [
     "aar_hip_cg", "aar_hip_aar_refined", "aar_hip_pinv", "aar_hip_history", "aar_hip_last_stats",
     "aar_hip_set_stream", "aar_hip_set_tuning", "aar_hip_set_option", "aar_hip_profile", "aar_hip_profile_reset",
     "aar_hip_profile_query", "aar_hip_release", "aar_hip_last_error",
+    "aar_hip_helm_matvec", "aar_hip_helm_aar", "aar_hip_helm_grad_c",
 ]
 
 
@@ -97,6 +98,9 @@ def load(require_gpu=True):
         "aar_hip_profile_query": [ctypes.c_char_p, ctypes.POINTER(I), ctypes.POINTER(D),
                                   ctypes.POINTER(D)],
         "aar_hip_release": [],
+        "aar_hip_helm_matvec": [P, P, P],
+        "aar_hip_helm_aar": [P, P, P, D, D, I, I, I, D, I, S],
+        "aar_hip_helm_grad_c": [ctypes.c_long, P, P, P],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

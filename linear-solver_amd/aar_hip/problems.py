@@ -140,6 +140,13 @@ def rhs_normal_torch(shape, seed, stream=0, device="cuda"):
     return normal_torch(int(np.prod(shape)), seed, stream, device)
 
 
+def rhs_complex_normal(shape, seed, stream=0):
+    """complex128 b with real and imaginary parts i.i.d. N(0,1)."""
+    n = int(np.prod(shape))
+    v = normal(2 * n, seed, stream)
+    return (v[0::2] + 1j * v[1::2]).astype(np.complex128)
+
+
 def helmholtz_coefficient(shape, seed, modes=8, stream=100):
     """c(x) = 1 + 0.5 * (sum of `modes` seeded random Fourier modes,
     normalised to max |.| = 1) on the periodic unit grid."""

@@ -193,6 +193,76 @@ struct CgUpdArgs {
 };
 hipError_t launch_cg_update(const CgUpdArgs &a, hipStream_t st);
 
+// ---------------------------------------------------------------------------
+// complex Helmholtz (no MPI_REAL counterpart): A u = a*Lap(u) + (c(x) + i eta) u,
+// complex128 interleaved (double2 = re, im), Jacobi D = (a*3*coeff[0] + c) + i eta
+// ---------------------------------------------------------------------------
+enum class HMode : int {
+    MATVEC = 0,    // y = A u
+    TRUE_RES = 1,  // f = b - A u; g = f / D; ||f||^2
+    REC = 2,       // g' = g - omega (A g) / D; ||D g'||^2 for the history
+};
+
+struct HArgs {
+    int nx = 0, ny = 0, nz = 0, nxy = 0;
+    int lz = 0;
+    int ntx = 0, nty = 0, nzc = 0, nitems = 0;
+    double w[R + 1] = {};
+    double dlap = 0.0;                // a * 3 * coeff[0]
+    const double *c = nullptr;        // c(x) (N) or null: c0 everywhere
+    double c0 = 0.0, eta = 0.0, omega = 0.0;
+    int trace = 0;
+    const double2 *in0 = nullptr, *bvec = nullptr;
+    double2 *out0 = nullptr;
+    double *partials = nullptr;       // [nitems]
+    Ctrl *ctrl = nullptr;
+    FinArgs fin = {};
+};
+hipError_t launch_hstencil(HMode m, const HArgs &a, hipStream_t st);
+int hstencil_items(const HArgs &a);
+
+// Hermitian Gram of the complex g ring (ages as in GramArgs): G_ij = <DF_i, DF_j>,
+// g_i = <DF_i, f> with <u, v> = sum conj(u) v, laid out as the real 2m x 2m
+// system [Gr -Gi; Gi Gr] [s_re; s_im] = [g_re; g_im] for the m x m solve.
+struct HGramArgs {
+    long n = 0;
+    int m = 0;
+    const double2 *F = nullptr;
+    long ld = 0;
+    int slot[MAXM + 1] = {};
+    int nvalid = 0;
+    int it = 0;
+    int nblocks = 0;
+    double *partials = nullptr;
+    Ctrl *ctrl = nullptr;
+};
+hipError_t launch_hgram(const HGramArgs &a, hipStream_t st);
+
+// complex anchor update: AncArgs with complex gamma = s[t] + i s[m + t]
+struct HAncArgs {
+    long n = 0;
+    double2 *x = nullptr;
+    const double2 *G = nullptr;
+    double2 *C = nullptr;
+    long ld = 0;
+    int m = 0;
+    int ng = 0, nc = 0, ncol = 0;
+    int gslot[NGMAX] = {};
+    int xlo = 0, xhi = 0;
+    int with_beta = 0;
+    int cslot[NCMAX] = {};
+    int col_ref[MAXM] = {};
+    int col_j[MAXM] = {};
+    int col_c[MAXM] = {};
+    double omega = 0.0, beta = 0.0;
+    int cout = -1;
+    const Ctrl *ctrl = nullptr;
+};
+hipError_t launch_hanchor_update(const HAncArgs &a, hipStream_t st);
+
+// dL/dc = -Re(conj(lambda) u) for a diagonal coefficient field c
+hipError_t launch_helm_grad_c(long n, const double2 *lambda, const double2 *u, double *dc, hipStream_t st);
+
 // Reducing kernels run the finaliser in their last block when given a
 // counter, or only write per-block partials when counter == nullptr; then
 // launch_finish (or, for the Gram, launch_gram itself) runs a one-block

@@ -46,6 +46,7 @@ struct Context {
     DevBuf ring64x, ring64f, ring32x, ring32f;
     DevBuf hist0, hist1, part0, part1;
     DevBuf vec32a, vec32b, vec64a, vec64b;
+    DevBuf ringc, corrc;         // complex Helmholtz g ring and corrections
     std::string err;
     int lz = 0, wy = 8, py = 4;
     long ring_pad = 512;           // extra elements per ring slot (multiple of 32)
@@ -88,6 +89,19 @@ int matvec(const Op &op, const T *x, T *y);
 int aar_refined(const Op &op, double *x, const double *b, double omega, double beta, int m, int p,
                 int max_outer, int max_inner, double tol, double inner_tol, int precond,
                 aar_hip_stats *st);
+// complex Helmholtz operator: a*Lap + (c(x) + i eta), complex128 interleaved
+struct HOp {
+    int nx = 0, ny = 0, nz = 0;
+    long n = 0;
+    double w[R + 1] = {};
+    double dlap = 0.0;
+    const double *c = nullptr;
+    double c0 = 0.0, eta = 0.0;
+};
+int make_hop(const aar_hip_helm *h, HOp &op);
+int helm_matvec(const HOp &op, const double *u, double *y);
+int helm_solve(const HOp &op, double *u, const double *b, double omega, double beta, int m, int p, int max_iter,
+               double tol, aar_hip_stats *st);
 int pinv_host(const double *A, const double *b, double *x, int m, double *w_out, double *v_out);
 void release_all();
 void prof_flush();

@@ -719,6 +719,32 @@ int aar_hip_aar_refined(const aar_hip_grid *g, double *x, const double *b, doubl
     return aar_refined(op, x, b, omega, beta, m, p, max_outer, max_inner, tol, inner_tol, precond, st);
 }
 
+int aar_hip_helm_matvec(const aar_hip_helm *h, const double *u, double *y)
+{
+    HOp op;
+    if (int rc = make_hop(h, op)) return rc;
+    return helm_matvec(op, u, y);
+}
+
+int aar_hip_helm_aar(const aar_hip_helm *h, double *u, const double *b, double omega, double beta, int m, int p,
+                     int max_iter, double tol, int adjoint, aar_hip_stats *st)
+{
+    HOp op;
+    if (int rc = make_hop(h, op)) return rc;
+    if (adjoint) op.eta = -op.eta;        // A^H = a Lap + (c - i eta)
+    return helm_solve(op, u, b, omega, beta, m, p, max_iter, tol, st);
+}
+
+int aar_hip_helm_grad_c(long n, const double *lambda, const double *u, double *dc)
+{
+    if (int rc = context_init()) return rc;
+    hipError_t e = launch_helm_grad_c(n, (const double2 *)lambda, (const double2 *)u, dc, ctx().stream);
+    if (e != hipSuccess) return fail(AAR_HIP_ERR_ARG, std::string("gradient: ") + hipGetErrorString(e));
+    e = hipStreamSynchronize(ctx().stream);
+    if (e != hipSuccess) return fail(AAR_HIP_ERR_HIP, std::string("gradient: ") + hipGetErrorString(e));
+    return 0;
+}
+
 int aar_hip_pinv(const double *A, const double *b, double *x, int m, double *w_out, double *v_out)
 {
     if (!A || !b || !x) return fail(AAR_HIP_ERR_ARG, "null pointer");

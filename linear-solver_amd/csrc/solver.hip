@@ -938,6 +938,291 @@ template int matvec<double>(const Op &, const double *, double *);
 template int matvec<float>(const Op &, const float *, float *);
 
 // ---------------------------------------------------------------------------
+// Complex Helmholtz AAR (extension; AAR.c:37-177 control flow with the
+// Hermitian Gram of PETSC_COMPLEX/AAR_Complex.c:207-246) by the same residual
+// recurrence as aar_rec: with the Jacobi diagonal D(x) = (a*3*c0 + c(x)) +
+// i eta, a Richardson step changes g = D^{-1}(b - A u) by
+// g' = g - omega D^{-1} A g, pointwise.
+// ---------------------------------------------------------------------------
+int make_hop(const aar_hip_helm *h, HOp &op)
+{
+    if (!h) return fail(AAR_HIP_ERR_ARG, "null operator");
+    if (h->fdn != R) return fail(AAR_HIP_ERR_ARG, "only fdn = 6 (FD12) is built");
+    if (h->nx < 1 || h->ny < 1 || h->nz < 1) return fail(AAR_HIP_ERR_ARG, "grid sizes must be positive");
+    if ((double)h->nx * h->ny * h->nz > 1073741823.0) return fail(AAR_HIP_ERR_ARG, "grid too large");
+    double c[R + 1];
+    lap_coefficient(c, R);
+    op.nx = h->nx;
+    op.ny = h->ny;
+    op.nz = h->nz;
+    op.n = (long)h->nx * h->ny * h->nz;
+    for (int o = 0; o <= R; o++) op.w[o] = h->a * c[o];
+    op.dlap = h->a * (3 * c[0]);
+    op.c = h->c;
+    op.c0 = h->c0;
+    op.eta = h->eta;
+    return 0;
+}
+
+static HArgs base_hargs(const HOp &op)
+{
+    HArgs a;
+    a.nx = op.nx;
+    a.ny = op.ny;
+    a.nz = op.nz;
+    for (int o = 0; o <= R; o++) a.w[o] = op.w[o];
+    a.dlap = op.dlap;
+    a.c = op.c;
+    a.c0 = op.c0;
+    a.eta = op.eta;
+    return a;
+}
+
+static const char *hmode_name(HMode m)
+{
+    switch (m) {
+    case HMode::MATVEC: return "helm_matvec";
+    case HMode::TRUE_RES: return "helm_true";
+    case HMode::REC: return "helm_rec";
+    }
+    return "helm";
+}
+
+static int run_hstencil(HMode mode, const HArgs &a, double bytes)
+{
+    Context &C = ctx();
+    prof_begin(hmode_name(mode));
+    hipError_t e = launch_hstencil(mode, a, C.stream);
+    prof_end(bytes);
+    if (e == hipSuccess && a.partials)
+        e = launch_finish(a.partials, hstencil_items(a), 1, a.fin, a.ctrl, C.stream);
+    if (e != hipSuccess) return fail(AAR_HIP_ERR_HIP, std::string("helmholtz stencil launch: ") + hipGetErrorString(e));
+    C.bytes_acc += bytes;
+    return 0;
+}
+
+int helm_matvec(const HOp &op, const double *u, double *y)
+{
+    Context &C = ctx();
+    if (int rc = context_init()) return rc;
+    if (!u || !y) return fail(AAR_HIP_ERR_ARG, "null vector");
+    HArgs a = base_hargs(op);
+    a.in0 = (const double2 *)u;
+    a.out0 = (double2 *)y;
+    if (int rc = run_hstencil(HMode::MATVEC, a, 32.0 * op.n + (op.c ? 8.0 * op.n : 0.0))) return rc;
+    HC(hipStreamSynchronize(C.stream));
+    prof_flush();
+    return 0;
+}
+
+int helm_solve(const HOp &op, double *u, const double *b, double omega, double beta, int m, int p, int max_iter,
+               double tol, aar_hip_stats *st)
+{
+    Context &C = ctx();
+    if (int rc = context_init()) return rc;
+    if (!u || !b) return fail(AAR_HIP_ERR_ARG, "null vector");
+    if (m < 1 || m > MAXM) return fail(AAR_HIP_ERR_ARG, "history m must be in 1..8");
+    if (p < 1) return fail(AAR_HIP_ERR_ARG, "p must be >= 1");
+    if (max_iter < 0) return fail(AAR_HIP_ERR_ARG, "max_iter must be >= 0");
+    const long n = op.n;
+    const int S = m + 2;
+    const int NC = (p <= m) ? m / p + 1 : 0;
+    const long ld = ring_ld(n, 16);
+    HC(ensure(C.ringc, 16 * (size_t)ld * S));
+    if (NC > 0) HC(ensure(C.corrc, 16 * (size_t)ld * NC));
+    HC(ensure(C.hist0, sizeof(double) * (size_t)(max_iter + 4)));
+    const int nb = nblocks_for(n);
+    HArgs probe = base_hargs(op);
+    const size_t need = std::max<size_t>((size_t)hstencil_items(probe), (size_t)nb * 96);
+    HC(ensure(C.part0, sizeof(double) * need));
+    double2 *G = (double2 *)C.ringc.p;
+    double2 *CR = NC > 0 ? (double2 *)C.corrc.p : nullptr;
+    double2 *x = (double2 *)u;
+    const double2 *bb = (const double2 *)b;
+    double *hist = (double *)C.hist0.p, *part = (double *)C.part0.p;
+    Ctrl *ctrl = C.ctrl;
+    unsigned *cnt = C.counters;
+    auto gs = [&](long k) { return (int)(k % S); };
+    auto gptr = [&](long k) { return G + (size_t)gs(k) * ld; };
+    auto is_and = [&](long k) { return k > 1 && k % p == 0; };
+    auto next_and = [&](long k) { long q = (k / p + 1) * p; return q > 1 ? q : q + p; };
+    const double sv = 16.0 * n, sc = op.c ? 8.0 * n : 0.0;
+    C.bytes_acc = 0.0;
+
+    auto true_res = [&](long k, int check) -> int {
+        HArgs a = base_hargs(op);
+        a.in0 = x;
+        a.bvec = bb;
+        a.out0 = gptr(k);
+        a.partials = part;
+        a.ctrl = ctrl;
+        a.fin.fin = FIN_AAR_RES;
+        a.fin.it = (int)k;
+        a.fin.check = check;
+        a.fin.hist = hist;
+        a.fin.hist_cap = max_iter + 4;
+        return run_hstencil(HMode::TRUE_RES, a, 3 * sv + sc);
+    };
+    auto anchor = [&](HAncArgs &hu, double bytes) -> int {
+        prof_begin("helm_anchor_update");
+        hipError_t e = launch_hanchor_update(hu, C.stream);
+        prof_end(bytes);
+        if (e != hipSuccess) return fail(AAR_HIP_ERR_HIP, std::string("helmholtz update: ") + hipGetErrorString(e));
+        C.bytes_acc += bytes;
+        return 0;
+    };
+
+    HC(launch_zero_ctrl(ctrl, C.stream));
+    HC(hipEventRecord(C.t0, C.stream));
+    {
+        DotArgs<double> d;              // ||b||^2 over the 2n reals
+        d.n = 2 * n;
+        d.v = b;
+        d.nblocks = nb;
+        d.partials = part;
+        d.counter = cnt + 0;
+        d.ctrl = ctrl;
+        d.fin.fin = FIN_NORM_B;
+        d.fin.tol = tol;
+        d.fin.check = 1;
+        HC(run_dot<double>(d));
+        C.bytes_acc += sv;
+    }
+    if (int rc = true_res(1, 0)) return rc;
+    long first = 1;
+    int cyc = 0;
+    long it = 1;
+    bool stopped = false;
+    for (; it <= max_iter && !stopped; it++) {
+        if (!is_and(it)) {
+            HArgs a = base_hargs(op);
+            a.omega = omega;
+            a.in0 = gptr(it);
+            a.out0 = gptr(it + 1);
+            a.trace = 1;
+            a.partials = part;
+            a.ctrl = ctrl;
+            a.fin.fin = FIN_AAR_RES;
+            a.fin.it = (int)it + 1;
+            a.fin.check = 0;
+            a.fin.hist = hist;
+            a.fin.hist_cap = max_iter + 4;
+            if (int rc = run_hstencil(HMode::REC, a, 2 * sv + sc)) return rc;
+            if (it < next_and(it) - m) {
+                HAncArgs hu;
+                hu.n = n;
+                hu.x = x;
+                hu.G = G;
+                hu.ld = ld;
+                hu.m = m;
+                hu.ng = 1;
+                hu.gslot[0] = gs(it);
+                hu.xlo = 0;
+                hu.xhi = 1;
+                hu.omega = omega;
+                hu.ctrl = ctrl;
+                if (int rc = anchor(hu, 3 * sv)) return rc;
+                first = it + 1;
+            }
+            continue;
+        }
+        const int nv = (int)std::min<long>(m, it - 1);
+        {
+            HGramArgs g;
+            for (int k = 0; k <= m; k++) g.slot[k] = it - k >= 1 ? gs(it - k) : 0;
+            g.n = n;
+            g.m = m;
+            g.nvalid = nv;
+            g.it = (int)it;
+            g.F = G;
+            g.ld = ld;
+            g.nblocks = nb;
+            g.partials = part;
+            g.ctrl = ctrl;
+            prof_begin("helm_gram");
+            HC(launch_hgram(g, C.stream));
+            prof_end(sv * (nv + 1));
+            C.bytes_acc += sv * (nv + 1);
+        }
+        prof_begin("ls");
+        HC(launch_ls(ctrl, 2 * m, 1, C.stream));
+        prof_end(0);
+        {
+            HAncArgs hu;
+            const long jlo = std::min(first, it - nv);
+            hu.n = n;
+            hu.x = x;
+            hu.G = G;
+            hu.C = CR;
+            hu.ld = ld;
+            hu.m = m;
+            hu.ng = (int)(it - jlo + 1);
+            for (int i = 0; i < hu.ng; i++) hu.gslot[i] = gs(jlo + i);
+            hu.xlo = (int)(first - jlo);
+            hu.xhi = (int)(it - jlo);
+            hu.with_beta = 1;
+            hu.ncol = nv;
+            hu.nc = 0;
+            for (int t = 0; t < nv; t++) {
+                const long j = it - t;
+                hu.col_ref[t] = (int)((j - 2) % m);
+                hu.col_j[t] = (int)(j - jlo);
+                hu.col_c[t] = -1;
+                if (is_and(j - 1)) {
+                    hu.cslot[hu.nc] = (int)(((j - 1) / p) % NC);
+                    hu.col_c[t] = hu.nc++;
+                }
+            }
+            hu.cout = NC > 0 ? (int)((it / p) % NC) : -1;
+            hu.omega = omega;
+            hu.beta = beta;
+            hu.ctrl = ctrl;
+            if (int rc = anchor(hu, sv * (2 + hu.ng + hu.nc + (hu.cout >= 0 ? 1 : 0)))) return rc;
+        }
+        first = it + 1;
+        if (int rc = true_res(it + 1, 1)) return rc;
+        if (int rc = poll_record(ctrl, cyc)) return rc;
+        if (poll_check(cyc - 2)) stopped = true;
+        cyc++;
+    }
+    if (!stopped && first <= max_iter) {
+        HAncArgs hu;
+        hu.n = n;
+        hu.x = x;
+        hu.G = G;
+        hu.ld = ld;
+        hu.m = m;
+        hu.ng = (int)(max_iter - first + 1);
+        for (int i = 0; i < hu.ng; i++) hu.gslot[i] = gs(first + i);
+        hu.xlo = 0;
+        hu.xhi = hu.ng;
+        hu.omega = omega;
+        hu.ctrl = ctrl;
+        if (int rc = anchor(hu, sv * (2 + hu.ng))) return rc;
+    }
+    HC(hipEventRecord(C.t1, C.stream));
+    Ctrl &hc = C.ctrl_h[0];
+    if (int rc = finish_ctrl(ctrl, &hc)) return rc;
+    float ms = 0.f;
+    HC(hipEventElapsedTime(&ms, C.t0, C.t1));
+    prof_flush();
+    int iterations, loop_iter;
+    if (hc.done) {
+        iterations = hc.final_iter;
+        loop_iter = hc.final_iter + 1;
+    } else {
+        iterations = max_iter;
+        loop_iter = max_iter + 1;
+    }
+    const int hl = std::min(iterations + 1, max_iter + 4);
+    if (int rc = store_history(hist, hl)) return rc;
+    fill_stats(st, hc, iterations, loop_iter, max_iter, ms, C.bytes_acc, hl);
+    if (hc.status == 1) return fail(AAR_HIP_ERR_SVD, "no convergence in 250 svd iterations");
+    if (hc.status == 2) return fail(AAR_HIP_ERR_NONFINITE, "residual is not finite");
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // FP64 residual refinement around FP32 AAR (extension)
 // ---------------------------------------------------------------------------
 int aar_refined(const Op &op, double *x, const double *b, double omega, double beta, int m, int p,
@@ -1074,7 +1359,8 @@ void release_all()
 {
     Context &C = ctx();
     DevBuf *bufs[] = {&C.ring64x, &C.ring64f, &C.ring32x, &C.ring32f, &C.hist0, &C.hist1,
-                      &C.part0, &C.part1, &C.vec32a, &C.vec32b, &C.vec64a, &C.vec64b};
+                      &C.part0, &C.part1, &C.vec32a, &C.vec32b, &C.vec64a, &C.vec64b,
+                      &C.ringc, &C.corrc};
     for (DevBuf *b : bufs) {
         if (b->p) (void)hipFree(b->p);
         b->p = nullptr;

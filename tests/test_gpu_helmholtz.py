@@ -1,0 +1,145 @@
+"""GPU complex Helmholtz solves, adjoint and gradients against the oracle.
+
+A u = a Lap_FD12 u + (c(x) + i eta) u, periodic.  The complex matvec is
+bit-exact; AAR iteration counts (forward and adjoint) equal the oracle's,
+solutions within 1e-8; gradients of L = 1/2 ||u - u*||^2 through the
+torch.autograd.Function harness match central finite differences (step
+1e-6, 20 random directions) to 1e-6 relative, and the adjoint solution
+matches the oracle's adjoint solve.
+"""
+import numpy as np
+import pytest
+
+from aar_hip import problems as P
+from test_oracle import fft_solve_complex
+
+pytestmark = pytest.mark.gpu
+
+ETA = 0.5
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def cdev(v):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(v, dtype=np.complex128)).to("cuda")
+
+
+def rdev(v):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64)).to("cuda")
+
+
+@pytest.mark.parametrize("shape", [(24, 24, 24), (20, 12, 16), (70, 33, 17)])
+def test_helm_matvec_bitwise(gpu, oracle, shape):
+    say("helm matvec", shape)
+    n = int(np.prod(shape))
+    rng = np.random.default_rng(11)
+    u = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    c = P.helmholtz_coefficient(shape, 7)
+    y = gpu.helm_matvec(cdev(u), shape, -4.0, rdev(c), 0.0, ETA).cpu().numpy()
+    assert np.array_equal(y, oracle.matvec_complex(u, shape, -4.0, c, 0.0, ETA))
+    y0 = gpu.helm_matvec(cdev(u), shape, -4.0, None, 1.25, ETA).cpu().numpy()
+    assert np.array_equal(y0, oracle.matvec_complex(u, shape, -4.0, None, 1.25, ETA))
+
+
+CASES = [
+    # shape, h, m, p, max_iter, tol
+    ((16, 16, 16), 0.5, 7, 6, 2000, 1e-8),
+    ((24, 24, 24), 0.5, 7, 6, 2000, 1e-10),
+    ((20, 12, 16), 0.5, 3, 5, 2000, 1e-8),
+    ((16, 16, 16), 0.5, 4, 1, 2000, 1e-8),
+    ((16, 16, 16), 0.5, 7, 6, 13, 1e-8),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "x".join(map(str, c[0])) + f"_m{c[2]}p{c[3]}i{c[4]}")
+@pytest.mark.parametrize("adjoint", [False, True])
+def test_helm_aar_parity(gpu, oracle, case, adjoint):
+    shape, h, m, p, max_iter, tol = case
+    say("helm aar", shape, m, p, max_iter, "adjoint" if adjoint else "")
+    a = P.poisson_a(h)
+    c = P.helmholtz_coefficient(shape, 1005)
+    b = P.rhs_complex_normal(shape, 1005)
+    eta = -ETA if adjoint else ETA
+    ur, dr = oracle.aar_complex(np.zeros_like(b), b, shape, a, c, 0.0, eta, 0.6, 0.6, m, p, max_iter, tol)
+    ud = cdev(np.zeros_like(b))
+    st = gpu.helm_aar(ud, cdev(b), shape, a, rdev(c), 0.0, ETA, 0.6, 0.6, m, p, max_iter, tol, adjoint=adjoint)
+    assert st["iterations"] == dr["iterations"]
+    assert bool(st["printed_converged"]) == bool(dr["printed_converged"])
+    u = ud.cpu().numpy()
+    tol_x = 1e-8 if dr["printed_converged"] else 1e-6
+    assert np.linalg.norm(u - ur) / np.linalg.norm(ur) <= tol_x
+    h_gpu = gpu.history()
+    tr = dr["trace"]
+    assert len(h_gpu) == len(tr)
+    assert np.all(np.abs(h_gpu - tr) <= 1e-6 * tr + 1e-10 * tr[0])
+
+
+def test_helm_vs_fft(gpu, oracle):
+    shape = (32, 32, 32)
+    a = P.poisson_a(0.5)
+    b = P.rhs_complex_normal(shape, 1006)
+    ud = cdev(np.zeros_like(b))
+    st = gpu.helm_aar(ud, cdev(b), shape, a, None, 1.0, ETA, 0.6, 0.6, 7, 6, 5000, 1e-12)
+    assert st["converged"]
+    uf = fft_solve_complex(b, shape, a, oracle.lap_coefficient(6), 1.0, ETA)
+    assert np.linalg.norm(ud.cpu().numpy() - uf) / np.linalg.norm(uf) < 1e-10
+
+
+def test_helm_gradients_fd_and_oracle_adjoint(gpu, oracle):
+    """Gradients of L = 1/2 ||u(b, c) - u*||^2 by the adjoint harness vs
+    central finite differences (step 1e-6) on 20 random directions for b
+    and for c, and the adjoint solution vs the oracle's adjoint solve."""
+    import torch
+    shape = (16, 16, 16)
+    n = int(np.prod(shape))
+    a = P.poisson_a(0.5)
+    opts = dict(omega=0.6, beta=0.6, m=7, p=6, max_iter=5000, tol=1e-13)
+    c_np = P.helmholtz_coefficient(shape, 1005)
+    b_np = P.rhs_complex_normal(shape, 1005)
+    # u* from a second seeded solve
+    ustar = torch.zeros(n, dtype=torch.complex128, device="cuda")
+    gpu.helm_aar(ustar, cdev(P.rhs_complex_normal(shape, 1005, stream=1)), shape, a, rdev(c_np), 0.0, ETA,
+                 **opts)
+
+    def loss(b, c):
+        u = gpu.HelmholtzSolve.apply(b, c, shape, a, ETA, opts)
+        return 0.5 * torch.sum(torch.abs(u - ustar) ** 2)
+
+    b = cdev(b_np).requires_grad_(True)
+    c = rdev(c_np).requires_grad_(True)
+    L = loss(b, c)
+    L.backward()
+    gb, gc = b.grad.detach().clone(), c.grad.detach().clone()
+
+    # the adjoint solution is dL/db; compare with the oracle's adjoint solve
+    with torch.no_grad():
+        uu = torch.zeros(n, dtype=torch.complex128, device="cuda")
+        gpu.helm_aar(uu, cdev(b_np), shape, a, rdev(c_np), 0.0, ETA, **opts)
+        rhs_adj = (uu - ustar).cpu().numpy()
+    lam_ref, dl = oracle.aar_complex(np.zeros(n, complex), rhs_adj, shape, a, c_np, 0.0, -ETA, 0.6, 0.6, 7, 6,
+                                     5000, 1e-13)
+    assert dl["converged"]
+    assert np.linalg.norm(gb.cpu().numpy() - lam_ref) / np.linalg.norm(lam_ref) < 1e-8
+
+    rng = np.random.default_rng(2024)
+    hstep = 1e-6
+    worst = 0.0
+    with torch.no_grad():
+        for k in range(20):
+            db = cdev(rng.standard_normal(n) + 1j * rng.standard_normal(n))
+            dc = rdev(rng.standard_normal(n))
+            # b direction: dL = Re(<grad_b, db>)
+            fd = (loss(b.detach() + hstep * db, c.detach()) - loss(b.detach() - hstep * db, c.detach())) / (2 * hstep)
+            ad = torch.sum((gb.conj() * db).real)
+            rb = abs(float(fd - ad)) / abs(float(ad))
+            # c direction
+            fd = (loss(b.detach(), c.detach() + hstep * dc) - loss(b.detach(), c.detach() - hstep * dc)) / (2 * hstep)
+            ad = torch.sum(gc * dc)
+            rc = abs(float(fd - ad)) / abs(float(ad))
+            worst = max(worst, rb, rc)
+            say(f"direction {k}: rel err b {rb:.2e} c {rc:.2e}")
+    assert worst <= 1e-6

@@ -327,3 +327,75 @@ def test_generator_numpy_vs_torch_bits():
     b = P.rhs_uniform((8, 8, 8), 1)
     assert abs(b.mean()) < 1e-16
     del torch
+
+
+# ---------------------------------------------------------------------------
+# the complex Helmholtz restatement: A u = a Lap u + (c + i eta) u
+# ---------------------------------------------------------------------------
+
+def fft_solve_complex(b, shape, a, coeff, c0, eta):
+    nx, ny, nz = shape
+    lam = symbol(shape, a, coeff) + c0 + 1j * eta
+    return np.fft.ifftn(np.fft.fftn(b.reshape(nz, ny, nx)) / lam).reshape(-1)
+
+
+def test_oracle_complex_matvec_symbol(oracle):
+    """Every grid Fourier mode is an eigenvector with eigenvalue
+    a*sym(theta) + c0 + i*eta (constant c); a variable c adds c(x) u
+    pointwise."""
+    shape = (12, 10, 8)
+    nx, ny, nz = shape
+    a, c0, eta = -4.0, 1.3, 0.5
+    coeff = oracle.lap_coefficient(6)
+    lam = symbol(shape, a, coeff) + c0 + 1j * eta
+    rng = np.random.default_rng(3)
+    for _ in range(6):
+        kx, ky, kz = rng.integers(0, nx), rng.integers(0, ny), rng.integers(0, nz)
+        k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+        e = np.exp(2j * np.pi * (kx * i / nx + ky * j / ny + kz * k / nz)).reshape(-1)
+        y = oracle.matvec_complex(e, shape, a, None, c0, eta)
+        assert np.allclose(y, lam[kz, ky, kx] * e, rtol=0, atol=1e-12 * abs(lam).max())
+    # variable c: A(c) u - A(0) u = c u
+    u = rng.standard_normal(nx * ny * nz) + 1j * rng.standard_normal(nx * ny * nz)
+    c = rng.uniform(0.5, 1.5, nx * ny * nz)
+    d = oracle.matvec_complex(u, shape, a, c, 0.0, eta) - oracle.matvec_complex(u, shape, a, None, 0.0, eta)
+    assert np.allclose(d, c * u, rtol=1e-12, atol=1e-12)
+
+
+def test_oracle_complex_aar_vs_fft(oracle):
+    """Constant c: the converged complex AAR solution is the FFT solution."""
+    shape = (16, 16, 16)
+    a, c0, eta = P.poisson_a(0.5), 1.0, 0.5
+    b = P.rhs_complex_normal(shape, 1005)
+    u, d = oracle.aar_complex(np.zeros_like(b), b, shape, a, None, c0, eta, 0.6, 0.6, 7, 6, 2000, 1e-12)
+    assert d["converged"] and d["iterations"] % 6 == 0
+    uf = fft_solve_complex(b, shape, a, oracle.lap_coefficient(6), c0, eta)
+    assert np.linalg.norm(u - uf) / np.linalg.norm(uf) < 1e-10
+
+
+def test_oracle_complex_reduces_to_real_aar(oracle):
+    """eta = 0, c = sigma constant and a real b: the complex AAR is the real
+    (reference-pinned) AAR on the screened operator -- the same iteration
+    count and solution, with a zero imaginary part."""
+    shape = (16, 16, 16)
+    a, sigma = P.poisson_a(0.5), 0.5
+    b = P.rhs_normal(shape, 78)
+    xr, dr = oracle.aar(np.zeros(b.size), b, shape, a, 0.6, 0.6, 7, 6, 1000, 1e-8, sigma=sigma)
+    u, dc = oracle.aar_complex(np.zeros(b.size, complex), b.astype(complex), shape, a, None, sigma, 0.0,
+                               0.6, 0.6, 7, 6, 1000, 1e-8)
+    assert dc["iterations"] == dr["iterations"]
+    assert np.abs(u.imag).max() == 0.0
+    assert np.linalg.norm(u.real - xr) / np.linalg.norm(xr) < 1e-10
+
+
+def test_oracle_complex_adjoint_identity(oracle):
+    """<A^H v, u> = <v, A u> for a variable c (A^H: eta -> -eta)."""
+    shape = (8, 8, 8)
+    rng = np.random.default_rng(5)
+    n = 512
+    u = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    v = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    c = rng.uniform(0.5, 1.5, n)
+    Au = oracle.matvec_complex(u, shape, -4.0, c, 0.0, 0.5)
+    AHv = oracle.matvec_complex(v, shape, -4.0, c, 0.0, -0.5)
+    assert abs(np.vdot(AHv, u) - np.vdot(v, Au)) < 1e-10 * abs(np.vdot(v, Au))
