@@ -222,13 +222,10 @@ template hipError_t launch_gram<double>(const GramArgs<double> &, hipStream_t);
 template hipError_t launch_gram<float>(const GramArgs<float> &, hipStream_t);
 
 // ---------------------------------------------------------------------------
-// m x m pseudo-inverse solve: Golub-Reinsch SVD as in MPI_REAL/tools.c,
-// register-resident with compile-time indices (loops run to the bound M
-// and are predicated on the runtime size n), executed by one lane.  The
-// operation sequence is the reference's, so for the same input matrix the
-// result is bit-identical.  U is never formed: the reference replaces it
-// by V (tools.c:324-329) and no U-only step feeds w, the superdiagonal or
-// V.  The left reflector is applied to columns 1..n-1 (tools.c:137).
+// m x m pseudo-inverse solve: the Golub-Reinsch SVD of MPI_REAL/tools.c
+// (tools.c:113-352) and x = V diag(1/w) V^T b (tools.c:52-105).  U is
+// never formed: the reference replaces it by V (tools.c:324-329) and no
+// U-only step feeds w, the superdiagonal or V.
 // ---------------------------------------------------------------------------
 __device__ __attribute__((always_inline)) inline double d_pythag(double a, double b)
 {
@@ -238,19 +235,28 @@ __device__ __attribute__((always_inline)) inline double d_pythag(double a, doubl
 }
 __device__ inline double d_sign(double mag, double s) { return s >= 0.0 ? fabs(mag) : -fabs(mag); }
 
-// One singular value of the implicit-shift QR phase (tools.c:254-350),
-// k a compile-time constant so that every array index below is static and
-// w, rv1 and V stay in registers.
+// ---------------------------------------------------------------------------
+// The pinv is executed by one wavefront.  The operation sequence is the
+// reference's (bit-identical results); loops whose iterations are
+// independent are spread over the lanes -- lane = column (left reflector,
+// right-transform accumulation, the final products) or row (right
+// reflector; the QR phase, where each V row is rotated on its own and
+// stays in the lane's registers).  Scalars (w, rv1, the reflector and
+// rotation coefficients) are computed by every lane alike.  The left
+// reflector of tools.c:137 also runs over columns j < i and j = i; column
+// i itself is updated before the columns j > i read it, as in the
+// sequential loop.
+// ---------------------------------------------------------------------------
 template <int M, int k>
-__device__ __attribute__((always_inline)) inline void gr_qr_step(double (&w)[M], double (&rv1)[M], double (&v)[M][M], const double anorm,
-                                  const int n, int &status)
+__device__ __attribute__((always_inline)) inline void wgr_qr_step(double (&w)[M], double (&rv1)[M], double (&vr)[M],
+                                                                   const double anorm, const int n, int &status)
 {
     double g = 0.0, s = 0.0, f = 0.0, h = 0.0;
     if (k < n && !status) {
         for (int its = 0; its <= 250; its++) {
             int flag = 1, l = 0;
             bool found = false;
-    #pragma unroll
+#pragma unroll
             for (int ll = k; ll >= 0; ll--) {
                 if (!found) {
                     if ((double)(fabs(rv1[ll]) + anorm) == anorm) {
@@ -267,7 +273,7 @@ __device__ __attribute__((always_inline)) inline void gr_qr_step(double (&w)[M],
                 double c = 0.0;
                 s = 1.0;
                 bool stop = false;
-    #pragma unroll
+#pragma unroll
                 for (int i = 0; i <= k; i++) {
                     if (i >= l && !stop) {
                         f = s * rv1[i];
@@ -289,9 +295,7 @@ __device__ __attribute__((always_inline)) inline void gr_qr_step(double (&w)[M],
             if (l == k) {
                 if (z < 0.0) {
                     w[k] = -z;
-    #pragma unroll
-                    for (int j = 0; j < M; j++)
-                        if (j < n) v[j][k] = -v[j][k];
+                    vr[k] = -vr[k];
                 }
                 break;
             }
@@ -300,7 +304,7 @@ __device__ __attribute__((always_inline)) inline void gr_qr_step(double (&w)[M],
                 break;
             }
             double x = 0.0;
-    #pragma unroll
+#pragma unroll
             for (int ll = 0; ll <= k; ll++)
                 if (ll == l) x = w[ll];
             const int nm = k - 1;
@@ -312,7 +316,7 @@ __device__ __attribute__((always_inline)) inline void gr_qr_step(double (&w)[M],
             f = ((x - z) * (x + z) + h * ((y / (f + d_sign(g, f))) - h)) / x;
             double c = 1.0;
             s = 1.0;
-    #pragma unroll
+#pragma unroll
             for (int j = 0; j < k; j++) {
                 if (j >= l) {
                     const int i = j + 1;
@@ -328,13 +332,10 @@ __device__ __attribute__((always_inline)) inline void gr_qr_step(double (&w)[M],
                     g = g * c - x * s;
                     h = y * s;
                     y *= c;
-    #pragma unroll
-                    for (int jj = 0; jj < M; jj++) {
-                        if (jj < n) {
-                            const double vx = v[jj][j], vz = v[jj][i];
-                            v[jj][j] = vx * c + vz * s;
-                            v[jj][i] = vz * c - vx * s;
-                        }
+                    {
+                        const double vx = vr[j], vz = vr[i];
+                        vr[j] = vx * c + vz * s;
+                        vr[i] = vz * c - vx * s;
                     }
                     z = d_pythag(f, h);
                     w[j] = z;
@@ -347,7 +348,7 @@ __device__ __attribute__((always_inline)) inline void gr_qr_step(double (&w)[M],
                     x = c * y - s * g;
                 }
             }
-    #pragma unroll
+#pragma unroll
             for (int ll = 0; ll <= k; ll++)
                 if (ll == l) rv1[ll] = 0.0;
             rv1[k] = f;
@@ -357,108 +358,108 @@ __device__ __attribute__((always_inline)) inline void gr_qr_step(double (&w)[M],
 }
 
 template <int M, int k>
-struct GrQr {
-    __device__ __attribute__((always_inline)) static inline void run(double (&w)[M], double (&rv1)[M], double (&v)[M][M], const double anorm,
-                                      const int n, int &status)
+struct WGrQr {
+    __device__ __attribute__((always_inline)) static inline void run(double (&w)[M], double (&rv1)[M], double (&vr)[M],
+                                                                      const double anorm, const int n, int &status)
     {
-        gr_qr_step<M, k>(w, rv1, v, anorm, n, status);
-        GrQr<M, k - 1>::run(w, rv1, v, anorm, n, status);
+        wgr_qr_step<M, k>(w, rv1, vr, anorm, n, status);
+        WGrQr<M, k - 1>::run(w, rv1, vr, anorm, n, status);
     }
 };
 template <int M>
-struct GrQr<M, -1> {
-    __device__ static inline void run(double (&)[M], double (&)[M], double (&)[M][M], const double, const int,
-                                      int &)
-    {
-    }
+struct WGrQr<M, -1> {
+    __device__ static inline void run(double (&)[M], double (&)[M], double (&)[M], const double, const int, int &) {}
 };
 
 template <int M>
-__device__ __attribute__((always_inline)) inline int gr_pinv(double (&a)[M][M], const int n, const double (&bv)[M], double (&xv)[M],
-                       double (&w)[M], double (&v)[M][M])
+__global__ __launch_bounds__(64) void ls_kernel(Ctrl *c, int n)
 {
-    double rv1[M];
+    __shared__ double A[M][M];
+    __shared__ double V[M][M];
+    __shared__ double T[M];
+    if (c->done) return;
+    const int lane = threadIdx.x;
+    // FtF[i][j] = allredvec[j*m + i], Ftf[j] = allredvec[m*m + j] (AAR.c:214-221)
+    for (int e = lane; e < M * M; e += 64) {
+        const int i = e / M, j = e % M;
+        A[i][j] = (i < n && j < n) ? c->G[j * n + i] : 0.0;
+        V[i][j] = 0.0;
+    }
+    __syncthreads();
+
+    double w[M], rv1[M];
 #pragma unroll
     for (int i = 0; i < M; i++) {
-        rv1[i] = 0.0;
         w[i] = 0.0;
-#pragma unroll
-        for (int j = 0; j < M; j++) v[i][j] = 0.0;
+        rv1[i] = 0.0;
     }
     double g = 0.0, scale = 0.0, anorm = 0.0, s = 0.0, f = 0.0, h = 0.0;
 
+    // Householder bidiagonalisation (tools.c:122-176)
 #pragma unroll
     for (int i = 0; i < M; i++) {
         if (i < n) {
             const int l = i + 1;
             rv1[i] = scale * g;
             g = s = scale = 0.0;
-#pragma unroll
-            for (int k = i; k < M; k++)
-                if (k < n) scale += fabs(a[k][i]);
+            for (int k = i; k < n; k++) scale += fabs(A[k][i]);
             if (scale != 0.0) {
-#pragma unroll
-                for (int k = i; k < M; k++)
-                    if (k < n) {
-                        a[k][i] /= scale;
-                        s += a[k][i] * a[k][i];
-                    }
-                f = a[i][i];
+                __syncthreads();
+                if (lane >= i && lane < n) A[lane][i] /= scale;
+                __syncthreads();
+                for (int k = i; k < n; k++) s += A[k][i] * A[k][i];
+                f = A[i][i];
                 g = -d_sign(sqrt(s), f);
                 h = f * g - s;
-                a[i][i] = f - g;
-#pragma unroll
-                for (int j = 1; j < M; j++) {
-                    if (j < n) {
-                        double t = 0.0;
-#pragma unroll
-                        for (int k = i; k < M; k++)
-                            if (k < n) t += a[k][i] * a[k][j];
-                        const double fac = t / h;
-#pragma unroll
-                        for (int k = i; k < M; k++)
-                            if (k < n) a[k][j] += fac * a[k][i];
-                    }
-                }
-#pragma unroll
-                for (int k = i; k < M; k++)
-                    if (k < n) a[k][i] *= scale;
+                __syncthreads();
+                if (lane == 0) A[i][i] = f - g;
+                __syncthreads();
+                // columns j = 1 .. n-1 in the order the sequential loop sees
+                // column i: j < i, then j = i, then j > i
+                auto colupd = [&](int j) {
+                    double t = 0.0;
+                    for (int k = i; k < n; k++) t += A[k][i] * A[k][j];
+                    const double fac = t / h;
+                    for (int k = i; k < n; k++) A[k][j] += fac * A[k][i];
+                };
+                if (lane >= 1 && lane < i && lane < n) colupd(lane);
+                __syncthreads();
+                if (i >= 1 && lane == i) colupd(i);
+                __syncthreads();
+                if (lane > i && lane >= 1 && lane < n) colupd(lane);
+                __syncthreads();
+                if (lane >= i && lane < n) A[lane][i] *= scale;
+                __syncthreads();
             }
             w[i] = scale * g;
             g = s = scale = 0.0;
             if (i != n - 1) {
-#pragma unroll
-                for (int k = l; k < M; k++)
-                    if (k < n) scale += fabs(a[i][k]);
+                for (int k = l; k < n; k++) scale += fabs(A[i][k]);
                 if (scale != 0.0) {
-#pragma unroll
-                    for (int k = l; k < M; k++)
-                        if (k < n) {
-                            a[i][k] /= scale;
-                            s += a[i][k] * a[i][k];
-                        }
-                    f = a[i][l < M ? l : M - 1];
+                    __syncthreads();
+                    if (lane >= l && lane < n) A[i][lane] /= scale;
+                    __syncthreads();
+                    for (int k = l; k < n; k++) s += A[i][k] * A[i][k];
+                    f = A[i][l < M ? l : M - 1];
                     g = -d_sign(sqrt(s), f);
                     h = f * g - s;
-                    a[i][l < M ? l : M - 1] = f - g;
+                    __syncthreads();
+                    if (lane == 0) A[i][l < M ? l : M - 1] = f - g;
+                    __syncthreads();
+                    // rv1[k] = a[i][k] / h, kept by every lane
 #pragma unroll
-                    for (int k = l; k < M; k++)
-                        if (k < n) rv1[k] = a[i][k] / h;
+                    for (int k = 0; k < M; k++)
+                        if (k >= l && k < n) rv1[k] = A[i][k] / h;
+                    if (lane >= l && lane < n) {
+                        double t = 0.0;
+                        for (int k = l; k < n; k++) t += A[lane][k] * A[i][k];
 #pragma unroll
-                    for (int j = l; j < M; j++) {
-                        if (j < n) {
-                            double t = 0.0;
-#pragma unroll
-                            for (int k = l; k < M; k++)
-                                if (k < n) t += a[j][k] * a[i][k];
-#pragma unroll
-                            for (int k = l; k < M; k++)
-                                if (k < n) a[j][k] += t * rv1[k];
-                        }
+                        for (int k = 0; k < M; k++)
+                            if (k >= l && k < n) A[lane][k] += t * rv1[k];
                     }
-#pragma unroll
-                    for (int k = l; k < M; k++)
-                        if (k < n) a[i][k] *= scale;
+                    __syncthreads();
+                    if (lane >= l && lane < n) A[i][lane] *= scale;
+                    __syncthreads();
                 }
             }
             {
@@ -468,7 +469,7 @@ __device__ __attribute__((always_inline)) inline int gr_pinv(double (&a)[M][M], 
         }
     }
 
-    // accumulation of right-hand transforms (for i < n-1: l = i+1, g = rv1[i+1])
+    // accumulation of right-hand transforms (tools.c:178-196)
 #pragma unroll
     for (int i = M - 1; i >= 0; i--) {
         if (i < n) {
@@ -476,37 +477,39 @@ __device__ __attribute__((always_inline)) inline int gr_pinv(double (&a)[M][M], 
                 const int l = i + 1;
                 const double gg = rv1[l < M ? l : M - 1];
                 if (gg != 0.0) {
-                    const double ail = a[i][l < M ? l : M - 1];
-#pragma unroll
-                    for (int j = l; j < M; j++)
-                        if (j < n) v[j][i] = (a[i][j] / ail) / gg;
-#pragma unroll
-                    for (int j = l; j < M; j++) {
-                        if (j < n) {
-                            double t = 0.0;
-#pragma unroll
-                            for (int k = l; k < M; k++)
-                                if (k < n) t += a[i][k] * v[k][j];
-#pragma unroll
-                            for (int k = l; k < M; k++)
-                                if (k < n) v[k][j] += t * v[k][i];
-                        }
+                    const double ail = A[i][l < M ? l : M - 1];
+                    if (lane >= l && lane < n) V[lane][i] = (A[i][lane] / ail) / gg;
+                    __syncthreads();
+                    if (lane >= l && lane < n) {
+                        double t = 0.0;
+                        for (int k = l; k < n; k++) t += A[i][k] * V[k][lane];
+                        for (int k = l; k < n; k++) V[k][lane] += t * V[k][i];
                     }
+                    __syncthreads();
                 }
-#pragma unroll
-                for (int j = l; j < M; j++)
-                    if (j < n) {
-                        v[i][j] = 0.0;
-                        v[j][i] = 0.0;
-                    }
+                if (lane >= l && lane < n) {
+                    V[i][lane] = 0.0;
+                    V[lane][i] = 0.0;
+                }
+                __syncthreads();
             }
-            v[i][i] = 1.0;
+            if (lane == 0) V[i][i] = 1.0;
+            __syncthreads();
         }
     }
 
-    // implicit-shift QR on the bidiagonal, k = n-1 .. 0
+    // implicit-shift QR on the bidiagonal (tools.c:254-350): lane = V row
+    double vr[M];
+#pragma unroll
+    for (int j = 0; j < M; j++) vr[j] = (lane < n) ? V[lane < M ? lane : 0][j] : 0.0;
     int status = 0;
-    GrQr<M, M - 1>::run(w, rv1, v, anorm, n, status);
+    WGrQr<M, M - 1>::run(w, rv1, vr, anorm, n, status);
+    __syncthreads();
+    if (lane < n) {
+#pragma unroll
+        for (int j = 0; j < M; j++) V[lane][j] = vr[j];
+    }
+    __syncthreads();
 
     // truncation (tools.c:331-335) and x = V diag(1/w) V^T b (tools.c:76-94)
     double wmax = 0.0;
@@ -517,53 +520,31 @@ __device__ __attribute__((always_inline)) inline int gr_pinv(double (&a)[M][M], 
 #pragma unroll
     for (int j = 0; j < M; j++)
         if (j < n && w[j] < wmin) w[j] = 0.0;
-    double tmp[M];
+    if (lane < n) {
+        double wl = 0.0;
 #pragma unroll
-    for (int j = 0; j < M; j++) {
+        for (int j = 0; j < M; j++)
+            if (j == lane) wl = w[j];
         double t = 0.0;
-        if (j < n && w[j] != 0.0) {
-#pragma unroll
-            for (int i = 0; i < M; i++)
-                if (i < n) t += v[i][j] * bv[i];
-            t /= w[j];
+        if (wl != 0.0) {
+            for (int i = 0; i < n; i++) t += V[i][lane] * c->G[n * n + i];
+            t /= wl;
         }
-        tmp[j] = t;
+        T[lane] = t;
     }
-#pragma unroll
-    for (int j = 0; j < M; j++) {
+    __syncthreads();
+    if (lane < n) {
         double t = 0.0;
+        for (int jj = 0; jj < n; jj++) t += V[lane][jj] * T[jj];
+        c->s[lane] = t;
+        double wl = 0.0;
 #pragma unroll
-        for (int jj = 0; jj < M; jj++)
-            if (jj < n) t += v[j][jj] * tmp[jj];
-        xv[j] = t;
+        for (int j = 0; j < M; j++)
+            if (j == lane) wl = w[j];
+        c->w[lane] = wl;
+        for (int q = 0; q < n; q++) c->V[lane * n + q] = V[lane][q];
     }
-    return status;
-}
-
-template <int M>
-__global__ __launch_bounds__(64) void ls_kernel(Ctrl *c, int n)
-{
-    if (c->done || threadIdx.x != 0) return;
-    double a[M][M], b[M], x[M], w[M], v[M][M];
-    // FtF[i][j] = allredvec[j*m + i], Ftf[j] = allredvec[m*m + j] (AAR.c:214-221)
-#pragma unroll
-    for (int i = 0; i < M; i++) {
-#pragma unroll
-        for (int j = 0; j < M; j++) a[i][j] = (i < n && j < n) ? c->G[j * n + i] : 0.0;
-        b[i] = i < n ? c->G[n * n + i] : 0.0;
-    }
-    const int st = gr_pinv<M>(a, n, b, x, w, v);
-#pragma unroll
-    for (int j = 0; j < M; j++) {
-        if (j < n) {
-            c->s[j] = x[j];
-            c->w[j] = w[j];
-#pragma unroll
-            for (int q = 0; q < M; q++)
-                if (q < n) c->V[j * n + q] = v[j][q];
-        }
-    }
-    if (st) {
+    if (status && lane == 0) {
         // tools.c:270 exits the process; here the solve stops with a status
         c->status = 1;
         c->done = 1;
