@@ -47,6 +47,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--cpu-iters", type=int, default=12)
+    ap.add_argument("--no-helm", action="store_true")
     return ap.parse_args()
 
 
@@ -92,6 +93,65 @@ def cpu_baseline(n_cpu, iters, gpu_iterations, n_gpu):
                    f"{n_gpu}^3) x {gpu_iterations} iterations (the GPU solve's count)"),
         "cpu": platform.processor() or platform.machine(),
     }
+
+
+def helmholtz_case(n, dev, a, directions=20, hstep=1e-6):
+    """One forward solve and its adjoint (autograd backward) of
+    L = 1/2 ||u(b, c) - u*||^2 on the n^3 complex Helmholtz problem, then
+    the gradients checked by central differences."""
+    import numpy as np
+    import torch
+    import aar_hip
+    from aar_hip import problems as P
+    shape = (n,) * 3
+    N = n ** 3
+    eta = 0.5
+    c = torch.from_numpy(P.helmholtz_coefficient(shape, 1005)).to(dev)
+
+    def crhs(stream):
+        v = P.normal_torch(2 * N, 1005, stream, dev)
+        return torch.complex(v[0::2].contiguous(), v[1::2].contiguous())
+
+    b, bs = crhs(0), crhs(1)
+    opts = dict(omega=0.6, beta=0.6, m=7, p=6, max_iter=20000, tol=1e-13)
+    ustar = torch.zeros_like(b)
+    aar_hip.helm_aar(ustar, bs, shape, a, c, 0.0, eta, **opts)
+
+    def loss(b_, c_):
+        u = aar_hip.HelmholtzSolve.apply(b_, c_, shape, a, eta, opts)
+        return 0.5 * torch.sum(torch.abs(u - ustar) ** 2)
+
+    bb = b.clone().requires_grad_(True)
+    cc = c.clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    L = loss(bb, cc)
+    L.backward()
+    torch.cuda.synchronize()
+    t_fb = time.perf_counter() - t0
+    last = aar_hip.HelmholtzSolve.last
+    gb, gc = bb.grad.detach(), cc.grad.detach()
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(2024)
+    eb, ec = [], []
+    with torch.no_grad():
+        for _ in range(directions):
+            db = torch.complex(torch.randn(N, dtype=torch.float64, device=dev, generator=gen),
+                               torch.randn(N, dtype=torch.float64, device=dev, generator=gen))
+            dc = torch.randn(N, dtype=torch.float64, device=dev, generator=gen)
+            fd = (loss(b + hstep * db, c) - loss(b - hstep * db, c)) / (2 * hstep)
+            ad = torch.sum((gb.conj() * db).real)
+            eb.append(abs(float(fd - ad)) / abs(float(ad)))
+            fd = (loss(b, c + hstep * dc) - loss(b, c - hstep * dc)) / (2 * hstep)
+            ad = torch.sum(gc * dc)
+            ec.append(abs(float(fd - ad)) / abs(float(ad)))
+    f, adj = last["forward"], last["adjoint"]
+    return {"grid": [n, n, n], "h": 0.25, "eta": eta, "tol": opts["tol"],
+            "forward_iterations": f["iterations"], "forward_ms": f["solve_ms"],
+            "adjoint_iterations": adj["iterations"], "adjoint_ms": adj["solve_ms"],
+            "solves_per_s": 1e3 / f["solve_ms"], "forward_backward_s": t_fb,
+            "fd_directions": directions, "fd_step": hstep,
+            "fd_max_rel_err_b": float(np.max(eb)), "fd_max_rel_err_c": float(np.max(ec))}
 
 
 def main():
@@ -214,6 +274,15 @@ def main():
                         "algorithmic_GBps": st["bytes"] / st["solve_ms"] / 1e6}
         log(f"{name}: {st['iterations']} its {st['solve_ms']:.0f} ms")
 
+    # differentiable complex Helmholtz (-Lap + c(x) + i eta) u = b, h = 0.25,
+    # eta = 0.5: forward + adjoint solves through the autograd harness, and
+    # the gradients against central differences (step 1e-6) on 20 directions
+    helm = {}
+    if not args.no_helm:
+        for nh in (128, 256):
+            helm[str(nh)] = helmholtz_case(nh, dev, a)
+            log(f"helmholtz {nh}^3: {helm[str(nh)]}")
+
     # roofline of the dominant kernel: one profiled FP64 solve, HIP events
     # around every launch of the recurrence stencil sweep on the library stream
     aar_hip.profile(True)
@@ -291,6 +360,7 @@ def main():
             "kernels": ktab(kern_mx),
         },
         "other_solvers": others,
+        "helmholtz": helm,
         "matvec": {"GBps": mv_gbs, "frac_of_peak": mv_gbs / HBM_PEAK_GBS,
                    "avg_ms": mv["ms"] / mv["count"], "bytes_per_launch": 16.0 * N},
         "kernels": ktab(kern),

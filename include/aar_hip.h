@@ -77,6 +77,26 @@ int aar_hip_aar_f32(const aar_hip_grid *g, float *x, const float *b, double omeg
 int aar_hip_aar_mixed(const aar_hip_grid *g, double *x, const double *b, double omega, double beta,
                       int m, int p, int max_iter, double tol, int precond, aar_hip_stats *st);
 
+/* One solve of a batch: its own grid, operator a*Lap + sigma and vectors
+ * (device pointers; x: initial guess in, solution out). */
+typedef struct {
+    int nx, ny, nz;
+    double a, sigma;
+    double *x;
+    const double *b;
+} aar_hip_instance;
+
+/* Extension (no reference counterpart): `count` independent AAR solves
+ * (AAR.c:37-177 semantics each, FD12, fdn = 6) sharing omega, beta, m, p,
+ * max_iter and tol, run together: every batched kernel covers all
+ * instances through a work table and one Anderson cycle is replayed as a
+ * captured hipGraph.  mixed = 0: FP64; mixed = 1: the mixed-precision
+ * iteration of aar_hip_aar_mixed.  Needs 1 <= p <= m+1.  stats: `count`
+ * entries (solve_ms is the whole batch's).  The batch runs on the library's
+ * own stream, ordered with the stream set by aar_hip_set_stream. */
+int aar_hip_aar_batch(const aar_hip_instance *inst, int count, double omega, double beta, int m, int p,
+                      int max_iter, double tol, int precond, int mixed, aar_hip_stats *stats);
+
 /* PGR.c:35-173 (l2 = 0) and PL2R.c:35-172 (l2 = 1) */
 int aar_hip_pgr(const aar_hip_grid *g, double *x, const double *b, double omega, int m, int p,
                 int max_iter, double tol, int precond, int l2, aar_hip_stats *st);

@@ -20,6 +20,12 @@ class Grid(ctypes.Structure):
                 ("fdn", ctypes.c_int), ("a", ctypes.c_double), ("sigma", ctypes.c_double)]
 
 
+class Instance(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("a", ctypes.c_double), ("sigma", ctypes.c_double),
+                ("x", ctypes.c_void_p), ("b", ctypes.c_void_p)]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [
         ("iterations", ctypes.c_int),
@@ -54,7 +60,7 @@ EXTENSION_SYMBOLS = [
     "aar_hip_cg", "aar_hip_aar_refined", "aar_hip_pinv", "aar_hip_history", "aar_hip_last_stats",
     "aar_hip_set_stream", "aar_hip_set_tuning", "aar_hip_set_option", "aar_hip_profile", "aar_hip_profile_reset",
     "aar_hip_profile_query", "aar_hip_release", "aar_hip_last_error",
-    "aar_hip_helm_matvec", "aar_hip_helm_aar", "aar_hip_helm_grad_c",
+    "aar_hip_helm_matvec", "aar_hip_helm_aar", "aar_hip_helm_grad_c", "aar_hip_aar_batch",
 ]
 
 
@@ -101,6 +107,7 @@ def load(require_gpu=True):
         "aar_hip_helm_matvec": [P, P, P],
         "aar_hip_helm_aar": [P, P, P, D, D, I, I, I, D, I, S],
         "aar_hip_helm_grad_c": [ctypes.c_long, P, P, P],
+        "aar_hip_aar_batch": [ctypes.POINTER(Instance), I, D, D, I, I, I, D, I, I, S],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

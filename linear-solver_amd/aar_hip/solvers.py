@@ -20,7 +20,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import Grid, Stats, check
+from ._lib import Grid, Instance, Stats, check
 
 
 def _L():
@@ -105,6 +105,28 @@ def AAR_mixed(x, rhs, shape, a, omega, beta, m, p, max_iter, tol, sigma=0.0, pre
                                  _dev(rhs, torch.float64, "rhs"), omega, beta, m, p, max_iter, tol,
                                  int(bool(precond)), ctypes.byref(st)))
     return st.as_dict()
+
+
+def AAR_batch(instances, omega, beta, m, p, max_iter, tol, precond=True, mixed=False):
+    """Independent AAR solves run together (no reference counterpart).
+
+    instances: sequence of (x, rhs, shape, a, sigma) with float64 CUDA
+    tensors; every x is overwritten with its solution.  Returns one stats
+    dict per instance (AAR.c:37-177 semantics each)."""
+    import torch
+    n = len(instances)
+    arr = (Instance * n)()
+    for i, (x, rhs, shape, a, sigma) in enumerate(instances):
+        _check_len(x, shape, f"x[{i}]")
+        _check_len(rhs, shape, f"rhs[{i}]")
+        nx, ny, nz = (int(s) for s in shape)
+        arr[i] = Instance(nx, ny, nz, float(a), float(sigma), _dev(x, torch.float64, f"x[{i}]").value,
+                          _dev(rhs, torch.float64, f"rhs[{i}]").value)
+    sts = (Stats * n)()
+    _bind_stream()
+    check(_L().aar_hip_aar_batch(arr, n, omega, beta, m, p, max_iter, tol, int(bool(precond)),
+                                 int(bool(mixed)), sts))
+    return [s.as_dict() for s in sts]
 
 
 def PGR(x, rhs, shape, a, omega, m, p, max_iter, tol, sigma=0.0, precond=True, _l2=False):

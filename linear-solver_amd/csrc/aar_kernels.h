@@ -45,6 +45,9 @@ template <typename T>
 hipError_t launch_stencil(SMode m, const SArgs<T> &a, int wy, int py, hipStream_t st);
 template <typename T>
 int stencil_items(const SArgs<T> &a, int wy, int py);
+// fills the tile geometry of `a` (nxy, ntx, nty, nzc, nitems; lz clamped)
+template <typename T>
+void stencil_geometry(SArgs<T> &a, int wy, int py);
 
 // ---------------------------------------------------------------------------
 // history sweeps
@@ -87,6 +90,8 @@ hipError_t launch_gram(const GramArgs<T> &a, hipStream_t st);
 // the Golub-Reinsch SVD of tools.c:52-352 (bit-faithful).  Reads
 // ctrl->G (allredvec layout), writes ctrl->s, ctrl->w.
 hipError_t launch_ls(Ctrl *ctrl, int m, int complex_embed, hipStream_t st);
+// the same for `count` independent solves, ctrls a device array of pointers
+hipError_t launch_ls_batch(Ctrl *const *ctrls, int count, int m, hipStream_t st);
 
 // AAR.c:225-231 + 116-117:  x_new = (x_k + beta f_k) - sum_k (DX_k + beta DF_k) s_col(k)
 template <typename T>

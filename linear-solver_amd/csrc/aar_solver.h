@@ -9,6 +9,7 @@
 
 #include "aar_common.h"
 #include "aar_hip.h"
+#include "aar_kernels.h"
 
 namespace aar {
 
@@ -71,6 +72,18 @@ struct Op {
 
 Context &ctx();
 int context_init();
+hipError_t ensure(DevBuf &b, size_t bytes);
+int auto_lz(const Op &op, int wy, int py, size_t es);
+long ring_ld(long n, size_t es);
+int nblocks_for(long n);
+template <typename T>
+SArgs<T> base_sargs(const Op &op);
+template <typename T>
+hipError_t run_dot(DotArgs<T> d);
+int poll_record(Ctrl *ctrl, int cyc);
+bool poll_check(int cyc);
+void prof_begin(const char *name);
+void prof_end(double bytes);
 int fail(int code, const std::string &msg);
 void lap_coefficient(double *c, int fdn);
 int make_op(const aar_hip_grid *g, const double *coeff, Op &op);
@@ -102,6 +115,9 @@ int make_hop(const aar_hip_helm *h, HOp &op);
 int helm_matvec(const HOp &op, const double *u, double *y);
 int helm_solve(const HOp &op, double *u, const double *b, double omega, double beta, int m, int p, int max_iter,
                double tol, aar_hip_stats *st);
+template <typename Tx, typename Tg>
+int batch_solve(const aar_hip_instance *ins, int count, double omega, double beta, int m, int p, int max_iter,
+                double tol, int precond, aar_hip_stats *stats);
 int pinv_host(const double *A, const double *b, double *x, int m, double *w_out, double *v_out);
 void release_all();
 void prof_flush();

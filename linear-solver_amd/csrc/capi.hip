@@ -719,6 +719,13 @@ int aar_hip_aar_refined(const aar_hip_grid *g, double *x, const double *b, doubl
     return aar_refined(op, x, b, omega, beta, m, p, max_outer, max_inner, tol, inner_tol, precond, st);
 }
 
+int aar_hip_aar_batch(const aar_hip_instance *inst, int count, double omega, double beta, int m, int p,
+                      int max_iter, double tol, int precond, int mixed, aar_hip_stats *stats)
+{
+    if (mixed) return batch_solve<double, float>(inst, count, omega, beta, m, p, max_iter, tol, precond, stats);
+    return batch_solve<double, double>(inst, count, omega, beta, m, p, max_iter, tol, precond, stats);
+}
+
 int aar_hip_helm_matvec(const aar_hip_helm *h, const double *u, double *y)
 {
     HOp op;

@@ -372,7 +372,7 @@ struct WGrQr<M, -1> {
 };
 
 template <int M>
-__global__ __launch_bounds__(64) void ls_kernel(Ctrl *c, int n)
+__device__ __forceinline__ void ls_body(Ctrl *c, const int n)
 {
     __shared__ double A[M][M];
     __shared__ double V[M][M];
@@ -549,6 +549,27 @@ __global__ __launch_bounds__(64) void ls_kernel(Ctrl *c, int n)
         c->status = 1;
         c->done = 1;
     }
+}
+
+template <int M>
+__global__ __launch_bounds__(64) void ls_kernel(Ctrl *c, int n)
+{
+    ls_body<M>(c, n);
+}
+
+// one wavefront per solve of a batch
+template <int M>
+__global__ __launch_bounds__(64) void ls_batch_kernel(Ctrl *const *ctrls, int n)
+{
+    ls_body<M>(ctrls[blockIdx.x], n);
+}
+
+hipError_t launch_ls_batch(Ctrl *const *ctrls, int count, int n, hipStream_t st)
+{
+    if (n < 1 || n > MAXM_LS || count < 1) return hipErrorInvalidValue;
+    if (n <= 8) hipLaunchKernelGGL((ls_batch_kernel<8>), dim3(count), dim3(64), 0, st, ctrls, n);
+    else hipLaunchKernelGGL((ls_batch_kernel<16>), dim3(count), dim3(64), 0, st, ctrls, n);
+    return hipGetLastError();
 }
 
 hipError_t launch_ls(Ctrl *ctrl, int n, int /*complex_embed*/, hipStream_t st)

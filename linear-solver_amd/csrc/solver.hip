@@ -34,7 +34,7 @@ namespace aar {
 static Context g_ctx;
 Context &ctx() { return g_ctx; }
 
-static hipError_t ensure(DevBuf &b, size_t bytes)
+hipError_t ensure(DevBuf &b, size_t bytes)
 {
     if (b.p && b.bytes >= bytes) return hipSuccess;
     if (b.p) {
@@ -121,7 +121,7 @@ int make_op(const aar_hip_grid *g, const double *coeff, Op &op)
 // ---------------------------------------------------------------------------
 // launch helpers
 // ---------------------------------------------------------------------------
-static int auto_lz(const Op &op, int wy, int py, size_t es)
+int auto_lz(const Op &op, int wy, int py, size_t es)
 {
     const int ntx = (op.nx + TX - 1) / TX, nty = (op.ny + wy * py - 1) / (wy * py);
     const int cols = ntx * nty;
@@ -154,13 +154,13 @@ SArgs<T> base_sargs(const Op &op)
 // option "ring_pad" (aar_hip_set_option), so that equal offsets in different slots
 // do not fall on the same HBM channel when N*sizeof(T) is a large power of
 // two (512^3 FP64 slots are exactly 2^30 B apart).
-static long ring_ld(long n, size_t es)
+long ring_ld(long n, size_t es)
 {
     const long q = 256 / (long)es;
     return (n + q - 1) / q * q + ctx().ring_pad;
 }
 
-static int nblocks_for(long n)
+int nblocks_for(long n)
 {
     const long cap = ctx().hist_blocks > 0 ? ctx().hist_blocks : 2048;
     long nb = (n + 4095) / 4096;
@@ -248,7 +248,7 @@ int run_stencil(SMode mode, SArgs<T> a, double bytes)
 }
 
 template <typename T>
-static hipError_t run_dot(DotArgs<T> d)
+hipError_t run_dot(DotArgs<T> d)
 {
     Context &C = ctx();
     const bool two = C.two_pass;
@@ -293,7 +293,7 @@ int stencil_partials_needed(const Op &op)
 }
 
 // poll ring: record ctrl->done after a cycle; true once done is observed
-static int poll_record(Ctrl *ctrl, int cyc)
+int poll_record(Ctrl *ctrl, int cyc)
 {
     Context &C = ctx();
     const int s = cyc % NPOLL;
@@ -302,7 +302,7 @@ static int poll_record(Ctrl *ctrl, int cyc)
     return 0;
 }
 
-static bool poll_check(int cyc)
+bool poll_check(int cyc)
 {
     Context &C = ctx();
     if (cyc < 0) return false;
@@ -622,6 +622,9 @@ int aar_mixed(const Op &op, double *x, const double *b, double omega, double bet
 
 template int aar_solve<double>(const Op &, double *, const double *, double, double, int, int, int, double,
                                int, int, aar_hip_stats *, int, int);
+template SArgs<double> base_sargs<double>(const Op &);
+template SArgs<float> base_sargs<float>(const Op &);
+template hipError_t run_dot<double>(DotArgs<double>);
 template int aar_solve<float>(const Op &, float *, const float *, double, double, int, int, int, double,
                               int, int, aar_hip_stats *, int, int);
 

@@ -20,388 +20,9 @@
 //   (whose halos overlap) run on the same XCD and share its L2.
 // The epilogue of each mode fuses the pointwise work of the solver step
 // that consumes A*x, so that step costs one pass over the grid.
-#include "aar_common.h"
-#include "aar_device.h"
-#include "aar_kernels.h"
+#include "stencil_body.h"
 
 namespace aar {
-
-template <typename T> struct Arith;
-template <> struct Arith<double> {
-    using W = double;
-    __device__ static inline double add(double a, double b) { return __dadd_rn(a, b); }
-    __device__ static inline double sub(double a, double b) { return __dsub_rn(a, b); }
-    __device__ static inline double mul(double a, double b) { return __dmul_rn(a, b); }
-    __device__ static inline double div(double a, double b) { return a / b; }
-    __device__ static inline double sq(double a) { return a * a; }
-};
-template <> struct Arith<float> {
-    using W = float;
-    __device__ static inline float add(float a, float b) { return __fadd_rn(a, b); }
-    __device__ static inline float sub(float a, float b) { return __fsub_rn(a, b); }
-    __device__ static inline float mul(float a, float b) { return __fmul_rn(a, b); }
-    __device__ static inline float div(float a, float b) { return __fdiv_rn(a, b); }
-    __device__ static inline double sq(float a) { return (double)a * (double)a; }
-};
-
-template <SMode M> struct ModeInfo;
-template <> struct ModeInfo<SMode::MATVEC> { static constexpr int NF = 1, NE = 0, NACC = 1; };
-template <> struct ModeInfo<SMode::MATVEC_RES> { static constexpr int NF = 1, NE = 1, NACC = 1; };
-template <> struct ModeInfo<SMode::AAR_TRUE> { static constexpr int NF = 1, NE = 1, NACC = 1; };
-template <> struct ModeInfo<SMode::AAR_REC> { static constexpr int NF = 1, NE = 0, NACC = 1; };
-template <> struct ModeInfo<SMode::RESID> { static constexpr int NF = 1, NE = 1, NACC = 1; };
-template <> struct ModeInfo<SMode::PGR_RICH> { static constexpr int NF = 3, NE = 1, NACC = 1; };
-template <> struct ModeInfo<SMode::CG_Q> { static constexpr int NF = 2, NE = 0, NACC = 1; };
-template <> struct ModeInfo<SMode::CG_RESID> { static constexpr int NF = 1, NE = 1, NACC = 2; };
-
-__device__ inline int modp(int i, int n) { i %= n; return i < 0 ? i + n : i; }
-
-template <typename T, int WY, int PY, SMode MODE>
-__global__ __launch_bounds__(64 * WY) void stencil_kernel(const SArgs<T> a)
-{
-    using A = Arith<T>;
-    constexpr int NT = 64 * WY;                       // threads: WY wavefronts stacked in y
-    constexpr int NF = ModeInfo<MODE>::NF;
-    constexpr int NE = ModeInfo<MODE>::NE;
-    constexpr int NACC = ModeInfo<MODE>::NACC;
-    constexpr int TY = WY * PY;
-    constexpr int LH = TY + 2 * R;
-    constexpr int LW = TX + 2 * R;
-    constexpr int NYH = (2 * R + WY - 1) / WY;        // y-halo rows per thread row
-    constexpr int NXH = (2 * R * TY + NT - 1) / NT;   // x-halo cells per thread
-    constexpr int QN = 2 * R + 1;
-
-    constexpr int HR = R + 1;                         // halo ring depth (planes)
-    constexpr int NH = NYH + NXH;                     // halo cells per thread
-    __shared__ T lds[2][LH][LW];
-    // Halo cells are fetched R planes ahead, in the same step as the
-    // neighbouring blocks fetch those lines as their own queue planes, so
-    // that they are served from L2; each thread parks its cells here until
-    // their plane is computed (slots private to the thread: no barrier).
-    __shared__ T hring[HR][NH][NT];
-    __shared__ double red_scratch[8 * WY];
-    __shared__ double red_out[8];
-
-    if (a.ctrl && a.ctrl->done) return;
-
-    const int item = xcd_item(blockIdx.x, a.nitems);
-    const int itx = item % a.ntx;
-    const int rest = item / a.ntx;
-    const int ity = rest % a.nty;
-    const int izc = rest / a.nty;
-    const int x0 = itx * TX, y0 = ity * TY;
-    const int z0 = izc * a.lz;
-    const int z1 = min(z0 + a.lz, a.nz);
-    const int tid = threadIdx.x, tx = tid & 63, ty = tid >> 6;
-    const int nx = a.nx, ny = a.ny, nz = a.nz, nxy = a.nxy;
-
-    // in-plane offsets of this thread's points, halo cells and LDS slots
-    const int cx = modp(x0 + tx, nx);
-    int c_off[PY];
-    bool c_ok[PY];
-#pragma unroll
-    for (int p = 0; p < PY; p++) {
-        const int y = y0 + ty * PY + p;
-        c_off[p] = cx + modp(y, ny) * nx;
-        c_ok[p] = (x0 + tx < nx) && (y < ny);
-    }
-    int yh_off[NYH], yh_row[NYH];
-    bool yh_ok[NYH];
-#pragma unroll
-    for (int k = 0; k < NYH; k++) {
-        const int hr0 = ty + WY * k;                       // 0 .. 2R-1 when valid
-        yh_ok[k] = hr0 < 2 * R;
-        const int hr = yh_ok[k] ? hr0 : 0;
-        const int gy = hr < R ? y0 - R + hr : y0 + TY + (hr - R);
-        yh_off[k] = cx + modp(gy, ny) * nx;
-        yh_row[k] = hr < R ? hr : TY + hr;
-    }
-    int xh_off[NXH], xh_row[NXH], xh_col[NXH];
-    bool xh_ok[NXH];
-#pragma unroll
-    for (int k = 0; k < NXH; k++) {
-        const int c = tid + NT * k;
-        xh_ok[k] = c < 2 * R * TY;
-        const int r = xh_ok[k] ? c / (2 * R) : 0, cc = xh_ok[k] ? c % (2 * R) : 0;
-        const int gx = cc < R ? x0 - R + cc : x0 + TX + (cc - R);
-        // cells past the halo load a valid (unused) address so that every
-        // load is unconditional; only their LDS write is predicated
-        xh_off[k] = xh_ok[k] ? modp(gx, nx) + modp(y0 + r, ny) * nx : c_off[0];
-        xh_row[k] = R + r;
-        xh_col[k] = cc < R ? cc : TX + cc;
-    }
-
-    const T wt0 = (T)a.w[0];
-    T wt[R + 1];
-#pragma unroll
-    for (int o = 0; o <= R; o++) wt[o] = (T)a.w[o];
-    const T sig = (T)a.sigma, dg = (T)a.diag, om = (T)a.omega;
-    T cf[R + 1];
-#pragma unroll
-    for (int o = 0; o <= R; o++) cf[o] = (T)a.cf[o];
-    const double trace_scale = a.precond ? a.diag : 1.0;
-    const T cgb = (MODE == SMode::CG_Q && a.cg_beta) ? (T)(*a.cg_beta) : (T)0;
-
-    auto load_raw = [&](int off, T (&v)[NF]) {
-        v[0] = a.in0[off];
-        if constexpr (NF > 1) v[1] = a.in1[off];
-        if constexpr (NF > 2) v[2] = a.in2[off];
-    };
-    // stencil operand u at a point from its loaded fields
-    auto make_u = [&](const T (&v)[NF]) -> T {
-        if constexpr (MODE == SMode::PGR_RICH) {
-            // PGR.c:92-98: f = (rhs - Ax) / diag; x_new = x + omega*f
-            T f = A::sub(v[1], v[2]);
-            if (a.precond) f = A::div(f, dg);
-            return A::add(v[0], A::mul(om, f));
-        } else if constexpr (MODE == SMode::CG_Q) {
-            // CG.c:128-150: s = r/diag; d = s + beta*d  (first pass: d as given)
-            if (a.cg_first) return v[1];
-            T s = v[0];
-            if (a.precond) s = A::div(s, dg);
-            return A::add(s, A::mul(cgb, v[1]));
-        } else {
-            return v[0];
-        }
-    };
-
-    T q[PY][QN];
-    // prefetch register set: halo + epilogue inputs of one plane and the
-    // queue plane R ahead of it.  Two sets alternate (the plane loop is
-    // unrolled by two) so that a set stays in flight during the barrier and
-    // compute of the previous plane with no register copies of pending loads.
-    struct Pre {
-        T qn[PY][NF];
-        T hy[NYH][NF];
-        T hx[NXH][NF];
-        T en[PY][NE > 0 ? NE : 1];
-    };
-    Pre s0, s1;
-
-    // prologue: queue planes z0-R .. z0+R-1
-#pragma unroll
-    for (int s = 0; s < 2 * R; s++) {
-        const int zo = modp(z0 - R + s, nz) * nxy;
-#pragma unroll
-        for (int p = 0; p < PY; p++) {
-            T v[NF];
-            load_raw(c_off[p] + zo, v);
-            q[p][s] = make_u(v);
-        }
-    }
-    // halo cells of planes z0 .. z0+R-1 into the ring
-#pragma unroll
-    for (int s = 0; s < R; s++) {
-        const int zo = modp(z0 + s, nz) * nxy;
-#pragma unroll
-        for (int k = 0; k < NYH; k++) {
-            T v[NF];
-            load_raw(yh_off[k] + zo, v);
-            hring[s][k][tid] = make_u(v);
-        }
-#pragma unroll
-        for (int k = 0; k < NXH; k++) {
-            T v[NF];
-            load_raw(xh_off[k] + zo, v);
-            hring[s][NYH + k][tid] = make_u(v);
-        }
-    }
-    // queue plane and halo cells of plane zp+R, epilogue inputs of plane zp
-    // (indices wrap, so the prefetch past the last plane is a harmless
-    // unused load)
-    auto prefetch = [&](int zp, Pre &st) {
-        const int zq = modp(zp + R, nz) * nxy, zc = modp(zp, nz) * nxy;
-#pragma unroll
-        for (int p = 0; p < PY; p++) load_raw(c_off[p] + zq, st.qn[p]);
-#pragma unroll
-        for (int k = 0; k < NYH; k++) load_raw(yh_off[k] + zq, st.hy[k]);
-#pragma unroll
-        for (int k = 0; k < NXH; k++) load_raw(xh_off[k] + zq, st.hx[k]);
-        if constexpr (NE > 0) {
-#pragma unroll
-            for (int p = 0; p < PY; p++) st.en[p][0] = a.bvec[c_off[p] + zc];
-        }
-    };
-
-    double acc[NACC];
-#pragma unroll
-    for (int k = 0; k < NACC; k++) acc[k] = 0.0;
-
-    auto plane = [&](const int z, Pre &cur, Pre &nxt) {
-        const int buf = (z - z0) & 1;
-        // queue top <- plane z+R; LDS <- plane z (centre + halo)
-#pragma unroll
-        for (int p = 0; p < PY; p++) {
-            q[p][2 * R] = make_u(cur.qn[p]);
-            lds[buf][R + ty * PY + p][R + tx] = q[p][R];
-        }
-        // halo of plane z from the ring; park the cells of plane z+R
-        const int rs = (z - z0) % HR, ws = (z - z0 + R) % HR;
-#pragma unroll
-        for (int k = 0; k < NYH; k++)
-            if (yh_ok[k]) lds[buf][yh_row[k]][R + tx] = hring[rs][k][tid];
-#pragma unroll
-        for (int k = 0; k < NXH; k++)
-            if (xh_ok[k]) lds[buf][xh_row[k]][xh_col[k]] = hring[rs][NYH + k][tid];
-#pragma unroll
-        for (int k = 0; k < NYH; k++) hring[ws][k][tid] = make_u(cur.hy[k]);
-#pragma unroll
-        for (int k = 0; k < NXH; k++) hring[ws][NYH + k][tid] = make_u(cur.hx[k]);
-        prefetch(z + 1, nxt);
-        __syncthreads();
-
-            const int zoff = z * nxy;
-#pragma unroll
-            for (int p = 0; p < PY; p++) {
-                const int lr = R + ty * PY + p, lc = R + tx;
-                const T u = q[p][R];
-                // y-neighbour at row offset d: rows of this thread come from
-                // its own registers, the others from the LDS tile (p and d are
-                // compile-time constants after unrolling, so this folds)
-                auto ynb = [&](int d) -> T {
-                    const int pp = p + d;
-                    if (pp >= 0 && pp < PY) return q[pp < 0 ? 0 : (pp < PY ? pp : PY - 1)][R];
-                    return lds[buf][lr + d][lc];
-                };
-                // system.c:195-201 order: centre, then o = 1..6 with
-                // (x-, x+, y-, y+, z-, z+) summed left to right
-                T y;
-                if constexpr (MODE == SMode::AAR_REC) {
-                    // the Richardson step folded into the weights: g' = cf0 g +
-                    // sum_o cf_o S_o, cf0 = 1 - c (3 w0 + sigma), cf_o = -c w_o,
-                    // c = omega/diag (no reference counterpart; the reference
-                    // recomputes b - A x, see the solver)
-                    y = A::mul(u, cf[0]);
-#pragma unroll
-                    for (int o = 1; o <= R; o++) {
-                        T nb = A::add(lds[buf][lr][lc - o], lds[buf][lr][lc + o]);
-                        nb = A::add(nb, ynb(-o));
-                        nb = A::add(nb, ynb(o));
-                        nb = A::add(nb, q[p][R - o]);
-                        nb = A::add(nb, q[p][R + o]);
-                        y = fma(nb, cf[o], y);
-                    }
-                } else if constexpr (sizeof(T) == 8) {
-                    y = A::mul(A::mul(u, (T)3), wt0);
-#pragma unroll
-                    for (int o = 1; o <= R; o++) {
-                        T nb = A::add(lds[buf][lr][lc - o], lds[buf][lr][lc + o]);
-                        nb = A::add(nb, ynb(-o));
-                        nb = A::add(nb, ynb(o));
-                        nb = A::add(nb, q[p][R - o]);
-                        nb = A::add(nb, q[p][R + o]);
-                        y = A::add(y, A::mul(nb, wt[o]));
-                    }
-                } else {
-                    // FP32 (no reference counterpart): difference form
-                    // sum_o w_o * sum_nb (x_nb - x_c), i.e. the centre weight taken
-                    // as -2 sum_o w_o.  Row sums are exactly zero and each rounding
-                    // is relative to a local difference, so the smooth modes, whose
-                    // eigenvalues are tiny, keep their accuracy; the reference form
-                    // would shift them by ~eps32 * |w_0| and stall FP32 iterations.
-                    y = (T)0;
-#pragma unroll
-                    for (int o = 1; o <= R; o++) {
-                        T nb = A::add(A::sub(lds[buf][lr][lc - o], u), A::sub(lds[buf][lr][lc + o], u));
-                        nb = A::add(nb, A::sub(ynb(-o), u));
-                        nb = A::add(nb, A::sub(ynb(o), u));
-                        nb = A::add(nb, A::sub(q[p][R - o], u));
-                        nb = A::add(nb, A::sub(q[p][R + o], u));
-                        y = A::add(y, A::mul(nb, wt[o]));
-                    }
-                }
-                if (MODE != SMode::AAR_REC && a.sigma != 0.0) y = A::add(y, A::mul(sig, u));
-                // points past the grid edge (partial tiles) computed on wrapped
-                // inputs: only their stores and norm terms are masked, so the
-                // stencil itself runs without branches
-                const bool ok = c_ok[p];
-                const int g = c_off[p] + zoff;
-                if constexpr (MODE == SMode::MATVEC) {
-                    if (ok) a.out0[g] = y;
-                } else if constexpr (MODE == SMode::MATVEC_RES) {
-                    // y = A x and ||b - A x||^2 (PGR.c:79-81 start-up)
-                    if (ok) a.out0[g] = y;
-                    acc[0] += ok ? A::sq(A::sub(cur.en[p][0], y)) : 0.0;
-                } else if constexpr (MODE == SMode::AAR_TRUE) {
-                    // AAR.c:80-82/93-94/119-123: f = (rhs - A x)/diag, stored in
-                    // T or (mixed precision) rounded to FP32; ||rhs - A x||^2
-                    const T r = A::sub(cur.en[p][0], y);
-                    acc[0] += ok ? A::sq(r) : 0.0;
-                    T f = r;
-                    if (a.precond) f = A::div(f, dg);
-                    if (ok) {
-                        if (a.out32) a.out32[g] = (float)f;
-                        else a.out0[g] = f;
-                    }
-                } else if constexpr (MODE == SMode::AAR_REC) {
-                    // y = g_{k+1} = (rhs - A(x_k + omega g_k))/diag; scale^2 *
-                    // ||g_{k+1}||^2 = ||rhs - A x_{k+1}||^2 for the history
-                    if (ok) a.out0[g] = y;
-                    if (a.trace) {
-                        const double sv = ok ? (double)y * trace_scale : 0.0;
-                        acc[0] = fma(sv, sv, acc[0]);
-                    }
-                } else if constexpr (MODE == SMode::RESID) {
-                    const T r = A::sub(cur.en[p][0], y);
-                    acc[0] += ok ? A::sq(r) : 0.0;
-                    if (ok) {
-                        if (a.out32) a.out32[g] = (float)r;
-                        if (a.out0) a.out0[g] = r;
-                    }
-                } else if constexpr (MODE == SMode::PGR_RICH) {
-                    // PGR.c:95-103: x_old=x; x += omega*f; Ax = A x; f = rhs - Ax
-                    if (ok) {
-                        a.out0[g] = u;
-                        a.out1[g] = y;
-                    }
-                    if (a.trace) acc[0] += ok ? A::sq(A::sub(cur.en[p][0], y)) : 0.0;
-                } else if constexpr (MODE == SMode::CG_Q) {
-                    // CG.c:94-100: q = A d; alpha partial d.q
-                    if (ok) {
-                        a.out0[g] = u;
-                        a.out1[g] = y;
-                    }
-                    acc[0] += ok ? (double)u * (double)y : 0.0;
-                } else if constexpr (MODE == SMode::CG_RESID) {
-                    // CG.c:65-84 / 112-119: r = b - A x; s = r/diag; r.s, r.r
-                    const T r = A::sub(cur.en[p][0], y);
-                    T s = r;
-                    if (a.precond) s = A::div(s, dg);
-                    if (ok) {
-                        a.out0[g] = r;
-                        if (a.out1) a.out1[g] = s;
-                    }
-                    acc[0] += ok ? (double)r * (double)s : 0.0;
-                    acc[1] += ok ? A::sq(r) : 0.0;
-                }
-            }
-        // shift the queue one plane
-#pragma unroll
-        for (int p = 0; p < PY; p++)
-#pragma unroll
-            for (int s = 0; s < 2 * R; s++) q[p][s] = q[p][s + 1];
-    };
-
-    prefetch(z0, s0);
-    int z = z0;
-    for (; z + 1 < z1; z += 2) {
-        plane(z, s0, s1);
-        plane(z + 1, s1, s0);
-    }
-    if (z < z1) plane(z, s0, s1);
-
-    if (a.partials) {
-        block_reduce_to<NACC, NT>(acc, a.partials + (size_t)item * NACC, red_scratch);
-        if (!a.counter) return;                 // two-pass: finish_kernel follows
-        if (last_block_arrive(a.counter, gridDim.x)) {
-            final_sum<NT>(a.partials, a.nitems, NACC, red_out, red_scratch);
-            if (tid == 0) {
-                finalize(a.fin, a.ctrl, red_out, NACC);
-                *a.counter = 0u;
-            }
-        }
-    }
-}
 
 // ---------------------------------------------------------------------------
 // host-side launch
@@ -463,6 +84,8 @@ int stencil_items(const SArgs<T> &a0, int wy, int py)
 template hipError_t launch_stencil<double>(SMode, const SArgs<double> &, int, int, hipStream_t);
 template hipError_t launch_stencil<float>(SMode, const SArgs<float> &, int, int, hipStream_t);
 template int stencil_items<double>(const SArgs<double> &, int, int);
+template void stencil_geometry<double>(SArgs<double> &, int, int);
+template void stencil_geometry<float>(SArgs<float> &, int, int);
 template int stencil_items<float>(const SArgs<float> &, int, int);
 
 }  // namespace aar

@@ -1,0 +1,79 @@
+"""Batched AAR: uneven grids, screened operators and several right-hand
+sides solved together (one graph-captured Anderson cycle per replay).
+
+Every instance must give the reference's iteration count (the oracle's,
+bit-faithful to MPI_REAL/AAR.c) and its solution within 1e-8, in FP64 and
+in mixed precision, and the FP64 batch must agree with the single-solve
+path on the same instance.
+"""
+import numpy as np
+import pytest
+
+from aar_hip import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def dev(v):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64)).to("cuda")
+
+
+def instances():
+    """(shape, h, sigma, rhs): uneven grids, Poisson and screened, and three
+    right-hand sides sharing one operator."""
+    out = [((16, 16, 16), 0.5, 0.5, P.rhs_normal((16, 16, 16), 21)),
+           ((24, 24, 24), 0.5, 0.0, P.rhs_uniform((24, 24, 24), 22)),
+           ((20, 12, 16), 0.5, 1.7, P.rhs_normal((20, 12, 16), 23)),
+           ((32, 32, 32), 0.25, 0.1, P.rhs_normal((32, 32, 32), 24))]
+    for s in range(3):
+        out.append(((24, 24, 24), 0.5, 0.8, P.rhs_normal((24, 24, 24), 30, stream=s)))
+    return out
+
+
+SCHEDULES = [  # m, p, max_iter, tol
+    (7, 6, 1000, 1e-8),
+    (4, 1, 1000, 1e-8),
+    (3, 4, 1000, 1e-8),
+    (7, 6, 13, 1e-8),
+]
+
+
+@pytest.mark.parametrize("sched", SCHEDULES, ids=lambda s: f"m{s[0]}p{s[1]}i{s[2]}")
+@pytest.mark.parametrize("mixed", [False, True], ids=["fp64", "mixed"])
+def test_batch_vs_oracle(gpu, oracle, sched, mixed):
+    m, p, max_iter, tol = sched
+    say("batch", sched, "mixed" if mixed else "fp64")
+    inst = instances()
+    xs = [dev(np.zeros(b.size)) for (_, _, _, b) in inst]
+    bs = [dev(b) for (_, _, _, b) in inst]
+    args = [(xs[i], bs[i], shape, P.poisson_a(h), sig) for i, (shape, h, sig, _) in enumerate(inst)]
+    sts = gpu.AAR_batch(args, 0.6, 0.6, m, p, max_iter, tol, mixed=mixed)
+    for i, (shape, h, sig, b) in enumerate(inst):
+        xr, dr = oracle.aar(np.zeros(b.size), b, shape, P.poisson_a(h), 0.6, 0.6, m, p, max_iter, tol,
+                            sigma=sig, trace=False)
+        say(f"  instance {i} {shape} sigma={sig}: gpu {sts[i]['iterations']} oracle {dr['iterations']}")
+        assert sts[i]["iterations"] == dr["iterations"]
+        assert bool(sts[i]["printed_converged"]) == bool(dr["printed_converged"])
+        x = xs[i].cpu().numpy()
+        lim = 1e-8 if dr["printed_converged"] else 1e-6
+        assert np.linalg.norm(x - xr) / np.linalg.norm(xr) <= lim
+
+
+def test_batch_matches_single_solves(gpu):
+    """The FP64 batch and the single-solve path compute the same iterates."""
+    inst = instances()
+    xs = [dev(np.zeros(b.size)) for (_, _, _, b) in inst]
+    bs = [dev(b) for (_, _, _, b) in inst]
+    args = [(xs[i], bs[i], shape, P.poisson_a(h), sig) for i, (shape, h, sig, _) in enumerate(inst)]
+    sts = gpu.AAR_batch(args, 0.6, 0.6, 7, 6, 1000, 1e-10)
+    for i, (shape, h, sig, b) in enumerate(inst):
+        x1 = dev(np.zeros(b.size))
+        st = gpu.AAR(x1, bs[i], shape, P.poisson_a(h), 0.6, 0.6, 7, 6, 1000, 1e-10, sigma=sig)
+        assert st["iterations"] == sts[i]["iterations"]
+        r = float(((x1 - xs[i]).norm() / x1.norm()).cpu())
+        assert r < 1e-10, (i, r)
