@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--cpu-iters", type=int, default=12)
     ap.add_argument("--no-helm", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     return ap.parse_args()
 
 
@@ -95,7 +96,7 @@ def cpu_baseline(n_cpu, iters, gpu_iterations, n_gpu):
     }
 
 
-def helmholtz_case(n, dev, a, directions=20, hstep=1e-6):
+def helmholtz_case(n, dev, a, directions=20, hstep=1e-6, tol=1e-13):
     """One forward solve and its adjoint (autograd backward) of
     L = 1/2 ||u(b, c) - u*||^2 on the n^3 complex Helmholtz problem, then
     the gradients checked by central differences."""
@@ -113,7 +114,7 @@ def helmholtz_case(n, dev, a, directions=20, hstep=1e-6):
         return torch.complex(v[0::2].contiguous(), v[1::2].contiguous())
 
     b, bs = crhs(0), crhs(1)
-    opts = dict(omega=0.6, beta=0.6, m=7, p=6, max_iter=20000, tol=1e-13)
+    opts = dict(omega=0.6, beta=0.6, m=7, p=6, max_iter=20000, tol=tol)
     ustar = torch.zeros_like(b)
     aar_hip.helm_aar(ustar, bs, shape, a, c, 0.0, eta, **opts)
 
@@ -134,17 +135,27 @@ def helmholtz_case(n, dev, a, directions=20, hstep=1e-6):
     gen = torch.Generator(device=dev)
     gen.manual_seed(2024)
     eb, ec = [], []
+
+    def solve(b_, c_):
+        return aar_hip.HelmholtzSolve.apply(b_, c_, shape, a, eta, opts)
+
+    def central(up, um):
+        # (L(+) - L(-)) / 2h with L(+) - L(-) = 1/2 (||u+ - u*||^2 - ||u- - u*||^2)
+        # = Re <u+ - u-, (u+ + u-)/2 - u*>, exactly; summed in this form the
+        # difference does not cancel against L itself
+        return float(torch.sum((torch.conj(up - um) * (0.5 * (up + um) - ustar)).real)) / (2 * hstep)
+
     with torch.no_grad():
         for _ in range(directions):
             db = torch.complex(torch.randn(N, dtype=torch.float64, device=dev, generator=gen),
                                torch.randn(N, dtype=torch.float64, device=dev, generator=gen))
             dc = torch.randn(N, dtype=torch.float64, device=dev, generator=gen)
-            fd = (loss(b + hstep * db, c) - loss(b - hstep * db, c)) / (2 * hstep)
-            ad = torch.sum((gb.conj() * db).real)
-            eb.append(abs(float(fd - ad)) / abs(float(ad)))
-            fd = (loss(b, c + hstep * dc) - loss(b, c - hstep * dc)) / (2 * hstep)
-            ad = torch.sum(gc * dc)
-            ec.append(abs(float(fd - ad)) / abs(float(ad)))
+            fd = central(solve(b + hstep * db, c), solve(b - hstep * db, c))
+            ad = float(torch.sum((gb.conj() * db).real))
+            eb.append(abs(fd - ad) / abs(ad))
+            fd = central(solve(b, c + hstep * dc), solve(b, c - hstep * dc))
+            ad = float(torch.sum(gc * dc))
+            ec.append(abs(fd - ad) / abs(ad))
     f, adj = last["forward"], last["adjoint"]
     return {"grid": [n, n, n], "h": 0.25, "eta": eta, "tol": opts["tol"],
             "forward_iterations": f["iterations"], "forward_ms": f["solve_ms"],
@@ -152,6 +163,71 @@ def helmholtz_case(n, dev, a, directions=20, hstep=1e-6):
             "solves_per_s": 1e3 / f["solve_ms"], "forward_backward_s": t_fb,
             "fd_directions": directions, "fd_step": hstep,
             "fd_max_rel_err_b": float(np.max(eb)), "fd_max_rel_err_c": float(np.max(ec))}
+
+
+def batch_case(dev, seed=1004):
+    """96 screened-Poisson instances, side drawn from {24,32,48,64,96,128},
+    sigma_i ~ U[0.1, 2], b_i ~ N(0,1), plus 16 right-hand sides on one 128^3
+    operator; h = 0.5.  FP64 and mixed batches (one graph-replayed Anderson
+    cycle), slab depths autotuned per size; the same instances solved one
+    by one on the single-solve path for comparison."""
+    import numpy as np
+    import torch
+    import aar_hip
+    from aar_hip import problems as P
+    sides = np.array([24, 32, 48, 64, 96, 128])
+    u = P.uniform01(3 * 96 + 1, seed, 7)
+    inst = []
+    for i in range(96):
+        side = int(sides[min(5, int(u[2 * i] * 6))])
+        sigma = 0.1 + 1.9 * float(u[2 * i + 1])
+        inst.append(((side,) * 3, sigma, i))
+    sig16 = 0.1 + 1.9 * float(u[-1])
+    for r in range(16):
+        inst.append(((128,) * 3, sig16, 96 + r))
+    a = P.poisson_a(0.5)
+    m, p, tol, max_iter = 7, 6, 1e-6, 5000
+    t0 = time.perf_counter()
+    tuned = {s_: aar_hip.autotune((s_,) * 3, batch=True)["lz"] for s_ in sorted({sh[0] for sh, _, _ in inst})}
+    tuned.update({f"{s_}_f32": aar_hip.autotune((s_,) * 3, fp32=True, batch=True)["lz"]
+                  for s_ in sorted({sh[0] for sh, _, _ in inst})})
+    t_tune = time.perf_counter() - t0
+    bs = [P.normal_torch(int(np.prod(sh)), seed, 100 + k, dev) for sh, _, k in inst]
+    xs = [torch.zeros_like(b_) for b_ in bs]
+    points = int(sum(np.prod(sh) for sh, _, _ in inst))
+    out = {"instances": len(inst), "points": points, "sides": [sh[0] for sh, _, _ in inst],
+           "sigma": [round(sg, 4) for _, sg, _ in inst], "h": 0.5, "m": m, "p": p, "tol": tol,
+           "autotune_s": t_tune, "tuned_lz": tuned}
+    for label, mixed in (("fp64", False), ("mixed", True)):
+        for x_ in xs:
+            x_.zero_()
+        args = [(xs[i], bs[i], sh, a, sg) for i, (sh, sg, _) in enumerate(inst)]
+        aar_hip.AAR_batch(args, 0.6, 0.6, m, p, max_iter, tol, mixed=mixed)       # warm-up
+        for x_ in xs:
+            x_.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sts = aar_hip.AAR_batch(args, 0.6, 0.6, m, p, max_iter, tol, mixed=mixed)
+        torch.cuda.synchronize()
+        tb = time.perf_counter() - t0
+        out[label] = {"seconds": tb, "instance_solves_per_s": len(inst) / tb,
+                      "iterations": [s_["iterations"] for s_ in sts],
+                      "all_converged": all(bool(s_["converged"]) for s_ in sts)}
+        log(f"batch {label}: {tb * 1e3:.0f} ms, {len(inst) / tb:.1f} instance-solves/s")
+    # one by one (FP64 single-solve path) on the same instances
+    its1 = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i, (sh, sg, _) in enumerate(inst):
+        x1 = torch.zeros_like(bs[i])
+        its1.append(aar_hip.AAR(x1, bs[i], sh, a, 0.6, 0.6, m, p, max_iter, tol, sigma=sg)["iterations"])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter() - t0
+    out["single_fp64"] = {"seconds": t1, "instance_solves_per_s": len(inst) / t1,
+                          "iterations_equal_batch": its1 == out["fp64"]["iterations"]}
+    out["batch_speedup_vs_single"] = t1 / out["fp64"]["seconds"]
+    log(f"single fp64: {t1 * 1e3:.0f} ms; iterations equal: {its1 == out['fp64']['iterations']}")
+    return out
 
 
 def main():
@@ -279,9 +355,19 @@ def main():
     # the gradients against central differences (step 1e-6) on 20 directions
     helm = {}
     if not args.no_helm:
-        for nh in (128, 256):
-            helm[str(nh)] = helmholtz_case(nh, dev, a)
+        for nh, tl in ((128, 1e-13), (256, 1e-13)):
+            try:
+                helm[str(nh)] = helmholtz_case(nh, dev, a, tol=tl)
+            except Exception as e:  # noqa: BLE001
+                helm[str(nh)] = {"error": str(e)}
             log(f"helmholtz {nh}^3: {helm[str(nh)]}")
+
+    batch = None
+    if not args.no_batch:
+        try:
+            batch = batch_case(dev)
+        except Exception as e:  # noqa: BLE001
+            batch = {"error": str(e)}
 
     # roofline of the dominant kernel: one profiled FP64 solve, HIP events
     # around every launch of the recurrence stencil sweep on the library stream
@@ -361,6 +447,7 @@ def main():
         },
         "other_solvers": others,
         "helmholtz": helm,
+        "batch": batch,
         "matvec": {"GBps": mv_gbs, "frac_of_peak": mv_gbs / HBM_PEAK_GBS,
                    "avg_ms": mv["ms"] / mv["count"], "bytes_per_launch": 16.0 * N},
         "kernels": ktab(kern),

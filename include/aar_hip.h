@@ -165,8 +165,18 @@ int aar_hip_set_stream(void *stream);
 
 /* Stencil launch shape: lz planes per block (0 = automatic), wy
  * wavefronts per block stacked in y (4 or 8), py rows per thread (2 or 4);
- * a block owns a 64 x (wy*py) column.  Applies to subsequent calls. */
+ * a block owns a 64 x (wy*py) column.  Applies to subsequent calls and to
+ * every grid; (0, 8, 4) returns to the automatic shapes, tuned ones where
+ * a grid has been tuned. */
 int aar_hip_set_tuning(int lz, int wy, int py);
+
+/* Autotuner (extension): times the recurrence stencil sweep on an nx x ny x
+ * nz grid (fp32 = 1: the FP32 sweep of the mixed solver) over the launch
+ * shapes (wy, py) and z-slab depths lz and keeps the fastest for that grid;
+ * later solves on the grid use it.  batch_shape = 1 tunes only lz for the
+ * batched launch (aar_hip_aar_batch).  Returns the shape and its time per
+ * sweep in us.  Allocates two temporary vectors of the grid. */
+int aar_hip_autotune(int nx, int ny, int nz, int fp32, int batch_shape, int *lz, int *wy, int *py, double *us);
 
 /* Tuning options of the history sweeps (apply to subsequent solves):
  *   "ring_pad"    extra elements per ring slot, a multiple of 32 (default
@@ -174,7 +184,13 @@ int aar_hip_set_tuning(int lz, int wy, int py);
  *                 do not alias onto the same HBM channels;
  *   "hist_blocks" grid of the reducing sweeps (0 = automatic, <= 2048);
  *   "two_pass"    1 (default): reductions finish in a one-block second
- *                 kernel; 0: in the last block of the reducing kernel. */
+ *                 kernel; 0: in the last block of the reducing kernel;
+ *   "autotune"    1: a grid's stencil shape is tuned on its first solve
+ *                 (aar_hip_autotune); 0 (default): automatic shapes;
+ *   "complex_pinv_reference_order"  1: the complex solver's m x m SVD runs
+ *                 the left reflector in the order of tools.c:137 (from
+ *                 column 1); 0 (default): the textbook order, which
+ *                 converges on systems where that order does not. */
 int aar_hip_set_option(const char *name, long value);
 
 /* Per-kernel GPU timing with hipEvents (off by default).  Kernel names:
@@ -184,6 +200,13 @@ int aar_hip_set_option(const char *name, long value);
 int aar_hip_profile(int enable);
 int aar_hip_profile_reset(void);
 int aar_hip_profile_query(const char *name, int *count, double *total_ms, double *bytes);
+
+/* Diagnostics: the last m x m (complex: 2m x 2m) normal-equation system
+ * of the last solve, in the reference's allredvec layout (matrix A[i][j]
+ * at G[j*n + i], right-hand side at G[n*n + i]), and its truncated
+ * singular values.  n is the system size passed in (the solver's m, or
+ * 2m for the complex solver).  Returns 0. */
+int aar_hip_last_system(int n, double *G, double *w);
 
 /* Release cached device workspace. */
 int aar_hip_release(void);

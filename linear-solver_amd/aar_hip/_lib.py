@@ -61,6 +61,7 @@ EXTENSION_SYMBOLS = [
     "aar_hip_set_stream", "aar_hip_set_tuning", "aar_hip_set_option", "aar_hip_profile", "aar_hip_profile_reset",
     "aar_hip_profile_query", "aar_hip_release", "aar_hip_last_error",
     "aar_hip_helm_matvec", "aar_hip_helm_aar", "aar_hip_helm_grad_c", "aar_hip_aar_batch",
+    "aar_hip_autotune", "aar_hip_last_system",
 ]
 
 
@@ -108,6 +109,9 @@ def load(require_gpu=True):
         "aar_hip_helm_aar": [P, P, P, D, D, I, I, I, D, I, S],
         "aar_hip_helm_grad_c": [ctypes.c_long, P, P, P],
         "aar_hip_aar_batch": [ctypes.POINTER(Instance), I, D, D, I, I, I, D, I, I, S],
+        "aar_hip_autotune": [I, I, I, I, I, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I),
+                             ctypes.POINTER(D)],
+        "aar_hip_last_system": [I, P, P],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

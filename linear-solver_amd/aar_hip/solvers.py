@@ -189,6 +189,16 @@ def pinv(A, b):
     return x, w, V
 
 
+def last_system(n):
+    """The last normal-equation system of the last solve (aar_hip_last_system):
+    (A, rhs, w) with A n x n."""
+    G = np.zeros(n * n + n)
+    w = np.zeros(n)
+    _L().aar_hip_last_system(int(n), G.ctypes.data_as(ctypes.c_void_p), w.ctypes.data_as(ctypes.c_void_p))
+    A = G[: n * n].reshape(n, n).T.copy()      # A[i][j] = G[j*n + i]
+    return A, G[n * n:].copy(), w
+
+
 def history(cap=1 << 20):
     """Residual history ||b - A x_k|| of the last solve (see aar_hip.h)."""
     buf = np.zeros(cap)
@@ -196,9 +206,21 @@ def history(cap=1 << 20):
     return buf[:n].copy()
 
 
-def set_tuning(lz=0, wy=4, py=4):
-    """Stencil launch shape (see aar_hip_set_tuning in include/aar_hip.h)."""
+def set_tuning(lz=0, wy=8, py=4):
+    """Stencil launch shape for every grid; (0, 8, 4) returns to the
+    automatic (or tuned) shapes (aar_hip_set_tuning)."""
     check(_L().aar_hip_set_tuning(int(lz), int(wy), int(py)))
+
+
+def autotune(shape, fp32=False, batch=False):
+    """Tune the stencil launch shape for a grid (aar_hip_autotune); later
+    solves on that grid use it.  Returns {lz, wy, py, us}."""
+    nx, ny, nz = (int(s) for s in shape)
+    lz, wy, py, us = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+    _bind_stream()
+    check(_L().aar_hip_autotune(nx, ny, nz, int(bool(fp32)), int(bool(batch)), ctypes.byref(lz),
+                                ctypes.byref(wy), ctypes.byref(py), ctypes.byref(us)))
+    return {"lz": lz.value, "wy": wy.value, "py": py.value, "us": us.value}
 
 
 def set_option(name, value):

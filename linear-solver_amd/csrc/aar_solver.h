@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <map>
 #include <string>
 #include <vector>
@@ -22,6 +23,12 @@ struct ProfEntry {
     int count = 0;
     double ms = 0.0;
     double bytes = 0.0;
+};
+
+// launch shape of the stencil sweeps for one grid and element size
+struct Shape {
+    int lz = 0, wy = 8, py = 4;
+    double us = 0.0;          // measured time of one recurrence sweep (tuned shapes)
 };
 
 struct PendingEv {
@@ -54,6 +61,11 @@ struct Context {
     int hist_blocks = 0;           // grid of the reducing history kernels (0: automatic)
     bool two_pass = true;          // reductions finish in a one-block second kernel
     bool prof = false;
+    bool tuning_set = false;       // aar_hip_set_tuning() fixed the shape for every grid
+    bool autotune = false;         // tune a grid's shape on its first solve
+    bool complex_ref_order = false;  // complex solver's SVD in the tools.c order (default: textbook)
+    std::map<std::array<int, 4>, Shape> tuned;        // (nx, ny, nz, bytes) -> shape
+    std::map<std::array<int, 4>, Shape> tuned_batch;  // lz for the batched 8x4 launch
     std::map<std::string, ProfEntry> profile;
     std::vector<PendingEv> pending;
     std::vector<hipEvent_t> evpool;
@@ -81,6 +93,8 @@ SArgs<T> base_sargs(const Op &op);
 template <typename T>
 hipError_t run_dot(DotArgs<T> d);
 int poll_record(Ctrl *ctrl, int cyc);
+Shape stencil_shape(int nx, int ny, int nz, size_t es);
+int autotune_stencil(int nx, int ny, int nz, size_t es, bool batch_shape, Shape *out);
 bool poll_check(int cyc);
 void prof_begin(const char *name);
 void prof_end(double bytes);

@@ -382,10 +382,15 @@ int batch_solve(const aar_hip_instance *ins, int count, double omega, double bet
         if (NC > 0) BHC(bsp_alloc(&I.C, (size_t)I.ld * NC, owned));
         BHC(bsp_alloc(&I.hist, (size_t)I.hist_cap, owned));
         // stencil args (auto z-chunk for this grid and the batch launch shape)
+        // slab depth: tuned for the batched 8 x 4 launch where available
+        auto batch_lz = [&](size_t es) {
+            auto t = C.tuned_batch.find({op.nx, op.ny, op.nz, (int)es});
+            return t != C.tuned_batch.end() ? t->second.lz : auto_lz(op, BWY, BPY, es);
+        };
         SArgs<Tx> tr = base_sargs<Tx>(op);
-        tr.lz = auto_lz(op, BWY, BPY, sizeof(Tx));
+        tr.lz = batch_lz(sizeof(Tx));
         SArgs<Tg> rcg = base_sargs<Tg>(op);
-        rcg.lz = auto_lz(op, BWY, BPY, sizeof(Tg));
+        rcg.lz = batch_lz(sizeof(Tg));
         const int ntr = stencil_items<Tx>(tr, BWY, BPY), nrc = stencil_items<Tg>(rcg, BWY, BPY);
         stencil_geometry<Tx>(tr, BWY, BPY);
         stencil_geometry<Tg>(rcg, BWY, BPY);

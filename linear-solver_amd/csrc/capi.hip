@@ -790,6 +790,31 @@ int aar_hip_set_tuning(int lz, int wy, int py)
     ctx().lz = lz;
     ctx().wy = wy;
     ctx().py = py;
+    // the defaults (0, 8, 4) hand the choice back to the automatic/tuned shapes
+    ctx().tuning_set = (lz != 0 || wy != 8 || py != 4);
+    return 0;
+}
+
+int aar_hip_last_system(int n, double *G, double *w)
+{
+    if (n < 1 || n > MAXM_LS) return fail(AAR_HIP_ERR_ARG, "system size must be in 1..16");
+    const Ctrl &c = ctx().ctrl_h[0];
+    if (G)
+        for (int k = 0; k < n * n + n; k++) G[k] = c.G[k];
+    if (w)
+        for (int k = 0; k < n; k++) w[k] = c.w[k];
+    return 0;
+}
+
+int aar_hip_autotune(int nx, int ny, int nz, int fp32, int batch_shape, int *lz, int *wy, int *py, double *us)
+{
+    if (nx < 1 || ny < 1 || nz < 1) return fail(AAR_HIP_ERR_ARG, "grid sizes must be positive");
+    Shape sh;
+    if (int rc = autotune_stencil(nx, ny, nz, fp32 ? 4 : 8, batch_shape != 0, &sh)) return rc;
+    if (lz) *lz = sh.lz;
+    if (wy) *wy = sh.wy;
+    if (py) *py = sh.py;
+    if (us) *us = sh.us;
     return 0;
 }
 
@@ -805,6 +830,10 @@ int aar_hip_set_option(const char *name, long value)
         C.hist_blocks = (int)value;
     } else if (k == "two_pass") {
         C.two_pass = value != 0;
+    } else if (k == "autotune") {
+        C.autotune = value != 0;
+    } else if (k == "complex_pinv_reference_order") {
+        C.complex_ref_order = value != 0;
     } else {
         return fail(AAR_HIP_ERR_ARG, "unknown option " + k);
     }

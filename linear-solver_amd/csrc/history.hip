@@ -245,7 +245,9 @@ __device__ inline double d_sign(double mag, double s) { return s >= 0.0 ? fabs(m
 // rotation coefficients) are computed by every lane alike.  The left
 // reflector of tools.c:137 also runs over columns j < i and j = i; column
 // i itself is updated before the columns j > i read it, as in the
-// sequential loop.
+// sequential loop.  textbook = 1 (the complex solver, no MPI_REAL
+// counterpart) runs the textbook step over columns i+1..n-1 only: on some
+// embedded Hermitian systems the tools.c order does not converge.
 // ---------------------------------------------------------------------------
 template <int M, int k>
 __device__ __attribute__((always_inline)) inline void wgr_qr_step(double (&w)[M], double (&rv1)[M], double (&vr)[M],
@@ -372,7 +374,7 @@ struct WGrQr<M, -1> {
 };
 
 template <int M>
-__device__ __forceinline__ void ls_body(Ctrl *c, const int n)
+__device__ __forceinline__ void ls_body(Ctrl *c, const int n, const int textbook)
 {
     __shared__ double A[M][M];
     __shared__ double V[M][M];
@@ -422,10 +424,12 @@ __device__ __forceinline__ void ls_body(Ctrl *c, const int n)
                     const double fac = t / h;
                     for (int k = i; k < n; k++) A[k][j] += fac * A[k][i];
                 };
-                if (lane >= 1 && lane < i && lane < n) colupd(lane);
-                __syncthreads();
-                if (i >= 1 && lane == i) colupd(i);
-                __syncthreads();
+                if (!textbook) {
+                    if (lane >= 1 && lane < i && lane < n) colupd(lane);
+                    __syncthreads();
+                    if (i >= 1 && lane == i) colupd(i);
+                    __syncthreads();
+                }
                 if (lane > i && lane >= 1 && lane < n) colupd(lane);
                 __syncthreads();
                 if (lane >= i && lane < n) A[lane][i] *= scale;
@@ -552,16 +556,16 @@ __device__ __forceinline__ void ls_body(Ctrl *c, const int n)
 }
 
 template <int M>
-__global__ __launch_bounds__(64) void ls_kernel(Ctrl *c, int n)
+__global__ __launch_bounds__(64) void ls_kernel(Ctrl *c, int n, int textbook)
 {
-    ls_body<M>(c, n);
+    ls_body<M>(c, n, textbook);
 }
 
 // one wavefront per solve of a batch
 template <int M>
 __global__ __launch_bounds__(64) void ls_batch_kernel(Ctrl *const *ctrls, int n)
 {
-    ls_body<M>(ctrls[blockIdx.x], n);
+    ls_body<M>(ctrls[blockIdx.x], n, 0);
 }
 
 hipError_t launch_ls_batch(Ctrl *const *ctrls, int count, int n, hipStream_t st)
@@ -572,11 +576,11 @@ hipError_t launch_ls_batch(Ctrl *const *ctrls, int count, int n, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_ls(Ctrl *ctrl, int n, int /*complex_embed*/, hipStream_t st)
+hipError_t launch_ls(Ctrl *ctrl, int n, int complex_embed, hipStream_t st)
 {
     if (n < 1 || n > MAXM_LS) return hipErrorInvalidValue;
-    if (n <= 8) hipLaunchKernelGGL((ls_kernel<8>), dim3(1), dim3(64), 0, st, ctrl, n);
-    else hipLaunchKernelGGL((ls_kernel<16>), dim3(1), dim3(64), 0, st, ctrl, n);
+    if (n <= 8) hipLaunchKernelGGL((ls_kernel<8>), dim3(1), dim3(64), 0, st, ctrl, n, complex_embed);
+    else hipLaunchKernelGGL((ls_kernel<16>), dim3(1), dim3(64), 0, st, ctrl, n, complex_embed);
     return hipGetLastError();
 }
 

@@ -135,15 +135,38 @@ int auto_lz(const Op &op, int wy, int py, size_t es)
     return lz;
 }
 
+// Launch shape for a grid: the one fixed by aar_hip_set_tuning, else a
+// tuned one (aar_hip_autotune, or tuned here on first use with the
+// "autotune" option), else the defaults with the automatic z-chunk.
+Shape stencil_shape(int nx, int ny, int nz, size_t es)
+{
+    Context &C = ctx();
+    if (!C.tuning_set) {
+        const std::array<int, 4> key = {nx, ny, nz, (int)es};
+        auto it = C.tuned.find(key);
+        if (it == C.tuned.end() && C.autotune && autotune_stencil(nx, ny, nz, es, false, nullptr) == 0)
+            it = C.tuned.find(key);
+        if (it != C.tuned.end()) return it->second;
+    }
+    Op op;
+    op.nx = nx;
+    op.ny = ny;
+    op.nz = nz;
+    Shape sh;
+    sh.wy = C.wy;
+    sh.py = C.py;
+    sh.lz = C.lz > 0 ? C.lz : auto_lz(op, C.wy, C.py, es);
+    return sh;
+}
+
 template <typename T>
 SArgs<T> base_sargs(const Op &op)
 {
-    Context &C = ctx();
     SArgs<T> a;
     a.nx = op.nx;
     a.ny = op.ny;
     a.nz = op.nz;
-    a.lz = C.lz > 0 ? C.lz : auto_lz(op, C.wy, C.py, sizeof(T));
+    a.lz = stencil_shape(op.nx, op.ny, op.nz, sizeof(T)).lz;
     for (int o = 0; o <= R; o++) a.w[o] = op.w[o];
     a.sigma = op.sigma;
     a.diag = op.diag;
@@ -236,11 +259,12 @@ int run_stencil(SMode mode, SArgs<T> a, double bytes)
     Context &C = ctx();
     const bool two = C.two_pass && a.partials;
     if (two) a.counter = nullptr;
+    const Shape sh = stencil_shape(a.nx, a.ny, a.nz, sizeof(T));
     prof_begin(smode_name(mode));
-    hipError_t e = launch_stencil<T>(mode, a, C.wy, C.py, C.stream);
+    hipError_t e = launch_stencil<T>(mode, a, sh.wy, sh.py, C.stream);
     prof_end(bytes);
     if (two && e == hipSuccess)
-        e = launch_finish(a.partials, stencil_items<T>(a, C.wy, C.py), mode == SMode::CG_RESID ? 2 : 1, a.fin,
+        e = launch_finish(a.partials, stencil_items<T>(a, sh.wy, sh.py), mode == SMode::CG_RESID ? 2 : 1, a.fin,
                           a.ctrl, C.stream);
     if (e != hipSuccess) return fail(AAR_HIP_ERR_HIP, std::string("stencil launch: ") + hipGetErrorString(e));
     C.bytes_acc += bytes;
@@ -289,7 +313,95 @@ template <typename T>
 int stencil_partials_needed(const Op &op)
 {
     SArgs<T> a = base_sargs<T>(op);
-    return stencil_items<T>(a, ctx().wy, ctx().py) * 2;
+    const Shape sh = stencil_shape(op.nx, op.ny, op.nz, sizeof(T));
+    return stencil_items<T>(a, sh.wy, sh.py) * 2;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: times the recurrence sweep (the dominant kernel of AAR) on the
+// grid over the launch shapes -- wavefronts per block x rows per thread
+// (the tile height; tiles are one wavefront = 64 points wide in x) and the
+// z-slab depth -- and keeps the fastest.  batch_shape restricts the search
+// to the slab depth of the batched 8 x 4 launch.
+// ---------------------------------------------------------------------------
+template <typename T>
+static int autotune_t(int nx, int ny, int nz, bool batch_shape, Shape *out)
+{
+    Context &C = ctx();
+    if (int rc = context_init()) return rc;
+    aar_hip_grid g = {nx, ny, nz, R, -1.0, 0.0};
+    Op op;
+    if (int rc = make_op(&g, nullptr, op)) return rc;
+    T *in = nullptr, *o = nullptr;
+    HC(hipMalloc((void **)&in, sizeof(T) * op.n));
+    if (hipMalloc((void **)&o, sizeof(T) * op.n) != hipSuccess) {
+        (void)hipFree(in);
+        return fail(AAR_HIP_ERR_HIP, "autotune: out of device memory");
+    }
+    if (sizeof(T) == 8) (void)launch_fill_f64((double *)in, 1.0, op.n, C.stream);
+    else (void)launch_fill_f32((float *)in, 1.0f, op.n, C.stream);
+    const int shapes[4][2] = {{8, 4}, {8, 2}, {4, 4}, {4, 2}};
+    const int nshape = batch_shape ? 1 : 4;
+    Shape best;
+    best.us = 1e300;
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    int rc = 0;
+    for (int si = 0; si < nshape && !rc; si++) {
+        const int wy = shapes[si][0], py = shapes[si][1];
+        std::vector<int> lzs = {auto_lz(op, wy, py, sizeof(T)), nz, (nz + 1) / 2, (nz + 3) / 4, (nz + 7) / 8};
+        std::sort(lzs.begin(), lzs.end());
+        lzs.erase(std::unique(lzs.begin(), lzs.end()), lzs.end());
+        for (int lz : lzs) {
+            if (lz < std::min(nz, 16)) continue;
+            SArgs<T> a;
+            a.nx = nx;
+            a.ny = ny;
+            a.nz = nz;
+            a.lz = lz;
+            for (int q = 0; q <= R; q++) {
+                a.w[q] = op.w[q];
+                a.cf[q] = q == 0 ? 0.5 : -1e-3 * op.w[q];
+            }
+            a.in0 = in;
+            a.out0 = o;
+            hipError_t e = launch_stencil<T>(SMode::AAR_REC, a, wy, py, C.stream);
+            const int reps = 4;
+            e = e == hipSuccess ? hipEventRecord(e0, C.stream) : e;
+            for (int r = 0; r < reps && e == hipSuccess; r++) e = launch_stencil<T>(SMode::AAR_REC, a, wy, py, C.stream);
+            e = e == hipSuccess ? hipEventRecord(e1, C.stream) : e;
+            e = e == hipSuccess ? hipEventSynchronize(e1) : e;
+            float ms = 0.f;
+            e = e == hipSuccess ? hipEventElapsedTime(&ms, e0, e1) : e;
+            if (e != hipSuccess) {
+                rc = fail(AAR_HIP_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+                break;
+            }
+            const double us = 1e3 * ms / reps;
+            if (us < best.us) {
+                best.us = us;
+                best.lz = lz;
+                best.wy = wy;
+                best.py = py;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(C.stream);
+    (void)hipFree(in);
+    (void)hipFree(o);
+    if (rc) return rc;
+    const std::array<int, 4> key = {nx, ny, nz, (int)sizeof(T)};
+    (batch_shape ? C.tuned_batch : C.tuned)[key] = best;
+    if (out) *out = best;
+    return 0;
+}
+
+int autotune_stencil(int nx, int ny, int nz, size_t es, bool batch_shape, Shape *out)
+{
+    return es == 8 ? autotune_t<double>(nx, ny, nz, batch_shape, out) : autotune_t<float>(nx, ny, nz, batch_shape, out);
 }
 
 // poll ring: record ctrl->done after a cycle; true once done is observed
@@ -1148,7 +1260,7 @@ int helm_solve(const HOp &op, double *u, const double *b, double omega, double b
             C.bytes_acc += sv * (nv + 1);
         }
         prof_begin("ls");
-        HC(launch_ls(ctrl, 2 * m, 1, C.stream));
+        HC(launch_ls(ctrl, 2 * m, C.complex_ref_order ? 0 : 1, C.stream));
         prof_end(0);
         {
             HAncArgs hu;

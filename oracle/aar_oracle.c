@@ -115,8 +115,12 @@ double orc_pythag(double a, double b)
 static inline double sign_of(double mag, double s) { return s >= 0.0 ? fabs(mag) : -fabs(mag); }
 static inline double dmax(double a, double b) { return a > b ? a : b; }
 
-/* a: n*n row-major (overwritten); w: n; v: n*n row-major.  Returns 0/1. */
-static int gr_svd_wv(double *a, int n, double *w, double *v)
+/* a: n*n row-major (overwritten); w: n; v: n*n row-major.  Returns 0/1.
+ * textbook = 1 runs the left reflector over columns i+1..n-1 (the textbook
+ * Golub-Reinsch step) instead of tools.c's 1..n-1; used by the complex
+ * solver, which has no MPI_REAL counterpart: on some of its embedded
+ * Hermitian systems the tools.c order does not converge in 250 sweeps. */
+static int gr_svd_wv(double *a, int n, double *w, double *v, int textbook)
 {
 #define A(r, c) a[(r) * n + (c)]
 #define V(r, c) v[(r) * n + (c)]
@@ -140,7 +144,7 @@ static int gr_svd_wv(double *a, int n, double *w, double *v)
                 g = -sign_of(sqrt(s), f);
                 h = f * g - s;
                 A(i, i) = f - g;
-                for (int j = 1; j < n; j++) {          /* tools.c:137 starts at 1 */
+                for (int j = textbook ? l : 1; j < n; j++) {   /* tools.c:137 starts at 1 */
                     double t = 0.0;
                     for (int k = i; k < m; k++) t += A(k, i) * A(k, j);
                     double fac = t / h;
@@ -283,15 +287,30 @@ static int gr_svd_wv(double *a, int n, double *w, double *v)
 #undef V
 }
 
+static int pinv_times_vec(const double *Ain, const double *b, double *x, int m, double *w_out, double *v_out,
+                          int textbook);
+
 int orc_pinv_times_vec(const double *Ain, const double *b, double *x, int m,
                        double *w_out, double *v_out)
+{
+    return pinv_times_vec(Ain, b, x, m, w_out, v_out, 0);
+}
+
+int orc_pinv_times_vec_textbook(const double *Ain, const double *b, double *x, int m,
+                                double *w_out, double *v_out)
+{
+    return pinv_times_vec(Ain, b, x, m, w_out, v_out, 1);
+}
+
+static int pinv_times_vec(const double *Ain, const double *b, double *x, int m, double *w_out, double *v_out,
+                          int textbook)
 {
     double *a = (double *)malloc(sizeof(double) * m * m);
     double *v = (double *)calloc((size_t)m * m, sizeof(double));
     double *w = (double *)calloc((size_t)m, sizeof(double));
     double *tmp = (double *)calloc((size_t)m, sizeof(double));
     memcpy(a, Ain, sizeof(double) * m * m);
-    int st = gr_svd_wv(a, m, w, v);
+    int st = gr_svd_wv(a, m, w, v, textbook);
     /* tools.c:76-94: x = V diag(1/w) V^T b, with U replaced by V */
     for (int j = 0; j < m; j++) {
         double s = 0.0;
@@ -709,7 +728,7 @@ int orc_aar_complex(int nx, int ny, int nz, int fdn, double a,
                     E[(m + i) * m2 + j] = Gi[i * m + j];
                     E[(m + i) * m2 + m + j] = Gr[i * m + j];
                 }
-            if (orc_pinv_times_vec(E, e, se, m2, NULL, NULL)) { status = 1; break; }
+            if (orc_pinv_times_vec_textbook(E, e, se, m2, NULL, NULL)) { status = 1; break; }
             for (long k = 0; k < n; k++) {
                 double tr = 0.0, ti = 0.0;
                 for (int j = 0; j < m; j++) {

@@ -50,6 +50,10 @@ double orc_dot(const double *u, const double *v, long n);
  * A is m*m row-major (A[i*m+j] is the reference's A[i][j]).  Returns 0, or
  * 1 when the QR sweep exceeds 250 iterations (the reference exits(1)).
  * w_out, v_out (optional) receive the truncated singular values and V. */
+/* The same with the textbook Golub-Reinsch left reflector (columns
+ * i+1..n-1); the complex solver's m x m solves use it. */
+int orc_pinv_times_vec_textbook(const double *A, const double *b, double *x, int m,
+                                double *w_out, double *v_out);
 int orc_pinv_times_vec(const double *A, const double *b, double *x, int m,
                        double *w_out, double *v_out);
 double orc_pythag(double a, double b);
@@ -86,7 +90,9 @@ int orc_cg(int nx, int ny, int nz, int fdn, double a, double sigma,
  * A u = a*Lap(u) + (c(x) + i*eta) u, complex128 interleaved (re, im).
  * c may be NULL (c = c0 everywhere).  The Anderson LS uses the Hermitian
  * Gram DF^H DF and DF^H f solved through the real 2m x 2m embedding with
- * the pinv of tools.c.  Parity with PETSc/zgelsd is unpinned. */
+ * the pinv of tools.c in its textbook form (orc_pinv_times_vec_textbook).
+ * Parity with PETSc/zgelsd is unpinned; the solver is pinned by the FFT
+ * solution, the adjoint identity and its reduction to the real AAR. */
 int orc_aar_complex(int nx, int ny, int nz, int fdn, double a,
                     const double *c, double c0, double eta,
                     double *u, const double *rhs, double omega, double beta,

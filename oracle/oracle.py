@@ -71,6 +71,8 @@ def lib():
         L.orc_norm2.restype = Dd
         L.orc_pinv_times_vec.argtypes = [_D, _D, _D, I, _D, _D]
         L.orc_pinv_times_vec.restype = I
+        L.orc_pinv_times_vec_textbook.argtypes = [_D, _D, _D, I, _D, _D]
+        L.orc_pinv_times_vec_textbook.restype = I
         L.orc_pythag.argtypes = [Dd, Dd]
         L.orc_pythag.restype = Dd
         L.orc_anderson_extrapolation.argtypes = [_D, _D, _D, Dd, I, ctypes.c_long, _D, _D, _D]
@@ -117,14 +119,15 @@ def norm2(v):
     return lib().orc_norm2(_p(v), v.size)
 
 
-def pinv_times_vec(A, b):
+def pinv_times_vec(A, b, textbook=False):
     A = _f64(A)
     m = A.shape[0]
     b = _f64(b)
     x = np.zeros(m)
     w = np.zeros(m)
     V = np.zeros((m, m))
-    st = lib().orc_pinv_times_vec(_p(A), _p(b), _p(x), m, _p(w), _p(V))
+    fn = lib().orc_pinv_times_vec_textbook if textbook else lib().orc_pinv_times_vec
+    st = fn(_p(A), _p(b), _p(x), m, _p(w), _p(V))
     if st:
         raise RuntimeError("SVD did not converge in 250 sweeps")
     return x, w, V

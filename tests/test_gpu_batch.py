@@ -77,3 +77,29 @@ def test_batch_matches_single_solves(gpu):
         assert st["iterations"] == sts[i]["iterations"]
         r = float(((x1 - xs[i]).norm() / x1.norm()).cpu())
         assert r < 1e-10, (i, r)
+
+
+def test_autotune_shapes_keep_parity(gpu, oracle):
+    """Tuned launch shapes are valid candidates and the stencil and the
+    solver give the same results under them (bit-exact matvec, the
+    reference's iteration count)."""
+    shape = (40, 36, 44)
+    for fp32 in (False, True):
+        t = gpu.autotune(shape, fp32=fp32)
+        say("tuned", shape, "fp32" if fp32 else "fp64", t)
+        assert (t["wy"], t["py"]) in {(8, 4), (8, 2), (4, 4), (4, 2)}
+        assert min(44, 16) <= t["lz"] <= 44 and t["us"] > 0
+    tb = gpu.autotune(shape, batch=True)
+    assert (tb["wy"], tb["py"]) == (8, 4)
+    x = np.random.default_rng(9).standard_normal(int(np.prod(shape)))
+    y = gpu.Lap_Vec_mult(dev(x), shape, -4.0).cpu().numpy()
+    assert np.array_equal(y, oracle.matvec(x, shape, -4.0))
+    b = P.rhs_uniform(shape, 41)
+    xr, dr = oracle.aar(np.zeros(b.size), b, shape, -4.0, 0.6, 0.6, 7, 6, 1000, 1e-8, trace=False)
+    xg = dev(np.zeros(b.size))
+    st = gpu.AAR(xg, dev(b), shape, -4.0, 0.6, 0.6, 7, 6, 1000, 1e-8)
+    assert st["iterations"] == dr["iterations"]
+    assert np.linalg.norm(xg.cpu().numpy() - xr) / np.linalg.norm(xr) <= 1e-8
+    xb = dev(np.zeros(b.size))
+    sts = gpu.AAR_batch([(xb, dev(b), shape, -4.0, 0.0)], 0.6, 0.6, 7, 6, 1000, 1e-8)
+    assert sts[0]["iterations"] == dr["iterations"]

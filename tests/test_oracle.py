@@ -399,3 +399,36 @@ def test_oracle_complex_adjoint_identity(oracle):
     Au = oracle.matvec_complex(u, shape, -4.0, c, 0.0, 0.5)
     AHv = oracle.matvec_complex(v, shape, -4.0, c, 0.0, -0.5)
     assert abs(np.vdot(AHv, u) - np.vdot(v, Au)) < 1e-10 * abs(np.vdot(v, Au))
+
+
+@pytest.mark.parametrize("m", [2, 7, 14, 16])
+def test_pinv_textbook_matches_numpy(oracle, m):
+    """The textbook Golub-Reinsch variant (the complex solver's m x m solve)
+    is the pseudo-inverse: V diag(1/w) V^T b against numpy's SVD on
+    symmetric PSD and rank-deficient matrices."""
+    rng = np.random.default_rng(m)
+    B = rng.standard_normal((m, m + 3))
+    for A in (B @ B.T, (B[:, : m // 2 + 1] @ B[:, : m // 2 + 1].T)):
+        b = rng.standard_normal(m)
+        x, w, V = oracle.pinv_times_vec(A, b, textbook=True)
+        U, S, Vt = np.linalg.svd(A)
+        keep = S >= m * S[0] * 2.22044605e-16
+        xr = Vt.T[:, keep] @ ((Vt[keep] @ b) / S[keep])
+        assert np.linalg.norm(x - xr) <= 1e-10 * np.linalg.norm(xr)
+
+
+def test_pinv_textbook_converges_where_reference_order_does_not(oracle):
+    """tests/golden/pinv_embedded_*.npy: the embedded Hermitian Anderson
+    system on which the tools.c:137 order (left reflector from column 1)
+    does not converge in 250 sweeps; the textbook order solves it."""
+    import os
+    from conftest import GOLDEN_DIR
+    A = np.load(os.path.join(GOLDEN_DIR, "pinv_embedded_A.npy"))
+    r = np.load(os.path.join(GOLDEN_DIR, "pinv_embedded_r.npy"))
+    with pytest.raises(RuntimeError):
+        oracle.pinv_times_vec(A, r)
+    x, w, V = oracle.pinv_times_vec(A, r, textbook=True)
+    U, S, Vt = np.linalg.svd(A)
+    keep = S >= 14 * S[0] * 2.22044605e-16
+    xr = Vt.T[:, keep] @ ((Vt[keep] @ r) / S[keep])
+    assert np.linalg.norm(x - xr) <= 1e-8 * np.linalg.norm(xr)
