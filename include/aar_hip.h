@@ -164,7 +164,8 @@ int aar_hip_last_stats(aar_hip_stats *st);
 int aar_hip_set_stream(void *stream);
 
 /* Stencil launch shape: lz planes per block (0 = automatic), wy
- * wavefronts per block stacked in y (4 or 8), py rows per thread (2 or 4);
+ * wavefronts per block stacked in y (4 or 8), py rows per thread (2 or 4;
+ * 8 with wy = 4);
  * a block owns a 64 x (wy*py) column.  Applies to subsequent calls and to
  * every grid; (0, 8, 4) returns to the automatic shapes, tuned ones where
  * a grid has been tuned. */

@@ -116,6 +116,29 @@ void report(const char *name, const aar_hip_stats &s)
     std::fflush(stdout);
 }
 
+// With AAR_HIP_DEBUG set (non-empty, not "0"), the trace the reference
+// prints when built with -DDEBUG: the relative residual of the start and
+// of every iteration (AAR.c:84-86/125-126/141-143, PGR.c:83-85/135-138,
+// PL2R.c:83-85/134-137), or CG's absolute residuals (CG.c:141-142).
+static bool debug_trace_on()
+{
+    const char *e = std::getenv("AAR_HIP_DEBUG");
+    return e && *e && std::strcmp(e, "0") != 0;
+}
+
+static void debug_trace(const char *name, const aar_hip_stats &s)
+{
+    if (!debug_trace_on()) return;
+    const std::vector<double> &h = ctx().hist_host;
+    const bool cg = std::strcmp(name, "CG") == 0;
+    const int n = std::min<int>((int)h.size(), s.hist_len);
+    // CG's history starts with the initial residual, which CG.c does not print
+    for (int k = cg ? 1 : 0; k < n; k++) {
+        if (cg) std::printf("norm of res: %g\n", h[k]);
+        else std::printf("Relative Residual: %g\n", s.rhs_norm > 0 ? h[k] / s.rhs_norm : h[k]);
+    }
+}
+
 void report_failure(const char *name, int rc)
 {
     std::printf("%s: libaar_hip error %d: %s\n", name, rc, ctx().err.c_str());
@@ -149,6 +172,7 @@ void host_solve(const char *name, POISSON *sys, aar_op_cb opcb, aar_pc_cb pccb, 
         report_failure(name, AAR_HIP_ERR_HIP);
         return;
     }
+    debug_trace(name, st);
     if (rc == AAR_HIP_ERR_SVD) std::printf("no convergence in %d svd iterations \n", 250);
     report(name, st);
 }
@@ -784,8 +808,9 @@ int aar_hip_set_stream(void *stream)
 
 int aar_hip_set_tuning(int lz, int wy, int py)
 {
-    if (py != 2 && py != 4) return fail(AAR_HIP_ERR_ARG, "py must be 2 or 4");
+    if (py != 2 && py != 4 && py != 8) return fail(AAR_HIP_ERR_ARG, "py must be 2, 4 or 8");
     if (wy != 4 && wy != 8) return fail(AAR_HIP_ERR_ARG, "wy must be 4 or 8");
+    if (py == 8 && wy != 4) return fail(AAR_HIP_ERR_ARG, "py = 8 needs wy = 4");
     if (lz < 0) return fail(AAR_HIP_ERR_ARG, "lz must be >= 0");
     ctx().lz = lz;
     ctx().wy = wy;

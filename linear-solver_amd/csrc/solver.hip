@@ -340,8 +340,8 @@ static int autotune_t(int nx, int ny, int nz, bool batch_shape, Shape *out)
     }
     if (sizeof(T) == 8) (void)launch_fill_f64((double *)in, 1.0, op.n, C.stream);
     else (void)launch_fill_f32((float *)in, 1.0f, op.n, C.stream);
-    const int shapes[4][2] = {{8, 4}, {8, 2}, {4, 4}, {4, 2}};
-    const int nshape = batch_shape ? 1 : 4;
+    const int shapes[5][2] = {{8, 4}, {4, 8}, {8, 2}, {4, 4}, {4, 2}};
+    const int nshape = batch_shape ? 1 : 5;
     Shape best;
     best.us = 1e300;
     hipEvent_t e0, e1;

@@ -52,6 +52,7 @@ hipError_t launch_stencil_t(SArgs<T> a, int wy, int py, hipStream_t st)
     case 44: hipLaunchKernelGGL((stencil_kernel<T, 4, 4, M>), grid, block, 0, st, a); break;
     case 82: hipLaunchKernelGGL((stencil_kernel<T, 8, 2, M>), grid, block, 0, st, a); break;
     case 84: hipLaunchKernelGGL((stencil_kernel<T, 8, 4, M>), grid, block, 0, st, a); break;
+    case 48: hipLaunchKernelGGL((stencil_kernel<T, 4, 8, M>), grid, block, 0, st, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -87,7 +87,7 @@ def test_autotune_shapes_keep_parity(gpu, oracle):
     for fp32 in (False, True):
         t = gpu.autotune(shape, fp32=fp32)
         say("tuned", shape, "fp32" if fp32 else "fp64", t)
-        assert (t["wy"], t["py"]) in {(8, 4), (8, 2), (4, 4), (4, 2)}
+        assert (t["wy"], t["py"]) in {(8, 4), (4, 8), (8, 2), (4, 4), (4, 2)}
         assert min(44, 16) <= t["lz"] <= 44 and t["us"] > 0
     tb = gpu.autotune(shape, batch=True)
     assert (tb["wy"], tb["py"]) == (8, 4)

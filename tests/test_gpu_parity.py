@@ -41,7 +41,7 @@ def test_stencil_bitwise(gpu, oracle, shape, sigma):
 
 
 @pytest.mark.parametrize("lz,wy,py", [(0, 4, 4), (0, 4, 2), (7, 4, 4), (1, 4, 2), (512, 4, 4),
-                                      (0, 8, 4), (0, 8, 2), (5, 8, 4), (3, 8, 2)])
+                                      (0, 8, 4), (0, 8, 2), (5, 8, 4), (3, 8, 2), (0, 4, 8), (9, 4, 8)])
 def test_stencil_tuning_bitwise(gpu, oracle, lz, wy, py):
     say("stencil tuning", lz, wy, py)
     shape = (72, 40, 36)
@@ -270,6 +270,32 @@ def test_dropin_reference_driver(golden):
         m = re.search(sv + r" converged!:\s+Iterations = (\d+)", out)
         assert m, f"{sv} line missing"
         assert int(m.group(1)) == ref[sv.lower()]["iterations"]
+
+
+def test_dropin_debug_trace(golden):
+    """With AAR_HIP_DEBUG=1 the drop-in driver prints the reference's -DDEBUG
+    trace: one "Relative Residual" line for the start and for every AAR
+    iteration, matching the reference's residual history to the printed
+    precision."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    exe = os.path.join(os.path.dirname(__file__), "..", "oracle", "_ref", "main_on_libaar")
+    if not os.path.exists(exe):
+        pytest.skip("drop-in demo not built")
+    meta, g = golden
+    tmp = os.path.join(os.path.dirname(__file__), "..", "gpurun_out")
+    os.makedirs(tmp, exist_ok=True)
+    env = dict(os.environ, AAR_HIP_DEBUG="1")
+    out = subprocess.run([os.path.abspath(exe), "1e-6", "7", "6", "0.6", "0.6"], capture_output=True,
+                         text=True, timeout=300, cwd=tmp, env=env).stdout
+    aar_part = out.split("AAR converged!")[0]
+    vals = [float(v) for v in __import__("re").findall(r"Relative Residual: (\S+)", aar_part)]
+    c = meta["cases"]["refdrv_48"]
+    tr = g["refdrv_48__aar__trace"] / c["rhs_norm"]
+    assert len(vals) == c["solvers"]["aar"]["iterations"] + 1 == len(tr)
+    assert np.allclose(vals, tr, rtol=1e-5, atol=1e-10 * tr[0])
+    assert "norm of res:" in out
 
 
 def test_foreign_callback_rejected(gpu):
