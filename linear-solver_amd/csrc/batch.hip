@@ -89,7 +89,9 @@ __global__ __launch_bounds__(64 * BWY) void bstencil_kernel(const BInst<Tx, Tg> 
         }
     }
     __syncthreads();
-    stencil_body<T, BWY, BPY, MODE>(sa, wi.idx);
+    // FP32 with the plain index keeps the sweep within 128 VGPRs (two
+    // blocks per CU with the batch's per-block argument staging)
+    stencil_body<T, BWY, BPY, MODE, sizeof(T) == 4 ? 0 : -1>(sa, wi.idx);
 }
 
 // per-instance finish of the true residual: relres check at the Anderson

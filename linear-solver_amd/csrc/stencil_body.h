@@ -38,10 +38,38 @@ template <> struct ModeInfo<SMode::CG_RESID> { static constexpr int NF = 1, NE =
 
 __device__ inline int modp(int i, int n) { i %= n; return i < 0 ? i + n : i; }
 
+// base[eoff] with a uniform (per-plane) base pointer and a 32-bit in-plane
+// offset: the access compiles to the scalar-base + vector-offset form, with
+// no per-access 64-bit address arithmetic (in-plane byte offsets are below
+// 2^32 for any plane the host accepts)
+template <typename T> __device__ __forceinline__ T ld_at(const T *base, unsigned eoff)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + (size_t)(eoff * (unsigned)sizeof(T)));
+}
+template <typename T> __device__ __forceinline__ void st_at(T *base, unsigned eoff, T v)
+{
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + (size_t)(eoff * (unsigned)sizeof(T))) = v;
+}
+// Plane-relative access: the form above when SA, else one signed index
+// (the sweeps with two or three streams keep fewer live registers with the
+// plain index; measured per mode)
+template <bool SA, typename T> __device__ __forceinline__ T ld_pl(const T *base, size_t zo, unsigned off)
+{
+    if constexpr (SA) return ld_at(base + zo, off);
+    else return base[(int)off + (int)zo];
+}
+template <bool SA, typename T> __device__ __forceinline__ void st_pl(T *base, size_t zo, unsigned off, T v)
+{
+    if constexpr (SA) st_at(base + zo, off, v);
+    else base[(int)off + (int)zo] = v;
+}
+
 // The stencil sweep of one work item (a 64 x TY column over a z-chunk) of
 // the operator and fields in `a`; shared by the single-operator kernel
 // below and the batched kernel of batch.hip.
-template <typename T, int WY, int PY, SMode MODE>
+// ADDR: 1 = plane pointer + 32-bit offset accesses, 0 = one signed index,
+// -1 = the per-mode choice below
+template <typename T, int WY, int PY, SMode MODE, int ADDR = -1>
 __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
 {
     using A = Arith<T>;
@@ -55,6 +83,17 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
     constexpr int NYH = (2 * R + WY - 1) / WY;        // y-halo rows per thread row
     constexpr int NXH = (2 * R * TY + NT - 1) / NT;   // x-halo cells per thread
     constexpr int QN = 2 * R + 1;
+    constexpr bool SA = ADDR >= 0 ? ADDR == 1
+                                  : MODE == SMode::AAR_REC ||
+                                        (sizeof(T) == 8 && (MODE == SMode::MATVEC || MODE == SMode::CG_Q));
+#ifndef AAR_STENCIL_PD64
+#define AAR_STENCIL_PD64 2
+#endif
+    // prefetch distance (planes): 2 for the FP64 sweeps without epilogue
+    // inputs, whose one block per CU (LDS-limited) leaves VGPRs to spare; 1
+    // where a third register set would cost a block per CU (FP32), spill
+    // (two or three fields) or measured slower (epilogue inputs)
+    constexpr int PD = (sizeof(T) == 8 && NF == 1 && NE == 0) ? AAR_STENCIL_PD64 : 1;
 
     constexpr int HR = R + 1;                         // halo ring depth (planes)
     constexpr int NH = NYH + NXH;                     // halo cells per thread
@@ -79,15 +118,16 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
 
     // in-plane offsets of this thread's points, halo cells and LDS slots
     const int cx = modp(x0 + tx, nx);
-    int c_off[PY];
+    unsigned c_off[PY];
     bool c_ok[PY];
 #pragma unroll
     for (int p = 0; p < PY; p++) {
         const int y = y0 + ty * PY + p;
-        c_off[p] = cx + modp(y, ny) * nx;
+        c_off[p] = (unsigned)(cx + modp(y, ny) * nx);
         c_ok[p] = (x0 + tx < nx) && (y < ny);
     }
-    int yh_off[NYH], yh_row[NYH];
+    unsigned yh_off[NYH];
+    int yh_row[NYH];
     bool yh_ok[NYH];
 #pragma unroll
     for (int k = 0; k < NYH; k++) {
@@ -95,10 +135,11 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
         yh_ok[k] = hr0 < 2 * R;
         const int hr = yh_ok[k] ? hr0 : 0;
         const int gy = hr < R ? y0 - R + hr : y0 + TY + (hr - R);
-        yh_off[k] = cx + modp(gy, ny) * nx;
+        yh_off[k] = (unsigned)(cx + modp(gy, ny) * nx);
         yh_row[k] = hr < R ? hr : TY + hr;
     }
-    int xh_off[NXH], xh_row[NXH], xh_col[NXH];
+    unsigned xh_off[NXH];
+    int xh_row[NXH], xh_col[NXH];
     bool xh_ok[NXH];
 #pragma unroll
     for (int k = 0; k < NXH; k++) {
@@ -108,7 +149,7 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
         const int gx = cc < R ? x0 - R + cc : x0 + TX + (cc - R);
         // cells past the halo load a valid (unused) address so that every
         // load is unconditional; only their LDS write is predicated
-        xh_off[k] = xh_ok[k] ? modp(gx, nx) + modp(y0 + r, ny) * nx : c_off[0];
+        xh_off[k] = xh_ok[k] ? (unsigned)(modp(gx, nx) + modp(y0 + r, ny) * nx) : c_off[0];
         xh_row[k] = R + r;
         xh_col[k] = cc < R ? cc : TX + cc;
     }
@@ -124,10 +165,11 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
     const double trace_scale = a.precond ? a.diag : 1.0;
     const T cgb = (MODE == SMode::CG_Q && a.cg_beta) ? (T)(*a.cg_beta) : (T)0;
 
-    auto load_raw = [&](int off, T (&v)[NF]) {
-        v[0] = a.in0[off];
-        if constexpr (NF > 1) v[1] = a.in1[off];
-        if constexpr (NF > 2) v[2] = a.in2[off];
+    // fields at in-plane offset `off` of the plane starting at element zo
+    auto load_raw = [&](size_t zo, unsigned off, T (&v)[NF]) {
+        v[0] = ld_pl<SA>(a.in0, zo, off);
+        if constexpr (NF > 1) v[1] = ld_pl<SA>(a.in1, zo, off);
+        if constexpr (NF > 2) v[2] = ld_pl<SA>(a.in2, zo, off);
     };
     // stencil operand u at a point from its loaded fields
     auto make_u = [&](const T (&v)[NF]) -> T {
@@ -149,9 +191,10 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
 
     T q[PY][QN];
     // prefetch register set: halo + epilogue inputs of one plane and the
-    // queue plane R ahead of it.  Two sets alternate (the plane loop is
-    // unrolled by two) so that a set stays in flight during the barrier and
-    // compute of the previous plane with no register copies of pending loads.
+    // queue plane R ahead of it.  The loads of plane z+PD are issued while
+    // plane z is computed, so PD planes of loads are in flight per block;
+    // PD+1 sets rotate (the plane loop is unrolled by PD+1) so that no
+    // pending load is copied between registers.
     struct Pre {
         T qn[PY][NF];
         T hy[NYH][NF];
@@ -163,28 +206,28 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
     // prologue: queue planes z0-R .. z0+R-1
 #pragma unroll
     for (int s = 0; s < 2 * R; s++) {
-        const int zo = modp(z0 - R + s, nz) * nxy;
+        const size_t zo = (size_t)modp(z0 - R + s, nz) * nxy;
 #pragma unroll
         for (int p = 0; p < PY; p++) {
             T v[NF];
-            load_raw(c_off[p] + zo, v);
+            load_raw(zo, c_off[p], v);
             q[p][s] = make_u(v);
         }
     }
     // halo cells of planes z0 .. z0+R-1 into the ring
 #pragma unroll
     for (int s = 0; s < R; s++) {
-        const int zo = modp(z0 + s, nz) * nxy;
+        const size_t zo = (size_t)modp(z0 + s, nz) * nxy;
 #pragma unroll
         for (int k = 0; k < NYH; k++) {
             T v[NF];
-            load_raw(yh_off[k] + zo, v);
+            load_raw(zo, yh_off[k], v);
             hring[s][k][tid] = make_u(v);
         }
 #pragma unroll
         for (int k = 0; k < NXH; k++) {
             T v[NF];
-            load_raw(xh_off[k] + zo, v);
+            load_raw(zo, xh_off[k], v);
             hring[s][NYH + k][tid] = make_u(v);
         }
     }
@@ -192,16 +235,16 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
     // (indices wrap, so the prefetch past the last plane is a harmless
     // unused load)
     auto prefetch = [&](int zp, Pre &st) {
-        const int zq = modp(zp + R, nz) * nxy, zc = modp(zp, nz) * nxy;
+        const size_t zq = (size_t)modp(zp + R, nz) * nxy, zc = (size_t)modp(zp, nz) * nxy;
 #pragma unroll
-        for (int p = 0; p < PY; p++) load_raw(c_off[p] + zq, st.qn[p]);
+        for (int p = 0; p < PY; p++) load_raw(zq, c_off[p], st.qn[p]);
 #pragma unroll
-        for (int k = 0; k < NYH; k++) load_raw(yh_off[k] + zq, st.hy[k]);
+        for (int k = 0; k < NYH; k++) load_raw(zq, yh_off[k], st.hy[k]);
 #pragma unroll
-        for (int k = 0; k < NXH; k++) load_raw(xh_off[k] + zq, st.hx[k]);
+        for (int k = 0; k < NXH; k++) load_raw(zq, xh_off[k], st.hx[k]);
         if constexpr (NE > 0) {
 #pragma unroll
-            for (int p = 0; p < PY; p++) st.en[p][0] = a.bvec[c_off[p] + zc];
+            for (int p = 0; p < PY; p++) st.en[p][0] = ld_pl<SA>(a.bvec, zc, c_off[p]);
         }
     };
 
@@ -229,10 +272,10 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
         for (int k = 0; k < NYH; k++) hring[ws][k][tid] = make_u(cur.hy[k]);
 #pragma unroll
         for (int k = 0; k < NXH; k++) hring[ws][NYH + k][tid] = make_u(cur.hx[k]);
-        prefetch(z + 1, nxt);
+        prefetch(z + PD, nxt);
         __syncthreads();
 
-            const int zoff = z * nxy;
+            const size_t zoff = (size_t)z * nxy;
 #pragma unroll
             for (int p = 0; p < PY; p++) {
                 const int lr = R + ty * PY + p, lc = R + tx;
@@ -297,12 +340,12 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
                 // inputs: only their stores and norm terms are masked, so the
                 // stencil itself runs without branches
                 const bool ok = c_ok[p];
-                const int g = c_off[p] + zoff;
+                const unsigned g = c_off[p];
                 if constexpr (MODE == SMode::MATVEC) {
-                    if (ok) a.out0[g] = y;
+                    if (ok) st_pl<SA>(a.out0, zoff, g, y);
                 } else if constexpr (MODE == SMode::MATVEC_RES) {
                     // y = A x and ||b - A x||^2 (PGR.c:79-81 start-up)
-                    if (ok) a.out0[g] = y;
+                    if (ok) st_pl<SA>(a.out0, zoff, g, y);
                     acc[0] += ok ? A::sq(A::sub(cur.en[p][0], y)) : 0.0;
                 } else if constexpr (MODE == SMode::AAR_TRUE) {
                     // AAR.c:80-82/93-94/119-123: f = (rhs - A x)/diag, stored in
@@ -312,13 +355,13 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
                     T f = r;
                     if (a.precond) f = A::div(f, dg);
                     if (ok) {
-                        if (a.out32) a.out32[g] = (float)f;
-                        else a.out0[g] = f;
+                        if (a.out32) st_pl<SA>(a.out32, zoff, g, (float)f);
+                        else st_pl<SA>(a.out0, zoff, g, f);
                     }
                 } else if constexpr (MODE == SMode::AAR_REC) {
                     // y = g_{k+1} = (rhs - A(x_k + omega g_k))/diag; scale^2 *
                     // ||g_{k+1}||^2 = ||rhs - A x_{k+1}||^2 for the history
-                    if (ok) a.out0[g] = y;
+                    if (ok) st_pl<SA>(a.out0, zoff, g, y);
                     if (a.trace) {
                         const double sv = ok ? (double)y * trace_scale : 0.0;
                         acc[0] = fma(sv, sv, acc[0]);
@@ -327,21 +370,21 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
                     const T r = A::sub(cur.en[p][0], y);
                     acc[0] += ok ? A::sq(r) : 0.0;
                     if (ok) {
-                        if (a.out32) a.out32[g] = (float)r;
-                        if (a.out0) a.out0[g] = r;
+                        if (a.out32) st_pl<SA>(a.out32, zoff, g, (float)r);
+                        if (a.out0) st_pl<SA>(a.out0, zoff, g, r);
                     }
                 } else if constexpr (MODE == SMode::PGR_RICH) {
                     // PGR.c:95-103: x_old=x; x += omega*f; Ax = A x; f = rhs - Ax
                     if (ok) {
-                        a.out0[g] = u;
-                        a.out1[g] = y;
+                        st_pl<SA>(a.out0, zoff, g, u);
+                        st_pl<SA>(a.out1, zoff, g, y);
                     }
                     if (a.trace) acc[0] += ok ? A::sq(A::sub(cur.en[p][0], y)) : 0.0;
                 } else if constexpr (MODE == SMode::CG_Q) {
                     // CG.c:94-100: q = A d; alpha partial d.q
                     if (ok) {
-                        a.out0[g] = u;
-                        a.out1[g] = y;
+                        st_pl<SA>(a.out0, zoff, g, u);
+                        st_pl<SA>(a.out1, zoff, g, y);
                     }
                     acc[0] += ok ? (double)u * (double)y : 0.0;
                 } else if constexpr (MODE == SMode::CG_RESID) {
@@ -350,8 +393,8 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
                     T s = r;
                     if (a.precond) s = A::div(s, dg);
                     if (ok) {
-                        a.out0[g] = r;
-                        if (a.out1) a.out1[g] = s;
+                        st_pl<SA>(a.out0, zoff, g, r);
+                        if (a.out1) st_pl<SA>(a.out1, zoff, g, s);
                     }
                     acc[0] += ok ? (double)r * (double)s : 0.0;
                     acc[1] += ok ? A::sq(r) : 0.0;
@@ -364,13 +407,28 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
             for (int s = 0; s < 2 * R; s++) q[p][s] = q[p][s + 1];
     };
 
-    prefetch(z0, s0);
+    // named sets (not an array) keep every set in registers
+    [[maybe_unused]] Pre s2;
     int z = z0;
-    for (; z + 1 < z1; z += 2) {
-        plane(z, s0, s1);
-        plane(z + 1, s1, s0);
+    if constexpr (PD == 1) {
+        prefetch(z0, s0);
+        for (; z + 1 < z1; z += 2) {
+            plane(z, s0, s1);
+            plane(z + 1, s1, s0);
+        }
+        if (z < z1) plane(z, s0, s1);
+    } else {
+        static_assert(PD == 2, "prefetch distance 1 or 2");
+        prefetch(z0, s0);
+        prefetch(z0 + 1, s1);
+        for (; z + 2 < z1; z += 3) {
+            plane(z, s0, s2);
+            plane(z + 1, s1, s0);
+            plane(z + 2, s2, s1);
+        }
+        if (z < z1) plane(z, s0, s2);
+        if (z + 1 < z1) plane(z + 1, s1, s0);
     }
-    if (z < z1) plane(z, s0, s1);
 
     if (a.partials) {
         block_reduce_to<NACC, NT>(acc, a.partials + (size_t)item * NACC, red_scratch);
