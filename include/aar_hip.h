@@ -77,6 +77,25 @@ int aar_hip_aar_f32(const aar_hip_grid *g, float *x, const float *b, double omeg
 int aar_hip_aar_mixed(const aar_hip_grid *g, double *x, const double *b, double omega, double beta,
                       int m, int p, int max_iter, double tol, int precond, aar_hip_stats *st);
 
+/* Variable-coefficient operator (extension; no MPI_REAL counterpart -- the
+ * device-side operator behind the reference's Lap_Vec_mult callback slot,
+ * README:25, and MPI_REAL/system.h:82-84's Lap_Vec_mult/Precondition):
+ *     A = a*Lap_FD12 + c(x),  periodic,  Jacobi diagonal D(x) = a*(3*coeff[0]) + c(x).
+ * c is a DEVICE array of N doubles; g->sigma must be 0 (a constant shift
+ * belongs in c).  With c = sigma everywhere every result equals the screened
+ * operator's bit for bit. */
+int aar_hip_matvec_vc(const aar_hip_grid *g, const double *c, const double *x, double *y);
+/* AAR.c:37-177 on A(c) with the pointwise Jacobi step r/D(x) (precond = 1)
+ * or none; mixed = 1: the FP32 recurrence sweeps of aar_hip_aar_mixed. */
+int aar_hip_aar_vc(const aar_hip_grid *g, const double *c, double *x, const double *b, double omega,
+                   double beta, int m, int p, int max_iter, double tol, int precond, int mixed,
+                   aar_hip_stats *st);
+/* Registers c(x) -- a HOST array of n doubles, copied to the GPU -- for the
+ * drop-in slots Lap_Vec_mult_vc / Precondition_vc (system.h), which the
+ * reference's AAR() then runs on the GPU; c = NULL clears it.  Kept until
+ * cleared, aar_hip_release() or Deallocate_memory(). */
+int aar_hip_set_coefficient(const double *c, long n);
+
 /* One solve of a batch: its own grid, operator a*Lap + sigma and vectors
  * (device pointers; x: initial guess in, solution out). */
 typedef struct {

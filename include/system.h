@@ -101,6 +101,15 @@ void Lap_Vec_mult(POISSON *system, double a, double *phi, double *Lap_phi, MPI_C
 /* MPI_REAL/system.h:84 -- res /= diag on the GPU (host array) */
 void Precondition(double diag, double *res, int Np);
 
+/* Extension, no reference counterpart: the variable-coefficient operator
+ * a*Lap + c(x) and its pointwise Jacobi step res /= (diag + c(x)), with the
+ * c(x) registered by aar_hip_set_coefficient (aar_hip.h).  Same signatures
+ * as Lap_Vec_mult / Precondition (MPI_REAL/system.h:82-84), so a caller
+ * passes them to AAR() in their place (README:25's callback slot); AAR()
+ * then solves A(c) x = rhs on the GPU.  PGR/PL2R/CG refuse them. */
+void Lap_Vec_mult_vc(POISSON *system, double a, double *phi, double *Lap_phi, MPI_Comm comm_laplacian);
+void Precondition_vc(double diag, double *res, int Np);
+
 /* MPI_REAL/system.h:86 */
 void Find_size_dir(int rem, int coords, int psize, int *small, int *large);
 

@@ -51,6 +51,7 @@ REFERENCE_SYMBOLS = [
     "AAR", "AndersonExtrapolation", "PGR", "Galerkin_Richardson", "PL2R", "L2_Richardson", "CG",
     "CheckInputs", "Processor_Domain", "Comm_topologies", "Max_layer", "Initialize",
     "Deallocate_memory", "Vec_copy", "Lap_coefficient", "Lap_Vec_mult", "Precondition",
+    "Lap_Vec_mult_vc", "Precondition_vc",
     "Find_size_dir", "Get_block_origin_global_coords",
     "pythag", "SingularValueDecomp", "PseudoInverseTimesVec", "Vector2Norm", "VectorDotProduct",
 ]
@@ -61,7 +62,7 @@ EXTENSION_SYMBOLS = [
     "aar_hip_set_stream", "aar_hip_set_tuning", "aar_hip_set_option", "aar_hip_profile", "aar_hip_profile_reset",
     "aar_hip_profile_query", "aar_hip_release", "aar_hip_last_error",
     "aar_hip_helm_matvec", "aar_hip_helm_aar", "aar_hip_helm_grad_c", "aar_hip_aar_batch",
-    "aar_hip_autotune", "aar_hip_last_system",
+    "aar_hip_autotune", "aar_hip_last_system", "aar_hip_matvec_vc", "aar_hip_aar_vc", "aar_hip_set_coefficient",
 ]
 
 
@@ -112,6 +113,9 @@ def load(require_gpu=True):
         "aar_hip_autotune": [I, I, I, I, I, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I),
                              ctypes.POINTER(D)],
         "aar_hip_last_system": [I, P, P],
+        "aar_hip_matvec_vc": [G, P, P, P],
+        "aar_hip_aar_vc": [G, P, P, P, D, D, I, I, I, D, I, I, S],
+        "aar_hip_set_coefficient": [P, ctypes.c_long],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -107,6 +107,49 @@ def AAR_mixed(x, rhs, shape, a, omega, beta, m, p, max_iter, tol, sigma=0.0, pre
     return st.as_dict()
 
 
+def Lap_Vec_mult_vc(x, shape, a, c, out=None):
+    """y = a*Lap(x) + c(x) x (no reference counterpart: the device operator
+    behind libaar_hip's Lap_Vec_mult_vc slot).  c: float64 CUDA/HIP tensor,
+    one value per grid point."""
+    import torch
+    _check_len(x, shape, "x")
+    _check_len(c, shape, "c")
+    y = torch.empty_like(x) if out is None else out
+    g = _grid(shape, a, 0.0)
+    _bind_stream()
+    check(_L().aar_hip_matvec_vc(ctypes.byref(g), _dev(c, torch.float64, "c"), _dev(x, torch.float64, "x"),
+                                 _dev(y, torch.float64, "y")))
+    return y
+
+
+def AAR_vc(x, rhs, shape, a, c, omega, beta, m, p, max_iter, tol, precond=True, mixed=False):
+    """AAR.c:37-177 on A = a*Lap + c(x) with the pointwise Jacobi diagonal
+    a*(3*coeff[0]) + c(x) (no reference counterpart); mixed: FP32 sweeps as
+    AAR_mixed."""
+    import torch
+    _check_len(x, shape, "x")
+    _check_len(rhs, shape, "rhs")
+    _check_len(c, shape, "c")
+    g = _grid(shape, a, 0.0)
+    st = Stats()
+    _bind_stream()
+    check(_L().aar_hip_aar_vc(ctypes.byref(g), _dev(c, torch.float64, "c"), _dev(x, torch.float64, "x"),
+                              _dev(rhs, torch.float64, "rhs"), omega, beta, m, p, max_iter, tol,
+                              int(bool(precond)), int(bool(mixed)), ctypes.byref(st)))
+    return st.as_dict()
+
+
+def set_coefficient(c):
+    """Registers c(x) (a host numpy array, or None to clear) for the drop-in
+    Lap_Vec_mult_vc / Precondition_vc slots."""
+    import numpy as np
+    if c is None:
+        check(_L().aar_hip_set_coefficient(None, 0))
+        return
+    arr = np.ascontiguousarray(c, dtype=np.float64)
+    check(_L().aar_hip_set_coefficient(arr.ctypes.data_as(ctypes.c_void_p), arr.size))
+
+
 def AAR_batch(instances, omega, beta, m, p, max_iter, tol, precond=True, mixed=False):
     """Independent AAR solves run together (no reference counterpart).
 

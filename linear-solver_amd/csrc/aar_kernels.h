@@ -31,6 +31,10 @@ struct SArgs {
     int cg_first = 0;
     const T *in0 = nullptr, *in1 = nullptr, *in2 = nullptr;   // stencil operand fields
     const T *bvec = nullptr;          // right-hand side for the epilogue
+    // variable-coefficient operator A = a*Lap + c(x) (the VC sweeps only):
+    // MATVEC/AAR_TRUE read c(x); AAR_REC reads h(x) = omega/(diag + c(x))
+    // with a Jacobi preconditioner, else c(x)
+    const T *vf = nullptr;
     T *out0 = nullptr, *out1 = nullptr;
     float *out32 = nullptr;
     const double *cg_beta = nullptr;
@@ -45,6 +49,15 @@ template <typename T>
 hipError_t launch_stencil(SMode m, const SArgs<T> &a, int wy, int py, hipStream_t st);
 template <typename T>
 int stencil_items(const SArgs<T> &a, int wy, int py);
+// the variable-coefficient sweeps (MATVEC, AAR_TRUE, AAR_REC with a.vf set)
+template <typename T>
+hipError_t launch_stencil_vc(SMode m, const SArgs<T> &a, int wy, int py, hipStream_t st);
+// h = omega/(diag + c) (precond) or c, rounded to Tg
+template <typename Tg>
+hipError_t launch_vc_field(long n, const double *c, double diag, double omega, int precond, Tg *h,
+                           hipStream_t st);
+// r /= (diag + c), pointwise
+hipError_t launch_div_vc(double *r, double diag, const double *c, long n, hipStream_t st);
 // fills the tile geometry of `a` (nxy, ntx, nty, nzc, nitems; lz clamped)
 template <typename T>
 void stencil_geometry(SArgs<T> &a, int wy, int py);

@@ -55,6 +55,7 @@ struct Context {
     DevBuf hist0, hist1, part0, part1;
     DevBuf vec32a, vec32b, vec64a, vec64b;
     DevBuf ringc, corrc;         // complex Helmholtz g ring and corrections
+    DevBuf vch;                  // variable coefficient: the recurrence field h(x)
     std::string err;
     int lz = 0, wy = 8, py = 4;
     long ring_pad = 512;           // extra elements per ring slot (multiple of 32)
@@ -80,6 +81,9 @@ struct Op {
     double w[R + 1] = {};
     double sigma = 0.0;
     double diag = 1.0;
+    // variable coefficient (device, n doubles): A = a*Lap + c(x) with the
+    // pointwise Jacobi diagonal diag + c(x); sigma is 0 then
+    const double *c = nullptr;
 };
 
 Context &ctx();

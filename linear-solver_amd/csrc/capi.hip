@@ -35,6 +35,8 @@ using namespace aar;
 extern "C" {
 void Lap_Vec_mult(POISSON *system, double a, double *phi, double *Lap_phi, aar_mpi_comm comm_laplacian);
 void Precondition(double diag, double *res, int Np);
+void Lap_Vec_mult_vc(POISSON *system, double a, double *phi, double *Lap_phi, aar_mpi_comm comm_laplacian);
+void Precondition_vc(double diag, double *res, int Np);
 void Lap_coefficient(double *coeff_lap, int FDn);
 }
 
@@ -62,29 +64,64 @@ int stage(long n)
     return 0;
 }
 
+// the coefficient field registered for the Lap_Vec_mult_vc /
+// Precondition_vc slots (aar_hip_set_coefficient)
+DevBuf g_coef;
+long g_coef_n = 0;
+
 void stage_release()
 {
     if (g_stage.x) (void)hipFree(g_stage.x);
     if (g_stage.b) (void)hipFree(g_stage.b);
     g_stage = Staging();
+    if (g_coef.p) (void)hipFree(g_coef.p);
+    g_coef = DevBuf();
+    g_coef_n = 0;
 }
 
-// The reference's operator slot: only libaar_hip's own stencil runs (on
-// the GPU); a foreign callback is not executed on the host.
-int check_callbacks(const char *who, aar_op_cb op, aar_pc_cb pc, int *precond)
+// The reference's operator slot: only libaar_hip's own device operators
+// run (on the GPU) -- the FD12 stencil (Lap_Vec_mult, Precondition) or the
+// variable-coefficient operator with the registered c(x) (Lap_Vec_mult_vc,
+// Precondition_vc); a foreign callback is not executed on the host.
+int check_callbacks(const char *who, aar_op_cb op, aar_pc_cb pc, int *precond, int *vc)
 {
-    if (op != &Lap_Vec_mult) {
-        std::printf("%s: operator callback is not libaar_hip's Lap_Vec_mult; refusing to run it "
-                    "(register a device operator instead)\n", who);
+    const bool opv = op == &Lap_Vec_mult_vc;
+    if (op != &Lap_Vec_mult && !opv) {
+        std::printf("%s: operator callback is not libaar_hip's Lap_Vec_mult or Lap_Vec_mult_vc; refusing to "
+                    "run it (register a device operator instead)\n", who);
         fail(AAR_HIP_ERR_CALLBACK, "foreign operator callback");
         return AAR_HIP_ERR_CALLBACK;
     }
-    if (pc != nullptr && pc != &Precondition) {
-        std::printf("%s: preconditioner callback is not libaar_hip's Precondition; refusing to run it\n", who);
+    const aar_pc_cb own = opv ? &Precondition_vc : &Precondition;
+    if (pc != nullptr && pc != own) {
+        std::printf("%s: preconditioner callback is not libaar_hip's %s; refusing to run it\n", who,
+                    opv ? "Precondition_vc (the operator is Lap_Vec_mult_vc)" : "Precondition");
         fail(AAR_HIP_ERR_CALLBACK, "foreign preconditioner callback");
         return AAR_HIP_ERR_CALLBACK;
     }
     *precond = pc != nullptr;
+    *vc = opv;
+    return 0;
+}
+
+// variable-coefficient operator (extension): A = a*Lap + c(x), Jacobi
+// diagonal a*(3*coeff[0]) + c(x); g->sigma must be 0 (a shift goes into c)
+int make_op_vc(const aar_hip_grid *g, const double *c, Op &op)
+{
+    if (int rc = make_op(g, nullptr, op)) return rc;
+    if (!c) return fail(AAR_HIP_ERR_ARG, "null coefficient field");
+    if (g->sigma != 0.0) return fail(AAR_HIP_ERR_ARG, "sigma must be 0 with a coefficient field (add it to c)");
+    op.c = c;
+    return 0;
+}
+
+// the registered coefficient for an n-point grid
+int registered_coefficient(long n, const double **c)
+{
+    if (g_coef_n != n || !g_coef.p)
+        return fail(AAR_HIP_ERR_ARG, "no coefficient field of this grid's size is registered "
+                                     "(aar_hip_set_coefficient)");
+    *c = (const double *)g_coef.p;
     return 0;
 }
 
@@ -150,10 +187,12 @@ template <class F>
 void host_solve(const char *name, POISSON *sys, aar_op_cb opcb, aar_pc_cb pccb, double a, double *x,
                 double *rhs, int Np, F &&run)
 {
-    int precond = 0;
-    if (check_callbacks(name, opcb, pccb, &precond)) return;
+    int precond = 0, vc = 0;
+    if (check_callbacks(name, opcb, pccb, &precond, &vc)) return;
     Op op;
     if (int rc = op_from_system(sys, a, Np, op)) { report_failure(name, rc); return; }
+    if (vc)
+        if (int rc = registered_coefficient(op.n, &op.c)) { report_failure(name, rc); return; }
     if (int rc = stage(op.n)) { report_failure(name, rc); return; }
     Context &C = ctx();
     const size_t bytes = sizeof(double) * (size_t)op.n;
@@ -470,6 +509,39 @@ void Precondition(double diag, double *res, int Np)
     (void)hipStreamSynchronize(C.stream);
 }
 
+// Variable-coefficient operator slots (no MPI_REAL counterpart: the
+// device operator behind the reference's callback slot, README:25):
+// Lap_phi = a * Lap * phi + c(x) phi with the registered c(x), on the GPU
+void Lap_Vec_mult_vc(POISSON *system, double a, double *phi, double *Lap_phi, aar_mpi_comm)
+{
+    Op op;
+    const int np = system->psize[0] * system->psize[1] * system->psize[2];
+    if (int rc = op_from_system(system, a, np, op)) { report_failure("Lap_Vec_mult_vc", rc); return; }
+    if (int rc = registered_coefficient(op.n, &op.c)) { report_failure("Lap_Vec_mult_vc", rc); return; }
+    if (int rc = stage(op.n)) { report_failure("Lap_Vec_mult_vc", rc); return; }
+    Context &C = ctx();
+    const size_t bytes = sizeof(double) * (size_t)op.n;
+    if (hipMemcpyAsync(g_stage.x, phi, bytes, hipMemcpyHostToDevice, C.stream) != hipSuccess) return;
+    if (int rc = matvec<double>(op, g_stage.x, g_stage.b)) { report_failure("Lap_Vec_mult_vc", rc); return; }
+    (void)hipMemcpyAsync(Lap_phi, g_stage.b, bytes, hipMemcpyDeviceToHost, C.stream);
+    (void)hipStreamSynchronize(C.stream);
+}
+
+// res /= (diag + c(x)) pointwise with the registered c(x), on the GPU
+// (diag = a*(3*coeff[0]), as the solvers pass it, AAR.c:94)
+void Precondition_vc(double diag, double *res, int Np)
+{
+    const double *c = nullptr;
+    if (int rc = registered_coefficient(Np, &c)) { report_failure("Precondition_vc", rc); return; }
+    if (stage(Np)) return;
+    Context &C = ctx();
+    const size_t bytes = sizeof(double) * (size_t)Np;
+    (void)hipMemcpyAsync(g_stage.x, res, bytes, hipMemcpyHostToDevice, C.stream);
+    (void)launch_div_vc(g_stage.x, diag, c, Np, C.stream);
+    (void)hipMemcpyAsync(res, g_stage.x, bytes, hipMemcpyDeviceToHost, C.stream);
+    (void)hipStreamSynchronize(C.stream);
+}
+
 // MPI_REAL/tools.c:345-352
 double pythag(double a, double b)
 {
@@ -684,6 +756,37 @@ int aar_hip_matvec(const aar_hip_grid *g, const double *x, double *y)
     Op op;
     if (int rc = make_op(g, nullptr, op)) return rc;
     return matvec<double>(op, x, y);
+}
+
+int aar_hip_set_coefficient(const double *c, long n)
+{
+    if (int rc = context_init()) return rc;
+    if (!c || n <= 0) {
+        g_coef_n = 0;
+        return 0;
+    }
+    Context &C = ctx();
+    HCR(ensure(g_coef, sizeof(double) * (size_t)n), AAR_HIP_ERR_HIP);
+    HCR(hipMemcpyAsync(g_coef.p, c, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, C.stream), AAR_HIP_ERR_HIP);
+    HCR(hipStreamSynchronize(C.stream), AAR_HIP_ERR_HIP);
+    g_coef_n = n;
+    return 0;
+}
+
+int aar_hip_matvec_vc(const aar_hip_grid *g, const double *c, const double *x, double *y)
+{
+    Op op;
+    if (int rc = make_op_vc(g, c, op)) return rc;
+    return matvec<double>(op, x, y);
+}
+
+int aar_hip_aar_vc(const aar_hip_grid *g, const double *c, double *x, const double *b, double omega, double beta,
+                   int m, int p, int max_iter, double tol, int precond, int mixed, aar_hip_stats *st)
+{
+    Op op;
+    if (int rc = make_op_vc(g, c, op)) return rc;
+    if (mixed) return aar_mixed(op, x, b, omega, beta, m, p, max_iter, tol, precond, st);
+    return aar_solve<double>(op, x, b, omega, beta, m, p, max_iter, tol, precond, 0, st, 1);
 }
 
 int aar_hip_matvec_f32(const aar_hip_grid *g, const float *x, float *y)

@@ -170,6 +170,8 @@ SArgs<T> base_sargs(const Op &op)
     for (int o = 0; o <= R; o++) a.w[o] = op.w[o];
     a.sigma = op.sigma;
     a.diag = op.diag;
+    // the FP64 sweeps read c(x) itself; the FP32 recurrence gets its own field
+    if constexpr (sizeof(T) == 8) a.vf = op.c;
     return a;
 }
 
@@ -491,6 +493,8 @@ int aar_rec(const Op &op, Tx *x, const Tx *b, double omega, double beta, int m, 
     if (m < 1 || m > MAXM) return fail(AAR_HIP_ERR_ARG, "history m must be in 1..8");
     if (p < 1) return fail(AAR_HIP_ERR_ARG, "p must be >= 1");
     if (max_iter < 0) return fail(AAR_HIP_ERR_ARG, "max_iter must be >= 0");
+    if (op.c && sizeof(Tx) != 8)
+        return fail(AAR_HIP_ERR_ARG, "the variable-coefficient operator needs FP64 x and b (FP64 or mixed AAR)");
     const long n = op.n;
     // g ring: g_k in slot k % S.  Live at any time: the m+1 residuals of the
     // next Anderson window and the one being written.
@@ -521,6 +525,14 @@ int aar_rec(const Op &op, Tx *x, const Tx *b, double omega, double beta, int m, 
     auto next_and = [&](long k) { long q = (k / p + 1) * p; return q > 1 ? q : q + p; };
     const double sx = (double)sizeof(Tx) * n, sg = (double)sizeof(Tg) * n;
     C.bytes_acc = 0.0;
+    // variable coefficient: the recurrence reads h = omega/(diag + c(x))
+    // (Jacobi) or c(x) itself (no preconditioner), rounded to Tg
+    const Tg *vch = nullptr;
+    if (op.c) {
+        HC(ensure(C.vch, sizeof(Tg) * (size_t)n));
+        vch = (const Tg *)C.vch.p;
+    }
+    const double sv_true = op.c ? sx : 0.0, sv_rec = op.c ? sg : 0.0;
 
     // true residual of x into g_k (AAR.c:80-82 / 119-123)
     auto true_res = [&](long k, int check) -> int {
@@ -539,11 +551,17 @@ int aar_rec(const Op &op, Tx *x, const Tx *b, double omega, double beta, int m, 
         a.fin.check = check;
         a.fin.hist = hist;
         a.fin.hist_cap = max_iter + 4;
-        return run_stencil<Tx>(SMode::AAR_TRUE, a, 2 * sx + sg);
+        return run_stencil<Tx>(SMode::AAR_TRUE, a, 2 * sx + sg + sv_true);
     };
 
     HC(launch_zero_ctrl(ctrl, C.stream));
     HC(hipEventRecord(C.t0, C.stream));
+    if (vch) {
+        prof_begin("vc_field");
+        HC(launch_vc_field<Tg>(n, op.c, op.diag, omega, precond, (Tg *)C.vch.p, C.stream));
+        prof_end(sx + sg);
+        C.bytes_acc += sx + sg;
+    }
     // ||b|| and the scaled tolerance (AAR.c:78-79)
     {
         DotArgs<Tx> d;
@@ -573,7 +591,10 @@ int aar_rec(const Op &op, Tx *x, const Tx *b, double omega, double beta, int m, 
             SArgs<Tg> a = base_sargs<Tg>(op);
             a.omega = omega;
             a.precond = precond;
-            {
+            if (vch) {
+                a.vf = vch;
+                a.cf[0] = 1.0 - omega;            // (1 - omega) g - h sum_o w_o S_o
+            } else {
                 const double c = precond ? omega / op.diag : omega;
                 a.cf[0] = 1.0 - c * (3.0 * op.w[0] + op.sigma);
                 for (int o = 1; o <= R; o++) a.cf[o] = -c * op.w[o];
@@ -589,7 +610,7 @@ int aar_rec(const Op &op, Tx *x, const Tx *b, double omega, double beta, int m, 
             a.fin.check = 0;
             a.fin.hist = hist;
             a.fin.hist_cap = max_iter + 4;
-            if (int rc = run_stencil<Tg>(SMode::AAR_REC, a, 2 * sg)) return rc;
+            if (int rc = run_stencil<Tg>(SMode::AAR_REC, a, 2 * sg + sv_rec)) return rc;
             // a residual that no Anderson window will read again is folded
             // into the anchor before its slot is reused (only when p > m+1)
             if (it < next_and(it) - m) {
@@ -746,6 +767,8 @@ template int aar_solve<float>(const Op &, float *, const float *, double, double
 int pgr_solve(const Op &op, double *x, const double *b, double omega, int m, int p, int max_iter,
               double tol, int precond, int l2, aar_hip_stats *st)
 {
+    if (op.c)
+        return fail(AAR_HIP_ERR_ARG, "the variable-coefficient operator is built for AAR (FP64, mixed) and the matvec");
     Context &C = ctx();
     if (int rc = context_init()) return rc;
     if (!x || !b) return fail(AAR_HIP_ERR_ARG, "null vector");
@@ -903,6 +926,8 @@ int pgr_solve(const Op &op, double *x, const double *b, double omega, int m, int
 int cg_solve(const Op &op, double *x, const double *b, int max_iter, double tol, int precond,
              aar_hip_stats *st)
 {
+    if (op.c)
+        return fail(AAR_HIP_ERR_ARG, "the variable-coefficient operator is built for AAR (FP64, mixed) and the matvec");
     Context &C = ctx();
     if (int rc = context_init()) return rc;
     if (!x || !b) return fail(AAR_HIP_ERR_ARG, "null vector");
@@ -1041,10 +1066,12 @@ int matvec(const Op &op, const T *x, T *y)
     Context &C = ctx();
     if (int rc = context_init()) return rc;
     if (!x || !y) return fail(AAR_HIP_ERR_ARG, "null vector");
+    if (op.c && sizeof(T) != 8) return fail(AAR_HIP_ERR_ARG, "the variable-coefficient matvec is FP64");
     SArgs<T> a = base_sargs<T>(op);
     a.in0 = x;
     a.out0 = y;
-    if (int rc = run_stencil<T>(SMode::MATVEC, a, 2.0 * sizeof(T) * op.n)) return rc;
+    const double by = (op.c ? 3.0 : 2.0) * sizeof(T) * op.n;
+    if (int rc = run_stencil<T>(SMode::MATVEC, a, by)) return rc;
     HC(hipStreamSynchronize(C.stream));
     prof_flush();
     return 0;
@@ -1344,6 +1371,8 @@ int aar_refined(const Op &op, double *x, const double *b, double omega, double b
                 int max_outer, int max_inner, double tol, double inner_tol, int precond,
                 aar_hip_stats *st)
 {
+    if (op.c)
+        return fail(AAR_HIP_ERR_ARG, "the variable-coefficient operator is built for AAR (FP64, mixed) and the matvec");
     Context &C = ctx();
     if (int rc = context_init()) return rc;
     if (!x || !b) return fail(AAR_HIP_ERR_ARG, "null vector");
@@ -1475,7 +1504,7 @@ void release_all()
     Context &C = ctx();
     DevBuf *bufs[] = {&C.ring64x, &C.ring64f, &C.ring32x, &C.ring32f, &C.hist0, &C.hist1,
                       &C.part0, &C.part1, &C.vec32a, &C.vec32b, &C.vec64a, &C.vec64b,
-                      &C.ringc, &C.corrc};
+                      &C.ringc, &C.corrc, &C.vch};
     for (DevBuf *b : bufs) {
         if (b->p) (void)hipFree(b->p);
         b->p = nullptr;

@@ -61,6 +61,7 @@ hipError_t launch_stencil_t(SArgs<T> a, int wy, int py, hipStream_t st)
 template <typename T>
 hipError_t launch_stencil(SMode m, const SArgs<T> &a, int wy, int py, hipStream_t st)
 {
+    if (a.vf) return launch_stencil_vc<T>(m, a, wy, py, st);
     switch (m) {
     case SMode::MATVEC: return launch_stencil_t<T, SMode::MATVEC>(a, wy, py, st);
     case SMode::AAR_TRUE: return launch_stencil_t<T, SMode::AAR_TRUE>(a, wy, py, st);

@@ -68,10 +68,13 @@ template <bool SA, typename T> __device__ __forceinline__ void st_pl(T *base, si
 // the operator and fields in `a`; shared by the single-operator kernel
 // below and the batched kernel of batch.hip.
 // ADDR: 1 = plane pointer + 32-bit offset accesses, 0 = one signed index,
-// -1 = the per-mode choice below
-template <typename T, int WY, int PY, SMode MODE, int ADDR = -1>
+// -1 = the per-mode choice below.  VC: the variable-coefficient operator
+// a*Lap + c(x), with a pointwise field a.vf read in the epilogue.
+template <typename T, int WY, int PY, SMode MODE, int ADDR = -1, bool VC = false>
 __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
 {
+    static_assert(!VC || MODE == SMode::MATVEC || MODE == SMode::AAR_TRUE || MODE == SMode::AAR_REC,
+                  "variable-coefficient sweeps: MATVEC, AAR_TRUE, AAR_REC");
     using A = Arith<T>;
     constexpr int NT = 64 * WY;                       // threads: WY wavefronts stacked in y
     constexpr int NF = ModeInfo<MODE>::NF;
@@ -93,7 +96,7 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
     // inputs, whose one block per CU (LDS-limited) leaves VGPRs to spare; 1
     // where a third register set would cost a block per CU (FP32), spill
     // (two or three fields) or measured slower (epilogue inputs)
-    constexpr int PD = (sizeof(T) == 8 && NF == 1 && NE == 0) ? AAR_STENCIL_PD64 : 1;
+    constexpr int PD = (sizeof(T) == 8 && NF == 1 && NE == 0 && !VC) ? AAR_STENCIL_PD64 : 1;
 
     constexpr int HR = R + 1;                         // halo ring depth (planes)
     constexpr int NH = NYH + NXH;                     // halo cells per thread
@@ -159,6 +162,7 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
 #pragma unroll
     for (int o = 0; o <= R; o++) wt[o] = (T)a.w[o];
     const T sig = (T)a.sigma, dg = (T)a.diag, om = (T)a.omega;
+    [[maybe_unused]] const T w3 = (T)(3.0 * a.w[0]);
     T cf[R + 1];
 #pragma unroll
     for (int o = 0; o <= R; o++) cf[o] = (T)a.cf[o];
@@ -200,6 +204,7 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
         T hy[NYH][NF];
         T hx[NXH][NF];
         T en[PY][NE > 0 ? NE : 1];
+        T vv[PY];                                     // VC: a.vf at the plane
     };
     Pre s0, s1;
 
@@ -246,6 +251,10 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
 #pragma unroll
             for (int p = 0; p < PY; p++) st.en[p][0] = ld_pl<SA>(a.bvec, zc, c_off[p]);
         }
+        if constexpr (VC) {
+#pragma unroll
+            for (int p = 0; p < PY; p++) st.vv[p] = ld_pl<SA>(a.vf, zc, c_off[p]);
+        }
     };
 
     double acc[NACC];
@@ -291,7 +300,26 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
                 // system.c:195-201 order: centre, then o = 1..6 with
                 // (x-, x+, y-, y+, z-, z+) summed left to right
                 T y;
-                if constexpr (MODE == SMode::AAR_REC) {
+                if constexpr (MODE == SMode::AAR_REC && VC) {
+                    // variable coefficient, D = diag + c(x): A g = D g +
+                    // sum_o w_o S_o, so with the Jacobi step g' = g -
+                    // (omega/D) A g = (1 - omega) g - h sum_o w_o S_o, h =
+                    // omega/D = a.vf; without a preconditioner a.vf = c and
+                    // g' = (1 - omega (3 w0 + c)) g - omega sum_o w_o S_o
+                    T ns = (T)0;
+#pragma unroll
+                    for (int o = 1; o <= R; o++) {
+                        T nb = A::add(lds[buf][lr][lc - o], lds[buf][lr][lc + o]);
+                        nb = A::add(nb, ynb(-o));
+                        nb = A::add(nb, ynb(o));
+                        nb = A::add(nb, q[p][R - o]);
+                        nb = A::add(nb, q[p][R + o]);
+                        ns = fma(nb, wt[o], ns);
+                    }
+                    const T hv = cur.vv[p];
+                    if (a.precond) y = fma(-hv, ns, A::mul(u, cf[0]));
+                    else y = fma(-om, ns, A::mul(u, A::sub((T)1, A::mul(om, A::add(w3, hv)))));
+                } else if constexpr (MODE == SMode::AAR_REC) {
                     // the Richardson step folded into the weights: g' = cf0 g +
                     // sum_o cf_o S_o, cf0 = 1 - c (3 w0 + sigma), cf_o = -c w_o,
                     // c = omega/diag (no reference counterpart; the reference
@@ -336,6 +364,8 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
                     }
                 }
                 if (MODE != SMode::AAR_REC && a.sigma != 0.0) y = A::add(y, A::mul(sig, u));
+                // y + c(x) x, the order of the screened term above
+                if constexpr (VC && MODE != SMode::AAR_REC) y = A::add(y, A::mul(cur.vv[p], u));
                 // points past the grid edge (partial tiles) computed on wrapped
                 // inputs: only their stores and norm terms are masked, so the
                 // stencil itself runs without branches
@@ -353,7 +383,11 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
                     const T r = A::sub(cur.en[p][0], y);
                     acc[0] += ok ? A::sq(r) : 0.0;
                     T f = r;
-                    if (a.precond) f = A::div(f, dg);
+                    if constexpr (VC) {
+                        if (a.precond) f = A::div(f, A::add(dg, cur.vv[p]));
+                    } else {
+                        if (a.precond) f = A::div(f, dg);
+                    }
                     if (ok) {
                         if (a.out32) st_pl<SA>(a.out32, zoff, g, (float)f);
                         else st_pl<SA>(a.out0, zoff, g, f);
@@ -363,7 +397,13 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
                     // ||g_{k+1}||^2 = ||rhs - A x_{k+1}||^2 for the history
                     if (ok) st_pl<SA>(a.out0, zoff, g, y);
                     if (a.trace) {
-                        const double sv = ok ? (double)y * trace_scale : 0.0;
+                        // ||rhs - A x||: D g' pointwise (VC: D = omega/h, by
+                        // the hardware reciprocal -- the trace is a record,
+                        // convergence is decided on the AAR_TRUE sweep)
+                        double sc = trace_scale;
+                        if constexpr (VC)
+                            sc = a.precond ? a.omega * __builtin_amdgcn_rcp((double)cur.vv[p]) : 1.0;
+                        const double sv = ok ? (double)y * sc : 0.0;
                         acc[0] = fma(sv, sv, acc[0]);
                     }
                 } else if constexpr (MODE == SMode::RESID) {
@@ -443,11 +483,11 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
     }
 }
 
-template <typename T, int WY, int PY, SMode MODE>
+template <typename T, int WY, int PY, SMode MODE, bool VC = false>
 __global__ __launch_bounds__(64 * WY) void stencil_kernel(const SArgs<T> a)
 {
     if (a.ctrl && a.ctrl->done) return;
-    stencil_body<T, WY, PY, MODE>(a, xcd_item(blockIdx.x, a.nitems));
+    stencil_body<T, WY, PY, MODE, -1, VC>(a, xcd_item(blockIdx.x, a.nitems));
 }
 
 }  // namespace aar

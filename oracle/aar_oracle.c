@@ -386,16 +386,34 @@ static void finish_stats(orc_stats *st, int iters, double relres, double rhs_nor
 /* ------------------------------------------------------------------ */
 /* AAR.c:37-177.                                                       */
 /* ------------------------------------------------------------------ */
-int orc_aar(int nx, int ny, int nz, int fdn, double a, double sigma,
-            double *x, const double *rhs, double omega, double beta,
-            int m, int p, int max_iter, double tol, int precond,
-            double *trace, int trace_cap, orc_stats *st)
+/* y = a*Lap(x) + c(x) x: the reference operator (system.c:189-205) plus a
+ * pointwise term added after it, as the screened term of orc_matvec. */
+void orc_matvec_vc(int nx, int ny, int nz, int fdn, const double *coeff,
+                   double a, const double *c, const double *x, double *y)
+{
+    orc_matvec(nx, ny, nz, fdn, coeff, a, 0.0, x, y);
+    const long n = (long)nx * ny * nz;
+    for (long i = 0; i < n; i++) y[i] = y[i] + c[i] * x[i];
+}
+
+/* AAR.c:37-177 for the operator a*Lap + sigma (c == NULL) or a*Lap + c(x);
+ * with c the Jacobi step of Precondition (system.c:222-227) divides by the
+ * pointwise diagonal a*(3*coeff[0]) + c(x). */
+static int aar_impl(int nx, int ny, int nz, int fdn, double a, double sigma, const double *cv,
+                    double *x, const double *rhs, double omega, double beta,
+                    int m, int p, int max_iter, double tol, int precond,
+                    double *trace, int trace_cap, orc_stats *st)
 {
     const long n = (long)nx * ny * nz;
     double coeff[32] = {0};
     orc_lap_coefficient(coeff, fdn);
     double diag = a * (3 * coeff[0]);
     if (sigma != 0.0) diag += sigma;
+#define AAR_OP(xin, yout)                                                   \
+    do {                                                                    \
+        if (cv) orc_matvec_vc(nx, ny, nz, fdn, coeff, a, cv, xin, yout);    \
+        else orc_matvec(nx, ny, nz, fdn, coeff, a, sigma, xin, yout);       \
+    } while (0)
 
     double *f = (double *)calloc((size_t)n, sizeof(double));
     double *x_old = (double *)calloc((size_t)n, sizeof(double));
@@ -409,14 +427,16 @@ int orc_aar(int nx, int ny, int nz, int fdn, double a, double sigma,
     double relres = tol + 1;                 /* AAR.c:76, before tol is scaled */
     double rhs_norm = orc_norm2(rhs, n);
     tol *= rhs_norm;
-    orc_matvec(nx, ny, nz, fdn, coeff, a, sigma, x, f);
+    AAR_OP(x, f);
     for (long i = 0; i < n; i++) f[i] = rhs[i] - f[i];
     if (trace && tlen < trace_cap) trace[tlen++] = orc_norm2(f, n);
 
     double t0 = now_seconds();
     while (relres > tol && iter <= max_iter) {
-        if (precond)
-            for (long i = 0; i < n; i++) f[i] /= diag;
+        if (precond) {
+            if (cv) for (long i = 0; i < n; i++) f[i] /= (diag + cv[i]);
+            else for (long i = 0; i < n; i++) f[i] /= diag;
+        }
         if (iter > 1) {
             const int col = (iter - 2) % m;
             double *dx = DX + (long)col * n, *df = DF + (long)col * n;
@@ -431,13 +451,13 @@ int orc_aar(int nx, int ny, int nz, int fdn, double a, double sigma,
         if (iter % p == 0 && iter > 1) {
             if (orc_anderson_extrapolation(DX, DF, f, beta, m, n, am, NULL, NULL)) { status = 1; break; }
             for (long i = 0; i < n; i++) x[i] = x_old[i] + beta * f[i] - am[i];
-            orc_matvec(nx, ny, nz, fdn, coeff, a, sigma, x, f);
+            AAR_OP(x, f);
             for (long i = 0; i < n; i++) f[i] = rhs[i] - f[i];
             relres = orc_norm2(f, n);
             rn = relres;
         } else {
             for (long i = 0; i < n; i++) x[i] = x_old[i] + omega * f[i];
-            orc_matvec(nx, ny, nz, fdn, coeff, a, sigma, x, f);
+            AAR_OP(x, f);
             for (long i = 0; i < n; i++) f[i] = rhs[i] - f[i];
         }
         if (trace && tlen < trace_cap) trace[tlen++] = rn >= 0.0 ? rn : orc_norm2(f, n);
@@ -447,6 +467,25 @@ int orc_aar(int nx, int ny, int nz, int fdn, double a, double sigma,
     finish_stats(st, iter - 1, relres, rhs_norm, tol, iter, max_iter, tlen, t1 - t0, status);
     free(f); free(x_old); free(f_old); free(am); free(DX); free(DF);
     return status;
+#undef AAR_OP
+}
+
+int orc_aar(int nx, int ny, int nz, int fdn, double a, double sigma,
+            double *x, const double *rhs, double omega, double beta,
+            int m, int p, int max_iter, double tol, int precond,
+            double *trace, int trace_cap, orc_stats *st)
+{
+    return aar_impl(nx, ny, nz, fdn, a, sigma, NULL, x, rhs, omega, beta, m, p, max_iter, tol, precond,
+                    trace, trace_cap, st);
+}
+
+int orc_aar_vc(int nx, int ny, int nz, int fdn, double a, const double *c,
+               double *x, const double *rhs, double omega, double beta,
+               int m, int p, int max_iter, double tol, int precond,
+               double *trace, int trace_cap, orc_stats *st)
+{
+    return aar_impl(nx, ny, nz, fdn, a, 0.0, c, x, rhs, omega, beta, m, p, max_iter, tol, precond,
+                    trace, trace_cap, st);
 }
 
 /* ------------------------------------------------------------------ */

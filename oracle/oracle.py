@@ -85,7 +85,10 @@ def lib():
         L.orc_aar_complex.argtypes = [I, I, I, I, Dd, _D, Dd, Dd, _D, _D, Dd, Dd, I, I, I, Dd, _D, I,
                                       ctypes.POINTER(OrcStats)]
         L.orc_matvec_complex.argtypes = [I, I, I, I, _D, Dd, _D, Dd, Dd, _D, _D]
-        for fn in ("orc_aar", "orc_pgr", "orc_cg", "orc_aar_complex"):
+        L.orc_matvec_vc.argtypes = [I, I, I, I, _D, Dd, _D, _D, _D]
+        L.orc_aar_vc.argtypes = [I, I, I, I, Dd, _D, _D, _D, Dd, Dd, I, I, I, Dd, I, _D, I,
+                                 ctypes.POINTER(OrcStats)]
+        for fn in ("orc_aar", "orc_pgr", "orc_cg", "orc_aar_complex", "orc_aar_vc"):
             getattr(L, fn).restype = I
         _lib = L
     return _lib
@@ -162,6 +165,27 @@ def aar(x0, rhs, shape, a, omega, beta, m, p, max_iter, tol, sigma=0.0, precond=
     b = _f64(rhs).reshape(-1)
     d = _run(lib().orc_aar, (nx, ny, nz, fdn, a, sigma, _p(x), _p(b), omega, beta, m, p,
                              max_iter, tol, int(precond)), trace, max_iter + 8)
+    return x, d
+
+
+def matvec_vc(x, shape, a, c, fdn=6):
+    """y = a*Lap(x) + c(x) x (variable-coefficient operator)."""
+    nx, ny, nz = shape
+    x = _f64(x).reshape(-1)
+    cv = _f64(c).reshape(-1)
+    y = np.empty_like(x)
+    lib().orc_matvec_vc(nx, ny, nz, fdn, _p(lap_coefficient(fdn)), a, _p(cv), _p(x), _p(y))
+    return y
+
+
+def aar_vc(x0, rhs, shape, a, c, omega, beta, m, p, max_iter, tol, precond=True, fdn=6, trace=True):
+    """AAR.c:37-177 on a*Lap + c(x) with the pointwise Jacobi diagonal."""
+    nx, ny, nz = shape
+    x = _f64(x0).reshape(-1).copy()
+    b = _f64(rhs).reshape(-1)
+    cv = _f64(c).reshape(-1)
+    d = _run(lib().orc_aar_vc, (nx, ny, nz, fdn, a, _p(cv), _p(x), _p(b), omega, beta, m, p,
+                                max_iter, tol, int(precond)), trace, max_iter + 8)
     return x, d
 
 

@@ -432,3 +432,76 @@ def test_pinv_textbook_converges_where_reference_order_does_not(oracle):
     keep = S >= 14 * S[0] * 2.22044605e-16
     xr = Vt.T[:, keep] @ ((Vt[keep] @ r) / S[keep])
     assert np.linalg.norm(x - xr) <= 1e-8 * np.linalg.norm(xr)
+
+
+# ---------------------------------------------------------------------------
+# variable-coefficient operator a*Lap + c(x) (extension)
+# ---------------------------------------------------------------------------
+
+def test_vc_constant_field_is_the_screened_oracle(oracle):
+    """c = sigma everywhere: orc_matvec_vc and orc_aar_vc equal the
+    (reference-pinned) screened orc_matvec / orc_aar bit for bit -- solution,
+    residual trace and iteration count, with and without Jacobi."""
+    shape = (16, 12, 10)
+    n = int(np.prod(shape))
+    a, sigma = P.poisson_a(0.5), 0.75
+    c = np.full(n, sigma)
+    x = np.random.default_rng(5).standard_normal(n)
+    assert np.array_equal(oracle.matvec_vc(x, shape, a, c), oracle.matvec(x, shape, a, sigma))
+    b = P.rhs_normal(shape, 81)
+    for precond, om, mi in ((True, 0.6, 600), (False, 0.015, 300)):
+        xs, ds = oracle.aar(np.zeros(n), b, shape, a, om, om, 7, 6, mi, 1e-9, sigma=sigma, precond=precond)
+        xv, dv = oracle.aar_vc(np.zeros(n), b, shape, a, c, om, om, 7, 6, mi, 1e-9, precond=precond)
+        assert dv["iterations"] == ds["iterations"]
+        assert np.array_equal(xv, xs)
+        assert np.array_equal(dv["trace"], ds["trace"])
+
+
+def test_vc_matvec_dense_bruteforce(oracle):
+    """a*Lap + diag(c) as a dense matrix (periodic wraps overlapping at sides
+    below 13) against orc_matvec_vc; a dropped or misplaced c term fails."""
+    shape = (8, 6, 7)
+    n = int(np.prod(shape))
+    coeff = oracle.lap_coefficient(6)
+    c = P.helmholtz_coefficient(shape, 9)
+    M = dense_operator(shape, -4.0, coeff) + np.diag(c)
+    x = np.random.default_rng(6).standard_normal(n)
+    y = oracle.matvec_vc(x, shape, -4.0, c)
+    assert np.allclose(y, M @ x, rtol=0, atol=1e-11)
+    assert not np.allclose(y, M @ x - c * x, rtol=0, atol=1e-3)
+    assert not np.allclose(y, M @ x + np.roll(c, 1) * x - c * x, rtol=0, atol=1e-3)
+
+
+@pytest.mark.parametrize("precond", [True, False])
+def test_vc_aar_solution_dense(oracle, precond):
+    """orc_aar_vc's converged solution against numpy's dense solve of the
+    same operator (a 9 x 8 x 7 grid, c(x) in [0.5, 1.5])."""
+    shape = (9, 8, 7)
+    n = int(np.prod(shape))
+    a = P.poisson_a(0.5)
+    c = P.helmholtz_coefficient(shape, 10)
+    M = dense_operator(shape, a, oracle.lap_coefficient(6)) + np.diag(c)
+    b = P.rhs_normal(shape, 82)
+    xd = np.linalg.solve(M, b)
+    om, mi = (0.6, 3000) if precond else (0.015, 20000)
+    x, d = oracle.aar_vc(np.zeros(n), b, shape, a, c, om, om, 7, 6, mi, 1e-12, precond=precond)
+    assert d["converged"]
+    assert d["iterations"] % 6 == 0
+    assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-9
+
+
+def test_vc_real_oracle_vs_complex_oracle(oracle):
+    """The real variable-coefficient AAR against the complex oracle with
+    eta = 0 on the same c(x) and a real b: the same converged solution and a
+    zero imaginary part (the two differ only in the order of their SVD)."""
+    shape = (16, 16, 16)
+    n = int(np.prod(shape))
+    a = P.poisson_a(0.5)
+    c = P.helmholtz_coefficient(shape, 11)
+    b = P.rhs_normal(shape, 83)
+    x, d = oracle.aar_vc(np.zeros(n), b, shape, a, c, 0.6, 0.6, 7, 6, 2000, 1e-12)
+    u, dc = oracle.aar_complex(np.zeros(n, complex), b.astype(complex), shape, a, c, 0.0, 0.0,
+                               0.6, 0.6, 7, 6, 2000, 1e-12)
+    assert d["converged"] and dc["converged"]
+    assert np.abs(u.imag).max() == 0.0
+    assert np.linalg.norm(u.real - x) / np.linalg.norm(x) < 1e-9
