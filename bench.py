@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--cpu-iters", type=int, default=12)
     ap.add_argument("--no-helm", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-vc", action="store_true")
     return ap.parse_args()
 
 
@@ -163,6 +164,46 @@ def helmholtz_case(n, dev, a, directions=20, hstep=1e-6, tol=1e-13):
             "solves_per_s": 1e3 / f["solve_ms"], "forward_backward_s": t_fb,
             "fd_directions": directions, "fd_step": hstep,
             "fd_max_rel_err_b": float(np.max(eb)), "fd_max_rel_err_c": float(np.max(ec))}
+
+
+def varcoef_case(dev, shape, a, b, tol):
+    """AAR on the variable-coefficient operator a*Lap + c(x) (c = 1 + 0.5 *
+    8 seeded Fourier modes, in [0.5, 1.5]) with the pointwise Jacobi
+    diagonal: FP64 and mixed-precision solves, true residuals and the
+    recurrence sweep's kernel time (3 algorithmic vectors per launch)."""
+    import torch
+    import aar_hip
+    from aar_hip import problems as P
+    c = P.helmholtz_coefficient_torch(shape, 1010, dev)
+    x = torch.zeros_like(b)
+    out = {"operator": "a*Lap_FD12 + c(x), c = 1 + 0.5*(8 seeded Fourier modes), Jacobi diag a*3*c0 + c(x)",
+           "rhs": "the headline RHS"}
+    for label, mixed in (("fp64", False), ("mixed", True)):
+        x.zero_()
+        aar_hip.AAR_vc(x, b, shape, a, c, 0.6, 0.6, 7, 6, 20000, tol, mixed=mixed)    # warm-up
+        x.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = aar_hip.AAR_vc(x, b, shape, a, c, 0.6, 0.6, 7, 6, 20000, tol, mixed=mixed)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        r = b - aar_hip.Lap_Vec_mult_vc(x, shape, a, c)
+        rel = float(torch.linalg.norm(r) / torch.linalg.norm(b))
+        x.zero_()
+        aar_hip.profile(True)
+        aar_hip.profile_reset()
+        aar_hip.AAR_vc(x, b, shape, a, c, 0.6, 0.6, 7, 6, 20000, tol, mixed=mixed)
+        aar_hip.profile(False)
+        kr = aar_hip.profile_query("stencil_aar_rec")
+        out[label] = {"solves_per_s": 1.0 / dt, "seconds": dt, "iterations": st["iterations"],
+                      "converged": bool(st["converged"]), "relres_true": rel,
+                      "algorithmic_GBps": st["bytes"] / st["solve_ms"] / 1e6,
+                      "stencil_aar_rec_vc": {"avg_ms": kr["ms"] / kr["count"],
+                                             "GBps": kr["bytes"] / kr["ms"] / 1e6}}
+        log(f"varcoef {label}: {st['iterations']} its {dt * 1e3:.0f} ms relres {rel:.2e}")
+    out["mixed_speedup_vs_fp64"] = out["fp64"]["seconds"] / out["mixed"]["seconds"]
+    del c, x
+    return out
 
 
 def batch_case(dev, seed=1004):
@@ -364,6 +405,14 @@ def main():
                         "algorithmic_GBps": st["bytes"] / st["solve_ms"] / 1e6}
         log(f"{name}: {st['iterations']} its {st['solve_ms']:.0f} ms")
 
+    # the variable-coefficient operator on the same grid and RHS
+    vc = None
+    if not args.no_vc:
+        try:
+            vc = varcoef_case(dev, shape, a, b, args.tol)
+        except Exception as e:  # noqa: BLE001
+            vc = {"error": str(e)}
+
     # differentiable complex Helmholtz (-Lap + c(x) + i eta) u = b, h = 0.25,
     # eta = 0.5: forward + adjoint solves through the autograd harness, and
     # the gradients against central differences (step 1e-6) on 20 directions
@@ -463,6 +512,7 @@ def main():
             "kernels": ktab(kern_mx),
         },
         "other_solvers": others,
+        "variable_coefficient": vc,
         "helmholtz": helm,
         "batch": batch,
         "matvec": {"GBps": mv_gbs, "frac_of_peak": mv_gbs / HBM_PEAK_GBS,

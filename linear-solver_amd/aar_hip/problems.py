@@ -165,6 +165,26 @@ def helmholtz_coefficient(shape, seed, modes=8, stream=100):
     return (1.0 + 0.5 * acc).reshape(-1)
 
 
+def helmholtz_coefficient_torch(shape, seed, device, modes=8, stream=100):
+    """helmholtz_coefficient evaluated on the device (float64; the same
+    modes, amplitudes and phases, cosines from the device's libm), for
+    grids where the host evaluation is slow."""
+    import torch
+    nx, ny, nz = shape
+    u = uniform01(5 * modes, seed, stream)
+    k = torch.arange(nz, dtype=torch.float64, device=device)[:, None, None] / nz
+    j = torch.arange(ny, dtype=torch.float64, device=device)[None, :, None] / ny
+    i = torch.arange(nx, dtype=torch.float64, device=device)[None, None, :] / nx
+    acc = torch.zeros((nz, ny, nx), dtype=torch.float64, device=device)
+    for q in range(modes):
+        kx, ky, kz = (1 + int(u[5 * q + r] * 4) for r in range(3))
+        amp = 2.0 * float(u[5 * q + 3]) - 1.0
+        ph = 2.0 * math.pi * float(u[5 * q + 4])
+        acc += amp * torch.cos(2.0 * math.pi * (kx * i + ky * j + kz * k) + ph)
+    acc /= acc.abs().max()
+    return (1.0 + 0.5 * acc).reshape(-1)
+
+
 # Parameter sets of the measured configurations (FD12, m=7, p=6 unless said).
 C1 = dict(shape=(48, 48, 48), h=0.5, omega=0.6, beta=0.6, m=7, p=6, tol=1e-6, max_iter=1000,
           seed=1001)

@@ -505,3 +505,13 @@ def test_vc_real_oracle_vs_complex_oracle(oracle):
     assert d["converged"] and dc["converged"]
     assert np.abs(u.imag).max() == 0.0
     assert np.linalg.norm(u.real - x) / np.linalg.norm(x) < 1e-9
+
+
+def test_device_coefficient_generator_matches_host():
+    """problems.helmholtz_coefficient_torch (the bench's 512^3 coefficient)
+    is the host generator's formula."""
+    import torch
+    shape = (20, 12, 16)
+    h = P.helmholtz_coefficient(shape, 1010)
+    d = P.helmholtz_coefficient_torch(shape, 1010, "cpu").numpy()
+    assert np.allclose(d, h, rtol=0, atol=1e-13)
