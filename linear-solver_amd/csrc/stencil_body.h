@@ -86,6 +86,11 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
     constexpr int NYH = (2 * R + WY - 1) / WY;        // y-halo rows per thread row
     constexpr int NXH = (2 * R * TY + NT - 1) / NT;   // x-halo cells per thread
     constexpr int QN = 2 * R + 1;
+    // unconditional stores and an exact memory wait before the plane loop:
+    // measured faster for these sweeps, slower for the others (which keep
+    // fewer live registers with masked stores)
+    constexpr bool USTORE = MODE == SMode::MATVEC || MODE == SMode::MATVEC_RES || MODE == SMode::AAR_TRUE ||
+                            MODE == SMode::RESID || MODE == SMode::CG_RESID;
     constexpr bool SA = ADDR >= 0 ? ADDR == 1
                                   : MODE == SMode::AAR_REC ||
                                         (sizeof(T) == 8 && (MODE == SMode::MATVEC || MODE == SMode::CG_Q));
@@ -366,16 +371,23 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
                 if (MODE != SMode::AAR_REC && a.sigma != 0.0) y = A::add(y, A::mul(sig, u));
                 // y + c(x) x, the order of the screened term above
                 if constexpr (VC && MODE != SMode::AAR_REC) y = A::add(y, A::mul(cur.vv[p], u));
-                // points past the grid edge (partial tiles) computed on wrapped
-                // inputs: only their stores and norm terms are masked, so the
-                // stencil itself runs without branches
+                // Points past the grid edge (partial tiles) are computed on
+                // wrapped inputs.  Such a lane reads exactly what the lane
+                // owning the wrapped point reads, through the same operations,
+                // so it may store the same bits to the same address.  With
+                // USTORE the stores are unconditional (no branch per point, so
+                // the memory-counter waits before the next plane stay exact);
+                // otherwise they are masked.  Norm terms are always masked.
                 const bool ok = c_ok[p];
                 const unsigned g = c_off[p];
+                auto put = [&](auto *base, auto v) {
+                    if (USTORE || ok) st_pl<SA>(base, zoff, g, v);
+                };
                 if constexpr (MODE == SMode::MATVEC) {
-                    if (ok) st_pl<SA>(a.out0, zoff, g, y);
+                    put(a.out0, y);
                 } else if constexpr (MODE == SMode::MATVEC_RES) {
                     // y = A x and ||b - A x||^2 (PGR.c:79-81 start-up)
-                    if (ok) st_pl<SA>(a.out0, zoff, g, y);
+                    put(a.out0, y);
                     acc[0] += ok ? A::sq(A::sub(cur.en[p][0], y)) : 0.0;
                 } else if constexpr (MODE == SMode::AAR_TRUE) {
                     // AAR.c:80-82/93-94/119-123: f = (rhs - A x)/diag, stored in
@@ -388,14 +400,12 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
                     } else {
                         if (a.precond) f = A::div(f, dg);
                     }
-                    if (ok) {
-                        if (a.out32) st_pl<SA>(a.out32, zoff, g, (float)f);
-                        else st_pl<SA>(a.out0, zoff, g, f);
-                    }
+                    if (a.out32) put(a.out32, (float)f);
+                    else put(a.out0, f);
                 } else if constexpr (MODE == SMode::AAR_REC) {
                     // y = g_{k+1} = (rhs - A(x_k + omega g_k))/diag; scale^2 *
                     // ||g_{k+1}||^2 = ||rhs - A x_{k+1}||^2 for the history
-                    if (ok) st_pl<SA>(a.out0, zoff, g, y);
+                    put(a.out0, y);
                     if (a.trace) {
                         // ||rhs - A x||: D g' pointwise (VC: D = omega/h, by
                         // the hardware reciprocal -- the trace is a record,
@@ -409,33 +419,25 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
                 } else if constexpr (MODE == SMode::RESID) {
                     const T r = A::sub(cur.en[p][0], y);
                     acc[0] += ok ? A::sq(r) : 0.0;
-                    if (ok) {
-                        if (a.out32) st_pl<SA>(a.out32, zoff, g, (float)r);
-                        if (a.out0) st_pl<SA>(a.out0, zoff, g, r);
-                    }
+                    if (a.out32) put(a.out32, (float)r);
+                    if (a.out0) put(a.out0, r);
                 } else if constexpr (MODE == SMode::PGR_RICH) {
                     // PGR.c:95-103: x_old=x; x += omega*f; Ax = A x; f = rhs - Ax
-                    if (ok) {
-                        st_pl<SA>(a.out0, zoff, g, u);
-                        st_pl<SA>(a.out1, zoff, g, y);
-                    }
+                    put(a.out0, u);
+                    put(a.out1, y);
                     if (a.trace) acc[0] += ok ? A::sq(A::sub(cur.en[p][0], y)) : 0.0;
                 } else if constexpr (MODE == SMode::CG_Q) {
                     // CG.c:94-100: q = A d; alpha partial d.q
-                    if (ok) {
-                        st_pl<SA>(a.out0, zoff, g, u);
-                        st_pl<SA>(a.out1, zoff, g, y);
-                    }
+                    put(a.out0, u);
+                    put(a.out1, y);
                     acc[0] += ok ? (double)u * (double)y : 0.0;
                 } else if constexpr (MODE == SMode::CG_RESID) {
                     // CG.c:65-84 / 112-119: r = b - A x; s = r/diag; r.s, r.r
                     const T r = A::sub(cur.en[p][0], y);
                     T s = r;
                     if (a.precond) s = A::div(s, dg);
-                    if (ok) {
-                        st_pl<SA>(a.out0, zoff, g, r);
-                        if (a.out1) st_pl<SA>(a.out1, zoff, g, s);
-                    }
+                    put(a.out0, r);
+                    if (a.out1) put(a.out1, s);
                     acc[0] += ok ? (double)r * (double)s : 0.0;
                     acc[1] += ok ? A::sq(r) : 0.0;
                 }
@@ -452,6 +454,7 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
     int z = z0;
     if constexpr (PD == 1) {
         prefetch(z0, s0);
+        if constexpr (USTORE) __builtin_amdgcn_s_waitcnt(0);
         for (; z + 1 < z1; z += 2) {
             plane(z, s0, s1);
             plane(z + 1, s1, s0);
@@ -461,6 +464,9 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
         static_assert(PD == 2, "prefetch distance 1 or 2");
         prefetch(z0, s0);
         prefetch(z0 + 1, s1);
+        // the prologue's loads complete here, so the loop's memory waits are
+        // set by its own steady state rather than by the prologue's order
+        if constexpr (USTORE) __builtin_amdgcn_s_waitcnt(0);
         for (; z + 2 < z1; z += 3) {
             plane(z, s0, s2);
             plane(z + 1, s1, s0);
