@@ -90,9 +90,16 @@ int aar_hip_matvec_vc(const aar_hip_grid *g, const double *c, const double *x, d
 int aar_hip_aar_vc(const aar_hip_grid *g, const double *c, double *x, const double *b, double omega,
                    double beta, int m, int p, int max_iter, double tol, int precond, int mixed,
                    aar_hip_stats *st);
+/* PGR.c:35-173 (l2 = 0) / PL2R.c:35-172 (l2 = 1) and CG.c:33-175 on A(c), with
+ * the same pointwise Jacobi step. */
+int aar_hip_pgr_vc(const aar_hip_grid *g, const double *c, double *x, const double *b, double omega, int m,
+                   int p, int max_iter, double tol, int precond, int l2, aar_hip_stats *st);
+int aar_hip_cg_vc(const aar_hip_grid *g, const double *c, double *x, const double *b, int max_iter, double tol,
+                  int precond, aar_hip_stats *st);
 /* Registers c(x) -- a HOST array of n doubles, copied to the GPU -- for the
  * drop-in slots Lap_Vec_mult_vc / Precondition_vc (system.h), which the
- * reference's AAR() then runs on the GPU; c = NULL clears it.  Kept until
+ * reference's AAR(), PGR(), PL2R() and CG() then run on the GPU; c = NULL
+ * clears it.  Kept until
  * cleared, aar_hip_release() or Deallocate_memory(). */
 int aar_hip_set_coefficient(const double *c, long n);
 

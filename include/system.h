@@ -105,8 +105,8 @@ void Precondition(double diag, double *res, int Np);
  * a*Lap + c(x) and its pointwise Jacobi step res /= (diag + c(x)), with the
  * c(x) registered by aar_hip_set_coefficient (aar_hip.h).  Same signatures
  * as Lap_Vec_mult / Precondition (MPI_REAL/system.h:82-84), so a caller
- * passes them to AAR() in their place (README:25's callback slot); AAR()
- * then solves A(c) x = rhs on the GPU.  PGR/PL2R/CG refuse them. */
+ * passes them to AAR(), PGR(), PL2R() or CG() in their place (README:25's
+ * callback slot), which then solve A(c) x = rhs on the GPU. */
 void Lap_Vec_mult_vc(POISSON *system, double a, double *phi, double *Lap_phi, MPI_Comm comm_laplacian);
 void Precondition_vc(double diag, double *res, int Np);
 

@@ -172,35 +172,48 @@ def AAR_batch(instances, omega, beta, m, p, max_iter, tol, precond=True, mixed=F
     return [s.as_dict() for s in sts]
 
 
-def PGR(x, rhs, shape, a, omega, m, p, max_iter, tol, sigma=0.0, precond=True, _l2=False):
-    """PGR.c:35-173."""
+def PGR(x, rhs, shape, a, omega, m, p, max_iter, tol, sigma=0.0, precond=True, _l2=False, c=None):
+    """PGR.c:35-173.  c: a float64 coefficient field (one value per point)
+    for the variable-coefficient operator a*Lap + c(x) (then sigma must be 0)."""
     import torch
     _check_len(x, shape, "x")
     _check_len(rhs, shape, "rhs")
     g = _grid(shape, a, sigma)
     st = Stats()
     _bind_stream()
-    check(_L().aar_hip_pgr(ctypes.byref(g), _dev(x, torch.float64, "x"),
-                           _dev(rhs, torch.float64, "rhs"), omega, m, p, max_iter, tol,
-                           int(bool(precond)), int(bool(_l2)), ctypes.byref(st)))
+    if c is None:
+        check(_L().aar_hip_pgr(ctypes.byref(g), _dev(x, torch.float64, "x"),
+                               _dev(rhs, torch.float64, "rhs"), omega, m, p, max_iter, tol,
+                               int(bool(precond)), int(bool(_l2)), ctypes.byref(st)))
+    else:
+        _check_len(c, shape, "c")
+        check(_L().aar_hip_pgr_vc(ctypes.byref(g), _dev(c, torch.float64, "c"), _dev(x, torch.float64, "x"),
+                                  _dev(rhs, torch.float64, "rhs"), omega, m, p, max_iter, tol,
+                                  int(bool(precond)), int(bool(_l2)), ctypes.byref(st)))
     return st.as_dict()
 
 
-def PL2R(x, rhs, shape, a, omega, m, p, max_iter, tol, sigma=0.0, precond=True):
-    """PL2R.c:35-172."""
-    return PGR(x, rhs, shape, a, omega, m, p, max_iter, tol, sigma, precond, _l2=True)
+def PL2R(x, rhs, shape, a, omega, m, p, max_iter, tol, sigma=0.0, precond=True, c=None):
+    """PL2R.c:35-172 (c as in PGR)."""
+    return PGR(x, rhs, shape, a, omega, m, p, max_iter, tol, sigma, precond, _l2=True, c=c)
 
 
-def CG(x, b, shape, a, max_iter, tol, sigma=0.0, precond=True):
-    """CG.c:33-175."""
+def CG(x, b, shape, a, max_iter, tol, sigma=0.0, precond=True, c=None):
+    """CG.c:33-175 (c as in PGR)."""
     import torch
     _check_len(x, shape, "x")
     _check_len(b, shape, "b")
     g = _grid(shape, a, sigma)
     st = Stats()
     _bind_stream()
-    check(_L().aar_hip_cg(ctypes.byref(g), _dev(x, torch.float64, "x"), _dev(b, torch.float64, "b"),
-                          max_iter, tol, int(bool(precond)), ctypes.byref(st)))
+    if c is None:
+        check(_L().aar_hip_cg(ctypes.byref(g), _dev(x, torch.float64, "x"), _dev(b, torch.float64, "b"),
+                              max_iter, tol, int(bool(precond)), ctypes.byref(st)))
+    else:
+        _check_len(c, shape, "c")
+        check(_L().aar_hip_cg_vc(ctypes.byref(g), _dev(c, torch.float64, "c"), _dev(x, torch.float64, "x"),
+                                 _dev(b, torch.float64, "b"), max_iter, tol, int(bool(precond)),
+                                 ctypes.byref(st)))
     return st.as_dict()
 
 

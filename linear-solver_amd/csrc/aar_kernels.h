@@ -203,6 +203,7 @@ struct CgUpdArgs {
     double *x = nullptr, *r = nullptr;
     const double *d = nullptr, *q = nullptr;
     double diag = 1.0;
+    const double *c = nullptr;  // variable coefficient: divisor diag + c(x)
     int precond = 1;
     int update_r = 1;          // 0 on restart iterations (r recomputed by a stencil pass)
     int nblocks = 0;

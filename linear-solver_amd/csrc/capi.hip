@@ -789,6 +789,22 @@ int aar_hip_aar_vc(const aar_hip_grid *g, const double *c, double *x, const doub
     return aar_solve<double>(op, x, b, omega, beta, m, p, max_iter, tol, precond, 0, st, 1);
 }
 
+int aar_hip_pgr_vc(const aar_hip_grid *g, const double *c, double *x, const double *b, double omega, int m, int p,
+                   int max_iter, double tol, int precond, int l2, aar_hip_stats *st)
+{
+    Op op;
+    if (int rc = make_op_vc(g, c, op)) return rc;
+    return pgr_solve(op, x, b, omega, m, p, max_iter, tol, precond, l2, st);
+}
+
+int aar_hip_cg_vc(const aar_hip_grid *g, const double *c, double *x, const double *b, int max_iter, double tol,
+                  int precond, aar_hip_stats *st)
+{
+    Op op;
+    if (int rc = make_op_vc(g, c, op)) return rc;
+    return cg_solve(op, x, b, max_iter, tol, precond, st);
+}
+
 int aar_hip_matvec_f32(const aar_hip_grid *g, const float *x, float *y)
 {
     Op op;

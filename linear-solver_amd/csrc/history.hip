@@ -996,7 +996,7 @@ __global__ __launch_bounds__(BLOCK) void cg_update_kernel(const CgUpdArgs a)
             const double r = a.r[i] - alpha * a.q[i];   // CG.c:121-124
             a.r[i] = r;
             double s = r;
-            if (a.precond) s = s / a.diag;
+            if (a.precond) s = s / (a.c ? a.diag + a.c[i] : a.diag);
             acc[0] += r * s;
             acc[1] += r * r;
         }

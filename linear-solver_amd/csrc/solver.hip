@@ -767,8 +767,6 @@ template int aar_solve<float>(const Op &, float *, const float *, double, double
 int pgr_solve(const Op &op, double *x, const double *b, double omega, int m, int p, int max_iter,
               double tol, int precond, int l2, aar_hip_stats *st)
 {
-    if (op.c)
-        return fail(AAR_HIP_ERR_ARG, "the variable-coefficient operator is built for AAR (FP64, mixed) and the matvec");
     Context &C = ctx();
     if (int rc = context_init()) return rc;
     if (!x || !b) return fail(AAR_HIP_ERR_ARG, "null vector");
@@ -790,6 +788,7 @@ int pgr_solve(const Op &op, double *x, const double *b, double omega, int m, int
     unsigned *cnt = C.counters;
     auto slot = [&](long t) { return (int)(t % ns); };
     const double s_b = 8.0 * n;
+    const double vcb = op.c ? s_b : 0.0;    // the coefficient field's reads
     C.bytes_acc = 0.0;
 
     HC(launch_zero_ctrl(ctrl, C.stream));
@@ -822,7 +821,7 @@ int pgr_solve(const Op &op, double *x, const double *b, double omega, int m, int
         a.fin.check = 0;
         a.fin.hist = hist;
         a.fin.hist_cap = max_iter + 4;
-        if (int rc = run_stencil<double>(SMode::MATVEC_RES, a, s_b * 3)) return rc;
+        if (int rc = run_stencil<double>(SMode::MATVEC_RES, a, s_b * 3 + vcb)) return rc;
     }
     int cyc = 0;
     int t = 1;
@@ -846,7 +845,7 @@ int pgr_solve(const Op &op, double *x, const double *b, double omega, int m, int
         a.fin.check = 0;
         a.fin.hist = hist;
         a.fin.hist_cap = max_iter + 4;
-        if (int rc = run_stencil<double>(SMode::PGR_RICH, a, s_b * 5)) return rc;
+        if (int rc = run_stencil<double>(SMode::PGR_RICH, a, s_b * 5 + 2 * vcb)) return rc;
         if (t % p == 0) {                                           // PGR.c:111
             // history by age relative to the newest iterate x_{t+1}: the
             // column of age k belongs to iteration t-k (PGR.c:105-109)
@@ -926,8 +925,6 @@ int pgr_solve(const Op &op, double *x, const double *b, double omega, int m, int
 int cg_solve(const Op &op, double *x, const double *b, int max_iter, double tol, int precond,
              aar_hip_stats *st)
 {
-    if (op.c)
-        return fail(AAR_HIP_ERR_ARG, "the variable-coefficient operator is built for AAR (FP64, mixed) and the matvec");
     Context &C = ctx();
     if (int rc = context_init()) return rc;
     if (!x || !b) return fail(AAR_HIP_ERR_ARG, "null vector");
@@ -945,6 +942,7 @@ int cg_solve(const Op &op, double *x, const double *b, int max_iter, double tol,
     Ctrl *ctrl = C.ctrl;
     unsigned *cnt = C.counters;
     const double s_b = 8.0 * n;
+    const double vcb = op.c ? s_b : 0.0;    // the coefficient field's reads
     C.bytes_acc = 0.0;
 
     HC(launch_zero_ctrl(ctrl, C.stream));
@@ -976,7 +974,7 @@ int cg_solve(const Op &op, double *x, const double *b, int max_iter, double tol,
         a.fin.it = 0;
         a.fin.hist = hist;
         a.fin.hist_cap = max_iter + 4;
-        if (int rc = run_stencil<double>(SMode::CG_RESID, a, s_b * 4)) return rc;
+        if (int rc = run_stencil<double>(SMode::CG_RESID, a, s_b * 4 + vcb)) return rc;
     }
     int k = 0;
     bool stopped = false;
@@ -996,7 +994,7 @@ int cg_solve(const Op &op, double *x, const double *b, int max_iter, double tol,
             a.counter = cnt + 2;
             a.ctrl = ctrl;
             a.fin.fin = FIN_CG_ALPHA;
-            if (int rc = run_stencil<double>(SMode::CG_Q, a, s_b * 4)) return rc;
+            if (int rc = run_stencil<double>(SMode::CG_Q, a, s_b * 4 + 2 * vcb)) return rc;
         }
         const bool restart = (k % 50) == 0;                          // CG.c:109
         {
@@ -1007,6 +1005,7 @@ int cg_solve(const Op &op, double *x, const double *b, int max_iter, double tol,
             u.d = dd[cur];
             u.q = q;
             u.diag = op.diag;
+            u.c = op.c;
             u.precond = precond;
             u.update_r = restart ? 0 : 1;
             u.nblocks = nb;
@@ -1019,8 +1018,8 @@ int cg_solve(const Op &op, double *x, const double *b, int max_iter, double tol,
             u.fin.hist_cap = max_iter + 4;
             prof_begin("cg_update");
             HC(run_cg_update(u));
-            prof_end(s_b * (restart ? 3 : 6));
-            C.bytes_acc += s_b * (restart ? 3 : 6);
+            prof_end(s_b * (restart ? 3 : 6) + (restart ? 0 : vcb));
+            C.bytes_acc += s_b * (restart ? 3 : 6) + (restart ? 0 : vcb);
         }
         if (restart) {
             SArgs<double> a = base_sargs<double>(op);
@@ -1035,7 +1034,7 @@ int cg_solve(const Op &op, double *x, const double *b, int max_iter, double tol,
             a.fin.it = k + 1;
             a.fin.hist = hist;
             a.fin.hist_cap = max_iter + 4;
-            if (int rc = run_stencil<double>(SMode::CG_RESID, a, s_b * 3)) return rc;
+            if (int rc = run_stencil<double>(SMode::CG_RESID, a, s_b * 3 + vcb)) return rc;
         }
         if ((k + 1) % 8 == 0) {
             if (int rc = poll_record(ctrl, cyc)) return rc;
@@ -1372,7 +1371,7 @@ int aar_refined(const Op &op, double *x, const double *b, double omega, double b
                 aar_hip_stats *st)
 {
     if (op.c)
-        return fail(AAR_HIP_ERR_ARG, "the variable-coefficient operator is built for AAR (FP64, mixed) and the matvec");
+        return fail(AAR_HIP_ERR_ARG, "the refined solver has no variable-coefficient form (use AAR mixed)");
     Context &C = ctx();
     if (int rc = context_init()) return rc;
     if (!x || !b) return fail(AAR_HIP_ERR_ARG, "null vector");

@@ -73,11 +73,15 @@ template <bool SA, typename T> __device__ __forceinline__ void st_pl(T *base, si
 template <typename T, int WY, int PY, SMode MODE, int ADDR = -1, bool VC = false>
 __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
 {
-    static_assert(!VC || MODE == SMode::MATVEC || MODE == SMode::AAR_TRUE || MODE == SMode::AAR_REC,
-                  "variable-coefficient sweeps: MATVEC, AAR_TRUE, AAR_REC");
+    static_assert(!VC || MODE != SMode::RESID, "no variable-coefficient RESID sweep");
     using A = Arith<T>;
     constexpr int NT = 64 * WY;                       // threads: WY wavefronts stacked in y
     constexpr int NF = ModeInfo<MODE>::NF;
+    // VC sweeps whose pre-stencil step divides by the pointwise diagonal
+    // (PGR's Richardson update, CG's preconditioned direction) load c(x) as
+    // one more field at every queue and halo point
+    constexpr bool VCF = VC && (MODE == SMode::PGR_RICH || MODE == SMode::CG_Q);
+    constexpr int NFX = NF + (VCF ? 1 : 0);
     constexpr int NE = ModeInfo<MODE>::NE;
     constexpr int NACC = ModeInfo<MODE>::NACC;
     constexpr int TY = WY * PY;
@@ -93,7 +97,7 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
                             MODE == SMode::RESID || MODE == SMode::CG_RESID;
     constexpr bool SA = ADDR >= 0 ? ADDR == 1
                                   : MODE == SMode::AAR_REC ||
-                                        (sizeof(T) == 8 && (MODE == SMode::MATVEC || MODE == SMode::CG_Q));
+                                        (sizeof(T) == 8 && (MODE == SMode::MATVEC || (MODE == SMode::CG_Q && !VC)));
 #ifndef AAR_STENCIL_PD64
 #define AAR_STENCIL_PD64 2
 #endif
@@ -175,23 +179,29 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
     const T cgb = (MODE == SMode::CG_Q && a.cg_beta) ? (T)(*a.cg_beta) : (T)0;
 
     // fields at in-plane offset `off` of the plane starting at element zo
-    auto load_raw = [&](size_t zo, unsigned off, T (&v)[NF]) {
+    auto load_raw = [&](size_t zo, unsigned off, T (&v)[NFX]) {
         v[0] = ld_pl<SA>(a.in0, zo, off);
         if constexpr (NF > 1) v[1] = ld_pl<SA>(a.in1, zo, off);
         if constexpr (NF > 2) v[2] = ld_pl<SA>(a.in2, zo, off);
+        if constexpr (VCF) v[NF] = ld_pl<SA>(a.vf, zo, off);
+    };
+    // the Jacobi divisor at a point: diag, or diag + c(x) (VCF)
+    auto dv = [&](const T (&v)[NFX]) -> T {
+        if constexpr (VCF) return A::add(dg, v[NFX - 1]);
+        else return dg;
     };
     // stencil operand u at a point from its loaded fields
-    auto make_u = [&](const T (&v)[NF]) -> T {
+    auto make_u = [&](const T (&v)[NFX]) -> T {
         if constexpr (MODE == SMode::PGR_RICH) {
             // PGR.c:92-98: f = (rhs - Ax) / diag; x_new = x + omega*f
             T f = A::sub(v[1], v[2]);
-            if (a.precond) f = A::div(f, dg);
+            if (a.precond) f = A::div(f, dv(v));
             return A::add(v[0], A::mul(om, f));
         } else if constexpr (MODE == SMode::CG_Q) {
             // CG.c:128-150: s = r/diag; d = s + beta*d  (first pass: d as given)
             if (a.cg_first) return v[1];
             T s = v[0];
-            if (a.precond) s = A::div(s, dg);
+            if (a.precond) s = A::div(s, dv(v));
             return A::add(s, A::mul(cgb, v[1]));
         } else {
             return v[0];
@@ -205,9 +215,9 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
     // PD+1 sets rotate (the plane loop is unrolled by PD+1) so that no
     // pending load is copied between registers.
     struct Pre {
-        T qn[PY][NF];
-        T hy[NYH][NF];
-        T hx[NXH][NF];
+        T qn[PY][NFX];
+        T hy[NYH][NFX];
+        T hx[NXH][NFX];
         T en[PY][NE > 0 ? NE : 1];
         T vv[PY];                                     // VC: a.vf at the plane
     };
@@ -219,7 +229,7 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
         const size_t zo = (size_t)modp(z0 - R + s, nz) * nxy;
 #pragma unroll
         for (int p = 0; p < PY; p++) {
-            T v[NF];
+            T v[NFX];
             load_raw(zo, c_off[p], v);
             q[p][s] = make_u(v);
         }
@@ -230,13 +240,13 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
         const size_t zo = (size_t)modp(z0 + s, nz) * nxy;
 #pragma unroll
         for (int k = 0; k < NYH; k++) {
-            T v[NF];
+            T v[NFX];
             load_raw(zo, yh_off[k], v);
             hring[s][k][tid] = make_u(v);
         }
 #pragma unroll
         for (int k = 0; k < NXH; k++) {
-            T v[NF];
+            T v[NFX];
             load_raw(zo, xh_off[k], v);
             hring[s][NYH + k][tid] = make_u(v);
         }
@@ -435,7 +445,11 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
                     // CG.c:65-84 / 112-119: r = b - A x; s = r/diag; r.s, r.r
                     const T r = A::sub(cur.en[p][0], y);
                     T s = r;
-                    if (a.precond) s = A::div(s, dg);
+                    if constexpr (VC) {
+                        if (a.precond) s = A::div(s, A::add(dg, cur.vv[p]));
+                    } else {
+                        if (a.precond) s = A::div(s, dg);
+                    }
                     put(a.out0, r);
                     if (a.out1) put(a.out1, s);
                     acc[0] += ok ? (double)r * (double)s : 0.0;

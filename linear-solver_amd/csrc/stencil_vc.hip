@@ -9,6 +9,9 @@
 //   MATVEC    y = a*Lap(x) + c x          (the order of the screened term)
 //   AAR_TRUE  r = b - A x; g = r / (diag + c); ||r||^2
 //   AAR_REC   g' = (1 - omega) g - h sum_o w_o S_o,  h = omega / D
+// and, for PGR/PL2R and CG, MATVEC_RES, CG_RESID (s = r / (diag + c)) and
+// PGR_RICH, CG_Q, whose pre-stencil step divides by diag + c at every
+// queue and halo point (c loaded there as one more field)
 // Kept in their own translation unit so the constant-coefficient kernels
 // compile exactly as before.
 #include "stencil_body.h"
@@ -46,6 +49,10 @@ hipError_t launch_stencil_vc(SMode m, const SArgs<T> &a, int wy, int py, hipStre
         case SMode::MATVEC: return launch_stencil_vc_t<T, SMode::MATVEC>(a, wy, py, st);
         case SMode::AAR_TRUE: return launch_stencil_vc_t<T, SMode::AAR_TRUE>(a, wy, py, st);
         case SMode::AAR_REC: return launch_stencil_vc_t<T, SMode::AAR_REC>(a, wy, py, st);
+        case SMode::MATVEC_RES: return launch_stencil_vc_t<T, SMode::MATVEC_RES>(a, wy, py, st);
+        case SMode::PGR_RICH: return launch_stencil_vc_t<T, SMode::PGR_RICH>(a, wy, py, st);
+        case SMode::CG_Q: return launch_stencil_vc_t<T, SMode::CG_Q>(a, wy, py, st);
+        case SMode::CG_RESID: return launch_stencil_vc_t<T, SMode::CG_RESID>(a, wy, py, st);
         default: return hipErrorInvalidValue;
         }
     }

@@ -396,6 +396,13 @@ void orc_matvec_vc(int nx, int ny, int nz, int fdn, const double *coeff,
     for (long i = 0; i < n; i++) y[i] = y[i] + c[i] * x[i];
 }
 
+/* the operator of the *_impl solvers: a*Lap + sigma, or a*Lap + c(x) */
+#define OPV(xin, yout)                                                      \
+    do {                                                                    \
+        if (cv) orc_matvec_vc(nx, ny, nz, fdn, coeff, a, cv, xin, yout);    \
+        else orc_matvec(nx, ny, nz, fdn, coeff, a, sigma, xin, yout);       \
+    } while (0)
+
 /* AAR.c:37-177 for the operator a*Lap + sigma (c == NULL) or a*Lap + c(x);
  * with c the Jacobi step of Precondition (system.c:222-227) divides by the
  * pointwise diagonal a*(3*coeff[0]) + c(x). */
@@ -409,11 +416,6 @@ static int aar_impl(int nx, int ny, int nz, int fdn, double a, double sigma, con
     orc_lap_coefficient(coeff, fdn);
     double diag = a * (3 * coeff[0]);
     if (sigma != 0.0) diag += sigma;
-#define AAR_OP(xin, yout)                                                   \
-    do {                                                                    \
-        if (cv) orc_matvec_vc(nx, ny, nz, fdn, coeff, a, cv, xin, yout);    \
-        else orc_matvec(nx, ny, nz, fdn, coeff, a, sigma, xin, yout);       \
-    } while (0)
 
     double *f = (double *)calloc((size_t)n, sizeof(double));
     double *x_old = (double *)calloc((size_t)n, sizeof(double));
@@ -427,7 +429,7 @@ static int aar_impl(int nx, int ny, int nz, int fdn, double a, double sigma, con
     double relres = tol + 1;                 /* AAR.c:76, before tol is scaled */
     double rhs_norm = orc_norm2(rhs, n);
     tol *= rhs_norm;
-    AAR_OP(x, f);
+    OPV(x, f);
     for (long i = 0; i < n; i++) f[i] = rhs[i] - f[i];
     if (trace && tlen < trace_cap) trace[tlen++] = orc_norm2(f, n);
 
@@ -451,13 +453,13 @@ static int aar_impl(int nx, int ny, int nz, int fdn, double a, double sigma, con
         if (iter % p == 0 && iter > 1) {
             if (orc_anderson_extrapolation(DX, DF, f, beta, m, n, am, NULL, NULL)) { status = 1; break; }
             for (long i = 0; i < n; i++) x[i] = x_old[i] + beta * f[i] - am[i];
-            AAR_OP(x, f);
+            OPV(x, f);
             for (long i = 0; i < n; i++) f[i] = rhs[i] - f[i];
             relres = orc_norm2(f, n);
             rn = relres;
         } else {
             for (long i = 0; i < n; i++) x[i] = x_old[i] + omega * f[i];
-            AAR_OP(x, f);
+            OPV(x, f);
             for (long i = 0; i < n; i++) f[i] = rhs[i] - f[i];
         }
         if (trace && tlen < trace_cap) trace[tlen++] = rn >= 0.0 ? rn : orc_norm2(f, n);
@@ -467,7 +469,6 @@ static int aar_impl(int nx, int ny, int nz, int fdn, double a, double sigma, con
     finish_stats(st, iter - 1, relres, rhs_norm, tol, iter, max_iter, tlen, t1 - t0, status);
     free(f); free(x_old); free(f_old); free(am); free(DX); free(DF);
     return status;
-#undef AAR_OP
 }
 
 int orc_aar(int nx, int ny, int nz, int fdn, double a, double sigma,
@@ -491,10 +492,12 @@ int orc_aar_vc(int nx, int ny, int nz, int fdn, double a, const double *c,
 /* ------------------------------------------------------------------ */
 /* PGR.c:35-218 and PL2R.c:35-218.                                     */
 /* ------------------------------------------------------------------ */
-int orc_pgr(int nx, int ny, int nz, int fdn, double a, double sigma,
-            double *x, const double *rhs, double omega, int m, int p,
-            int max_iter, double tol, int precond, int l2,
-            double *trace, int trace_cap, orc_stats *st)
+/* PGR.c:35-218 / PL2R.c:35-218 for a*Lap + sigma (cv == NULL) or
+ * a*Lap + c(x) with the pointwise Jacobi diagonal. */
+static int pgr_impl(int nx, int ny, int nz, int fdn, double a, double sigma, const double *cv,
+                    double *x, const double *rhs, double omega, int m, int p,
+                    int max_iter, double tol, int precond, int l2,
+                    double *trace, int trace_cap, orc_stats *st)
 {
     const long n = (long)nx * ny * nz;
     double coeff[32] = {0};
@@ -518,20 +521,22 @@ int orc_pgr(int nx, int ny, int nz, int fdn, double a, double sigma,
     double relres = tol + 1;
     double rhs_norm = orc_norm2(rhs, n);
     tol *= rhs_norm;
-    orc_matvec(nx, ny, nz, fdn, coeff, a, sigma, x, Ax);
+    OPV(x, Ax);
     for (long i = 0; i < n; i++) f[i] = rhs[i] - Ax[i];
     if (trace && tlen < trace_cap) trace[tlen++] = orc_norm2(f, n);
 
     double t0 = now_seconds();
     while (relres > tol && iter <= max_iter) {
-        if (precond)
-            for (long i = 0; i < n; i++) f[i] /= diag;
+        if (precond) {
+            if (cv) for (long i = 0; i < n; i++) f[i] /= (diag + cv[i]);
+            else for (long i = 0; i < n; i++) f[i] /= diag;
+        }
         for (long i = 0; i < n; i++) {
             x_old[i] = x[i];
             Ax_old[i] = Ax[i];
             x[i] += (omega * f[i]);
         }
-        orc_matvec(nx, ny, nz, fdn, coeff, a, sigma, x, Ax);
+        OPV(x, Ax);
         for (long i = 0; i < n; i++) f[i] = rhs[i] - Ax[i];
         if (trace && tlen < trace_cap) trace[tlen++] = orc_norm2(f, n);
 
@@ -584,12 +589,32 @@ int orc_pgr(int nx, int ny, int nz, int fdn, double a, double sigma,
     return status;
 }
 
+int orc_pgr(int nx, int ny, int nz, int fdn, double a, double sigma,
+            double *x, const double *rhs, double omega, int m, int p,
+            int max_iter, double tol, int precond, int l2,
+            double *trace, int trace_cap, orc_stats *st)
+{
+    return pgr_impl(nx, ny, nz, fdn, a, sigma, NULL, x, rhs, omega, m, p, max_iter, tol, precond, l2,
+                    trace, trace_cap, st);
+}
+
+int orc_pgr_vc(int nx, int ny, int nz, int fdn, double a, const double *c,
+               double *x, const double *rhs, double omega, int m, int p,
+               int max_iter, double tol, int precond, int l2,
+               double *trace, int trace_cap, orc_stats *st)
+{
+    return pgr_impl(nx, ny, nz, fdn, a, 0.0, c, x, rhs, omega, m, p, max_iter, tol, precond, l2,
+                    trace, trace_cap, st);
+}
+
 /* ------------------------------------------------------------------ */
 /* CG.c:33-175.                                                        */
 /* ------------------------------------------------------------------ */
-int orc_cg(int nx, int ny, int nz, int fdn, double a, double sigma,
-           double *x, const double *b, int max_iter, double tol, int precond,
-           double *trace, int trace_cap, orc_stats *st)
+/* CG.c:33-175 for a*Lap + sigma (cv == NULL) or a*Lap + c(x) with the
+ * pointwise Jacobi diagonal. */
+static int cg_impl(int nx, int ny, int nz, int fdn, double a, double sigma, const double *cv,
+                   double *x, const double *b, int max_iter, double tol, int precond,
+                   double *trace, int trace_cap, orc_stats *st)
 {
     const long n = (long)nx * ny * nz;
     double coeff[32] = {0};
@@ -603,13 +628,15 @@ int orc_cg(int nx, int ny, int nz, int fdn, double a, double sigma,
     int tlen = 0;
 
     double b_2norm = orc_norm2(b, n);
-    orc_matvec(nx, ny, nz, fdn, coeff, a, sigma, x, r);
+    OPV(x, r);
     for (long i = 0; i < n; i++) {
         r[i] = b[i] - r[i];
         d[i] = r[i];
     }
-    if (precond)
-        for (long i = 0; i < n; i++) d[i] /= diag;
+    if (precond) {
+        if (cv) for (long i = 0; i < n; i++) d[i] /= (diag + cv[i]);
+        else for (long i = 0; i < n; i++) d[i] /= diag;
+    }
     double delta_new = orc_dot(r, d, n), delta_old;
     double relres = precond ? orc_norm2(r, n) : sqrt(delta_new);
     if (trace && tlen < trace_cap) trace[tlen++] = orc_norm2(r, n);
@@ -618,12 +645,12 @@ int orc_cg(int nx, int ny, int nz, int fdn, double a, double sigma,
     int iter_count = 0;
     double t0 = now_seconds();
     while (iter_count < max_iter && relres > tol) {
-        orc_matvec(nx, ny, nz, fdn, coeff, a, sigma, d, q);
+        OPV(d, q);
         double alpha = orc_dot(d, q, n);
         alpha = delta_new / alpha;
         for (long i = 0; i < n; i++) x[i] = x[i] + alpha * d[i];
         if ((iter_count % 50) == 0) {
-            orc_matvec(nx, ny, nz, fdn, coeff, a, sigma, x, r);
+            OPV(x, r);
             for (long i = 0; i < n; i++) {
                 r[i] = b[i] - r[i];
                 s[i] = r[i];
@@ -635,8 +662,10 @@ int orc_cg(int nx, int ny, int nz, int fdn, double a, double sigma,
                 s[i] = r[i];
             }
         }
-        if (precond)
-            for (long i = 0; i < n; i++) s[i] /= diag;
+        if (precond) {
+            if (cv) for (long i = 0; i < n; i++) s[i] /= (diag + cv[i]);
+            else for (long i = 0; i < n; i++) s[i] /= diag;
+        }
         delta_old = delta_new;
         delta_new = orc_dot(r, s, n);
         relres = precond ? orc_norm2(r, n) : sqrt(delta_new);
@@ -648,6 +677,20 @@ int orc_cg(int nx, int ny, int nz, int fdn, double a, double sigma,
     finish_stats(st, iter_count, relres, b_2norm, tol, iter_count, max_iter, tlen, t1 - t0, 0);
     free(r); free(d); free(s); free(q);
     return 0;
+}
+
+int orc_cg(int nx, int ny, int nz, int fdn, double a, double sigma,
+           double *x, const double *b, int max_iter, double tol, int precond,
+           double *trace, int trace_cap, orc_stats *st)
+{
+    return cg_impl(nx, ny, nz, fdn, a, sigma, NULL, x, b, max_iter, tol, precond, trace, trace_cap, st);
+}
+
+int orc_cg_vc(int nx, int ny, int nz, int fdn, double a, const double *c,
+              double *x, const double *b, int max_iter, double tol, int precond,
+              double *trace, int trace_cap, orc_stats *st)
+{
+    return cg_impl(nx, ny, nz, fdn, a, 0.0, c, x, b, max_iter, tol, precond, trace, trace_cap, st);
 }
 
 /* ------------------------------------------------------------------ */

@@ -76,15 +76,24 @@ int orc_aar(int nx, int ny, int nz, int fdn, double a, double sigma,
 /* Variable-coefficient operator (no MPI_REAL counterpart; the operator a
  * caller would pass through the reference's Lap_Vec_mult slot, README:25):
  * y = a*Lap(x) + c(x) x, the term added after the stencil as orc_matvec
- * adds sigma*x.  orc_aar_vc is orc_aar on that operator with the Jacobi
- * diagonal a*(3*coeff[0]) + c(x) per point; with c = sigma everywhere both
- * equal orc_matvec / orc_aar with that sigma bit for bit. */
+ * adds sigma*x.  orc_aar_vc / orc_pgr_vc / orc_cg_vc are orc_aar / orc_pgr /
+ * orc_cg on that operator with the Jacobi diagonal a*(3*coeff[0]) + c(x) per
+ * point; with c = sigma everywhere each equals its screened counterpart
+ * with that sigma bit for bit. */
 void orc_matvec_vc(int nx, int ny, int nz, int fdn, const double *coeff,
                    double a, const double *c, const double *x, double *y);
 int orc_aar_vc(int nx, int ny, int nz, int fdn, double a, const double *c,
                double *x, const double *rhs, double omega, double beta,
                int m, int p, int max_iter, double tol, int precond,
                double *trace, int trace_cap, orc_stats *st);
+
+int orc_pgr_vc(int nx, int ny, int nz, int fdn, double a, const double *c,
+               double *x, const double *rhs, double omega, int m, int p,
+               int max_iter, double tol, int precond, int l2,
+               double *trace, int trace_cap, orc_stats *st);
+int orc_cg_vc(int nx, int ny, int nz, int fdn, double a, const double *c,
+              double *x, const double *b, int max_iter, double tol, int precond,
+              double *trace, int trace_cap, orc_stats *st);
 
 /* PGR.c:35-218 (l2 = 0) and PL2R.c:35-218 (l2 = 1). */
 int orc_pgr(int nx, int ny, int nz, int fdn, double a, double sigma,

@@ -88,7 +88,10 @@ def lib():
         L.orc_matvec_vc.argtypes = [I, I, I, I, _D, Dd, _D, _D, _D]
         L.orc_aar_vc.argtypes = [I, I, I, I, Dd, _D, _D, _D, Dd, Dd, I, I, I, Dd, I, _D, I,
                                  ctypes.POINTER(OrcStats)]
-        for fn in ("orc_aar", "orc_pgr", "orc_cg", "orc_aar_complex", "orc_aar_vc"):
+        L.orc_pgr_vc.argtypes = [I, I, I, I, Dd, _D, _D, _D, Dd, I, I, I, Dd, I, I, _D, I,
+                                 ctypes.POINTER(OrcStats)]
+        L.orc_cg_vc.argtypes = [I, I, I, I, Dd, _D, _D, _D, I, Dd, I, _D, I, ctypes.POINTER(OrcStats)]
+        for fn in ("orc_aar", "orc_pgr", "orc_cg", "orc_aar_complex", "orc_aar_vc", "orc_pgr_vc", "orc_cg_vc"):
             getattr(L, fn).restype = I
         _lib = L
     return _lib
@@ -186,6 +189,28 @@ def aar_vc(x0, rhs, shape, a, c, omega, beta, m, p, max_iter, tol, precond=True,
     cv = _f64(c).reshape(-1)
     d = _run(lib().orc_aar_vc, (nx, ny, nz, fdn, a, _p(cv), _p(x), _p(b), omega, beta, m, p,
                                 max_iter, tol, int(precond)), trace, max_iter + 8)
+    return x, d
+
+
+def pgr_vc(x0, rhs, shape, a, c, omega, m, p, max_iter, tol, precond=True, l2=False, fdn=6, trace=True):
+    """PGR.c / PL2R.c on a*Lap + c(x) with the pointwise Jacobi diagonal."""
+    nx, ny, nz = shape
+    x = _f64(x0).reshape(-1).copy()
+    b = _f64(rhs).reshape(-1)
+    cv = _f64(c).reshape(-1)
+    d = _run(lib().orc_pgr_vc, (nx, ny, nz, fdn, a, _p(cv), _p(x), _p(b), omega, m, p, max_iter, tol,
+                                int(precond), int(l2)), trace, max_iter + 8)
+    return x, d
+
+
+def cg_vc(x0, rhs, shape, a, c, max_iter, tol, precond=True, fdn=6, trace=True):
+    """CG.c on a*Lap + c(x) with the pointwise Jacobi diagonal."""
+    nx, ny, nz = shape
+    x = _f64(x0).reshape(-1).copy()
+    b = _f64(rhs).reshape(-1)
+    cv = _f64(c).reshape(-1)
+    d = _run(lib().orc_cg_vc, (nx, ny, nz, fdn, a, _p(cv), _p(x), _p(b), max_iter, tol, int(precond)),
+             trace, max_iter // 50 + 8)
     return x, d
 
 

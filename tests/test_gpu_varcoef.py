@@ -6,8 +6,9 @@ a*(3*coeff[0]) + c(x) (or none) gives the oracle's iteration counts
 the reference-pinned screened AAR bit for bit when c is constant and the
 dense solve when not), solutions within 1e-8 (1e-6 for an iterate stopped
 at max_iter) and residual histories within 1e-6, in FP64 and in mixed
-precision.  The drop-in slots Lap_Vec_mult_vc / Precondition_vc run through
-the reference's own AAR() signature with a registered c(x).
+precision; PGR, PL2R and CG (orc_pgr_vc, orc_cg_vc, pinned the same way)
+likewise.  The drop-in slots Lap_Vec_mult_vc / Precondition_vc run through
+the reference's own AAR(), PGR() and CG() signatures with a registered c(x).
 """
 import ctypes
 
@@ -78,6 +79,57 @@ def test_vc_aar_parity(gpu, oracle, case, mixed):
         assert np.all(np.abs(h_gpu - tr) <= 1e-6 * tr + 1e-10 * tr[0])
 
 
+PGR_CASES = [
+    # solver, shape, m, p, max_iter, tol, precond, omega
+    ("pgr", (16, 16, 16), 7, 6, 1000, 1e-8, True, 0.6),
+    ("pl2r", (16, 16, 16), 7, 6, 1000, 1e-8, True, 0.6),
+    ("pgr", (20, 12, 16), 3, 5, 1000, 1e-8, True, 0.6),
+    ("pl2r", (24, 24, 24), 7, 6, 17, 1e-8, True, 0.6),
+    ("pgr", (16, 16, 16), 7, 6, 600, 1e-6, False, 0.015),
+    ("cg", (16, 16, 16), 0, 0, 1000, 1e-8, True, 0.0),
+    ("cg", (20, 12, 16), 0, 0, 1000, 1e-10, True, 0.0),
+    ("cg", (24, 24, 24), 0, 0, 37, 1e-10, True, 0.0),
+    ("cg", (16, 16, 16), 0, 0, 1000, 1e-8, False, 0.0),
+]
+
+
+@pytest.mark.parametrize("case", PGR_CASES, ids=lambda c: f"{c[0]}_" + "x".join(map(str, c[1])) +
+                         f"_i{c[4]}" + ("" if c[6] else "_noprec"))
+def test_vc_pgr_cg_parity(gpu, oracle, case):
+    """PGR / PL2R / CG on a*Lap + c(x): the oracle's iteration counts and
+    printed conditions, solutions within 1e-8 (CG 1e-7; 1e-6 at a max_iter
+    stop) and residual traces within 1e-6."""
+    solver, shape, m, p, max_iter, tol, precond, om = case
+    say("vc", solver, shape, max_iter, precond)
+    n = int(np.prod(shape))
+    a = P.poisson_a(0.5)
+    c = P.helmholtz_coefficient(shape, 1011)
+    b = P.rhs_normal(shape, 1011)
+    xd = dev(np.zeros(n))
+    if solver == "cg":
+        xr, dr = oracle.cg_vc(np.zeros(n), b, shape, a, c, max_iter, tol, precond=precond)
+        st = gpu.CG(xd, dev(b), shape, a, max_iter, tol, precond=precond, c=dev(c))
+    else:
+        l2 = solver == "pl2r"
+        xr, dr = oracle.pgr_vc(np.zeros(n), b, shape, a, c, om, m, p, max_iter, tol, precond=precond, l2=l2)
+        fn = gpu.PL2R if l2 else gpu.PGR
+        st = fn(xd, dev(b), shape, a, om, m, p, max_iter, tol, precond=precond, c=dev(c))
+    assert st["iterations"] == dr["iterations"]
+    assert bool(st["printed_converged"]) == bool(dr["printed_converged"])
+    x = xd.cpu().numpy()
+    tol_x = (1e-7 if solver == "cg" else 1e-8) if dr["printed_converged"] else 1e-6
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) <= tol_x
+    h_gpu = gpu.history()
+    tr = dr["trace"]
+    if solver == "cg":
+        # CG's trace holds the explicit residuals: the start and every 50th
+        # iteration (CG.c:109); the history has one entry per iteration
+        h_gpu = np.concatenate([h_gpu[:1], h_gpu[1::50]])
+    k = min(len(h_gpu), len(tr))
+    assert k == len(tr)
+    assert np.all(np.abs(h_gpu[:k] - tr) <= 1e-6 * tr + 1e-10 * tr[0])
+
+
 def test_vc_constant_field_matches_screened_solve(gpu):
     """c = sigma everywhere: the same iteration count and solution (to the
     rounding of the two recurrence forms) as the screened-operator solver."""
@@ -130,10 +182,10 @@ def _system(shape, coeff):
 
 
 def test_vc_dropin_slots(gpu, oracle):
-    """The reference's AAR() entry with Lap_Vec_mult_vc / Precondition_vc and
-    a registered c(x): the oracle's iteration count and solution; the host
-    slots themselves are bit-exact; mismatched or unsupported uses are
-    refused without touching x."""
+    """The reference's AAR(), PGR() and CG() entries with Lap_Vec_mult_vc /
+    Precondition_vc and a registered c(x): the oracle's iteration counts and
+    solutions; the host slots themselves are bit-exact; a mismatched pair or
+    a missing registration is refused without touching x."""
     from aar_hip import _lib
     L = _lib.load()
     fp = _dropin(L)
@@ -164,14 +216,29 @@ def test_vc_dropin_slots(gpu, oracle):
     xr, dr = oracle.aar_vc(np.zeros(n), b, shape, a, c, 0.6, 0.6, 7, 6, 1000, 1e-8)
     assert st.iterations == dr["iterations"]
     assert np.linalg.norm(x - xr) / np.linalg.norm(xr) <= 1e-8
+    # PGR() and CG() through the reference signatures
+    xp = np.zeros(n)
+    L.PGR(ctypes.byref(sysd), fp["Lap_Vec_mult_vc"], fp["Precondition_vc"], a, xp.ctypes.data, bb.ctypes.data,
+          0.6, 7, 6, 1000, 1e-8, n, 0)
+    L.aar_hip_last_stats(ctypes.byref(st))
+    xr, dr = oracle.pgr_vc(np.zeros(n), b, shape, a, c, 0.6, 7, 6, 1000, 1e-8)
+    assert st.iterations == dr["iterations"]
+    assert np.linalg.norm(xp - xr) / np.linalg.norm(xr) <= 1e-8
+    L.CG.argtypes = [ctypes.POINTER(POISSON), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p,
+                     ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+    L.CG.restype = None
+    xc = np.zeros(n)
+    L.CG(ctypes.byref(sysd), fp["Lap_Vec_mult_vc"], fp["Precondition_vc"], a, xc.ctypes.data, bb.ctypes.data,
+         1000, 1e-8, n, 0)
+    L.aar_hip_last_stats(ctypes.byref(st))
+    xr, dr = oracle.cg_vc(np.zeros(n), b, shape, a, c, 1000, 1e-8)
+    assert st.iterations == dr["iterations"]
+    assert np.linalg.norm(xc - xr) / np.linalg.norm(xr) <= 1e-7
     # the variable-coefficient operator with the constant-diagonal
-    # preconditioner, and PGR on it, are refused; x is left alone
+    # preconditioner is refused; x is left alone
     x2 = np.zeros(n)
     L.AAR(ctypes.byref(sysd), fp["Lap_Vec_mult_vc"], fp["Precondition"], a, x2.ctypes.data, bb.ctypes.data,
           0.6, 0.6, 7, 6, 1000, 1e-8, n, 0)
-    assert not x2.any()
-    L.PGR(ctypes.byref(sysd), fp["Lap_Vec_mult_vc"], fp["Precondition_vc"], a, x2.ctypes.data, bb.ctypes.data,
-          0.6, 7, 6, 1000, 1e-8, n, 0)
     assert not x2.any()
     gpu.set_coefficient(None)
     L.AAR(ctypes.byref(sysd), fp["Lap_Vec_mult_vc"], fp["Precondition_vc"], a, x2.ctypes.data, bb.ctypes.data,

@@ -515,3 +515,44 @@ def test_device_coefficient_generator_matches_host():
     h = P.helmholtz_coefficient(shape, 1010)
     d = P.helmholtz_coefficient_torch(shape, 1010, "cpu").numpy()
     assert np.allclose(d, h, rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("solver", ["pgr", "pl2r", "cg"])
+def test_vc_pgr_cg_constant_field_is_the_screened_oracle(oracle, solver):
+    """orc_pgr_vc / orc_cg_vc with c = sigma everywhere equal the
+    reference-pinned screened orc_pgr / orc_cg bit for bit."""
+    shape = (16, 12, 10)
+    n = int(np.prod(shape))
+    a, sigma = P.poisson_a(0.5), 0.75
+    c = np.full(n, sigma)
+    b = P.rhs_normal(shape, 84)
+    for precond, om in ((True, 0.6), (False, 0.015)):
+        if solver == "cg":
+            xs, ds = oracle.cg(np.zeros(n), b, shape, a, 500, 1e-9, sigma=sigma, precond=precond)
+            xv, dv = oracle.cg_vc(np.zeros(n), b, shape, a, c, 500, 1e-9, precond=precond)
+        else:
+            l2 = solver == "pl2r"
+            xs, ds = oracle.pgr(np.zeros(n), b, shape, a, om, 7, 6, 400, 1e-9, sigma=sigma, precond=precond, l2=l2)
+            xv, dv = oracle.pgr_vc(np.zeros(n), b, shape, a, c, om, 7, 6, 400, 1e-9, precond=precond, l2=l2)
+        assert dv["iterations"] == ds["iterations"]
+        assert np.array_equal(xv, xs)
+        assert np.array_equal(dv["trace"], ds["trace"])
+
+
+@pytest.mark.parametrize("solver", ["pgr", "pl2r", "cg"])
+def test_vc_pgr_cg_solution_dense(oracle, solver):
+    """orc_pgr_vc / orc_cg_vc converged solutions against numpy's dense solve
+    (9 x 8 x 7 grid, c(x) in [0.5, 1.5])."""
+    shape = (9, 8, 7)
+    n = int(np.prod(shape))
+    a = P.poisson_a(0.5)
+    c = P.helmholtz_coefficient(shape, 12)
+    M = dense_operator(shape, a, oracle.lap_coefficient(6)) + np.diag(c)
+    b = P.rhs_normal(shape, 85)
+    xd = np.linalg.solve(M, b)
+    if solver == "cg":
+        x, d = oracle.cg_vc(np.zeros(n), b, shape, a, c, 3000, 1e-12)
+    else:
+        x, d = oracle.pgr_vc(np.zeros(n), b, shape, a, c, 0.6, 7, 6, 3000, 1e-12, l2=solver == "pl2r")
+    assert d["converged"]
+    assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-9
