@@ -98,14 +98,11 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
     constexpr bool SA = ADDR >= 0 ? ADDR == 1
                                   : MODE == SMode::AAR_REC ||
                                         (sizeof(T) == 8 && (MODE == SMode::MATVEC || (MODE == SMode::CG_Q && !VC)));
-#ifndef AAR_STENCIL_PD64
-#define AAR_STENCIL_PD64 2
-#endif
     // prefetch distance (planes): 2 for the FP64 sweeps without epilogue
     // inputs, whose one block per CU (LDS-limited) leaves VGPRs to spare; 1
     // where a third register set would cost a block per CU (FP32), spill
     // (two or three fields) or measured slower (epilogue inputs)
-    constexpr int PD = (sizeof(T) == 8 && NF == 1 && NE == 0 && !VC) ? AAR_STENCIL_PD64 : 1;
+    constexpr int PD = (sizeof(T) == 8 && NF == 1 && NE == 0 && !VC) ? 2 : 1;
 
     constexpr int HR = R + 1;                         // halo ring depth (planes)
     constexpr int NH = NYH + NXH;                     // halo cells per thread
