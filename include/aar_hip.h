@@ -60,7 +60,8 @@ int aar_hip_matvec(const aar_hip_grid *g, const double *x, double *y);
 /* same in FP32 (the arithmetic of the refined solver's inner sweeps) */
 int aar_hip_matvec_f32(const aar_hip_grid *g, const float *x, float *y);
 
-/* AAR.c:37-177 -- x: initial guess in, solution out */
+/* AAR.c:37-177 -- x: initial guess in, solution out; history m <= 16 (also
+ * for PGR/PL2R and the stand-alone Anderson/Galerkin/L2 steps) */
 int aar_hip_aar(const aar_hip_grid *g, double *x, const double *b, double omega, double beta,
                 int m, int p, int max_iter, double tol, int precond, aar_hip_stats *st);
 

@@ -14,7 +14,9 @@ namespace aar {
 constexpr int R = 6;                 // stencil radius: FD12 (FDn = 6, MPI_REAL/main.c:35)
 constexpr int TX = 64;               // tile width in x: one wavefront
 constexpr int BLOCK = 256;           // threads per block of the streaming (history/dot) kernels
-constexpr int MAXM = 8;              // largest Anderson history the fused Gram kernels hold in registers
+constexpr int MAXM = 16;             // largest Anderson history m of the real solvers (m x m LS)
+constexpr int MAXM_FAST = 8;         // histories up to this use the register-resident Gram instantiations
+constexpr int MAXM_CPLX = 8;         // complex and batched solvers: m <= 8 (the complex LS is 2m x 2m)
 constexpr int MAXM_LS = 16;          // largest LS system (complex path embeds m -> 2m)
 constexpr int NXCD = 8;              // MI355X: 8 XCDs, blocks dispatched round-robin
 

@@ -361,13 +361,13 @@ __global__ __launch_bounds__(BLOCK) void hgram_finish_kernel(const HGramArgs a)
 
 hipError_t launch_hgram(const HGramArgs &a, hipStream_t st)
 {
-    if (a.m < 1 || a.m > MAXM || 2 * a.m > MAXM_LS || a.nvalid < 0 || a.nvalid > a.m) return hipErrorInvalidValue;
+    if (a.m < 1 || a.m > MAXM_CPLX || 2 * a.m > MAXM_LS || a.nvalid < 0 || a.nvalid > a.m) return hipErrorInvalidValue;
     if (a.m <= 7) {
         hipLaunchKernelGGL((hgram_kernel<7>), dim3(a.nblocks), dim3(BLOCK), 0, st, a);
         hipLaunchKernelGGL((hgram_finish_kernel<7>), dim3(1), dim3(BLOCK), 0, st, a);
     } else {
-        hipLaunchKernelGGL((hgram_kernel<MAXM>), dim3(a.nblocks), dim3(BLOCK), 0, st, a);
-        hipLaunchKernelGGL((hgram_finish_kernel<MAXM>), dim3(1), dim3(BLOCK), 0, st, a);
+        hipLaunchKernelGGL((hgram_kernel<MAXM_CPLX>), dim3(a.nblocks), dim3(BLOCK), 0, st, a);
+        hipLaunchKernelGGL((hgram_finish_kernel<MAXM_CPLX>), dim3(1), dim3(BLOCK), 0, st, a);
     }
     return hipGetLastError();
 }

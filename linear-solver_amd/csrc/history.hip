@@ -9,6 +9,8 @@
 // subtraction the reference stores (DX = x - x_old, DF = f - f_old), and
 // the whole normal-equation build is ONE pass over the m+1 slots with all
 // products accumulated in FP64 registers.  The m x m solve runs on device.
+#include <type_traits>
+
 #include "aar_common.h"
 #include "aar_device.h"
 #include "aar_kernels.h"
@@ -205,19 +207,27 @@ hipError_t launch_gram(const GramArgs<T> &a, hipStream_t st)
         if (!a.counter) hipLaunchKernelGGL((gram_finish_kernel<T, 7, 0>), dim3(1), block, 0, st, a);
         return hipGetLastError();
     }
-    switch (a.kind) {
-    case 0: hipLaunchKernelGGL((gram_kernel<T, MAXM, 0>), grid, block, 0, st, a); break;
-    case 1: hipLaunchKernelGGL((gram_kernel<T, MAXM, 1>), grid, block, 0, st, a); break;
-    case 2: hipLaunchKernelGGL((gram_kernel<T, MAXM, 2>), grid, block, 0, st, a); break;
-    default: return hipErrorInvalidValue;
-    }
-    if (!a.counter) {
+    if (a.kind < 0 || a.kind > 2) return hipErrorInvalidValue;
+    // m <= 8: every accumulator in registers; up to 16 (the reference's
+    // documented m = 13): the same kernel with 152 accumulators, at lower
+    // occupancy
+    auto go = [&](auto mm) {
+        constexpr int M = decltype(mm)::value;
         switch (a.kind) {
-        case 0: hipLaunchKernelGGL((gram_finish_kernel<T, MAXM, 0>), dim3(1), block, 0, st, a); break;
-        case 1: hipLaunchKernelGGL((gram_finish_kernel<T, MAXM, 1>), dim3(1), block, 0, st, a); break;
-        default: hipLaunchKernelGGL((gram_finish_kernel<T, MAXM, 2>), dim3(1), block, 0, st, a); break;
+        case 0: hipLaunchKernelGGL((gram_kernel<T, M, 0>), grid, block, 0, st, a); break;
+        case 1: hipLaunchKernelGGL((gram_kernel<T, M, 1>), grid, block, 0, st, a); break;
+        default: hipLaunchKernelGGL((gram_kernel<T, M, 2>), grid, block, 0, st, a); break;
         }
-    }
+        if (!a.counter) {
+            switch (a.kind) {
+            case 0: hipLaunchKernelGGL((gram_finish_kernel<T, M, 0>), dim3(1), block, 0, st, a); break;
+            case 1: hipLaunchKernelGGL((gram_finish_kernel<T, M, 1>), dim3(1), block, 0, st, a); break;
+            default: hipLaunchKernelGGL((gram_finish_kernel<T, M, 2>), dim3(1), block, 0, st, a); break;
+            }
+        }
+    };
+    if (a.m <= MAXM_FAST) go(std::integral_constant<int, MAXM_FAST>());
+    else go(std::integral_constant<int, MAXM>());
     return hipGetLastError();
 }
 
@@ -684,7 +694,8 @@ hipError_t launch_aar_update(const UpdArgs<T> &a, hipStream_t st)
     if (a.m < 1 || a.m > MAXM || a.nvalid < 0 || a.nvalid > a.m) return hipErrorInvalidValue;
     long nb = (a.n / 2 + BLOCK - 1) / BLOCK;
     nb = nb < 1 ? 1 : (nb > 4096 ? 4096 : nb);
-    hipLaunchKernelGGL((aar_update_kernel<T, MAXM>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a);
+    if (a.m <= MAXM_FAST) hipLaunchKernelGGL((aar_update_kernel<T, MAXM_FAST>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a);
+    else hipLaunchKernelGGL((aar_update_kernel<T, MAXM>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a);
     return hipGetLastError();
 }
 template hipError_t launch_aar_update<double>(const UpdArgs<double> &, hipStream_t);
@@ -806,6 +817,9 @@ hipError_t launch_anchor_update(const AncArgs<Tx, Tg> &a, hipStream_t st)
     nb = nb < 1 ? 1 : (nb > 4096 ? 4096 : nb);
     if (a.ng <= 8 && a.nc <= 1)
         hipLaunchKernelGGL((anchor_update_kernel<Tx, Tg, 8, 1>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a);
+    else if (a.ng <= MAXM_FAST + 2 && a.nc <= MAXM_FAST + 1)
+        hipLaunchKernelGGL((anchor_update_kernel<Tx, Tg, MAXM_FAST + 2, MAXM_FAST + 1>), dim3((unsigned)nb), dim3(BLOCK),
+                           0, st, a);
     else
         hipLaunchKernelGGL((anchor_update_kernel<Tx, Tg, NGMAX, NCMAX>), dim3((unsigned)nb), dim3(BLOCK), 0,
                            st, a);
@@ -916,7 +930,8 @@ template <typename T>
 hipError_t launch_pgr_update(const PUpdArgs<T> &a, hipStream_t st)
 {
     if (a.m < 1 || a.m > MAXM || a.nvalid < 0 || a.nvalid > a.m) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((pgr_update_kernel<T, MAXM>), dim3(a.nblocks), dim3(BLOCK), 0, st, a);
+    if (a.m <= MAXM_FAST) hipLaunchKernelGGL((pgr_update_kernel<T, MAXM_FAST>), dim3(a.nblocks), dim3(BLOCK), 0, st, a);
+    else hipLaunchKernelGGL((pgr_update_kernel<T, MAXM>), dim3(a.nblocks), dim3(BLOCK), 0, st, a);
     return hipGetLastError();
 }
 template hipError_t launch_pgr_update<double>(const PUpdArgs<double> &, hipStream_t);

@@ -490,7 +490,7 @@ int aar_rec(const Op &op, Tx *x, const Tx *b, double omega, double beta, int m, 
     Context &C = ctx();
     if (int rc = context_init()) return rc;
     if (!x || !b) return fail(AAR_HIP_ERR_ARG, "null vector");
-    if (m < 1 || m > MAXM) return fail(AAR_HIP_ERR_ARG, "history m must be in 1..8");
+    if (m < 1 || m > MAXM) return fail(AAR_HIP_ERR_ARG, "history m must be in 1..16");
     if (p < 1) return fail(AAR_HIP_ERR_ARG, "p must be >= 1");
     if (max_iter < 0) return fail(AAR_HIP_ERR_ARG, "max_iter must be >= 0");
     if (op.c && sizeof(Tx) != 8)
@@ -770,7 +770,7 @@ int pgr_solve(const Op &op, double *x, const double *b, double omega, int m, int
     Context &C = ctx();
     if (int rc = context_init()) return rc;
     if (!x || !b) return fail(AAR_HIP_ERR_ARG, "null vector");
-    if (m < 1 || m > MAXM) return fail(AAR_HIP_ERR_ARG, "history m must be in 1..8");
+    if (m < 1 || m > MAXM) return fail(AAR_HIP_ERR_ARG, "history m must be in 1..16");
     if (p < 1) return fail(AAR_HIP_ERR_ARG, "p must be >= 1");
     if (max_iter < 0) return fail(AAR_HIP_ERR_ARG, "max_iter must be >= 0");
     const long n = op.n;
@@ -1162,7 +1162,7 @@ int helm_solve(const HOp &op, double *u, const double *b, double omega, double b
     Context &C = ctx();
     if (int rc = context_init()) return rc;
     if (!u || !b) return fail(AAR_HIP_ERR_ARG, "null vector");
-    if (m < 1 || m > MAXM) return fail(AAR_HIP_ERR_ARG, "history m must be in 1..8");
+    if (m < 1 || m > MAXM_CPLX) return fail(AAR_HIP_ERR_ARG, "complex solver: history m must be in 1..8");
     if (p < 1) return fail(AAR_HIP_ERR_ARG, "p must be >= 1");
     if (max_iter < 0) return fail(AAR_HIP_ERR_ARG, "max_iter must be >= 0");
     const long n = op.n;

@@ -73,6 +73,13 @@ def cases():
                             rhs="reference", x0=1.0)
     out["refdrv_16"] = dict(base, shape=[16, 16, 16], a=-1.0 / (4 * math.pi), seed=0,
                             rhs="reference", x0=1.0)
+    # the solver parameters of the reference's own README examples
+    # (MPI_REAL/README: mpirun -n 8 ./mpi_real 1e-6 13 11 1.0 0.9) on the
+    # driver's own problem
+    out["readme_48"] = dict(base, shape=[48, 48, 48], a=-1.0 / (4 * math.pi), seed=0,
+                            rhs="reference", x0=1.0, m=13, p=11, omega=1.0, beta=0.9)
+    # the deepest history libaar_hip takes
+    out["m16p3_16"] = dict(base, shape=[16, 16, 16], a=P.poisson_a(0.5), seed=1001, m=16, p=3)
     return out
 
 

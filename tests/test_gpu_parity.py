@@ -91,7 +91,7 @@ def _gpu_solve(gpu, solver, c, shape, b, x0):
     return xd.cpu().numpy(), st
 
 
-@pytest.mark.parametrize("name", SMALL_CASES + ["c1_48", "refdrv_48"])
+@pytest.mark.parametrize("name", SMALL_CASES + ["c1_48", "refdrv_48", "readme_48"])
 @pytest.mark.parametrize("solver", ["aar", "pgr", "pl2r", "cg"])
 def test_solver_parity(gpu, oracle, golden, name, solver):
     meta, g = golden
@@ -121,8 +121,19 @@ def test_solver_parity(gpu, oracle, golden, name, solver):
     assert k == len(tr)
     if k and tr[0] > 0:
         # rounding-level drift: relative to each entry, or to ||r_0|| once the
-        # residual itself is tiny
-        assert np.all(np.abs(h[:k] - tr[:k]) <= 1e-6 * tr[:k] + 1e-10 * tr[0])
+        # residual itself is tiny.  With omega above plain Richardson's
+        # stability limit (0.843: omega * 2.37 < 2) the Richardson steps grow
+        # rounding differences by up to |1 - omega*2.37| per step: AAR's
+        # entries between Anderson steps (its recurrence residual) and PGR's /
+        # PL2R's residuals (updated as Ax += DF s in both codes, PGR.c:117-121)
+        # get 1e-4 there; AAR's true residuals keep 1e-6
+        rel = np.full(k, 1e-6)
+        if c["omega"] > 0.843:
+            if solver == "aar":
+                rel[np.arange(k) % c["p"] != 0] = 1e-4
+            elif solver in ("pgr", "pl2r"):
+                rel[:] = 1e-4
+        assert np.all(np.abs(h[:k] - tr[:k]) <= rel * tr[:k] + 1e-10 * tr[0])
 
 
 def test_aar_fp32(gpu):
@@ -137,7 +148,7 @@ def test_aar_fp32(gpu):
     assert np.linalg.norm(xx - xf) / np.linalg.norm(xf) < 1e-3
 
 
-@pytest.mark.parametrize("name", SMALL_CASES + ["c1_48", "refdrv_48"])
+@pytest.mark.parametrize("name", SMALL_CASES + ["c1_48", "refdrv_48", "readme_48"])
 def test_mixed_parity(gpu, oracle, golden, name):
     """Mixed-precision AAR (FP32 sweeps and history, FP64 true residual,
     anchor and normal equations) on the reference's AAR fixtures: the same
@@ -265,6 +276,30 @@ def test_dropin_reference_driver(golden):
                          text=True, timeout=300, cwd=tmp).stdout
     say(out[-1200:])
     ref = meta["cases"]["refdrv_48"]["solvers"]
+    import re
+    for sv in ("AAR", "PGR", "PL2R", "CG"):
+        m = re.search(sv + r" converged!:\s+Iterations = (\d+)", out)
+        assert m, f"{sv} line missing"
+        assert int(m.group(1)) == ref[sv.lower()]["iterations"]
+
+
+def test_dropin_readme_example(golden):
+    """The reference's documented run (MPI_REAL/README: ./mpi_real 1e-6 13 11
+    1.0 0.9 -- history m = 13, p = 11, omega = 1.0, beta = 0.9) through its own
+    main.c relinked with libaar_hip: the reference's iteration counts."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    exe = os.path.join(os.path.dirname(__file__), "..", "oracle", "_ref", "main_on_libaar")
+    if not os.path.exists(exe):
+        pytest.skip("drop-in demo not built")
+    meta, _ = golden
+    tmp = os.path.join(os.path.dirname(__file__), "..", "gpurun_out")
+    os.makedirs(tmp, exist_ok=True)
+    out = subprocess.run([os.path.abspath(exe), "1e-6", "13", "11", "1.0", "0.9"], capture_output=True,
+                         text=True, timeout=300, cwd=tmp).stdout
+    say(out[-1200:])
+    ref = meta["cases"]["readme_48"]["solvers"]
     import re
     for sv in ("AAR", "PGR", "PL2R", "CG"):
         m = re.search(sv + r" converged!:\s+Iterations = (\d+)", out)

@@ -226,10 +226,10 @@ def run_oracle(O, solver, c, shape, b, x0):
 
 
 SMALL_CASES = ["c1_16", "c1_24", "screened_24", "aniso_20x12x16", "maxiter13_16", "maxiter12_16",
-               "p1_16", "m1_16", "m3p5_16", "noprec_16", "zero_rhs_16", "refdrv_16"]
+               "p1_16", "m1_16", "m3p5_16", "noprec_16", "zero_rhs_16", "refdrv_16", "m16p3_16"]
 
 
-@pytest.mark.parametrize("name", SMALL_CASES + ["c1_48", "refdrv_48"])
+@pytest.mark.parametrize("name", SMALL_CASES + ["c1_48", "refdrv_48", "readme_48"])
 @pytest.mark.parametrize("solver", ["aar", "pgr", "pl2r", "cg"])
 def test_oracle_bitwise_vs_reference(oracle, golden, name, solver):
     meta, g = golden
