@@ -161,7 +161,7 @@ int aar_hip_helm_matvec(const aar_hip_helm *h, const double *u, double *y);
 /* AAR on A u = b (adjoint = 0) or on A^H u = b (adjoint = 1: eta -> -eta),
  * the control flow of AAR.c:37-177 with the Hermitian Gram DF^H DF, DF^H f
  * solved through the real 2m x 2m embedding by the pinv of tools.c;
- * m <= 8.  u: initial guess in, solution out. */
+ * m <= 16.  u: initial guess in, solution out. */
 int aar_hip_helm_aar(const aar_hip_helm *h, double *u, const double *b, double omega, double beta,
                      int m, int p, int max_iter, double tol, int adjoint, aar_hip_stats *st);
 

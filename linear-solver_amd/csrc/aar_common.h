@@ -16,8 +16,9 @@ constexpr int TX = 64;               // tile width in x: one wavefront
 constexpr int BLOCK = 256;           // threads per block of the streaming (history/dot) kernels
 constexpr int MAXM = 16;             // largest Anderson history m of the real solvers (m x m LS)
 constexpr int MAXM_FAST = 8;         // histories up to this use the register-resident Gram instantiations
-constexpr int MAXM_CPLX = 8;         // complex and batched solvers: m <= 8 (the complex LS is 2m x 2m)
-constexpr int MAXM_LS = 16;          // largest LS system (complex path embeds m -> 2m)
+constexpr int MAXM_CPLX = 16;        // complex solver: m <= 16 (its LS is the 2m x 2m real embedding)
+constexpr int MAXM_BATCH = 8;        // batched solves: m <= 8
+constexpr int MAXM_LS = 32;          // largest LS system (complex path embeds m -> 2m)
 constexpr int NXCD = 8;              // MI355X: 8 XCDs, blocks dispatched round-robin
 
 // Per-solve control block in device memory.  Kernels read `done` on entry

@@ -323,7 +323,7 @@ int batch_solve(const aar_hip_instance *ins, int count, double omega, double bet
     Context &C = ctx();
     if (int rc0 = context_init()) return rc0;
     if (!ins || count < 1) return fail(AAR_HIP_ERR_ARG, "empty batch");
-    if (m < 1 || m > MAXM_CPLX) return fail(AAR_HIP_ERR_ARG, "batched solves: history m must be in 1..8");
+    if (m < 1 || m > MAXM_BATCH) return fail(AAR_HIP_ERR_ARG, "batched solves: history m must be in 1..8");
     if (p < 1 || p > m + 1) return fail(AAR_HIP_ERR_ARG, "batched solves need 1 <= p <= m+1");
     if (max_iter < 0) return fail(AAR_HIP_ERR_ARG, "max_iter must be >= 0");
     int rc = 0;
@@ -342,7 +342,7 @@ int batch_solve(const aar_hip_instance *ins, int count, double omega, double bet
     hipGraphExec_t exec = nullptr;
     const int S = m + 2;
     const int NC = (p <= m) ? m / p + 1 : 0;
-    const int NA = (m == 7) ? 35 : (MAXM_CPLX * (MAXM_CPLX + 1) / 2 + MAXM_CPLX);
+    const int NA = (m == 7) ? 35 : (MAXM_BATCH * (MAXM_BATCH + 1) / 2 + MAXM_BATCH);
     BShared sh = {m, p, m, omega, beta};
     // graphs cannot be captured on the legacy default stream that framework
     // callers bind: the batch runs on the library's own stream, ordered
@@ -444,13 +444,13 @@ int batch_solve(const aar_hip_instance *ins, int count, double omega, double bet
                 hipLaunchKernelGGL((bgram_kernel<Tg, 7>), dim3((unsigned)wch.size()), dim3(BLOCK), 0, st, ID, IF, dch,
                                    clock, jrel, sh);
             else
-                hipLaunchKernelGGL((bgram_kernel<Tg, MAXM_CPLX>), dim3((unsigned)wch.size()), dim3(BLOCK), 0, st, ID, IF,
+                hipLaunchKernelGGL((bgram_kernel<Tg, MAXM_BATCH>), dim3((unsigned)wch.size()), dim3(BLOCK), 0, st, ID, IF,
                                    dch, clock, jrel, sh);
             if (m == 7)
                 hipLaunchKernelGGL((bgram_finish_kernel<Tg, 7>), dim3(count), dim3(BLOCK), 0, st, ID, IF, clock, jrel,
                                    sh);
             else
-                hipLaunchKernelGGL((bgram_finish_kernel<Tg, MAXM_CPLX>), dim3(count), dim3(BLOCK), 0, st, ID, IF, clock,
+                hipLaunchKernelGGL((bgram_finish_kernel<Tg, MAXM_BATCH>), dim3(count), dim3(BLOCK), 0, st, ID, IF, clock,
                                    jrel, sh);
             (void)launch_ls_batch(dctrl, count, m, st);
             hipLaunchKernelGGL((banchor_kernel<Tx, Tg>), dim3((unsigned)wch.size()), dim3(BLOCK), 0, st, dI, dch, clock,

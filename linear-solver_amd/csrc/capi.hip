@@ -582,7 +582,7 @@ void SingularValueDecomp(double **a, int m, int n, double *w, double **v)
 void PseudoInverseTimesVec(double **Am, double *b, double *x, int m)
 {
     if (m < 1 || m > MAXM_LS) {
-        std::printf("PseudoInverseTimesVec: libaar_hip supports m <= 16\n");
+        std::printf("PseudoInverseTimesVec: libaar_hip supports m <= 32\n");
         return;
     }
     std::vector<double> A((size_t)m * m);
@@ -941,7 +941,7 @@ int aar_hip_set_tuning(int lz, int wy, int py)
 
 int aar_hip_last_system(int n, double *G, double *w)
 {
-    if (n < 1 || n > MAXM_LS) return fail(AAR_HIP_ERR_ARG, "system size must be in 1..16");
+    if (n < 1 || n > MAXM_LS) return fail(AAR_HIP_ERR_ARG, "system size must be in 1..32");
     const Ctrl &c = ctx().ctrl_h[0];
     if (G)
         for (int k = 0; k < n * n + n; k++) G[k] = c.G[k];

@@ -1162,7 +1162,7 @@ int helm_solve(const HOp &op, double *u, const double *b, double omega, double b
     Context &C = ctx();
     if (int rc = context_init()) return rc;
     if (!u || !b) return fail(AAR_HIP_ERR_ARG, "null vector");
-    if (m < 1 || m > MAXM_CPLX) return fail(AAR_HIP_ERR_ARG, "complex solver: history m must be in 1..8");
+    if (m < 1 || m > MAXM_CPLX) return fail(AAR_HIP_ERR_ARG, "complex solver: history m must be in 1..16");
     if (p < 1) return fail(AAR_HIP_ERR_ARG, "p must be >= 1");
     if (max_iter < 0) return fail(AAR_HIP_ERR_ARG, "max_iter must be >= 0");
     const long n = op.n;
@@ -1477,7 +1477,7 @@ int pinv_host(const double *A, const double *b, double *x, int m, double *w_out,
 {
     Context &C = ctx();
     if (int rc = context_init()) return rc;
-    if (m < 1 || m > MAXM_LS) return fail(AAR_HIP_ERR_ARG, "pinv: m must be in 1..16");
+    if (m < 1 || m > MAXM_LS) return fail(AAR_HIP_ERR_ARG, "pinv: m must be in 1..32");
     Ctrl &hc = C.ctrl_h[1];
     std::memset(&hc, 0, sizeof(Ctrl));
     // allredvec layout: A[i][j] at G[j*m + i], b at G[m*m + j]

@@ -75,6 +75,24 @@ def test_pinv_kat_bitwise(gpu, golden):
         ib += m
 
 
+@pytest.mark.parametrize("m", [20, 26, 32])
+def test_pinv_large_bitwise(gpu, oracle, m):
+    """The one-wavefront SVD at the sizes of the deep histories (m <= 16
+    real, 2m <= 32 complex): bit-identical to the tools.c restatement on an
+    SPD and a rank-deficient (zero row and column) normal matrix."""
+    rng = np.random.default_rng(300 + m)
+    F = rng.standard_normal((3 * m, m))
+    G0 = F.T @ F
+    G1 = G0.copy()
+    G1[m - 1, :] = 0.0
+    G1[:, m - 1] = 0.0
+    for A in (G0, G1):
+        b = rng.standard_normal(m)
+        x, w, _ = gpu.pinv(A, b)
+        xo, wo, _ = oracle.pinv_times_vec(A, b)
+        assert np.array_equal(x, xo) and np.array_equal(w, wo)
+
+
 def _gpu_solve(gpu, solver, c, shape, b, x0):
     xd = dev(x0)
     bd = dev(b)

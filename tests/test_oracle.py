@@ -136,7 +136,7 @@ def numpy_pinv_ref(A, b):
     return V @ (inv * (V.T @ b)), w
 
 
-@pytest.mark.parametrize("m", [1, 2, 4, 7, 8, 13])
+@pytest.mark.parametrize("m", [1, 2, 4, 7, 8, 13, 20, 26, 32])
 def test_pinv_matches_numpy_svd(oracle, m):
     rng = np.random.default_rng(10 + m)
     F = rng.standard_normal((40, m))
@@ -401,7 +401,7 @@ def test_oracle_complex_adjoint_identity(oracle):
     assert abs(np.vdot(AHv, u) - np.vdot(v, Au)) < 1e-10 * abs(np.vdot(v, Au))
 
 
-@pytest.mark.parametrize("m", [2, 7, 14, 16])
+@pytest.mark.parametrize("m", [2, 7, 14, 16, 26, 32])
 def test_pinv_textbook_matches_numpy(oracle, m):
     """The textbook Golub-Reinsch variant (the complex solver's m x m solve)
     is the pseudo-inverse: V diag(1/w) V^T b against numpy's SVD on
