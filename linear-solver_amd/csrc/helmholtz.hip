@@ -458,9 +458,13 @@ hipError_t launch_hanchor_update(const HAncArgs &a, hipStream_t st)
             return hipErrorInvalidValue;
     long nb = (a.n + BLOCK - 1) / BLOCK;
     nb = nb < 1 ? 1 : (nb > 4096 ? 4096 : nb);
-    // one instantiation: an (8, 1) form with half the registers measured 4%
-    // slower at 256^3
-    hipLaunchKernelGGL((hanchor_update_kernel<NGMAX, NCMAX>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a);
+    // histories up to 8 (an (8, 1) form with half the registers measured 4%
+    // slower at 256^3), and the deeper ones
+    if (a.ng <= MAXM_FAST + 2 && a.nc <= MAXM_FAST + 1)
+        hipLaunchKernelGGL((hanchor_update_kernel<MAXM_FAST + 2, MAXM_FAST + 1>), dim3((unsigned)nb), dim3(BLOCK), 0,
+                           st, a);
+    else
+        hipLaunchKernelGGL((hanchor_update_kernel<NGMAX, NCMAX>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a);
     return hipGetLastError();
 }
 
