@@ -406,3 +406,34 @@ def test_anderson_step_reference_abi(gpu, golden):
     assert np.allclose(red, ref_red, rtol=1e-13, atol=1e-12 * np.abs(ref_red).max())
     assert np.array_equal(red[m * m:], Ftf)
     assert np.allclose(am, g["and_am"], rtol=1e-10, atol=1e-12 * np.abs(g["and_am"]).max())
+
+
+@pytest.mark.parametrize("m", [13, 16])
+def test_anderson_step_deep_history(gpu, oracle, m):
+    """AndersonExtrapolation (AAR.h:24-25) at the README's m = 13 and at the
+    deepest m = 16 (the m <= 16 Gram and update kernels) against the
+    restatement of AAR.c:185-232."""
+    import ctypes
+    from aar_hip import _lib
+    L = _lib.load()
+    rng = np.random.default_rng(400 + m)
+    n = 5000
+    DX = np.ascontiguousarray(rng.standard_normal((m, n)))
+    DF = np.ascontiguousarray(rng.standard_normal((m, n)))
+    f = rng.standard_normal(n)
+    am = np.zeros(n)
+    red = np.zeros(m * m + m)
+    Ftf = np.zeros(m)
+    sv = np.zeros(m)
+    FtF = np.zeros((m, m))
+    D = ctypes.POINTER(ctypes.c_double)
+    P = ctypes.POINTER(D)
+    L.AndersonExtrapolation.argtypes = [P, P, D, ctypes.c_double, ctypes.c_int, ctypes.c_int, D, P, D, D,
+                                        D, ctypes.c_int]
+    L.AndersonExtrapolation(_ptr_rows(DX), _ptr_rows(DF), f.ctypes.data_as(D), 0.6, m, n,
+                            am.ctypes.data_as(D), _ptr_rows(FtF), red.ctypes.data_as(D),
+                            Ftf.ctypes.data_as(D), sv.ctypes.data_as(D), 0)
+    am_o, red_o, s_o = oracle.anderson_extrapolation(DX, DF, f, 0.6)
+    assert np.allclose(red, red_o, rtol=1e-13, atol=1e-12 * np.abs(red_o).max())
+    assert np.allclose(sv, s_o, rtol=1e-9, atol=1e-12 * np.abs(s_o).max())
+    assert np.allclose(am, am_o, rtol=1e-10, atol=1e-12 * np.abs(am_o).max())
