@@ -19,6 +19,11 @@ constexpr int MAXM_FAST = 8;         // histories up to this use the register-re
 constexpr int MAXM_CPLX = 16;        // complex solver: m <= 16 (its LS is the 2m x 2m real embedding)
 constexpr int MAXM_BATCH = 8;        // batched solves: m <= 8
 constexpr int MAXM_LS = 32;          // largest LS system (complex path embeds m -> 2m)
+// per-block partial sums of the largest Gram instantiations, which every
+// buffer handed to a Gram kernel must hold per block: PGR's full DX^T DF
+// (m^2 + m) at m = MAXM, and the complex Gram (2 (m(m+1)/2 + m)) at MAXM_CPLX
+constexpr int GRAM_NA_MAX = MAXM * MAXM + MAXM;
+constexpr int HGRAM_NA_MAX = 2 * (MAXM_CPLX * (MAXM_CPLX + 1) / 2 + MAXM_CPLX);
 constexpr int NXCD = 8;              // MI355X: 8 XCDs, blocks dispatched round-robin
 
 // Per-solve control block in device memory.  Kernels read `done` on entry

@@ -262,12 +262,12 @@ int step_gram_ls(int kind, const StepBufs &sb, int m, int n)
     Ctrl *dc = C.ctrl + 1;
     static DevBuf part;
     const int nb = (int)std::max(1L, std::min(2048L, ((long)n + 4095) / 4096));
-    if (part.bytes < sizeof(double) * nb * 80) {
+    if (part.bytes < sizeof(double) * nb * GRAM_NA_MAX) {
         if (part.p) (void)hipFree(part.p);
         part.p = nullptr;
         part.bytes = 0;
-        HCR(hipMalloc(&part.p, sizeof(double) * nb * 80), AAR_HIP_ERR_HIP);
-        part.bytes = sizeof(double) * nb * 80;
+        HCR(hipMalloc(&part.p, sizeof(double) * nb * GRAM_NA_MAX), AAR_HIP_ERR_HIP);
+        part.bytes = sizeof(double) * nb * GRAM_NA_MAX;
     }
     HCR(launch_zero_ctrl(dc, C.stream), AAR_HIP_ERR_HIP);
     GramArgs<double> g;

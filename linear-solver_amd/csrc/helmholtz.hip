@@ -276,6 +276,7 @@ __global__ __launch_bounds__(BLOCK) void hgram_kernel(const HGramArgs a)
 {
     constexpr int NG = MM * (MM + 1) / 2;
     constexpr int NA = 2 * (NG + MM);
+    static_assert(NA <= HGRAM_NA_MAX, "partials buffers hold HGRAM_NA_MAX per block");
     constexpr int NS = MM + 1;
     __shared__ double scratch[NA * (BLOCK / 64)];
     if (a.ctrl->done) return;

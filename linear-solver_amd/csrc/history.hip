@@ -62,6 +62,7 @@ __global__ __launch_bounds__(BLOCK) void gram_kernel(const GramArgs<T> a)
 {
     constexpr int NG = (KIND == 1) ? MM * MM : MM * (MM + 1) / 2;
     constexpr int NA = NG + MM;
+    static_assert(NA <= GRAM_NA_MAX, "partials buffers hold GRAM_NA_MAX per block");
     constexpr int NS = MM + 1;
     __shared__ double scratch[NA * (BLOCK / 64)];
     __shared__ double sums[NA];

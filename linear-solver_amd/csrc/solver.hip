@@ -511,7 +511,7 @@ int aar_rec(const Op &op, Tx *x, const Tx *b, double omega, double beta, int m, 
     HC(ensure(HB, sizeof(double) * (size_t)(max_iter + 4)));
     const int nb = nblocks_for(n);
     const size_t need = std::max<size_t>(
-        std::max((size_t)stencil_partials_needed<Tg>(op), (size_t)stencil_partials_needed<Tx>(op)), (size_t)nb * 80);
+        std::max((size_t)stencil_partials_needed<Tg>(op), (size_t)stencil_partials_needed<Tx>(op)), (size_t)nb * GRAM_NA_MAX);
     DevBuf &PB = ci == 0 ? C.part0 : C.part1;
     HC(ensure(PB, sizeof(double) * need));
     Tg *G = (Tg *)RG.p;
@@ -780,7 +780,7 @@ int pgr_solve(const Op &op, double *x, const double *b, double omega, int m, int
     HC(ensure(C.ring64f, sizeof(double) * ld * ns));
     HC(ensure(C.hist0, sizeof(double) * (size_t)(max_iter + 4)));
     const int nb = nblocks_for(n);
-    const size_t need = std::max<size_t>((size_t)stencil_partials_needed<double>(op), (size_t)nb * 80);
+    const size_t need = std::max<size_t>((size_t)stencil_partials_needed<double>(op), (size_t)nb * GRAM_NA_MAX);
     HC(ensure(C.part0, sizeof(double) * need));
     double *X = (double *)C.ring64x.p, *Y = (double *)C.ring64f.p;
     double *hist = (double *)C.hist0.p, *part = (double *)C.part0.p;
@@ -935,7 +935,7 @@ int cg_solve(const Op &op, double *x, const double *b, int max_iter, double tol,
     HC(ensure(C.ring64x, sizeof(double) * ld * 4));
     HC(ensure(C.hist0, sizeof(double) * (size_t)(max_iter + 4)));
     const int nb = nblocks_for(n);
-    const size_t need = std::max<size_t>((size_t)stencil_partials_needed<double>(op), (size_t)nb * 80);
+    const size_t need = std::max<size_t>((size_t)stencil_partials_needed<double>(op), (size_t)nb * GRAM_NA_MAX);
     HC(ensure(C.part0, sizeof(double) * need));
     double *r = (double *)C.ring64x.p, *dd[2] = {r + ld, r + 2 * ld}, *q = r + 3 * ld;
     double *hist = (double *)C.hist0.p, *part = (double *)C.part0.p;
@@ -1174,7 +1174,7 @@ int helm_solve(const HOp &op, double *u, const double *b, double omega, double b
     HC(ensure(C.hist0, sizeof(double) * (size_t)(max_iter + 4)));
     const int nb = nblocks_for(n);
     HArgs probe = base_hargs(op);
-    const size_t need = std::max<size_t>((size_t)hstencil_items(probe), (size_t)nb * 96);
+    const size_t need = std::max<size_t>((size_t)hstencil_items(probe), (size_t)nb * HGRAM_NA_MAX);
     HC(ensure(C.part0, sizeof(double) * need));
     double2 *G = (double2 *)C.ringc.p;
     double2 *CR = NC > 0 ? (double2 *)C.corrc.p : nullptr;
@@ -1380,7 +1380,7 @@ int aar_refined(const Op &op, double *x, const double *b, double omega, double b
     HC(ensure(C.vec32b, sizeof(float) * n));   // e (FP32 correction)
     HC(ensure(C.hist0, sizeof(double) * (size_t)(max_outer + 4)));
     const int nb = nblocks_for(n);
-    const size_t need = std::max<size_t>((size_t)stencil_partials_needed<double>(op), (size_t)nb * 80);
+    const size_t need = std::max<size_t>((size_t)stencil_partials_needed<double>(op), (size_t)nb * GRAM_NA_MAX);
     HC(ensure(C.part0, sizeof(double) * need));
     float *r32 = (float *)C.vec32a.p, *e32 = (float *)C.vec32b.p;
     double *part = (double *)C.part0.p;
