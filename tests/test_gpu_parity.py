@@ -437,3 +437,38 @@ def test_anderson_step_deep_history(gpu, oracle, m):
     assert np.allclose(red, red_o, rtol=1e-13, atol=1e-12 * np.abs(red_o).max())
     assert np.allclose(sv, s_o, rtol=1e-9, atol=1e-12 * np.abs(s_o).max())
     assert np.allclose(am, am_o, rtol=1e-10, atol=1e-12 * np.abs(am_o).max())
+
+
+@pytest.mark.parametrize("solver", ["aar", "mixed", "pgr", "pl2r"])
+def test_deep_history_at_scale(gpu, oracle, solver):
+    """m = 13 at 128^3, where the Gram runs ~512 reduction blocks and its
+    per-block partials (152 values; 272 for PGR's DX^T DF) far exceed the
+    stencil's, so every partials buffer is used at its Gram size: three
+    Anderson cycles (p = 11) against the restatement -- iterates and true
+    residuals within 1e-6 (mixed precision: 1e-3)."""
+    shape = (128, 128, 128)
+    n = 128 ** 3
+    a = P.poisson_a(0.5)
+    b = P.rhs_uniform(shape, 1014)
+    m, p, its = 13, 11, 33
+    say("deep history at scale", solver)
+    xd = dev(np.zeros(n))
+    if solver in ("aar", "mixed"):
+        fn = gpu.AAR if solver == "aar" else gpu.AAR_mixed
+        st = fn(xd, dev(b), shape, a, 0.6, 0.6, m, p, its, 1e-30)
+        xr, dr = oracle.aar(np.zeros(n), b, shape, a, 0.6, 0.6, m, p, its, 1e-30)
+    else:
+        fn = gpu.PL2R if solver == "pl2r" else gpu.PGR
+        st = fn(xd, dev(b), shape, a, 0.6, m, p, its, 1e-30)
+        xr, dr = oracle.pgr(np.zeros(n), b, shape, a, 0.6, m, p, its, 1e-30, l2=solver == "pl2r")
+    assert st["iterations"] == dr["iterations"] == its
+    x = xd.cpu().numpy()
+    # a mid-solve iterate (three Anderson steps of a 13-column least-squares
+    # problem): agreement to the LS problem's sensitivity, not to rounding;
+    # mixed precision against the FP64 restatement: to its FP32 sweeps
+    tol = 1e-3 if solver == "mixed" else 1e-6
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) <= tol
+    h = gpu.history()
+    tr = dr["trace"]
+    k = np.arange(0, len(tr), p)                   # the start and the Anderson iterations
+    assert np.all(np.abs(h[k] - tr[k]) <= tol * tr[k])
