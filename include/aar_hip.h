@@ -53,6 +53,8 @@ typedef struct {
     double bytes;           /* algorithmic HBM bytes moved by the solve */
     int outer_steps;        /* FP32-refined solver: FP64 residual-refinement steps */
     int inner_iterations;   /* FP32-refined solver: total FP32 AAR iterations */
+    int reason;             /* Krylov comparators: PETSc's KSPConvergedReason (2 rtol,
+                               3 atol, -3 max_it, -4 dtol, -5/-6 breakdown, -9 nan) */
 } aar_hip_stats;
 
 /* system.c:189-205 -- y = a*Lap(x) + sigma*x (device pointers, FP64) */
@@ -131,6 +133,23 @@ int aar_hip_pgr(const aar_hip_grid *g, double *x, const double *b, double omega,
 /* CG.c:33-175 */
 int aar_hip_cg(const aar_hip_grid *g, double *x, const double *b, int max_iter, double tol,
                int precond, aar_hip_stats *st);
+
+/* The Krylov comparators of the PETSc variant, PETSC_REAL/main.c:86-122
+ * (KSPGMRES) and 124-150 (KSPBICG): PETSc's algorithms with its defaults
+ * (GMRES: restart 30, classical Gram-Schmidt, left preconditioning; BiCG:
+ * shadow residual r0), the Jacobi preconditioner of PCJACOBI (precond = 1:
+ * multiply by 1/diag; 0: none) and KSPConvergedDefault on the
+ * preconditioned residual norm, ttol = tol*||B b|| for the nonzero guess.
+ * c: a variable-coefficient field (device, a*Lap + c(x), diagonal
+ * a*3*c0 + c(x); sigma must then be 0) or NULL.  restart <= 32.  stats:
+ * iterations = PETSc's its, relres = ||b - A x||/||b|| of the returned x
+ * (main.c:112-118), reason = the KSPConvergedReason; aar_hip_history()
+ * returns the norm of every convergence test. */
+#define AAR_HIP_GMRES_MAXK 32
+int aar_hip_gmres(const aar_hip_grid *g, const double *c, double *x, const double *b, int restart,
+                  int max_iter, double tol, int precond, aar_hip_stats *st);
+int aar_hip_bicg(const aar_hip_grid *g, const double *c, double *x, const double *b, int max_iter,
+                 double tol, int precond, aar_hip_stats *st);
 
 /* Extension (no reference counterpart): FP64 residual refinement around
  * FP32 AAR.  Outer: r = b - A x in FP64, stop at ||r|| <= tol*||b||;

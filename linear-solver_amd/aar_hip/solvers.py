@@ -217,6 +217,43 @@ def CG(x, b, shape, a, max_iter, tol, sigma=0.0, precond=True, c=None):
     return st.as_dict()
 
 
+def GMRES(x, b, shape, a, max_iter=2000, tol=1e-6, restart=30, sigma=0.0, precond=True, c=None):
+    """PETSc's KSPGMRES with PCJACOBI as PETSC_REAL/main.c:86-122 runs it
+    (restart 30, classical Gram-Schmidt, left preconditioning, converged on
+    the preconditioned residual norm).  Returns the stats; 'reason' is the
+    KSPConvergedReason, 'relres' the true ||b - A x||/||b||."""
+    import torch
+    _check_len(x, shape, "x")
+    _check_len(b, shape, "b")
+    g = _grid(shape, a, sigma)
+    st = Stats()
+    _bind_stream()
+    cp = None
+    if c is not None:
+        _check_len(c, shape, "c")
+        cp = _dev(c, torch.float64, "c")
+    check(_L().aar_hip_gmres(ctypes.byref(g), cp, _dev(x, torch.float64, "x"), _dev(b, torch.float64, "b"),
+                             restart, max_iter, tol, int(bool(precond)), ctypes.byref(st)))
+    return st.as_dict()
+
+
+def BiCG(x, b, shape, a, max_iter=2000, tol=1e-6, sigma=0.0, precond=True, c=None):
+    """PETSc's KSPBICG with PCJACOBI as PETSC_REAL/main.c:124-150 runs it."""
+    import torch
+    _check_len(x, shape, "x")
+    _check_len(b, shape, "b")
+    g = _grid(shape, a, sigma)
+    st = Stats()
+    _bind_stream()
+    cp = None
+    if c is not None:
+        _check_len(c, shape, "c")
+        cp = _dev(c, torch.float64, "c")
+    check(_L().aar_hip_bicg(ctypes.byref(g), cp, _dev(x, torch.float64, "x"), _dev(b, torch.float64, "b"),
+                            max_iter, tol, int(bool(precond)), ctypes.byref(st)))
+    return st.as_dict()
+
+
 def AAR_refined(x, rhs, shape, a, omega, beta, m, p, tol, inner_tol=1e-5, max_outer=30,
                 max_inner=100000, sigma=0.0, precond=True):
     """FP64 residual refinement around FP32 AAR (no reference counterpart)."""

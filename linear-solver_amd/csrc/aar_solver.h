@@ -56,6 +56,7 @@ struct Context {
     DevBuf vec32a, vec32b, vec64a, vec64b;
     DevBuf ringc, corrc;         // complex Helmholtz g ring and corrections
     DevBuf vch;                  // variable coefficient: the recurrence field h(x)
+    DevBuf kctl, kvec;           // Krylov comparators: control block, basis / work vectors
     std::string err;
     int lz = 0, wy = 8, py = 4;
     long ring_pad = 512;           // extra elements per ring slot (multiple of 32)
@@ -115,6 +116,11 @@ int pgr_solve(const Op &op, double *x, const double *b, double omega, int m, int
               double tol, int precond, int l2, aar_hip_stats *st);
 int cg_solve(const Op &op, double *x, const double *b, int max_iter, double tol, int precond,
              aar_hip_stats *st);
+int gmres_solve(const Op &op, double *x, const double *b, int restart, int max_it, double tol, int precond,
+                aar_hip_stats *st);
+int bicg_solve(const Op &op, double *x, const double *b, int max_it, double tol, int precond, aar_hip_stats *st);
+template <typename T>
+int run_stencil(SMode mode, SArgs<T> a, double bytes);
 template <typename T>
 int matvec(const Op &op, const T *x, T *y);
 int aar_refined(const Op &op, double *x, const double *b, double omega, double beta, int m, int p,

@@ -805,6 +805,22 @@ int aar_hip_cg_vc(const aar_hip_grid *g, const double *c, double *x, const doubl
     return cg_solve(op, x, b, max_iter, tol, precond, st);
 }
 
+int aar_hip_gmres(const aar_hip_grid *g, const double *c, double *x, const double *b, int restart, int max_iter,
+                  double tol, int precond, aar_hip_stats *st)
+{
+    Op op;
+    if (int rc = c ? make_op_vc(g, c, op) : make_op(g, nullptr, op)) return rc;
+    return gmres_solve(op, x, b, restart, max_iter, tol, precond, st);
+}
+
+int aar_hip_bicg(const aar_hip_grid *g, const double *c, double *x, const double *b, int max_iter, double tol,
+                 int precond, aar_hip_stats *st)
+{
+    Op op;
+    if (int rc = c ? make_op_vc(g, c, op) : make_op(g, nullptr, op)) return rc;
+    return bicg_solve(op, x, b, max_iter, tol, precond, st);
+}
+
 int aar_hip_matvec_f32(const aar_hip_grid *g, const float *x, float *y)
 {
     Op op;

@@ -1,0 +1,743 @@
+// krylov.hip -- the Krylov comparators of the PETSc variant of the reference
+// (PETSC_REAL/main.c:86-150): GMRES and BiCG with a Jacobi preconditioner,
+// run the way PETSc's KSPGMRES and KSPBICG run them there (rtol, max_it
+// 2000, nonzero initial guess).
+//
+// PETSc itself is not part of the reference tree.  The control flow below
+// follows its published algorithms and defaults:
+//  * convergence (KSPConvergedDefault): ttol = max(rtol*||B b||, atol) and
+//    rnorm0 = rnorm at the first test; converged when rnorm <= ttol,
+//    diverged when rnorm >= dtol*rnorm0 (atol 1e-50, dtol 1e5);
+//  * Jacobi (PCJACOBI): y = x * (1/diag), the reciprocal taken once;
+//  * GMRES: restart k (30 by default), left preconditioning, the
+//    preconditioned residual norm, classical Gram-Schmidt without
+//    refinement, Givens rotations, happy-breakdown tolerance 1e-30, the
+//    solution built by back-substitution at each cycle's end;
+//  * BiCG: left-preconditioned BiCG with the shadow residual r~0 = r0.
+//    The operators here are symmetric and the Jacobi scaling is diagonal,
+//    so the shadow sequence (r~, z~, p~) equals the primal one value for
+//    value (same operations on the same numbers); it is computed once.
+//
+// Data layout: the GMRES basis is restart+1 padded slots of one buffer; all
+// vectors FP64.  Each iteration is a few streaming kernels (the FD12 stencil
+// for A v, one pass of dot products, one pass that orthogonalises and takes
+// the norm, one scaling pass) and one-block control kernels that reduce the
+// per-block partial sums in a fixed order and run the scalar recurrences
+// (Hessenberg rotations, convergence test) on the device, so the host only
+// enqueues and polls.
+#include "aar_device.h"
+#include "aar_solver.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace aar {
+
+#define HC(x)                                                                         \
+    do {                                                                              \
+        hipError_t e_ = (x);                                                          \
+        if (e_ != hipSuccess)                                                         \
+            return fail(AAR_HIP_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int KMAX = AAR_HIP_GMRES_MAXK;   // largest restart
+constexpr int KB = 256;                    // threads per block
+constexpr int KP = KMAX + 1;               // partial sums per block (stride)
+
+// PETSc's KSPConvergedReason codes
+enum KReason : int {
+    K_ITERATING = 0,
+    K_CONVERGED_RTOL = 2,
+    K_CONVERGED_ATOL = 3,
+    K_DIVERGED_NULL = -2,
+    K_DIVERGED_ITS = -3,
+    K_DIVERGED_DTOL = -4,
+    K_DIVERGED_BREAKDOWN = -5,
+    K_DIVERGED_BREAKDOWN_BICG = -6,
+    K_DIVERGED_NANORINF = -9,
+};
+constexpr double K_ATOL = 1e-50, K_DTOL = 1e5, K_HAPTOL = 1e-30;
+
+// the Jacobi scaling: 1/diag per point (1 without a preconditioner)
+struct KDiag {
+    const double *c;   // variable coefficient c(x) (diagonal diag + c) or null
+    double diag, dinv;
+    int precond;
+};
+
+__device__ __forceinline__ double kdinv(const KDiag &d, long i)
+{
+    if (!d.precond) return 1.0;
+    return d.c ? 1.0 / (d.diag + d.c[i]) : d.dinv;
+}
+
+struct KCtrl {
+    Ctrl c;        // c.done: the solve has a reason (stencil early exit, host polling)
+    Ctrl gate;     // gate.done: no Arnoldi step pending (early exit of the step's kernels)
+    int reason, its, it, open, noscale, nosol, nh, hcap;
+    int max_it, restart, first, pad;
+    double rtol, ttol, rnorm0, bpnorm, rhs_norm, scale, relres;
+    double beta, betaold, bcoef, alpha;
+    double h[KMAX + 1];
+    double HH[(KMAX + 2) * (KMAX + 1)];   // column-major, columns of KMAX + 2
+    double cc[KMAX + 1], ss[KMAX + 1], grs[KMAX + 2], nrs[KMAX + 1];
+};
+
+__device__ __forceinline__ double &HHe(KCtrl *k, int r, int c) { return k->HH[c * (KMAX + 2) + r]; }
+
+__device__ int k_converged(KCtrl *k, int n, double rnorm)
+{
+    if (n == 0) {
+        k->ttol = fmax(k->rtol * k->bpnorm, K_ATOL);
+        k->rnorm0 = rnorm;
+    }
+    if (isnan(rnorm) || isinf(rnorm)) return K_DIVERGED_NANORINF;
+    if (rnorm <= k->ttol) return rnorm < K_ATOL ? K_CONVERGED_ATOL : K_CONVERGED_RTOL;
+    if (rnorm >= K_DTOL * k->rnorm0) return K_DIVERGED_DTOL;
+    return K_ITERATING;
+}
+
+__device__ void k_push(KCtrl *k, double *hist, double v)
+{
+    if (hist && k->nh < k->hcap) hist[k->nh] = v;
+    k->nh++;
+}
+
+__device__ void k_stop(KCtrl *k)
+{
+    k->c.done = 1;
+    k->c.final_iter = k->its;
+    k->gate.done = 1;
+}
+
+// Block sums of NV per-thread accumulators (the first nv used) into
+// part[blockIdx.x * KP + j], fixed order.
+template <int NV>
+__device__ void k_block_sums(const double (&acc)[NV], int nv, double *part)
+{
+    __shared__ double red[KB / 64][NV];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < NV; j++) {
+        if (j < nv) {
+            const double v = wave_sum(acc[j]);
+            if (lane == 0) red[wv][j] = v;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < nv) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < KB / 64; w++) s += red[w][threadIdx.x];
+        part[(size_t)blockIdx.x * KP + threadIdx.x] = s;
+    }
+}
+
+// One-block reduction of the nv partial sums of nb blocks, fixed order;
+// out[j] valid for every thread after return.
+__device__ void k_reduce(const double *part, int nb, int nv, double *out)
+{
+    __shared__ double red[KB / 64][KP];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int j = 0; j < nv; j++) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nb; b += KB) s += part[(size_t)b * KP + j];
+        s = wave_sum(s);
+        if (lane == 0) red[wv][j] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < nv) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < KB / 64; w++) s += red[w][threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// streaming kernels (grid-stride over the n points)
+// ---------------------------------------------------------------------------
+#define KLOOP(i) for (long i = (long)blockIdx.x * KB + threadIdx.x; i < n; i += (long)gridDim.x * KB)
+
+__global__ __launch_bounds__(KB) void k_zero_kernel(unsigned *p, long words)
+{
+    for (long i = (long)blockIdx.x * KB + threadIdx.x; i < words; i += (long)gridDim.x * KB) p[i] = 0u;
+}
+
+// ||B b||^2 and ||b||^2
+__global__ __launch_bounds__(KB) void k_bnorm_kernel(long n, const double *b, KDiag dg, double *part)
+{
+    double acc[2] = {0.0, 0.0};
+    KLOOP(i)
+    {
+        const double bi = b[i];
+        const double z = kdinv(dg, i) * bi;
+        acc[0] += z * z;
+        acc[1] += bi * bi;
+    }
+    k_block_sums<2>(acc, 2, part);
+}
+
+// ||b - t||^2 (t = A x): the true residual of the returned solution
+__global__ __launch_bounds__(KB) void k_resnorm_kernel(long n, const double *b, const double *t, double *part)
+{
+    double acc[1] = {0.0};
+    KLOOP(i)
+    {
+        const double r = b[i] - t[i];
+        acc[0] += r * r;
+    }
+    k_block_sums<1>(acc, 1, part);
+}
+
+// GMRES cycle start: v0 = B (b - t), t = A x; ||v0||^2
+__global__ __launch_bounds__(KB) void k_gm_resid_kernel(long n, const double *b, const double *t, double *v0,
+                                                       KDiag dg, double *part, const int *skip)
+{
+    if (*skip) return;
+    double acc[1] = {0.0};
+    KLOOP(i)
+    {
+        const double v = kdinv(dg, i) * (b[i] - t[i]);
+        v0[i] = v;
+        acc[0] += v * v;
+    }
+    k_block_sums<1>(acc, 1, part);
+}
+
+// h_j = <B t, v_j>, j < nv (t = A v_{nv-1})
+template <int NV>
+__global__ __launch_bounds__(KB) void k_gm_mdot_kernel(long n, const double *t, const double *V, long ld, int nv,
+                                                      KDiag dg, double *part, const int *skip)
+{
+    if (*skip) return;
+    double acc[NV];
+#pragma unroll
+    for (int j = 0; j < NV; j++) acc[j] = 0.0;
+    KLOOP(i)
+    {
+        const double w = kdinv(dg, i) * t[i];
+#pragma unroll
+        for (int j = 0; j < NV; j++)
+            if (j < nv) acc[j] += w * V[(size_t)j * ld + i];
+    }
+    k_block_sums<NV>(acc, nv, part);
+}
+
+// v_nv = B t - sum_j h_j v_j (classical Gram-Schmidt), ||v_nv||^2
+template <int NV>
+__global__ __launch_bounds__(KB) void k_gm_orth_kernel(long n, const double *t, double *V, long ld, int nv,
+                                                      KDiag dg, const KCtrl *k, double *part)
+{
+    if (k->gate.done) return;
+    double nh[NV];
+#pragma unroll
+    for (int j = 0; j < NV; j++) nh[j] = j < nv ? -k->h[j] : 0.0;
+    double *out = V + (size_t)nv * ld;
+    double acc[1] = {0.0};
+    KLOOP(i)
+    {
+        double w = kdinv(dg, i) * t[i];
+#pragma unroll
+        for (int j = 0; j < NV; j++)
+            if (j < nv) w = w + nh[j] * V[(size_t)j * ld + i];
+        out[i] = w;
+        acc[0] += w * w;
+    }
+    k_block_sums<1>(acc, 1, part);
+}
+
+// v *= scale (VecNormalize), unless the control says otherwise; an
+// Arnoldi step's vector (gated) only when that step ran
+__global__ __launch_bounds__(KB) void k_gm_scale_kernel(long n, double *v, const KCtrl *k, int gated)
+{
+    if (k->noscale || (gated && k->gate.done)) return;
+    const double s = k->scale;
+    KLOOP(i) v[i] = s * v[i];
+}
+
+// x += sum_j nrs_j v_j, j < it (the cycle's solution update)
+template <int NV>
+__global__ __launch_bounds__(KB) void k_gm_soln_kernel(long n, double *x, const double *V, long ld, const KCtrl *k)
+{
+    if (k->nosol) return;
+    const int nv = k->it;
+    double c[NV];
+#pragma unroll
+    for (int j = 0; j < NV; j++) c[j] = j < nv ? k->nrs[j] : 0.0;
+    KLOOP(i)
+    {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < NV; j++)
+            if (j < nv) s = s + c[j] * V[(size_t)j * ld + i];
+        x[i] = x[i] + s;
+    }
+}
+
+// BiCG start: r = b - t (t = A x), z = B r; ||z||^2, <z, r>
+__global__ __launch_bounds__(KB) void k_bi_init_kernel(long n, const double *b, const double *t, double *r, double *z,
+                                                      KDiag dg, double *part)
+{
+    double acc[2] = {0.0, 0.0};
+    KLOOP(i)
+    {
+        const double ri = b[i] + (-1.0) * t[i];
+        const double zi = kdinv(dg, i) * ri;
+        r[i] = ri;
+        z[i] = zi;
+        acc[0] += zi * zi;
+        acc[1] += zi * ri;
+    }
+    k_block_sums<2>(acc, 2, part);
+}
+
+// p = z (first iteration) or z + (beta/betaold) p
+__global__ __launch_bounds__(KB) void k_bi_p_kernel(long n, const double *z, double *p, const KCtrl *k)
+{
+    if (k->c.done) return;
+    if (k->first) {
+        KLOOP(i) p[i] = z[i];
+    } else {
+        const double bc = k->bcoef;
+        KLOOP(i) p[i] = z[i] + bc * p[i];
+    }
+}
+
+// <q, p> (q = A p)
+__global__ __launch_bounds__(KB) void k_bi_dot_kernel(long n, const double *q, const double *p, double *part,
+                                                     const KCtrl *k)
+{
+    if (k->c.done) return;
+    double acc[1] = {0.0};
+    KLOOP(i) acc[0] += q[i] * p[i];
+    k_block_sums<1>(acc, 1, part);
+}
+
+// x += a p; r -= a q; z = B r; ||z||^2, <z, r>
+__global__ __launch_bounds__(KB) void k_bi_upd_kernel(long n, double *x, double *r, double *z, const double *p,
+                                                     const double *q, KDiag dg, double *part, const KCtrl *k)
+{
+    if (k->c.done) return;
+    const double al = k->alpha, ma = -k->alpha;
+    double acc[2] = {0.0, 0.0};
+    KLOOP(i)
+    {
+        x[i] = x[i] + al * p[i];
+        const double ri = r[i] + ma * q[i];
+        const double zi = kdinv(dg, i) * ri;
+        r[i] = ri;
+        z[i] = zi;
+        acc[0] += zi * zi;
+        acc[1] += zi * ri;
+    }
+    k_block_sums<2>(acc, 2, part);
+}
+
+// ---------------------------------------------------------------------------
+// control kernels (one block; thread 0 runs the scalar recurrences)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(KB) void k_ctl_init_kernel(KCtrl *k, double rtol, int max_it, int restart, int hcap,
+                                                       const double *part, int nb)
+{
+    __shared__ double s[KP];
+    k_reduce(part, nb, 2, s);
+    if (threadIdx.x != 0) return;
+    k->rtol = rtol;
+    k->max_it = max_it;
+    k->restart = restart;
+    k->hcap = hcap;
+    k->bpnorm = sqrt(s[0]);
+    k->rhs_norm = sqrt(s[1]);
+    k->c.rhs_norm = k->rhs_norm;
+    k->gate.done = 1;
+    k->nosol = 1;
+    k->noscale = 1;
+}
+
+// GMRES cycle start (KSPGMRESCycle's prologue): normalise v0, test
+__global__ __launch_bounds__(KB) void k_gm_start_kernel(KCtrl *k, const double *part, int nb, double *hist)
+{
+    if (k->c.done) return;
+    __shared__ double s[KP];
+    k_reduce(part, nb, 1, s);
+    if (threadIdx.x != 0) return;
+    const double res = sqrt(s[0]);
+    k->open = 1;
+    k->it = 0;
+    k->grs[0] = res;
+    k->scale = 1.0 / res;
+    k->noscale = (res == 0.0 || res == 1.0);
+    k_push(k, hist, res);
+    k->reason = res == 0.0 ? K_CONVERGED_ATOL : k_converged(k, k->its, res);
+    const bool active = !k->reason && k->restart > 0 && k->its < k->max_it;
+    k->gate.done = active ? 0 : 1;
+}
+
+__global__ __launch_bounds__(KB) void k_gm_dots_kernel(KCtrl *k, const double *part, int nb, int nv)
+{
+    if (k->gate.done) return;
+    __shared__ double s[KP];
+    k_reduce(part, nb, nv, s);
+    if (threadIdx.x < nv) k->h[threadIdx.x] = s[threadIdx.x];
+}
+
+// after the orthogonalisation: the new Hessenberg column, its rotations
+// (KSPGMRESUpdateHessenberg), the residual estimate and the test
+__global__ __launch_bounds__(KB) void k_gm_arnoldi_kernel(KCtrl *k, const double *part, int nb, double *hist)
+{
+    if (k->gate.done) return;
+    __shared__ double s[KP];
+    k_reduce(part, nb, 1, s);
+    if (threadIdx.x != 0) return;
+    const int it = k->it;
+    const double tt = sqrt(s[0]);
+    for (int j = 0; j <= it; j++) HHe(k, j, it) = k->h[j];
+    HHe(k, it + 1, it) = tt;
+    k->scale = 1.0 / tt;
+    double hapbnd = fabs(tt / k->grs[it]);
+    if (hapbnd > K_HAPTOL) hapbnd = K_HAPTOL;
+    const bool hapend = tt < hapbnd;
+    double *hh = &HHe(k, 0, it);
+    for (int j = 1; j <= it; j++) {
+        const double t1 = hh[j - 1];
+        hh[j - 1] = k->cc[j - 1] * t1 + k->ss[j - 1] * hh[j];
+        hh[j] = k->cc[j - 1] * hh[j] - k->ss[j - 1] * t1;
+    }
+    double res = 0.0;
+    int reason = K_ITERATING;
+    if (!hapend) {
+        const double r = sqrt(hh[it] * hh[it] + hh[it + 1] * hh[it + 1]);
+        if (r == 0.0) {
+            reason = K_DIVERGED_NULL;
+        } else {
+            k->cc[it] = hh[it] / r;
+            k->ss[it] = hh[it + 1] / r;
+            k->grs[it + 1] = -(k->ss[it] * k->grs[it]);
+            k->grs[it] = k->cc[it] * k->grs[it];
+            hh[it] = k->cc[it] * hh[it] + k->ss[it] * hh[it + 1];
+            res = fabs(k->grs[it + 1]);
+        }
+    }
+    k->it = it + 1;
+    k->its++;
+    k_push(k, hist, res);
+    if (!reason) reason = k_converged(k, k->its, res);
+    if (hapend && !reason) reason = K_DIVERGED_BREAKDOWN;
+    k->reason = reason;
+    const bool active = !reason && !hapend && k->it < k->restart && k->its < k->max_it;
+    k->gate.done = active ? 0 : 1;
+    k->noscale = (!active || tt == 0.0 || tt == 1.0);
+}
+
+// KSPGMRESBuildSoln: back-substitution for the cycle's coefficients
+__global__ void k_gm_soln_ctl_kernel(KCtrl *k)
+{
+    if (!k->open || k->it == 0) {
+        k->nosol = 1;
+        return;
+    }
+    const int kk = k->it - 1;
+    k->nrs[kk] = HHe(k, kk, kk) != 0.0 ? k->grs[kk] / HHe(k, kk, kk) : 0.0;
+    for (int ii = 1; ii <= kk; ii++) {
+        const int r = kk - ii;
+        double tt = k->grs[r];
+        for (int j = r + 1; j <= kk; j++) tt = tt - HHe(k, r, j) * k->nrs[j];
+        k->nrs[r] = tt / HHe(k, r, r);
+    }
+    k->nosol = 0;
+}
+
+__global__ void k_gm_end_kernel(KCtrl *k)
+{
+    if (!k->open) return;
+    if (k->its >= k->max_it && !k->reason) k->reason = K_DIVERGED_ITS;
+    if (k->reason) k_stop(k);
+    k->open = 0;
+    k->nosol = 1;
+    k->noscale = 1;
+    k->gate.done = 1;
+}
+
+__global__ __launch_bounds__(KB) void k_bi_start_kernel(KCtrl *k, const double *part, int nb, double *hist)
+{
+    __shared__ double s[KP];
+    k_reduce(part, nb, 2, s);
+    if (threadIdx.x != 0) return;
+    const double dp = sqrt(s[0]);
+    k->beta = s[1];
+    k->first = 1;
+    k_push(k, hist, dp);
+    int reason = k_converged(k, 0, dp);
+    if (!reason && k->beta == 0.0) reason = K_DIVERGED_BREAKDOWN_BICG;
+    k->reason = reason;
+    if (reason) k_stop(k);
+}
+
+__global__ __launch_bounds__(KB) void k_bi_alpha_kernel(KCtrl *k, const double *part, int nb)
+{
+    if (k->c.done) return;
+    __shared__ double s[KP];
+    k_reduce(part, nb, 1, s);
+    if (threadIdx.x != 0) return;
+    k->alpha = k->beta / s[0];
+}
+
+__global__ __launch_bounds__(KB) void k_bi_step_kernel(KCtrl *k, const double *part, int nb, double *hist)
+{
+    if (k->c.done) return;
+    __shared__ double s[KP];
+    k_reduce(part, nb, 2, s);
+    if (threadIdx.x != 0) return;
+    const double dp = sqrt(s[0]);
+    k->its++;
+    k_push(k, hist, dp);
+    int reason = k_converged(k, k->its, dp);
+    if (!reason && k->its >= k->max_it) reason = K_DIVERGED_ITS;
+    k->reason = reason;
+    if (reason) {
+        k_stop(k);
+        return;
+    }
+    k->betaold = k->beta;
+    k->beta = s[1];
+    k->bcoef = k->beta / k->betaold;
+    k->first = 0;
+}
+
+__global__ __launch_bounds__(KB) void k_final_kernel(KCtrl *k, const double *part, int nb)
+{
+    __shared__ double s[KP];
+    k_reduce(part, nb, 1, s);
+    if (threadIdx.x != 0) return;
+    k->relres = sqrt(s[0]);
+    k->c.relres = k->relres;
+}
+
+// ---------------------------------------------------------------------------
+// host drivers
+// ---------------------------------------------------------------------------
+namespace {
+
+struct KrylovRun {
+    KCtrl *k = nullptr;
+    double *part = nullptr, *hist = nullptr;
+    int nb = 0, hcap = 0;
+    long n = 0;
+    KDiag dg{};
+};
+
+int k_setup(const Op &op, int precond, double tol, int max_it, int restart, int hcap, const double *b,
+            KrylovRun &R)
+{
+    Context &C = ctx();
+    R.n = op.n;
+    R.nb = std::min(nblocks_for(op.n), 1024);
+    R.hcap = hcap;
+    HC(ensure(C.kctl, sizeof(KCtrl)));
+    HC(ensure(C.hist0, sizeof(double) * (size_t)hcap));
+    HC(ensure(C.part0, sizeof(double) * (size_t)R.nb * KP));
+    R.k = (KCtrl *)C.kctl.p;
+    R.part = (double *)C.part0.p;
+    R.hist = (double *)C.hist0.p;
+    R.dg.c = op.c;
+    R.dg.diag = op.diag;
+    R.dg.dinv = 1.0 / op.diag;
+    R.dg.precond = precond;
+    const long words = (long)(sizeof(KCtrl) / sizeof(unsigned));
+    hipLaunchKernelGGL(k_zero_kernel, dim3(64), dim3(KB), 0, C.stream, (unsigned *)R.k, words);
+    hipLaunchKernelGGL(k_bnorm_kernel, dim3(R.nb), dim3(KB), 0, C.stream, R.n, b, R.dg, R.part);
+    hipLaunchKernelGGL(k_ctl_init_kernel, dim3(1), dim3(KB), 0, C.stream, R.k, tol, max_it, restart, hcap,
+                       (const double *)R.part, R.nb);
+    HC(hipGetLastError());
+    C.bytes_acc += 8.0 * op.n * (op.c ? 2 : 1);
+    return 0;
+}
+
+// y = A x, skipped once *gate->done is set
+int k_matvec(const Op &op, const double *x, double *y, Ctrl *gate)
+{
+    SArgs<double> a = base_sargs<double>(op);
+    a.in0 = x;
+    a.out0 = y;
+    a.ctrl = gate;
+    return run_stencil<double>(SMode::MATVEC, a, (op.c ? 3.0 : 2.0) * 8.0 * op.n);
+}
+
+int k_finish(const Op &op, const double *x, const double *b, double *t, KrylovRun &R, aar_hip_stats *st)
+{
+    Context &C = ctx();
+    {
+        // the true residual of the returned x (PETSC_REAL/main.c:112-118)
+        SArgs<double> a = base_sargs<double>(op);
+        a.in0 = x;
+        a.out0 = t;
+        if (int rc = run_stencil<double>(SMode::MATVEC, a, (op.c ? 3.0 : 2.0) * 8.0 * op.n)) return rc;
+        hipLaunchKernelGGL(k_resnorm_kernel, dim3(R.nb), dim3(KB), 0, C.stream, R.n, b, (const double *)t, R.part);
+        hipLaunchKernelGGL(k_final_kernel, dim3(1), dim3(KB), 0, C.stream, R.k, (const double *)R.part, R.nb);
+        HC(hipGetLastError());
+    }
+    HC(hipEventRecord(C.t1, C.stream));
+    KCtrl hk;
+    HC(hipMemcpyAsync(&hk, R.k, sizeof(KCtrl), hipMemcpyDeviceToHost, C.stream));
+    HC(hipStreamSynchronize(C.stream));
+    float ms = 0.f;
+    HC(hipEventElapsedTime(&ms, C.t0, C.t1));
+    prof_flush();
+    const int hl = std::min(hk.nh, R.hcap);
+    C.hist_host.assign(std::max(hl, 0), 0.0);
+    if (hl > 0) HC(hipMemcpy(C.hist_host.data(), R.hist, sizeof(double) * hl, hipMemcpyDeviceToHost));
+    aar_hip_stats s;
+    std::memset(&s, 0, sizeof(s));
+    s.iterations = hk.its;
+    s.rhs_norm = hk.rhs_norm;
+    s.relres = hk.rhs_norm > 0 ? hk.relres / hk.rhs_norm : hk.relres;
+    s.converged = hk.reason > 0;
+    s.printed_converged = hk.reason > 0;
+    s.printed_warning = hk.reason == K_DIVERGED_ITS;
+    s.status = hk.reason == K_DIVERGED_NANORINF ? AAR_HIP_ERR_NONFINITE : 0;
+    s.hist_len = hl;
+    s.solve_ms = ms;
+    s.bytes = C.bytes_acc;
+    s.reason = hk.reason;
+    C.last = s;
+    if (st) *st = s;
+    if (hk.reason == K_DIVERGED_NANORINF) return fail(AAR_HIP_ERR_NONFINITE, "residual is not finite");
+    return 0;
+}
+
+template <int NV>
+hipError_t launch_gm_step(const Op &op, const double *t, double *V, long ld, int nv, KrylovRun &R)
+{
+    Context &C = ctx();
+    hipLaunchKernelGGL((k_gm_mdot_kernel<NV>), dim3(R.nb), dim3(KB), 0, C.stream, R.n, t, (const double *)V, ld, nv,
+                       R.dg, R.part, (const int *)&R.k->gate.done);
+    hipLaunchKernelGGL(k_gm_dots_kernel, dim3(1), dim3(KB), 0, C.stream, R.k, (const double *)R.part, R.nb, nv);
+    hipLaunchKernelGGL((k_gm_orth_kernel<NV>), dim3(R.nb), dim3(KB), 0, C.stream, R.n, t, V, ld, nv, R.dg,
+                       (const KCtrl *)R.k, R.part);
+    return hipGetLastError();
+}
+
+template <int NV>
+hipError_t launch_gm_soln(double *x, const double *V, long ld, KrylovRun &R)
+{
+    hipLaunchKernelGGL((k_gm_soln_kernel<NV>), dim3(R.nb), dim3(KB), 0, ctx().stream, R.n, x, V, ld,
+                       (const KCtrl *)R.k);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+int gmres_solve(const Op &op, double *x, const double *b, int restart, int max_it, double tol, int precond,
+                aar_hip_stats *st)
+{
+    Context &C = ctx();
+    if (int rc = context_init()) return rc;
+    if (!x || !b) return fail(AAR_HIP_ERR_ARG, "null vector");
+    if (max_it < 0) return fail(AAR_HIP_ERR_ARG, "max_iter must be >= 0");
+    if (restart < 1 || restart > KMAX) return fail(AAR_HIP_ERR_ARG, "restart must be in 1..32");
+    const long n = op.n;
+    const long ld = ring_ld(n, sizeof(double));
+    // basis v_0 .. v_restart, then t
+    HC(ensure(C.kvec, sizeof(double) * (size_t)ld * (restart + 2)));
+    double *V = (double *)C.kvec.p, *t = V + (size_t)ld * (restart + 1);
+    const int hcap = max_it + max_it / restart + 8;
+    C.bytes_acc = 0.0;
+    HC(hipEventRecord(C.t0, C.stream));
+    KrylovRun R;
+    if (int rc = k_setup(op, precond, tol, max_it, restart, hcap, b, R)) return rc;
+    const double s = 8.0 * n;
+    const double cb = op.c ? s : 0.0;      // the coefficient field's reads, where 1/diag is per point
+    bool stopped = false;
+    for (int cyc = 0; cyc <= max_it && !stopped; cyc++) {
+        // v0 = B (b - A x), ||v0||, test
+        if (int rc = k_matvec(op, x, t, &R.k->c)) return rc;
+        hipLaunchKernelGGL(k_gm_resid_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, b, (const double *)t, V, R.dg,
+                           R.part, (const int *)&R.k->c.done);
+        hipLaunchKernelGGL(k_gm_start_kernel, dim3(1), dim3(KB), 0, C.stream, R.k, (const double *)R.part, R.nb,
+                           R.hist);
+        hipLaunchKernelGGL(k_gm_scale_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, V, (const KCtrl *)R.k, 0);
+        HC(hipGetLastError());
+        C.bytes_acc += 5 * s + cb;
+        for (int it = 0; it < restart; it++) {
+            const int nv = it + 1;
+            if (int rc = k_matvec(op, V + (size_t)it * ld, t, &R.k->gate)) return rc;
+            prof_begin("krylov_gmres_orth");
+            hipError_t e = nv <= 8 ? launch_gm_step<8>(op, t, V, ld, nv, R)
+                         : nv <= 16 ? launch_gm_step<16>(op, t, V, ld, nv, R)
+                                    : launch_gm_step<32>(op, t, V, ld, nv, R);
+            prof_end((2.0 * nv + 3) * s + 2 * cb);
+            if (e != hipSuccess) return fail(AAR_HIP_ERR_HIP, std::string("gmres step: ") + hipGetErrorString(e));
+            hipLaunchKernelGGL(k_gm_arnoldi_kernel, dim3(1), dim3(KB), 0, C.stream, R.k, (const double *)R.part, R.nb,
+                               R.hist);
+            hipLaunchKernelGGL(k_gm_scale_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, V + (size_t)nv * ld,
+                               (const KCtrl *)R.k, 1);
+            HC(hipGetLastError());
+            C.bytes_acc += (2.0 * nv + 5) * s + 2 * cb;
+        }
+        hipLaunchKernelGGL(k_gm_soln_ctl_kernel, dim3(1), dim3(1), 0, C.stream, R.k);
+        hipError_t e = restart <= 8 ? launch_gm_soln<8>(x, V, ld, R)
+                     : restart <= 16 ? launch_gm_soln<16>(x, V, ld, R)
+                                     : launch_gm_soln<32>(x, V, ld, R);
+        if (e != hipSuccess) return fail(AAR_HIP_ERR_HIP, std::string("gmres solution: ") + hipGetErrorString(e));
+        hipLaunchKernelGGL(k_gm_end_kernel, dim3(1), dim3(1), 0, C.stream, R.k);
+        HC(hipGetLastError());
+        C.bytes_acc += (restart + 2.0) * s;
+        if (int rc = poll_record(&R.k->c, cyc)) return rc;
+        if (poll_check(cyc - 2)) stopped = true;
+    }
+    return k_finish(op, x, b, t, R, st);
+}
+
+int bicg_solve(const Op &op, double *x, const double *b, int max_it, double tol, int precond, aar_hip_stats *st)
+{
+    Context &C = ctx();
+    if (int rc = context_init()) return rc;
+    if (!x || !b) return fail(AAR_HIP_ERR_ARG, "null vector");
+    if (max_it < 0) return fail(AAR_HIP_ERR_ARG, "max_iter must be >= 0");
+    const long n = op.n;
+    const long ld = ring_ld(n, sizeof(double));
+    HC(ensure(C.kvec, sizeof(double) * (size_t)ld * 4));
+    double *r = (double *)C.kvec.p, *z = r + ld, *p = r + 2 * ld, *q = r + 3 * ld;
+    const int hcap = max_it + 8;
+    C.bytes_acc = 0.0;
+    HC(hipEventRecord(C.t0, C.stream));
+    KrylovRun R;
+    if (int rc = k_setup(op, precond, tol, max_it, 0, hcap, b, R)) return rc;
+    const double s = 8.0 * n;
+    const double cb = op.c ? s : 0.0;
+    if (int rc = k_matvec(op, x, q, nullptr)) return rc;
+    hipLaunchKernelGGL(k_bi_init_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, b, (const double *)q, r, z, R.dg,
+                       R.part);
+    hipLaunchKernelGGL(k_bi_start_kernel, dim3(1), dim3(KB), 0, C.stream, R.k, (const double *)R.part, R.nb, R.hist);
+    HC(hipGetLastError());
+    C.bytes_acc += 4 * s + cb;
+    bool stopped = false;
+    const int loops = std::max(max_it, 1);    // PETSc's do-while runs one step even at max_it = 0
+    int cyc = 0;
+    for (int i = 0; i < loops && !stopped; i++) {
+        hipLaunchKernelGGL(k_bi_p_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, (const double *)z, p,
+                           (const KCtrl *)R.k);
+        if (int rc = k_matvec(op, p, q, &R.k->c)) return rc;
+        prof_begin("krylov_bicg_update");
+        hipLaunchKernelGGL(k_bi_dot_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, (const double *)q,
+                           (const double *)p, R.part, (const KCtrl *)R.k);
+        hipLaunchKernelGGL(k_bi_alpha_kernel, dim3(1), dim3(KB), 0, C.stream, R.k, (const double *)R.part, R.nb);
+        hipLaunchKernelGGL(k_bi_upd_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, x, r, z, (const double *)p,
+                           (const double *)q, R.dg, R.part, (const KCtrl *)R.k);
+        hipLaunchKernelGGL(k_bi_step_kernel, dim3(1), dim3(KB), 0, C.stream, R.k, (const double *)R.part, R.nb,
+                           R.hist);
+        prof_end(9 * s + cb);
+        HC(hipGetLastError());
+        C.bytes_acc += 12 * s + cb;
+        if ((i + 1) % 8 == 0) {
+            if (int rc = poll_record(&R.k->c, cyc)) return rc;
+            if (poll_check(cyc - 2)) stopped = true;
+            cyc++;
+        }
+    }
+    return k_finish(op, x, b, q, R, st);
+}
+
+}  // namespace aar

@@ -758,6 +758,7 @@ template int aar_solve<double>(const Op &, double *, const double *, double, dou
 template SArgs<double> base_sargs<double>(const Op &);
 template SArgs<float> base_sargs<float>(const Op &);
 template hipError_t run_dot<double>(DotArgs<double>);
+template int run_stencil<double>(SMode, SArgs<double>, double);
 template int aar_solve<float>(const Op &, float *, const float *, double, double, int, int, int, double,
                               int, int, aar_hip_stats *, int, int);
 
@@ -1503,7 +1504,7 @@ void release_all()
     Context &C = ctx();
     DevBuf *bufs[] = {&C.ring64x, &C.ring64f, &C.ring32x, &C.ring32f, &C.hist0, &C.hist1,
                       &C.part0, &C.part1, &C.vec32a, &C.vec32b, &C.vec64a, &C.vec64b,
-                      &C.ringc, &C.corrc, &C.vch};
+                      &C.ringc, &C.corrc, &C.vch, &C.kctl, &C.kvec};
     for (DevBuf *b : bufs) {
         if (b->p) (void)hipFree(b->p);
         b->p = nullptr;

@@ -841,3 +841,295 @@ int orc_aar_complex(int nx, int ny, int nz, int fdn, double a,
     free(E); free(e); free(se); free(Gr); free(Gi);
     return status;
 }
+
+/* ------------------------------------------------------------------ */
+/* Krylov comparators of the PETSc variant (PETSC_REAL/main.c:86-150): */
+/* KSPGMRES and KSPBICG with PCJACOBI, rtol 1e-6, max_it 2000 and a    */
+/* nonzero initial guess (main.c:97-101, 131-133).  PETSc (3.11.3 in   */
+/* the reference's README) is neither vendored nor installed; these    */
+/* follow its published algorithms and defaults:                       */
+/*  - KSPConvergedDefault (src/ksp/ksp/interface/iterativ.c): at n = 0 */
+/*    ttol = max(rtol*||B b||, atol) for a nonzero guess, rnorm0 =     */
+/*    rnorm; converged when rnorm <= ttol (ATOL if rnorm < atol, else  */
+/*    RTOL); diverged when rnorm >= dtol*rnorm0; atol 1e-50, dtol 1e5; */
+/*  - PCJACOBI (src/ksp/pc/impls/jacobi/jacobi.c): y = x * (1/diag),   */
+/*    the reciprocal taken once (the MPI_REAL Precondition divides);   */
+/*  - KSPGMRES (src/ksp/ksp/impls/gmres/gmres.c, borthog.c): restart   */
+/*    30, left preconditioning, the preconditioned residual norm,      */
+/*    classical Gram-Schmidt without refinement, Givens rotations,     */
+/*    happy-breakdown tolerance 1e-30;                                 */
+/*  - KSPBICG (src/ksp/ksp/impls/bicg/bicg.c): left-preconditioned     */
+/*    BiCG with the shadow residual r~0 = r0, preconditioned norm.     */
+/* Parity with PETSc itself is unpinned (rounding order of its BLAS    */
+/* kernels); the tests pin these routines by what the mathematics      */
+/* fixes (GMRES minimises the preconditioned residual over the Krylov */
+/* space; BiCG on a symmetric operator is PCG).                        */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    double ttol, rnorm0, rtol, bpnorm;
+} ksp_conv;
+
+static int ksp_converged(ksp_conv *c, int n, double rnorm)
+{
+    if (n == 0) {
+        c->ttol = c->rtol * c->bpnorm;
+        if (c->ttol < ORC_KSP_ATOL) c->ttol = ORC_KSP_ATOL;
+        c->rnorm0 = rnorm;
+    }
+    if (isnan(rnorm) || isinf(rnorm)) return ORC_KSP_DIVERGED_NANORINF;
+    if (rnorm <= c->ttol) return rnorm < ORC_KSP_ATOL ? ORC_KSP_CONVERGED_ATOL : ORC_KSP_CONVERGED_RTOL;
+    if (rnorm >= ORC_KSP_DTOL * c->rnorm0) return ORC_KSP_DIVERGED_DTOL;
+    return 0;
+}
+
+/* the Jacobi reciprocal 1/diag per point (1 without a preconditioner) */
+static double *ksp_dinv(long n, double diag, const double *cv, int precond)
+{
+    double *d = (double *)malloc(sizeof(double) * n);
+    for (long i = 0; i < n; i++) d[i] = precond ? 1.0 / (cv ? diag + cv[i] : diag) : 1.0;
+    return d;
+}
+
+/* VecNormalize: scale by 1/norm unless the norm is 0 or 1 */
+static double ksp_normalize(double *v, long n)
+{
+    double nrm = orc_norm2(v, n);
+    if (nrm != 0.0 && nrm != 1.0) {
+        const double s = 1.0 / nrm;
+        for (long i = 0; i < n; i++) v[i] = s * v[i];
+    }
+    return nrm;
+}
+
+static void ksp_finish(orc_stats *st, int its, int reason, const double *x, const double *b, double *t,
+                       int nx, int ny, int nz, int fdn, double a, double sigma, const double *cv,
+                       const double *coeff, int tlen, double secs)
+{
+    if (!st) return;
+    const long n = (long)nx * ny * nz;
+    OPV(x, t);
+    for (long i = 0; i < n; i++) t[i] = b[i] - t[i];
+    const double bn = orc_norm2(b, n);
+    st->iterations = its;
+    st->relres = orc_norm2(t, n) / bn;      /* main.c:112-118: ||b - A x|| / ||b|| */
+    st->rhs_norm = bn;
+    st->converged = reason > 0;
+    st->printed_converged = reason > 0;
+    st->printed_warning = reason == ORC_KSP_DIVERGED_ITS;
+    st->trace_len = tlen;
+    st->seconds = secs;
+    st->status = reason;
+}
+
+static int gmres_impl(int nx, int ny, int nz, int fdn, double a, double sigma, const double *cv,
+                      double *x, const double *b, int restart, int max_it, double rtol, int precond,
+                      double *trace, int trace_cap, orc_stats *st)
+{
+    if (restart < 1 || restart > ORC_GMRES_MAXK) return -1;
+    const long n = (long)nx * ny * nz;
+    const int K = restart;
+    double coeff[32] = {0};
+    orc_lap_coefficient(coeff, fdn);
+    double diag = a * (3 * coeff[0]);
+    if (sigma != 0.0) diag += sigma;
+    double *dinv = ksp_dinv(n, diag, cv, precond);
+    double *V = (double *)malloc(sizeof(double) * n * (K + 1));
+    double *t = (double *)malloc(sizeof(double) * n);
+    double HH[(ORC_GMRES_MAXK + 2) * (ORC_GMRES_MAXK + 1)];
+    double cc[ORC_GMRES_MAXK + 1], ss[ORC_GMRES_MAXK + 1], grs[ORC_GMRES_MAXK + 2], nrs[ORC_GMRES_MAXK + 1];
+    double h[ORC_GMRES_MAXK + 1];
+#define HHE(r, c) HH[(c) * (K + 2) + (r)]
+    int tlen = 0;
+    ksp_conv cv_ = {0.0, 0.0, rtol, 0.0};
+    for (long i = 0; i < n; i++) t[i] = dinv[i] * b[i];
+    cv_.bpnorm = orc_norm2(t, n);                       /* ||B b|| */
+
+    int its = 0, reason = 0;
+    double t0 = now_seconds();
+    while (!reason) {
+        /* KSPInitialResidual: v0 = B (b - A x) */
+        double *v0 = V;
+        OPV(x, t);
+        for (long i = 0; i < n; i++) v0[i] = dinv[i] * (b[i] - t[i]);
+        double res = ksp_normalize(v0, n);
+        grs[0] = res;
+        int it = 0;
+        if (trace && tlen < trace_cap) trace[tlen++] = res;
+        if (res == 0.0) {
+            reason = ORC_KSP_CONVERGED_ATOL;
+            break;
+        }
+        reason = ksp_converged(&cv_, its, res);
+        while (!reason && it < K && its < max_it) {
+            double *vn = V + (long)(it + 1) * n;
+            OPV(V + (long)it * n, t);
+            for (long i = 0; i < n; i++) vn[i] = dinv[i] * t[i];
+            /* classical Gram-Schmidt: h_j = <w, v_j>, then w -= sum_j h_j v_j */
+            for (int j = 0; j <= it; j++) h[j] = orc_dot(vn, V + (long)j * n, n);
+            for (long i = 0; i < n; i++) {
+                double w = vn[i];
+                for (int j = 0; j <= it; j++) w = w + (-h[j]) * V[(long)j * n + i];
+                vn[i] = w;
+            }
+            for (int j = 0; j <= it; j++) HHE(j, it) = h[j];
+            const double tt = ksp_normalize(vn, n);
+            HHE(it + 1, it) = tt;
+            double hapbnd = fabs(tt / grs[it]);
+            if (hapbnd > ORC_GMRES_HAPTOL) hapbnd = ORC_GMRES_HAPTOL;
+            const int hapend = tt < hapbnd;
+            /* KSPGMRESUpdateHessenberg: earlier rotations, then a new one */
+            double *hh = &HHE(0, it);
+            for (int j = 1; j <= it; j++) {
+                const double t1 = hh[j - 1];
+                hh[j - 1] = cc[j - 1] * t1 + ss[j - 1] * hh[j];
+                hh[j] = cc[j - 1] * hh[j] - ss[j - 1] * t1;
+            }
+            if (!hapend) {
+                const double r = sqrt(hh[it] * hh[it] + hh[it + 1] * hh[it + 1]);
+                if (r == 0.0) {
+                    reason = ORC_KSP_DIVERGED_NULL;
+                } else {
+                    cc[it] = hh[it] / r;
+                    ss[it] = hh[it + 1] / r;
+                    grs[it + 1] = -(ss[it] * grs[it]);
+                    grs[it] = cc[it] * grs[it];
+                    hh[it] = cc[it] * hh[it] + ss[it] * hh[it + 1];
+                    res = fabs(grs[it + 1]);
+                }
+            } else {
+                res = 0.0;
+            }
+            it++;
+            its++;
+            if (trace && tlen < trace_cap) trace[tlen++] = res;
+            if (!reason) reason = ksp_converged(&cv_, its, res);
+            if (hapend) {
+                if (!reason) reason = ORC_KSP_DIVERGED_BREAKDOWN;
+                break;
+            }
+        }
+        /* KSPGMRESBuildSoln: back-substitution, x += sum_j nrs_j v_j */
+        if (it > 0) {
+            const int k = it - 1;
+            nrs[k] = HHE(k, k) != 0.0 ? grs[k] / HHE(k, k) : 0.0;
+            for (int ii = 1; ii <= k; ii++) {
+                const int kk = k - ii;
+                double tt = grs[kk];
+                for (int j = kk + 1; j <= k; j++) tt = tt - HHE(kk, j) * nrs[j];
+                nrs[kk] = tt / HHE(kk, kk);
+            }
+            for (long i = 0; i < n; i++) {
+                double s = 0.0;
+                for (int j = 0; j <= k; j++) s = s + nrs[j] * V[(long)j * n + i];
+                x[i] = x[i] + s;
+            }
+        }
+        if (its >= max_it) {
+            if (!reason) reason = ORC_KSP_DIVERGED_ITS;
+            break;
+        }
+    }
+#undef HHE
+    double t1 = now_seconds();
+    ksp_finish(st, its, reason, x, b, t, nx, ny, nz, fdn, a, sigma, cv, coeff, tlen, t1 - t0);
+    free(dinv); free(V); free(t);
+    return 0;
+}
+
+int orc_gmres(int nx, int ny, int nz, int fdn, double a, double sigma, const double *c,
+              double *x, const double *b, int restart, int max_it, double rtol, int precond,
+              double *trace, int trace_cap, orc_stats *st)
+{
+    return gmres_impl(nx, ny, nz, fdn, a, sigma, c, x, b, restart, max_it, rtol, precond, trace, trace_cap, st);
+}
+
+static int bicg_impl(int nx, int ny, int nz, int fdn, double a, double sigma, const double *cv,
+                     double *x, const double *b, int max_it, double rtol, int precond,
+                     double *trace, int trace_cap, orc_stats *st, double *shadow_gap)
+{
+    const long n = (long)nx * ny * nz;
+    double coeff[32] = {0};
+    orc_lap_coefficient(coeff, fdn);
+    double diag = a * (3 * coeff[0]);
+    if (sigma != 0.0) diag += sigma;
+    double *dinv = ksp_dinv(n, diag, cv, precond);
+    double *Rr = (double *)malloc(sizeof(double) * n), *Rl = (double *)malloc(sizeof(double) * n);
+    double *Zr = (double *)malloc(sizeof(double) * n), *Zl = (double *)malloc(sizeof(double) * n);
+    double *Pr = (double *)malloc(sizeof(double) * n), *Pl = (double *)malloc(sizeof(double) * n);
+    int tlen = 0;
+    ksp_conv cv_ = {0.0, 0.0, rtol, 0.0};
+    for (long i = 0; i < n; i++) Zr[i] = dinv[i] * b[i];
+    cv_.bpnorm = orc_norm2(Zr, n);
+    double gap = 0.0;
+
+    double t0 = now_seconds();
+    OPV(x, Rr);
+    for (long i = 0; i < n; i++) Rr[i] = b[i] + (-1.0) * Rr[i];     /* VecAYPX(Rr, -1, b) */
+    memcpy(Rl, Rr, sizeof(double) * n);
+    for (long i = 0; i < n; i++) {
+        Zr[i] = dinv[i] * Rr[i];
+        Zl[i] = dinv[i] * Rl[i];              /* B^T = B (diagonal) */
+    }
+    double dp = orc_norm2(Zr, n);
+    if (trace && tlen < trace_cap) trace[tlen++] = dp;
+    int its = 0;
+    int reason = ksp_converged(&cv_, 0, dp);
+    double beta, betaold = 0.0;
+    int i = 0;
+    if (!reason) {
+        do {
+            beta = orc_dot(Zr, Rl, n);
+            if (!i) {
+                if (beta == 0.0) {
+                    reason = ORC_KSP_DIVERGED_BREAKDOWN_BICG;
+                    break;
+                }
+                memcpy(Pr, Zr, sizeof(double) * n);
+                memcpy(Pl, Zl, sizeof(double) * n);
+            } else {
+                const double bb = beta / betaold;
+                for (long k = 0; k < n; k++) {
+                    Pr[k] = Zr[k] + bb * Pr[k];
+                    Pl[k] = Zl[k] + bb * Pl[k];
+                }
+            }
+            betaold = beta;
+            OPV(Pr, Zr);
+            OPV(Pl, Zl);                      /* A^T p~: the FD12 operator is symmetric */
+            const double dpi = orc_dot(Zr, Pl, n);
+            const double al = beta / dpi;
+            for (long k = 0; k < n; k++) {
+                x[k] = x[k] + al * Pr[k];
+                Rr[k] = Rr[k] + (-al) * Zr[k];
+                Rl[k] = Rl[k] + (-al) * Zl[k];
+            }
+            for (long k = 0; k < n; k++) {
+                Zr[k] = dinv[k] * Rr[k];
+                Zl[k] = dinv[k] * Rl[k];
+            }
+            dp = orc_norm2(Zr, n);
+            for (long k = 0; k < n; k++) {
+                const double d = fabs(Rl[k] - Rr[k]);
+                if (d > gap) gap = d;
+            }
+            its = i + 1;
+            if (trace && tlen < trace_cap) trace[tlen++] = dp;
+            reason = ksp_converged(&cv_, i + 1, dp);
+            if (reason) break;
+            i++;
+        } while (i < max_it);
+        if (i >= max_it && !reason) reason = ORC_KSP_DIVERGED_ITS;
+    }
+    double t1 = now_seconds();
+    if (shadow_gap) *shadow_gap = gap;
+    ksp_finish(st, its, reason, x, b, Rr, nx, ny, nz, fdn, a, sigma, cv, coeff, tlen, t1 - t0);
+    free(dinv); free(Rr); free(Rl); free(Zr); free(Zl); free(Pr); free(Pl);
+    return 0;
+}
+
+int orc_bicg(int nx, int ny, int nz, int fdn, double a, double sigma, const double *c,
+             double *x, const double *b, int max_it, double rtol, int precond,
+             double *trace, int trace_cap, orc_stats *st, double *shadow_gap)
+{
+    return bicg_impl(nx, ny, nz, fdn, a, sigma, c, x, b, max_it, rtol, precond, trace, trace_cap, st, shadow_gap);
+}

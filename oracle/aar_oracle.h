@@ -125,6 +125,39 @@ void orc_matvec_complex(int nx, int ny, int nz, int fdn, const double *coeff,
                         double a, const double *c, double c0, double eta,
                         const double *u, double *y);
 
+/* Krylov comparators of the PETSc variant (PETSC_REAL/main.c:86-150):
+ * PETSc's KSPGMRES (restart `restart`, classical Gram-Schmidt, left
+ * preconditioning) and KSPBICG (shadow residual r0), both with PCJACOBI
+ * (precond = 1: multiply by 1/diag, diag = a*(3*coeff[0]) + sigma, or
+ * + c(x) when c != NULL) or no preconditioner (precond = 0), converged by
+ * KSPConvergedDefault on the preconditioned residual norm with
+ * ttol = rtol*||B b||.  PETSc is absent: these restate its published
+ * algorithms (aar_oracle.c lists the sources and defaults).  trace
+ * receives the norm of every convergence test (GMRES: the start of each
+ * cycle and each Arnoldi step's estimate); stats: iterations = PETSc's
+ * its, relres = ||b - A x||/||b|| of the returned x (main.c:112-118),
+ * status = the KSPConvergedReason below.  orc_bicg's shadow_gap (may be
+ * NULL) receives max |r~ - r| over the iterations, 0 on a symmetric
+ * operator. */
+#define ORC_GMRES_MAXK 32
+#define ORC_GMRES_HAPTOL 1e-30
+#define ORC_KSP_ATOL 1e-50
+#define ORC_KSP_DTOL 1e5
+#define ORC_KSP_CONVERGED_RTOL 2
+#define ORC_KSP_CONVERGED_ATOL 3
+#define ORC_KSP_DIVERGED_NULL (-2)
+#define ORC_KSP_DIVERGED_ITS (-3)
+#define ORC_KSP_DIVERGED_DTOL (-4)
+#define ORC_KSP_DIVERGED_BREAKDOWN (-5)
+#define ORC_KSP_DIVERGED_BREAKDOWN_BICG (-6)
+#define ORC_KSP_DIVERGED_NANORINF (-9)
+int orc_gmres(int nx, int ny, int nz, int fdn, double a, double sigma, const double *c,
+              double *x, const double *b, int restart, int max_it, double rtol, int precond,
+              double *trace, int trace_cap, orc_stats *st);
+int orc_bicg(int nx, int ny, int nz, int fdn, double a, double sigma, const double *c,
+             double *x, const double *b, int max_it, double rtol, int precond,
+             double *trace, int trace_cap, orc_stats *st, double *shadow_gap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -91,7 +91,10 @@ def lib():
         L.orc_pgr_vc.argtypes = [I, I, I, I, Dd, _D, _D, _D, Dd, I, I, I, Dd, I, I, _D, I,
                                  ctypes.POINTER(OrcStats)]
         L.orc_cg_vc.argtypes = [I, I, I, I, Dd, _D, _D, _D, I, Dd, I, _D, I, ctypes.POINTER(OrcStats)]
-        for fn in ("orc_aar", "orc_pgr", "orc_cg", "orc_aar_complex", "orc_aar_vc", "orc_pgr_vc", "orc_cg_vc"):
+        L.orc_gmres.argtypes = [I, I, I, I, Dd, Dd, _D, _D, _D, I, I, Dd, I, _D, I, ctypes.POINTER(OrcStats)]
+        L.orc_bicg.argtypes = [I, I, I, I, Dd, Dd, _D, _D, _D, I, Dd, I, _D, I, ctypes.POINTER(OrcStats), _D]
+        for fn in ("orc_aar", "orc_pgr", "orc_cg", "orc_aar_complex", "orc_aar_vc", "orc_pgr_vc", "orc_cg_vc",
+                   "orc_gmres", "orc_bicg"):
             getattr(L, fn).restype = I
         _lib = L
     return _lib
@@ -232,6 +235,47 @@ def cg(x0, rhs, shape, a, max_iter, tol, sigma=0.0, precond=True, fdn=6, trace=T
     b = _f64(rhs).reshape(-1)
     d = _run(lib().orc_cg, (nx, ny, nz, fdn, a, sigma, _p(x), _p(b), max_iter, tol,
                             int(precond)), trace, max_iter // 50 + 8)
+    return x, d
+
+
+# KSPConvergedReason codes of the Krylov comparators
+KSP_CONVERGED_RTOL, KSP_CONVERGED_ATOL = 2, 3
+KSP_DIVERGED_ITS, KSP_DIVERGED_BREAKDOWN, KSP_DIVERGED_BREAKDOWN_BICG = -3, -5, -6
+
+
+def gmres(x0, rhs, shape, a, max_iter, tol, restart=30, sigma=0.0, c=None, precond=True, fdn=6,
+          trace=True):
+    """PETSc KSPGMRES + PCJACOBI as PETSC_REAL/main.c:86-122 runs it
+    (restatement of PETSc's published algorithm).  stats['status'] is the
+    KSPConvergedReason; 'trace' holds every convergence-test norm."""
+    nx, ny, nz = shape
+    x = _f64(x0).reshape(-1).copy()
+    b = _f64(rhs).reshape(-1)
+    cv = None if c is None else _f64(c).reshape(-1)
+    cap = max_iter + max_iter // max(1, restart) + 8
+    d = _run(lib().orc_gmres, (nx, ny, nz, fdn, a, sigma, _p(cv), _p(x), _p(b), restart, max_iter, tol,
+                               int(precond)), trace, cap)
+    return x, d
+
+
+def bicg(x0, rhs, shape, a, max_iter, tol, sigma=0.0, c=None, precond=True, fdn=6, trace=True):
+    """PETSc KSPBICG + PCJACOBI as PETSC_REAL/main.c:124-150 runs it
+    (restatement).  stats['shadow_gap'] = max |r~ - r| over the iterations."""
+    nx, ny, nz = shape
+    x = _f64(x0).reshape(-1).copy()
+    b = _f64(rhs).reshape(-1)
+    cv = None if c is None else _f64(c).reshape(-1)
+    gap = np.zeros(1)
+    st = OrcStats()
+    cap = max_iter + 8
+    tr = np.zeros(cap) if trace else None
+    rc = lib().orc_bicg(nx, ny, nz, fdn, a, sigma, _p(cv), _p(x), _p(b), max_iter, tol, int(precond),
+                        _p(tr), cap if trace else 0, ctypes.byref(st), _p(gap))
+    d = st.as_dict()
+    d["rc"] = rc
+    d["shadow_gap"] = float(gap[0])
+    if trace:
+        d["trace"] = tr[: st.trace_len].copy()
     return x, d
 
 

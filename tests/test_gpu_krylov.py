@@ -1,0 +1,174 @@
+"""GPU Krylov comparators (GMRES, BiCG of the PETSc variant) against the oracle.
+
+libaar_hip's aar_hip_gmres / aar_hip_bicg run PETSc's KSPGMRES / KSPBICG
+control flow with PCJACOBI (PETSC_REAL/main.c:86-150).  The oracle's
+orc_gmres / orc_bicg restate the same algorithms sequentially and are
+pinned on the CPU (tests/test_oracle_krylov.py).  The GPU sums in parallel,
+so the bar is floating-point: the same iteration count and reason, every
+convergence-test norm within 1e-6 relative or 1e-8 of the first norm (the
+recursively updated residuals of BiCG/CG carry rounding of the order of
+eps times the starting residual, which dominates deep into a 1e-10 solve),
+and the returned x within 1e-8 of the oracle's (relative to max |x|).
+"""
+import numpy as np
+import pytest
+
+from aar_hip import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def dev(v):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64)).to("cuda")
+
+
+def rhs(shape, seed):
+    return P.rhs_uniform(shape, seed)
+
+
+def op_norm(oracle, a, sigma=0.0, c=None):
+    """An upper bound of ||A||_2 for a*Lap_FD12 + sigma (+ c(x)): the
+    largest |symbol| of the stencil plus the largest diagonal term."""
+    coeff = oracle.lap_coefficient(6)
+    lap = 3 * (abs(coeff[0]) + 2 * np.abs(coeff[1:]).sum())
+    return abs(a) * lap + abs(sigma) + (0.0 if c is None else float(np.abs(c).max()))
+
+
+def check_close(gpu, st, d, xg, xo, b, anorm, trace_rtol=1e-6):
+    assert st["iterations"] == d["iterations"]
+    assert st["reason"] == d["status"]
+    h = gpu.history()
+    assert len(h) == len(d["trace"])
+    np.testing.assert_allclose(h, d["trace"], rtol=trace_rtol, atol=1e-8 * d["trace"][0] + 1e-300)
+    scale = max(np.abs(xo).max(), 1e-300)
+    assert np.abs(xg - xo).max() <= 1e-8 * scale
+    # | ||b - A xg|| - ||b - A xo|| | <= ||A|| ||xg - xo||
+    bound = anorm * np.linalg.norm(xg - xo) / np.linalg.norm(b)
+    assert abs(st["relres"] - d["relres"]) <= bound + 1e-12 * d["relres"] + 1e-300
+
+
+GM_CASES = [
+    # shape, h, sigma, restart, max_it, tol, precond
+    ((16, 16, 16), 0.5, 0.0, 30, 2000, 1e-6, 1),
+    ((16, 16, 16), 0.5, 0.5, 30, 2000, 1e-8, 1),
+    ((24, 20, 16), 0.5, 0.0, 7, 2000, 1e-6, 1),
+    ((20, 12, 16), 0.5, 0.5, 30, 2000, 1e-6, 0),
+    ((16, 16, 16), 0.5, 0.0, 30, 17, 1e-10, 1),
+    ((16, 16, 16), 0.5, 0.0, 1, 300, 1e-3, 1),
+    ((16, 16, 16), 0.5, 0.0, 32, 2000, 1e-6, 1),
+    ((48, 48, 48), 0.5, 0.0, 30, 2000, 1e-6, 1),
+]
+
+
+def _gid(c):
+    return "x".join(map(str, c[0])) + f"_s{c[2]}_k{c[3]}_i{c[4]}" + ("" if c[6] else "_noprec")
+
+
+@pytest.mark.parametrize("case", GM_CASES, ids=_gid)
+def test_gmres_parity(gpu, oracle, case):
+    shape, h, sigma, restart, max_it, tol, precond = case
+    say("gmres", _gid(case))
+    a = P.poisson_a(h)
+    b = rhs(shape, 1001)
+    x0 = np.ones(b.size)
+    xo, d = oracle.gmres(x0, b, shape, a, max_it, tol, restart=restart, sigma=sigma, precond=precond)
+    xg = dev(x0)
+    st = gpu.GMRES(xg, dev(b), shape, a, max_it, tol, restart=restart, sigma=sigma, precond=precond)
+    check_close(gpu, st, d, xg.cpu().numpy(), xo, b, op_norm(oracle, a, sigma))
+
+
+BI_CASES = [
+    ((16, 16, 16), 0.5, 0.0, 2000, 1e-6, 1),
+    ((16, 16, 16), 0.5, 0.5, 2000, 1e-10, 1),
+    ((20, 12, 16), 0.5, 0.0, 2000, 1e-6, 0),
+    ((16, 16, 16), 0.5, 0.0, 9, 1e-10, 1),
+    ((48, 48, 48), 0.5, 0.0, 2000, 1e-6, 1),
+]
+
+
+def _bid(c):
+    return "x".join(map(str, c[0])) + f"_s{c[2]}_i{c[3]}" + ("" if c[5] else "_noprec")
+
+
+@pytest.mark.parametrize("case", BI_CASES, ids=_bid)
+def test_bicg_parity(gpu, oracle, case):
+    shape, h, sigma, max_it, tol, precond = case
+    say("bicg", _bid(case))
+    a = P.poisson_a(h)
+    b = rhs(shape, 1001)
+    x0 = np.ones(b.size)
+    xo, d = oracle.bicg(x0, b, shape, a, max_it, tol, sigma=sigma, precond=precond)
+    xg = dev(x0)
+    st = gpu.BiCG(xg, dev(b), shape, a, max_it, tol, sigma=sigma, precond=precond)
+    check_close(gpu, st, d, xg.cpu().numpy(), xo, b, op_norm(oracle, a, sigma))
+
+
+@pytest.mark.parametrize("solver", ["gmres", "bicg"])
+def test_krylov_variable_coefficient(gpu, oracle, solver):
+    shape = (24, 24, 24)
+    say("krylov vc", solver, shape)
+    a = P.poisson_a(0.5)
+    b = rhs(shape, 1005)
+    c = P.helmholtz_coefficient(shape, 7)
+    x0 = np.ones(b.size)
+    xg = dev(x0)
+    if solver == "gmres":
+        xo, d = oracle.gmres(x0, b, shape, a, 2000, 1e-8, c=c)
+        st = gpu.GMRES(xg, dev(b), shape, a, 2000, 1e-8, c=dev(c))
+    else:
+        xo, d = oracle.bicg(x0, b, shape, a, 2000, 1e-8, c=c)
+        st = gpu.BiCG(xg, dev(b), shape, a, 2000, 1e-8, c=dev(c))
+    check_close(gpu, st, d, xg.cpu().numpy(), xo, b, op_norm(oracle, a, c=c))
+
+
+def test_krylov_edges(gpu, oracle):
+    shape = (16, 16, 16)
+    say("krylov edges", shape)
+    a = P.poisson_a(0.5)
+    n = int(np.prod(shape))
+    b = rhs(shape, 1001)
+    # zero residual at the start: converged on atol with no step, x untouched
+    for fn in (gpu.GMRES, gpu.BiCG):
+        xg = dev(np.zeros(n))
+        st = fn(xg, dev(np.zeros(n)), shape, a, 100, 1e-6)
+        assert st["reason"] == 3 and st["iterations"] == 0
+        assert not xg.cpu().numpy().any()
+    # max_it = 0 (BiCG still takes its one do-while step)
+    for fn, ofn in ((gpu.GMRES, oracle.gmres), (gpu.BiCG, oracle.bicg)):
+        xo, d = ofn(np.ones(n), b, shape, a, 0, 1e-6)
+        xg = dev(np.ones(n))
+        st = fn(xg, dev(b), shape, a, 0, 1e-6)
+        assert st["reason"] == d["status"] == -3
+        assert st["iterations"] == d["iterations"]
+        assert np.abs(xg.cpu().numpy() - xo).max() <= 1e-12
+    # restart out of range
+    from aar_hip._lib import AarHipError
+    with pytest.raises(AarHipError):
+        gpu.GMRES(dev(np.ones(n)), dev(b), shape, a, 10, 1e-6, restart=33)
+    with pytest.raises(AarHipError):
+        gpu.GMRES(dev(np.ones(n)), dev(b), shape, a, 10, 1e-6, restart=0)
+
+
+def test_krylov_repeatable_and_converged_at_scale(gpu):
+    """128^3 Poisson (h = 0.25): converged to rtol on the preconditioned
+    norm, true relative residual near the tolerance, and two runs identical
+    bit for bit (fixed-order reductions)."""
+    import torch
+    shape = (128, 128, 128)
+    say("krylov 128^3")
+    a = P.poisson_a(0.25)
+    b = P.rhs_uniform_torch(shape, 1002, device="cuda")
+    for fn in (gpu.GMRES, gpu.BiCG):
+        x1 = torch.ones_like(b)
+        s1 = fn(x1, b, shape, a, 2000, 1e-6)
+        x2 = torch.ones_like(b)
+        s2 = fn(x2, b, shape, a, 2000, 1e-6)
+        assert s1["reason"] == 2 and s1["iterations"] == s2["iterations"]
+        assert torch.equal(x1, x2)
+        assert s1["relres"] < 1e-4
+        say(fn.__name__, s1["iterations"], s1["relres"], s1["solve_ms"])
