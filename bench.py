@@ -396,13 +396,19 @@ def main():
     others = {}
     for name, fn in (("pgr", lambda: aar_hip.PGR(x, b, shape, a, omega, m, p, max_iter, args.tol)),
                      ("pl2r", lambda: aar_hip.PL2R(x, b, shape, a, omega, m, p, max_iter, args.tol)),
-                     ("cg", lambda: aar_hip.CG(x, b, shape, a, max_iter, args.tol))):
+                     ("cg", lambda: aar_hip.CG(x, b, shape, a, max_iter, args.tol)),
+                     # the PETSc variant's Krylov comparators with its max_it 2000
+                     # (PETSC_REAL/main.c:100,132)
+                     ("bicg", lambda: aar_hip.BiCG(x, b, shape, a, 2000, args.tol)),
+                     ("gmres30", lambda: aar_hip.GMRES(x, b, shape, a, 2000, args.tol, restart=30))):
         x.zero_()
         st = fn()
         others[name] = {"iterations": st["iterations"], "converged": bool(st["converged"]),
                         "relres": st["relres"], "solve_ms": st["solve_ms"],
                         "solves_per_s": 1e3 / st["solve_ms"],
                         "algorithmic_GBps": st["bytes"] / st["solve_ms"] / 1e6}
+        if name in ("bicg", "gmres30"):
+            others[name]["ksp_reason"] = st["reason"]
         log(f"{name}: {st['iterations']} its {st['solve_ms']:.0f} ms")
 
     # the variable-coefficient operator on the same grid and RHS
