@@ -146,10 +146,20 @@ int aar_hip_cg(const aar_hip_grid *g, double *x, const double *b, int max_iter, 
  * (main.c:112-118), reason = the KSPConvergedReason; aar_hip_history()
  * returns the norm of every convergence test. */
 #define AAR_HIP_GMRES_MAXK 32
+/* precond = 2 (both solvers): the pc = 1 preconditioner of the PETSc
+ * variant on one rank, PCBJACOBI -> PCILU at 0 levels (main.c:88-91) = the
+ * ILU(0) factors of the whole periodic matrix, computed on the device in
+ * each call; every grid side must be >= 13.  BiCG then carries the shadow
+ * sequence with the transposed factors. */
 int aar_hip_gmres(const aar_hip_grid *g, const double *c, double *x, const double *b, int restart,
                   int max_iter, double tol, int precond, aar_hip_stats *st);
 int aar_hip_bicg(const aar_hip_grid *g, const double *c, double *x, const double *b, int max_iter,
                  double tol, int precond, aar_hip_stats *st);
+/* PCApply / PCApplyTranspose of that ILU(0) preconditioner: factors the
+ * matrix of g (+ c, may be NULL) and sets z = M^{-1} r (trans = 0) or
+ * M^{-T} r (trans = 1); device pointers, z may equal r.  Level-scheduled
+ * (x + y + z = const planes), bit-identical to a sequential IKJ ILU(0). */
+int aar_hip_ilu0_apply(const aar_hip_grid *g, const double *c, const double *r, double *z, int trans);
 
 /* Extension (no reference counterpart): FP64 residual refinement around
  * FP32 AAR.  Outer: r = b - A x in FP64, stop at ||r|| <= tol*||b||;

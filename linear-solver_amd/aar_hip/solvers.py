@@ -217,10 +217,18 @@ def CG(x, b, shape, a, max_iter, tol, sigma=0.0, precond=True, c=None):
     return st.as_dict()
 
 
+ILU0 = 2     # precond value of GMRES/BiCG for the ILU(0) preconditioner
+
+
+def _pc(precond):
+    return ILU0 if precond == ILU0 and not isinstance(precond, bool) else int(bool(precond))
+
+
 def GMRES(x, b, shape, a, max_iter=2000, tol=1e-6, restart=30, sigma=0.0, precond=True, c=None):
-    """PETSc's KSPGMRES with PCJACOBI as PETSC_REAL/main.c:86-122 runs it
-    (restart 30, classical Gram-Schmidt, left preconditioning, converged on
-    the preconditioned residual norm).  Returns the stats; 'reason' is the
+    """PETSc's KSPGMRES as PETSC_REAL/main.c:86-122 runs it (restart 30,
+    classical Gram-Schmidt, left preconditioning, converged on the
+    preconditioned residual norm); precond: False/0 none, True/1 Jacobi,
+    2 ILU(0) (the PETSc variant's pc = 1).  Returns the stats; 'reason' is the
     KSPConvergedReason, 'relres' the true ||b - A x||/||b||."""
     import torch
     _check_len(x, shape, "x")
@@ -233,12 +241,30 @@ def GMRES(x, b, shape, a, max_iter=2000, tol=1e-6, restart=30, sigma=0.0, precon
         _check_len(c, shape, "c")
         cp = _dev(c, torch.float64, "c")
     check(_L().aar_hip_gmres(ctypes.byref(g), cp, _dev(x, torch.float64, "x"), _dev(b, torch.float64, "b"),
-                             restart, max_iter, tol, int(bool(precond)), ctypes.byref(st)))
+                             restart, max_iter, tol, _pc(precond), ctypes.byref(st)))
     return st.as_dict()
 
 
+def ilu0_apply(r, shape, a, sigma=0.0, c=None, trans=False):
+    """z = M^{-1} r (trans: M^{-T} r) for the ILU(0) factors of a*Lap +
+    sigma (+ c): the PETSc variant's pc = 1 preconditioner on one rank."""
+    import torch
+    _check_len(r, shape, "r")
+    g = _grid(shape, a, sigma)
+    _bind_stream()
+    cp = None
+    if c is not None:
+        _check_len(c, shape, "c")
+        cp = _dev(c, torch.float64, "c")
+    z = torch.empty_like(r)
+    check(_L().aar_hip_ilu0_apply(ctypes.byref(g), cp, _dev(r, torch.float64, "r"), _dev(z, torch.float64, "z"),
+                                  int(bool(trans))))
+    return z
+
+
 def BiCG(x, b, shape, a, max_iter=2000, tol=1e-6, sigma=0.0, precond=True, c=None):
-    """PETSc's KSPBICG with PCJACOBI as PETSC_REAL/main.c:124-150 runs it."""
+    """PETSc's KSPBICG as PETSC_REAL/main.c:124-150 runs it; precond as
+    in GMRES (2: ILU(0), the shadow sequence then carried explicitly)."""
     import torch
     _check_len(x, shape, "x")
     _check_len(b, shape, "b")
@@ -250,7 +276,7 @@ def BiCG(x, b, shape, a, max_iter=2000, tol=1e-6, sigma=0.0, precond=True, c=Non
         _check_len(c, shape, "c")
         cp = _dev(c, torch.float64, "c")
     check(_L().aar_hip_bicg(ctypes.byref(g), cp, _dev(x, torch.float64, "x"), _dev(b, torch.float64, "b"),
-                            max_iter, tol, int(bool(precond)), ctypes.byref(st)))
+                            max_iter, tol, _pc(precond), ctypes.byref(st)))
     return st.as_dict()
 
 

@@ -56,7 +56,7 @@ struct Context {
     DevBuf vec32a, vec32b, vec64a, vec64b;
     DevBuf ringc, corrc;         // complex Helmholtz g ring and corrections
     DevBuf vch;                  // variable coefficient: the recurrence field h(x)
-    DevBuf kctl, kvec;           // Krylov comparators: control block, basis / work vectors
+    DevBuf kctl, kvec, kilu;     // Krylov comparators: control block, basis / work vectors, ILU(0) factors
     std::string err;
     int lz = 0, wy = 8, py = 4;
     long ring_pad = 512;           // extra elements per ring slot (multiple of 32)
@@ -119,6 +119,7 @@ int cg_solve(const Op &op, double *x, const double *b, int max_iter, double tol,
 int gmres_solve(const Op &op, double *x, const double *b, int restart, int max_it, double tol, int precond,
                 aar_hip_stats *st);
 int bicg_solve(const Op &op, double *x, const double *b, int max_it, double tol, int precond, aar_hip_stats *st);
+int ilu0_apply(const Op &op, const double *r, double *z, int trans);
 template <typename T>
 int run_stencil(SMode mode, SArgs<T> a, double bytes);
 template <typename T>

@@ -821,6 +821,13 @@ int aar_hip_bicg(const aar_hip_grid *g, const double *c, double *x, const double
     return bicg_solve(op, x, b, max_iter, tol, precond, st);
 }
 
+int aar_hip_ilu0_apply(const aar_hip_grid *g, const double *c, const double *r, double *z, int trans)
+{
+    Op op;
+    if (int rc = c ? make_op_vc(g, c, op) : make_op(g, nullptr, op)) return rc;
+    return ilu0_apply(op, r, z, trans);
+}
+
 int aar_hip_matvec_f32(const aar_hip_grid *g, const float *x, float *y)
 {
     Op op;

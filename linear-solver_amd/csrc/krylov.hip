@@ -75,7 +75,7 @@ struct KCtrl {
     Ctrl c;        // c.done: the solve has a reason (stencil early exit, host polling)
     Ctrl gate;     // gate.done: no Arnoldi step pending (early exit of the step's kernels)
     int reason, its, it, open, noscale, nosol, nh, hcap;
-    int max_it, restart, first, pad;
+    int max_it, restart, first, zero;   // zero: never set (a skip flag that never skips)
     double rtol, ttol, rnorm0, bpnorm, rhs_norm, scale, relres;
     double beta, betaold, bcoef, alpha;
     double h[KMAX + 1];
@@ -335,6 +335,90 @@ __global__ __launch_bounds__(KB) void k_bi_upd_kernel(long n, double *x, double 
     k_block_sums<2>(acc, 2, part);
 }
 
+// ||v||^2
+__global__ __launch_bounds__(KB) void k_norm_kernel(long n, const double *v, double *part, const int *skip)
+{
+    if (*skip) return;
+    double acc[1] = {0.0};
+    KLOOP(i) acc[0] += v[i] * v[i];
+    k_block_sums<1>(acc, 1, part);
+}
+
+// ||B b||^2 and ||b||^2 with B b given (zb)
+__global__ __launch_bounds__(KB) void k_bnorm2_kernel(long n, const double *b, const double *zb, double *part)
+{
+    double acc[2] = {0.0, 0.0};
+    KLOOP(i)
+    {
+        acc[0] += zb[i] * zb[i];
+        acc[1] += b[i] * b[i];
+    }
+    k_block_sums<2>(acc, 2, part);
+}
+
+// BiCG with a non-diagonal preconditioner, whose transpose differs from
+// it in rounding: the shadow sequence is carried explicitly.
+// r = b - t (t = A x); r~ = r
+__global__ __launch_bounds__(KB) void k_bi2_init_kernel(long n, const double *b, const double *t, double *r, double *rl)
+{
+    KLOOP(i)
+    {
+        const double ri = b[i] + (-1.0) * t[i];
+        r[i] = ri;
+        rl[i] = ri;
+    }
+}
+
+// ||z||^2, <z, r~>
+__global__ __launch_bounds__(KB) void k_bi2_norms_kernel(long n, const double *z, const double *rl, double *part,
+                                                        const KCtrl *k)
+{
+    if (k->c.done) return;
+    double acc[2] = {0.0, 0.0};
+    KLOOP(i)
+    {
+        const double zi = z[i];
+        acc[0] += zi * zi;
+        acc[1] += zi * rl[i];
+    }
+    k_block_sums<2>(acc, 2, part);
+}
+
+// p = z, p~ = z~ (first) or p = z + b p, p~ = z~ + b p~
+__global__ __launch_bounds__(KB) void k_bi2_p_kernel(long n, const double *z, const double *zl, double *p, double *pl,
+                                                    const KCtrl *k)
+{
+    if (k->c.done) return;
+    if (k->first) {
+        KLOOP(i)
+        {
+            p[i] = z[i];
+            pl[i] = zl[i];
+        }
+    } else {
+        const double bc = k->bcoef;
+        KLOOP(i)
+        {
+            p[i] = z[i] + bc * p[i];
+            pl[i] = zl[i] + bc * pl[i];
+        }
+    }
+}
+
+// x += a p; r -= a q; r~ -= a q~
+__global__ __launch_bounds__(KB) void k_bi2_upd_kernel(long n, double *x, double *r, double *rl, const double *p,
+                                                      const double *q, const double *ql, const KCtrl *k)
+{
+    if (k->c.done) return;
+    const double al = k->alpha, ma = -k->alpha;
+    KLOOP(i)
+    {
+        x[i] = x[i] + al * p[i];
+        r[i] = r[i] + ma * q[i];
+        rl[i] = rl[i] + ma * ql[i];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // control kernels (one block; thread 0 runs the scalar recurrences)
 // ---------------------------------------------------------------------------
@@ -516,6 +600,136 @@ __global__ __launch_bounds__(KB) void k_final_kernel(KCtrl *k, const double *par
 }
 
 // ---------------------------------------------------------------------------
+// ILU(0) of the periodic FD12 star matrix (the PETSc variant's pc = 1:
+// PCBJACOBI -> PCILU on one block, PETSC_REAL/main.c:88-91; PCICC of
+// AAR_Real.c:58-60 is the same preconditioner in exact arithmetic on this
+// symmetric matrix).  Every dependency of a row (x, y, z) in the
+// elimination and in the forward solve has a smaller x + y + z (periodic
+// wrap included), so each level x + y + z = L is one launch; the backward
+// solves run the levels downwards.  Rows are eliminated in PETSc's IKJ
+// order and solved in ascending column order, with no sums across rows:
+// the factors and solves are bit-identical to the oracle's sequential ones.
+//
+// Factor layout: IFS doubles per row (37 used): [0] = 1/u_ii, [1 + 12*axis
+// + k] the entry at periodic offset d along axis (k = d + 6 for d < 0,
+// d + 5 for d > 0): l_ij below the row, u_ij above.
+// ---------------------------------------------------------------------------
+constexpr int IFS = 40;
+
+__host__ __device__ constexpr int islot(int ax, int d) { return 1 + 12 * ax + (d < 0 ? d + 6 : d + 5); }
+
+struct IluArgs {
+    int nx, ny, nz;
+    int level, x0, y0, wx, wy;   // the level and its (x, y) box
+    double w[R + 1];             // a * coeff[o]
+    double diag;                 // a * 3 * coeff[0] (+ sigma)
+    const double *c;             // + c(x) on the diagonal, or null
+    double *F;
+    const int *skip;             // nonzero: no work (solver stopped / step gated)
+};
+
+// the point of this thread on the level, or false
+__device__ __forceinline__ bool ilu_point(const IluArgs &a, int &x, int &y, int &z)
+{
+    const int t = blockIdx.x * KB + threadIdx.x;
+    if (t >= a.wx * a.wy) return false;
+    x = a.x0 + t % a.wx;
+    y = a.y0 + t / a.wx;
+    z = a.level - x - y;
+    return z >= 0 && z < a.nz;
+}
+
+// periodic offsets of a row's columns along one axis in ascending column
+// order: q = 0..5 -> d = 1..6 (below the row only when wrapped), q = 6..11
+// -> d = -6..-1 (below unless wrapped)
+__device__ __forceinline__ constexpr int qoff(int q) { return q < 6 ? q + 1 : q - 12; }
+
+__global__ __launch_bounds__(KB) void ilu_factor_kernel(IluArgs a)
+{
+    int x, y, z;
+    if (!ilu_point(a, x, y, z)) return;
+    const int cr[3] = {x, y, z}, dims[3] = {a.nx, a.ny, a.nz};
+    const long strides[3] = {1, a.nx, (long)a.nx * a.ny};
+    const long i = x + (long)y * a.nx + (long)z * a.nx * a.ny;
+    double v[37];
+    v[0] = a.c ? a.diag + a.c[i] : a.diag;
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++)
+#pragma unroll
+        for (int d = 1; d <= R; d++) {
+            v[islot(ax, -d)] = a.w[d];
+            v[islot(ax, d)] = a.w[d];
+        }
+#pragma unroll
+    for (int ax = 2; ax >= 0; ax--) {            // z, y, x: ascending column index
+        const int c = cr[ax], n = dims[ax];
+        const long st = strides[ax];
+#pragma unroll
+        for (int q = 0; q < 12; q++) {
+            const int dk = qoff(q);
+            const bool lower = dk > 0 ? (c + dk >= n) : (c + dk >= 0);
+            if (!lower) continue;
+            const int ck = dk > 0 ? c + dk - n : c + dk;
+            const double *Fk = a.F + (size_t)(i + (long)(ck - c) * st) * IFS;
+            const double l = v[islot(ax, dk)] * Fk[0];
+            v[islot(ax, dk)] = l;
+            // row k's columns above k that are in row i's pattern: on this axis
+#pragma unroll
+            for (int dij = -R; dij <= R; dij++) {
+                if (dij == dk) continue;
+                int cj = c + dij;
+                cj = cj < 0 ? cj + n : (cj >= n ? cj - n : cj);
+                if (cj <= ck) continue;
+                int dkj = dij - dk;
+                if (dkj > R) dkj -= n;
+                else if (dkj < -R) dkj += n;
+                if (dkj < -R || dkj > R) continue;
+                const int si = dij == 0 ? 0 : islot(ax, dij);
+                v[si] = v[si] - l * Fk[islot(ax, dkj)];
+            }
+        }
+    }
+    double *Fi = a.F + (size_t)i * IFS;
+    Fi[0] = 1.0 / v[0];
+#pragma unroll
+    for (int s = 1; s < 37; s++) Fi[s] = v[s];
+}
+
+// One level of a triangular solve, in place on io (in -> io when in != io
+// at the first solve of an apply).  lowerpass: the columns below the row
+// (forward levels); trans: the transposed factor (entries of the other
+// row); diagmul: multiply by 1/u_ii at the end.
+template <bool LOWER, bool TRANS, bool DIAG>
+__global__ __launch_bounds__(KB) void ilu_solve_kernel(IluArgs a, const double *in, double *io)
+{
+    if (*a.skip) return;
+    int x, y, z;
+    if (!ilu_point(a, x, y, z)) return;
+    const int cr[3] = {x, y, z}, dims[3] = {a.nx, a.ny, a.nz};
+    const long strides[3] = {1, a.nx, (long)a.nx * a.ny};
+    const long i = x + (long)y * a.nx + (long)z * a.nx * a.ny;
+    const double *Fi = a.F + (size_t)i * IFS;
+    double s = in[i];
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        const int ax = LOWER ? 2 - t : t;        // ascending column index
+        const int c = cr[ax], n = dims[ax];
+        const long st = strides[ax];
+#pragma unroll
+        for (int q = 0; q < 12; q++) {
+            const int d = qoff(q);
+            const bool below = d > 0 ? (c + d >= n) : (c + d >= 0);
+            if (below != LOWER) continue;
+            const int cj = d > 0 ? (below ? c + d - n : c + d) : (below ? c + d : c + d + n);
+            const long j = i + (long)(cj - c) * st;
+            const double f = TRANS ? a.F[(size_t)j * IFS + islot(ax, -d)] : Fi[islot(ax, d)];
+            s = s - f * io[j];
+        }
+    }
+    io[i] = DIAG ? s * Fi[0] : s;
+}
+
+// ---------------------------------------------------------------------------
 // host drivers
 // ---------------------------------------------------------------------------
 namespace {
@@ -526,10 +740,80 @@ struct KrylovRun {
     int nb = 0, hcap = 0;
     long n = 0;
     KDiag dg{};
+    IluArgs ilu{};
 };
 
+int check_ilu_grid(const Op &op)
+{
+    if (op.nx < 13 || op.ny < 13 || op.nz < 13)
+        return fail(AAR_HIP_ERR_ARG, "the ILU(0) preconditioner needs every grid side >= 13");
+    return 0;
+}
+
+// the (x, y) box of level L: x + y + z = L inside the grid
+void ilu_box(IluArgs &a, int L)
+{
+    const int x0 = std::max(0, L - (a.ny - 1) - (a.nz - 1)), x1 = std::min(a.nx - 1, L);
+    const int y0 = std::max(0, L - (a.nx - 1) - (a.nz - 1)), y1 = std::min(a.ny - 1, L);
+    a.level = L;
+    a.x0 = x0;
+    a.y0 = y0;
+    a.wx = x1 - x0 + 1;
+    a.wy = y1 - y0 + 1;
+}
+
+IluArgs ilu_args(const Op &op, double *F)
+{
+    IluArgs a{};
+    a.nx = op.nx;
+    a.ny = op.ny;
+    a.nz = op.nz;
+    for (int o = 0; o <= R; o++) a.w[o] = op.w[o];
+    a.diag = op.diag;
+    a.c = op.c;
+    a.F = F;
+    return a;
+}
+
+int ilu_nlev(const IluArgs &a) { return a.nx + a.ny + a.nz - 2; }
+
+hipError_t ilu_factor(IluArgs a, hipStream_t st)
+{
+    for (int L = 0; L < ilu_nlev(a); L++) {
+        ilu_box(a, L);
+        hipLaunchKernelGGL(ilu_factor_kernel, dim3((a.wx * a.wy + KB - 1) / KB), dim3(KB), 0, st, a);
+    }
+    return hipGetLastError();
+}
+
+// out = M^{-1} in (trans: M^{-T} in); out may be in
+hipError_t ilu_apply(IluArgs a, const double *in, double *out, bool trans, const int *skip, hipStream_t st)
+{
+    a.skip = skip;
+    const int nl = ilu_nlev(a);
+    for (int L = 0; L < nl; L++) {
+        ilu_box(a, L);
+        const dim3 g((a.wx * a.wy + KB - 1) / KB);
+        if (trans) hipLaunchKernelGGL((ilu_solve_kernel<true, true, true>), g, dim3(KB), 0, st, a, in, out);
+        else hipLaunchKernelGGL((ilu_solve_kernel<true, false, false>), g, dim3(KB), 0, st, a, in, out);
+    }
+    for (int L = nl - 1; L >= 0; L--) {
+        ilu_box(a, L);
+        const dim3 g((a.wx * a.wy + KB - 1) / KB);
+        if (trans) hipLaunchKernelGGL((ilu_solve_kernel<false, true, false>), g, dim3(KB), 0, st, a,
+                                      (const double *)out, out);
+        else hipLaunchKernelGGL((ilu_solve_kernel<false, false, true>), g, dim3(KB), 0, st, a, (const double *)out,
+                                out);
+    }
+    return hipGetLastError();
+}
+
+// algorithmic bytes of one ILU(0) apply: each row's 37 factor entries, the
+// input and both passes' output
+constexpr double ILU_APPLY_BYTES = (37 + 4) * 8.0;
+
 int k_setup(const Op &op, int precond, double tol, int max_it, int restart, int hcap, const double *b,
-            KrylovRun &R)
+            KrylovRun &R, double *F = nullptr, double *zb = nullptr)
 {
     Context &C = ctx();
     R.n = op.n;
@@ -544,10 +828,19 @@ int k_setup(const Op &op, int precond, double tol, int max_it, int restart, int 
     R.dg.c = op.c;
     R.dg.diag = op.diag;
     R.dg.dinv = 1.0 / op.diag;
-    R.dg.precond = precond;
+    R.dg.precond = precond == 1;     // the fused Jacobi scaling (ILU(0) is applied by its own sweeps)
     const long words = (long)(sizeof(KCtrl) / sizeof(unsigned));
     hipLaunchKernelGGL(k_zero_kernel, dim3(64), dim3(KB), 0, C.stream, (unsigned *)R.k, words);
-    hipLaunchKernelGGL(k_bnorm_kernel, dim3(R.nb), dim3(KB), 0, C.stream, R.n, b, R.dg, R.part);
+    if (F) {
+        // PCSetUp: factor, then B b for the tolerance
+        R.ilu = ilu_args(op, F);
+        HC(ilu_factor(R.ilu, C.stream));
+        HC(ilu_apply(R.ilu, b, zb, false, &R.k->zero, C.stream));
+        hipLaunchKernelGGL(k_bnorm2_kernel, dim3(R.nb), dim3(KB), 0, C.stream, R.n, b, (const double *)zb, R.part);
+        C.bytes_acc += op.n * (37 * 8.0 + ILU_APPLY_BYTES);
+    } else {
+        hipLaunchKernelGGL(k_bnorm_kernel, dim3(R.nb), dim3(KB), 0, C.stream, R.n, b, R.dg, R.part);
+    }
     hipLaunchKernelGGL(k_ctl_init_kernel, dim3(1), dim3(KB), 0, C.stream, R.k, tol, max_it, restart, hcap,
                        (const double *)R.part, R.nb);
     HC(hipGetLastError());
@@ -637,16 +930,26 @@ int gmres_solve(const Op &op, double *x, const double *b, int restart, int max_i
     if (!x || !b) return fail(AAR_HIP_ERR_ARG, "null vector");
     if (max_it < 0) return fail(AAR_HIP_ERR_ARG, "max_iter must be >= 0");
     if (restart < 1 || restart > KMAX) return fail(AAR_HIP_ERR_ARG, "restart must be in 1..32");
+    if (precond < 0 || precond > 2) return fail(AAR_HIP_ERR_ARG, "precond must be 0 (none), 1 (Jacobi) or 2 (ILU(0))");
+    const bool ilu = precond == 2;
+    if (ilu)
+        if (int rc = check_ilu_grid(op)) return rc;
     const long n = op.n;
     const long ld = ring_ld(n, sizeof(double));
     // basis v_0 .. v_restart, then t
     HC(ensure(C.kvec, sizeof(double) * (size_t)ld * (restart + 2)));
     double *V = (double *)C.kvec.p, *t = V + (size_t)ld * (restart + 1);
+    double *F = nullptr;
+    if (ilu) {
+        HC(ensure(C.kilu, sizeof(double) * (size_t)n * IFS));
+        F = (double *)C.kilu.p;
+    }
     const int hcap = max_it + max_it / restart + 8;
     C.bytes_acc = 0.0;
     HC(hipEventRecord(C.t0, C.stream));
     KrylovRun R;
-    if (int rc = k_setup(op, precond, tol, max_it, restart, hcap, b, R)) return rc;
+    if (int rc = k_setup(op, precond, tol, max_it, restart, hcap, b, R, F, t)) return rc;
+    const double ib = ilu ? ILU_APPLY_BYTES * n : 0.0;
     const double s = 8.0 * n;
     const double cb = op.c ? s : 0.0;      // the coefficient field's reads, where 1/diag is per point
     bool stopped = false;
@@ -655,6 +958,12 @@ int gmres_solve(const Op &op, double *x, const double *b, int restart, int max_i
         if (int rc = k_matvec(op, x, t, &R.k->c)) return rc;
         hipLaunchKernelGGL(k_gm_resid_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, b, (const double *)t, V, R.dg,
                            R.part, (const int *)&R.k->c.done);
+        if (ilu) {
+            HC(ilu_apply(R.ilu, V, V, false, &R.k->c.done, C.stream));
+            hipLaunchKernelGGL(k_norm_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, (const double *)V, R.part,
+                               (const int *)&R.k->c.done);
+            C.bytes_acc += ib + s;
+        }
         hipLaunchKernelGGL(k_gm_start_kernel, dim3(1), dim3(KB), 0, C.stream, R.k, (const double *)R.part, R.nb,
                            R.hist);
         hipLaunchKernelGGL(k_gm_scale_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, V, (const KCtrl *)R.k, 0);
@@ -663,10 +972,19 @@ int gmres_solve(const Op &op, double *x, const double *b, int restart, int max_i
         for (int it = 0; it < restart; it++) {
             const int nv = it + 1;
             if (int rc = k_matvec(op, V + (size_t)it * ld, t, &R.k->gate)) return rc;
+            const double *w = t;
+            if (ilu) {
+                // w = M^{-1} A v_it, built in the next basis slot
+                prof_begin("krylov_ilu_apply");
+                HC(ilu_apply(R.ilu, t, V + (size_t)nv * ld, false, &R.k->gate.done, C.stream));
+                prof_end(ib);
+                C.bytes_acc += ib;
+                w = V + (size_t)nv * ld;
+            }
             prof_begin("krylov_gmres_orth");
-            hipError_t e = nv <= 8 ? launch_gm_step<8>(op, t, V, ld, nv, R)
-                         : nv <= 16 ? launch_gm_step<16>(op, t, V, ld, nv, R)
-                                    : launch_gm_step<32>(op, t, V, ld, nv, R);
+            hipError_t e = nv <= 8 ? launch_gm_step<8>(op, w, V, ld, nv, R)
+                         : nv <= 16 ? launch_gm_step<16>(op, w, V, ld, nv, R)
+                                    : launch_gm_step<32>(op, w, V, ld, nv, R);
             prof_end((2.0 * nv + 3) * s + 2 * cb);
             if (e != hipSuccess) return fail(AAR_HIP_ERR_HIP, std::string("gmres step: ") + hipGetErrorString(e));
             hipLaunchKernelGGL(k_gm_arnoldi_kernel, dim3(1), dim3(KB), 0, C.stream, R.k, (const double *)R.part, R.nb,
@@ -690,12 +1008,93 @@ int gmres_solve(const Op &op, double *x, const double *b, int restart, int max_i
     return k_finish(op, x, b, t, R, st);
 }
 
+// z = M^{-1} r (trans: M^{-T} r) with the ILU(0) factors of op, factored here
+int ilu0_apply(const Op &op, const double *r, double *z, int trans)
+{
+    Context &C = ctx();
+    if (int rc = context_init()) return rc;
+    if (!r || !z) return fail(AAR_HIP_ERR_ARG, "null vector");
+    if (int rc = check_ilu_grid(op)) return rc;
+    HC(ensure(C.kilu, sizeof(double) * (size_t)op.n * IFS));
+    HC(ensure(C.kctl, sizeof(KCtrl)));
+    KCtrl *k = (KCtrl *)C.kctl.p;
+    hipLaunchKernelGGL(k_zero_kernel, dim3(64), dim3(KB), 0, C.stream, (unsigned *)k,
+                       (long)(sizeof(KCtrl) / sizeof(unsigned)));
+    IluArgs a = ilu_args(op, (double *)C.kilu.p);
+    HC(ilu_factor(a, C.stream));
+    HC(ilu_apply(a, r, z, trans != 0, &k->zero, C.stream));
+    HC(hipStreamSynchronize(C.stream));
+    return 0;
+}
+
+// BiCG with ILU(0): M^{-T} differs from M^{-1} in rounding, so the shadow
+// sequence (r~, z~ = M^{-T} r~, p~, A^T p~ = A p~) is carried explicitly.
+static int bicg_ilu_solve(const Op &op, double *x, const double *b, int max_it, double tol, aar_hip_stats *st)
+{
+    Context &C = ctx();
+    if (int rc = check_ilu_grid(op)) return rc;
+    const long n = op.n;
+    const long ld = ring_ld(n, sizeof(double));
+    HC(ensure(C.kvec, sizeof(double) * (size_t)ld * 8));
+    HC(ensure(C.kilu, sizeof(double) * (size_t)n * IFS));
+    double *r = (double *)C.kvec.p, *rl = r + ld, *z = r + 2 * ld, *zl = r + 3 * ld;
+    double *p = r + 4 * ld, *pl = r + 5 * ld, *q = r + 6 * ld, *ql = r + 7 * ld;
+    const int hcap = max_it + 8;
+    C.bytes_acc = 0.0;
+    HC(hipEventRecord(C.t0, C.stream));
+    KrylovRun R;
+    if (int rc = k_setup(op, 2, tol, max_it, 0, hcap, b, R, (double *)C.kilu.p, z)) return rc;
+    const double s = 8.0 * n, ib = ILU_APPLY_BYTES * n;
+    const double cb = op.c ? s : 0.0;
+    if (int rc = k_matvec(op, x, q, nullptr)) return rc;
+    hipLaunchKernelGGL(k_bi2_init_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, b, (const double *)q, r, rl);
+    HC(ilu_apply(R.ilu, r, z, false, &R.k->zero, C.stream));
+    HC(ilu_apply(R.ilu, rl, zl, true, &R.k->zero, C.stream));
+    hipLaunchKernelGGL(k_bi2_norms_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, (const double *)z,
+                       (const double *)rl, R.part, (const KCtrl *)R.k);
+    hipLaunchKernelGGL(k_bi_start_kernel, dim3(1), dim3(KB), 0, C.stream, R.k, (const double *)R.part, R.nb, R.hist);
+    HC(hipGetLastError());
+    C.bytes_acc += 6 * s + cb + 2 * ib;
+    bool stopped = false;
+    const int loops = std::max(max_it, 1);
+    int cyc = 0;
+    for (int i = 0; i < loops && !stopped; i++) {
+        hipLaunchKernelGGL(k_bi2_p_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, (const double *)z,
+                           (const double *)zl, p, pl, (const KCtrl *)R.k);
+        if (int rc = k_matvec(op, p, q, &R.k->c)) return rc;
+        if (int rc = k_matvec(op, pl, ql, &R.k->c)) return rc;     // A^T p~ (symmetric operator)
+        hipLaunchKernelGGL(k_bi_dot_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, (const double *)q,
+                           (const double *)pl, R.part, (const KCtrl *)R.k);
+        hipLaunchKernelGGL(k_bi_alpha_kernel, dim3(1), dim3(KB), 0, C.stream, R.k, (const double *)R.part, R.nb);
+        hipLaunchKernelGGL(k_bi2_upd_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, x, r, rl, (const double *)p,
+                           (const double *)q, (const double *)ql, (const KCtrl *)R.k);
+        prof_begin("krylov_ilu_apply");
+        HC(ilu_apply(R.ilu, r, z, false, &R.k->c.done, C.stream));
+        HC(ilu_apply(R.ilu, rl, zl, true, &R.k->c.done, C.stream));
+        prof_end(2 * ib);
+        hipLaunchKernelGGL(k_bi2_norms_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, (const double *)z,
+                           (const double *)rl, R.part, (const KCtrl *)R.k);
+        hipLaunchKernelGGL(k_bi_step_kernel, dim3(1), dim3(KB), 0, C.stream, R.k, (const double *)R.part, R.nb,
+                           R.hist);
+        HC(hipGetLastError());
+        C.bytes_acc += 21 * s + 2 * cb + 2 * ib;
+        if ((i + 1) % 8 == 0) {
+            if (int rc = poll_record(&R.k->c, cyc)) return rc;
+            if (poll_check(cyc - 2)) stopped = true;
+            cyc++;
+        }
+    }
+    return k_finish(op, x, b, q, R, st);
+}
+
 int bicg_solve(const Op &op, double *x, const double *b, int max_it, double tol, int precond, aar_hip_stats *st)
 {
     Context &C = ctx();
     if (int rc = context_init()) return rc;
     if (!x || !b) return fail(AAR_HIP_ERR_ARG, "null vector");
     if (max_it < 0) return fail(AAR_HIP_ERR_ARG, "max_iter must be >= 0");
+    if (precond < 0 || precond > 2) return fail(AAR_HIP_ERR_ARG, "precond must be 0 (none), 1 (Jacobi) or 2 (ILU(0))");
+    if (precond == 2) return bicg_ilu_solve(op, x, b, max_it, tol, st);
     const long n = op.n;
     const long ld = ring_ld(n, sizeof(double));
     HC(ensure(C.kvec, sizeof(double) * (size_t)ld * 4));

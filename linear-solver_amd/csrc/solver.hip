@@ -1504,7 +1504,7 @@ void release_all()
     Context &C = ctx();
     DevBuf *bufs[] = {&C.ring64x, &C.ring64f, &C.ring32x, &C.ring32f, &C.hist0, &C.hist1,
                       &C.part0, &C.part1, &C.vec32a, &C.vec32b, &C.vec64a, &C.vec64b,
-                      &C.ringc, &C.corrc, &C.vch, &C.kctl, &C.kvec};
+                      &C.ringc, &C.corrc, &C.vch, &C.kctl, &C.kvec, &C.kilu};
     for (DevBuf *b : bufs) {
         if (b->p) (void)hipFree(b->p);
         b->p = nullptr;

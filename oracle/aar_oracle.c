@@ -11,6 +11,8 @@
  */
 #include "aar_oracle.h"
 
+#define ILU_SLOTS ORC_ILU_SLOTS
+
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -890,6 +892,43 @@ static double *ksp_dinv(long n, double diag, const double *cv, int precond)
     return d;
 }
 
+/* the preconditioner B: none (kind 0), Jacobi (1: y = x * (1/diag)) or
+   ILU(0) (2: orc_ilu0_apply with the factors F) */
+typedef struct {
+    int kind;
+    const double *dinv, *F;
+    int nx, ny, nz;
+} ksp_pc;
+
+static void pc_apply(const ksp_pc *pc, const double *in, double *out, long n, int trans)
+{
+    if (pc->kind == 2) {
+        orc_ilu0_apply(pc->nx, pc->ny, pc->nz, pc->F, in, out, trans);
+        return;
+    }
+    for (long i = 0; i < n; i++) out[i] = pc->dinv[i] * in[i];
+}
+
+static int pc_setup(ksp_pc *pc, int precond, int nx, int ny, int nz, int fdn, double a, double sigma,
+                    const double *cv, double diag, double **dinv, double **F)
+{
+    const long n = (long)nx * ny * nz;
+    pc->kind = precond;
+    pc->nx = nx;
+    pc->ny = ny;
+    pc->nz = nz;
+    *dinv = ksp_dinv(n, diag, cv, precond == 1);
+    *F = NULL;
+    pc->dinv = *dinv;
+    pc->F = NULL;
+    if (precond == 2) {
+        *F = (double *)malloc(sizeof(double) * n * ILU_SLOTS);
+        if (orc_ilu0_factor(nx, ny, nz, fdn, a, sigma, cv, *F)) return -1;
+        pc->F = *F;
+    }
+    return 0;
+}
+
 /* VecNormalize: scale by 1/norm unless the norm is 0 or 1 */
 static double ksp_normalize(double *v, long n)
 {
@@ -932,7 +971,12 @@ static int gmres_impl(int nx, int ny, int nz, int fdn, double a, double sigma, c
     orc_lap_coefficient(coeff, fdn);
     double diag = a * (3 * coeff[0]);
     if (sigma != 0.0) diag += sigma;
-    double *dinv = ksp_dinv(n, diag, cv, precond);
+    double *dinv, *F;
+    ksp_pc pc;
+    if (pc_setup(&pc, precond, nx, ny, nz, fdn, a, sigma, cv, diag, &dinv, &F)) {
+        free(dinv); free(F);
+        return -2;
+    }
     double *V = (double *)malloc(sizeof(double) * n * (K + 1));
     double *t = (double *)malloc(sizeof(double) * n);
     double HH[(ORC_GMRES_MAXK + 2) * (ORC_GMRES_MAXK + 1)];
@@ -941,7 +985,7 @@ static int gmres_impl(int nx, int ny, int nz, int fdn, double a, double sigma, c
 #define HHE(r, c) HH[(c) * (K + 2) + (r)]
     int tlen = 0;
     ksp_conv cv_ = {0.0, 0.0, rtol, 0.0};
-    for (long i = 0; i < n; i++) t[i] = dinv[i] * b[i];
+    pc_apply(&pc, b, t, n, 0);
     cv_.bpnorm = orc_norm2(t, n);                       /* ||B b|| */
 
     int its = 0, reason = 0;
@@ -950,7 +994,8 @@ static int gmres_impl(int nx, int ny, int nz, int fdn, double a, double sigma, c
         /* KSPInitialResidual: v0 = B (b - A x) */
         double *v0 = V;
         OPV(x, t);
-        for (long i = 0; i < n; i++) v0[i] = dinv[i] * (b[i] - t[i]);
+        for (long i = 0; i < n; i++) t[i] = b[i] - t[i];
+        pc_apply(&pc, t, v0, n, 0);
         double res = ksp_normalize(v0, n);
         grs[0] = res;
         int it = 0;
@@ -963,7 +1008,7 @@ static int gmres_impl(int nx, int ny, int nz, int fdn, double a, double sigma, c
         while (!reason && it < K && its < max_it) {
             double *vn = V + (long)(it + 1) * n;
             OPV(V + (long)it * n, t);
-            for (long i = 0; i < n; i++) vn[i] = dinv[i] * t[i];
+            pc_apply(&pc, t, vn, n, 0);
             /* classical Gram-Schmidt: h_j = <w, v_j>, then w -= sum_j h_j v_j */
             for (int j = 0; j <= it; j++) h[j] = orc_dot(vn, V + (long)j * n, n);
             for (long i = 0; i < n; i++) {
@@ -1032,7 +1077,7 @@ static int gmres_impl(int nx, int ny, int nz, int fdn, double a, double sigma, c
 #undef HHE
     double t1 = now_seconds();
     ksp_finish(st, its, reason, x, b, t, nx, ny, nz, fdn, a, sigma, cv, coeff, tlen, t1 - t0);
-    free(dinv); free(V); free(t);
+    free(dinv); free(F); free(V); free(t);
     return 0;
 }
 
@@ -1052,13 +1097,18 @@ static int bicg_impl(int nx, int ny, int nz, int fdn, double a, double sigma, co
     orc_lap_coefficient(coeff, fdn);
     double diag = a * (3 * coeff[0]);
     if (sigma != 0.0) diag += sigma;
-    double *dinv = ksp_dinv(n, diag, cv, precond);
+    double *dinv, *F;
+    ksp_pc pc;
+    if (pc_setup(&pc, precond, nx, ny, nz, fdn, a, sigma, cv, diag, &dinv, &F)) {
+        free(dinv); free(F);
+        return -2;
+    }
     double *Rr = (double *)malloc(sizeof(double) * n), *Rl = (double *)malloc(sizeof(double) * n);
     double *Zr = (double *)malloc(sizeof(double) * n), *Zl = (double *)malloc(sizeof(double) * n);
     double *Pr = (double *)malloc(sizeof(double) * n), *Pl = (double *)malloc(sizeof(double) * n);
     int tlen = 0;
     ksp_conv cv_ = {0.0, 0.0, rtol, 0.0};
-    for (long i = 0; i < n; i++) Zr[i] = dinv[i] * b[i];
+    pc_apply(&pc, b, Zr, n, 0);
     cv_.bpnorm = orc_norm2(Zr, n);
     double gap = 0.0;
 
@@ -1066,10 +1116,8 @@ static int bicg_impl(int nx, int ny, int nz, int fdn, double a, double sigma, co
     OPV(x, Rr);
     for (long i = 0; i < n; i++) Rr[i] = b[i] + (-1.0) * Rr[i];     /* VecAYPX(Rr, -1, b) */
     memcpy(Rl, Rr, sizeof(double) * n);
-    for (long i = 0; i < n; i++) {
-        Zr[i] = dinv[i] * Rr[i];
-        Zl[i] = dinv[i] * Rl[i];              /* B^T = B (diagonal) */
-    }
+    pc_apply(&pc, Rr, Zr, n, 0);
+    pc_apply(&pc, Rl, Zl, n, 1);              /* B^T r~ */
     double dp = orc_norm2(Zr, n);
     if (trace && tlen < trace_cap) trace[tlen++] = dp;
     int its = 0;
@@ -1103,10 +1151,8 @@ static int bicg_impl(int nx, int ny, int nz, int fdn, double a, double sigma, co
                 Rr[k] = Rr[k] + (-al) * Zr[k];
                 Rl[k] = Rl[k] + (-al) * Zl[k];
             }
-            for (long k = 0; k < n; k++) {
-                Zr[k] = dinv[k] * Rr[k];
-                Zl[k] = dinv[k] * Rl[k];
-            }
+            pc_apply(&pc, Rr, Zr, n, 0);
+            pc_apply(&pc, Rl, Zl, n, 1);
             dp = orc_norm2(Zr, n);
             for (long k = 0; k < n; k++) {
                 const double d = fabs(Rl[k] - Rr[k]);
@@ -1123,7 +1169,7 @@ static int bicg_impl(int nx, int ny, int nz, int fdn, double a, double sigma, co
     double t1 = now_seconds();
     if (shadow_gap) *shadow_gap = gap;
     ksp_finish(st, its, reason, x, b, Rr, nx, ny, nz, fdn, a, sigma, cv, coeff, tlen, t1 - t0);
-    free(dinv); free(Rr); free(Rl); free(Zr); free(Zl); free(Pr); free(Pl);
+    free(dinv); free(F); free(Rr); free(Rl); free(Zr); free(Zl); free(Pr); free(Pl);
     return 0;
 }
 
@@ -1132,4 +1178,160 @@ int orc_bicg(int nx, int ny, int nz, int fdn, double a, double sigma, const doub
              double *trace, int trace_cap, orc_stats *st, double *shadow_gap)
 {
     return bicg_impl(nx, ny, nz, fdn, a, sigma, c, x, b, max_it, rtol, precond, trace, trace_cap, st, shadow_gap);
+}
+
+/* ------------------------------------------------------------------ */
+/* ILU(0) of the periodic FD12 star matrix: the preconditioner of the  */
+/* PETSc variant's pc = 1 (PCBJACOBI with its default sub-PC, PCILU at */
+/* 0 levels, PETSC_REAL/main.c:88-91; on one rank the block is the     */
+/* whole matrix, assembled as in PETSC_REAL/system.c:243-273).  For    */
+/* this symmetric matrix ICC(0) (PCICC of AAR_Real.c:58-60) defines    */
+/* the same preconditioner in exact arithmetic; no pivot is            */
+/* nonpositive on these operators, so PETSc's shifts never act.        */
+/* Needs every grid side >= 13 (the 37 columns of a row distinct).     */
+/*                                                                     */
+/* Entry layout per row: slot 0 = 1/u_ii (stored inverted, as PETSc's */
+/* factored AIJ keeps its diagonal), slot 1 + 12*axis + k for the      */
+/* column at periodic offset d (k = d + 6 for d < 0, d + 5 for d > 0) */
+/* along axis 0 = x, 1 = y, 2 = z: l_ij for columns below the row,    */
+/* u_ij above.  Elimination (IKJ, PETSc's MatLUFactorNumeric_SeqAIJ    */
+/* order): for the row's lower columns k in ascending index, l_ik =    */
+/* a_ik * (1/u_kk), then a_ij -= l_ik u_kj for the columns j > k of    */
+/* row k that are in row i's pattern.  On the star pattern those j lie */
+/* on the axis shared by i and k.  Solves: forward y_i = r_i - sum     */
+/* l_ik y_k (ascending k), backward z_i = (y_i - sum u_ij z_j)         */
+/* (ascending j) * (1/u_ii).                                           */
+/* ------------------------------------------------------------------ */
+static inline int ilu_slot(int axis, int d) { return 1 + 12 * axis + (d < 0 ? d + 6 : d + 5); }
+
+/* periodic offset of coordinate cj from c, in [-n/2, n/2] */
+static inline int pdiff(int cj, int c, int n)
+{
+    int d = cj - c;
+    if (d > n / 2) d -= n;
+    if (d < -n / 2) d += n;
+    return d;
+}
+
+int orc_ilu0_factor(int nx, int ny, int nz, int fdn, double a, double sigma, const double *cv, double *F)
+{
+    if (fdn != 6 || nx < 13 || ny < 13 || nz < 13) return -1;
+    const long n = (long)nx * ny * nz;
+    const int dims[3] = {nx, ny, nz};
+    const long strides[3] = {1, nx, (long)nx * ny};
+    double coeff[32] = {0};
+    orc_lap_coefficient(coeff, fdn);
+    double v[ILU_SLOTS];
+    for (long i = 0; i < n; i++) {
+        const int cr[3] = {(int)(i % nx), (int)((i / nx) % ny), (int)(i / ((long)nx * ny))};
+        v[0] = a * (3 * coeff[0]);
+        if (sigma != 0.0) v[0] += sigma;
+        if (cv) v[0] = v[0] + cv[i];
+        for (int ax = 0; ax < 3; ax++)
+            for (int d = 1; d <= 6; d++) {
+                v[ilu_slot(ax, -d)] = a * coeff[d];
+                v[ilu_slot(ax, d)] = a * coeff[d];
+            }
+        for (int ax = 2; ax >= 0; ax--) {           /* z, y, x: ascending global index */
+            const int c = cr[ax], na = dims[ax];
+            for (int cp = 0; cp < c; cp++) {
+                const int dk = pdiff(cp, c, na);
+                if (dk < -6 || dk > 6) continue;
+                const long kk = i + (long)(cp - c) * strides[ax];
+                const double l = v[ilu_slot(ax, dk)] * F[kk * ILU_SLOTS];
+                v[ilu_slot(ax, dk)] = l;
+                /* row kk's upper columns on this axis, ascending */
+                for (int cj = cp + 1; cj < na; cj++) {
+                    const int dkj = pdiff(cj, cp, na);
+                    if (dkj < -6 || dkj > 6) continue;
+                    const int dij = pdiff(cj, c, na);
+                    if (dij < -6 || dij > 6) continue;
+                    const int si = dij == 0 ? 0 : ilu_slot(ax, dij);
+                    v[si] = v[si] - l * F[kk * ILU_SLOTS + ilu_slot(ax, dkj)];
+                }
+            }
+        }
+        if (v[0] == 0.0) return -2;
+        v[0] = 1.0 / v[0];
+        memcpy(F + i * ILU_SLOTS, v, sizeof(v));
+    }
+    return 0;
+}
+
+/* lower (below = 1) or upper (below = 0) columns of row i along `ax`, in
+   ascending coordinate order: returns their count, coordinates in cs[] and
+   periodic offsets in ds[] */
+static int axis_cols(int c, int na, int below, int *cs, int *ds)
+{
+    int m = 0;
+    const int lo = below ? 0 : c + 1, hi = below ? c : na;
+    for (int cj = lo; cj < hi; cj++) {
+        const int d = pdiff(cj, c, na);
+        if (d < -6 || d > 6) continue;
+        cs[m] = cj;
+        ds[m] = d;
+        m++;
+    }
+    return m;
+}
+
+/* out = M^{-1} in (trans = 0) or M^{-T} in (trans = 1); in-place allowed */
+void orc_ilu0_apply(int nx, int ny, int nz, const double *F, const double *in, double *out, int trans)
+{
+    const long n = (long)nx * ny * nz;
+    const int dims[3] = {nx, ny, nz};
+    const long strides[3] = {1, nx, (long)nx * ny};
+    int cs[12], ds[12];
+    if (out != in) memcpy(out, in, sizeof(double) * n);
+    if (!trans) {
+        /* L y = r: lower columns ascending (z, y, x groups) */
+        for (long i = 0; i < n; i++) {
+            const int cr[3] = {(int)(i % nx), (int)((i / nx) % ny), (int)(i / ((long)nx * ny))};
+            double s = out[i];
+            for (int ax = 2; ax >= 0; ax--) {
+                const int m = axis_cols(cr[ax], dims[ax], 1, cs, ds);
+                for (int q = 0; q < m; q++)
+                    s = s - F[i * ILU_SLOTS + ilu_slot(ax, ds[q])] * out[i + (long)(cs[q] - cr[ax]) * strides[ax]];
+            }
+            out[i] = s;
+        }
+        /* U z = y: upper columns ascending (x, y, z groups), times 1/u_ii */
+        for (long i = n - 1; i >= 0; i--) {
+            const int cr[3] = {(int)(i % nx), (int)((i / nx) % ny), (int)(i / ((long)nx * ny))};
+            double s = out[i];
+            for (int ax = 0; ax < 3; ax++) {
+                const int m = axis_cols(cr[ax], dims[ax], 0, cs, ds);
+                for (int q = 0; q < m; q++)
+                    s = s - F[i * ILU_SLOTS + ilu_slot(ax, ds[q])] * out[i + (long)(cs[q] - cr[ax]) * strides[ax]];
+            }
+            out[i] = s * F[i * ILU_SLOTS];
+        }
+    } else {
+        /* U^T w = r: w_i = (r_i - sum_{j < i} u_ji w_j) / u_ii, j ascending */
+        for (long i = 0; i < n; i++) {
+            const int cr[3] = {(int)(i % nx), (int)((i / nx) % ny), (int)(i / ((long)nx * ny))};
+            double s = out[i];
+            for (int ax = 2; ax >= 0; ax--) {
+                const int m = axis_cols(cr[ax], dims[ax], 1, cs, ds);
+                for (int q = 0; q < m; q++) {
+                    const long j = i + (long)(cs[q] - cr[ax]) * strides[ax];
+                    s = s - F[j * ILU_SLOTS + ilu_slot(ax, -ds[q])] * out[j];
+                }
+            }
+            out[i] = s * F[i * ILU_SLOTS];
+        }
+        /* L^T z = w: z_i = w_i - sum_{j > i} l_ji z_j, j ascending */
+        for (long i = n - 1; i >= 0; i--) {
+            const int cr[3] = {(int)(i % nx), (int)((i / nx) % ny), (int)(i / ((long)nx * ny))};
+            double s = out[i];
+            for (int ax = 0; ax < 3; ax++) {
+                const int m = axis_cols(cr[ax], dims[ax], 0, cs, ds);
+                for (int q = 0; q < m; q++) {
+                    const long j = i + (long)(cs[q] - cr[ax]) * strides[ax];
+                    s = s - F[j * ILU_SLOTS + ilu_slot(ax, -ds[q])] * out[j];
+                }
+            }
+            out[i] = s;
+        }
+    }
 }

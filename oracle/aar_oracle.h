@@ -151,6 +151,20 @@ void orc_matvec_complex(int nx, int ny, int nz, int fdn, const double *coeff,
 #define ORC_KSP_DIVERGED_BREAKDOWN (-5)
 #define ORC_KSP_DIVERGED_BREAKDOWN_BICG (-6)
 #define ORC_KSP_DIVERGED_NANORINF (-9)
+/* precond for orc_gmres / orc_bicg: 0 none, 1 Jacobi, 2 ILU(0) (below). */
+
+/* ILU(0) of the periodic FD12 star matrix a*Lap + sigma (+ c(x)), the
+ * PETSc variant's pc = 1 preconditioner on one rank (PCBJACOBI -> PCILU,
+ * PETSC_REAL/main.c:88-91; PCICC of AAR_Real.c:58-60 is the same
+ * preconditioner in exact arithmetic on this symmetric matrix).  F holds
+ * ORC_ILU_SLOTS doubles per row: [0] = 1/u_ii, [1 + 12*axis + k] the entry
+ * at periodic offset d along axis (0 x, 1 y, 2 z), k = d + 6 (d < 0) or
+ * d + 5 (d > 0); l_ij below the row, u_ij above.  Every side >= 13.
+ * Returns 0, -1 (bad size) or -2 (zero pivot).  orc_ilu0_apply: out =
+ * M^{-1} in (trans = 0) or M^{-T} in (trans = 1), in place allowed. */
+#define ORC_ILU_SLOTS 37
+int orc_ilu0_factor(int nx, int ny, int nz, int fdn, double a, double sigma, const double *c, double *F);
+void orc_ilu0_apply(int nx, int ny, int nz, const double *F, const double *in, double *out, int trans);
 int orc_gmres(int nx, int ny, int nz, int fdn, double a, double sigma, const double *c,
               double *x, const double *b, int restart, int max_it, double rtol, int precond,
               double *trace, int trace_cap, orc_stats *st);

@@ -93,8 +93,11 @@ def lib():
         L.orc_cg_vc.argtypes = [I, I, I, I, Dd, _D, _D, _D, I, Dd, I, _D, I, ctypes.POINTER(OrcStats)]
         L.orc_gmres.argtypes = [I, I, I, I, Dd, Dd, _D, _D, _D, I, I, Dd, I, _D, I, ctypes.POINTER(OrcStats)]
         L.orc_bicg.argtypes = [I, I, I, I, Dd, Dd, _D, _D, _D, I, Dd, I, _D, I, ctypes.POINTER(OrcStats), _D]
+        L.orc_ilu0_factor.argtypes = [I, I, I, I, Dd, Dd, _D, _D]
+        L.orc_ilu0_apply.argtypes = [I, I, I, _D, _D, _D, I]
+        L.orc_ilu0_apply.restype = None
         for fn in ("orc_aar", "orc_pgr", "orc_cg", "orc_aar_complex", "orc_aar_vc", "orc_pgr_vc", "orc_cg_vc",
-                   "orc_gmres", "orc_bicg"):
+                   "orc_gmres", "orc_bicg", "orc_ilu0_factor"):
             getattr(L, fn).restype = I
         _lib = L
     return _lib
@@ -238,6 +241,32 @@ def cg(x0, rhs, shape, a, max_iter, tol, sigma=0.0, precond=True, fdn=6, trace=T
     return x, d
 
 
+ILU_SLOTS = 37
+PC_NONE, PC_JACOBI, PC_ILU0 = 0, 1, 2
+
+
+def ilu0_factor(shape, a, sigma=0.0, c=None, fdn=6):
+    """ILU(0) factors of the periodic FD12 matrix, (n, 37) per row (see
+    aar_oracle.h for the slot layout)."""
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    F = np.zeros(n * ILU_SLOTS)
+    cv = None if c is None else _f64(c).reshape(-1)
+    rc = lib().orc_ilu0_factor(nx, ny, nz, fdn, a, sigma, _p(cv), _p(F))
+    if rc:
+        raise ValueError(f"orc_ilu0_factor failed ({rc})")
+    return F.reshape(n, ILU_SLOTS)
+
+
+def ilu0_apply(shape, F, r, trans=False):
+    nx, ny, nz = shape
+    F = np.ascontiguousarray(F, dtype=np.float64).reshape(-1)
+    r = _f64(r).reshape(-1)
+    out = np.empty_like(r)
+    lib().orc_ilu0_apply(nx, ny, nz, _p(F), _p(r), _p(out), int(bool(trans)))
+    return out
+
+
 # KSPConvergedReason codes of the Krylov comparators
 KSP_CONVERGED_RTOL, KSP_CONVERGED_ATOL = 2, 3
 KSP_DIVERGED_ITS, KSP_DIVERGED_BREAKDOWN, KSP_DIVERGED_BREAKDOWN_BICG = -3, -5, -6
@@ -245,9 +274,10 @@ KSP_DIVERGED_ITS, KSP_DIVERGED_BREAKDOWN, KSP_DIVERGED_BREAKDOWN_BICG = -3, -5, 
 
 def gmres(x0, rhs, shape, a, max_iter, tol, restart=30, sigma=0.0, c=None, precond=True, fdn=6,
           trace=True):
-    """PETSc KSPGMRES + PCJACOBI as PETSC_REAL/main.c:86-122 runs it
-    (restatement of PETSc's published algorithm).  stats['status'] is the
-    KSPConvergedReason; 'trace' holds every convergence-test norm."""
+    """PETSc KSPGMRES as PETSC_REAL/main.c:86-122 runs it (restatement of
+    PETSc's published algorithm); precond: 0/False none, 1/True Jacobi,
+    2 ILU(0).  stats['status'] is the KSPConvergedReason; 'trace' holds
+    every convergence-test norm."""
     nx, ny, nz = shape
     x = _f64(x0).reshape(-1).copy()
     b = _f64(rhs).reshape(-1)

@@ -174,3 +174,179 @@ def test_gmres_variable_coefficient_reduces_to_screened(oracle):
     x1, s1 = oracle.bicg(np.ones(b.size), b, shape, a, 300, 1e-8, sigma=sigma)
     x2, s2 = oracle.bicg(np.ones(b.size), b, shape, a, 300, 1e-8, c=c)
     assert np.array_equal(x1, x2) and np.array_equal(s1["trace"], s2["trace"])
+
+
+# ---------------------------------------------------------------------------
+# ILU(0) (the pc = 1 preconditioner of the PETSc variant, one block)
+# ---------------------------------------------------------------------------
+def slot(axis, d):
+    return 1 + 12 * axis + (d + 6 if d < 0 else d + 5)
+
+
+def pdiff(cj, c, n):
+    d = cj - c
+    if d > n // 2:
+        d -= n
+    if d < -(n // 2):
+        d += n
+    return d
+
+
+def star_rows(shape, a, sigma, c=None):
+    """The assembled matrix as {column: value} per row (PETSC_REAL/system.c:243-273)."""
+    from oracle import oracle as O
+    nx, ny, nz = shape
+    cf = O.lap_coefficient(6)
+    rows = []
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                r = len(rows)
+                d = {r: a * (3 * cf[0]) + sigma + (0.0 if c is None else c[r])}
+                for l in range(1, 7):
+                    for q in (((i - l) % nx) + j * nx + k * nx * ny, ((i + l) % nx) + j * nx + k * nx * ny,
+                              i + ((j - l) % ny) * nx + k * nx * ny, i + ((j + l) % ny) * nx + k * nx * ny,
+                              i + j * nx + ((k - l) % nz) * nx * ny, i + j * nx + ((k + l) % nz) * nx * ny):
+                        d[q] = d.get(q, 0.0) + a * cf[l]
+                rows.append(d)
+    return rows
+
+
+def generic_ilu0(rows):
+    """Textbook IKJ ILU(0) on a general sparse pattern (columns ascending;
+    the pivot used through its reciprocal); returns per-row dicts of L
+    (below the diagonal) and U entries and the reciprocal pivots."""
+    n = len(rows)
+    U = [dict(r) for r in rows]
+    inv = [0.0] * n
+    for i in range(n):
+        ri = U[i]
+        for kk in sorted(q for q in ri if q < i):
+            l = ri[kk] * inv[kk]
+            ri[kk] = l
+            for jj in sorted(q for q in U[kk] if q > kk):
+                if jj in ri:
+                    ri[jj] = ri[jj] - l * U[kk][jj]
+        inv[i] = 1.0 / ri[i]
+    return U, inv
+
+
+def to_slots(shape, U, inv):
+    nx, ny, nz = shape
+    n = len(U)
+    F = np.zeros((n, 37))
+    dims, strides = (nx, ny, nz), (1, nx, nx * ny)
+    for i in range(n):
+        cr = (i % nx, (i // nx) % ny, i // (nx * ny))
+        F[i, 0] = inv[i]
+        for col, v in U[i].items():
+            if col == i:
+                continue
+            cc = (col % nx, (col // nx) % ny, col // (nx * ny))
+            ax = [q for q in range(3) if cc[q] != cr[q]]
+            assert len(ax) == 1
+            ax = ax[0]
+            F[i, slot(ax, pdiff(cc[ax], cr[ax], dims[ax]))] = v
+    return F
+
+
+@pytest.mark.parametrize("shape,sigma,vc", [((13, 13, 13), 0.0, False), ((14, 13, 15), 0.5, False),
+                                            ((13, 14, 13), 0.0, True)])
+def test_ilu0_factor_equals_generic_ikj(oracle, shape, sigma, vc):
+    a = -4.0
+    n = int(np.prod(shape))
+    c = np.random.default_rng(3).uniform(0.5, 1.5, n) if vc else None
+    F = oracle.ilu0_factor(shape, a, sigma=sigma, c=c)
+    U, inv = generic_ilu0(star_rows(shape, a, sigma, c))
+    assert np.array_equal(F, to_slots(shape, U, inv))
+
+
+def test_ilu0_defining_identity_and_solves(oracle):
+    """(L U)_ij = A_ij on the pattern; the apply is the dense triangular
+    solves with L and U, and its transpose."""
+    import scipy.linalg as sl
+    shape = (13, 13, 13)
+    a, sigma = -4.0, 0.25
+    rows = star_rows(shape, a, sigma)
+    F = oracle.ilu0_factor(shape, a, sigma=sigma)
+    U, inv = generic_ilu0(rows)
+    n = len(rows)
+    Ld = np.eye(n)
+    Ud = np.zeros((n, n))
+    for i in range(n):
+        for col, v in U[i].items():
+            if col < i:
+                Ld[i, col] = v
+            else:
+                Ud[i, col] = v
+    LU = Ld @ Ud
+    for i in range(0, n, 7):
+        for col, v in rows[i].items():
+            assert LU[i, col] == pytest.approx(v, rel=1e-12, abs=1e-12)
+    r = np.random.default_rng(4).standard_normal(n)
+    z = sl.solve_triangular(Ud, sl.solve_triangular(Ld, r, lower=True, unit_diagonal=True), lower=False)
+    np.testing.assert_allclose(oracle.ilu0_apply(shape, F, r), z, rtol=1e-11, atol=1e-14)
+    zt = sl.solve_triangular(Ld.T, sl.solve_triangular(Ud.T, r, lower=True), lower=False, unit_diagonal=True)
+    np.testing.assert_allclose(oracle.ilu0_apply(shape, F, r, trans=True), zt, rtol=1e-11, atol=1e-14)
+    # in exact arithmetic ILU(0) of a symmetric matrix is symmetric (= ICC(0))
+    M = LU
+    assert np.abs(M - M.T).max() <= 1e-11 * np.abs(M).max()
+
+
+def test_gmres_ilu0_estimate_is_krylov_minimum(oracle):
+    shape = (13, 13, 13)
+    a, sigma = -4.0, 0.5
+    b = rhs(shape, 21)
+    x0 = np.ones(b.size)
+    A = dense_operator(oracle, shape, a, sigma)
+    F = oracle.ilu0_factor(shape, a, sigma=sigma)
+    Minv = lambda v: oracle.ilu0_apply(shape, F, v)  # noqa: E731
+    r0 = Minv(b - A @ x0)
+    _, st = oracle.gmres(x0, b, shape, a, 10, 1e-14, restart=30, sigma=sigma, precond=2)
+    tr = st["trace"]
+    assert tr[0] == pytest.approx(np.linalg.norm(r0), rel=1e-13)
+    Q = [r0 / np.linalg.norm(r0)]
+    MA = np.array([Minv(A[:, j]) for j in range(A.shape[1])]).T
+    for k in range(1, 11):
+        v = MA @ Q[-1]
+        for _ in range(2):
+            for q in Q:
+                v = v - (q @ v) * q
+        Q.append(v / np.linalg.norm(v))
+        K = np.array(Q[:k]).T
+        y, *_ = np.linalg.lstsq(MA @ K, r0, rcond=None)
+        assert tr[k] == pytest.approx(np.linalg.norm(r0 - MA @ K @ y), rel=1e-7), k
+
+
+def test_bicg_ilu0_is_pcg_with_the_ilu_preconditioner(oracle):
+    shape = (13, 13, 13)
+    a, sigma = -4.0, 0.25
+    b = rhs(shape, 22)
+    A = dense_operator(oracle, shape, a, sigma)
+    F = oracle.ilu0_factor(shape, a, sigma=sigma)
+    Minv = lambda v: oracle.ilu0_apply(shape, F, v)  # noqa: E731
+    x0 = np.ones(b.size)
+    x, st = oracle.bicg(x0, b, shape, a, 500, 1e-8, sigma=sigma, precond=2)
+    assert st["status"] == oracle.KSP_CONVERGED_RTOL
+    assert st["shadow_gap"] < 1e-10 * np.abs(b).max()      # M^-T = M^-1 up to rounding
+    xx = x0.copy()
+    r = b - A @ xx
+    z = Minv(r)
+    p = z.copy()
+    rz = r @ z
+    norms = [np.linalg.norm(z)]
+    ttol = 1e-8 * np.linalg.norm(Minv(b))
+    while norms[-1] > ttol:
+        q = A @ p
+        al = rz / (p @ q)
+        xx = xx + al * p
+        r = r - al * q
+        z = Minv(r)
+        norms.append(np.linalg.norm(z))
+        rzn = r @ z
+        p = z + (rzn / rz) * p
+        rz = rzn
+    assert st["iterations"] == len(norms) - 1
+    np.testing.assert_allclose(st["trace"], norms, rtol=1e-5)
+    ref = fft_solve(b, shape, a, oracle.lap_coefficient(6), sigma)
+    assert np.abs(x - ref).max() <= 1e-6 * np.abs(ref).max()
