@@ -161,6 +161,19 @@ int aar_hip_bicg(const aar_hip_grid *g, const double *c, double *x, const double
  * (x + y + z = const planes), bit-identical to a sequential IKJ ILU(0). */
 int aar_hip_ilu0_apply(const aar_hip_grid *g, const double *c, const double *r, double *z, int trans);
 
+/* PETSC_REAL/AAR_Real.c:32-199 -- the PETSc variant's AAR with its
+ * preconditioner choice (AAR_Real.c:57-66): precond 1 = PCJACOBI (multiply
+ * by 1/diag), 2 = its pc = 1, PCICC on the one block, realised as the ILU(0)
+ * above (the same preconditioner in exact arithmetic on this symmetric
+ * matrix), 0 = none.  Its semantics where they differ from AAR.c: the
+ * Anderson step runs whenever iter % p == 0 (iter = 1 included), the
+ * update is x += sum_i (-s_i)(beta DF_i + DX_i) column by column, then
+ * x += beta f.  Its dgelsd least squares is replaced by the tools.c
+ * pseudo-inverse.  m <= 16; ||b - A x|| is checked every p iterations
+ * (aar_hip_history); iterations = iter - 1 as AAR_Real.c:162-166 prints. */
+int aar_hip_aar_petsc(const aar_hip_grid *g, const double *c, double *x, const double *b, double omega,
+                      double beta, int m, int p, int max_iter, double tol, int precond, aar_hip_stats *st);
+
 /* Extension (no reference counterpart): FP64 residual refinement around
  * FP32 AAR.  Outer: r = b - A x in FP64, stop at ||r|| <= tol*||b||;
  * inner: FP32 AAR on A e = r from e = 0 to relative tolerance

@@ -245,6 +245,24 @@ def GMRES(x, b, shape, a, max_iter=2000, tol=1e-6, restart=30, sigma=0.0, precon
     return st.as_dict()
 
 
+def AAR_petsc(x, b, shape, a, omega, beta, m, p, max_iter, tol, sigma=0.0, precond=ILU0, c=None):
+    """The PETSc variant's AAR (PETSC_REAL/AAR_Real.c:32-199): precond
+    ILU0 (its pc = 1, PCICC, as ILU(0)), True/1 Jacobi, False/0 none."""
+    import torch
+    _check_len(x, shape, "x")
+    _check_len(b, shape, "b")
+    g = _grid(shape, a, sigma)
+    st = Stats()
+    _bind_stream()
+    cp = None
+    if c is not None:
+        _check_len(c, shape, "c")
+        cp = _dev(c, torch.float64, "c")
+    check(_L().aar_hip_aar_petsc(ctypes.byref(g), cp, _dev(x, torch.float64, "x"), _dev(b, torch.float64, "b"),
+                                 omega, beta, m, p, max_iter, tol, _pc(precond), ctypes.byref(st)))
+    return st.as_dict()
+
+
 def ilu0_apply(r, shape, a, sigma=0.0, c=None, trans=False):
     """z = M^{-1} r (trans: M^{-T} r) for the ILU(0) factors of a*Lap +
     sigma (+ c): the PETSc variant's pc = 1 preconditioner on one rank."""

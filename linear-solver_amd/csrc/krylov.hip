@@ -419,6 +419,91 @@ __global__ __launch_bounds__(KB) void k_bi2_upd_kernel(long n, double *x, double
     }
 }
 
+// --- AAR of the PETSc variant (AAR_Real.c:32-199) with a general B ---
+// f = B r for the pointwise preconditioners (Jacobi or none)
+__global__ __launch_bounds__(KB) void k_pcj_kernel(long n, const double *r, double *f, KDiag dg, const int *skip)
+{
+    if (*skip) return;
+    KLOOP(i) f[i] = kdinv(dg, i) * r[i];
+}
+
+// AAR_Real.c:107-119: DX_k = x - x_old, DF_k = f - f_old (iter > 1);
+// x_old = x, f_old = f; Richardson x += omega f
+__global__ __launch_bounds__(KB) void k_aa_hist_kernel(long n, double *x, const double *f, double *x_old, double *f_old,
+                                                      double *dx, double *df, int hist, int rich, double omega,
+                                                      const KCtrl *k)
+{
+    if (k->c.done) return;
+    KLOOP(i)
+    {
+        const double xi = x[i], fi = f[i];
+        if (hist) {
+            dx[i] = (-1.0) * x_old[i] + xi;
+            df[i] = (-1.0) * f_old[i] + fi;
+        }
+        x_old[i] = xi;
+        f_old[i] = fi;
+        if (rich) x[i] = xi + omega * fi;
+    }
+}
+
+// AAR_Real.c:128-137: x += sum_q (-s_q)(beta DF_q + DX_q), then x += beta f
+template <int MM>
+__global__ __launch_bounds__(KB) void k_aa_upd_kernel(long n, double *x, const double *f, const double *W, long ld,
+                                                     int m, double beta, const KCtrl *k)
+{
+    if (k->c.done) return;
+    double ns[MM];
+#pragma unroll
+    for (int q = 0; q < MM; q++) ns[q] = q < m ? -k->c.s[q] : 0.0;
+    KLOOP(i)
+    {
+        double xi = x[i];
+#pragma unroll
+        for (int q = 0; q < MM; q++)
+            if (q < m) xi = xi + ns[q] * (beta * W[(size_t)(1 + q) * ld + i] + W[(size_t)(1 + m + q) * ld + i]);
+        x[i] = xi + beta * f[i];
+    }
+}
+
+// res = b - t (VecAYPX(res, -1, b)); ||res||^2 when checked
+__global__ __launch_bounds__(KB) void k_aa_res_kernel(long n, const double *b, double *t, int check, double *part,
+                                                     const int *skip)
+{
+    if (*skip) return;
+    double acc[1] = {0.0};
+    KLOOP(i)
+    {
+        const double r = b[i] + (-1.0) * t[i];
+        t[i] = r;
+        acc[0] += r * r;
+    }
+    if (check) k_block_sums<1>(acc, 1, part);
+}
+
+__global__ void k_aa_init_kernel(KCtrl *k, double tol)
+{
+    k->c.tol_abs = tol * k->rhs_norm;      // AAR_Real.c:81
+    k->c.relres = k->c.tol_abs + 1;        // r_2norm = tol + 1 (AAR_Real.c:91)
+}
+
+// the check of iteration `iter` (AAR_Real.c:149-150, 98)
+__global__ __launch_bounds__(KB) void k_aa_check_kernel(KCtrl *k, const double *part, int nb, double *hist, int iter)
+{
+    if (k->c.done) return;
+    __shared__ double s[KP];
+    k_reduce(part, nb, 1, s);
+    if (threadIdx.x != 0) return;
+    const double r = sqrt(s[0]);
+    k->c.relres = r;
+    k->its = iter;
+    k_push(k, hist, r);
+    if (!(r > k->c.tol_abs)) {
+        k->c.done = 1;
+        k->c.final_iter = iter;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // control kernels (one block; thread 0 runs the scalar recurrences)
 // ---------------------------------------------------------------------------
@@ -1024,6 +1109,141 @@ int ilu0_apply(const Op &op, const double *r, double *z, int trans)
     HC(ilu_factor(a, C.stream));
     HC(ilu_apply(a, r, z, trans != 0, &k->zero, C.stream));
     HC(hipStreamSynchronize(C.stream));
+    return 0;
+}
+
+// AAR of the PETSc variant (PETSC_REAL/AAR_Real.c:32-199) with its
+// preconditioner choice: Jacobi by the reciprocal diagonal (precond 1),
+// ILU(0) standing in for PCICC (2; the same preconditioner in exact
+// arithmetic on this symmetric matrix) or none (0).  A plain iteration on
+// x: each step applies B to r = b - A x, updates the history columns,
+// takes the Richardson or Anderson step and recomputes r; the Anderson
+// least squares uses the Gram and pseudo-inverse kernels of the MPI_REAL
+// solver (the tools.c SVD in place of the variant's dgelsd).
+int aar_petsc_solve(const Op &op, double *x, const double *b, double omega, double beta, int m, int p,
+                    int max_iter, double tol, int precond, aar_hip_stats *st)
+{
+    Context &C = ctx();
+    if (int rc = context_init()) return rc;
+    if (!x || !b) return fail(AAR_HIP_ERR_ARG, "null vector");
+    if (max_iter < 0) return fail(AAR_HIP_ERR_ARG, "max_iter must be >= 0");
+    if (m < 1 || m > MAXM) return fail(AAR_HIP_ERR_ARG, "history m must be in 1..16");
+    if (p < 1) return fail(AAR_HIP_ERR_ARG, "p must be >= 1");
+    if (precond < 0 || precond > 2) return fail(AAR_HIP_ERR_ARG, "precond must be 0 (none), 1 (Jacobi) or 2 (ILU(0))");
+    const bool ilu = precond == 2;
+    if (ilu)
+        if (int rc = check_ilu_grid(op)) return rc;
+    const long n = op.n;
+    const long ld = ring_ld(n, sizeof(double));
+    // slots: 0 f, 1..m DF, m+1..2m DX, 2m+1 x_old, 2m+2 f_old, 2m+3 r
+    HC(ensure(C.kvec, sizeof(double) * (size_t)ld * (2 * m + 4)));
+    double *W = (double *)C.kvec.p;
+    double *f = W, *x_old = W + (size_t)(2 * m + 1) * ld, *f_old = W + (size_t)(2 * m + 2) * ld;
+    double *r = W + (size_t)(2 * m + 3) * ld;
+    double *F = nullptr;
+    if (ilu) {
+        HC(ensure(C.kilu, sizeof(double) * (size_t)n * IFS));
+        F = (double *)C.kilu.p;
+    }
+    const int gnb = nblocks_for(n);
+    HC(ensure(C.part0, sizeof(double) * (size_t)gnb * GRAM_NA_MAX));
+    const int hcap = max_iter / p + 8;
+    C.bytes_acc = 0.0;
+    HC(hipEventRecord(C.t0, C.stream));
+    KrylovRun R;
+    if (int rc = k_setup(op, precond, tol, max_iter, 0, hcap, b, R, F, r)) return rc;
+    // the history columns start at zero (VecDuplicateVecs)
+    hipLaunchKernelGGL(k_zero_kernel, dim3(1024), dim3(KB), 0, C.stream, (unsigned *)(W + ld),
+                       (long)(2 * m * ld * 2));
+    hipLaunchKernelGGL(k_aa_init_kernel, dim3(1), dim3(1), 0, C.stream, R.k, tol);
+    const double s = 8.0 * n, ib = ilu ? ILU_APPLY_BYTES * n : 0.0;
+    const double cb = op.c ? s : 0.0;
+    const int *done = &R.k->c.done;
+    if (int rc = k_matvec(op, x, r, nullptr)) return rc;
+    hipLaunchKernelGGL(k_aa_res_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, b, r, 0, R.part, (const int *)&R.k->zero);
+    HC(hipGetLastError());
+    C.bytes_acc += 5 * s + cb;
+    bool stopped = false;
+    int cyc = 0;
+    for (int iter = 1; iter <= max_iter && !stopped; iter++) {
+        if (ilu) {
+            prof_begin("krylov_ilu_apply");
+            HC(ilu_apply(R.ilu, r, f, false, done, C.stream));
+            prof_end(ib);
+        } else {
+            hipLaunchKernelGGL(k_pcj_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, (const double *)r, f, R.dg, done);
+        }
+        const int hk = iter > 1 ? (iter - 2) % m : 0;
+        const bool rich = (iter % p) != 0;
+        hipLaunchKernelGGL(k_aa_hist_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, x, (const double *)f, x_old, f_old,
+                           W + (size_t)(1 + m + hk) * ld, W + (size_t)(1 + hk) * ld, iter > 1 ? 1 : 0, rich ? 1 : 0,
+                           omega, (const KCtrl *)R.k);
+        C.bytes_acc += (ilu ? ib : 2 * s + cb) + (iter > 1 ? 8 : 6) * s;
+        if (!rich) {
+            GramArgs<double> g;
+            g.n = n;
+            g.m = m;
+            g.kind = 0;
+            g.direct = 1;
+            g.F = W;
+            g.ld = ld;
+            g.nvalid = m;
+            for (int j = 0; j < m; j++) g.slot[j] = 1 + j;
+            g.fslot = 0;
+            g.nblocks = gnb;
+            g.partials = R.part;
+            g.counter = nullptr;
+            g.ctrl = &R.k->c;
+            HC(launch_gram<double>(g, C.stream));
+            HC(launch_ls(&R.k->c, m, 0, C.stream));
+            if (m <= 8) hipLaunchKernelGGL((k_aa_upd_kernel<8>), dim3(R.nb), dim3(KB), 0, C.stream, n, x,
+                                           (const double *)f, (const double *)W, ld, m, beta, (const KCtrl *)R.k);
+            else hipLaunchKernelGGL((k_aa_upd_kernel<16>), dim3(R.nb), dim3(KB), 0, C.stream, n, x,
+                                    (const double *)f, (const double *)W, ld, m, beta, (const KCtrl *)R.k);
+            HC(hipGetLastError());
+            C.bytes_acc += (m + 1) * s + (2.0 * m + 3) * s;
+        }
+        if (int rc = k_matvec(op, x, r, &R.k->c)) return rc;
+        const bool check = !rich;
+        hipLaunchKernelGGL(k_aa_res_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, b, r, check ? 1 : 0, R.part, done);
+        if (check)
+            hipLaunchKernelGGL(k_aa_check_kernel, dim3(1), dim3(KB), 0, C.stream, R.k, (const double *)R.part, R.nb,
+                               R.hist, iter);
+        HC(hipGetLastError());
+        C.bytes_acc += 3 * s;
+        if (check) {
+            if (int rc = poll_record(&R.k->c, cyc)) return rc;
+            if (poll_check(cyc - 2)) stopped = true;
+            cyc++;
+        }
+    }
+    HC(hipEventRecord(C.t1, C.stream));
+    KCtrl hk;
+    HC(hipMemcpyAsync(&hk, R.k, sizeof(KCtrl), hipMemcpyDeviceToHost, C.stream));
+    HC(hipStreamSynchronize(C.stream));
+    float ms = 0.f;
+    HC(hipEventElapsedTime(&ms, C.t0, C.t1));
+    prof_flush();
+    const int hl = std::min(hk.nh, R.hcap);
+    C.hist_host.assign(std::max(hl, 0), 0.0);
+    if (hl > 0) HC(hipMemcpy(C.hist_host.data(), R.hist, sizeof(double) * hl, hipMemcpyDeviceToHost));
+    // AAR_Real.c:162-166: iter - 1 iterations (iter stops one past the last)
+    const int iterations = hk.c.done ? hk.c.final_iter : max_iter;
+    aar_hip_stats sres;
+    std::memset(&sres, 0, sizeof(sres));
+    sres.iterations = iterations;
+    sres.rhs_norm = hk.rhs_norm;
+    sres.relres = hk.rhs_norm > 0 ? hk.c.relres / hk.rhs_norm : hk.c.relres;
+    sres.converged = hk.c.relres <= hk.c.tol_abs;
+    sres.printed_converged = iterations + 1 < max_iter;
+    sres.printed_warning = !(iterations + 1 < max_iter);
+    sres.status = hk.c.status == 1 ? AAR_HIP_ERR_SVD : 0;
+    sres.hist_len = hl;
+    sres.solve_ms = ms;
+    sres.bytes = C.bytes_acc;
+    C.last = sres;
+    if (st) *st = sres;
+    if (hk.c.status == 1) return fail(AAR_HIP_ERR_SVD, "no convergence in 250 svd iterations");
     return 0;
 }
 

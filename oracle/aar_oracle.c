@@ -1335,3 +1335,88 @@ void orc_ilu0_apply(int nx, int ny, int nz, const double *F, const double *in, d
         }
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* AAR of the PETSc variant, PETSC_REAL/AAR_Real.c:32-199, on one rank */
+/* with its preconditioner choice (AAR_Real.c:57-66): precond 1 =      */
+/* PCJACOBI (multiply by 1/diag), 2 = ILU(0) in place of PCICC (the    */
+/* same preconditioner in exact arithmetic on this symmetric matrix;   */
+/* no pivot shift acts), 0 = none.  Differences from MPI_REAL/AAR.c:   */
+/* the Anderson step runs whenever iter % p == 0, iter = 1 included    */
+/* (AAR_Real.c:116); the update is x += sum_i (-s_i)(beta DF_i + DX_i) */
+/* one column at a time, then x += beta f (AAR_Real.c:132-137); the    */
+/* least-squares solve is MKL's dgelsd (AAR_Real.c:223), absent here:  */
+/* the tools.c pseudo-inverse stands in for it (the two agree to       */
+/* rounding on a full-rank Gram; parity with dgelsd is unpinned).      */
+/* ------------------------------------------------------------------ */
+int orc_aar_petsc(int nx, int ny, int nz, int fdn, double a, double sigma, const double *cv,
+                  double *x, const double *b, double omega, double beta, int m, int p, int max_iter,
+                  double tol, int precond, double *trace, int trace_cap, orc_stats *st)
+{
+    if (m < 1 || m > 32 || p < 1) return -1;
+    const long n = (long)nx * ny * nz;
+    double coeff[32] = {0};
+    orc_lap_coefficient(coeff, fdn);
+    double diag = a * (3 * coeff[0]);
+    if (sigma != 0.0) diag += sigma;
+    double *dinv, *F;
+    ksp_pc pc;
+    if (pc_setup(&pc, precond, nx, ny, nz, fdn, a, sigma, cv, diag, &dinv, &F)) {
+        free(dinv); free(F);
+        return -2;
+    }
+    double *res = (double *)malloc(sizeof(double) * n), *f = (double *)malloc(sizeof(double) * n);
+    double *x_old = (double *)malloc(sizeof(double) * n), *f_old = (double *)malloc(sizeof(double) * n);
+    double *DX = (double *)calloc((size_t)m * n, sizeof(double)), *DF = (double *)calloc((size_t)m * n, sizeof(double));
+    double *G = (double *)malloc(sizeof(double) * m * m), *g = (double *)malloc(sizeof(double) * m);
+    double *s = (double *)malloc(sizeof(double) * m);
+    int tlen = 0, status = 0;
+
+    const double b_2norm = orc_norm2(b, n);
+    tol *= b_2norm;
+    OPV(x, res);
+    for (long i = 0; i < n; i++) res[i] = b[i] + (-1.0) * res[i];      /* VecAYPX(res, -1, b) */
+    double r_2norm = tol + 1;
+    int iter = 1;
+    double t0 = now_seconds();
+    while (r_2norm > tol && iter <= max_iter) {
+        pc_apply(&pc, res, f, n, 0);
+        if (iter > 1) {
+            const int k = (iter - 2) % m;
+            for (long i = 0; i < n; i++) {
+                DX[(long)k * n + i] = (-1.0) * x_old[i] + x[i];
+                DF[(long)k * n + i] = (-1.0) * f_old[i] + f[i];
+            }
+        }
+        memcpy(x_old, x, sizeof(double) * n);
+        memcpy(f_old, f, sizeof(double) * n);
+        if (iter % p) {
+            for (long i = 0; i < n; i++) x[i] = x[i] + omega * f[i];
+        } else {
+            for (int i = 0; i < m; i++) {
+                for (int j = 0; j <= i; j++) {
+                    const double v = orc_dot(DF + (long)i * n, DF + (long)j * n, n);
+                    G[m * i + j] = v;
+                    G[i + m * j] = v;
+                }
+                g[i] = orc_dot(DF + (long)i * n, f, n);
+            }
+            if (orc_pinv_times_vec(G, g, s, m, NULL, NULL)) status = 1;
+            for (int q = 0; q < m; q++)
+                for (long i = 0; i < n; i++)
+                    x[i] = x[i] + (-s[q]) * (beta * DF[(long)q * n + i] + DX[(long)q * n + i]);
+            for (long i = 0; i < n; i++) x[i] = x[i] + beta * f[i];
+        }
+        OPV(x, res);
+        for (long i = 0; i < n; i++) res[i] = b[i] + (-1.0) * res[i];
+        if (iter % p == 0) {
+            r_2norm = orc_norm2(res, n);
+            if (trace && tlen < trace_cap) trace[tlen++] = r_2norm;
+        }
+        iter++;
+    }
+    double t1 = now_seconds();
+    finish_stats(st, iter - 1, r_2norm, b_2norm, tol, iter, max_iter, tlen, t1 - t0, status);
+    free(dinv); free(F); free(res); free(f); free(x_old); free(f_old); free(DX); free(DF); free(G); free(g); free(s);
+    return 0;
+}

@@ -165,6 +165,14 @@ void orc_matvec_complex(int nx, int ny, int nz, int fdn, const double *coeff,
 #define ORC_ILU_SLOTS 37
 int orc_ilu0_factor(int nx, int ny, int nz, int fdn, double a, double sigma, const double *c, double *F);
 void orc_ilu0_apply(int nx, int ny, int nz, const double *F, const double *in, double *out, int trans);
+/* AAR of the PETSc variant, PETSC_REAL/AAR_Real.c:32-199 (its pc = 0/1:
+ * precond 1 Jacobi by the reciprocal diagonal, 2 ILU(0) standing in for
+ * PCICC, 0 none), with the tools.c pseudo-inverse standing in for
+ * dgelsd.  trace: ||b - A x|| at every check (iter % p == 0); stats as
+ * orc_aar (iterations = iter - 1). */
+int orc_aar_petsc(int nx, int ny, int nz, int fdn, double a, double sigma, const double *c,
+                  double *x, const double *b, double omega, double beta, int m, int p, int max_iter,
+                  double tol, int precond, double *trace, int trace_cap, orc_stats *st);
 int orc_gmres(int nx, int ny, int nz, int fdn, double a, double sigma, const double *c,
               double *x, const double *b, int restart, int max_it, double rtol, int precond,
               double *trace, int trace_cap, orc_stats *st);

@@ -94,10 +94,12 @@ def lib():
         L.orc_gmres.argtypes = [I, I, I, I, Dd, Dd, _D, _D, _D, I, I, Dd, I, _D, I, ctypes.POINTER(OrcStats)]
         L.orc_bicg.argtypes = [I, I, I, I, Dd, Dd, _D, _D, _D, I, Dd, I, _D, I, ctypes.POINTER(OrcStats), _D]
         L.orc_ilu0_factor.argtypes = [I, I, I, I, Dd, Dd, _D, _D]
+        L.orc_aar_petsc.argtypes = [I, I, I, I, Dd, Dd, _D, _D, _D, Dd, Dd, I, I, I, Dd, I, _D, I,
+                                    ctypes.POINTER(OrcStats)]
         L.orc_ilu0_apply.argtypes = [I, I, I, _D, _D, _D, I]
         L.orc_ilu0_apply.restype = None
         for fn in ("orc_aar", "orc_pgr", "orc_cg", "orc_aar_complex", "orc_aar_vc", "orc_pgr_vc", "orc_cg_vc",
-                   "orc_gmres", "orc_bicg", "orc_ilu0_factor"):
+                   "orc_gmres", "orc_bicg", "orc_ilu0_factor", "orc_aar_petsc"):
             getattr(L, fn).restype = I
         _lib = L
     return _lib
@@ -285,6 +287,19 @@ def gmres(x0, rhs, shape, a, max_iter, tol, restart=30, sigma=0.0, c=None, preco
     cap = max_iter + max_iter // max(1, restart) + 8
     d = _run(lib().orc_gmres, (nx, ny, nz, fdn, a, sigma, _p(cv), _p(x), _p(b), restart, max_iter, tol,
                                int(precond)), trace, cap)
+    return x, d
+
+
+def aar_petsc(x0, rhs, shape, a, omega, beta, m, p, max_iter, tol, sigma=0.0, c=None, precond=2, fdn=6,
+              trace=True):
+    """AAR of the PETSc variant (PETSC_REAL/AAR_Real.c:32-199): precond 1
+    Jacobi, 2 ILU(0) for its pc = 1 (PCICC), 0 none."""
+    nx, ny, nz = shape
+    x = _f64(x0).reshape(-1).copy()
+    b = _f64(rhs).reshape(-1)
+    cv = None if c is None else _f64(c).reshape(-1)
+    d = _run(lib().orc_aar_petsc, (nx, ny, nz, fdn, a, sigma, _p(cv), _p(x), _p(b), omega, beta, m, p, max_iter,
+                                   tol, int(precond)), trace, max_iter // max(1, p) + 8)
     return x, d
 
 

@@ -350,3 +350,56 @@ def test_bicg_ilu0_is_pcg_with_the_ilu_preconditioner(oracle):
     np.testing.assert_allclose(st["trace"], norms, rtol=1e-5)
     ref = fft_solve(b, shape, a, oracle.lap_coefficient(6), sigma)
     assert np.abs(x - ref).max() <= 1e-6 * np.abs(ref).max()
+
+
+# ---------------------------------------------------------------------------
+# AAR of the PETSc variant (AAR_Real.c) with Jacobi or ILU(0)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("m,p", [(7, 6), (5, 4), (3, 3), (4, 2)])
+def test_aar_petsc_jacobi_matches_the_reference_pinned_aar(oracle, m, p):
+    """With p >= 2 the PETSc variant's AAR (Jacobi) runs the control flow of
+    MPI_REAL/AAR.c, which the golden vectors pin: same iteration count,
+    solutions equal to rounding (reciprocal vs division, update order).
+    (m = 3, p = 2 is left out: its Gram is ill-conditioned enough that the
+    rounding differences grow ~10x every few checks and move the count.)"""
+    shape = (16, 16, 16)
+    a, sigma = -4.0, 0.0
+    b = rhs(shape, 31)
+    x1, d1 = oracle.aar_petsc(np.ones(b.size), b, shape, a, 0.6, 0.6, m, p, 1000, 1e-8, precond=1)
+    x2, d2 = oracle.aar(np.ones(b.size), b, shape, a, 0.6, 0.6, m, p, 1000, 1e-8)
+    assert d1["iterations"] == d2["iterations"] and d1["converged"]
+    assert np.abs(x1 - x2).max() <= 1e-10 * np.abs(x2).max()
+    np.testing.assert_allclose(d1["trace"], d2["trace"][p::p][:len(d1["trace"])], rtol=1e-6,
+                               atol=1e-10 * d2["trace"][0])
+
+
+def test_aar_petsc_first_steps(oracle):
+    """p = 1: the Anderson step at iter = 1 has an empty history, so x1 =
+    x0 + beta B r0 (AAR_Real.c:116 has no iter > 1 guard); pure Richardson
+    (p > max_iter) is x_{k+1} = x_k + omega B (b - A x_k) with B = M^{-1}."""
+    shape = (13, 13, 13)
+    a, sigma = -4.0, 0.5
+    b = rhs(shape, 32)
+    A = dense_operator(oracle, shape, a, sigma)
+    F = oracle.ilu0_factor(shape, a, sigma=sigma)
+    Minv = lambda v: oracle.ilu0_apply(shape, F, v)  # noqa: E731
+    x0 = np.ones(b.size)
+    x1, _ = oracle.aar_petsc(x0, b, shape, a, 0.6, 0.9, 4, 1, 1, 1e-14, sigma=sigma, precond=2)
+    np.testing.assert_allclose(x1, x0 + 0.9 * Minv(b - A @ x0), rtol=1e-12, atol=1e-14)
+    xr, _ = oracle.aar_petsc(x0, b, shape, a, 0.7, 0.6, 4, 100, 5, 1e-14, sigma=sigma, precond=2)
+    xx = x0.copy()
+    for _ in range(5):
+        xx = xx + 0.7 * Minv(b - A @ xx)
+    np.testing.assert_allclose(xr, xx, rtol=1e-11, atol=1e-13)
+
+
+def test_aar_petsc_ilu0_converges_to_fft_solution(oracle):
+    shape = (16, 16, 16)
+    a, sigma = -4.0, 0.5
+    b = rhs(shape, 33)
+    ref = fft_solve(b, shape, a, oracle.lap_coefficient(6), sigma)
+    x, d = oracle.aar_petsc(np.ones(b.size), b, shape, a, 0.6, 0.6, 7, 6, 1000, 1e-10, sigma=sigma, precond=2)
+    assert d["converged"] and d["iterations"] % 6 == 0
+    assert np.abs(x - ref).max() <= 1e-7 * np.abs(ref).max()
+    _, dj = oracle.aar_petsc(np.ones(b.size), b, shape, a, 0.6, 0.6, 7, 6, 1000, 1e-10, sigma=sigma, precond=1)
+    assert d["iterations"] < dj["iterations"]
