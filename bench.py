@@ -50,6 +50,7 @@ def parse():
     ap.add_argument("--no-helm", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-vc", action="store_true")
+    ap.add_argument("--no-petsc", action="store_true")
     return ap.parse_args()
 
 
@@ -203,6 +204,45 @@ def varcoef_case(dev, shape, a, b, tol):
         log(f"varcoef {label}: {st['iterations']} its {dt * 1e3:.0f} ms relres {rel:.2e}")
     out["mixed_speedup_vs_fp64"] = out["fp64"]["seconds"] / out["mixed"]["seconds"]
     del c, x
+    return out
+
+
+def petsc_variant_case(dev, tol, n=128):
+    """The PETSc variant's solvers (PETSC_REAL/main.c with pc = 0 and 1) on
+    the 128^3 Poisson system (h = 0.25, the same seeded RHS scheme): AAR
+    with Jacobi and with ILU(0) (its PCICC), GMRES(30) and BiCG with ILU(0)
+    (its PCBJACOBI/PCILU), max_it 2000 for the Krylov methods.  The
+    level-scheduled ILU(0) sweeps are launch-latency bound (n+n+n-2 levels
+    per triangular solve)."""
+    import torch
+    import aar_hip
+    from aar_hip import problems as P
+    shape = (n, n, n)
+    a = P.poisson_a(0.25)
+    b = P.rhs_uniform_torch(shape, 1002, device=dev)
+    x = torch.zeros_like(b)
+    out = {"grid": list(shape)}
+    runs = (("aar_jacobi", lambda: aar_hip.AAR_petsc(x, b, shape, a, 0.6, 0.6, 7, 6, 100000, tol, precond=True)),
+            ("aar_ilu0", lambda: aar_hip.AAR_petsc(x, b, shape, a, 0.6, 0.6, 7, 6, 100000, tol,
+                                                   precond=aar_hip.ILU0)),
+            ("gmres30_ilu0", lambda: aar_hip.GMRES(x, b, shape, a, 2000, tol, precond=aar_hip.ILU0)),
+            ("bicg_ilu0", lambda: aar_hip.BiCG(x, b, shape, a, 2000, tol, precond=aar_hip.ILU0)))
+    for name, fn in runs:
+        x.zero_()
+        st = fn()
+        out[name] = {"iterations": st["iterations"], "converged": bool(st["converged"]), "relres": st["relres"],
+                     "solve_ms": st["solve_ms"]}
+        log(f"petsc variant {name}: {st['iterations']} its {st['solve_ms']:.0f} ms")
+    # one ILU(0) factor + apply, timed on the device
+    r = torch.randn_like(b)
+    aar_hip.ilu0_apply(r, shape, a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    aar_hip.ilu0_apply(r, shape, a)
+    e1.record()
+    torch.cuda.synchronize()
+    out["ilu0_factor_apply_ms"] = e0.elapsed_time(e1)
     return out
 
 
@@ -412,6 +452,13 @@ def main():
         log(f"{name}: {st['iterations']} its {st['solve_ms']:.0f} ms")
 
     # the variable-coefficient operator on the same grid and RHS
+    petsc = None
+    if not args.no_petsc:
+        try:
+            petsc = petsc_variant_case(dev, args.tol)
+        except Exception as e:  # noqa: BLE001
+            petsc = {"error": str(e)}
+
     vc = None
     if not args.no_vc:
         try:
@@ -519,6 +566,7 @@ def main():
         },
         "other_solvers": others,
         "variable_coefficient": vc,
+        "petsc_variant": petsc,
         "helmholtz": helm,
         "batch": batch,
         "matvec": {"GBps": mv_gbs, "frac_of_peak": mv_gbs / HBM_PEAK_GBS,
