@@ -174,6 +174,14 @@ int aar_hip_ilu0_apply(const aar_hip_grid *g, const double *c, const double *r, 
 int aar_hip_aar_petsc(const aar_hip_grid *g, const double *c, double *x, const double *b, double omega,
                       double beta, int m, int p, int max_iter, double tol, int precond, aar_hip_stats *st);
 
+/* PETSC_REAL/PGR_Real.c:32-195 (l2 = 0) and PL2R_Real.c:32-197 (l2 = 1) --
+ * the variant's PGR/PL2R with the same preconditioner choice (PCICC for
+ * pc = 1, PGR_Real.c:55-62), the DX^T DF (DF^T DF) system solved by the
+ * tools.c pseudo-inverse in place of dgelsd (whose column-major call takes
+ * the transpose, equal in exact arithmetic for these symmetric operators). */
+int aar_hip_pgr_petsc(const aar_hip_grid *g, const double *c, double *x, const double *b, double omega, int m,
+                      int p, int max_iter, double tol, int precond, int l2, aar_hip_stats *st);
+
 /* Extension (no reference counterpart): FP64 residual refinement around
  * FP32 AAR.  Outer: r = b - A x in FP64, stop at ||r|| <= tol*||b||;
  * inner: FP32 AAR on A e = r from e = 0 to relative tolerance

@@ -263,6 +263,24 @@ def AAR_petsc(x, b, shape, a, omega, beta, m, p, max_iter, tol, sigma=0.0, preco
     return st.as_dict()
 
 
+def PGR_petsc(x, b, shape, a, omega, m, p, max_iter, tol, sigma=0.0, precond=ILU0, c=None, l2=False):
+    """The PETSc variant's PGR (l2=False, PGR_Real.c) / PL2R (l2=True,
+    PL2R_Real.c); precond as AAR_petsc."""
+    import torch
+    _check_len(x, shape, "x")
+    _check_len(b, shape, "b")
+    g = _grid(shape, a, sigma)
+    st = Stats()
+    _bind_stream()
+    cp = None
+    if c is not None:
+        _check_len(c, shape, "c")
+        cp = _dev(c, torch.float64, "c")
+    check(_L().aar_hip_pgr_petsc(ctypes.byref(g), cp, _dev(x, torch.float64, "x"), _dev(b, torch.float64, "b"),
+                                 omega, m, p, max_iter, tol, _pc(precond), int(bool(l2)), ctypes.byref(st)))
+    return st.as_dict()
+
+
 def ilu0_apply(r, shape, a, sigma=0.0, c=None, trans=False):
     """z = M^{-1} r (trans: M^{-T} r) for the ILU(0) factors of a*Lap +
     sigma (+ c): the PETSc variant's pc = 1 preconditioner on one rank."""

@@ -122,6 +122,8 @@ int bicg_solve(const Op &op, double *x, const double *b, int max_it, double tol,
 int ilu0_apply(const Op &op, const double *r, double *z, int trans);
 int aar_petsc_solve(const Op &op, double *x, const double *b, double omega, double beta, int m, int p,
                     int max_iter, double tol, int precond, aar_hip_stats *st);
+int pgr_petsc_solve(const Op &op, double *x, const double *b, double omega, int m, int p, int max_iter, double tol,
+                    int precond, int l2, aar_hip_stats *st);
 template <typename T>
 int run_stencil(SMode mode, SArgs<T> a, double bytes);
 template <typename T>

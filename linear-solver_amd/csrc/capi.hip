@@ -829,6 +829,14 @@ int aar_hip_aar_petsc(const aar_hip_grid *g, const double *c, double *x, const d
     return aar_petsc_solve(op, x, b, omega, beta, m, p, max_iter, tol, precond, st);
 }
 
+int aar_hip_pgr_petsc(const aar_hip_grid *g, const double *c, double *x, const double *b, double omega, int m, int p,
+                      int max_iter, double tol, int precond, int l2, aar_hip_stats *st)
+{
+    Op op;
+    if (int rc = c ? make_op_vc(g, c, op) : make_op(g, nullptr, op)) return rc;
+    return pgr_petsc_solve(op, x, b, omega, m, p, max_iter, tol, precond, l2, st);
+}
+
 int aar_hip_ilu0_apply(const aar_hip_grid *g, const double *c, const double *r, double *z, int trans)
 {
     Op op;

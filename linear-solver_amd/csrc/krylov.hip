@@ -504,6 +504,75 @@ __global__ __launch_bounds__(KB) void k_aa_check_kernel(KCtrl *k, const double *
     }
 }
 
+// --- PGR / PL2R of the PETSc variant (PGR_Real.c, PL2R_Real.c) ---
+// x_old = x, Ax_prev = Ax; x += omega f; DX_k = x - x_old
+__global__ __launch_bounds__(KB) void k_pg_rich_kernel(long n, double *x, const double *f, double *x_old,
+                                                      const double *Ax, double *Ax_prev, double *dx, double omega,
+                                                      const KCtrl *k)
+{
+    if (k->c.done) return;
+    KLOOP(i)
+    {
+        const double xo = x[i];
+        x_old[i] = xo;
+        Ax_prev[i] = Ax[i];
+        const double xn = xo + omega * f[i];
+        x[i] = xn;
+        dx[i] = (-1.0) * xo + xn;
+    }
+}
+
+// res = b - Ax; DF_k = Ax - Ax_prev; ||res||^2 when checked (no Galerkin step)
+__global__ __launch_bounds__(KB) void k_pg_res_kernel(long n, const double *b, const double *Ax, const double *Ax_prev,
+                                                     double *res, double *df, int check, double *part, const KCtrl *k)
+{
+    if (k->c.done) return;
+    double acc[1] = {0.0};
+    KLOOP(i)
+    {
+        const double ax = Ax[i];
+        const double r = (-1.0) * ax + b[i];
+        res[i] = r;
+        df[i] = (-1.0) * Ax_prev[i] + ax;
+        acc[0] += r * r;
+    }
+    if (check) k_block_sums<1>(acc, 1, part);
+}
+
+// the Galerkin / L2 step: x += sum s_q DX_q; DX_k = x - x_old; Ax += sum
+// s_q DF_q; DF_k = Ax - Ax_prev; res = b - Ax with ||res||^2
+template <int MM>
+__global__ __launch_bounds__(KB) void k_pg_upd_kernel(long n, double *x, const double *x_old, double *Ax,
+                                                     const double *Ax_prev, double *res, const double *b,
+                                                     double *X, double *Y, long ld, int m, int kk, double *part,
+                                                     const KCtrl *k)
+{
+    if (k->c.done) return;
+    double s[MM];
+#pragma unroll
+    for (int q = 0; q < MM; q++) s[q] = q < m ? k->c.s[q] : 0.0;
+    double acc[1] = {0.0};
+    KLOOP(i)
+    {
+        double xi = x[i];
+#pragma unroll
+        for (int q = 0; q < MM; q++)
+            if (q < m) xi = xi + s[q] * X[(size_t)(1 + q) * ld + i];
+        x[i] = xi;
+        X[(size_t)(1 + kk) * ld + i] = (-1.0) * x_old[i] + xi;
+        double ax = Ax[i];
+#pragma unroll
+        for (int q = 0; q < MM; q++)
+            if (q < m) ax = ax + s[q] * Y[(size_t)(1 + q) * ld + i];
+        Ax[i] = ax;
+        Y[(size_t)(1 + kk) * ld + i] = (-1.0) * Ax_prev[i] + ax;
+        const double r = (-1.0) * ax + b[i];
+        res[i] = r;
+        acc[0] += r * r;
+    }
+    k_block_sums<1>(acc, 1, part);
+}
+
 // ---------------------------------------------------------------------------
 // control kernels (one block; thread 0 runs the scalar recurrences)
 // ---------------------------------------------------------------------------
@@ -1228,6 +1297,139 @@ int aar_petsc_solve(const Op &op, double *x, const double *b, double omega, doub
     C.hist_host.assign(std::max(hl, 0), 0.0);
     if (hl > 0) HC(hipMemcpy(C.hist_host.data(), R.hist, sizeof(double) * hl, hipMemcpyDeviceToHost));
     // AAR_Real.c:162-166: iter - 1 iterations (iter stops one past the last)
+    const int iterations = hk.c.done ? hk.c.final_iter : max_iter;
+    aar_hip_stats sres;
+    std::memset(&sres, 0, sizeof(sres));
+    sres.iterations = iterations;
+    sres.rhs_norm = hk.rhs_norm;
+    sres.relres = hk.rhs_norm > 0 ? hk.c.relres / hk.rhs_norm : hk.c.relres;
+    sres.converged = hk.c.relres <= hk.c.tol_abs;
+    sres.printed_converged = iterations + 1 < max_iter;
+    sres.printed_warning = !(iterations + 1 < max_iter);
+    sres.status = hk.c.status == 1 ? AAR_HIP_ERR_SVD : 0;
+    sres.hist_len = hl;
+    sres.solve_ms = ms;
+    sres.bytes = C.bytes_acc;
+    C.last = sres;
+    if (st) *st = sres;
+    if (hk.c.status == 1) return fail(AAR_HIP_ERR_SVD, "no convergence in 250 svd iterations");
+    return 0;
+}
+
+// PGR (l2 = 0) / PL2R (l2 = 1) of the PETSc variant, PGR_Real.c:32-195 /
+// PL2R_Real.c:32-197, preconditioner as aar_petsc_solve.  The normal
+// equations come from the Gram kernel (kind 1: DX^T DF, kind 2: DF^T DF,
+// with res = b - Ax formed in its loads) and the tools.c pseudo-inverse.
+int pgr_petsc_solve(const Op &op, double *x, const double *b, double omega, int m, int p, int max_iter, double tol,
+                    int precond, int l2, aar_hip_stats *st)
+{
+    Context &C = ctx();
+    if (int rc = context_init()) return rc;
+    if (!x || !b) return fail(AAR_HIP_ERR_ARG, "null vector");
+    if (max_iter < 0) return fail(AAR_HIP_ERR_ARG, "max_iter must be >= 0");
+    if (m < 1 || m > MAXM) return fail(AAR_HIP_ERR_ARG, "history m must be in 1..16");
+    if (p < 1) return fail(AAR_HIP_ERR_ARG, "p must be >= 1");
+    if (precond < 0 || precond > 2) return fail(AAR_HIP_ERR_ARG, "precond must be 0 (none), 1 (Jacobi) or 2 (ILU(0))");
+    const bool ilu = precond == 2;
+    if (ilu)
+        if (int rc = check_ilu_grid(op)) return rc;
+    const long n = op.n;
+    const long ld = ring_ld(n, sizeof(double));
+    // Y: slot 0 Ax, 1..m DF;  X: slot 0 f, 1..m DX;  then x_old, Ax_prev, res
+    HC(ensure(C.kvec, sizeof(double) * (size_t)ld * (2 * m + 5)));
+    double *Y = (double *)C.kvec.p, *X = Y + (size_t)(m + 1) * ld;
+    double *Ax = Y, *f = X;
+    double *x_old = X + (size_t)(m + 1) * ld, *Ax_prev = x_old + ld, *res = Ax_prev + ld;
+    double *F = nullptr;
+    if (ilu) {
+        HC(ensure(C.kilu, sizeof(double) * (size_t)n * IFS));
+        F = (double *)C.kilu.p;
+    }
+    const int gnb = nblocks_for(n);
+    HC(ensure(C.part0, sizeof(double) * (size_t)gnb * GRAM_NA_MAX));
+    const int hcap = max_iter / p + 8;
+    C.bytes_acc = 0.0;
+    HC(hipEventRecord(C.t0, C.stream));
+    KrylovRun R;
+    if (int rc = k_setup(op, precond, tol, max_iter, 0, hcap, b, R, F, res)) return rc;
+    // history columns start at zero
+    hipLaunchKernelGGL(k_zero_kernel, dim3(1024), dim3(KB), 0, C.stream, (unsigned *)(Y + ld), (long)(m * ld * 2));
+    hipLaunchKernelGGL(k_zero_kernel, dim3(1024), dim3(KB), 0, C.stream, (unsigned *)(X + ld), (long)(m * ld * 2));
+    hipLaunchKernelGGL(k_aa_init_kernel, dim3(1), dim3(1), 0, C.stream, R.k, tol);
+    const double s = 8.0 * n, ib = ilu ? ILU_APPLY_BYTES * n : 0.0;
+    const double cb = op.c ? s : 0.0;
+    const int *done = &R.k->c.done;
+    if (int rc = k_matvec(op, x, Ax, nullptr)) return rc;
+    HC(hipMemcpyAsync(res, Ax, sizeof(double) * n, hipMemcpyDeviceToDevice, C.stream));
+    hipLaunchKernelGGL(k_aa_res_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, b, res, 0, R.part,
+                       (const int *)&R.k->zero);
+    HC(hipGetLastError());
+    C.bytes_acc += 6 * s + cb;
+    bool stopped = false;
+    int cyc = 0;
+    for (int iter = 1; iter <= max_iter && !stopped; iter++) {
+        if (ilu) {
+            prof_begin("krylov_ilu_apply");
+            HC(ilu_apply(R.ilu, res, f, false, done, C.stream));
+            prof_end(ib);
+        } else {
+            hipLaunchKernelGGL(k_pcj_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, (const double *)res, f, R.dg,
+                               done);
+        }
+        const int kk = (iter - 2 + m) % m;
+        const bool gal = (iter % p) == 0;
+        hipLaunchKernelGGL(k_pg_rich_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, x, (const double *)f, x_old,
+                           (const double *)Ax, Ax_prev, X + (size_t)(1 + kk) * ld, omega, (const KCtrl *)R.k);
+        if (int rc = k_matvec(op, x, Ax, &R.k->c)) return rc;
+        hipLaunchKernelGGL(k_pg_res_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, b, (const double *)Ax,
+                           (const double *)Ax_prev, res, Y + (size_t)(1 + kk) * ld, 0, R.part, (const KCtrl *)R.k);
+        C.bytes_acc += (ilu ? ib : 2 * s + cb) + 7 * s + 2 * s + cb + 5 * s;
+        if (gal) {
+            GramArgs<double> g;
+            g.n = n;
+            g.m = m;
+            g.kind = l2 ? 2 : 1;
+            g.direct = 1;
+            g.F = X;
+            g.Y = Y;
+            g.b = b;
+            g.ld = ld;
+            g.nvalid = m;
+            for (int j = 0; j < m; j++) g.slot[j] = 1 + j;
+            g.fslot = 0;
+            g.nblocks = gnb;
+            g.partials = R.part;
+            g.counter = nullptr;
+            g.ctrl = &R.k->c;
+            HC(launch_gram<double>(g, C.stream));
+            HC(launch_ls(&R.k->c, m, 0, C.stream));
+            if (m <= 8)
+                hipLaunchKernelGGL((k_pg_upd_kernel<8>), dim3(R.nb), dim3(KB), 0, C.stream, n, x,
+                                   (const double *)x_old, Ax, (const double *)Ax_prev, res, b, X, Y, ld, m, kk, R.part,
+                                   (const KCtrl *)R.k);
+            else
+                hipLaunchKernelGGL((k_pg_upd_kernel<16>), dim3(R.nb), dim3(KB), 0, C.stream, n, x,
+                                   (const double *)x_old, Ax, (const double *)Ax_prev, res, b, X, Y, ld, m, kk, R.part,
+                                   (const KCtrl *)R.k);
+            hipLaunchKernelGGL(k_aa_check_kernel, dim3(1), dim3(KB), 0, C.stream, R.k, (const double *)R.part, R.nb,
+                               R.hist, iter);
+            HC(hipGetLastError());
+            C.bytes_acc += ((l2 ? 1.0 : 2.0) * m + 2) * s + (2.0 * m + 9) * s;
+            if (int rc = poll_record(&R.k->c, cyc)) return rc;
+            if (poll_check(cyc - 2)) stopped = true;
+            cyc++;
+        }
+    }
+    HC(hipEventRecord(C.t1, C.stream));
+    KCtrl hk;
+    HC(hipMemcpyAsync(&hk, R.k, sizeof(KCtrl), hipMemcpyDeviceToHost, C.stream));
+    HC(hipStreamSynchronize(C.stream));
+    float ms = 0.f;
+    HC(hipEventElapsedTime(&ms, C.t0, C.t1));
+    prof_flush();
+    const int hl = std::min(hk.nh, R.hcap);
+    C.hist_host.assign(std::max(hl, 0), 0.0);
+    if (hl > 0) HC(hipMemcpy(C.hist_host.data(), R.hist, sizeof(double) * hl, hipMemcpyDeviceToHost));
     const int iterations = hk.c.done ? hk.c.final_iter : max_iter;
     aar_hip_stats sres;
     std::memset(&sres, 0, sizeof(sres));

@@ -1420,3 +1420,91 @@ int orc_aar_petsc(int nx, int ny, int nz, int fdn, double a, double sigma, const
     free(dinv); free(F); free(res); free(f); free(x_old); free(f_old); free(DX); free(DF); free(G); free(g); free(s);
     return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* PGR / PL2R of the PETSc variant, PETSC_REAL/PGR_Real.c:32-195 and   */
+/* PL2R_Real.c:32-197 (l2 = 1), with the preconditioner choice of      */
+/* PGR_Real.c:55-62 (precond as orc_aar_petsc).  The loop of           */
+/* PGR_Real.c:98-161: f = B res; x_old = x, Ax_prev = Ax; k = (iter-2  */
+/* + m) % m; x += omega f; DX_k = x - x_old; Ax = A x; res = b - Ax;    */
+/* DF_k = Ax - Ax_prev; every p: s from the m x m system, x += sum     */
+/* s_i DX_i, DX_k = x - x_old, Ax += sum s_i DF_i, DF_k = Ax - Ax_prev;*/
+/* res = b - Ax, its norm checked every p.  The m x m system is        */
+/* A[i][j] = DX_i . DF_j (PGR; PL2R: DF_i . DF_j), rhs DX_i . res (DF_i */
+/* . res), solved by the tools.c pseudo-inverse in place of dgelsd.    */
+/* The variant's column-major dgelsd call receives the transpose of    */
+/* PGR's matrix; for these symmetric operators DF = A DX makes the two */
+/* equal in exact arithmetic.                                          */
+/* ------------------------------------------------------------------ */
+int orc_pgr_petsc(int nx, int ny, int nz, int fdn, double a, double sigma, const double *cv,
+                  double *x, const double *b, double omega, int m, int p, int max_iter, double tol,
+                  int precond, int l2, double *trace, int trace_cap, orc_stats *st)
+{
+    if (m < 1 || m > 32 || p < 1) return -1;
+    const long n = (long)nx * ny * nz;
+    double coeff[32] = {0};
+    orc_lap_coefficient(coeff, fdn);
+    double diag = a * (3 * coeff[0]);
+    if (sigma != 0.0) diag += sigma;
+    double *dinv, *F;
+    ksp_pc pc;
+    if (pc_setup(&pc, precond, nx, ny, nz, fdn, a, sigma, cv, diag, &dinv, &F)) {
+        free(dinv); free(F);
+        return -2;
+    }
+    double *res = (double *)malloc(sizeof(double) * n), *f = (double *)malloc(sizeof(double) * n);
+    double *Ax = (double *)malloc(sizeof(double) * n), *Ax_prev = (double *)malloc(sizeof(double) * n);
+    double *x_old = (double *)malloc(sizeof(double) * n);
+    double *DX = (double *)calloc((size_t)m * n, sizeof(double)), *DF = (double *)calloc((size_t)m * n, sizeof(double));
+    double *G = (double *)malloc(sizeof(double) * m * m), *g = (double *)malloc(sizeof(double) * m);
+    double *s = (double *)malloc(sizeof(double) * m);
+    int tlen = 0, status = 0;
+    const double *L = l2 ? DF : DX;           /* the left factor of the normal equations */
+
+    const double b_2norm = orc_norm2(b, n);
+    tol *= b_2norm;
+    OPV(x, Ax);
+    for (long i = 0; i < n; i++) res[i] = (-1.0) * Ax[i] + b[i];        /* VecWAXPY(res, -1, Ax, b) */
+    double r_2norm = tol + 1;
+    int iter = 1;
+    double t0 = now_seconds();
+    while (r_2norm > tol && iter <= max_iter) {
+        pc_apply(&pc, res, f, n, 0);
+        memcpy(x_old, x, sizeof(double) * n);
+        memcpy(Ax_prev, Ax, sizeof(double) * n);
+        const int k = (iter - 2 + m) % m;
+        for (long i = 0; i < n; i++) {
+            x[i] = x[i] + omega * f[i];
+            DX[(long)k * n + i] = (-1.0) * x_old[i] + x[i];
+        }
+        OPV(x, Ax);
+        for (long i = 0; i < n; i++) {
+            res[i] = (-1.0) * Ax[i] + b[i];
+            DF[(long)k * n + i] = (-1.0) * Ax_prev[i] + Ax[i];
+        }
+        if (iter % p == 0) {
+            for (int i = 0; i < m; i++) {
+                for (int j = 0; j < m; j++) G[i * m + j] = orc_dot(L + (long)i * n, DF + (long)j * n, n);
+                g[i] = orc_dot(L + (long)i * n, res, n);
+            }
+            if (orc_pinv_times_vec(G, g, s, m, NULL, NULL)) status = 1;
+            for (int q = 0; q < m; q++)
+                for (long i = 0; i < n; i++) x[i] = x[i] + s[q] * DX[(long)q * n + i];
+            for (long i = 0; i < n; i++) DX[(long)k * n + i] = (-1.0) * x_old[i] + x[i];
+            for (int q = 0; q < m; q++)
+                for (long i = 0; i < n; i++) Ax[i] = Ax[i] + s[q] * DF[(long)q * n + i];
+            for (long i = 0; i < n; i++) DF[(long)k * n + i] = (-1.0) * Ax_prev[i] + Ax[i];
+        }
+        for (long i = 0; i < n; i++) res[i] = (-1.0) * Ax[i] + b[i];
+        if (iter % p == 0) {
+            r_2norm = orc_norm2(res, n);
+            if (trace && tlen < trace_cap) trace[tlen++] = r_2norm;
+        }
+        iter++;
+    }
+    double t1 = now_seconds();
+    finish_stats(st, iter - 1, r_2norm, b_2norm, tol, iter, max_iter, tlen, t1 - t0, status);
+    free(dinv); free(F); free(res); free(f); free(Ax); free(Ax_prev); free(x_old); free(DX); free(DF);
+    free(G); free(g); free(s);
+    return 0;
+}

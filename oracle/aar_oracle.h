@@ -173,6 +173,11 @@ void orc_ilu0_apply(int nx, int ny, int nz, const double *F, const double *in, d
 int orc_aar_petsc(int nx, int ny, int nz, int fdn, double a, double sigma, const double *c,
                   double *x, const double *b, double omega, double beta, int m, int p, int max_iter,
                   double tol, int precond, double *trace, int trace_cap, orc_stats *st);
+/* PGR (l2 = 0) / PL2R (l2 = 1) of the PETSc variant, PGR_Real.c:32-195 /
+ * PL2R_Real.c:32-197, preconditioner and least squares as orc_aar_petsc. */
+int orc_pgr_petsc(int nx, int ny, int nz, int fdn, double a, double sigma, const double *c,
+                  double *x, const double *b, double omega, int m, int p, int max_iter, double tol,
+                  int precond, int l2, double *trace, int trace_cap, orc_stats *st);
 int orc_gmres(int nx, int ny, int nz, int fdn, double a, double sigma, const double *c,
               double *x, const double *b, int restart, int max_it, double rtol, int precond,
               double *trace, int trace_cap, orc_stats *st);

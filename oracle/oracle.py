@@ -96,10 +96,12 @@ def lib():
         L.orc_ilu0_factor.argtypes = [I, I, I, I, Dd, Dd, _D, _D]
         L.orc_aar_petsc.argtypes = [I, I, I, I, Dd, Dd, _D, _D, _D, Dd, Dd, I, I, I, Dd, I, _D, I,
                                     ctypes.POINTER(OrcStats)]
+        L.orc_pgr_petsc.argtypes = [I, I, I, I, Dd, Dd, _D, _D, _D, Dd, I, I, I, Dd, I, I, _D, I,
+                                    ctypes.POINTER(OrcStats)]
         L.orc_ilu0_apply.argtypes = [I, I, I, _D, _D, _D, I]
         L.orc_ilu0_apply.restype = None
         for fn in ("orc_aar", "orc_pgr", "orc_cg", "orc_aar_complex", "orc_aar_vc", "orc_pgr_vc", "orc_cg_vc",
-                   "orc_gmres", "orc_bicg", "orc_ilu0_factor", "orc_aar_petsc"):
+                   "orc_gmres", "orc_bicg", "orc_ilu0_factor", "orc_aar_petsc", "orc_pgr_petsc"):
             getattr(L, fn).restype = I
         _lib = L
     return _lib
@@ -300,6 +302,19 @@ def aar_petsc(x0, rhs, shape, a, omega, beta, m, p, max_iter, tol, sigma=0.0, c=
     cv = None if c is None else _f64(c).reshape(-1)
     d = _run(lib().orc_aar_petsc, (nx, ny, nz, fdn, a, sigma, _p(cv), _p(x), _p(b), omega, beta, m, p, max_iter,
                                    tol, int(precond)), trace, max_iter // max(1, p) + 8)
+    return x, d
+
+
+def pgr_petsc(x0, rhs, shape, a, omega, m, p, max_iter, tol, sigma=0.0, c=None, precond=2, l2=False, fdn=6,
+              trace=True):
+    """PGR (l2=False) / PL2R (l2=True) of the PETSc variant (PGR_Real.c,
+    PL2R_Real.c); precond as aar_petsc."""
+    nx, ny, nz = shape
+    x = _f64(x0).reshape(-1).copy()
+    b = _f64(rhs).reshape(-1)
+    cv = None if c is None else _f64(c).reshape(-1)
+    d = _run(lib().orc_pgr_petsc, (nx, ny, nz, fdn, a, sigma, _p(cv), _p(x), _p(b), omega, m, p, max_iter, tol,
+                                   int(precond), int(bool(l2))), trace, max_iter // max(1, p) + 8)
     return x, d
 
 

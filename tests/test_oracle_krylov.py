@@ -403,3 +403,35 @@ def test_aar_petsc_ilu0_converges_to_fft_solution(oracle):
     assert np.abs(x - ref).max() <= 1e-7 * np.abs(ref).max()
     _, dj = oracle.aar_petsc(np.ones(b.size), b, shape, a, 0.6, 0.6, 7, 6, 1000, 1e-10, sigma=sigma, precond=1)
     assert d["iterations"] < dj["iterations"]
+
+
+@pytest.mark.parametrize("l2", [False, True])
+@pytest.mark.parametrize("m,p", [(7, 6), (5, 4), (4, 3)])
+def test_pgr_petsc_jacobi_matches_the_reference_pinned_pgr(oracle, l2, m, p):
+    """The PETSc variant's PGR/PL2R with Jacobi runs the control flow of
+    MPI_REAL/PGR.c / PL2R.c (pinned by the golden vectors): same counts,
+    solutions equal to rounding."""
+    shape = (16, 16, 16)
+    a = -4.0
+    b = rhs(shape, 41)
+    x1, d1 = oracle.pgr_petsc(np.ones(b.size), b, shape, a, 0.6, m, p, 1000, 1e-8, precond=1, l2=l2)
+    x2, d2 = oracle.pgr(np.ones(b.size), b, shape, a, 0.6, m, p, 1000, 1e-8, l2=l2)
+    assert d1["iterations"] == d2["iterations"] and d1["converged"]
+    assert np.abs(x1 - x2).max() <= 1e-9 * np.abs(x2).max()
+
+
+@pytest.mark.parametrize("l2", [False, True])
+def test_pgr_petsc_ilu0_first_step_and_solution(oracle, l2):
+    """One iteration is x0 + omega B r0 (the history step alone, p > 1);
+    converged solves reach the FFT solution."""
+    shape = (13, 13, 13)
+    a, sigma = -4.0, 0.5
+    b = rhs(shape, 42)
+    A = dense_operator(oracle, shape, a, sigma)
+    F = oracle.ilu0_factor(shape, a, sigma=sigma)
+    x0 = np.ones(b.size)
+    x1, _ = oracle.pgr_petsc(x0, b, shape, a, 0.6, 4, 3, 1, 1e-14, sigma=sigma, precond=2, l2=l2)
+    np.testing.assert_allclose(x1, x0 + 0.6 * oracle.ilu0_apply(shape, F, b - A @ x0), rtol=1e-12, atol=1e-14)
+    x, d = oracle.pgr_petsc(x0, b, shape, a, 0.6, 7, 6, 1000, 1e-10, sigma=sigma, precond=2, l2=l2)
+    ref = fft_solve(b, shape, a, oracle.lap_coefficient(6), sigma)
+    assert d["converged"] and np.abs(x - ref).max() <= 1e-7 * np.abs(ref).max()
