@@ -210,7 +210,8 @@ def varcoef_case(dev, shape, a, b, tol):
 def petsc_variant_case(dev, tol, n=128):
     """The PETSc variant's solvers (PETSC_REAL/main.c with pc = 0 and 1) on
     the 128^3 Poisson system (h = 0.25, the same seeded RHS scheme): AAR
-    with Jacobi and with ILU(0) (its PCICC), GMRES(30) and BiCG with ILU(0)
+    with Jacobi and with ILU(0) (its PCICC), PGR and PL2R with ILU(0),
+    GMRES(30) and BiCG with ILU(0)
     (its PCBJACOBI/PCILU), max_it 2000 for the Krylov methods.  The
     level-scheduled ILU(0) sweeps are launch-latency bound (n+n+n-2 levels
     per triangular solve)."""
@@ -225,6 +226,9 @@ def petsc_variant_case(dev, tol, n=128):
     runs = (("aar_jacobi", lambda: aar_hip.AAR_petsc(x, b, shape, a, 0.6, 0.6, 7, 6, 100000, tol, precond=True)),
             ("aar_ilu0", lambda: aar_hip.AAR_petsc(x, b, shape, a, 0.6, 0.6, 7, 6, 100000, tol,
                                                    precond=aar_hip.ILU0)),
+            ("pgr_ilu0", lambda: aar_hip.PGR_petsc(x, b, shape, a, 0.6, 7, 6, 100000, tol, precond=aar_hip.ILU0)),
+            ("pl2r_ilu0", lambda: aar_hip.PGR_petsc(x, b, shape, a, 0.6, 7, 6, 100000, tol, precond=aar_hip.ILU0,
+                                                    l2=True)),
             ("gmres30_ilu0", lambda: aar_hip.GMRES(x, b, shape, a, 2000, tol, precond=aar_hip.ILU0)),
             ("bicg_ilu0", lambda: aar_hip.BiCG(x, b, shape, a, 2000, tol, precond=aar_hip.ILU0)))
     for name, fn in runs:
