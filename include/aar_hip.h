@@ -215,6 +215,14 @@ int aar_hip_helm_matvec(const aar_hip_helm *h, const double *u, double *y);
 int aar_hip_helm_aar(const aar_hip_helm *h, double *u, const double *b, double omega, double beta,
                      int m, int p, int max_iter, double tol, int adjoint, aar_hip_stats *st);
 
+/* GMRES of the complex PETSc variant, PETSC_COMPLEX/main.c:80-118 (KSPGMRES
+ * with PCJACOBI on the Helmholtz operator): aar_hip_gmres's algorithm in
+ * complex arithmetic, <x, y> = sum x conj(y), complex Givens rotations;
+ * precond 1 = Jacobi 1/D, 0 = none; restart <= 32.  u, b complex128
+ * interleaved (device).  stats as aar_hip_gmres. */
+int aar_hip_helm_gmres(const aar_hip_helm *h, double *u, const double *b, int restart, int max_iter, double tol,
+                       int precond, aar_hip_stats *st);
+
 /* Gradient of a loss through u = A(c)^{-1} b with respect to the diagonal
  * coefficient field c, given the adjoint solution lambda of A^H lambda =
  * dL/du:  dc[k] = -Re(conj(lambda[k]) u[k]).  (dL/db = lambda.) */

@@ -83,6 +83,18 @@ def helm_aar(u, b, shape, a, c=None, c0=0.0, eta=0.0, omega=0.6, beta=0.6, m=7, 
     return st.as_dict()
 
 
+def helm_gmres(u, b, shape, a, c=None, c0=0.0, eta=0.0, max_iter=2000, tol=1e-6, restart=30, precond=True):
+    """GMRES of the complex PETSc variant (PETSC_COMPLEX/main.c:80-118) on
+    A u = b: u is the initial guess, overwritten; 'reason' in the stats is
+    PETSc's KSPConvergedReason."""
+    h = _helm(shape, a, c, c0, eta)
+    st = Stats()
+    _bind_stream()
+    check(_L().aar_hip_helm_gmres(ctypes.byref(h), _cplx(u, "u"), _cplx(b, "b"), int(restart), int(max_iter),
+                                  float(tol), int(bool(precond)), ctypes.byref(st)))
+    return st.as_dict()
+
+
 def helm_grad_c(lam, u):
     """-Re(conj(lambda) u), float64."""
     import torch

@@ -1562,3 +1562,489 @@ int bicg_solve(const Op &op, double *x, const double *b, int max_it, double tol,
 }
 
 }  // namespace aar
+
+// ===========================================================================
+// GMRES of the complex PETSc variant (PETSC_COMPLEX/main.c:80-118): the
+// algorithm above on the complex Helmholtz operator, as PETSc's complex
+// build runs it: <x, y> = sum x conj(y), Givens rotations with complex
+// c and s.  Vectors complex128 interleaved (double2).  Complex products,
+// quotients and the Jacobi reciprocal (dr - i eta) / (dr^2 + eta^2), D =
+// (a*3*coeff[0] + c) + i eta, use explicit formulas (no fused operations),
+// identical to the oracle's, so pointwise results agree bit for bit.
+// ===========================================================================
+namespace aar {
+
+constexpr int KPZ = 2 * (KMAX + 1);   // partial sums per block, complex dots
+
+__device__ __forceinline__ double2 zmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 zconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double2 zadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 zsub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 zdiv(double2 a, double2 b)
+{
+    const double den = b.x * b.x + b.y * b.y;
+    return make_double2((a.x * b.x + a.y * b.y) / den, (a.y * b.x - a.x * b.y) / den);
+}
+__device__ __forceinline__ double zabs(double2 a) { return sqrt(a.x * a.x + a.y * a.y); }
+
+struct ZDiag {
+    const double *c;
+    double dlap, c0, eta;
+    int precond;
+};
+
+// B x for the complex Jacobi preconditioner (identity without one)
+__device__ __forceinline__ double2 zpc(const ZDiag &d, long i, double2 x)
+{
+    if (!d.precond) return x;
+    const double dr = d.dlap + (d.c ? d.c[i] : d.c0);
+    const double den = dr * dr + d.eta * d.eta;
+    return zmul(make_double2(dr / den, -d.eta / den), x);
+}
+
+struct KCtrlZ {
+    Ctrl c;
+    Ctrl gate;
+    int reason, its, it, open, noscale, nosol, nh, hcap;
+    int max_it, restart, pad0, pad1;
+    double rtol, ttol, rnorm0, bpnorm, rhs_norm, scale, relres;
+    double2 h[KMAX + 1];
+    double2 HH[(KMAX + 2) * (KMAX + 1)];
+    double2 cc[KMAX + 1], ss[KMAX + 1], grs[KMAX + 2], nrs[KMAX + 1];
+};
+
+__device__ __forceinline__ double2 &ZHe(KCtrlZ *k, int r, int c) { return k->HH[c * (KMAX + 2) + r]; }
+
+__device__ int kz_converged(KCtrlZ *k, int n, double rnorm)
+{
+    if (n == 0) {
+        k->ttol = fmax(k->rtol * k->bpnorm, K_ATOL);
+        k->rnorm0 = rnorm;
+    }
+    if (isnan(rnorm) || isinf(rnorm)) return K_DIVERGED_NANORINF;
+    if (rnorm <= k->ttol) return rnorm < K_ATOL ? K_CONVERGED_ATOL : K_CONVERGED_RTOL;
+    if (rnorm >= K_DTOL * k->rnorm0) return K_DIVERGED_DTOL;
+    return K_ITERATING;
+}
+
+__device__ void kz_push(KCtrlZ *k, double *hist, double v)
+{
+    if (hist && k->nh < k->hcap) hist[k->nh] = v;
+    k->nh++;
+}
+
+// block sums of NA accumulators (first na used) into part[blockIdx.x * KPZ + j]
+template <int NA>
+__device__ void kz_block_sums(const double (&acc)[NA], int na, double *part)
+{
+    __shared__ double red[KB / 64][NA];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < NA; j++) {
+        if (j < na) {
+            const double v = wave_sum(acc[j]);
+            if (lane == 0) red[wv][j] = v;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < na) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < KB / 64; w++) s += red[w][threadIdx.x];
+        part[(size_t)blockIdx.x * KPZ + threadIdx.x] = s;
+    }
+}
+
+__device__ void kz_reduce(const double *part, int nb, int na, double *out)
+{
+    __shared__ double red[KB / 64][KPZ];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int j = 0; j < na; j++) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nb; b += KB) s += part[(size_t)b * KPZ + j];
+        s = wave_sum(s);
+        if (lane == 0) red[wv][j] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < na) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < KB / 64; w++) s += red[w][threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(KB) void kz_bnorm_kernel(long n, const double2 *b, ZDiag dg, double *part)
+{
+    double acc[2] = {0.0, 0.0};
+    KLOOP(i)
+    {
+        const double2 bi = b[i];
+        const double2 z = zpc(dg, i, bi);
+        acc[0] += z.x * z.x + z.y * z.y;
+        acc[1] += bi.x * bi.x + bi.y * bi.y;
+    }
+    kz_block_sums<2>(acc, 2, part);
+}
+
+__global__ __launch_bounds__(KB) void kz_resnorm_kernel(long n, const double2 *b, const double2 *t, double *part)
+{
+    double acc[1] = {0.0};
+    KLOOP(i)
+    {
+        const double2 r = zsub(b[i], t[i]);
+        acc[0] += r.x * r.x + r.y * r.y;
+    }
+    kz_block_sums<1>(acc, 1, part);
+}
+
+__global__ __launch_bounds__(KB) void kz_resid_kernel(long n, const double2 *b, const double2 *t, double2 *v0,
+                                                     ZDiag dg, double *part, const int *skip)
+{
+    if (*skip) return;
+    double acc[1] = {0.0};
+    KLOOP(i)
+    {
+        const double2 v = zpc(dg, i, zsub(b[i], t[i]));
+        v0[i] = v;
+        acc[0] += v.x * v.x + v.y * v.y;
+    }
+    kz_block_sums<1>(acc, 1, part);
+}
+
+// h_j = <B t, v_j> = sum (B t) conj(v_j)
+template <int NV>
+__global__ __launch_bounds__(KB) void kz_mdot_kernel(long n, const double2 *t, const double2 *V, long ld, int nv,
+                                                    ZDiag dg, double *part, const int *skip)
+{
+    if (*skip) return;
+    double acc[2 * NV];
+#pragma unroll
+    for (int j = 0; j < 2 * NV; j++) acc[j] = 0.0;
+    KLOOP(i)
+    {
+        const double2 w = zpc(dg, i, t[i]);
+#pragma unroll
+        for (int j = 0; j < NV; j++)
+            if (j < nv) {
+                const double2 p = zmul(w, zconj(V[(size_t)j * ld + i]));
+                acc[2 * j] += p.x;
+                acc[2 * j + 1] += p.y;
+            }
+    }
+    kz_block_sums<2 * NV>(acc, 2 * nv, part);
+}
+
+template <int NV>
+__global__ __launch_bounds__(KB) void kz_orth_kernel(long n, const double2 *t, double2 *V, long ld, int nv, ZDiag dg,
+                                                    const KCtrlZ *k, double *part)
+{
+    if (k->gate.done) return;
+    double2 nh[NV];
+#pragma unroll
+    for (int j = 0; j < NV; j++) nh[j] = j < nv ? make_double2(-k->h[j].x, -k->h[j].y) : make_double2(0.0, 0.0);
+    double2 *out = V + (size_t)nv * ld;
+    double acc[1] = {0.0};
+    KLOOP(i)
+    {
+        double2 w = zpc(dg, i, t[i]);
+#pragma unroll
+        for (int j = 0; j < NV; j++)
+            if (j < nv) w = zadd(w, zmul(nh[j], V[(size_t)j * ld + i]));
+        out[i] = w;
+        acc[0] += w.x * w.x + w.y * w.y;
+    }
+    kz_block_sums<1>(acc, 1, part);
+}
+
+__global__ __launch_bounds__(KB) void kz_scale_kernel(long n, double2 *v, const KCtrlZ *k, int gated)
+{
+    if (k->noscale || (gated && k->gate.done)) return;
+    const double s = k->scale;
+    KLOOP(i) v[i] = make_double2(s * v[i].x, s * v[i].y);
+}
+
+template <int NV>
+__global__ __launch_bounds__(KB) void kz_soln_kernel(long n, double2 *x, const double2 *V, long ld, const KCtrlZ *k)
+{
+    if (k->nosol) return;
+    const int nv = k->it;
+    double2 c[NV];
+#pragma unroll
+    for (int j = 0; j < NV; j++) c[j] = j < nv ? k->nrs[j] : make_double2(0.0, 0.0);
+    KLOOP(i)
+    {
+        double2 s = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int j = 0; j < NV; j++)
+            if (j < nv) s = zadd(s, zmul(c[j], V[(size_t)j * ld + i]));
+        x[i] = zadd(x[i], s);
+    }
+}
+
+__global__ __launch_bounds__(KB) void kz_ctl_init_kernel(KCtrlZ *k, double rtol, int max_it, int restart, int hcap,
+                                                        const double *part, int nb)
+{
+    __shared__ double s[KPZ];
+    kz_reduce(part, nb, 2, s);
+    if (threadIdx.x != 0) return;
+    k->rtol = rtol;
+    k->max_it = max_it;
+    k->restart = restart;
+    k->hcap = hcap;
+    k->bpnorm = sqrt(s[0]);
+    k->rhs_norm = sqrt(s[1]);
+    k->gate.done = 1;
+    k->nosol = 1;
+    k->noscale = 1;
+}
+
+__global__ __launch_bounds__(KB) void kz_start_kernel(KCtrlZ *k, const double *part, int nb, double *hist)
+{
+    if (k->c.done) return;
+    __shared__ double s[KPZ];
+    kz_reduce(part, nb, 1, s);
+    if (threadIdx.x != 0) return;
+    const double res = sqrt(s[0]);
+    k->open = 1;
+    k->it = 0;
+    k->grs[0] = make_double2(res, 0.0);
+    k->scale = 1.0 / res;
+    k->noscale = (res == 0.0 || res == 1.0);
+    kz_push(k, hist, res);
+    k->reason = res == 0.0 ? K_CONVERGED_ATOL : kz_converged(k, k->its, res);
+    const bool active = !k->reason && k->restart > 0 && k->its < k->max_it;
+    k->gate.done = active ? 0 : 1;
+}
+
+__global__ __launch_bounds__(KB) void kz_dots_kernel(KCtrlZ *k, const double *part, int nb, int nv)
+{
+    if (k->gate.done) return;
+    __shared__ double s[KPZ];
+    kz_reduce(part, nb, 2 * nv, s);
+    if (threadIdx.x < nv) k->h[threadIdx.x] = make_double2(s[2 * threadIdx.x], s[2 * threadIdx.x + 1]);
+}
+
+__global__ __launch_bounds__(KB) void kz_arnoldi_kernel(KCtrlZ *k, const double *part, int nb, double *hist)
+{
+    if (k->gate.done) return;
+    __shared__ double s[KPZ];
+    kz_reduce(part, nb, 1, s);
+    if (threadIdx.x != 0) return;
+    const int it = k->it;
+    const double tt = sqrt(s[0]);
+    for (int j = 0; j <= it; j++) ZHe(k, j, it) = k->h[j];
+    ZHe(k, it + 1, it) = make_double2(tt, 0.0);
+    k->scale = 1.0 / tt;
+    double hapbnd = tt / zabs(k->grs[it]);
+    if (hapbnd > K_HAPTOL) hapbnd = K_HAPTOL;
+    const bool hapend = tt < hapbnd;
+    double2 *hh = &ZHe(k, 0, it);
+    for (int j = 1; j <= it; j++) {
+        const double2 t1 = hh[j - 1];
+        hh[j - 1] = zadd(zmul(zconj(k->cc[j - 1]), t1), zmul(k->ss[j - 1], hh[j]));
+        hh[j] = zsub(zmul(k->cc[j - 1], hh[j]), zmul(k->ss[j - 1], t1));
+    }
+    double res = 0.0;
+    int reason = K_ITERATING;
+    if (!hapend) {
+        const double2 q = zadd(zmul(zconj(hh[it]), hh[it]), zmul(zconj(hh[it + 1]), hh[it + 1]));
+        const double r = sqrt(q.x);
+        if (r == 0.0) {
+            reason = K_DIVERGED_NULL;
+        } else {
+            const double2 rr = make_double2(r, 0.0);
+            k->cc[it] = zdiv(hh[it], rr);
+            k->ss[it] = zdiv(hh[it + 1], rr);
+            const double2 g = k->grs[it];
+            const double2 ng = zmul(k->ss[it], g);
+            k->grs[it + 1] = make_double2(-ng.x, -ng.y);
+            k->grs[it] = zmul(zconj(k->cc[it]), g);
+            hh[it] = zadd(zmul(zconj(k->cc[it]), hh[it]), zmul(k->ss[it], hh[it + 1]));
+            res = zabs(k->grs[it + 1]);
+        }
+    }
+    k->it = it + 1;
+    k->its++;
+    kz_push(k, hist, res);
+    if (!reason) reason = kz_converged(k, k->its, res);
+    if (hapend && !reason) reason = K_DIVERGED_BREAKDOWN;
+    k->reason = reason;
+    const bool active = !reason && !hapend && k->it < k->restart && k->its < k->max_it;
+    k->gate.done = active ? 0 : 1;
+    k->noscale = (!active || tt == 0.0 || tt == 1.0);
+}
+
+__global__ void kz_soln_ctl_kernel(KCtrlZ *k)
+{
+    if (!k->open || k->it == 0) {
+        k->nosol = 1;
+        return;
+    }
+    const int kk = k->it - 1;
+    k->nrs[kk] = zdiv(k->grs[kk], ZHe(k, kk, kk));
+    for (int ii = 1; ii <= kk; ii++) {
+        const int r = kk - ii;
+        double2 tt = k->grs[r];
+        for (int j = r + 1; j <= kk; j++) tt = zsub(tt, zmul(ZHe(k, r, j), k->nrs[j]));
+        k->nrs[r] = zdiv(tt, ZHe(k, r, r));
+    }
+    k->nosol = 0;
+}
+
+__global__ void kz_end_kernel(KCtrlZ *k)
+{
+    if (!k->open) return;
+    if (k->its >= k->max_it && !k->reason) k->reason = K_DIVERGED_ITS;
+    if (k->reason) {
+        k->c.done = 1;
+        k->c.final_iter = k->its;
+        k->gate.done = 1;
+    }
+    k->open = 0;
+    k->nosol = 1;
+    k->noscale = 1;
+    k->gate.done = 1;
+}
+
+__global__ __launch_bounds__(KB) void kz_final_kernel(KCtrlZ *k, const double *part, int nb)
+{
+    __shared__ double s[KPZ];
+    kz_reduce(part, nb, 1, s);
+    if (threadIdx.x != 0) return;
+    k->relres = sqrt(s[0]);
+}
+
+namespace {
+template <int NV>
+hipError_t launch_gz_step(const double2 *t, double2 *V, long ld, int nv, long n, int nb, ZDiag dg, KCtrlZ *k,
+                          double *part)
+{
+    hipStream_t st = ctx().stream;
+    hipLaunchKernelGGL((kz_mdot_kernel<NV>), dim3(nb), dim3(KB), 0, st, n, t, (const double2 *)V, ld, nv, dg, part,
+                       (const int *)&k->gate.done);
+    hipLaunchKernelGGL(kz_dots_kernel, dim3(1), dim3(KB), 0, st, k, (const double *)part, nb, nv);
+    hipLaunchKernelGGL((kz_orth_kernel<NV>), dim3(nb), dim3(KB), 0, st, n, t, V, ld, nv, dg, (const KCtrlZ *)k, part);
+    return hipGetLastError();
+}
+template <int NV>
+hipError_t launch_gz_soln(double2 *x, const double2 *V, long ld, long n, int nb, KCtrlZ *k)
+{
+    hipLaunchKernelGGL((kz_soln_kernel<NV>), dim3(nb), dim3(KB), 0, ctx().stream, n, x, V, ld, (const KCtrlZ *)k);
+    return hipGetLastError();
+}
+}  // namespace
+
+int helm_gmres_solve(const HOp &op, double *u, const double *b, int restart, int max_it, double tol, int precond,
+                     aar_hip_stats *st)
+{
+    Context &C = ctx();
+    if (int rc = context_init()) return rc;
+    if (!u || !b) return fail(AAR_HIP_ERR_ARG, "null vector");
+    if (max_it < 0) return fail(AAR_HIP_ERR_ARG, "max_iter must be >= 0");
+    if (restart < 1 || restart > KMAX) return fail(AAR_HIP_ERR_ARG, "restart must be in 1..32");
+    if (precond < 0 || precond > 1) return fail(AAR_HIP_ERR_ARG, "precond must be 0 (none) or 1 (Jacobi)");
+    const long n = op.n;
+    const long ld = ring_ld(n, sizeof(double2));
+    HC(ensure(C.kvec, sizeof(double2) * (size_t)ld * (restart + 2)));
+    double2 *V = (double2 *)C.kvec.p, *t = V + (size_t)ld * (restart + 1);
+    const int nb = std::min(nblocks_for(n), 1024);
+    const int hcap = max_it + max_it / restart + 8;
+    HC(ensure(C.kctl, sizeof(KCtrlZ)));
+    HC(ensure(C.hist0, sizeof(double) * (size_t)hcap));
+    HC(ensure(C.part0, sizeof(double) * (size_t)nb * KPZ));
+    KCtrlZ *k = (KCtrlZ *)C.kctl.p;
+    double *part = (double *)C.part0.p, *hist = (double *)C.hist0.p;
+    ZDiag dg;
+    dg.c = op.c;
+    dg.dlap = op.dlap;
+    dg.c0 = op.c0;
+    dg.eta = op.eta;
+    dg.precond = precond;
+    const double2 *bz = (const double2 *)b;
+    double2 *x = (double2 *)u;
+    C.bytes_acc = 0.0;
+    HC(hipEventRecord(C.t0, C.stream));
+    hipLaunchKernelGGL(k_zero_kernel, dim3(64), dim3(KB), 0, C.stream, (unsigned *)k,
+                       (long)(sizeof(KCtrlZ) / sizeof(unsigned)));
+    hipLaunchKernelGGL(kz_bnorm_kernel, dim3(nb), dim3(KB), 0, C.stream, n, bz, dg, part);
+    hipLaunchKernelGGL(kz_ctl_init_kernel, dim3(1), dim3(KB), 0, C.stream, k, tol, max_it, restart, hcap,
+                       (const double *)part, nb);
+    HC(hipGetLastError());
+    const double s = 16.0 * n, cb = op.c ? 8.0 * n : 0.0;
+    C.bytes_acc += s + cb;
+    bool stopped = false;
+    for (int cyc = 0; cyc <= max_it && !stopped; cyc++) {
+        if (int rc = helm_matvec_async(op, u, (double *)t, &k->c)) return rc;
+        hipLaunchKernelGGL(kz_resid_kernel, dim3(nb), dim3(KB), 0, C.stream, n, bz, (const double2 *)t, V, dg, part,
+                           (const int *)&k->c.done);
+        hipLaunchKernelGGL(kz_start_kernel, dim3(1), dim3(KB), 0, C.stream, k, (const double *)part, nb, hist);
+        hipLaunchKernelGGL(kz_scale_kernel, dim3(nb), dim3(KB), 0, C.stream, n, V, (const KCtrlZ *)k, 0);
+        HC(hipGetLastError());
+        C.bytes_acc += 5 * s + 2 * cb;
+        for (int it = 0; it < restart; it++) {
+            const int nv = it + 1;
+            if (int rc = helm_matvec_async(op, (const double *)(V + (size_t)it * ld), (double *)t, &k->gate))
+                return rc;
+            prof_begin("krylov_gmres_orth");
+            hipError_t e = nv <= 8 ? launch_gz_step<8>(t, V, ld, nv, n, nb, dg, k, part)
+                         : nv <= 16 ? launch_gz_step<16>(t, V, ld, nv, n, nb, dg, k, part)
+                                    : launch_gz_step<32>(t, V, ld, nv, n, nb, dg, k, part);
+            prof_end((2.0 * nv + 3) * s + 2 * cb);
+            if (e != hipSuccess) return fail(AAR_HIP_ERR_HIP, std::string("gmres step: ") + hipGetErrorString(e));
+            hipLaunchKernelGGL(kz_arnoldi_kernel, dim3(1), dim3(KB), 0, C.stream, k, (const double *)part, nb, hist);
+            hipLaunchKernelGGL(kz_scale_kernel, dim3(nb), dim3(KB), 0, C.stream, n, V + (size_t)nv * ld,
+                               (const KCtrlZ *)k, 1);
+            HC(hipGetLastError());
+            C.bytes_acc += (2.0 * nv + 5) * s + 2 * cb;
+        }
+        hipLaunchKernelGGL(kz_soln_ctl_kernel, dim3(1), dim3(1), 0, C.stream, k);
+        hipError_t e = restart <= 8 ? launch_gz_soln<8>(x, V, ld, n, nb, k)
+                     : restart <= 16 ? launch_gz_soln<16>(x, V, ld, n, nb, k)
+                                     : launch_gz_soln<32>(x, V, ld, n, nb, k);
+        if (e != hipSuccess) return fail(AAR_HIP_ERR_HIP, std::string("gmres solution: ") + hipGetErrorString(e));
+        hipLaunchKernelGGL(kz_end_kernel, dim3(1), dim3(1), 0, C.stream, k);
+        HC(hipGetLastError());
+        C.bytes_acc += (restart + 2.0) * s;
+        if (int rc = poll_record(&k->c, cyc)) return rc;
+        if (poll_check(cyc - 2)) stopped = true;
+    }
+    // the true residual of the returned u (PETSC_COMPLEX/main.c:110-113)
+    if (int rc = helm_matvec_async(op, u, (double *)t, nullptr)) return rc;
+    hipLaunchKernelGGL(kz_resnorm_kernel, dim3(nb), dim3(KB), 0, C.stream, n, bz, (const double2 *)t, part);
+    hipLaunchKernelGGL(kz_final_kernel, dim3(1), dim3(KB), 0, C.stream, k, (const double *)part, nb);
+    HC(hipGetLastError());
+    HC(hipEventRecord(C.t1, C.stream));
+    KCtrlZ hk;
+    HC(hipMemcpyAsync(&hk, k, sizeof(KCtrlZ), hipMemcpyDeviceToHost, C.stream));
+    HC(hipStreamSynchronize(C.stream));
+    float ms = 0.f;
+    HC(hipEventElapsedTime(&ms, C.t0, C.t1));
+    prof_flush();
+    const int hl = std::min(hk.nh, hcap);
+    C.hist_host.assign(std::max(hl, 0), 0.0);
+    if (hl > 0) HC(hipMemcpy(C.hist_host.data(), hist, sizeof(double) * hl, hipMemcpyDeviceToHost));
+    aar_hip_stats sres;
+    std::memset(&sres, 0, sizeof(sres));
+    sres.iterations = hk.its;
+    sres.rhs_norm = hk.rhs_norm;
+    sres.relres = hk.rhs_norm > 0 ? hk.relres / hk.rhs_norm : hk.relres;
+    sres.converged = hk.reason > 0;
+    sres.printed_converged = hk.reason > 0;
+    sres.printed_warning = hk.reason == K_DIVERGED_ITS;
+    sres.status = hk.reason == K_DIVERGED_NANORINF ? AAR_HIP_ERR_NONFINITE : 0;
+    sres.hist_len = hl;
+    sres.solve_ms = ms;
+    sres.bytes = C.bytes_acc;
+    sres.reason = hk.reason;
+    C.last = sres;
+    if (st) *st = sres;
+    if (hk.reason == K_DIVERGED_NANORINF) return fail(AAR_HIP_ERR_NONFINITE, "residual is not finite");
+    return 0;
+}
+
+}  // namespace aar

@@ -1143,6 +1143,16 @@ static int run_hstencil(HMode mode, const HArgs &a, double bytes)
     return 0;
 }
 
+// y = A u on the library stream, no wait; skipped once gate->done is set
+int helm_matvec_async(const HOp &op, const double *u, double *y, Ctrl *gate)
+{
+    HArgs a = base_hargs(op);
+    a.in0 = (const double2 *)u;
+    a.out0 = (double2 *)y;
+    a.ctrl = gate;
+    return run_hstencil(HMode::MATVEC, a, 32.0 * op.n + (op.c ? 8.0 * op.n : 0.0));
+}
+
 int helm_matvec(const HOp &op, const double *u, double *y)
 {
     Context &C = ctx();

@@ -1508,3 +1508,201 @@ int orc_pgr_petsc(int nx, int ny, int nz, int fdn, double a, double sigma, const
     free(G); free(g); free(s);
     return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* GMRES of the complex PETSc variant (PETSC_COMPLEX/main.c:80-118:    */
+/* KSPGMRES on the Helmholtz operator, PCJACOBI, rtol 1e-6, max_it     */
+/* 2000, nonzero guess), the algorithm of gmres_impl in complex        */
+/* arithmetic as PETSc's complex build runs it: <x, y> = sum x conj(y) */
+/* (VecMDot), Givens rotations with complex c and s,                   */
+/* h <- conj(c) h + s h', h' <- c h' - s h.  Complex operations use    */
+/* the explicit formulas below (no C99 complex type), so pointwise     */
+/* results are reproducible elsewhere bit for bit.  Jacobi: 1/D with  */
+/* D = (a*3*coeff[0] + c) + i eta, as (dr - i eta) / (dr^2 + eta^2).  */
+/* Vectors are complex128 interleaved (re, im).  Parity with PETSc is */
+/* unpinned (PETSc absent); pinned by the Krylov minimum and the FFT  */
+/* solution (tests).                                                   */
+/* ------------------------------------------------------------------ */
+typedef struct { double re, im; } zc;
+static inline zc zmul(zc a, zc b) { zc r = {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; return r; }
+static inline zc zconj(zc a) { zc r = {a.re, -a.im}; return r; }
+static inline zc zadd(zc a, zc b) { zc r = {a.re + b.re, a.im + b.im}; return r; }
+static inline zc zsub(zc a, zc b) { zc r = {a.re - b.re, a.im - b.im}; return r; }
+static inline zc zdiv(zc a, zc b)
+{
+    const double den = b.re * b.re + b.im * b.im;
+    zc r = {(a.re * b.re + a.im * b.im) / den, (a.im * b.re - a.re * b.im) / den};
+    return r;
+}
+static inline double zabs(zc a) { return sqrt(a.re * a.re + a.im * a.im); }
+
+int orc_gmres_complex(int nx, int ny, int nz, int fdn, double a, const double *c, double c0, double eta,
+                      double *u, const double *rhs, int restart, int max_it, double rtol, int precond,
+                      double *trace, int trace_cap, orc_stats *st)
+{
+    if (restart < 1 || restart > ORC_GMRES_MAXK) return -1;
+    const long n = (long)nx * ny * nz;
+    const int K = restart;
+    double coeff[32] = {0};
+    orc_lap_coefficient(coeff, fdn);
+    const double dlap = a * (3 * coeff[0]);
+    zc *x = (zc *)u;
+    const zc *b = (const zc *)rhs;
+    zc *dinv = (zc *)malloc(sizeof(zc) * n);
+    zc *V = (zc *)malloc(sizeof(zc) * n * (K + 1));
+    zc *t = (zc *)malloc(sizeof(zc) * n);
+    zc HH[(ORC_GMRES_MAXK + 2) * (ORC_GMRES_MAXK + 1)];
+    zc cc[ORC_GMRES_MAXK + 1], ss[ORC_GMRES_MAXK + 1], grs[ORC_GMRES_MAXK + 2], nrs[ORC_GMRES_MAXK + 1];
+    zc h[ORC_GMRES_MAXK + 1];
+#define ZH(r, col) HH[(col) * (K + 2) + (r)]
+    for (long i = 0; i < n; i++) {
+        if (precond) {
+            const double dr = dlap + (c ? c[i] : c0);
+            const double den = dr * dr + eta * eta;
+            dinv[i].re = dr / den;
+            dinv[i].im = -eta / den;
+        } else {
+            dinv[i].re = 1.0;
+            dinv[i].im = 0.0;
+        }
+    }
+    int tlen = 0;
+    ksp_conv cv_ = {0.0, 0.0, rtol, 0.0};
+    double s2 = 0.0;
+    for (long i = 0; i < n; i++) {
+        const zc z = precond ? zmul(dinv[i], b[i]) : b[i];
+        s2 = s2 + (z.re * z.re + z.im * z.im);
+    }
+    cv_.bpnorm = sqrt(s2);
+    int its = 0, reason = 0;
+    double t0 = now_seconds();
+    while (!reason) {
+        zc *v0 = V;
+        orc_matvec_complex(nx, ny, nz, fdn, coeff, a, c, c0, eta, (const double *)x, (double *)t);
+        s2 = 0.0;
+        for (long i = 0; i < n; i++) {
+            const zc r = zsub(b[i], t[i]);
+            v0[i] = precond ? zmul(dinv[i], r) : r;
+            s2 = s2 + (v0[i].re * v0[i].re + v0[i].im * v0[i].im);
+        }
+        double res = sqrt(s2);
+        if (res != 0.0 && res != 1.0) {
+            const double sc = 1.0 / res;
+            for (long i = 0; i < n; i++) { v0[i].re = sc * v0[i].re; v0[i].im = sc * v0[i].im; }
+        }
+        grs[0].re = res;
+        grs[0].im = 0.0;
+        int it = 0;
+        if (trace && tlen < trace_cap) trace[tlen++] = res;
+        if (res == 0.0) { reason = ORC_KSP_CONVERGED_ATOL; break; }
+        reason = ksp_converged(&cv_, its, res);
+        while (!reason && it < K && its < max_it) {
+            zc *vn = V + (long)(it + 1) * n;
+            orc_matvec_complex(nx, ny, nz, fdn, coeff, a, c, c0, eta, (const double *)(V + (long)it * n),
+                               (double *)t);
+            for (long i = 0; i < n; i++) vn[i] = precond ? zmul(dinv[i], t[i]) : t[i];
+            for (int j = 0; j <= it; j++) {
+                zc sj = {0.0, 0.0};
+                const zc *vj = V + (long)j * n;
+                for (long i = 0; i < n; i++) sj = zadd(sj, zmul(vn[i], zconj(vj[i])));
+                h[j] = sj;
+            }
+            s2 = 0.0;
+            for (long i = 0; i < n; i++) {
+                zc w = vn[i];
+                for (int j = 0; j <= it; j++) {
+                    const zc nh = {-h[j].re, -h[j].im};
+                    w = zadd(w, zmul(nh, V[(long)j * n + i]));
+                }
+                vn[i] = w;
+                s2 = s2 + (w.re * w.re + w.im * w.im);
+            }
+            for (int j = 0; j <= it; j++) ZH(j, it) = h[j];
+            const double tt = sqrt(s2);
+            if (tt != 0.0 && tt != 1.0) {
+                const double sc = 1.0 / tt;
+                for (long i = 0; i < n; i++) { vn[i].re = sc * vn[i].re; vn[i].im = sc * vn[i].im; }
+            }
+            ZH(it + 1, it).re = tt;
+            ZH(it + 1, it).im = 0.0;
+            double hapbnd = tt / zabs(grs[it]);
+            if (hapbnd > ORC_GMRES_HAPTOL) hapbnd = ORC_GMRES_HAPTOL;
+            const int hapend = tt < hapbnd;
+            zc *hh = &ZH(0, it);
+            for (int j = 1; j <= it; j++) {
+                const zc t1 = hh[j - 1];
+                hh[j - 1] = zadd(zmul(zconj(cc[j - 1]), t1), zmul(ss[j - 1], hh[j]));
+                hh[j] = zsub(zmul(cc[j - 1], hh[j]), zmul(ss[j - 1], t1));
+            }
+            if (!hapend) {
+                const zc q = zadd(zmul(zconj(hh[it]), hh[it]), zmul(zconj(hh[it + 1]), hh[it + 1]));
+                const double r = sqrt(q.re);
+                if (r == 0.0) {
+                    reason = ORC_KSP_DIVERGED_NULL;
+                } else {
+                    const zc rr = {r, 0.0};
+                    cc[it] = zdiv(hh[it], rr);
+                    ss[it] = zdiv(hh[it + 1], rr);
+                    const zc g = grs[it];
+                    const zc ng = zmul(ss[it], g);
+                    grs[it + 1].re = -ng.re;
+                    grs[it + 1].im = -ng.im;
+                    grs[it] = zmul(zconj(cc[it]), g);
+                    hh[it] = zadd(zmul(zconj(cc[it]), hh[it]), zmul(ss[it], hh[it + 1]));
+                    res = zabs(grs[it + 1]);
+                }
+            } else {
+                res = 0.0;
+            }
+            it++;
+            its++;
+            if (trace && tlen < trace_cap) trace[tlen++] = res;
+            if (!reason) reason = ksp_converged(&cv_, its, res);
+            if (hapend) {
+                if (!reason) reason = ORC_KSP_DIVERGED_BREAKDOWN;
+                break;
+            }
+        }
+        if (it > 0) {
+            const int k = it - 1;
+            nrs[k] = zdiv(grs[k], ZH(k, k));
+            for (int ii = 1; ii <= k; ii++) {
+                const int kk = k - ii;
+                zc tt = grs[kk];
+                for (int j = kk + 1; j <= k; j++) tt = zsub(tt, zmul(ZH(kk, j), nrs[j]));
+                nrs[kk] = zdiv(tt, ZH(kk, kk));
+            }
+            for (long i = 0; i < n; i++) {
+                zc sacc = {0.0, 0.0};
+                for (int j = 0; j <= k; j++) sacc = zadd(sacc, zmul(nrs[j], V[(long)j * n + i]));
+                x[i] = zadd(x[i], sacc);
+            }
+        }
+        if (its >= max_it) {
+            if (!reason) reason = ORC_KSP_DIVERGED_ITS;
+            break;
+        }
+    }
+#undef ZH
+    double t1 = now_seconds();
+    if (st) {
+        orc_matvec_complex(nx, ny, nz, fdn, coeff, a, c, c0, eta, (const double *)x, (double *)t);
+        double rn = 0.0, bn = 0.0;
+        for (long i = 0; i < n; i++) {
+            const zc r = zsub(b[i], t[i]);
+            rn = rn + (r.re * r.re + r.im * r.im);
+            bn = bn + (b[i].re * b[i].re + b[i].im * b[i].im);
+        }
+        st->iterations = its;
+        st->relres = sqrt(rn) / sqrt(bn);
+        st->rhs_norm = sqrt(bn);
+        st->converged = reason > 0;
+        st->printed_converged = reason > 0;
+        st->printed_warning = reason == ORC_KSP_DIVERGED_ITS;
+        st->trace_len = tlen;
+        st->seconds = t1 - t0;
+        st->status = reason;
+    }
+    free(dinv); free(V); free(t);
+    return 0;
+}

@@ -178,6 +178,14 @@ int orc_aar_petsc(int nx, int ny, int nz, int fdn, double a, double sigma, const
 int orc_pgr_petsc(int nx, int ny, int nz, int fdn, double a, double sigma, const double *c,
                   double *x, const double *b, double omega, int m, int p, int max_iter, double tol,
                   int precond, int l2, double *trace, int trace_cap, orc_stats *st);
+/* GMRES of the complex PETSc variant (PETSC_COMPLEX/main.c:80-118) on
+ * the Helmholtz operator of orc_aar_complex (c may be NULL: c0), u and rhs
+ * complex128 interleaved; precond 1 = Jacobi 1/D, 0 = none.  Complex
+ * operations by explicit formulas (aar_oracle.c); stats/trace as
+ * orc_gmres. */
+int orc_gmres_complex(int nx, int ny, int nz, int fdn, double a, const double *c, double c0, double eta,
+                      double *u, const double *rhs, int restart, int max_it, double rtol, int precond,
+                      double *trace, int trace_cap, orc_stats *st);
 int orc_gmres(int nx, int ny, int nz, int fdn, double a, double sigma, const double *c,
               double *x, const double *b, int restart, int max_it, double rtol, int precond,
               double *trace, int trace_cap, orc_stats *st);

@@ -96,12 +96,15 @@ def lib():
         L.orc_ilu0_factor.argtypes = [I, I, I, I, Dd, Dd, _D, _D]
         L.orc_aar_petsc.argtypes = [I, I, I, I, Dd, Dd, _D, _D, _D, Dd, Dd, I, I, I, Dd, I, _D, I,
                                     ctypes.POINTER(OrcStats)]
+        L.orc_gmres_complex.argtypes = [I, I, I, I, Dd, _D, Dd, Dd, _D, _D, I, I, Dd, I, _D, I,
+                                        ctypes.POINTER(OrcStats)]
         L.orc_pgr_petsc.argtypes = [I, I, I, I, Dd, Dd, _D, _D, _D, Dd, I, I, I, Dd, I, I, _D, I,
                                     ctypes.POINTER(OrcStats)]
         L.orc_ilu0_apply.argtypes = [I, I, I, _D, _D, _D, I]
         L.orc_ilu0_apply.restype = None
         for fn in ("orc_aar", "orc_pgr", "orc_cg", "orc_aar_complex", "orc_aar_vc", "orc_pgr_vc", "orc_cg_vc",
-                   "orc_gmres", "orc_bicg", "orc_ilu0_factor", "orc_aar_petsc", "orc_pgr_petsc"):
+                   "orc_gmres", "orc_bicg", "orc_ilu0_factor", "orc_aar_petsc", "orc_pgr_petsc",
+                   "orc_gmres_complex"):
             getattr(L, fn).restype = I
         _lib = L
     return _lib
@@ -316,6 +319,19 @@ def pgr_petsc(x0, rhs, shape, a, omega, m, p, max_iter, tol, sigma=0.0, c=None, 
     d = _run(lib().orc_pgr_petsc, (nx, ny, nz, fdn, a, sigma, _p(cv), _p(x), _p(b), omega, m, p, max_iter, tol,
                                    int(precond), int(bool(l2))), trace, max_iter // max(1, p) + 8)
     return x, d
+
+
+def gmres_complex(u0, rhs, shape, a, c, c0, eta, max_iter, tol, restart=30, precond=True, fdn=6, trace=True):
+    """GMRES of the complex PETSc variant on the Helmholtz operator
+    (complex128 arrays; c may be None)."""
+    nx, ny, nz = shape
+    u = np.ascontiguousarray(u0, dtype=np.complex128).reshape(-1).copy()
+    b = np.ascontiguousarray(rhs, dtype=np.complex128).reshape(-1)
+    cc = None if c is None else _f64(c).reshape(-1)
+    cap = max_iter + max_iter // max(1, restart) + 8
+    d = _run(lib().orc_gmres_complex, (nx, ny, nz, fdn, a, _p(cc), c0, eta, _p(u.view(np.float64)),
+                                       _p(b.view(np.float64)), restart, max_iter, tol, int(precond)), trace, cap)
+    return u, d
 
 
 def bicg(x0, rhs, shape, a, max_iter, tol, sigma=0.0, c=None, precond=True, fdn=6, trace=True):

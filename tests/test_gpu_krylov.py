@@ -400,3 +400,51 @@ def test_pgr_petsc_parity(gpu, oracle, case):
         assert np.linalg.norm(xg - xo) <= bound
     else:
         assert np.abs(xg - xo).max() <= (1e-5 if pc != 2 else 1e-2) * np.abs(xo).max()
+
+
+# ---------------------------------------------------------------------------
+# GMRES of the complex PETSc variant (Helmholtz operator)
+# ---------------------------------------------------------------------------
+def cdev(v):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(v, dtype=np.complex128)).to("cuda")
+
+
+HZ_CASES = [
+    # shape, variable c, c0, eta, restart, max_it, tol, precond
+    ((16, 16, 16), True, 0.0, 0.5, 30, 2000, 1e-6, True),
+    ((16, 16, 16), True, 0.0, 0.5, 30, 2000, 1e-6, False),
+    ((24, 20, 16), False, 0.3, 0.5, 7, 2000, 1e-8, True),
+    ((16, 16, 16), True, 0.0, 0.5, 30, 11, 1e-10, True),
+    ((36, 36, 36), False, -0.134992, 0.070225, 30, 2000, 1e-6, True),
+]
+
+
+def _hid(c):
+    return "x".join(map(str, c[0])) + ("_vc" if c[1] else f"_c{c[2]}") + f"_eta{c[3]}_k{c[4]}_i{c[5]}" + \
+        ("" if c[7] else "_noprec")
+
+
+@pytest.mark.parametrize("case", HZ_CASES, ids=_hid)
+def test_helm_gmres_parity(gpu, oracle, case):
+    """Counts and reasons equal to the oracle's, every convergence-test
+    norm within 1e-6 relative (or 1e-8 of the first), the solution within
+    1e-8.  The 36^3 case is PETSC_COMPLEX's own problem (a = -1/(4 pi),
+    Q = -0.134992 - 0.070225i as c0 + i eta)."""
+    shape, vc, c0, eta, restart, max_it, tol, pc = case
+    say("helm gmres", _hid(case))
+    n = int(np.prod(shape))
+    a = P.poisson_a(0.5) if shape[0] != 36 else -1.0 / (4 * np.pi)
+    c = P.helmholtz_coefficient(shape, 7) if vc else None
+    b = P.rhs_complex_normal(shape, 1005)
+    u0 = np.ones(n, dtype=complex)
+    uo, d = oracle.gmres_complex(u0, b, shape, a, c, c0, eta, max_it, tol, restart=restart, precond=pc)
+    ug = cdev(u0)
+    st = gpu.helm_gmres(ug, cdev(b), shape, a, c=None if c is None else dev(c), c0=c0, eta=eta, max_iter=max_it,
+                        tol=tol, restart=restart, precond=pc)
+    assert st["iterations"] == d["iterations"] and st["reason"] == d["status"]
+    h = gpu.history()
+    assert len(h) == len(d["trace"])
+    np.testing.assert_allclose(h, d["trace"], rtol=1e-6, atol=1e-8 * d["trace"][0])
+    ug = ug.cpu().numpy()
+    assert np.abs(ug - uo).max() <= 1e-8 * np.abs(uo).max()

@@ -435,3 +435,65 @@ def test_pgr_petsc_ilu0_first_step_and_solution(oracle, l2):
     x, d = oracle.pgr_petsc(x0, b, shape, a, 0.6, 7, 6, 1000, 1e-10, sigma=sigma, precond=2, l2=l2)
     ref = fft_solve(b, shape, a, oracle.lap_coefficient(6), sigma)
     assert d["converged"] and np.abs(x - ref).max() <= 1e-7 * np.abs(ref).max()
+
+
+# ---------------------------------------------------------------------------
+# GMRES of the complex PETSc variant on the Helmholtz operator
+# ---------------------------------------------------------------------------
+def test_gmres_complex_estimate_is_krylov_minimum(oracle):
+    shape = (8, 8, 8)
+    n = 512
+    a, c0, eta = -4.0, 0.3, 0.5
+    rng = np.random.default_rng(51)
+    c = rng.uniform(0.5, 1.5, n)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    u0 = np.ones(n, dtype=complex)
+    A = np.empty((n, n), dtype=complex)
+    e = np.zeros(n, dtype=complex)
+    for j in range(n):
+        e[:] = 0
+        e[j] = 1
+        A[:, j] = oracle.matvec_complex(e, shape, a, c, c0, eta)
+    D = a * 3 * oracle.lap_coefficient(6)[0] + c + 1j * eta
+    M = A / D[:, None]
+    r0 = (b - A @ u0) / D
+    _, st = oracle.gmres_complex(u0, b, shape, a, c, c0, eta, 10, 1e-14)
+    tr = st["trace"]
+    assert tr[0] == pytest.approx(np.linalg.norm(r0), rel=1e-12)
+    Q = [r0 / np.linalg.norm(r0)]
+    for k in range(1, 11):
+        v = M @ Q[-1]
+        for _ in range(2):
+            for q in Q:
+                v = v - (np.vdot(q, v)) * q
+        Q.append(v / np.linalg.norm(v))
+        K = np.array(Q[:k]).T
+        y, *_ = np.linalg.lstsq(M @ K, r0, rcond=None)
+        assert tr[k] == pytest.approx(np.linalg.norm(r0 - M @ K @ y), rel=1e-7), k
+
+
+def test_gmres_complex_reduces_to_real_and_solves(oracle):
+    """eta = 0, constant c, real b: the real screened GMRES with sigma = c
+    (same count, norms to rounding: the Jacobi reciprocal differs in its
+    last bit); eta > 0: converges to the FFT solution."""
+    shape = (12, 12, 12)
+    n = int(np.prod(shape))
+    a = -4.0
+    b = rhs(shape, 52)
+    xr, dr = oracle.gmres(np.ones(n), b, shape, a, 500, 1e-8, sigma=0.7)
+    xc, dc = oracle.gmres_complex(np.ones(n, dtype=complex), b.astype(complex), shape, a, None, 0.7, 0.0, 500, 1e-8)
+    assert dc["iterations"] == dr["iterations"]
+    np.testing.assert_allclose(dc["trace"], dr["trace"], rtol=1e-9, atol=1e-11 * dr["trace"][0])
+    assert np.abs(xc.imag).max() == 0.0 and np.abs(xc.real - xr).max() <= 1e-10 * np.abs(xr).max()
+    eta, c0 = 0.5, 0.3
+    bc = np.random.default_rng(53).standard_normal(n) + 1j * np.random.default_rng(54).standard_normal(n)
+    u, d = oracle.gmres_complex(np.ones(n, dtype=complex), bc, shape, a, None, c0, eta, 2000, 1e-10)
+    cf = oracle.lap_coefficient(6)
+    nx, ny, nz = shape
+    def s1(m):
+        th = 2 * np.pi * np.arange(m) / m
+        return cf[0] + 2 * sum(cf[q] * np.cos(q * th) for q in range(1, 7))
+    lam = a * (s1(nz)[:, None, None] + s1(ny)[None, :, None] + s1(nx)[None, None, :]) + c0 + 1j * eta
+    ref = np.fft.ifftn(np.fft.fftn(bc.reshape(nz, ny, nx)) / lam).reshape(-1)
+    assert d["status"] == oracle.KSP_CONVERGED_RTOL
+    assert np.abs(u - ref).max() <= 1e-7 * np.abs(ref).max()
