@@ -223,6 +223,12 @@ int aar_hip_helm_aar(const aar_hip_helm *h, double *u, const double *b, double o
 int aar_hip_helm_gmres(const aar_hip_helm *h, double *u, const double *b, int restart, int max_iter, double tol,
                        int precond, aar_hip_stats *st);
 
+/* BiCG of the complex PETSc variant, PETSC_COMPLEX/main.c:150-175 (KSPBICG
+ * with PCJACOBI): the shadow sequence with A^H (eta -> -eta; the operator
+ * is complex symmetric) and B^H = conj(1/D). */
+int aar_hip_helm_bicg(const aar_hip_helm *h, double *u, const double *b, int max_iter, double tol, int precond,
+                      aar_hip_stats *st);
+
 /* Gradient of a loss through u = A(c)^{-1} b with respect to the diagonal
  * coefficient field c, given the adjoint solution lambda of A^H lambda =
  * dL/du:  dc[k] = -Re(conj(lambda[k]) u[k]).  (dL/db = lambda.) */

@@ -95,6 +95,16 @@ def helm_gmres(u, b, shape, a, c=None, c0=0.0, eta=0.0, max_iter=2000, tol=1e-6,
     return st.as_dict()
 
 
+def helm_bicg(u, b, shape, a, c=None, c0=0.0, eta=0.0, max_iter=2000, tol=1e-6, precond=True):
+    """BiCG of the complex PETSc variant (PETSC_COMPLEX/main.c:150-175)."""
+    h = _helm(shape, a, c, c0, eta)
+    st = Stats()
+    _bind_stream()
+    check(_L().aar_hip_helm_bicg(ctypes.byref(h), _cplx(u, "u"), _cplx(b, "b"), int(max_iter), float(tol),
+                                 int(bool(precond)), ctypes.byref(st)))
+    return st.as_dict()
+
+
 def helm_grad_c(lam, u):
     """-Re(conj(lambda) u), float64."""
     import torch

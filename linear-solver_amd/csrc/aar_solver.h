@@ -145,6 +145,8 @@ int helm_matvec(const HOp &op, const double *u, double *y);
 int helm_matvec_async(const HOp &op, const double *u, double *y, Ctrl *gate);
 int helm_gmres_solve(const HOp &op, double *u, const double *b, int restart, int max_it, double tol, int precond,
                      aar_hip_stats *st);
+int helm_bicg_solve(const HOp &op, double *u, const double *b, int max_it, double tol, int precond,
+                    aar_hip_stats *st);
 int helm_solve(const HOp &op, double *u, const double *b, double omega, double beta, int m, int p, int max_iter,
                double tol, aar_hip_stats *st);
 template <typename Tx, typename Tg>

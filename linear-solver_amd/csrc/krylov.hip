@@ -1614,6 +1614,8 @@ struct KCtrlZ {
     double2 h[KMAX + 1];
     double2 HH[(KMAX + 2) * (KMAX + 1)];
     double2 cc[KMAX + 1], ss[KMAX + 1], grs[KMAX + 2], nrs[KMAX + 1];
+    double2 beta, betaold, bcoef, alpha;   // BiCG
+    int first, pad2;
 };
 
 __device__ __forceinline__ double2 &ZHe(KCtrlZ *k, int r, int c) { return k->HH[c * (KMAX + 2) + r]; }
@@ -1919,6 +1921,147 @@ __global__ __launch_bounds__(KB) void kz_final_kernel(KCtrlZ *k, const double *p
     k->relres = sqrt(s[0]);
 }
 
+// --- BiCG of the complex variant: shadow sequence with A^H and B^H ---
+__device__ __forceinline__ double2 zpch(const ZDiag &d, long i, double2 x)
+{
+    if (!d.precond) return x;
+    const double dr = d.dlap + (d.c ? d.c[i] : d.c0);
+    const double den = dr * dr + d.eta * d.eta;
+    return zmul(zconj(make_double2(dr / den, -d.eta / den)), x);
+}
+
+// r = b - t, r~ = r, z = B r, z~ = B^H r~; ||z||^2, <z, r~>
+__global__ __launch_bounds__(KB) void kz_bi_init_kernel(long n, const double2 *b, const double2 *t, double2 *r,
+                                                       double2 *rl, double2 *z, double2 *zl, ZDiag dg, double *part)
+{
+    double acc[3] = {0.0, 0.0, 0.0};
+    KLOOP(i)
+    {
+        const double2 ri = make_double2(b[i].x + (-1.0) * t[i].x, b[i].y + (-1.0) * t[i].y);
+        const double2 zi = zpc(dg, i, ri), zli = zpch(dg, i, ri);
+        r[i] = ri;
+        rl[i] = ri;
+        z[i] = zi;
+        zl[i] = zli;
+        const double2 p = zmul(zi, zconj(ri));
+        acc[0] += zi.x * zi.x + zi.y * zi.y;
+        acc[1] += p.x;
+        acc[2] += p.y;
+    }
+    kz_block_sums<3>(acc, 3, part);
+}
+
+__global__ __launch_bounds__(KB) void kz_bi_p_kernel(long n, const double2 *z, const double2 *zl, double2 *p,
+                                                    double2 *pl, const KCtrlZ *k)
+{
+    if (k->c.done) return;
+    if (k->first) {
+        KLOOP(i)
+        {
+            p[i] = z[i];
+            pl[i] = zl[i];
+        }
+    } else {
+        const double2 bc = k->bcoef, bcc = zconj(k->bcoef);
+        KLOOP(i)
+        {
+            p[i] = zadd(z[i], zmul(bc, p[i]));
+            pl[i] = zadd(zl[i], zmul(bcc, pl[i]));
+        }
+    }
+}
+
+// <q, p~> = sum q conj(p~)
+__global__ __launch_bounds__(KB) void kz_bi_dot_kernel(long n, const double2 *q, const double2 *pl, double *part,
+                                                      const KCtrlZ *k)
+{
+    if (k->c.done) return;
+    double acc[2] = {0.0, 0.0};
+    KLOOP(i)
+    {
+        const double2 v = zmul(q[i], zconj(pl[i]));
+        acc[0] += v.x;
+        acc[1] += v.y;
+    }
+    kz_block_sums<2>(acc, 2, part);
+}
+
+// x += a p; r -= a q; r~ -= conj(a) q~; z = B r; z~ = B^H r~; ||z||^2, <z, r~>
+__global__ __launch_bounds__(KB) void kz_bi_upd_kernel(long n, double2 *x, double2 *r, double2 *rl, double2 *z,
+                                                      double2 *zl, const double2 *p, const double2 *q,
+                                                      const double2 *ql, ZDiag dg, double *part, const KCtrlZ *k)
+{
+    if (k->c.done) return;
+    const double2 al = k->alpha, ma = make_double2(-k->alpha.x, -k->alpha.y), mac = zconj(ma);
+    double acc[3] = {0.0, 0.0, 0.0};
+    KLOOP(i)
+    {
+        x[i] = zadd(x[i], zmul(al, p[i]));
+        const double2 ri = zadd(r[i], zmul(ma, q[i]));
+        const double2 rli = zadd(rl[i], zmul(mac, ql[i]));
+        const double2 zi = zpc(dg, i, ri), zli = zpch(dg, i, rli);
+        r[i] = ri;
+        rl[i] = rli;
+        z[i] = zi;
+        zl[i] = zli;
+        const double2 pp = zmul(zi, zconj(rli));
+        acc[0] += zi.x * zi.x + zi.y * zi.y;
+        acc[1] += pp.x;
+        acc[2] += pp.y;
+    }
+    kz_block_sums<3>(acc, 3, part);
+}
+
+__global__ __launch_bounds__(KB) void kz_bi_start_kernel(KCtrlZ *k, const double *part, int nb, double *hist)
+{
+    __shared__ double s[KPZ];
+    kz_reduce(part, nb, 3, s);
+    if (threadIdx.x != 0) return;
+    const double dp = sqrt(s[0]);
+    k->beta = make_double2(s[1], s[2]);
+    k->first = 1;
+    kz_push(k, hist, dp);
+    int reason = kz_converged(k, 0, dp);
+    if (!reason && k->beta.x == 0.0 && k->beta.y == 0.0) reason = K_DIVERGED_BREAKDOWN_BICG;
+    k->reason = reason;
+    if (reason) {
+        k->c.done = 1;
+        k->c.final_iter = k->its;
+    }
+}
+
+__global__ __launch_bounds__(KB) void kz_bi_alpha_kernel(KCtrlZ *k, const double *part, int nb)
+{
+    if (k->c.done) return;
+    __shared__ double s[KPZ];
+    kz_reduce(part, nb, 2, s);
+    if (threadIdx.x != 0) return;
+    k->alpha = zdiv(k->beta, make_double2(s[0], s[1]));
+}
+
+__global__ __launch_bounds__(KB) void kz_bi_step_kernel(KCtrlZ *k, const double *part, int nb, double *hist)
+{
+    if (k->c.done) return;
+    __shared__ double s[KPZ];
+    kz_reduce(part, nb, 3, s);
+    if (threadIdx.x != 0) return;
+    const double dp = sqrt(s[0]);
+    k->its++;
+    kz_push(k, hist, dp);
+    int reason = kz_converged(k, k->its, dp);
+    if (!reason && k->its >= k->max_it) reason = K_DIVERGED_ITS;
+    k->reason = reason;
+    if (reason) {
+        k->c.done = 1;
+        k->c.final_iter = k->its;
+        return;
+    }
+    k->betaold = k->beta;
+    k->beta = make_double2(s[1], s[2]);
+    k->bcoef = zdiv(k->beta, k->betaold);
+    k->first = 0;
+}
+
 namespace {
 template <int NV>
 hipError_t launch_gz_step(const double2 *t, double2 *V, long ld, int nv, long n, int nb, ZDiag dg, KCtrlZ *k,
@@ -2016,6 +2159,109 @@ int helm_gmres_solve(const HOp &op, double *u, const double *b, int restart, int
     // the true residual of the returned u (PETSC_COMPLEX/main.c:110-113)
     if (int rc = helm_matvec_async(op, u, (double *)t, nullptr)) return rc;
     hipLaunchKernelGGL(kz_resnorm_kernel, dim3(nb), dim3(KB), 0, C.stream, n, bz, (const double2 *)t, part);
+    hipLaunchKernelGGL(kz_final_kernel, dim3(1), dim3(KB), 0, C.stream, k, (const double *)part, nb);
+    HC(hipGetLastError());
+    HC(hipEventRecord(C.t1, C.stream));
+    KCtrlZ hk;
+    HC(hipMemcpyAsync(&hk, k, sizeof(KCtrlZ), hipMemcpyDeviceToHost, C.stream));
+    HC(hipStreamSynchronize(C.stream));
+    float ms = 0.f;
+    HC(hipEventElapsedTime(&ms, C.t0, C.t1));
+    prof_flush();
+    const int hl = std::min(hk.nh, hcap);
+    C.hist_host.assign(std::max(hl, 0), 0.0);
+    if (hl > 0) HC(hipMemcpy(C.hist_host.data(), hist, sizeof(double) * hl, hipMemcpyDeviceToHost));
+    aar_hip_stats sres;
+    std::memset(&sres, 0, sizeof(sres));
+    sres.iterations = hk.its;
+    sres.rhs_norm = hk.rhs_norm;
+    sres.relres = hk.rhs_norm > 0 ? hk.relres / hk.rhs_norm : hk.relres;
+    sres.converged = hk.reason > 0;
+    sres.printed_converged = hk.reason > 0;
+    sres.printed_warning = hk.reason == K_DIVERGED_ITS;
+    sres.status = hk.reason == K_DIVERGED_NANORINF ? AAR_HIP_ERR_NONFINITE : 0;
+    sres.hist_len = hl;
+    sres.solve_ms = ms;
+    sres.bytes = C.bytes_acc;
+    sres.reason = hk.reason;
+    C.last = sres;
+    if (st) *st = sres;
+    if (hk.reason == K_DIVERGED_NANORINF) return fail(AAR_HIP_ERR_NONFINITE, "residual is not finite");
+    return 0;
+}
+
+// BiCG of the complex PETSc variant (PETSC_COMPLEX/main.c:150-175)
+int helm_bicg_solve(const HOp &op, double *u, const double *b, int max_it, double tol, int precond,
+                    aar_hip_stats *st)
+{
+    Context &C = ctx();
+    if (int rc = context_init()) return rc;
+    if (!u || !b) return fail(AAR_HIP_ERR_ARG, "null vector");
+    if (max_it < 0) return fail(AAR_HIP_ERR_ARG, "max_iter must be >= 0");
+    if (precond < 0 || precond > 1) return fail(AAR_HIP_ERR_ARG, "precond must be 0 (none) or 1 (Jacobi)");
+    const long n = op.n;
+    const long ld = ring_ld(n, sizeof(double2));
+    HC(ensure(C.kvec, sizeof(double2) * (size_t)ld * 8));
+    double2 *r = (double2 *)C.kvec.p, *rl = r + ld, *z = r + 2 * ld, *zl = r + 3 * ld;
+    double2 *p = r + 4 * ld, *pl = r + 5 * ld, *q = r + 6 * ld, *ql = r + 7 * ld;
+    const int nb = std::min(nblocks_for(n), 1024);
+    const int hcap = max_it + 8;
+    HC(ensure(C.kctl, sizeof(KCtrlZ)));
+    HC(ensure(C.hist0, sizeof(double) * (size_t)hcap));
+    HC(ensure(C.part0, sizeof(double) * (size_t)nb * KPZ));
+    KCtrlZ *k = (KCtrlZ *)C.kctl.p;
+    double *part = (double *)C.part0.p, *hist = (double *)C.hist0.p;
+    ZDiag dg;
+    dg.c = op.c;
+    dg.dlap = op.dlap;
+    dg.c0 = op.c0;
+    dg.eta = op.eta;
+    dg.precond = precond;
+    HOp oph = op;                 // A^H: eta -> -eta (the operator is complex symmetric)
+    oph.eta = -op.eta;
+    const double2 *bz = (const double2 *)b;
+    double2 *x = (double2 *)u;
+    C.bytes_acc = 0.0;
+    HC(hipEventRecord(C.t0, C.stream));
+    hipLaunchKernelGGL(k_zero_kernel, dim3(64), dim3(KB), 0, C.stream, (unsigned *)k,
+                       (long)(sizeof(KCtrlZ) / sizeof(unsigned)));
+    hipLaunchKernelGGL(kz_bnorm_kernel, dim3(nb), dim3(KB), 0, C.stream, n, bz, dg, part);
+    hipLaunchKernelGGL(kz_ctl_init_kernel, dim3(1), dim3(KB), 0, C.stream, k, tol, max_it, 0, hcap,
+                       (const double *)part, nb);
+    HC(hipGetLastError());
+    const double s = 16.0 * n, cb = op.c ? 8.0 * n : 0.0;
+    if (int rc = helm_matvec_async(op, u, (double *)q, nullptr)) return rc;
+    hipLaunchKernelGGL(kz_bi_init_kernel, dim3(nb), dim3(KB), 0, C.stream, n, bz, (const double2 *)q, r, rl, z, zl,
+                       dg, part);
+    hipLaunchKernelGGL(kz_bi_start_kernel, dim3(1), dim3(KB), 0, C.stream, k, (const double *)part, nb, hist);
+    HC(hipGetLastError());
+    C.bytes_acc += 7 * s + 2 * cb;
+    bool stopped = false;
+    const int loops = std::max(max_it, 1);
+    int cyc = 0;
+    for (int i = 0; i < loops && !stopped; i++) {
+        hipLaunchKernelGGL(kz_bi_p_kernel, dim3(nb), dim3(KB), 0, C.stream, n, (const double2 *)z,
+                           (const double2 *)zl, p, pl, (const KCtrlZ *)k);
+        if (int rc = helm_matvec_async(op, (const double *)p, (double *)q, &k->c)) return rc;
+        if (int rc = helm_matvec_async(oph, (const double *)pl, (double *)ql, &k->c)) return rc;
+        prof_begin("krylov_bicg_update");
+        hipLaunchKernelGGL(kz_bi_dot_kernel, dim3(nb), dim3(KB), 0, C.stream, n, (const double2 *)q,
+                           (const double2 *)pl, part, (const KCtrlZ *)k);
+        hipLaunchKernelGGL(kz_bi_alpha_kernel, dim3(1), dim3(KB), 0, C.stream, k, (const double *)part, nb);
+        hipLaunchKernelGGL(kz_bi_upd_kernel, dim3(nb), dim3(KB), 0, C.stream, n, x, r, rl, z, zl,
+                           (const double2 *)p, (const double2 *)q, (const double2 *)ql, dg, part, (const KCtrlZ *)k);
+        hipLaunchKernelGGL(kz_bi_step_kernel, dim3(1), dim3(KB), 0, C.stream, k, (const double *)part, nb, hist);
+        prof_end(16 * s + 2 * cb);
+        HC(hipGetLastError());
+        C.bytes_acc += 22 * s + 4 * cb;
+        if ((i + 1) % 8 == 0) {
+            if (int rc = poll_record(&k->c, cyc)) return rc;
+            if (poll_check(cyc - 2)) stopped = true;
+            cyc++;
+        }
+    }
+    if (int rc = helm_matvec_async(op, u, (double *)q, nullptr)) return rc;
+    hipLaunchKernelGGL(kz_resnorm_kernel, dim3(nb), dim3(KB), 0, C.stream, n, bz, (const double2 *)q, part);
     hipLaunchKernelGGL(kz_final_kernel, dim3(1), dim3(KB), 0, C.stream, k, (const double *)part, nb);
     HC(hipGetLastError());
     HC(hipEventRecord(C.t1, C.stream));

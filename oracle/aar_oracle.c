@@ -1706,3 +1706,128 @@ int orc_gmres_complex(int nx, int ny, int nz, int fdn, double a, const double *c
     free(dinv); free(V); free(t);
     return 0;
 }
+
+/* BiCG of the complex PETSc variant (PETSC_COMPLEX/main.c:150-175,
+ * KSPBICG with PCJACOBI): the shadow sequence of KSPBICG's complex build,
+ * z~ = B^H r~ and A^H p~ (VecConjugate around the transposed products);
+ * the Helmholtz operator is complex symmetric, so A^H is the operator
+ * with eta -> -eta and B^H = conj(1/D).  <x, y> = sum x conj(y). */
+int orc_bicg_complex(int nx, int ny, int nz, int fdn, double a, const double *c, double c0, double eta,
+                     double *u, const double *rhs, int max_it, double rtol, int precond,
+                     double *trace, int trace_cap, orc_stats *st)
+{
+    const long n = (long)nx * ny * nz;
+    double coeff[32] = {0};
+    orc_lap_coefficient(coeff, fdn);
+    const double dlap = a * (3 * coeff[0]);
+    zc *x = (zc *)u;
+    const zc *b = (const zc *)rhs;
+    zc *dinv = (zc *)malloc(sizeof(zc) * n);
+    zc *Rr = (zc *)malloc(sizeof(zc) * n), *Rl = (zc *)malloc(sizeof(zc) * n);
+    zc *Zr = (zc *)malloc(sizeof(zc) * n), *Zl = (zc *)malloc(sizeof(zc) * n);
+    zc *Pr = (zc *)malloc(sizeof(zc) * n), *Pl = (zc *)malloc(sizeof(zc) * n);
+    for (long i = 0; i < n; i++) {
+        if (precond) {
+            const double dr = dlap + (c ? c[i] : c0);
+            const double den = dr * dr + eta * eta;
+            dinv[i].re = dr / den;
+            dinv[i].im = -eta / den;
+        } else {
+            dinv[i].re = 1.0;
+            dinv[i].im = 0.0;
+        }
+    }
+#define ZPC(i, v) (precond ? zmul(dinv[i], (v)) : (v))
+#define ZPCH(i, v) (precond ? zmul(zconj(dinv[i]), (v)) : (v))
+    int tlen = 0;
+    ksp_conv cv_ = {0.0, 0.0, rtol, 0.0};
+    double s2 = 0.0;
+    for (long i = 0; i < n; i++) {
+        const zc z = ZPC(i, b[i]);
+        s2 = s2 + (z.re * z.re + z.im * z.im);
+    }
+    cv_.bpnorm = sqrt(s2);
+    double t0 = now_seconds();
+    orc_matvec_complex(nx, ny, nz, fdn, coeff, a, c, c0, eta, (const double *)x, (double *)Rr);
+    s2 = 0.0;
+    for (long i = 0; i < n; i++) {
+        Rr[i].re = b[i].re + (-1.0) * Rr[i].re;
+        Rr[i].im = b[i].im + (-1.0) * Rr[i].im;
+        Rl[i] = Rr[i];
+        Zr[i] = ZPC(i, Rr[i]);
+        Zl[i] = ZPCH(i, Rl[i]);
+        s2 = s2 + (Zr[i].re * Zr[i].re + Zr[i].im * Zr[i].im);
+    }
+    double dp = sqrt(s2);
+    if (trace && tlen < trace_cap) trace[tlen++] = dp;
+    int its = 0;
+    int reason = ksp_converged(&cv_, 0, dp);
+    zc beta, betaold = {0.0, 0.0};
+    int i = 0;
+    if (!reason) {
+        do {
+            zc sb = {0.0, 0.0};
+            for (long k = 0; k < n; k++) sb = zadd(sb, zmul(Zr[k], zconj(Rl[k])));
+            beta = sb;
+            if (!i) {
+                if (beta.re == 0.0 && beta.im == 0.0) {
+                    reason = ORC_KSP_DIVERGED_BREAKDOWN_BICG;
+                    break;
+                }
+                memcpy(Pr, Zr, sizeof(zc) * n);
+                memcpy(Pl, Zl, sizeof(zc) * n);
+            } else {
+                const zc bb = zdiv(beta, betaold), bbc = zconj(bb);
+                for (long k = 0; k < n; k++) {
+                    Pr[k] = zadd(Zr[k], zmul(bb, Pr[k]));
+                    Pl[k] = zadd(Zl[k], zmul(bbc, Pl[k]));
+                }
+            }
+            betaold = beta;
+            orc_matvec_complex(nx, ny, nz, fdn, coeff, a, c, c0, eta, (const double *)Pr, (double *)Zr);
+            orc_matvec_complex(nx, ny, nz, fdn, coeff, a, c, c0, -eta, (const double *)Pl, (double *)Zl);
+            zc dpi = {0.0, 0.0};
+            for (long k = 0; k < n; k++) dpi = zadd(dpi, zmul(Zr[k], zconj(Pl[k])));
+            const zc al = zdiv(beta, dpi), ma = {-al.re, -al.im}, mac = zconj(ma);
+            s2 = 0.0;
+            for (long k = 0; k < n; k++) {
+                x[k] = zadd(x[k], zmul(al, Pr[k]));
+                Rr[k] = zadd(Rr[k], zmul(ma, Zr[k]));
+                Rl[k] = zadd(Rl[k], zmul(mac, Zl[k]));
+                Zr[k] = ZPC(k, Rr[k]);
+                Zl[k] = ZPCH(k, Rl[k]);
+                s2 = s2 + (Zr[k].re * Zr[k].re + Zr[k].im * Zr[k].im);
+            }
+            dp = sqrt(s2);
+            its = i + 1;
+            if (trace && tlen < trace_cap) trace[tlen++] = dp;
+            reason = ksp_converged(&cv_, i + 1, dp);
+            if (reason) break;
+            i++;
+        } while (i < max_it);
+        if (i >= max_it && !reason) reason = ORC_KSP_DIVERGED_ITS;
+    }
+#undef ZPC
+#undef ZPCH
+    double t1 = now_seconds();
+    if (st) {
+        orc_matvec_complex(nx, ny, nz, fdn, coeff, a, c, c0, eta, (const double *)x, (double *)Zr);
+        double rn = 0.0, bn = 0.0;
+        for (long k = 0; k < n; k++) {
+            const zc r = zsub(b[k], Zr[k]);
+            rn = rn + (r.re * r.re + r.im * r.im);
+            bn = bn + (b[k].re * b[k].re + b[k].im * b[k].im);
+        }
+        st->iterations = its;
+        st->relres = sqrt(rn) / sqrt(bn);
+        st->rhs_norm = sqrt(bn);
+        st->converged = reason > 0;
+        st->printed_converged = reason > 0;
+        st->printed_warning = reason == ORC_KSP_DIVERGED_ITS;
+        st->trace_len = tlen;
+        st->seconds = t1 - t0;
+        st->status = reason;
+    }
+    free(dinv); free(Rr); free(Rl); free(Zr); free(Zl); free(Pr); free(Pl);
+    return 0;
+}

@@ -186,6 +186,11 @@ int orc_pgr_petsc(int nx, int ny, int nz, int fdn, double a, double sigma, const
 int orc_gmres_complex(int nx, int ny, int nz, int fdn, double a, const double *c, double c0, double eta,
                       double *u, const double *rhs, int restart, int max_it, double rtol, int precond,
                       double *trace, int trace_cap, orc_stats *st);
+/* BiCG of the complex PETSc variant (PETSC_COMPLEX/main.c:150-175); the
+ * shadow sequence with A^H (eta -> -eta) and B^H = conj(1/D). */
+int orc_bicg_complex(int nx, int ny, int nz, int fdn, double a, const double *c, double c0, double eta,
+                     double *u, const double *rhs, int max_it, double rtol, int precond,
+                     double *trace, int trace_cap, orc_stats *st);
 int orc_gmres(int nx, int ny, int nz, int fdn, double a, double sigma, const double *c,
               double *x, const double *b, int restart, int max_it, double rtol, int precond,
               double *trace, int trace_cap, orc_stats *st);

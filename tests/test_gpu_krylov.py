@@ -448,3 +448,26 @@ def test_helm_gmres_parity(gpu, oracle, case):
     np.testing.assert_allclose(h, d["trace"], rtol=1e-6, atol=1e-8 * d["trace"][0])
     ug = ug.cpu().numpy()
     assert np.abs(ug - uo).max() <= 1e-8 * np.abs(uo).max()
+
+
+@pytest.mark.parametrize("case", [c for c in HZ_CASES if c[4] == 30], ids=_hid)
+def test_helm_bicg_parity(gpu, oracle, case):
+    """BiCG of the complex variant: counts, reasons, norms (1e-6 relative or
+    1e-8 of the first) and the solution (1e-8) equal to the oracle's."""
+    shape, vc, c0, eta, _restart, max_it, tol, pc = case
+    say("helm bicg", _hid(case))
+    n = int(np.prod(shape))
+    a = P.poisson_a(0.5) if shape[0] != 36 else -1.0 / (4 * np.pi)
+    c = P.helmholtz_coefficient(shape, 7) if vc else None
+    b = P.rhs_complex_normal(shape, 1005)
+    u0 = np.ones(n, dtype=complex)
+    uo, d = oracle.bicg_complex(u0, b, shape, a, c, c0, eta, max_it, tol, precond=pc)
+    ug = cdev(u0)
+    st = gpu.helm_bicg(ug, cdev(b), shape, a, c=None if c is None else dev(c), c0=c0, eta=eta, max_iter=max_it,
+                       tol=tol, precond=pc)
+    assert st["iterations"] == d["iterations"] and st["reason"] == d["status"]
+    h = gpu.history()
+    assert len(h) == len(d["trace"])
+    np.testing.assert_allclose(h, d["trace"], rtol=1e-6, atol=1e-8 * d["trace"][0])
+    ug = ug.cpu().numpy()
+    assert np.abs(ug - uo).max() <= 1e-8 * np.abs(uo).max()

@@ -497,3 +497,28 @@ def test_gmres_complex_reduces_to_real_and_solves(oracle):
     ref = np.fft.ifftn(np.fft.fftn(bc.reshape(nz, ny, nx)) / lam).reshape(-1)
     assert d["status"] == oracle.KSP_CONVERGED_RTOL
     assert np.abs(u - ref).max() <= 1e-7 * np.abs(ref).max()
+
+
+def test_bicg_complex_reduces_to_real_and_solves(oracle):
+    shape = (12, 12, 12)
+    n = int(np.prod(shape))
+    a = -4.0
+    b = rhs(shape, 56)
+    xr, dr = oracle.bicg(np.ones(n), b, shape, a, 500, 1e-8, sigma=0.7)
+    xc, dc = oracle.bicg_complex(np.ones(n, dtype=complex), b.astype(complex), shape, a, None, 0.7, 0.0, 500, 1e-8)
+    assert dc["iterations"] == dr["iterations"]
+    np.testing.assert_allclose(dc["trace"], dr["trace"], rtol=1e-9, atol=1e-11 * dr["trace"][0])
+    assert np.abs(xc.imag).max() == 0.0 and np.abs(xc.real - xr).max() <= 1e-10 * np.abs(xr).max()
+    eta, c0 = 0.5, 0.3
+    bc = np.random.default_rng(57).standard_normal(n) + 1j * np.random.default_rng(58).standard_normal(n)
+    u, d = oracle.bicg_complex(np.ones(n, dtype=complex), bc, shape, a, None, c0, eta, 2000, 1e-10)
+    ug, dg = oracle.gmres_complex(np.ones(n, dtype=complex), bc, shape, a, None, c0, eta, 2000, 1e-10)
+    cf = oracle.lap_coefficient(6)
+    nx, ny, nz = shape
+    def s1(m):
+        th = 2 * np.pi * np.arange(m) / m
+        return cf[0] + 2 * sum(cf[q] * np.cos(q * th) for q in range(1, 7))
+    lam = a * (s1(nz)[:, None, None] + s1(ny)[None, :, None] + s1(nx)[None, None, :]) + c0 + 1j * eta
+    ref = np.fft.ifftn(np.fft.fftn(bc.reshape(nz, ny, nx)) / lam).reshape(-1)
+    assert d["status"] == oracle.KSP_CONVERGED_RTOL
+    assert np.abs(u - ref).max() <= 1e-7 * np.abs(ref).max()
