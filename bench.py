@@ -159,7 +159,18 @@ def helmholtz_case(n, dev, a, directions=20, hstep=1e-6, tol=1e-13):
             ad = float(torch.sum(gc * dc))
             ec.append(abs(fd - ad) / abs(ad))
     f, adj = last["forward"], last["adjoint"]
+    # the complex PETSc variant's Krylov comparators on the same system
+    # (PCJACOBI, rtol 1e-6, max_it 2000 as PETSC_COMPLEX/main.c:96,127,155)
+    # against AAR to the same tolerance
+    kry = {}
+    for name, fn in (("aar", lambda u: aar_hip.helm_aar(u, b, shape, a, c, 0.0, eta, **dict(opts, tol=1e-6))),
+                     ("gmres30", lambda u: aar_hip.helm_gmres(u, b, shape, a, c, 0.0, eta, 2000, 1e-6)),
+                     ("bicg", lambda u: aar_hip.helm_bicg(u, b, shape, a, c, 0.0, eta, 2000, 1e-6))):
+        u = torch.zeros_like(b)
+        st = fn(u)
+        kry[name] = {"iterations": st["iterations"], "solve_ms": st["solve_ms"], "relres": st["relres"]}
     return {"grid": [n, n, n], "h": 0.25, "eta": eta, "tol": opts["tol"],
+            "rtol_1e-6_solvers": kry,
             "forward_iterations": f["iterations"], "forward_ms": f["solve_ms"],
             "adjoint_iterations": adj["iterations"], "adjoint_ms": adj["solve_ms"],
             "solves_per_s": 1e3 / f["solve_ms"], "forward_backward_s": t_fb,
