@@ -228,6 +228,14 @@ int aar_hip_helm_gmres(const aar_hip_helm *h, double *u, const double *b, int re
  * is complex symmetric) and B^H = conj(1/D). */
 int aar_hip_helm_bicg(const aar_hip_helm *h, double *u, const double *b, int max_iter, double tol, int precond,
                       aar_hip_stats *st);
+/* precond = 2 for aar_hip_helm_gmres / aar_hip_helm_bicg: the complex
+ * variant's pc = 1, PCBJACOBI -> PCILU at 0 levels on one block = ILU(0)
+ * of the complex matrix (every grid side >= 13); BiCG's shadow then uses
+ * M^{-H} = conj(M^{-T} conj(r)).  aar_hip_helm_ilu0_apply: that
+ * preconditioner alone, z = M^{-1} r (trans 0), M^{-T} r (1), M^{-H} r (2);
+ * complex128 interleaved device arrays, bit-identical to a sequential
+ * restatement. */
+int aar_hip_helm_ilu0_apply(const aar_hip_helm *h, const double *r, double *z, int trans);
 
 /* Gradient of a loss through u = A(c)^{-1} b with respect to the diagonal
  * coefficient field c, given the adjoint solution lambda of A^H lambda =

@@ -5,4 +5,5 @@ from .solvers import (AAR, CG, GMRES, ILU0, PGR, PL2R, AAR_batch, AAR_petsc, BiC
                       Lap_Vec_mult_vc, autotune, history, last_system, pinv, profile, profile_query, profile_reset,
                       release, set_coefficient, set_option, set_tuning)
 from . import problems  # noqa: F401
-from .helmholtz import HelmholtzSolve, helm_aar, helm_bicg, helm_gmres, helm_grad_c, helm_matvec  # noqa: F401
+from .helmholtz import (HelmholtzSolve, helm_aar, helm_bicg, helm_gmres, helm_grad_c, helm_ilu0_apply,  # noqa: F401
+                        helm_matvec)

@@ -83,6 +83,22 @@ def helm_aar(u, b, shape, a, c=None, c0=0.0, eta=0.0, omega=0.6, beta=0.6, m=7, 
     return st.as_dict()
 
 
+def _pcz(precond):
+    """precond: False/0 none, True/1 Jacobi, 2 ILU(0) (the complex variant's pc = 1)."""
+    return 2 if precond == 2 and not isinstance(precond, bool) else int(bool(precond))
+
+
+def helm_ilu0_apply(r, shape, a, c=None, c0=0.0, eta=0.0, trans=0):
+    """z = M^{-1} r (trans 0), M^{-T} r (1) or M^{-H} r (2) for the complex
+    ILU(0) factors of the Helmholtz matrix."""
+    import torch
+    h = _helm(shape, a, c, c0, eta)
+    _bind_stream()
+    z = torch.empty_like(r)
+    check(_L().aar_hip_helm_ilu0_apply(ctypes.byref(h), _cplx(r, "r"), _cplx(z, "z"), int(trans)))
+    return z
+
+
 def helm_gmres(u, b, shape, a, c=None, c0=0.0, eta=0.0, max_iter=2000, tol=1e-6, restart=30, precond=True):
     """GMRES of the complex PETSc variant (PETSC_COMPLEX/main.c:80-118) on
     A u = b: u is the initial guess, overwritten; 'reason' in the stats is
@@ -91,7 +107,7 @@ def helm_gmres(u, b, shape, a, c=None, c0=0.0, eta=0.0, max_iter=2000, tol=1e-6,
     st = Stats()
     _bind_stream()
     check(_L().aar_hip_helm_gmres(ctypes.byref(h), _cplx(u, "u"), _cplx(b, "b"), int(restart), int(max_iter),
-                                  float(tol), int(bool(precond)), ctypes.byref(st)))
+                                  float(tol), _pcz(precond), ctypes.byref(st)))
     return st.as_dict()
 
 
@@ -101,7 +117,7 @@ def helm_bicg(u, b, shape, a, c=None, c0=0.0, eta=0.0, max_iter=2000, tol=1e-6, 
     st = Stats()
     _bind_stream()
     check(_L().aar_hip_helm_bicg(ctypes.byref(h), _cplx(u, "u"), _cplx(b, "b"), int(max_iter), float(tol),
-                                 int(bool(precond)), ctypes.byref(st)))
+                                 _pcz(precond), ctypes.byref(st)))
     return st.as_dict()
 
 

@@ -147,6 +147,7 @@ int helm_gmres_solve(const HOp &op, double *u, const double *b, int restart, int
                      aar_hip_stats *st);
 int helm_bicg_solve(const HOp &op, double *u, const double *b, int max_it, double tol, int precond,
                     aar_hip_stats *st);
+int helm_ilu0_apply(const HOp &op, const double *r, double *z, int trans);
 int helm_solve(const HOp &op, double *u, const double *b, double omega, double beta, int m, int p, int max_iter,
                double tol, aar_hip_stats *st);
 template <typename Tx, typename Tg>

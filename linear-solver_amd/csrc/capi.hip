@@ -940,6 +940,13 @@ int aar_hip_helm_bicg(const aar_hip_helm *h, double *u, const double *b, int max
     return helm_bicg_solve(op, u, b, max_iter, tol, precond, st);
 }
 
+int aar_hip_helm_ilu0_apply(const aar_hip_helm *h, const double *r, double *z, int trans)
+{
+    HOp op;
+    if (int rc = make_hop(h, op)) return rc;
+    return helm_ilu0_apply(op, r, z, trans);
+}
+
 int aar_hip_helm_grad_c(long n, const double *lambda, const double *u, double *dc)
 {
     if (int rc = context_init()) return rc;

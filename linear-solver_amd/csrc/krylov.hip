@@ -772,18 +772,41 @@ constexpr int IFS = 40;
 
 __host__ __device__ constexpr int islot(int ax, int d) { return 1 + 12 * ax + (d < 0 ? d + 6 : d + 5); }
 
-struct IluArgs {
+template <typename T>
+struct IluArgsT {
     int nx, ny, nz;
     int level, x0, y0, wx, wy;   // the level and its (x, y) box
     double w[R + 1];             // a * coeff[o]
-    double diag;                 // a * 3 * coeff[0] (+ sigma)
+    double diag;                 // real part of the diagonal without c(x): a * 3 * coeff[0] (+ sigma, + c0)
+    double eta;                  // its imaginary part (complex matrix)
     const double *c;             // + c(x) on the diagonal, or null
-    double *F;
+    T *F;
     const int *skip;             // nonzero: no work (solver stopped / step gated)
 };
+using IluArgs = IluArgsT<double>;
+
+// scalar operations of the real and complex factorizations (complex:
+// explicit formulas, as the oracle's)
+__device__ __forceinline__ double s_mul(double a, double b) { return a * b; }
+__device__ __forceinline__ double2 s_mul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double s_sub(double a, double b) { return a - b; }
+__device__ __forceinline__ double2 s_sub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double s_inv(double a) { return 1.0 / a; }
+__device__ __forceinline__ double2 s_inv(double2 b)
+{
+    const double den = b.x * b.x + b.y * b.y;
+    return make_double2((1.0 * b.x + 0.0 * b.y) / den, (0.0 * b.x - 1.0 * b.y) / den);
+}
+template <typename T> __device__ __forceinline__ T s_make(double re, double im);
+template <> __device__ __forceinline__ double s_make<double>(double re, double) { return re; }
+template <> __device__ __forceinline__ double2 s_make<double2>(double re, double im) { return make_double2(re, im); }
 
 // the point of this thread on the level, or false
-__device__ __forceinline__ bool ilu_point(const IluArgs &a, int &x, int &y, int &z)
+template <typename T>
+__device__ __forceinline__ bool ilu_point(const IluArgsT<T> &a, int &x, int &y, int &z)
 {
     const int t = blockIdx.x * KB + threadIdx.x;
     if (t >= a.wx * a.wy) return false;
@@ -798,21 +821,22 @@ __device__ __forceinline__ bool ilu_point(const IluArgs &a, int &x, int &y, int 
 // -> d = -6..-1 (below unless wrapped)
 __device__ __forceinline__ constexpr int qoff(int q) { return q < 6 ? q + 1 : q - 12; }
 
-__global__ __launch_bounds__(KB) void ilu_factor_kernel(IluArgs a)
+template <typename T>
+__global__ __launch_bounds__(KB) void ilu_factor_kernel(IluArgsT<T> a)
 {
     int x, y, z;
     if (!ilu_point(a, x, y, z)) return;
     const int cr[3] = {x, y, z}, dims[3] = {a.nx, a.ny, a.nz};
     const long strides[3] = {1, a.nx, (long)a.nx * a.ny};
     const long i = x + (long)y * a.nx + (long)z * a.nx * a.ny;
-    double v[37];
-    v[0] = a.c ? a.diag + a.c[i] : a.diag;
+    T v[37];
+    v[0] = s_make<T>(a.c ? a.diag + a.c[i] : a.diag, a.eta);
 #pragma unroll
     for (int ax = 0; ax < 3; ax++)
 #pragma unroll
         for (int d = 1; d <= R; d++) {
-            v[islot(ax, -d)] = a.w[d];
-            v[islot(ax, d)] = a.w[d];
+            v[islot(ax, -d)] = s_make<T>(a.w[d], 0.0);
+            v[islot(ax, d)] = s_make<T>(a.w[d], 0.0);
         }
 #pragma unroll
     for (int ax = 2; ax >= 0; ax--) {            // z, y, x: ascending column index
@@ -824,8 +848,8 @@ __global__ __launch_bounds__(KB) void ilu_factor_kernel(IluArgs a)
             const bool lower = dk > 0 ? (c + dk >= n) : (c + dk >= 0);
             if (!lower) continue;
             const int ck = dk > 0 ? c + dk - n : c + dk;
-            const double *Fk = a.F + (size_t)(i + (long)(ck - c) * st) * IFS;
-            const double l = v[islot(ax, dk)] * Fk[0];
+            const T *Fk = a.F + (size_t)(i + (long)(ck - c) * st) * IFS;
+            const T l = s_mul(v[islot(ax, dk)], Fk[0]);
             v[islot(ax, dk)] = l;
             // row k's columns above k that are in row i's pattern: on this axis
 #pragma unroll
@@ -839,12 +863,12 @@ __global__ __launch_bounds__(KB) void ilu_factor_kernel(IluArgs a)
                 else if (dkj < -R) dkj += n;
                 if (dkj < -R || dkj > R) continue;
                 const int si = dij == 0 ? 0 : islot(ax, dij);
-                v[si] = v[si] - l * Fk[islot(ax, dkj)];
+                v[si] = s_sub(v[si], s_mul(l, Fk[islot(ax, dkj)]));
             }
         }
     }
-    double *Fi = a.F + (size_t)i * IFS;
-    Fi[0] = 1.0 / v[0];
+    T *Fi = a.F + (size_t)i * IFS;
+    Fi[0] = s_inv(v[0]);
 #pragma unroll
     for (int s = 1; s < 37; s++) Fi[s] = v[s];
 }
@@ -853,8 +877,8 @@ __global__ __launch_bounds__(KB) void ilu_factor_kernel(IluArgs a)
 // at the first solve of an apply).  lowerpass: the columns below the row
 // (forward levels); trans: the transposed factor (entries of the other
 // row); diagmul: multiply by 1/u_ii at the end.
-template <bool LOWER, bool TRANS, bool DIAG>
-__global__ __launch_bounds__(KB) void ilu_solve_kernel(IluArgs a, const double *in, double *io)
+template <typename T, bool LOWER, bool TRANS, bool DIAG>
+__global__ __launch_bounds__(KB) void ilu_solve_kernel(IluArgsT<T> a, const T *in, T *io)
 {
     if (*a.skip) return;
     int x, y, z;
@@ -862,8 +886,8 @@ __global__ __launch_bounds__(KB) void ilu_solve_kernel(IluArgs a, const double *
     const int cr[3] = {x, y, z}, dims[3] = {a.nx, a.ny, a.nz};
     const long strides[3] = {1, a.nx, (long)a.nx * a.ny};
     const long i = x + (long)y * a.nx + (long)z * a.nx * a.ny;
-    const double *Fi = a.F + (size_t)i * IFS;
-    double s = in[i];
+    const T *Fi = a.F + (size_t)i * IFS;
+    T s = in[i];
 #pragma unroll
     for (int t = 0; t < 3; t++) {
         const int ax = LOWER ? 2 - t : t;        // ascending column index
@@ -876,11 +900,11 @@ __global__ __launch_bounds__(KB) void ilu_solve_kernel(IluArgs a, const double *
             if (below != LOWER) continue;
             const int cj = d > 0 ? (below ? c + d - n : c + d) : (below ? c + d : c + d + n);
             const long j = i + (long)(cj - c) * st;
-            const double f = TRANS ? a.F[(size_t)j * IFS + islot(ax, -d)] : Fi[islot(ax, d)];
-            s = s - f * io[j];
+            const T f = TRANS ? a.F[(size_t)j * IFS + islot(ax, -d)] : Fi[islot(ax, d)];
+            s = s_sub(s, s_mul(f, io[j]));
         }
     }
-    io[i] = DIAG ? s * Fi[0] : s;
+    io[i] = DIAG ? s_mul(s, Fi[0]) : s;
 }
 
 // ---------------------------------------------------------------------------
@@ -905,7 +929,8 @@ int check_ilu_grid(const Op &op)
 }
 
 // the (x, y) box of level L: x + y + z = L inside the grid
-void ilu_box(IluArgs &a, int L)
+template <typename T>
+void ilu_box(IluArgsT<T> &a, int L)
 {
     const int x0 = std::max(0, L - (a.ny - 1) - (a.nz - 1)), x1 = std::min(a.nx - 1, L);
     const int y0 = std::max(0, L - (a.nx - 1) - (a.nz - 1)), y1 = std::min(a.ny - 1, L);
@@ -919,6 +944,7 @@ void ilu_box(IluArgs &a, int L)
 IluArgs ilu_args(const Op &op, double *F)
 {
     IluArgs a{};
+    a.eta = 0.0;
     a.nx = op.nx;
     a.ny = op.ny;
     a.nz = op.nz;
@@ -929,34 +955,37 @@ IluArgs ilu_args(const Op &op, double *F)
     return a;
 }
 
-int ilu_nlev(const IluArgs &a) { return a.nx + a.ny + a.nz - 2; }
+template <typename T>
+int ilu_nlev(const IluArgsT<T> &a) { return a.nx + a.ny + a.nz - 2; }
 
-hipError_t ilu_factor(IluArgs a, hipStream_t st)
+template <typename T>
+hipError_t ilu_factor(IluArgsT<T> a, hipStream_t st)
 {
     for (int L = 0; L < ilu_nlev(a); L++) {
         ilu_box(a, L);
-        hipLaunchKernelGGL(ilu_factor_kernel, dim3((a.wx * a.wy + KB - 1) / KB), dim3(KB), 0, st, a);
+        hipLaunchKernelGGL((ilu_factor_kernel<T>), dim3((a.wx * a.wy + KB - 1) / KB), dim3(KB), 0, st, a);
     }
     return hipGetLastError();
 }
 
 // out = M^{-1} in (trans: M^{-T} in); out may be in
-hipError_t ilu_apply(IluArgs a, const double *in, double *out, bool trans, const int *skip, hipStream_t st)
+template <typename T>
+hipError_t ilu_apply(IluArgsT<T> a, const T *in, T *out, bool trans, const int *skip, hipStream_t st)
 {
     a.skip = skip;
     const int nl = ilu_nlev(a);
     for (int L = 0; L < nl; L++) {
         ilu_box(a, L);
         const dim3 g((a.wx * a.wy + KB - 1) / KB);
-        if (trans) hipLaunchKernelGGL((ilu_solve_kernel<true, true, true>), g, dim3(KB), 0, st, a, in, out);
-        else hipLaunchKernelGGL((ilu_solve_kernel<true, false, false>), g, dim3(KB), 0, st, a, in, out);
+        if (trans) hipLaunchKernelGGL((ilu_solve_kernel<T, true, true, true>), g, dim3(KB), 0, st, a, in, out);
+        else hipLaunchKernelGGL((ilu_solve_kernel<T, true, false, false>), g, dim3(KB), 0, st, a, in, out);
     }
     for (int L = nl - 1; L >= 0; L--) {
         ilu_box(a, L);
         const dim3 g((a.wx * a.wy + KB - 1) / KB);
-        if (trans) hipLaunchKernelGGL((ilu_solve_kernel<false, true, false>), g, dim3(KB), 0, st, a,
-                                      (const double *)out, out);
-        else hipLaunchKernelGGL((ilu_solve_kernel<false, false, true>), g, dim3(KB), 0, st, a, (const double *)out,
+        if (trans) hipLaunchKernelGGL((ilu_solve_kernel<T, false, true, false>), g, dim3(KB), 0, st, a,
+                                      (const T *)out, out);
+        else hipLaunchKernelGGL((ilu_solve_kernel<T, false, false, true>), g, dim3(KB), 0, st, a, (const T *)out,
                                 out);
     }
     return hipGetLastError();
@@ -1609,7 +1638,7 @@ struct KCtrlZ {
     Ctrl c;
     Ctrl gate;
     int reason, its, it, open, noscale, nosol, nh, hcap;
-    int max_it, restart, pad0, pad1;
+    int max_it, restart, zero, pad1;   // zero: never set (a skip flag that never skips)
     double rtol, ttol, rnorm0, bpnorm, rhs_norm, scale, relres;
     double2 h[KMAX + 1];
     double2 HH[(KMAX + 2) * (KMAX + 1)];
@@ -1921,6 +1950,50 @@ __global__ __launch_bounds__(KB) void kz_final_kernel(KCtrlZ *k, const double *p
     k->relres = sqrt(s[0]);
 }
 
+// --- ILU(0) of the complex matrix (the complex variant's pc = 1) ---
+__global__ __launch_bounds__(KB) void kz_norm_kernel(long n, const double2 *v, double *part, const int *skip)
+{
+    if (*skip) return;
+    double acc[1] = {0.0};
+    KLOOP(i) acc[0] += v[i].x * v[i].x + v[i].y * v[i].y;
+    kz_block_sums<1>(acc, 1, part);
+}
+
+// ||B b||^2 and ||b||^2 with B b given
+__global__ __launch_bounds__(KB) void kz_bnorm2_kernel(long n, const double2 *b, const double2 *zb, double *part)
+{
+    double acc[2] = {0.0, 0.0};
+    KLOOP(i)
+    {
+        acc[0] += zb[i].x * zb[i].x + zb[i].y * zb[i].y;
+        acc[1] += b[i].x * b[i].x + b[i].y * b[i].y;
+    }
+    kz_block_sums<2>(acc, 2, part);
+}
+
+__global__ __launch_bounds__(KB) void kz_conj_kernel(long n, double2 *v, const int *skip)
+{
+    if (*skip) return;
+    KLOOP(i) v[i].y = -v[i].y;
+}
+
+// ||z||^2, <z, r~>
+__global__ __launch_bounds__(KB) void kz_bi_norms_kernel(long n, const double2 *z, const double2 *rl, double *part,
+                                                        const int *skip)
+{
+    if (*skip) return;
+    double acc[3] = {0.0, 0.0, 0.0};
+    KLOOP(i)
+    {
+        const double2 zi = z[i];
+        const double2 pp = zmul(zi, zconj(rl[i]));
+        acc[0] += zi.x * zi.x + zi.y * zi.y;
+        acc[1] += pp.x;
+        acc[2] += pp.y;
+    }
+    kz_block_sums<3>(acc, 3, part);
+}
+
 // --- BiCG of the complex variant: shadow sequence with A^H and B^H ---
 __device__ __forceinline__ double2 zpch(const ZDiag &d, long i, double2 x)
 {
@@ -2082,6 +2155,49 @@ hipError_t launch_gz_soln(double2 *x, const double2 *V, long ld, long n, int nb,
 }
 }  // namespace
 
+static IluArgsT<double2> helm_ilu_args(const HOp &op, double2 *F)
+{
+    IluArgsT<double2> a{};
+    a.nx = op.nx;
+    a.ny = op.ny;
+    a.nz = op.nz;
+    for (int o = 0; o <= R; o++) a.w[o] = op.w[o];
+    a.diag = op.c ? op.dlap : op.dlap + op.c0;   // (a*3*coeff[0] + c) + i eta
+    a.eta = op.eta;
+    a.c = op.c;
+    a.F = F;
+    return a;
+}
+
+// z = M^{-1} r (trans 0), M^{-T} r (1) or M^{-H} r (2) for the complex
+// ILU(0) factors of op, factored here
+int helm_ilu0_apply(const HOp &op, const double *r, double *z, int trans)
+{
+    Context &C = ctx();
+    if (int rc = context_init()) return rc;
+    if (!r || !z) return fail(AAR_HIP_ERR_ARG, "null vector");
+    if (trans < 0 || trans > 2) return fail(AAR_HIP_ERR_ARG, "trans must be 0, 1 or 2");
+    if (op.nx < 13 || op.ny < 13 || op.nz < 13)
+        return fail(AAR_HIP_ERR_ARG, "the ILU(0) preconditioner needs every grid side >= 13");
+    const long n = op.n;
+    HC(ensure(C.kilu, sizeof(double2) * (size_t)n * IFS));
+    HC(ensure(C.kctl, sizeof(KCtrlZ)));
+    KCtrlZ *k = (KCtrlZ *)C.kctl.p;
+    hipLaunchKernelGGL(k_zero_kernel, dim3(64), dim3(KB), 0, C.stream, (unsigned *)k,
+                       (long)(sizeof(KCtrlZ) / sizeof(unsigned)));
+    IluArgsT<double2> a = helm_ilu_args(op, (double2 *)C.kilu.p);
+    HC(ilu_factor(a, C.stream));
+    double2 *zz = (double2 *)z;
+    const int nb = std::min(nblocks_for(n), 1024);
+    HC(hipMemcpyAsync(zz, r, sizeof(double2) * n, hipMemcpyDeviceToDevice, C.stream));
+    if (trans == 2) hipLaunchKernelGGL(kz_conj_kernel, dim3(nb), dim3(KB), 0, C.stream, n, zz, (const int *)&k->zero);
+    HC(ilu_apply(a, (const double2 *)zz, zz, trans != 0, &k->zero, C.stream));
+    if (trans == 2) hipLaunchKernelGGL(kz_conj_kernel, dim3(nb), dim3(KB), 0, C.stream, n, zz, (const int *)&k->zero);
+    HC(hipGetLastError());
+    HC(hipStreamSynchronize(C.stream));
+    return 0;
+}
+
 int helm_gmres_solve(const HOp &op, double *u, const double *b, int restart, int max_it, double tol, int precond,
                      aar_hip_stats *st)
 {
@@ -2090,7 +2206,10 @@ int helm_gmres_solve(const HOp &op, double *u, const double *b, int restart, int
     if (!u || !b) return fail(AAR_HIP_ERR_ARG, "null vector");
     if (max_it < 0) return fail(AAR_HIP_ERR_ARG, "max_iter must be >= 0");
     if (restart < 1 || restart > KMAX) return fail(AAR_HIP_ERR_ARG, "restart must be in 1..32");
-    if (precond < 0 || precond > 1) return fail(AAR_HIP_ERR_ARG, "precond must be 0 (none) or 1 (Jacobi)");
+    if (precond < 0 || precond > 2) return fail(AAR_HIP_ERR_ARG, "precond must be 0 (none), 1 (Jacobi) or 2 (ILU(0))");
+    const bool ilu = precond == 2;
+    if (ilu && (op.nx < 13 || op.ny < 13 || op.nz < 13))
+        return fail(AAR_HIP_ERR_ARG, "the ILU(0) preconditioner needs every grid side >= 13");
     const long n = op.n;
     const long ld = ring_ld(n, sizeof(double2));
     HC(ensure(C.kvec, sizeof(double2) * (size_t)ld * (restart + 2)));
@@ -2107,24 +2226,40 @@ int helm_gmres_solve(const HOp &op, double *u, const double *b, int restart, int
     dg.dlap = op.dlap;
     dg.c0 = op.c0;
     dg.eta = op.eta;
-    dg.precond = precond;
+    dg.precond = precond == 1;
     const double2 *bz = (const double2 *)b;
     double2 *x = (double2 *)u;
     C.bytes_acc = 0.0;
     HC(hipEventRecord(C.t0, C.stream));
     hipLaunchKernelGGL(k_zero_kernel, dim3(64), dim3(KB), 0, C.stream, (unsigned *)k,
                        (long)(sizeof(KCtrlZ) / sizeof(unsigned)));
-    hipLaunchKernelGGL(kz_bnorm_kernel, dim3(nb), dim3(KB), 0, C.stream, n, bz, dg, part);
+    const double s = 16.0 * n, cb = op.c ? 8.0 * n : 0.0;
+    IluArgsT<double2> za{};
+    if (ilu) {
+        HC(ensure(C.kilu, sizeof(double2) * (size_t)n * IFS));
+        za = helm_ilu_args(op, (double2 *)C.kilu.p);
+        HC(ilu_factor(za, C.stream));
+        HC(ilu_apply(za, bz, t, false, &k->zero, C.stream));
+        hipLaunchKernelGGL(kz_bnorm2_kernel, dim3(nb), dim3(KB), 0, C.stream, n, bz, (const double2 *)t, part);
+        C.bytes_acc += 2 * ILU_APPLY_BYTES * n;
+    } else {
+        hipLaunchKernelGGL(kz_bnorm_kernel, dim3(nb), dim3(KB), 0, C.stream, n, bz, dg, part);
+    }
     hipLaunchKernelGGL(kz_ctl_init_kernel, dim3(1), dim3(KB), 0, C.stream, k, tol, max_it, restart, hcap,
                        (const double *)part, nb);
     HC(hipGetLastError());
-    const double s = 16.0 * n, cb = op.c ? 8.0 * n : 0.0;
     C.bytes_acc += s + cb;
     bool stopped = false;
     for (int cyc = 0; cyc <= max_it && !stopped; cyc++) {
         if (int rc = helm_matvec_async(op, u, (double *)t, &k->c)) return rc;
         hipLaunchKernelGGL(kz_resid_kernel, dim3(nb), dim3(KB), 0, C.stream, n, bz, (const double2 *)t, V, dg, part,
                            (const int *)&k->c.done);
+        if (ilu) {
+            HC(ilu_apply(za, (const double2 *)V, V, false, &k->c.done, C.stream));
+            hipLaunchKernelGGL(kz_norm_kernel, dim3(nb), dim3(KB), 0, C.stream, n, (const double2 *)V, part,
+                               (const int *)&k->c.done);
+            C.bytes_acc += 2 * ILU_APPLY_BYTES * n;
+        }
         hipLaunchKernelGGL(kz_start_kernel, dim3(1), dim3(KB), 0, C.stream, k, (const double *)part, nb, hist);
         hipLaunchKernelGGL(kz_scale_kernel, dim3(nb), dim3(KB), 0, C.stream, n, V, (const KCtrlZ *)k, 0);
         HC(hipGetLastError());
@@ -2133,10 +2268,16 @@ int helm_gmres_solve(const HOp &op, double *u, const double *b, int restart, int
             const int nv = it + 1;
             if (int rc = helm_matvec_async(op, (const double *)(V + (size_t)it * ld), (double *)t, &k->gate))
                 return rc;
+            const double2 *w = t;
+            if (ilu) {
+                HC(ilu_apply(za, (const double2 *)t, V + (size_t)nv * ld, false, &k->gate.done, C.stream));
+                w = V + (size_t)nv * ld;
+                C.bytes_acc += 2 * ILU_APPLY_BYTES * n;
+            }
             prof_begin("krylov_gmres_orth");
-            hipError_t e = nv <= 8 ? launch_gz_step<8>(t, V, ld, nv, n, nb, dg, k, part)
-                         : nv <= 16 ? launch_gz_step<16>(t, V, ld, nv, n, nb, dg, k, part)
-                                    : launch_gz_step<32>(t, V, ld, nv, n, nb, dg, k, part);
+            hipError_t e = nv <= 8 ? launch_gz_step<8>(w, V, ld, nv, n, nb, dg, k, part)
+                         : nv <= 16 ? launch_gz_step<16>(w, V, ld, nv, n, nb, dg, k, part)
+                                    : launch_gz_step<32>(w, V, ld, nv, n, nb, dg, k, part);
             prof_end((2.0 * nv + 3) * s + 2 * cb);
             if (e != hipSuccess) return fail(AAR_HIP_ERR_HIP, std::string("gmres step: ") + hipGetErrorString(e));
             hipLaunchKernelGGL(kz_arnoldi_kernel, dim3(1), dim3(KB), 0, C.stream, k, (const double *)part, nb, hist);
@@ -2198,7 +2339,10 @@ int helm_bicg_solve(const HOp &op, double *u, const double *b, int max_it, doubl
     if (int rc = context_init()) return rc;
     if (!u || !b) return fail(AAR_HIP_ERR_ARG, "null vector");
     if (max_it < 0) return fail(AAR_HIP_ERR_ARG, "max_iter must be >= 0");
-    if (precond < 0 || precond > 1) return fail(AAR_HIP_ERR_ARG, "precond must be 0 (none) or 1 (Jacobi)");
+    if (precond < 0 || precond > 2) return fail(AAR_HIP_ERR_ARG, "precond must be 0 (none), 1 (Jacobi) or 2 (ILU(0))");
+    const bool ilu = precond == 2;
+    if (ilu && (op.nx < 13 || op.ny < 13 || op.nz < 13))
+        return fail(AAR_HIP_ERR_ARG, "the ILU(0) preconditioner needs every grid side >= 13");
     const long n = op.n;
     const long ld = ring_ld(n, sizeof(double2));
     HC(ensure(C.kvec, sizeof(double2) * (size_t)ld * 8));
@@ -2216,7 +2360,7 @@ int helm_bicg_solve(const HOp &op, double *u, const double *b, int max_it, doubl
     dg.dlap = op.dlap;
     dg.c0 = op.c0;
     dg.eta = op.eta;
-    dg.precond = precond;
+    dg.precond = precond == 1;
     HOp oph = op;                 // A^H: eta -> -eta (the operator is complex symmetric)
     oph.eta = -op.eta;
     const double2 *bz = (const double2 *)b;
@@ -2225,7 +2369,29 @@ int helm_bicg_solve(const HOp &op, double *u, const double *b, int max_it, doubl
     HC(hipEventRecord(C.t0, C.stream));
     hipLaunchKernelGGL(k_zero_kernel, dim3(64), dim3(KB), 0, C.stream, (unsigned *)k,
                        (long)(sizeof(KCtrlZ) / sizeof(unsigned)));
-    hipLaunchKernelGGL(kz_bnorm_kernel, dim3(nb), dim3(KB), 0, C.stream, n, bz, dg, part);
+    IluArgsT<double2> za{};
+    // z = B r, z~ = B^H r~ = conj(M^{-T} conj(r~)) with ILU(0); the
+    // pointwise kernels then leave z = r, z~ = r~ for these sweeps
+    auto ilu_z = [&](const int *skip) -> int {
+        HC(ilu_apply(za, (const double2 *)z, z, false, skip, C.stream));
+        hipLaunchKernelGGL(kz_conj_kernel, dim3(nb), dim3(KB), 0, C.stream, n, zl, skip);
+        HC(ilu_apply(za, (const double2 *)zl, zl, true, skip, C.stream));
+        hipLaunchKernelGGL(kz_conj_kernel, dim3(nb), dim3(KB), 0, C.stream, n, zl, skip);
+        hipLaunchKernelGGL(kz_bi_norms_kernel, dim3(nb), dim3(KB), 0, C.stream, n, (const double2 *)z,
+                           (const double2 *)rl, part, skip);
+        HC(hipGetLastError());
+        C.bytes_acc += 4 * ILU_APPLY_BYTES * n;
+        return 0;
+    };
+    if (ilu) {
+        HC(ensure(C.kilu, sizeof(double2) * (size_t)n * IFS));
+        za = helm_ilu_args(op, (double2 *)C.kilu.p);
+        HC(ilu_factor(za, C.stream));
+        HC(ilu_apply(za, bz, z, false, &k->zero, C.stream));
+        hipLaunchKernelGGL(kz_bnorm2_kernel, dim3(nb), dim3(KB), 0, C.stream, n, bz, (const double2 *)z, part);
+    } else {
+        hipLaunchKernelGGL(kz_bnorm_kernel, dim3(nb), dim3(KB), 0, C.stream, n, bz, dg, part);
+    }
     hipLaunchKernelGGL(kz_ctl_init_kernel, dim3(1), dim3(KB), 0, C.stream, k, tol, max_it, 0, hcap,
                        (const double *)part, nb);
     HC(hipGetLastError());
@@ -2233,6 +2399,8 @@ int helm_bicg_solve(const HOp &op, double *u, const double *b, int max_it, doubl
     if (int rc = helm_matvec_async(op, u, (double *)q, nullptr)) return rc;
     hipLaunchKernelGGL(kz_bi_init_kernel, dim3(nb), dim3(KB), 0, C.stream, n, bz, (const double2 *)q, r, rl, z, zl,
                        dg, part);
+    if (ilu)
+        if (int rc = ilu_z(&k->zero)) return rc;
     hipLaunchKernelGGL(kz_bi_start_kernel, dim3(1), dim3(KB), 0, C.stream, k, (const double *)part, nb, hist);
     HC(hipGetLastError());
     C.bytes_acc += 7 * s + 2 * cb;
@@ -2250,6 +2418,8 @@ int helm_bicg_solve(const HOp &op, double *u, const double *b, int max_it, doubl
         hipLaunchKernelGGL(kz_bi_alpha_kernel, dim3(1), dim3(KB), 0, C.stream, k, (const double *)part, nb);
         hipLaunchKernelGGL(kz_bi_upd_kernel, dim3(nb), dim3(KB), 0, C.stream, n, x, r, rl, z, zl,
                            (const double2 *)p, (const double2 *)q, (const double2 *)ql, dg, part, (const KCtrlZ *)k);
+        if (ilu)
+            if (int rc = ilu_z(&k->c.done)) return rc;
         hipLaunchKernelGGL(kz_bi_step_kernel, dim3(1), dim3(KB), 0, C.stream, k, (const double *)part, nb, hist);
         prof_end(16 * s + 2 * cb);
         HC(hipGetLastError());

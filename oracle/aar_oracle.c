@@ -1524,6 +1524,9 @@ int orc_pgr_petsc(int nx, int ny, int nz, int fdn, double a, double sigma, const
 /* solution (tests).                                                   */
 /* ------------------------------------------------------------------ */
 typedef struct { double re, im; } zc;
+int orc_ilu0_factor_complex(int nx, int ny, int nz, int fdn, double a, const double *c, double c0, double eta,
+                            double *Fd);
+void orc_ilu0_apply_complex(int nx, int ny, int nz, const double *Fd, const double *ind, double *outd, int trans);
 static inline zc zmul(zc a, zc b) { zc r = {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; return r; }
 static inline zc zconj(zc a) { zc r = {a.re, -a.im}; return r; }
 static inline zc zadd(zc a, zc b) { zc r = {a.re + b.re, a.im + b.im}; return r; }
@@ -1551,6 +1554,15 @@ int orc_gmres_complex(int nx, int ny, int nz, int fdn, double a, const double *c
     zc *dinv = (zc *)malloc(sizeof(zc) * n);
     zc *V = (zc *)malloc(sizeof(zc) * n * (K + 1));
     zc *t = (zc *)malloc(sizeof(zc) * n);
+    zc *Fz = NULL;
+    const int ilu = precond == 2;
+    if (ilu) {
+        Fz = (zc *)malloc(sizeof(zc) * n * ILU_SLOTS);
+        if (orc_ilu0_factor_complex(nx, ny, nz, fdn, a, c, c0, eta, (double *)Fz)) {
+            free(dinv); free(V); free(t); free(Fz);
+            return -2;
+        }
+    }
     zc HH[(ORC_GMRES_MAXK + 2) * (ORC_GMRES_MAXK + 1)];
     zc cc[ORC_GMRES_MAXK + 1], ss[ORC_GMRES_MAXK + 1], grs[ORC_GMRES_MAXK + 2], nrs[ORC_GMRES_MAXK + 1];
     zc h[ORC_GMRES_MAXK + 1];
@@ -1569,8 +1581,9 @@ int orc_gmres_complex(int nx, int ny, int nz, int fdn, double a, const double *c
     int tlen = 0;
     ksp_conv cv_ = {0.0, 0.0, rtol, 0.0};
     double s2 = 0.0;
+    if (ilu) orc_ilu0_apply_complex(nx, ny, nz, (const double *)Fz, (const double *)b, (double *)t, 0);
     for (long i = 0; i < n; i++) {
-        const zc z = precond ? zmul(dinv[i], b[i]) : b[i];
+        const zc z = ilu ? t[i] : (precond ? zmul(dinv[i], b[i]) : b[i]);
         s2 = s2 + (z.re * z.re + z.im * z.im);
     }
     cv_.bpnorm = sqrt(s2);
@@ -1580,10 +1593,16 @@ int orc_gmres_complex(int nx, int ny, int nz, int fdn, double a, const double *c
         zc *v0 = V;
         orc_matvec_complex(nx, ny, nz, fdn, coeff, a, c, c0, eta, (const double *)x, (double *)t);
         s2 = 0.0;
-        for (long i = 0; i < n; i++) {
-            const zc r = zsub(b[i], t[i]);
-            v0[i] = precond ? zmul(dinv[i], r) : r;
-            s2 = s2 + (v0[i].re * v0[i].re + v0[i].im * v0[i].im);
+        if (ilu) {
+            for (long i = 0; i < n; i++) v0[i] = zsub(b[i], t[i]);
+            orc_ilu0_apply_complex(nx, ny, nz, (const double *)Fz, (const double *)v0, (double *)v0, 0);
+            for (long i = 0; i < n; i++) s2 = s2 + (v0[i].re * v0[i].re + v0[i].im * v0[i].im);
+        } else {
+            for (long i = 0; i < n; i++) {
+                const zc r = zsub(b[i], t[i]);
+                v0[i] = precond ? zmul(dinv[i], r) : r;
+                s2 = s2 + (v0[i].re * v0[i].re + v0[i].im * v0[i].im);
+            }
         }
         double res = sqrt(s2);
         if (res != 0.0 && res != 1.0) {
@@ -1600,7 +1619,9 @@ int orc_gmres_complex(int nx, int ny, int nz, int fdn, double a, const double *c
             zc *vn = V + (long)(it + 1) * n;
             orc_matvec_complex(nx, ny, nz, fdn, coeff, a, c, c0, eta, (const double *)(V + (long)it * n),
                                (double *)t);
-            for (long i = 0; i < n; i++) vn[i] = precond ? zmul(dinv[i], t[i]) : t[i];
+            if (ilu) orc_ilu0_apply_complex(nx, ny, nz, (const double *)Fz, (const double *)t, (double *)vn, 0);
+            else
+                for (long i = 0; i < n; i++) vn[i] = precond ? zmul(dinv[i], t[i]) : t[i];
             for (int j = 0; j <= it; j++) {
                 zc sj = {0.0, 0.0};
                 const zc *vj = V + (long)j * n;
@@ -1703,7 +1724,7 @@ int orc_gmres_complex(int nx, int ny, int nz, int fdn, double a, const double *c
         st->seconds = t1 - t0;
         st->status = reason;
     }
-    free(dinv); free(V); free(t);
+    free(dinv); free(V); free(t); free(Fz);
     return 0;
 }
 
@@ -1726,6 +1747,15 @@ int orc_bicg_complex(int nx, int ny, int nz, int fdn, double a, const double *c,
     zc *Rr = (zc *)malloc(sizeof(zc) * n), *Rl = (zc *)malloc(sizeof(zc) * n);
     zc *Zr = (zc *)malloc(sizeof(zc) * n), *Zl = (zc *)malloc(sizeof(zc) * n);
     zc *Pr = (zc *)malloc(sizeof(zc) * n), *Pl = (zc *)malloc(sizeof(zc) * n);
+    zc *Fz = NULL;
+    const int ilu = precond == 2;
+    if (ilu) {
+        Fz = (zc *)malloc(sizeof(zc) * n * ILU_SLOTS);
+        if (orc_ilu0_factor_complex(nx, ny, nz, fdn, a, c, c0, eta, (double *)Fz)) {
+            free(dinv); free(Rr); free(Rl); free(Zr); free(Zl); free(Pr); free(Pl); free(Fz);
+            return -2;
+        }
+    }
     for (long i = 0; i < n; i++) {
         if (precond) {
             const double dr = dlap + (c ? c[i] : c0);
@@ -1742,8 +1772,9 @@ int orc_bicg_complex(int nx, int ny, int nz, int fdn, double a, const double *c,
     int tlen = 0;
     ksp_conv cv_ = {0.0, 0.0, rtol, 0.0};
     double s2 = 0.0;
+    if (ilu) orc_ilu0_apply_complex(nx, ny, nz, (const double *)Fz, (const double *)b, (double *)Zr, 0);
     for (long i = 0; i < n; i++) {
-        const zc z = ZPC(i, b[i]);
+        const zc z = ilu ? Zr[i] : ZPC(i, b[i]);
         s2 = s2 + (z.re * z.re + z.im * z.im);
     }
     cv_.bpnorm = sqrt(s2);
@@ -1754,10 +1785,16 @@ int orc_bicg_complex(int nx, int ny, int nz, int fdn, double a, const double *c,
         Rr[i].re = b[i].re + (-1.0) * Rr[i].re;
         Rr[i].im = b[i].im + (-1.0) * Rr[i].im;
         Rl[i] = Rr[i];
-        Zr[i] = ZPC(i, Rr[i]);
-        Zl[i] = ZPCH(i, Rl[i]);
-        s2 = s2 + (Zr[i].re * Zr[i].re + Zr[i].im * Zr[i].im);
+        if (!ilu) {
+            Zr[i] = ZPC(i, Rr[i]);
+            Zl[i] = ZPCH(i, Rl[i]);
+        }
     }
+    if (ilu) {
+        orc_ilu0_apply_complex(nx, ny, nz, (const double *)Fz, (const double *)Rr, (double *)Zr, 0);
+        orc_ilu0_apply_complex(nx, ny, nz, (const double *)Fz, (const double *)Rl, (double *)Zl, 2);
+    }
+    for (long i = 0; i < n; i++) s2 = s2 + (Zr[i].re * Zr[i].re + Zr[i].im * Zr[i].im);
     double dp = sqrt(s2);
     if (trace && tlen < trace_cap) trace[tlen++] = dp;
     int its = 0;
@@ -1794,10 +1831,16 @@ int orc_bicg_complex(int nx, int ny, int nz, int fdn, double a, const double *c,
                 x[k] = zadd(x[k], zmul(al, Pr[k]));
                 Rr[k] = zadd(Rr[k], zmul(ma, Zr[k]));
                 Rl[k] = zadd(Rl[k], zmul(mac, Zl[k]));
-                Zr[k] = ZPC(k, Rr[k]);
-                Zl[k] = ZPCH(k, Rl[k]);
-                s2 = s2 + (Zr[k].re * Zr[k].re + Zr[k].im * Zr[k].im);
+                if (!ilu) {
+                    Zr[k] = ZPC(k, Rr[k]);
+                    Zl[k] = ZPCH(k, Rl[k]);
+                }
             }
+            if (ilu) {
+                orc_ilu0_apply_complex(nx, ny, nz, (const double *)Fz, (const double *)Rr, (double *)Zr, 0);
+                orc_ilu0_apply_complex(nx, ny, nz, (const double *)Fz, (const double *)Rl, (double *)Zl, 2);
+            }
+            for (long k = 0; k < n; k++) s2 = s2 + (Zr[k].re * Zr[k].re + Zr[k].im * Zr[k].im);
             dp = sqrt(s2);
             its = i + 1;
             if (trace && tlen < trace_cap) trace[tlen++] = dp;
@@ -1828,6 +1871,125 @@ int orc_bicg_complex(int nx, int ny, int nz, int fdn, double a, const double *c,
         st->seconds = t1 - t0;
         st->status = reason;
     }
-    free(dinv); free(Rr); free(Rl); free(Zr); free(Zl); free(Pr); free(Pl);
+    free(dinv); free(Rr); free(Rl); free(Zr); free(Zl); free(Pr); free(Pl); free(Fz);
     return 0;
+}
+
+/* ILU(0) of the complex Helmholtz matrix (PETSC_COMPLEX pc = 1:
+ * PCBJACOBI -> PCILU on one block): orc_ilu0_factor's elimination in
+ * complex arithmetic (explicit formulas), diagonal (a*3*coeff[0] + c) +
+ * i eta, off-diagonals a*coeff[d].  F: ORC_ILU_SLOTS complex entries per
+ * row (interleaved), slot 0 = 1/u_ii.  apply: z = M^{-1} r (trans = 0),
+ * M^{-T} r (trans = 1) or M^{-H} r = conj(M^{-T} conj(r)) (trans = 2). */
+int orc_ilu0_factor_complex(int nx, int ny, int nz, int fdn, double a, const double *c, double c0, double eta,
+                            double *Fd)
+{
+    if (fdn != 6 || nx < 13 || ny < 13 || nz < 13) return -1;
+    const long n = (long)nx * ny * nz;
+    const int dims[3] = {nx, ny, nz};
+    const long strides[3] = {1, nx, (long)nx * ny};
+    double coeff[32] = {0};
+    orc_lap_coefficient(coeff, fdn);
+    zc *F = (zc *)Fd;
+    zc v[ILU_SLOTS];
+    for (long i = 0; i < n; i++) {
+        const int cr[3] = {(int)(i % nx), (int)((i / nx) % ny), (int)(i / ((long)nx * ny))};
+        v[0].re = a * (3 * coeff[0]) + (c ? c[i] : c0);
+        v[0].im = eta;
+        for (int ax = 0; ax < 3; ax++)
+            for (int d = 1; d <= 6; d++) {
+                v[ilu_slot(ax, -d)].re = a * coeff[d];
+                v[ilu_slot(ax, -d)].im = 0.0;
+                v[ilu_slot(ax, d)].re = a * coeff[d];
+                v[ilu_slot(ax, d)].im = 0.0;
+            }
+        for (int ax = 2; ax >= 0; ax--) {
+            const int cc = cr[ax], na = dims[ax];
+            for (int cp = 0; cp < cc; cp++) {
+                const int dk = pdiff(cp, cc, na);
+                if (dk < -6 || dk > 6) continue;
+                const long kk = i + (long)(cp - cc) * strides[ax];
+                const zc l = zmul(v[ilu_slot(ax, dk)], F[kk * ILU_SLOTS]);
+                v[ilu_slot(ax, dk)] = l;
+                for (int cj = cp + 1; cj < na; cj++) {
+                    const int dkj = pdiff(cj, cp, na);
+                    if (dkj < -6 || dkj > 6) continue;
+                    const int dij = pdiff(cj, cc, na);
+                    if (dij < -6 || dij > 6) continue;
+                    const int si = dij == 0 ? 0 : ilu_slot(ax, dij);
+                    v[si] = zsub(v[si], zmul(l, F[kk * ILU_SLOTS + ilu_slot(ax, dkj)]));
+                }
+            }
+        }
+        if (v[0].re == 0.0 && v[0].im == 0.0) return -2;
+        const zc one = {1.0, 0.0};
+        v[0] = zdiv(one, v[0]);
+        memcpy(F + i * ILU_SLOTS, v, sizeof(v));
+    }
+    return 0;
+}
+
+void orc_ilu0_apply_complex(int nx, int ny, int nz, const double *Fd, const double *ind, double *outd, int trans)
+{
+    const long n = (long)nx * ny * nz;
+    const int dims[3] = {nx, ny, nz};
+    const long strides[3] = {1, nx, (long)nx * ny};
+    const zc *F = (const zc *)Fd;
+    const zc *in = (const zc *)ind;
+    zc *out = (zc *)outd;
+    int cs[12], ds[12];
+    if (out != in) memcpy(out, in, sizeof(zc) * n);
+    if (trans == 2)
+        for (long i = 0; i < n; i++) out[i] = zconj(out[i]);
+    if (!trans) {
+        for (long i = 0; i < n; i++) {
+            const int cr[3] = {(int)(i % nx), (int)((i / nx) % ny), (int)(i / ((long)nx * ny))};
+            zc s = out[i];
+            for (int ax = 2; ax >= 0; ax--) {
+                const int m = axis_cols(cr[ax], dims[ax], 1, cs, ds);
+                for (int q = 0; q < m; q++)
+                    s = zsub(s, zmul(F[i * ILU_SLOTS + ilu_slot(ax, ds[q])],
+                                     out[i + (long)(cs[q] - cr[ax]) * strides[ax]]));
+            }
+            out[i] = s;
+        }
+        for (long i = n - 1; i >= 0; i--) {
+            const int cr[3] = {(int)(i % nx), (int)((i / nx) % ny), (int)(i / ((long)nx * ny))};
+            zc s = out[i];
+            for (int ax = 0; ax < 3; ax++) {
+                const int m = axis_cols(cr[ax], dims[ax], 0, cs, ds);
+                for (int q = 0; q < m; q++)
+                    s = zsub(s, zmul(F[i * ILU_SLOTS + ilu_slot(ax, ds[q])],
+                                     out[i + (long)(cs[q] - cr[ax]) * strides[ax]]));
+            }
+            out[i] = zmul(s, F[i * ILU_SLOTS]);
+        }
+    } else {
+        for (long i = 0; i < n; i++) {
+            const int cr[3] = {(int)(i % nx), (int)((i / nx) % ny), (int)(i / ((long)nx * ny))};
+            zc s = out[i];
+            for (int ax = 2; ax >= 0; ax--) {
+                const int m = axis_cols(cr[ax], dims[ax], 1, cs, ds);
+                for (int q = 0; q < m; q++) {
+                    const long j = i + (long)(cs[q] - cr[ax]) * strides[ax];
+                    s = zsub(s, zmul(F[j * ILU_SLOTS + ilu_slot(ax, -ds[q])], out[j]));
+                }
+            }
+            out[i] = zmul(s, F[i * ILU_SLOTS]);
+        }
+        for (long i = n - 1; i >= 0; i--) {
+            const int cr[3] = {(int)(i % nx), (int)((i / nx) % ny), (int)(i / ((long)nx * ny))};
+            zc s = out[i];
+            for (int ax = 0; ax < 3; ax++) {
+                const int m = axis_cols(cr[ax], dims[ax], 0, cs, ds);
+                for (int q = 0; q < m; q++) {
+                    const long j = i + (long)(cs[q] - cr[ax]) * strides[ax];
+                    s = zsub(s, zmul(F[j * ILU_SLOTS + ilu_slot(ax, -ds[q])], out[j]));
+                }
+            }
+            out[i] = s;
+        }
+    }
+    if (trans == 2)
+        for (long i = 0; i < n; i++) out[i] = zconj(out[i]);
 }

@@ -191,6 +191,13 @@ int orc_gmres_complex(int nx, int ny, int nz, int fdn, double a, const double *c
 int orc_bicg_complex(int nx, int ny, int nz, int fdn, double a, const double *c, double c0, double eta,
                      double *u, const double *rhs, int max_it, double rtol, int precond,
                      double *trace, int trace_cap, orc_stats *st);
+/* ILU(0) of the complex Helmholtz matrix (PETSC_COMPLEX pc = 1), F with
+ * ORC_ILU_SLOTS complex entries per row; apply trans 0: M^{-1}, 1: M^{-T},
+ * 2: M^{-H} = conj(M^{-T} conj(r)).  orc_gmres_complex / orc_bicg_complex
+ * take precond = 2 for it. */
+int orc_ilu0_factor_complex(int nx, int ny, int nz, int fdn, double a, const double *c, double c0, double eta,
+                            double *F);
+void orc_ilu0_apply_complex(int nx, int ny, int nz, const double *F, const double *in, double *out, int trans);
 int orc_gmres(int nx, int ny, int nz, int fdn, double a, double sigma, const double *c,
               double *x, const double *b, int restart, int max_it, double rtol, int precond,
               double *trace, int trace_cap, orc_stats *st);

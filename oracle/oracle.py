@@ -100,6 +100,10 @@ def lib():
                                         ctypes.POINTER(OrcStats)]
         L.orc_bicg_complex.argtypes = [I, I, I, I, Dd, _D, Dd, Dd, _D, _D, I, Dd, I, _D, I,
                                        ctypes.POINTER(OrcStats)]
+        L.orc_ilu0_factor_complex.argtypes = [I, I, I, I, Dd, _D, Dd, Dd, _D]
+        L.orc_ilu0_factor_complex.restype = I
+        L.orc_ilu0_apply_complex.argtypes = [I, I, I, _D, _D, _D, I]
+        L.orc_ilu0_apply_complex.restype = None
         L.orc_pgr_petsc.argtypes = [I, I, I, I, Dd, Dd, _D, _D, _D, Dd, I, I, I, Dd, I, I, _D, I,
                                     ctypes.POINTER(OrcStats)]
         L.orc_ilu0_apply.argtypes = [I, I, I, _D, _D, _D, I]
@@ -273,6 +277,27 @@ def ilu0_apply(shape, F, r, trans=False):
     r = _f64(r).reshape(-1)
     out = np.empty_like(r)
     lib().orc_ilu0_apply(nx, ny, nz, _p(F), _p(r), _p(out), int(bool(trans)))
+    return out
+
+
+def ilu0_factor_complex(shape, a, c, c0, eta, fdn=6):
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    F = np.zeros(n * ILU_SLOTS, dtype=np.complex128)
+    cv = None if c is None else _f64(c).reshape(-1)
+    rc = lib().orc_ilu0_factor_complex(nx, ny, nz, fdn, a, _p(cv), c0, eta, _p(F.view(np.float64)))
+    if rc:
+        raise ValueError(f"orc_ilu0_factor_complex failed ({rc})")
+    return F.reshape(n, ILU_SLOTS)
+
+
+def ilu0_apply_complex(shape, F, r, trans=0):
+    nx, ny, nz = shape
+    F = np.ascontiguousarray(F, dtype=np.complex128).reshape(-1)
+    r = np.ascontiguousarray(r, dtype=np.complex128).reshape(-1)
+    out = np.empty_like(r)
+    lib().orc_ilu0_apply_complex(nx, ny, nz, _p(F.view(np.float64)), _p(r.view(np.float64)),
+                                 _p(out.view(np.float64)), int(trans))
     return out
 
 

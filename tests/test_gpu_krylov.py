@@ -471,3 +471,45 @@ def test_helm_bicg_parity(gpu, oracle, case):
     np.testing.assert_allclose(h, d["trace"], rtol=1e-6, atol=1e-8 * d["trace"][0])
     ug = ug.cpu().numpy()
     assert np.abs(ug - uo).max() <= 1e-8 * np.abs(uo).max()
+
+
+@pytest.mark.parametrize("shape,vc", [((13, 14, 13), True), ((24, 20, 16), False)])
+def test_helm_ilu0_apply_bitwise(gpu, oracle, shape, vc):
+    """Complex ILU(0): level-scheduled factorization and solves equal the
+    sequential restatement bit for bit, for M^-1, M^-T and M^-H."""
+    say("helm ilu0 apply", shape, vc)
+    n = int(np.prod(shape))
+    a, c0, eta = P.poisson_a(0.5), 0.3, 0.5
+    c = P.helmholtz_coefficient(shape, 7) if vc else None
+    r = P.rhs_complex_normal(shape, 1007)
+    F = oracle.ilu0_factor_complex(shape, a, c, c0, eta)
+    for trans in (0, 1, 2):
+        z = gpu.helm_ilu0_apply(cdev(r), shape, a, c=None if c is None else dev(c), c0=c0, eta=eta,
+                                trans=trans).cpu().numpy()
+        assert np.array_equal(z, oracle.ilu0_apply_complex(shape, F, r, trans)), trans
+
+
+@pytest.mark.parametrize("solver", ["gmres", "bicg"])
+@pytest.mark.parametrize("shape,vc,c0,eta", [((16, 16, 16), True, 0.0, 0.5),
+                                             ((36, 36, 36), False, -0.134992, 0.070225)])
+def test_helm_krylov_ilu0_parity(gpu, oracle, solver, shape, vc, c0, eta):
+    say("helm krylov ilu0", solver, shape)
+    n = int(np.prod(shape))
+    a = P.poisson_a(0.5) if shape[0] != 36 else -1.0 / (4 * np.pi)
+    c = P.helmholtz_coefficient(shape, 7) if vc else None
+    b = P.rhs_complex_normal(shape, 1005)
+    u0 = np.ones(n, dtype=complex)
+    ug = cdev(u0)
+    cg_ = None if c is None else dev(c)
+    if solver == "gmres":
+        uo, d = oracle.gmres_complex(u0, b, shape, a, c, c0, eta, 2000, 1e-6, precond=2)
+        st = gpu.helm_gmres(ug, cdev(b), shape, a, c=cg_, c0=c0, eta=eta, max_iter=2000, tol=1e-6, precond=2)
+    else:
+        uo, d = oracle.bicg_complex(u0, b, shape, a, c, c0, eta, 2000, 1e-6, precond=2)
+        st = gpu.helm_bicg(ug, cdev(b), shape, a, c=cg_, c0=c0, eta=eta, max_iter=2000, tol=1e-6, precond=2)
+    assert st["iterations"] == d["iterations"] and st["reason"] == d["status"]
+    h = gpu.history()
+    assert len(h) == len(d["trace"])
+    np.testing.assert_allclose(h, d["trace"], rtol=1e-6, atol=1e-8 * d["trace"][0])
+    ug = ug.cpu().numpy()
+    assert np.abs(ug - uo).max() <= 1e-8 * np.abs(uo).max()

@@ -522,3 +522,84 @@ def test_bicg_complex_reduces_to_real_and_solves(oracle):
     ref = np.fft.ifftn(np.fft.fftn(bc.reshape(nz, ny, nx)) / lam).reshape(-1)
     assert d["status"] == oracle.KSP_CONVERGED_RTOL
     assert np.abs(u - ref).max() <= 1e-7 * np.abs(ref).max()
+
+
+def generic_ilu0_complex(rows):
+    """generic_ilu0 for complex entries; the pivot reciprocal by the naive
+    formula (re - i im) / (re^2 + im^2), the oracle's."""
+    n = len(rows)
+    U = [dict(r) for r in rows]
+    inv = [0j] * n
+    for i in range(n):
+        ri = U[i]
+        for kk in sorted(q for q in ri if q < i):
+            l = ri[kk] * inv[kk]
+            ri[kk] = l
+            for jj in sorted(q for q in U[kk] if q > kk):
+                if jj in ri:
+                    ri[jj] = ri[jj] - l * U[kk][jj]
+        d = ri[i]
+        den = d.real * d.real + d.imag * d.imag
+        inv[i] = complex(d.real / den, -d.imag / den)
+    return U, inv
+
+
+def test_ilu0_complex_equals_generic_ikj_and_solves(oracle):
+    import scipy.linalg as sl
+    shape = (13, 14, 13)
+    a, c0, eta = -4.0, 0.0, 0.5
+    n = int(np.prod(shape))
+    c = np.random.default_rng(61).uniform(0.5, 1.5, n)
+    rows = star_rows(shape, a, 0.0, c)
+    for i, r in enumerate(rows):
+        r[i] = r[i] + 1j * eta
+    F = oracle.ilu0_factor_complex(shape, a, c, c0, eta)
+    U, inv = generic_ilu0_complex(rows)
+    G = np.zeros((n, 37), dtype=complex)
+    nx, ny, nz = shape
+    dims = (nx, ny, nz)
+    for i in range(n):
+        cr = (i % nx, (i // nx) % ny, i // (nx * ny))
+        G[i, 0] = inv[i]
+        for col, v in U[i].items():
+            if col == i:
+                continue
+            cc = (col % nx, (col // nx) % ny, col // (nx * ny))
+            ax = [q for q in range(3) if cc[q] != cr[q]][0]
+            G[i, slot(ax, pdiff(cc[ax], cr[ax], dims[ax]))] = v
+    assert np.array_equal(F, G)
+    # solves against dense triangular factors: M^-1, M^-T, M^-H
+    Ld = np.eye(n, dtype=complex)
+    Ud = np.zeros((n, n), dtype=complex)
+    for i in range(n):
+        for col, v in U[i].items():
+            if col < i:
+                Ld[i, col] = v
+            else:
+                Ud[i, col] = v
+    r = np.random.default_rng(62).standard_normal(n) + 1j * np.random.default_rng(63).standard_normal(n)
+    z = sl.solve_triangular(Ud, sl.solve_triangular(Ld, r, lower=True, unit_diagonal=True), lower=False)
+    np.testing.assert_allclose(oracle.ilu0_apply_complex(shape, F, r, 0), z, rtol=1e-11, atol=1e-14)
+    M = Ld @ Ud
+    np.testing.assert_allclose(oracle.ilu0_apply_complex(shape, F, r, 1), np.linalg.solve(M.T, r), rtol=1e-9,
+                               atol=1e-12)
+    np.testing.assert_allclose(oracle.ilu0_apply_complex(shape, F, r, 2), np.linalg.solve(M.conj().T, r),
+                               rtol=1e-9, atol=1e-12)
+
+
+def test_complex_krylov_ilu0_converge_to_fft_solution(oracle):
+    shape = (14, 14, 14)
+    n = int(np.prod(shape))
+    a, c0, eta = -4.0, 0.3, 0.5
+    bc = np.random.default_rng(64).standard_normal(n) + 1j * np.random.default_rng(65).standard_normal(n)
+    cf = oracle.lap_coefficient(6)
+    def s1(m):
+        th = 2 * np.pi * np.arange(m) / m
+        return cf[0] + 2 * sum(cf[q] * np.cos(q * th) for q in range(1, 7))
+    lam = a * (s1(14)[:, None, None] + s1(14)[None, :, None] + s1(14)[None, None, :]) + c0 + 1j * eta
+    ref = np.fft.ifftn(np.fft.fftn(bc.reshape(14, 14, 14)) / lam).reshape(-1)
+    for fn in (oracle.gmres_complex, oracle.bicg_complex):
+        u, d = fn(np.ones(n, dtype=complex), bc, shape, a, None, c0, eta, 2000, 1e-10, precond=2)
+        _, dj = fn(np.ones(n, dtype=complex), bc, shape, a, None, c0, eta, 2000, 1e-10, precond=1)
+        assert d["status"] == oracle.KSP_CONVERGED_RTOL and d["iterations"] < dj["iterations"]
+        assert np.abs(u - ref).max() <= 1e-7 * np.abs(ref).max()
