@@ -163,9 +163,14 @@ def helmholtz_case(n, dev, a, directions=20, hstep=1e-6, tol=1e-13):
     # (PCJACOBI, rtol 1e-6, max_it 2000 as PETSC_COMPLEX/main.c:96,127,155)
     # against AAR to the same tolerance
     kry = {}
-    for name, fn in (("aar", lambda u: aar_hip.helm_aar(u, b, shape, a, c, 0.0, eta, **dict(opts, tol=1e-6))),
-                     ("gmres30", lambda u: aar_hip.helm_gmres(u, b, shape, a, c, 0.0, eta, 2000, 1e-6)),
-                     ("bicg", lambda u: aar_hip.helm_bicg(u, b, shape, a, c, 0.0, eta, 2000, 1e-6))):
+    runs = [("aar", lambda u: aar_hip.helm_aar(u, b, shape, a, c, 0.0, eta, **dict(opts, tol=1e-6))),
+            ("gmres30", lambda u: aar_hip.helm_gmres(u, b, shape, a, c, 0.0, eta, 2000, 1e-6)),
+            ("bicg", lambda u: aar_hip.helm_bicg(u, b, shape, a, c, 0.0, eta, 2000, 1e-6))]
+    if n <= 128:
+        # the variant's pc = 1 (complex ILU(0), level-scheduled: 3n - 2 launches per triangular solve)
+        runs += [("gmres30_ilu0", lambda u: aar_hip.helm_gmres(u, b, shape, a, c, 0.0, eta, 2000, 1e-6, precond=2)),
+                 ("bicg_ilu0", lambda u: aar_hip.helm_bicg(u, b, shape, a, c, 0.0, eta, 2000, 1e-6, precond=2))]
+    for name, fn in runs:
         u = torch.zeros_like(b)
         st = fn(u)
         kry[name] = {"iterations": st["iterations"], "solve_ms": st["solve_ms"], "relres": st["relres"]}
