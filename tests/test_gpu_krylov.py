@@ -360,6 +360,8 @@ PG_CASES = [
     ((24, 20, 16), 0.5, 5, 4, 1000, 1e-8, 2, True),
     ((16, 16, 16), 0.0, 7, 6, 13, 1e-8, 2, False),
     ((48, 48, 48), 0.0, 7, 6, 1000, 1e-6, 2, False),
+    ((20, 16, 14), 0.0, 13, 11, 1000, 1e-6, 1, False),
+    ((20, 16, 14), 0.0, 13, 11, 1000, 1e-6, 2, True),
 ]
 
 
@@ -513,3 +515,23 @@ def test_helm_krylov_ilu0_parity(gpu, oracle, solver, shape, vc, c0, eta):
     np.testing.assert_allclose(h, d["trace"], rtol=1e-6, atol=1e-8 * d["trace"][0])
     ug = ug.cpu().numpy()
     assert np.abs(ug - uo).max() <= 1e-8 * np.abs(uo).max()
+
+
+def test_petsc_variant_max_iter_zero(gpu, oracle):
+    """max_iter = 0: no step; the printed conditions and the reported
+    iteration count follow the variant's loop (AAR_Real.c:98, 162-166)."""
+    shape = (16, 16, 16)
+    say("petsc variant max_iter 0")
+    a = P.poisson_a(0.5)
+    b = rhs(shape, 1001)
+    x0 = np.ones(b.size)
+    xo, d = oracle.aar_petsc(x0, b, shape, a, 0.6, 0.6, 7, 6, 0, 1e-6, precond=2)
+    xg = dev(x0)
+    st = gpu.AAR_petsc(xg, dev(b), shape, a, 0.6, 0.6, 7, 6, 0, 1e-6, precond=gpu.ILU0)
+    assert st["iterations"] == d["iterations"] == 0
+    assert st["printed_warning"] == d["printed_warning"] and not st["converged"]
+    assert np.array_equal(xg.cpu().numpy(), x0)
+    xo, d = oracle.pgr_petsc(x0, b, shape, a, 0.6, 7, 6, 0, 1e-6, precond=1)
+    xg = dev(x0)
+    st = gpu.PGR_petsc(xg, dev(b), shape, a, 0.6, 7, 6, 0, 1e-6, precond=True)
+    assert st["iterations"] == d["iterations"] == 0 and np.array_equal(xg.cpu().numpy(), x0)
