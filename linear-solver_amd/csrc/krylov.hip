@@ -1,7 +1,13 @@
-// krylov.hip -- the Krylov comparators of the PETSc variant of the reference
-// (PETSC_REAL/main.c:86-150): GMRES and BiCG with a Jacobi preconditioner,
-// run the way PETSc's KSPGMRES and KSPBICG run them there (rtol, max_it
-// 2000, nonzero initial guess).
+// krylov.hip -- the solvers of the PETSc variants of the reference:
+//  * the Krylov comparators of PETSC_REAL/main.c:86-150, GMRES and BiCG,
+//    run the way PETSc's KSPGMRES and KSPBICG run them there (rtol, max_it
+//    2000, nonzero initial guess), with PCJACOBI or the pc = 1 block-Jacobi
+//    ILU(0);
+//  * that ILU(0) (level-scheduled factorization and triangular solves);
+//  * the variant's own AAR, PGR and PL2R (AAR_Real.c, PGR_Real.c,
+//    PL2R_Real.c) with PCJACOBI or PCICC (as ILU(0));
+//  * GMRES and BiCG of PETSC_COMPLEX/main.c on the Helmholtz operator, with
+//    Jacobi or the complex ILU(0).
 //
 // PETSc itself is not part of the reference tree.  The control flow below
 // follows its published algorithms and defaults:
@@ -19,7 +25,7 @@
 //    value (same operations on the same numbers); it is computed once.
 //
 // Data layout: the GMRES basis is restart+1 padded slots of one buffer; all
-// vectors FP64.  Each iteration is a few streaming kernels (the FD12 stencil
+// vectors FP64 (complex128 interleaved for the Helmholtz solvers).  Each iteration is a few streaming kernels (the FD12 stencil
 // for A v, one pass of dot products, one pass that orthogonalises and takes
 // the norm, one scaling pass) and one-block control kernels that reduce the
 // per-block partial sums in a fixed order and run the scalar recurrences
