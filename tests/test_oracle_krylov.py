@@ -603,3 +603,33 @@ def test_complex_krylov_ilu0_converge_to_fft_solution(oracle):
         _, dj = fn(np.ones(n, dtype=complex), bc, shape, a, None, c0, eta, 2000, 1e-10, precond=1)
         assert d["status"] == oracle.KSP_CONVERGED_RTOL and d["iterations"] < dj["iterations"]
         assert np.abs(u - ref).max() <= 1e-7 * np.abs(ref).max()
+
+
+def test_icc0_equals_ilu0_on_the_symmetric_matrix(oracle):
+    """PCICC (the variant's pc = 1 for AAR/PGR/PL2R) is an incomplete
+    Cholesky factorization; it is realised here as ILU(0).  A textbook IC(0)
+    in LDL^T form on the same pattern gives L equal to ILU(0)'s unit lower
+    factor and U = D L^T, to rounding: the same preconditioner."""
+    shape = (13, 13, 13)
+    a, sigma = -4.0, 0.25
+    rows = star_rows(shape, a, sigma)
+    n = len(rows)
+    lower = [sorted(q for q in r if q < i) for i, r in enumerate(rows)]
+    L = [dict() for _ in range(n)]
+    d = [0.0] * n
+    for i in range(n):
+        li = L[i]
+        for k in lower[i]:
+            s = rows[i][k]
+            lk = L[k]
+            for j, v in li.items():           # columns j < k already computed in row i
+                if j in lk:
+                    s -= v * d[j] * lk[j]
+            li[k] = s / d[k]
+        d[i] = rows[i][i] - sum(v * v * d[k] for k, v in li.items())
+    U, inv = generic_ilu0(rows)
+    err_l = max(abs(U[i][k] - L[i][k]) / max(abs(L[i][k]), 1e-300) for i in range(n) for k in lower[i])
+    err_u = max(abs(U[i][j] - d[i] * L[j][i]) / max(abs(U[i][j]), 1e-300)
+                for i in range(n) for j in rows[i] if j > i)
+    err_d = max(abs(1.0 / inv[i] - d[i]) / abs(d[i]) for i in range(n))
+    assert err_l < 1e-11 and err_u < 1e-11 and err_d < 1e-12
