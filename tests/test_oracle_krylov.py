@@ -633,3 +633,86 @@ def test_icc0_equals_ilu0_on_the_symmetric_matrix(oracle):
                 for i in range(n) for j in rows[i] if j > i)
     err_d = max(abs(1.0 / inv[i] - d[i]) / abs(d[i]) for i in range(n))
     assert err_l < 1e-11 and err_u < 1e-11 and err_d < 1e-12
+
+
+def _np_gmres_estimates(M, r0, k, drop_first=False):
+    """Residual estimates of k Arnoldi/Givens steps of left-preconditioned
+    GMRES on M (dense), classical Gram-Schmidt; drop_first plants a mistake:
+    the projection on v_0 is skipped after the first step."""
+    n = r0.size
+    V = np.zeros((k + 1, n))
+    H = np.zeros((k + 1, k))
+    beta = np.linalg.norm(r0)
+    V[0] = r0 / beta
+    g = np.zeros(k + 1)
+    g[0] = beta
+    cs, sn = np.zeros(k), np.zeros(k)
+    est = [beta]
+    for j in range(k):
+        w = M @ V[j]
+        h = V[: j + 1] @ w
+        if drop_first and j > 0:
+            h[0] = 0.0
+        w = w - h @ V[: j + 1]
+        H[: j + 1, j] = h
+        H[j + 1, j] = np.linalg.norm(w)
+        V[j + 1] = w / H[j + 1, j]
+        for i in range(j):
+            t = H[i, j]
+            H[i, j] = cs[i] * t + sn[i] * H[i + 1, j]
+            H[i + 1, j] = cs[i] * H[i + 1, j] - sn[i] * t
+        r = np.hypot(H[j, j], H[j + 1, j])
+        cs[j], sn[j] = H[j, j] / r, H[j + 1, j] / r
+        g[j + 1] = -sn[j] * g[j]
+        g[j] = cs[j] * g[j]
+        est.append(abs(g[j + 1]))
+    return np.array(est)
+
+
+def test_krylov_checks_catch_planted_mistakes(oracle):
+    """The Krylov-minimum check passes a correct GMRES restatement and fails
+    one that drops a Gram-Schmidt projection; the PCG check fails a BiCG
+    whose beta has the wrong sign."""
+    shape = (8, 8, 8)
+    a, sigma = -4.0, 0.5
+    b = rhs(shape, 71)
+    A = dense_operator(oracle, shape, a, sigma)
+    M = A / diag_of(a, sigma)
+    r0 = (b - A @ np.ones(b.size)) / diag_of(a, sigma)
+    Q = [r0 / np.linalg.norm(r0)]
+    best = [np.linalg.norm(r0)]
+    for k in range(1, 11):
+        v = M @ Q[-1]
+        for _ in range(2):
+            for q in Q:
+                v = v - (q @ v) * q
+        Q.append(v / np.linalg.norm(v))
+        K = np.array(Q[:k]).T
+        y, *_ = np.linalg.lstsq(M @ K, r0, rcond=None)
+        best.append(np.linalg.norm(r0 - M @ K @ y))
+    good = _np_gmres_estimates(M, r0, 10)
+    bad = _np_gmres_estimates(M, r0, 10, drop_first=True)
+    np.testing.assert_allclose(good, best, rtol=1e-7)
+    assert np.max(np.abs(bad - best) / np.array(best)) > 1e-3
+    # BiCG with a flipped beta sign is no longer PCG
+    dinv = 1.0 / diag_of(a, sigma)
+    def pcg_norms(flip):
+        x = np.ones(b.size)
+        r = b - A @ x
+        z = dinv * r
+        p = z.copy()
+        rz = r @ z
+        out = [np.linalg.norm(z)]
+        for _ in range(8):
+            q = A @ p
+            al = rz / (p @ q)
+            r = r - al * q
+            z = dinv * r
+            out.append(np.linalg.norm(z))
+            rzn = r @ z
+            p = z + (-1.0 if flip else 1.0) * (rzn / rz) * p
+            rz = rzn
+        return np.array(out)
+    _, st = oracle.bicg(np.ones(b.size), b, shape, a, 8, 1e-14, sigma=sigma)
+    np.testing.assert_allclose(st["trace"], pcg_norms(False), rtol=1e-8)
+    assert np.max(np.abs(st["trace"] - pcg_norms(True)) / st["trace"]) > 1e-3
