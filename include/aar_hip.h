@@ -76,7 +76,7 @@ int aar_hip_aar_f32(const aar_hip_grid *g, float *x, const float *b, double omeg
  * the iterate itself and the m x m normal equations in FP64; the Richardson
  * sweeps (stencil), the residual history and the Gram reads in FP32.  The
  * FP64 true residual replaces the FP32 recurrence every p iterations, so
- * convergence is judged on b - A x in FP64 as in the reference. */
+ * convergence is checked on b - A x in FP64 as in the reference. */
 int aar_hip_aar_mixed(const aar_hip_grid *g, double *x, const double *b, double omega, double beta,
                       int m, int p, int max_iter, double tol, int precond, aar_hip_stats *st);
 
