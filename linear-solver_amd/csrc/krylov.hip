@@ -828,10 +828,8 @@ __device__ __forceinline__ bool ilu_point(const IluArgsT<T> &a, int &x, int &y, 
 __device__ __forceinline__ constexpr int qoff(int q) { return q < 6 ? q + 1 : q - 12; }
 
 template <typename T>
-__global__ __launch_bounds__(KB) void ilu_factor_kernel(IluArgsT<T> a)
+__device__ __forceinline__ void ilu_factor_point(const IluArgsT<T> &a, int x, int y, int z)
 {
-    int x, y, z;
-    if (!ilu_point(a, x, y, z)) return;
     const int cr[3] = {x, y, z}, dims[3] = {a.nx, a.ny, a.nz};
     const long strides[3] = {1, a.nx, (long)a.nx * a.ny};
     const long i = x + (long)y * a.nx + (long)z * a.nx * a.ny;
@@ -879,16 +877,21 @@ __global__ __launch_bounds__(KB) void ilu_factor_kernel(IluArgsT<T> a)
     for (int s = 1; s < 37; s++) Fi[s] = v[s];
 }
 
+template <typename T>
+__global__ __launch_bounds__(KB) void ilu_factor_kernel(IluArgsT<T> a)
+{
+    int x, y, z;
+    if (!ilu_point(a, x, y, z)) return;
+    ilu_factor_point(a, x, y, z);
+}
+
 // One level of a triangular solve, in place on io (in -> io when in != io
 // at the first solve of an apply).  lowerpass: the columns below the row
 // (forward levels); trans: the transposed factor (entries of the other
 // row); diagmul: multiply by 1/u_ii at the end.
 template <typename T, bool LOWER, bool TRANS, bool DIAG>
-__global__ __launch_bounds__(KB) void ilu_solve_kernel(IluArgsT<T> a, const T *in, T *io)
+__device__ __forceinline__ void ilu_solve_point(const IluArgsT<T> &a, const T *in, T *io, int x, int y, int z)
 {
-    if (*a.skip) return;
-    int x, y, z;
-    if (!ilu_point(a, x, y, z)) return;
     const int cr[3] = {x, y, z}, dims[3] = {a.nx, a.ny, a.nz};
     const long strides[3] = {1, a.nx, (long)a.nx * a.ny};
     const long i = x + (long)y * a.nx + (long)z * a.nx * a.ny;
@@ -911,6 +914,61 @@ __global__ __launch_bounds__(KB) void ilu_solve_kernel(IluArgsT<T> a, const T *i
         }
     }
     io[i] = DIAG ? s_mul(s, Fi[0]) : s;
+}
+
+template <typename T, bool LOWER, bool TRANS, bool DIAG>
+__global__ __launch_bounds__(KB) void ilu_solve_kernel(IluArgsT<T> a, const T *in, T *io)
+{
+    if (*a.skip) return;
+    int x, y, z;
+    if (!ilu_point(a, x, y, z)) return;
+    ilu_solve_point<T, LOWER, TRANS, DIAG>(a, in, io, x, y, z);
+}
+
+// The levels L0, L0 + step, ..., L1 (each with at most ILU_SMALL_PTS points
+// in its (x, y) box) in one workgroup, a barrier between levels: the same
+// per-point work as ilu_solve_kernel without a launch per level.
+constexpr int ILU_SMALL_WG = 1024, ILU_SMALL_PTS = 1024;
+
+template <typename T>
+__device__ __forceinline__ void ilu_level_box(const IluArgsT<T> &a, int L, int &x0, int &y0, int &wx, int &wy)
+{
+    x0 = max(0, L - (a.ny - 1) - (a.nz - 1));
+    y0 = max(0, L - (a.nx - 1) - (a.nz - 1));
+    wx = min(a.nx - 1, L) - x0 + 1;
+    wy = min(a.ny - 1, L) - y0 + 1;
+}
+
+template <typename T, bool LOWER, bool TRANS, bool DIAG>
+__global__ __launch_bounds__(ILU_SMALL_WG) void ilu_solve_small_kernel(IluArgsT<T> a, const T *in, T *io, int L0,
+                                                                      int L1, int step)
+{
+    if (*a.skip) return;
+    for (int L = L0;; L += step) {
+        int x0, y0, wx, wy;
+        ilu_level_box(a, L, x0, y0, wx, wy);
+        for (int t = threadIdx.x; t < wx * wy; t += ILU_SMALL_WG) {
+            const int x = x0 + t % wx, y = y0 + t / wx, z = L - x - y;
+            if (z >= 0 && z < a.nz) ilu_solve_point<T, LOWER, TRANS, DIAG>(a, in, io, x, y, z);
+        }
+        if (L == L1) break;
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(ILU_SMALL_WG) void ilu_factor_small_kernel(IluArgsT<T> a, int L0, int L1)
+{
+    for (int L = L0;; L++) {
+        int x0, y0, wx, wy;
+        ilu_level_box(a, L, x0, y0, wx, wy);
+        for (int t = threadIdx.x; t < wx * wy; t += ILU_SMALL_WG) {
+            const int x = x0 + t % wx, y = y0 + t / wx, z = L - x - y;
+            if (z >= 0 && z < a.nz) ilu_factor_point(a, x, y, z);
+        }
+        if (L == L1) break;
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -964,13 +1022,45 @@ IluArgs ilu_args(const Op &op, double *F)
 template <typename T>
 int ilu_nlev(const IluArgsT<T> &a) { return a.nx + a.ny + a.nz - 2; }
 
+// a level whose (x, y) box has at most ILU_SMALL_PTS points
+template <typename T>
+bool ilu_small(IluArgsT<T> a, int L)
+{
+    ilu_box(a, L);
+    return (long)a.wx * a.wy <= ILU_SMALL_PTS;
+}
+
+// Launch the levels from..to (step +1 or -1): runs of small levels go to
+// `small(L0, L1)` in one launch, the others to `big(L)` one launch each.
+template <typename T, typename Small, typename Big>
+void ilu_levels(const IluArgsT<T> &a, int from, int to, int step, Small small, Big big)
+{
+    for (int L = from; L != to + step;) {
+        if (ilu_small(a, L)) {
+            int L1 = L;
+            while (L1 != to && ilu_small(a, L1 + step)) L1 += step;
+            small(L, L1);
+            L = L1 + step;
+        } else {
+            big(L);
+            L += step;
+        }
+    }
+}
+
 template <typename T>
 hipError_t ilu_factor(IluArgsT<T> a, hipStream_t st)
 {
-    for (int L = 0; L < ilu_nlev(a); L++) {
-        ilu_box(a, L);
-        hipLaunchKernelGGL((ilu_factor_kernel<T>), dim3((a.wx * a.wy + KB - 1) / KB), dim3(KB), 0, st, a);
-    }
+    ilu_levels(
+        a, 0, ilu_nlev(a) - 1, 1,
+        [&](int L0, int L1) {
+            hipLaunchKernelGGL((ilu_factor_small_kernel<T>), dim3(1), dim3(ILU_SMALL_WG), 0, st, a, L0, L1);
+        },
+        [&](int L) {
+            IluArgsT<T> b = a;
+            ilu_box(b, L);
+            hipLaunchKernelGGL((ilu_factor_kernel<T>), dim3((b.wx * b.wy + KB - 1) / KB), dim3(KB), 0, st, b);
+        });
     return hipGetLastError();
 }
 
@@ -980,20 +1070,41 @@ hipError_t ilu_apply(IluArgsT<T> a, const T *in, T *out, bool trans, const int *
 {
     a.skip = skip;
     const int nl = ilu_nlev(a);
-    for (int L = 0; L < nl; L++) {
-        ilu_box(a, L);
-        const dim3 g((a.wx * a.wy + KB - 1) / KB);
-        if (trans) hipLaunchKernelGGL((ilu_solve_kernel<T, true, true, true>), g, dim3(KB), 0, st, a, in, out);
-        else hipLaunchKernelGGL((ilu_solve_kernel<T, true, false, false>), g, dim3(KB), 0, st, a, in, out);
-    }
-    for (int L = nl - 1; L >= 0; L--) {
-        ilu_box(a, L);
-        const dim3 g((a.wx * a.wy + KB - 1) / KB);
-        if (trans) hipLaunchKernelGGL((ilu_solve_kernel<T, false, true, false>), g, dim3(KB), 0, st, a,
-                                      (const T *)out, out);
-        else hipLaunchKernelGGL((ilu_solve_kernel<T, false, false, true>), g, dim3(KB), 0, st, a, (const T *)out,
-                                out);
-    }
+    const dim3 sw(ILU_SMALL_WG);
+    // forward: L (or U^T) over ascending levels, reading `in`
+    ilu_levels(
+        a, 0, nl - 1, 1,
+        [&](int L0, int L1) {
+            if (trans) hipLaunchKernelGGL((ilu_solve_small_kernel<T, true, true, true>), dim3(1), sw, 0, st, a, in, out,
+                                          L0, L1, 1);
+            else hipLaunchKernelGGL((ilu_solve_small_kernel<T, true, false, false>), dim3(1), sw, 0, st, a, in, out,
+                                    L0, L1, 1);
+        },
+        [&](int L) {
+            IluArgsT<T> b = a;
+            ilu_box(b, L);
+            const dim3 g((b.wx * b.wy + KB - 1) / KB);
+            if (trans) hipLaunchKernelGGL((ilu_solve_kernel<T, true, true, true>), g, dim3(KB), 0, st, b, in, out);
+            else hipLaunchKernelGGL((ilu_solve_kernel<T, true, false, false>), g, dim3(KB), 0, st, b, in, out);
+        });
+    // backward: U (or L^T) over descending levels, in place
+    ilu_levels(
+        a, nl - 1, 0, -1,
+        [&](int L0, int L1) {
+            if (trans) hipLaunchKernelGGL((ilu_solve_small_kernel<T, false, true, false>), dim3(1), sw, 0, st, a,
+                                          (const T *)out, out, L0, L1, -1);
+            else hipLaunchKernelGGL((ilu_solve_small_kernel<T, false, false, true>), dim3(1), sw, 0, st, a,
+                                    (const T *)out, out, L0, L1, -1);
+        },
+        [&](int L) {
+            IluArgsT<T> b = a;
+            ilu_box(b, L);
+            const dim3 g((b.wx * b.wy + KB - 1) / KB);
+            if (trans) hipLaunchKernelGGL((ilu_solve_kernel<T, false, true, false>), g, dim3(KB), 0, st, b,
+                                          (const T *)out, out);
+            else hipLaunchKernelGGL((ilu_solve_kernel<T, false, false, true>), g, dim3(KB), 0, st, b, (const T *)out,
+                                    out);
+        });
     return hipGetLastError();
 }
 
