@@ -475,7 +475,7 @@ def test_helm_bicg_parity(gpu, oracle, case):
     assert np.abs(ug - uo).max() <= 1e-8 * np.abs(uo).max()
 
 
-@pytest.mark.parametrize("shape,vc", [((13, 14, 13), True), ((24, 20, 16), False)])
+@pytest.mark.parametrize("shape,vc", [((13, 14, 13), True), ((24, 20, 16), False), ((40, 33, 27), True)])
 def test_helm_ilu0_apply_bitwise(gpu, oracle, shape, vc):
     """Complex ILU(0): level-scheduled factorization and solves equal the
     sequential restatement bit for bit, for M^-1, M^-T and M^-H."""
