@@ -810,11 +810,17 @@ template <typename T> __device__ __forceinline__ T s_make(double re, double im);
 template <> __device__ __forceinline__ double s_make<double>(double re, double) { return re; }
 template <> __device__ __forceinline__ double2 s_make<double2>(double re, double im) { return make_double2(re, im); }
 
+// Threads per block of the per-level kernels.  A level's points lie on the
+// plane x + y + z = L, so neighbouring lanes are 1 - nx*ny elements apart
+// and every load of a wave touches 64 cache lines; one-wave blocks spread a
+// level over as many CUs as it has waves instead of stacking four per CU.
+constexpr int ILU_WG = 64;
+
 // the point of this thread on the level, or false
 template <typename T>
 __device__ __forceinline__ bool ilu_point(const IluArgsT<T> &a, int &x, int &y, int &z)
 {
-    const int t = blockIdx.x * KB + threadIdx.x;
+    const int t = blockIdx.x * ILU_WG + threadIdx.x;
     if (t >= a.wx * a.wy) return false;
     x = a.x0 + t % a.wx;
     y = a.y0 + t / a.wx;
@@ -878,7 +884,7 @@ __device__ __forceinline__ void ilu_factor_point(const IluArgsT<T> &a, int x, in
 }
 
 template <typename T>
-__global__ __launch_bounds__(KB) void ilu_factor_kernel(IluArgsT<T> a)
+__global__ __launch_bounds__(ILU_WG) void ilu_factor_kernel(IluArgsT<T> a)
 {
     int x, y, z;
     if (!ilu_point(a, x, y, z)) return;
@@ -917,7 +923,7 @@ __device__ __forceinline__ void ilu_solve_point(const IluArgsT<T> &a, const T *i
 }
 
 template <typename T, bool LOWER, bool TRANS, bool DIAG>
-__global__ __launch_bounds__(KB) void ilu_solve_kernel(IluArgsT<T> a, const T *in, T *io)
+__global__ __launch_bounds__(ILU_WG) void ilu_solve_kernel(IluArgsT<T> a, const T *in, T *io)
 {
     if (*a.skip) return;
     int x, y, z;
@@ -1059,7 +1065,7 @@ hipError_t ilu_factor(IluArgsT<T> a, hipStream_t st)
         [&](int L) {
             IluArgsT<T> b = a;
             ilu_box(b, L);
-            hipLaunchKernelGGL((ilu_factor_kernel<T>), dim3((b.wx * b.wy + KB - 1) / KB), dim3(KB), 0, st, b);
+            hipLaunchKernelGGL((ilu_factor_kernel<T>), dim3((b.wx * b.wy + ILU_WG - 1) / ILU_WG), dim3(ILU_WG), 0, st, b);
         });
     return hipGetLastError();
 }
@@ -1083,9 +1089,9 @@ hipError_t ilu_apply(IluArgsT<T> a, const T *in, T *out, bool trans, const int *
         [&](int L) {
             IluArgsT<T> b = a;
             ilu_box(b, L);
-            const dim3 g((b.wx * b.wy + KB - 1) / KB);
-            if (trans) hipLaunchKernelGGL((ilu_solve_kernel<T, true, true, true>), g, dim3(KB), 0, st, b, in, out);
-            else hipLaunchKernelGGL((ilu_solve_kernel<T, true, false, false>), g, dim3(KB), 0, st, b, in, out);
+            const dim3 g((b.wx * b.wy + ILU_WG - 1) / ILU_WG), wg(ILU_WG);
+            if (trans) hipLaunchKernelGGL((ilu_solve_kernel<T, true, true, true>), g, wg, 0, st, b, in, out);
+            else hipLaunchKernelGGL((ilu_solve_kernel<T, true, false, false>), g, wg, 0, st, b, in, out);
         });
     // backward: U (or L^T) over descending levels, in place
     ilu_levels(
@@ -1099,10 +1105,10 @@ hipError_t ilu_apply(IluArgsT<T> a, const T *in, T *out, bool trans, const int *
         [&](int L) {
             IluArgsT<T> b = a;
             ilu_box(b, L);
-            const dim3 g((b.wx * b.wy + KB - 1) / KB);
-            if (trans) hipLaunchKernelGGL((ilu_solve_kernel<T, false, true, false>), g, dim3(KB), 0, st, b,
+            const dim3 g((b.wx * b.wy + ILU_WG - 1) / ILU_WG), wg(ILU_WG);
+            if (trans) hipLaunchKernelGGL((ilu_solve_kernel<T, false, true, false>), g, wg, 0, st, b,
                                           (const T *)out, out);
-            else hipLaunchKernelGGL((ilu_solve_kernel<T, false, false, true>), g, dim3(KB), 0, st, b, (const T *)out,
+            else hipLaunchKernelGGL((ilu_solve_kernel<T, false, false, true>), g, wg, 0, st, b, (const T *)out,
                                     out);
         });
     return hipGetLastError();
