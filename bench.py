@@ -45,8 +45,7 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--mode", default="bench", choices=["bench", "kernels"])
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-n", type=int, default=256)
-    ap.add_argument("--cpu-iters", type=int, default=12)
+    ap.add_argument("--cpu-ranks", type=int, default=0, help="MPI ranks of the all-core CPU run (0: min(16, cores))")
     ap.add_argument("--no-helm", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-vc", action="store_true")
@@ -58,44 +57,66 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n_cpu, iters, gpu_iterations, n_gpu):
-    """Time the reference AAR (oracle/_ref, MPI ranks on the host cores) on
-    a bounded sample: n_cpu^3 for `iters` iterations; scale the per-iteration
-    time by the point ratio and the GPU solve's iteration count."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fp:
+            for line in fp:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def cpu_baseline(gpu_rates, n_gpu, gpu_iterations, ranks_all=None):
+    """Full reference AAR solves (oracle/_ref: the reference MPI_REAL sources
+    built from their own files) on the host cores, timed around the AAR()
+    call as MPI_REAL/main.c:67-69 does: the C2 problem at 256^3 with one MPI
+    rank per core, and at 128^3 with one rank.  Each is a whole solve from
+    x0 = 0 to relres 1e-6.  `value` states the 256^3 all-core rate in the
+    metric's unit (solves/s of the n_gpu^3 solve) by the point ratio and the
+    iteration ratio; the measured rates and the GPU's at the same sizes are
+    reported beside it.  A failed run is recorded as such."""
     import numpy as np
     from aar_hip import problems as P
     from oracle import oracle as O
-    shape = (n_cpu,) * 3
-    b = P.rhs_uniform(shape, 1002)
-    x0 = np.zeros(b.size)
+    cores = ranks_all or min(16, len(os.sched_getaffinity(0)))
+    out = {"unit": "solves/s", "kind": "reference", "cores": cores, "cpu": cpu_model(),
+           "host_threads": os.cpu_count(), "runs": {}}
+    if not O.ref_available():
+        out["error"] = "oracle/_ref/ref_harness is not built (make -C oracle ref)"
+        out["value"] = None
+        return out
     a = P.poisson_a(0.25)
-    cores = min(16, os.cpu_count() or 1)
-    secs, kind, used = None, None, 1
-    if O.ref_available():
-        for ranks in ([cores, 1] if cores > 1 else [1]):
-            try:
-                _, d = O.ref_solve("aar", shape, b, x0, a, 0.0, 0.6, 0.6, 7, 6, iters, 1e-30, True,
-                                   ranks=ranks, timeout=600)
-                secs, kind, used = d["seconds"], "reference", ranks
-                break
-            except Exception as e:  # noqa: BLE001
-                log(f"cpu baseline: reference with {ranks} ranks failed: {e}")
-    if secs is None:
-        t0 = time.time()
-        O.aar(x0, b, shape, a, 0.6, 0.6, 7, 6, iters, 1e-30, trace=False)
-        secs, kind, used = time.time() - t0, "port", 1
-    per_iter = secs / iters * (n_gpu / n_cpu) ** 3
-    solve_s = per_iter * gpu_iterations
-    return {
-        "value": 1.0 / solve_s,
-        "unit": "solves/s",
-        "cores": used,
-        "kind": kind,
-        "sample": (f"AAR on {n_cpu}^3 for {iters} iterations ({iters // 6} Anderson cycles) in "
-                   f"{secs:.2f} s; per-iteration time x {(n_gpu / n_cpu) ** 3:.0f} (points of "
-                   f"{n_gpu}^3) x {gpu_iterations} iterations (the GPU solve's count)"),
-        "cpu": platform.processor() or platform.machine(),
-    }
+    for side, ranks in ((256, cores), (128, 1)):
+        shape = (side,) * 3
+        b = P.rhs_uniform(shape, 1002)
+        key = f"{side}^3_{ranks}rank" + ("s" if ranks > 1 else "")
+        try:
+            _, d = O.ref_solve("aar", shape, b, np.zeros(b.size), a, 0.0, 0.6, 0.6, 7, 6, 20000, 1e-6, True,
+                               ranks=ranks, timeout=900, trace=False)
+            rate = 1.0 / d["seconds"]
+            out["runs"][key] = {"ranks": ranks, "seconds": d["seconds"], "iterations": d["iterations"],
+                                "solves_per_s": rate, "relres": d["relres_printed"],
+                                "gpu_fp64_solves_per_s": gpu_rates.get(side),
+                                "gpu_over_cpu": (gpu_rates[side] / rate) if gpu_rates.get(side) else None}
+            log(f"cpu baseline {key}: {d['iterations']} its in {d['seconds']:.1f} s")
+        except Exception as e:  # noqa: BLE001
+            out["runs"][key] = {"ranks": ranks, "error": str(e)[-400:]}
+            log(f"cpu baseline {key} failed: {e}")
+    r = out["runs"].get(f"256^3_{cores}ranks", {})
+    if "seconds" in r:
+        scale = (n_gpu / 256) ** 3 * gpu_iterations / r["iterations"]
+        out["value"] = 1.0 / (r["seconds"] * scale)
+        out["sample"] = (f"full reference AAR solve of the same problem at 256^3 on {cores} MPI ranks "
+                         f"({r['iterations']} iterations, {r['seconds']:.1f} s), stated at {n_gpu}^3 by "
+                         f"x{(n_gpu / 256) ** 3:.0f} points and x{gpu_iterations / r['iterations']:.2f} "
+                         f"iterations ({gpu_iterations} at {n_gpu}^3); the full solves and the GPU's rates at "
+                         "the same sizes are under runs")
+    else:
+        out["value"] = None
+        out["error"] = "the 256^3 reference solve did not complete"
+    return out
 
 
 def helmholtz_case(n, dev, a, directions=20, hstep=1e-6, tol=1e-13):
@@ -276,16 +297,7 @@ def batch_case(dev, seed=1004):
     import torch
     import aar_hip
     from aar_hip import problems as P
-    sides = np.array([24, 32, 48, 64, 96, 128])
-    u = P.uniform01(3 * 96 + 1, seed, 7)
-    inst = []
-    for i in range(96):
-        side = int(sides[min(5, int(u[2 * i] * 6))])
-        sigma = 0.1 + 1.9 * float(u[2 * i + 1])
-        inst.append(((side,) * 3, sigma, i))
-    sig16 = 0.1 + 1.9 * float(u[-1])
-    for r in range(16):
-        inst.append(((128,) * 3, sig16, 96 + r))
+    inst = P.batch_instances(seed)
     a = P.poisson_a(0.5)
     m, p, tol, max_iter = 7, 6, 1e-6, 5000
     t0 = time.perf_counter()
@@ -293,7 +305,7 @@ def batch_case(dev, seed=1004):
     tuned.update({f"{s_}_f32": aar_hip.autotune((s_,) * 3, fp32=True, batch=True)["lz"]
                   for s_ in sorted({sh[0] for sh, _, _ in inst})})
     t_tune = time.perf_counter() - t0
-    bs = [P.normal_torch(int(np.prod(sh)), seed, 100 + k, dev) for sh, _, k in inst]
+    bs = [P.normal_torch(int(np.prod(sh)), seed, k, dev) for sh, _, k in inst]
     xs = [torch.zeros_like(b_) for b_ in bs]
     points = int(sum(np.prod(sh) for sh, _, _ in inst))
     out = {"instances": len(inst), "points": points, "sides": [sh[0] for sh, _, _ in inst],
@@ -360,6 +372,8 @@ def main():
         for _ in range(5):
             aar_hip.Lap_Vec_mult(b, shape, a, out=y)
         y.copy_(b)                                     # plain copy of 2*8*N bytes (counter scale)
+        for k in ("read", "write", "copy"):            # the library's stream kernels, 8*N bytes each way
+            aar_hip.stream_bandwidth(k, 8 * N, 2, dst=y, src=b)
         aar_hip.AAR(x, b, shape, a, omega, beta, m, p, 24, 1e-30)
         x.zero_()
         aar_hip.AAR_mixed(x, b, shape, a, omega, beta, m, p, 24, 1e-30)
@@ -434,22 +448,34 @@ def main():
         def fmx(xx, rhs, *args):
             return aar_hip.AAR_mixed(xx, rhs, *args, sigma=sigma)
 
+        # the exact solution of the discrete system (device FFT)
+        ue = P.spectral_solve_torch(bn, shape, a, sigma)
+        nue = float(torch.linalg.norm(ue))
+
+        def err_exact():
+            xx = x - x.mean() if sigma == 0.0 else x
+            return float(torch.linalg.norm(xx - ue)) / nue
+
         t64t, st64t = timed(f64, bn, 1e-10, 1, f"fp64 1e-10 sigma={sigma}")
         x64t = x.clone()
+        e64 = err_exact()
         tmxt, stmxt = timed(fmx, bn, 1e-10, 1, f"mixed 1e-10 sigma={sigma}")
+        emx = err_exact()
         r = bn - aar_hip.Lap_Vec_mult(x, shape, a, sigma)
         rel = float(torch.linalg.norm(r) / torch.linalg.norm(bn))
         diff = diff_mod_mean(x, x64t) if sigma == 0.0 else float(torch.linalg.norm(x - x64t) /
                                                                  torch.linalg.norm(x64t))
+        del ue, x64t
+        log(f"1e-10 sigma={sigma}: error vs exact fp64 {e64:.3e} mixed {emx:.3e}")
         return {"rhs": "N(0,1) seed 1003" + (", mean removed" if sigma == 0.0 else ""), "sigma": sigma,
                 "fp64_solves_per_s": 1.0 / t64t, "fp64_iterations": st64t["iterations"],
                 "mixed_solves_per_s": 1.0 / tmxt, "mixed_iterations": stmxt["iterations"],
                 "speedup_vs_fp64": t64t / tmxt, "mixed_relres_true": rel,
+                "fp64_err_vs_exact": e64, "mixed_err_vs_exact": emx,
                 "mixed_rel_diff_vs_fp64" + ("_mod_mean" if sigma == 0.0 else ""): diff}
 
     tol10 = pair_1e10(0.0)
     tol10_screened = pair_1e10(0.5)
-    del x64
 
     # the comparison solvers on the same 512^3 system (one solve each; GPU
     # time of the solve from the library's events)
@@ -541,12 +567,47 @@ def main():
             traffic = json.load(fp).get("bytes_per_launch")
 
     log(f"roofline: stencil_aar_rec {achieved:.0f} GB/s; matvec {mv_gbs:.0f} GB/s")
+
+    # attainable HBM bandwidth on this device: streaming kernels with 16-byte
+    # non-temporal accesses over 4 GiB (read-only, write-only, copy)
+    del x64
+    sb = torch.empty(4 << 27, dtype=torch.float64, device=dev)       # 4 GiB
+    sb.zero_()
+    sd = torch.empty_like(sb)
+    attainable = {k: aar_hip.stream_bandwidth(k, 4 << 30, 10, dst=sd, src=sb) for k in ("read", "write", "copy")}
+    del sb, sd
+    log(f"attainable (stream kernels, 4 GiB): {attainable}")
+
+    # the whole FP64 solve against the roofline: the library's algorithmic
+    # byte count of the solve (per Anderson cycle at m = 7, p = 6: 5
+    # recurrence sweeps x 2 vectors, the Gram 8, the anchor update 13, the
+    # true residual 3; 34 vectors) over the solve's wall time
+    solve_gbs = st64["bytes"] / t64 / 1e9
+
+    # GPU FP64 AAR at the CPU baseline's sizes (same problem scheme)
+    gpu_rates = {}
+    for side in (128, 256):
+        sh = (side,) * 3
+        bs_ = P.rhs_uniform_torch(sh, 1002, device=dev)
+        xs_ = torch.zeros_like(bs_)
+        aar_hip.AAR(xs_, bs_, sh, a, omega, beta, m, p, max_iter, args.tol)
+        ts = []
+        for _ in range(3):
+            xs_.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            aar_hip.AAR(xs_, bs_, sh, a, omega, beta, m, p, max_iter, args.tol)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        gpu_rates[side] = 1.0 / sorted(ts)[1]
+        del bs_, xs_
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(args.cpu_n, args.cpu_iters, st64["iterations"], n)
+            cpu = cpu_baseline(gpu_rates, n, st64["iterations"], args.cpu_ranks or None)
         except Exception as e:  # noqa: BLE001
             log(f"cpu baseline failed: {e}")
+            cpu = {"error": str(e), "value": None}
 
     def ktab(kk):
         return {k: {"count": v["count"], "avg_ms": (v["ms"] / v["count"]) if v["count"] else 0.0,
@@ -594,9 +655,15 @@ def main():
         "kernels": ktab(kern),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "attainable": attainable,
+                     "frac_of_attainable_read": achieved / attainable["read"],
                      "kernel": "stencil_aar_rec (FD12 stencil with the Richardson step folded in: "
                                "g' = g - (omega/d) A g; algorithmic 2x8 B/pt)",
                      "avg_ms": avg_ms, "bytes_per_launch": bytes_per_launch},
+        "solve_roofline": {"algorithmic_bytes": st64["bytes"], "seconds": t64, "GBps": solve_gbs,
+                           "frac": solve_gbs / HBM_PEAK_GBS,
+                           "byte_model": "per Anderson cycle (m=7, p=6): 5 recurrence sweeps x 2 vectors, "
+                                         "Gram 8, anchor update 13, true residual 3 = 34 vectors of 8 B/pt"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -22,7 +22,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("AAR_HIP_ARCH", "gfx950")
 
 SOURCES = ["stencil.hip", "stencil_vc.hip", "history.hip", "helmholtz.hip", "batch.hip", "solver.hip", "krylov.hip",
-           "capi.hip"]
+           "capi.hip", "stream.hip"]
 
 # -ffp-contract=off: the stencil, the Richardson/Anderson updates and the
 # SVD must not fuse multiply-adds, so that they round exactly as the
@@ -45,7 +45,7 @@ def _compile(src):
 
 def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=min(4, len(SOURCES))) as ex:
+    with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs

@@ -185,6 +185,71 @@ def helmholtz_coefficient_torch(shape, seed, device, modes=8, stream=100):
     return (1.0 + 0.5 * acc).reshape(-1)
 
 
+def lap_weights(fdn=6):
+    """1D FD weights of order 2*fdn (MPI_REAL/system.c:480-498): c0 =
+    -2 sum 1/p^2, c_p = 2 (-1)^(p+1) prod_{fdn-p+1..fdn} i / prod_{fdn+1..fdn+p} i
+    / p^2, evaluated in that order (so the doubles equal the reference's)."""
+    c = [0.0] * (fdn + 1)
+    for p in range(1, fdn + 1):
+        c[0] += -(2.0 / (p * p))
+    for p in range(1, fdn + 1):
+        num = den = 1.0
+        for i in range(fdn - p + 1, fdn + 1):
+            num *= i
+        for i in range(fdn + 1, fdn + p + 1):
+            den *= i
+        c[p] = 2.0 * (1.0 if p % 2 else -1.0) * (num / den) / (p * p)
+    return c
+
+
+def spectral_solve_torch(b, shape, a, sigma=0.0, fdn=6):
+    """The exact solution of (a*Lap_FD + sigma) u = b on the periodic grid,
+    by a device FFT: every grid Fourier mode is an eigenvector of the
+    operator, with eigenvalue a*(3 c0 + sum_d 2 sum_p c_p cos(p theta_d)) +
+    sigma.  For sigma = 0 the zero mode is dropped (the solution of zero
+    mean; b must then have zero mean).  b: float64 device tensor."""
+    import torch
+    nx, ny, nz = (int(s) for s in shape)
+    coeff = lap_weights(fdn)
+    dev = b.device
+
+    def sym(n, half=False):
+        th = 2.0 * math.pi * torch.arange(n // 2 + 1 if half else n, dtype=torch.float64, device=dev) / n
+        s = torch.zeros_like(th)
+        for p in range(1, fdn + 1):
+            s += 2.0 * coeff[p] * torch.cos(p * th)
+        return s
+
+    sx, sy, sz = sym(nx, True), sym(ny), sym(nz)
+    lam = a * (3.0 * coeff[0] + sx[None, None, :] + sy[None, :, None] + sz[:, None, None]) + sigma
+    B = torch.fft.rfftn(b.reshape(nz, ny, nx))
+    if sigma == 0.0:
+        lam[0, 0, 0] = 1.0
+        B[0, 0, 0] = 0.0
+    B /= lam
+    del lam
+    return torch.fft.irfftn(B, s=(nz, ny, nx)).reshape(-1)
+
+
+def batch_instances(seed=1004):
+    """The uneven batch: 96 screened-Poisson instances whose cubic side is
+    drawn from {24, 32, 48, 64, 96, 128} and sigma_i ~ U[0.1, 2], then 16
+    right-hand sides sharing one 128^3 operator (one sigma).  Returns
+    [(shape, sigma, rhs_stream)]; instance k's RHS is N(0,1) from stream
+    100 + k of `seed`."""
+    sides = (24, 32, 48, 64, 96, 128)
+    u = uniform01(3 * 96 + 1, seed, 7)
+    inst = []
+    for i in range(96):
+        side = sides[min(5, int(u[2 * i] * 6))]
+        sigma = 0.1 + 1.9 * float(u[2 * i + 1])
+        inst.append(((side,) * 3, sigma, 100 + i))
+    sig16 = 0.1 + 1.9 * float(u[-1])
+    for r in range(16):
+        inst.append(((128,) * 3, sig16, 100 + 96 + r))
+    return inst
+
+
 # Parameter sets of the measured configurations (FD12, m=7, p=6 unless said).
 C1 = dict(shape=(48, 48, 48), h=0.5, omega=0.6, beta=0.6, m=7, p=6, tol=1e-6, max_iter=1000,
           seed=1001)

@@ -399,5 +399,27 @@ def profile_query(name):
     return {"count": c.value, "ms": ms.value, "bytes": by.value}
 
 
+def stream_bandwidth(kind, nbytes=4 << 30, reps=10, dst=None, src=None):
+    """Attainable HBM bandwidth (GB/s) of a streaming kernel with 16-byte
+    non-temporal accesses: kind "read", "write" or "copy" over nbytes
+    (buffers allocated here unless given as device tensors)."""
+    import torch
+    k = {"read": 0, "write": 1, "copy": 2}[kind]
+    nbytes = int(nbytes) // 16 * 16
+    if src is None and k != 1:
+        src = torch.zeros(nbytes // 8, dtype=torch.float64, device="cuda")
+    if dst is None and k != 0:
+        dst = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda")
+    for t in (src, dst):
+        if t is not None and (not t.is_cuda or t.numel() * t.element_size() < nbytes):
+            raise ValueError("stream buffers must be device tensors of at least nbytes")
+    _bind_stream()
+    g = ctypes.c_double()
+    check(_L().aar_hip_stream_bandwidth(k, ctypes.c_void_p(dst.data_ptr() if dst is not None else 0),
+                                        ctypes.c_void_p(src.data_ptr() if src is not None else 0),
+                                        nbytes, int(reps), ctypes.byref(g)))
+    return g.value
+
+
 def release():
     _L().aar_hip_release()

@@ -182,6 +182,16 @@ void report_failure(const char *name, int rc)
     std::fflush(stdout);
 }
 
+// One HIP call of a host-array entry point: a failure is recorded
+// (aar_hip_last_error) and printed as the entry's failure line.
+bool hip_ok(hipError_t e, const char *name, const char *what)
+{
+    if (e == hipSuccess) return true;
+    fail(AAR_HIP_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    report_failure(name, AAR_HIP_ERR_HIP);
+    return false;
+}
+
 // run one of the four solvers on host arrays
 template <class F>
 void host_solve(const char *name, POISSON *sys, aar_op_cb opcb, aar_pc_cb pccb, double a, double *x,
@@ -212,7 +222,12 @@ void host_solve(const char *name, POISSON *sys, aar_op_cb opcb, aar_pc_cb pccb, 
         return;
     }
     debug_trace(name, st);
-    if (rc == AAR_HIP_ERR_SVD) std::printf("no convergence in %d svd iterations \n", 250);
+    if (rc == AAR_HIP_ERR_SVD) {
+        // tools.c:268-271: the reference prints this and ends the process
+        std::printf("no convergence in %d svd iterations \n", 250);
+        std::fflush(stdout);
+        std::exit(1);
+    }
     report(name, st);
 }
 
@@ -296,12 +311,13 @@ int step_gram_ls(int kind, const StepBufs &sb, int m, int n)
     return 0;
 }
 
-void step_outputs(int m, double **FtF, double *allredvec, double *Ftf, double *svec)
+hipError_t step_outputs(int m, double **FtF, double *allredvec, double *Ftf, double *svec)
 {
     Context &C = ctx();
     Ctrl &hc = C.ctrl_h[1];
-    (void)hipMemcpyAsync(&hc, C.ctrl + 1, sizeof(Ctrl), hipMemcpyDeviceToHost, C.stream);
-    (void)hipStreamSynchronize(C.stream);
+    hipError_t e = hipMemcpyAsync(&hc, C.ctrl + 1, sizeof(Ctrl), hipMemcpyDeviceToHost, C.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(C.stream);
+    if (e != hipSuccess) return e;
     // AAR.c:212-221: FtF[i][j] = allredvec[j*m + i]; Ftf[j] = allredvec[m*m + j]
     if (allredvec) std::memcpy(allredvec, hc.G, sizeof(double) * (m * m + m));
     if (FtF)
@@ -311,7 +327,13 @@ void step_outputs(int m, double **FtF, double *allredvec, double *Ftf, double *s
         for (int j = 0; j < m; j++) Ftf[j] = hc.G[m * m + j];
     if (svec)
         for (int j = 0; j < m; j++) svec[j] = hc.s[j];
-    if (hc.status == 1) std::printf("no convergence in %d svd iterations \n", 250);
+    if (hc.status == 1) {
+        // tools.c:268-271 ends the process when the SVD does not converge
+        std::printf("no convergence in %d svd iterations \n", 250);
+        std::fflush(stdout);
+        std::exit(1);
+    }
+    return hipSuccess;
 }
 
 }  // namespace
@@ -491,22 +513,32 @@ void Lap_Vec_mult(POISSON *system, double a, double *phi, double *Lap_phi, aar_m
     if (int rc = stage(op.n)) { report_failure("Lap_Vec_mult", rc); return; }
     Context &C = ctx();
     const size_t bytes = sizeof(double) * (size_t)op.n;
-    if (hipMemcpyAsync(g_stage.x, phi, bytes, hipMemcpyHostToDevice, C.stream) != hipSuccess) return;
-    if (int rc = matvec<double>(op, g_stage.x, g_stage.b)) { report_failure("Lap_Vec_mult", rc); return; }
-    (void)hipMemcpyAsync(Lap_phi, g_stage.b, bytes, hipMemcpyDeviceToHost, C.stream);
-    (void)hipStreamSynchronize(C.stream);
+    const char *nm = "Lap_Vec_mult";
+    if (!hip_ok(hipMemcpyAsync(g_stage.x, phi, bytes, hipMemcpyHostToDevice, C.stream), nm, "H2D copy")) return;
+    if (int rc = matvec<double>(op, g_stage.x, g_stage.b)) { report_failure(nm, rc); return; }
+    if (!hip_ok(hipMemcpyAsync(Lap_phi, g_stage.b, bytes, hipMemcpyDeviceToHost, C.stream), nm, "D2H copy")) return;
+    (void)hip_ok(hipStreamSynchronize(C.stream), nm, "synchronize");
+}
+
+// res /= d on the GPU, in place on host memory (Np doubles); c = nullptr:
+// d = diag, else d = diag + c(x) pointwise (device field)
+static void host_divide(const char *nm, double diag, const double *c, double *res, int Np)
+{
+    if (Np <= 0) return;
+    if (int rc = stage(Np)) { report_failure(nm, rc); return; }
+    Context &C = ctx();
+    const size_t bytes = sizeof(double) * (size_t)Np;
+    if (!hip_ok(hipMemcpyAsync(g_stage.x, res, bytes, hipMemcpyHostToDevice, C.stream), nm, "H2D copy")) return;
+    const hipError_t e = c ? launch_div_vc(g_stage.x, diag, c, Np, C.stream) : launch_div_f64(g_stage.x, diag, Np, C.stream);
+    if (!hip_ok(e, nm, "launch")) return;
+    if (!hip_ok(hipMemcpyAsync(res, g_stage.x, bytes, hipMemcpyDeviceToHost, C.stream), nm, "D2H copy")) return;
+    (void)hip_ok(hipStreamSynchronize(C.stream), nm, "synchronize");
 }
 
 // MPI_REAL/system.c:222-227: res /= diag, on the GPU
 void Precondition(double diag, double *res, int Np)
 {
-    if (stage(Np)) return;
-    Context &C = ctx();
-    const size_t bytes = sizeof(double) * (size_t)Np;
-    (void)hipMemcpyAsync(g_stage.x, res, bytes, hipMemcpyHostToDevice, C.stream);
-    (void)launch_div_f64(g_stage.x, diag, Np, C.stream);
-    (void)hipMemcpyAsync(res, g_stage.x, bytes, hipMemcpyDeviceToHost, C.stream);
-    (void)hipStreamSynchronize(C.stream);
+    host_divide("Precondition", diag, nullptr, res, Np);
 }
 
 // Variable-coefficient operator slots (no MPI_REAL counterpart: the
@@ -521,10 +553,11 @@ void Lap_Vec_mult_vc(POISSON *system, double a, double *phi, double *Lap_phi, aa
     if (int rc = stage(op.n)) { report_failure("Lap_Vec_mult_vc", rc); return; }
     Context &C = ctx();
     const size_t bytes = sizeof(double) * (size_t)op.n;
-    if (hipMemcpyAsync(g_stage.x, phi, bytes, hipMemcpyHostToDevice, C.stream) != hipSuccess) return;
-    if (int rc = matvec<double>(op, g_stage.x, g_stage.b)) { report_failure("Lap_Vec_mult_vc", rc); return; }
-    (void)hipMemcpyAsync(Lap_phi, g_stage.b, bytes, hipMemcpyDeviceToHost, C.stream);
-    (void)hipStreamSynchronize(C.stream);
+    const char *nm = "Lap_Vec_mult_vc";
+    if (!hip_ok(hipMemcpyAsync(g_stage.x, phi, bytes, hipMemcpyHostToDevice, C.stream), nm, "H2D copy")) return;
+    if (int rc = matvec<double>(op, g_stage.x, g_stage.b)) { report_failure(nm, rc); return; }
+    if (!hip_ok(hipMemcpyAsync(Lap_phi, g_stage.b, bytes, hipMemcpyDeviceToHost, C.stream), nm, "D2H copy")) return;
+    (void)hip_ok(hipStreamSynchronize(C.stream), nm, "synchronize");
 }
 
 // res /= (diag + c(x)) pointwise with the registered c(x), on the GPU
@@ -533,13 +566,7 @@ void Precondition_vc(double diag, double *res, int Np)
 {
     const double *c = nullptr;
     if (int rc = registered_coefficient(Np, &c)) { report_failure("Precondition_vc", rc); return; }
-    if (stage(Np)) return;
-    Context &C = ctx();
-    const size_t bytes = sizeof(double) * (size_t)Np;
-    (void)hipMemcpyAsync(g_stage.x, res, bytes, hipMemcpyHostToDevice, C.stream);
-    (void)launch_div_vc(g_stage.x, diag, c, Np, C.stream);
-    (void)hipMemcpyAsync(res, g_stage.x, bytes, hipMemcpyDeviceToHost, C.stream);
-    (void)hipStreamSynchronize(C.stream);
+    host_divide("Precondition_vc", diag, c, res, Np);
 }
 
 // MPI_REAL/tools.c:345-352
@@ -597,24 +624,24 @@ void PseudoInverseTimesVec(double **Am, double *b, double *x, int m)
     }
 }
 
-static double host_dot(const double *u, const double *v, int len)
+// sum of u*v (v = nullptr: u*u) over host arrays, on the GPU with the
+// library's deterministic two-pass reduction; 0 or an AAR_HIP_ERR_* code
+static int host_dot(const char *nm, const double *u, const double *v, int len, double *out)
 {
-    if (len <= 0) return 0.0;
-    if (stage(len)) return NAN;
+    *out = 0.0;
+    if (len <= 0) return 0;
+    if (int rc = stage(len)) { report_failure(nm, rc); return rc; }
     Context &C = ctx();
     const size_t bytes = sizeof(double) * (size_t)len;
-    (void)hipMemcpyAsync(g_stage.x, u, bytes, hipMemcpyHostToDevice, C.stream);
-    if (v) (void)hipMemcpyAsync(g_stage.b, v, bytes, hipMemcpyHostToDevice, C.stream);
+    if (!hip_ok(hipMemcpyAsync(g_stage.x, u, bytes, hipMemcpyHostToDevice, C.stream), nm, "H2D copy"))
+        return AAR_HIP_ERR_HIP;
+    if (v && !hip_ok(hipMemcpyAsync(g_stage.b, v, bytes, hipMemcpyHostToDevice, C.stream), nm, "H2D copy"))
+        return AAR_HIP_ERR_HIP;
     static DevBuf part;
     const int nb = (int)std::max(1L, std::min(2048L, ((long)len + 4095) / 4096));
-    if (part.bytes < sizeof(double) * nb) {
-        if (part.p) (void)hipFree(part.p);
-        part.p = nullptr;
-        if (hipMalloc(&part.p, sizeof(double) * nb) != hipSuccess) return NAN;
-        part.bytes = sizeof(double) * nb;
-    }
+    if (!hip_ok(ensure(part, sizeof(double) * nb), nm, "allocation")) return AAR_HIP_ERR_HIP;
     Ctrl *dc = C.ctrl + 1;
-    (void)launch_zero_ctrl(dc, C.stream);
+    if (!hip_ok(launch_zero_ctrl(dc, C.stream), nm, "launch")) return AAR_HIP_ERR_HIP;
     DotArgs<double> d;
     d.n = len;
     d.v = g_stage.x;
@@ -624,23 +651,28 @@ static double host_dot(const double *u, const double *v, int len)
     d.counter = C.counters + 40;
     d.ctrl = dc;
     d.fin.fin = FIN_SUMS;
-    (void)launch_dot<double>(d, C.stream);
+    if (!hip_ok(launch_dot<double>(d, C.stream), nm, "launch")) return AAR_HIP_ERR_HIP;
     Ctrl &hc = C.ctrl_h[1];
-    (void)hipMemcpyAsync(&hc, dc, sizeof(Ctrl), hipMemcpyDeviceToHost, C.stream);
-    (void)hipStreamSynchronize(C.stream);
-    return hc.red[0];
+    if (!hip_ok(hipMemcpyAsync(&hc, dc, sizeof(Ctrl), hipMemcpyDeviceToHost, C.stream), nm, "D2H copy"))
+        return AAR_HIP_ERR_HIP;
+    if (!hip_ok(hipStreamSynchronize(C.stream), nm, "synchronize")) return AAR_HIP_ERR_HIP;
+    *out = hc.red[0];
+    return 0;
 }
 
-// MPI_REAL/tools.c:18-28
+// MPI_REAL/tools.c:18-28 (the sum in a fixed order on the GPU, then sqrt);
+// on a failure the line is printed and *ResVal is NaN
 void Vector2Norm(double *Vec, int len, double *ResVal, aar_mpi_comm)
 {
-    *ResVal = sqrt(host_dot(Vec, nullptr, len));
+    double s;
+    *ResVal = host_dot("Vector2Norm", Vec, nullptr, len, &s) ? NAN : sqrt(s);
 }
 
 // MPI_REAL/tools.c:35-43
 void VectorDotProduct(double *Vec1, double *Vec2, int len, double *ResVal, aar_mpi_comm)
 {
-    *ResVal = host_dot(Vec1, Vec2, len);
+    double s;
+    *ResVal = host_dot("VectorDotProduct", Vec1, Vec2, len, &s) ? NAN : s;
 }
 
 // MPI_REAL/AAR.c:37-177
@@ -711,17 +743,13 @@ void AndersonExtrapolation(double **DX, double **DF, double *f, double beta, int
     u.fslot = 0;
     u.oslot = 0;   // written after all reads of slot 0 in the same pass (pointwise)
     u.ctrl = C.ctrl + 1;
-    static DevBuf out;
-    if (out.bytes < sizeof(double) * (size_t)N) {
-        if (out.p) (void)hipFree(out.p);
-        out.p = nullptr;
-        if (hipMalloc(&out.p, sizeof(double) * (size_t)N) != hipSuccess) return;
-        out.bytes = sizeof(double) * (size_t)N;
-    }
-    (void)launch_aar_update<double>(u, C.stream);
+    const char *nm = "AndersonExtrapolation";
+    if (!hip_ok(launch_aar_update<double>(u, C.stream), nm, "launch")) return;
     std::vector<double> tmp(N);
-    (void)hipMemcpyAsync(tmp.data(), sb.X, sizeof(double) * (size_t)N, hipMemcpyDeviceToHost, C.stream);
-    step_outputs(m, FtF, allredvec, Ftf, svec);
+    if (!hip_ok(hipMemcpyAsync(tmp.data(), sb.X, sizeof(double) * (size_t)N, hipMemcpyDeviceToHost, C.stream), nm,
+                "D2H copy"))
+        return;
+    if (!hip_ok(step_outputs(m, FtF, allredvec, Ftf, svec), nm, "D2H copy")) return;
     for (int k = 0; k < N; k++) am_vec[k] = -tmp[k];
 }
 
@@ -733,7 +761,7 @@ void Galerkin_Richardson(double **DX, double **DF, double *f, int m, int Np, dou
     StepBufs sb;
     if (int rc = step_upload(DX, DF, f, m, Np, sb)) { report_failure("Galerkin_Richardson", rc); return; }
     if (int rc = step_gram_ls(1, sb, m, Np)) { report_failure("Galerkin_Richardson", rc); return; }
-    step_outputs(m, XtF, allredvec, Xtf, svec);
+    (void)hip_ok(step_outputs(m, XtF, allredvec, Xtf, svec), "Galerkin_Richardson", "D2H copy");
 }
 
 // MPI_REAL/PL2R.c:180-218
@@ -744,7 +772,7 @@ void L2_Richardson(double **DF, double *f, int m, int Np, double **FtF, double *
     StepBufs sb;
     if (int rc = step_upload(nullptr, DF, f, m, Np, sb)) { report_failure("L2_Richardson", rc); return; }
     if (int rc = step_gram_ls(2, sb, m, Np)) { report_failure("L2_Richardson", rc); return; }
-    step_outputs(m, FtF, allredvec, Ftf, svec);
+    (void)hip_ok(step_outputs(m, FtF, allredvec, Ftf, svec), "L2_Richardson", "D2H copy");
 }
 
 // ===========================================================================

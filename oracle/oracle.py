@@ -442,10 +442,11 @@ _ITER_RE = re.compile(r"(AAR|PGR|PL2R|CG)(?: converged!)?:\s+Iterations = (\d+),
 
 
 def ref_solve(solver, shape, rhs, x0, a, sigma=0.0, omega=0.6, beta=0.6, m=7, p=6,
-              max_iter=1000, tol=1e-6, precond=True, ranks=1, timeout=600):
+              max_iter=1000, tol=1e-6, precond=True, ranks=1, timeout=600, trace=True):
     """Run the reference solver (one of aar, pgr, pl2r, cg).  Returns
     (x, dict(iterations, relres_printed, trace, stdout, converged_line,
-    warning_line, seconds))."""
+    warning_line, seconds)).  trace=False: no residual recording (an empty
+    trace), so `seconds` times the reference solver alone."""
     if not ref_available():
         raise RuntimeError("reference harness not built (oracle/_ref/ref_harness)")
     n = int(np.prod(shape))
@@ -454,13 +455,13 @@ def ref_solve(solver, shape, rhs, x0, a, sigma=0.0, omega=0.6, beta=0.6, m=7, p=
         prob = os.path.join(tmp, "p.bin")
         write_problem(prob, shape, rhs, x0, a, sigma, omega, beta, m, p, max_iter, tol, precond)
         xo, to = os.path.join(tmp, "x.bin"), os.path.join(tmp, "t.bin")
-        cmd = [REF_HARNESS, "solve", prob, solver, xo, to]
+        cmd = [REF_HARNESS, "solve", prob, solver, xo, to if trace else "-"]
         if ranks > 1:
             cmd = [MPIRUN, "-n", str(ranks)] + cmd
         out = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, check=True).stdout
         x = np.fromfile(xo, dtype=np.float64)
         assert x.size == n
-        tr = np.fromfile(to, dtype=np.float64)
+        tr = np.fromfile(to, dtype=np.float64) if trace else np.zeros(0)
         mm = _ITER_RE.search(out)
         hs = re.search(r"HARNESS .* seconds=([0-9.]+)", out)
         return x, {
@@ -518,6 +519,28 @@ def ref_pinv(A, b):
             fp.write(_f64(b).tobytes())
         subprocess.run([REF_HARNESS, "pinv", i, o], capture_output=True, timeout=60, check=True)
         return np.fromfile(o, dtype=np.float64)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def ref_step(kind, DX, DF, f, beta=0.0):
+    """The reference's stand-alone history steps: kind "anderson"
+    (AAR.h:24-25) returns (am, allredvec); "galerkin" (PGR.h:24-25) and
+    "l2" (PL2R.h:24-25) return (svec, allredvec)."""
+    DX, DF, f = _f64(DX), _f64(DF), _f64(f)
+    m, n = DF.shape
+    tmp = tempfile.mkdtemp(prefix="aarref_")
+    try:
+        i, o = os.path.join(tmp, "in.bin"), os.path.join(tmp, "out.bin")
+        with open(i, "wb") as fp:
+            fp.write(struct.pack("<iid", m, n, beta))
+            fp.write(DX.tobytes())
+            fp.write(DF.tobytes())
+            fp.write(f.tobytes())
+        subprocess.run([REF_HARNESS, kind, i, o], capture_output=True, timeout=60, check=True)
+        out = np.fromfile(o, dtype=np.float64)
+        k = n if kind == "anderson" else m
+        return out[:k], out[k:]
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
 

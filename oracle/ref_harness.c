@@ -18,6 +18,7 @@
  *   ref_harness init   <nx> <ny> <nz> <rhs_out>     (the reference RHS of system.c:425-471)
  *   ref_harness pinv   <in.bin> <out.bin>           (rank 0 only)
  *   ref_harness anderson <in.bin> <out.bin>         (rank 0 only)
+ *   ref_harness galerkin|l2 <in.bin> <out.bin>      (rank 0 only)
  *
  * problem.bin: "AARPRB01", int32 nx ny nz fdn m p max_iter precond,
  *              float64 a sigma omega beta tol, float64 rhs[N], x0[N].
@@ -177,7 +178,8 @@ static int mode_solve(int argc, char **argv, int rank)
     scatter_local(sys, x0_g, sys->phi);
     g_sigma = h.sigma; g_rhs_local = sys->rhs; g_iterate = sys->phi; g_np = np;
     g_trace_cap = h.max_iter + 8;
-    g_trace = (double *)calloc((size_t)g_trace_cap, sizeof(double));
+    /* trace_out "-": no recording, so the timed solve is the reference's alone */
+    g_trace = strcmp(argv[5], "-") ? (double *)calloc((size_t)g_trace_cap, sizeof(double)) : NULL;
     void (*pc)(double, double *, int) = h.precond ? rec_precond : NULL;
     const char *sv = argv[3];
     double t0 = MPI_Wtime();
@@ -258,8 +260,11 @@ static int mode_pinv(int argc, char **argv)
 }
 
 /* anderson in.bin: int32 m, int32 n, float64 beta, DX[m][n], DF[m][n], f[n];
- * out.bin: am[n], allredvec[m*m+m] */
-static int mode_anderson(int argc, char **argv)
+ * out.bin: am[n], allredvec[m*m+m].
+ * galerkin / l2 (the same input; beta unused, DX unused by l2): the PGR
+ * and PL2R steps (PGR.h:24-25, PL2R.h:24-25); out.bin: svec[m],
+ * allredvec[m*m+m]. */
+static int mode_anderson(int argc, char **argv, int kind)
 {
     if (argc != 4) return 2;
     FILE *fp = fopen(argv[2], "rb");
@@ -286,10 +291,16 @@ static int mode_anderson(int argc, char **argv)
     double *red = (double *)calloc((size_t)(m * m + m), sizeof(double));
     double *Ftf = (double *)calloc((size_t)m, sizeof(double));
     double *svec = (double *)calloc((size_t)m, sizeof(double));
-    AndersonExtrapolation(DX, DF, f, beta, m, n, am, FtF, red, Ftf, svec, MPI_COMM_WORLD);
+    if (kind == 0)
+        AndersonExtrapolation(DX, DF, f, beta, m, n, am, FtF, red, Ftf, svec, MPI_COMM_WORLD);
+    else if (kind == 1)
+        Galerkin_Richardson(DX, DF, f, m, n, FtF, red, Ftf, svec, MPI_COMM_WORLD);
+    else
+        L2_Richardson(DF, f, m, n, FtF, red, Ftf, svec, MPI_COMM_WORLD);
     FILE *fo = fopen(argv[3], "wb");
     if (!fo) return 3;
-    fwrite(am, sizeof(double), n, fo);
+    if (kind == 0) fwrite(am, sizeof(double), n, fo);
+    else fwrite(svec, sizeof(double), m, fo);
     fwrite(red, sizeof(double), m * m + m, fo);
     fclose(fo);
     return 0;
@@ -305,7 +316,9 @@ int main(int argc, char **argv)
         else if (!strcmp(argv[1], "matvec")) rc = mode_matvec(argc, argv, rank);
         else if (!strcmp(argv[1], "init")) rc = mode_init(argc, argv, rank);
         else if (!strcmp(argv[1], "pinv")) rc = rank == 0 ? mode_pinv(argc, argv) : 0;
-        else if (!strcmp(argv[1], "anderson")) rc = rank == 0 ? mode_anderson(argc, argv) : 0;
+        else if (!strcmp(argv[1], "anderson")) rc = rank == 0 ? mode_anderson(argc, argv, 0) : 0;
+        else if (!strcmp(argv[1], "galerkin")) rc = rank == 0 ? mode_anderson(argc, argv, 1) : 0;
+        else if (!strcmp(argv[1], "l2")) rc = rank == 0 ? mode_anderson(argc, argv, 2) : 0;
     }
     if (rc && !rank) fprintf(stderr, "ref_harness: mode failed (rc=%d)\n", rc);
     MPI_Finalize();

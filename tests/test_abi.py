@@ -100,3 +100,23 @@ def test_reference_driver_compiles_against_include(tmp_path):
         pytest.skip("reference tree or MPICH not present")
     subprocess.run(["gcc", "-c", ref_main, "-o", str(tmp_path / "main.o"), f"-I{INCLUDE}",
                     f"-I{MPI_PREFIX}/include"], check=True, capture_output=True)
+
+
+def test_dropin_wrong_argc_matches_reference():
+    """The reference driver (MPI_REAL/main.c) relinked with libaar_hip and
+    given a wrong argument count: CheckInputs (system.c:37-39) prints the
+    reference's banner and "Wrong inputs" and ends with its exit status, as
+    the reference build does (tests/golden/golden_edge.json).  No GPU call
+    happens before CheckInputs."""
+    import json
+    exe = os.path.join(REPO, "oracle", "_ref", "main_on_libaar")
+    if not os.path.exists(exe):
+        pytest.skip("drop-in driver not built (make -C oracle dropin)")
+    with open(os.path.join(REPO, "tests", "golden", "golden_edge.json")) as fp:
+        ref = json.load(fp)["argc"]
+    for argc, r in ref.items():
+        out = subprocess.run([exe] + r["args"], capture_output=True, text=True, timeout=120)
+        got = re.sub(r"Starting time: .*", "Starting time: <masked>", out.stdout)
+        assert out.returncode == r["rc"], argc
+        assert got == r["stdout"], argc
+

@@ -1,9 +1,12 @@
 """GPU parity of libaar_hip against the oracle and the reference's fixtures.
 
 Bar (north star): iteration counts equal to the reference's; solution
-relative difference <= 1e-8 (AAR, PGR, PL2R) at the reference tolerance;
-CG within 1e-7 (the reference itself moves its CG solution by 6e-9 between
-1 and 8 ranks).  The stencil and the pseudo-inverse are bit-exact.
+relative difference <= 1e-8 at the reference tolerance, for all four
+solvers.  CG is the tightest: on the 48^3 fixture the GPU's solution is
+7.3e-9 from the reference's, next to the reference's own 6.0e-9 between 1
+and 8 ranks (golden.json rank8_rel_diff); the GPU's reductions are
+deterministic, so that value does not vary between runs.  The stencil and
+the pseudo-inverse are bit-exact.
 """
 import os
 import subprocess
@@ -16,7 +19,7 @@ from test_oracle import SMALL_CASES, case_inputs, fft_solve, run_oracle
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"aar": 1e-8, "pgr": 1e-8, "pl2r": 1e-8, "cg": 1e-7}
+TOL = {"aar": 1e-8, "pgr": 1e-8, "pl2r": 1e-8, "cg": 1e-8}
 
 
 def say(*a):
@@ -129,6 +132,7 @@ def test_solver_parity(gpu, oracle, golden, name, solver):
     if nr == 0.0:
         assert np.linalg.norm(x) == 0.0
     else:
+        say(f"  solution rel diff {np.linalg.norm(x - xr) / nr:.3e}")
         assert np.linalg.norm(x - xr) / nr <= TOL[solver]
     # residual histories agree with the reference's (rounding-level drift only)
     h = gpu.history()
