@@ -247,6 +247,18 @@ int aar_hip_helm_grad_c(long n, const double *lambda, const double *u, double *d
  * w_out, v_out may be NULL. */
 int aar_hip_pinv(const double *A, const double *b, double *x, int m, double *w_out, double *v_out);
 
+/* No counterpart in the reference: aar_hip_pinv for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops).  DEVICE pointers: A (m x m, row-major), b and x (m doubles), and
+ * optionally w (m: truncated singular values), V (m x m) and status (one
+ * int: 0, or 1 if the SVD did not converge in 250 sweeps); m <= 32.  The
+ * work is enqueued on the library stream (the one given to
+ * aar_hip_set_stream, or the null stream before the library is set up);
+ * the call allocates nothing, copies nothing to the host and does not wait.
+ * Its work area is one static device buffer per GPU (18 KB, held until the
+ * process ends), so calls that overlap on one GPU race on it. */
+int aar_hip_pinv_noalloc(const double *A, const double *b, double *x, int m, double *w, double *V, int *status);
+
 /* Residual history of the last solve: ||b - A x_k|| for every iterate
  * (AAR: entry k is the residual of the iterate entering iteration k+1;
  * PGR/PL2R: of the Richardson iterate before projection; CG: of r_k).

@@ -22,7 +22,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("AAR_HIP_ARCH", "gfx950")
 
 SOURCES = ["stencil.hip", "stencil_vc.hip", "history.hip", "helmholtz.hip", "batch.hip", "solver.hip", "krylov.hip",
-           "capi.hip", "stream.hip"]
+           "capi.hip", "stream.hip", "noalloc.hip"]
 
 # -ffp-contract=off: the stencil, the Richardson/Anderson updates and the
 # SVD must not fuse multiply-adds, so that they round exactly as the
