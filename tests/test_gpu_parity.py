@@ -176,7 +176,11 @@ def test_mixed_parity(gpu, oracle, golden, name):
     anchor and normal equations) on the reference's AAR fixtures: the same
     iteration counts and printed conditions; solution within 1e-8 of the
     reference's when converged (1e-6 for an iterate stopped at max_iter,
-    whose FP32 sweep noise is not yet damped)."""
+    whose FP32 sweep noise is not yet damped).  With omega above plain
+    Richardson's stability limit (0.843: the README's omega = 1.0, p = 11)
+    each of the p-1 FP32 recurrence steps between FP64 restarts can grow
+    the FP32 rounding by |1 - omega*2.37| = 1.37, so the converged solution
+    carries ~1.37^10 x more of it: within 5e-8 there (measured 2.3e-8)."""
     import torch
     meta, g = golden
     c, shape, b, x0 = case_inputs(meta, name)
@@ -195,7 +199,8 @@ def test_mixed_parity(gpu, oracle, golden, name):
     if nr == 0.0:
         assert np.linalg.norm(x) == 0.0
     else:
-        tol = 1e-8 if ref["converged_line"] else 1e-6
+        tol = (1e-8 if c["omega"] <= 0.843 else 5e-8) if ref["converged_line"] else 1e-6
+        say(f"  solution rel diff {np.linalg.norm(x - xr) / nr:.3e} (bar {tol:g})")
         assert np.linalg.norm(x - xr) / nr <= tol
     assert xd.dtype == torch.float64
 
