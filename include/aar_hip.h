@@ -29,10 +29,15 @@ extern "C" {
 #define AAR_HIP_ERR_NONFINITE 5  /* residual became NaN */
 
 /* Grid and operator: y = a*Lap_FD(2*fdn)(x) + sigma*x, periodic.
- * fdn must be 6 (the 12th-order, radius-6 stencil of MPI_REAL/main.c:35).
+ * fdn = 1..6: the central FD weights of order 2*fdn that Lap_coefficient
+ * computes (system.c:480-498; MPI_REAL/main.c:35 uses 6, the 12th-order
+ * radius-6 stencil), or AAR_HIP_FDN_USER: the weights registered with
+ * aar_hip_set_stencil.  The sweeps have radius 6; a smaller fdn runs them
+ * with zero weights beyond fdn (the same sums, at the cost of FD12).
  * sigma = 0 is the reference operator (system.c:189-205); sigma != 0 is
  * the screened operator (extension).  The Jacobi diagonal is
  * a*(3*coeff[0]) + sigma (AAR.c:94 for sigma = 0). */
+#define AAR_HIP_FDN_USER (-1)
 typedef struct {
     int nx, ny, nz;
     int fdn;
@@ -99,6 +104,17 @@ int aar_hip_pgr_vc(const aar_hip_grid *g, const double *c, double *x, const doub
                    int p, int max_iter, double tol, int precond, int l2, aar_hip_stats *st);
 int aar_hip_cg_vc(const aar_hip_grid *g, const double *c, double *x, const double *b, int max_iter, double tol,
                   int precond, aar_hip_stats *st);
+/* The device operator behind the reference's callback slots (README:25):
+ * registers a user's 1D stencil weights coeff[0..fdn] (HOST array, fdn =
+ * 1..6), used as y = a*(3*coeff[0]*x + sum_{o=1..fdn} coeff[o]*S_o(x)) with
+ * S_o the sum of the six neighbours at distance o (the form of
+ * system.c:195-201), Jacobi diagonal a*3*coeff[0], by every call whose
+ * grid (or Helmholtz operator) says fdn = AAR_HIP_FDN_USER.  coeff = NULL
+ * clears them.  The reference-signature entry points take the weights the
+ * caller's POISSON holds (FDn, coeff_lap), which may equally be a user's.
+ * ILU(0) (precond = 2) needs the full FD12 stencil. */
+int aar_hip_set_stencil(const double *coeff, int fdn);
+
 /* Registers c(x) -- a HOST array of n doubles, copied to the GPU -- for the
  * drop-in slots Lap_Vec_mult_vc / Precondition_vc (system.h), which the
  * reference's AAR(), PGR(), PL2R() and CG() then run on the GPU; c = NULL

@@ -54,12 +54,12 @@ def _check_len(t, shape, name):
         raise ValueError(f"{name} has {t.numel()} elements, grid {tuple(shape)} needs {n}")
 
 
-def Lap_Vec_mult(x, shape, a, sigma=0.0, out=None):
+def Lap_Vec_mult(x, shape, a, sigma=0.0, out=None, fdn=6):
     """system.c:80-213 -- y = a*Lap(x) (+ sigma*x), FP64 or FP32."""
     import torch
     _check_len(x, shape, "x")
     y = torch.empty_like(x) if out is None else out
-    g = _grid(shape, a, sigma)
+    g = _grid(shape, a, sigma, fdn)
     _bind_stream()
     if x.dtype == torch.float64:
         check(_L().aar_hip_matvec(ctypes.byref(g), _dev(x, torch.float64, "x"),
@@ -72,13 +72,13 @@ def Lap_Vec_mult(x, shape, a, sigma=0.0, out=None):
     return y
 
 
-def AAR(x, rhs, shape, a, omega, beta, m, p, max_iter, tol, sigma=0.0, precond=True):
+def AAR(x, rhs, shape, a, omega, beta, m, p, max_iter, tol, sigma=0.0, precond=True, fdn=6):
     """AAR.c:37-177 (FP64 tensors), or the same iteration in FP32 (FP32 tensors;
     normal equations and the m x m solve stay FP64)."""
     import torch
     _check_len(x, shape, "x")
     _check_len(rhs, shape, "rhs")
-    g = _grid(shape, a, sigma)
+    g = _grid(shape, a, sigma, fdn)
     st = Stats()
     _bind_stream()
     if x.dtype == torch.float64:
@@ -92,13 +92,13 @@ def AAR(x, rhs, shape, a, omega, beta, m, p, max_iter, tol, sigma=0.0, precond=T
     return st.as_dict()
 
 
-def AAR_mixed(x, rhs, shape, a, omega, beta, m, p, max_iter, tol, sigma=0.0, precond=True):
+def AAR_mixed(x, rhs, shape, a, omega, beta, m, p, max_iter, tol, sigma=0.0, precond=True, fdn=6):
     """AAR.c:37-177 in mixed precision (no reference counterpart): FP64 x, b,
     true residual and normal equations; FP32 Richardson sweeps and history."""
     import torch
     _check_len(x, shape, "x")
     _check_len(rhs, shape, "rhs")
-    g = _grid(shape, a, sigma)
+    g = _grid(shape, a, sigma, fdn)
     st = Stats()
     _bind_stream()
     check(_L().aar_hip_aar_mixed(ctypes.byref(g), _dev(x, torch.float64, "x"),
@@ -139,6 +139,20 @@ def AAR_vc(x, rhs, shape, a, c, omega, beta, m, p, max_iter, tol, precond=True, 
     return st.as_dict()
 
 
+FDN_USER = -1   # fdn value selecting the weights registered with set_stencil
+
+
+def set_stencil(coeff):
+    """Registers user 1D stencil weights coeff[0..fdn] (fdn = len - 1 <= 6)
+    for calls made with fdn=FDN_USER: y = a*(3 c0 x + sum_o c_o S_o x);
+    None clears them."""
+    if coeff is None:
+        check(_L().aar_hip_set_stencil(None, 0))
+        return
+    c = np.ascontiguousarray(coeff, dtype=np.float64)
+    check(_L().aar_hip_set_stencil(c.ctypes.data_as(ctypes.c_void_p), int(c.size - 1)))
+
+
 def set_coefficient(c):
     """Registers c(x) (a host numpy array, or None to clear) for the drop-in
     Lap_Vec_mult_vc / Precondition_vc slots."""
@@ -172,13 +186,13 @@ def AAR_batch(instances, omega, beta, m, p, max_iter, tol, precond=True, mixed=F
     return [s.as_dict() for s in sts]
 
 
-def PGR(x, rhs, shape, a, omega, m, p, max_iter, tol, sigma=0.0, precond=True, _l2=False, c=None):
+def PGR(x, rhs, shape, a, omega, m, p, max_iter, tol, sigma=0.0, precond=True, _l2=False, c=None, fdn=6):
     """PGR.c:35-173.  c: a float64 coefficient field (one value per point)
     for the variable-coefficient operator a*Lap + c(x) (then sigma must be 0)."""
     import torch
     _check_len(x, shape, "x")
     _check_len(rhs, shape, "rhs")
-    g = _grid(shape, a, sigma)
+    g = _grid(shape, a, sigma, fdn)
     st = Stats()
     _bind_stream()
     if c is None:
@@ -193,17 +207,17 @@ def PGR(x, rhs, shape, a, omega, m, p, max_iter, tol, sigma=0.0, precond=True, _
     return st.as_dict()
 
 
-def PL2R(x, rhs, shape, a, omega, m, p, max_iter, tol, sigma=0.0, precond=True, c=None):
+def PL2R(x, rhs, shape, a, omega, m, p, max_iter, tol, sigma=0.0, precond=True, c=None, fdn=6):
     """PL2R.c:35-172 (c as in PGR)."""
-    return PGR(x, rhs, shape, a, omega, m, p, max_iter, tol, sigma, precond, _l2=True, c=c)
+    return PGR(x, rhs, shape, a, omega, m, p, max_iter, tol, sigma, precond, _l2=True, c=c, fdn=fdn)
 
 
-def CG(x, b, shape, a, max_iter, tol, sigma=0.0, precond=True, c=None):
+def CG(x, b, shape, a, max_iter, tol, sigma=0.0, precond=True, c=None, fdn=6):
     """CG.c:33-175 (c as in PGR)."""
     import torch
     _check_len(x, shape, "x")
     _check_len(b, shape, "b")
-    g = _grid(shape, a, sigma)
+    g = _grid(shape, a, sigma, fdn)
     st = Stats()
     _bind_stream()
     if c is None:

@@ -66,6 +66,8 @@ struct Context {
     bool tuning_set = false;       // aar_hip_set_tuning() fixed the shape for every grid
     bool autotune = false;         // tune a grid's shape on its first solve
     bool complex_ref_order = false;  // complex solver's SVD in the tools.c order (default: textbook)
+    int user_fdn = 0;              // registered stencil weights (aar_hip_set_stencil): order / 2, 0 = none
+    double user_w[R + 1] = {};
     std::map<std::array<int, 4>, Shape> tuned;        // (nx, ny, nz, bytes) -> shape
     std::map<std::array<int, 4>, Shape> tuned_batch;  // lz for the batched 8x4 launch
     std::map<std::string, ProfEntry> profile;
@@ -79,6 +81,7 @@ struct Context {
 struct Op {
     int nx = 0, ny = 0, nz = 0;
     long n = 0;
+    int fdn = R;              // stencil order / 2: weights w[o] = 0 for o > fdn
     double w[R + 1] = {};
     double sigma = 0.0;
     double diag = 1.0;
@@ -106,6 +109,7 @@ void prof_end(double bytes);
 int fail(int code, const std::string &msg);
 void lap_coefficient(double *c, int fdn);
 int make_op(const aar_hip_grid *g, const double *coeff, Op &op);
+int stencil_weights(int fdn_in, const double *coeff, double (&c)[R + 1], int *fdn_out);
 
 template <typename T>
 int aar_solve(const Op &op, T *x, const T *b, double omega, double beta, int m, int p, int max_iter,
@@ -135,6 +139,7 @@ int aar_refined(const Op &op, double *x, const double *b, double omega, double b
 struct HOp {
     int nx = 0, ny = 0, nz = 0;
     long n = 0;
+    int fdn = R;
     double w[R + 1] = {};
     double dlap = 0.0;
     const double *c = nullptr;

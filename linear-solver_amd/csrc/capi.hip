@@ -1006,6 +1006,19 @@ int aar_hip_last_stats(aar_hip_stats *st)
     return 0;
 }
 
+int aar_hip_set_stencil(const double *coeff, int fdn)
+{
+    Context &C = ctx();
+    if (!coeff) {
+        C.user_fdn = 0;
+        return 0;
+    }
+    if (fdn < 1 || fdn > R) return fail(AAR_HIP_ERR_ARG, "set_stencil: fdn must be 1..6");
+    for (int o = 0; o <= R; o++) C.user_w[o] = o <= fdn ? coeff[o] : 0.0;
+    C.user_fdn = fdn;
+    return 0;
+}
+
 int aar_hip_set_stream(void *stream)
 {
     Context &C = ctx();

@@ -1238,6 +1238,8 @@ int gmres_solve(const Op &op, double *x, const double *b, int restart, int max_i
     if (restart < 1 || restart > KMAX) return fail(AAR_HIP_ERR_ARG, "restart must be in 1..32");
     if (precond < 0 || precond > 2) return fail(AAR_HIP_ERR_ARG, "precond must be 0 (none), 1 (Jacobi) or 2 (ILU(0))");
     const bool ilu = precond == 2;
+    if (ilu && op.fdn != R)
+        return fail(AAR_HIP_ERR_ARG, "ILU(0) is built for the full FD12 stencil (fdn = 6)");
     if (ilu)
         if (int rc = check_ilu_grid(op)) return rc;
     const long n = op.n;
@@ -1317,6 +1319,7 @@ int gmres_solve(const Op &op, double *x, const double *b, int restart, int max_i
 // z = M^{-1} r (trans: M^{-T} r) with the ILU(0) factors of op, factored here
 int ilu0_apply(const Op &op, const double *r, double *z, int trans)
 {
+    if (op.fdn != R) return fail(AAR_HIP_ERR_ARG, "ILU(0) is built for the full FD12 stencil (fdn = 6)");
     Context &C = ctx();
     if (int rc = context_init()) return rc;
     if (!r || !z) return fail(AAR_HIP_ERR_ARG, "null vector");
@@ -1352,6 +1355,8 @@ int aar_petsc_solve(const Op &op, double *x, const double *b, double omega, doub
     if (p < 1) return fail(AAR_HIP_ERR_ARG, "p must be >= 1");
     if (precond < 0 || precond > 2) return fail(AAR_HIP_ERR_ARG, "precond must be 0 (none), 1 (Jacobi) or 2 (ILU(0))");
     const bool ilu = precond == 2;
+    if (ilu && op.fdn != R)
+        return fail(AAR_HIP_ERR_ARG, "ILU(0) is built for the full FD12 stencil (fdn = 6)");
     if (ilu)
         if (int rc = check_ilu_grid(op)) return rc;
     const long n = op.n;
@@ -1483,6 +1488,8 @@ int pgr_petsc_solve(const Op &op, double *x, const double *b, double omega, int 
     if (p < 1) return fail(AAR_HIP_ERR_ARG, "p must be >= 1");
     if (precond < 0 || precond > 2) return fail(AAR_HIP_ERR_ARG, "precond must be 0 (none), 1 (Jacobi) or 2 (ILU(0))");
     const bool ilu = precond == 2;
+    if (ilu && op.fdn != R)
+        return fail(AAR_HIP_ERR_ARG, "ILU(0) is built for the full FD12 stencil (fdn = 6)");
     if (ilu)
         if (int rc = check_ilu_grid(op)) return rc;
     const long n = op.n;
@@ -1668,7 +1675,10 @@ int bicg_solve(const Op &op, double *x, const double *b, int max_it, double tol,
     if (!x || !b) return fail(AAR_HIP_ERR_ARG, "null vector");
     if (max_it < 0) return fail(AAR_HIP_ERR_ARG, "max_iter must be >= 0");
     if (precond < 0 || precond > 2) return fail(AAR_HIP_ERR_ARG, "precond must be 0 (none), 1 (Jacobi) or 2 (ILU(0))");
-    if (precond == 2) return bicg_ilu_solve(op, x, b, max_it, tol, st);
+    if (precond == 2) {
+        if (op.fdn != R) return fail(AAR_HIP_ERR_ARG, "ILU(0) is built for the full FD12 stencil (fdn = 6)");
+        return bicg_ilu_solve(op, x, b, max_it, tol, st);
+    }
     const long n = op.n;
     const long ld = ring_ld(n, sizeof(double));
     HC(ensure(C.kvec, sizeof(double) * (size_t)ld * 4));
@@ -2296,6 +2306,7 @@ static IluArgsT<double2> helm_ilu_args(const HOp &op, double2 *F)
 // ILU(0) factors of op, factored here
 int helm_ilu0_apply(const HOp &op, const double *r, double *z, int trans)
 {
+    if (op.fdn != R) return fail(AAR_HIP_ERR_ARG, "ILU(0) is built for the full FD12 stencil (fdn = 6)");
     Context &C = ctx();
     if (int rc = context_init()) return rc;
     if (!r || !z) return fail(AAR_HIP_ERR_ARG, "null vector");
@@ -2331,6 +2342,8 @@ int helm_gmres_solve(const HOp &op, double *u, const double *b, int restart, int
     if (restart < 1 || restart > KMAX) return fail(AAR_HIP_ERR_ARG, "restart must be in 1..32");
     if (precond < 0 || precond > 2) return fail(AAR_HIP_ERR_ARG, "precond must be 0 (none), 1 (Jacobi) or 2 (ILU(0))");
     const bool ilu = precond == 2;
+    if (ilu && op.fdn != R)
+        return fail(AAR_HIP_ERR_ARG, "ILU(0) is built for the full FD12 stencil (fdn = 6)");
     if (ilu && (op.nx < 13 || op.ny < 13 || op.nz < 13))
         return fail(AAR_HIP_ERR_ARG, "the ILU(0) preconditioner needs every grid side >= 13");
     const long n = op.n;
@@ -2464,6 +2477,8 @@ int helm_bicg_solve(const HOp &op, double *u, const double *b, int max_it, doubl
     if (max_it < 0) return fail(AAR_HIP_ERR_ARG, "max_iter must be >= 0");
     if (precond < 0 || precond > 2) return fail(AAR_HIP_ERR_ARG, "precond must be 0 (none), 1 (Jacobi) or 2 (ILU(0))");
     const bool ilu = precond == 2;
+    if (ilu && op.fdn != R)
+        return fail(AAR_HIP_ERR_ARG, "ILU(0) is built for the full FD12 stencil (fdn = 6)");
     if (ilu && (op.nx < 13 || op.ny < 13 || op.nz < 13))
         return fail(AAR_HIP_ERR_ARG, "the ILU(0) preconditioner needs every grid side >= 13");
     const long n = op.n;

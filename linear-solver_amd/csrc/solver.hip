@@ -98,15 +98,36 @@ void lap_coefficient(double *c, int fdn)
     }
 }
 
+// The 1D weights c[0..R] of an operator of order 2*fdn: the caller's
+// (coeff[0..fdn], as MPI_REAL/system.h's coeff_lap holds them), the
+// registered user weights (fdn = AAR_HIP_FDN_USER), else Lap_coefficient's
+// (system.c:480-498).  Weights beyond fdn are 0: the radius-6 sweeps then
+// add exact zeros for the absent offsets.
+int stencil_weights(int fdn_in, const double *coeff, double (&c)[R + 1], int *fdn_out)
+{
+    int fdn = fdn_in;
+    const double *cw = coeff;
+    if (fdn == AAR_HIP_FDN_USER) {
+        if (ctx().user_fdn < 1) return fail(AAR_HIP_ERR_ARG, "no stencil weights registered (aar_hip_set_stencil)");
+        fdn = ctx().user_fdn;
+        cw = ctx().user_w;
+    }
+    if (fdn < 1 || fdn > R)
+        return fail(AAR_HIP_ERR_ARG, "fdn must be 1..6 (stencil radius at most 6) or AAR_HIP_FDN_USER");
+    for (int o = 0; o <= R; o++) c[o] = 0.0;
+    if (cw) for (int o = 0; o <= fdn; o++) c[o] = cw[o];
+    else lap_coefficient(c, fdn);
+    *fdn_out = fdn;
+    return 0;
+}
+
 int make_op(const aar_hip_grid *g, const double *coeff, Op &op)
 {
     if (!g) return fail(AAR_HIP_ERR_ARG, "null grid");
-    if (g->fdn != R) return fail(AAR_HIP_ERR_ARG, "only fdn = 6 (FD12) is built");
     if (g->nx < 1 || g->ny < 1 || g->nz < 1) return fail(AAR_HIP_ERR_ARG, "grid sizes must be positive");
     if ((double)g->nx * g->ny * g->nz > 2147483647.0) return fail(AAR_HIP_ERR_ARG, "grid too large");
     double c[R + 1];
-    if (coeff) for (int o = 0; o <= R; o++) c[o] = coeff[o];
-    else lap_coefficient(c, R);
+    if (int rc = stencil_weights(g->fdn, coeff, c, &op.fdn)) return rc;
     op.nx = g->nx;
     op.ny = g->ny;
     op.nz = g->nz;
@@ -1089,11 +1110,10 @@ template int matvec<float>(const Op &, const float *, float *);
 int make_hop(const aar_hip_helm *h, HOp &op)
 {
     if (!h) return fail(AAR_HIP_ERR_ARG, "null operator");
-    if (h->fdn != R) return fail(AAR_HIP_ERR_ARG, "only fdn = 6 (FD12) is built");
     if (h->nx < 1 || h->ny < 1 || h->nz < 1) return fail(AAR_HIP_ERR_ARG, "grid sizes must be positive");
     if ((double)h->nx * h->ny * h->nz > 1073741823.0) return fail(AAR_HIP_ERR_ARG, "grid too large");
     double c[R + 1];
-    lap_coefficient(c, R);
+    if (int rc = stencil_weights(h->fdn, nullptr, c, &op.fdn)) return rc;
     op.nx = h->nx;
     op.ny = h->ny;
     op.nz = h->nz;

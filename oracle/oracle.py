@@ -130,11 +130,13 @@ def lap_coefficient(fdn=6):
     return c
 
 
-def matvec(x, shape, a, sigma=0.0, fdn=6):
+def matvec(x, shape, a, sigma=0.0, fdn=6, coeff=None):
+    """system.c:189-205 with the weights of order 2*fdn, or the caller's
+    coeff[0..fdn]."""
     nx, ny, nz = shape
     x = _f64(x).reshape(-1)
     y = np.empty_like(x)
-    c = lap_coefficient(fdn)
+    c = lap_coefficient(fdn) if coeff is None else _f64(coeff).reshape(-1)[:fdn + 1].copy()
     lib().orc_matvec(nx, ny, nz, fdn, _p(c), a, sigma, _p(x), _p(y))
     return y
 
@@ -427,14 +429,20 @@ def ref_available():
 
 
 def write_problem(path, shape, rhs, x0, a, sigma=0.0, omega=0.6, beta=0.6, m=7, p=6,
-                  max_iter=1000, tol=1e-6, precond=True, fdn=6):
+                  max_iter=1000, tol=1e-6, precond=True, fdn=6, coeff=None):
+    """coeff: stencil weights coeff[0..fdn] replacing the reference's own
+    (written as the "AARPRB02" form)."""
     nx, ny, nz = shape
     with open(path, "wb") as fp:
-        fp.write(b"AARPRB01")
+        fp.write(b"AARPRB01" if coeff is None else b"AARPRB02")
         fp.write(struct.pack("<8i", nx, ny, nz, fdn, m, p, max_iter, int(precond)))
         fp.write(struct.pack("<5d", a, sigma, omega, beta, tol))
         fp.write(_f64(rhs).reshape(-1).tobytes())
         fp.write(_f64(x0).reshape(-1).tobytes())
+        if coeff is not None:
+            c = _f64(coeff).reshape(-1)
+            fp.write(struct.pack("<i", c.size))
+            fp.write(c.tobytes())
 
 
 _ITER_RE = re.compile(r"(AAR|PGR|PL2R|CG)(?: converged!)?:\s+Iterations = (\d+), "
@@ -442,7 +450,7 @@ _ITER_RE = re.compile(r"(AAR|PGR|PL2R|CG)(?: converged!)?:\s+Iterations = (\d+),
 
 
 def ref_solve(solver, shape, rhs, x0, a, sigma=0.0, omega=0.6, beta=0.6, m=7, p=6,
-              max_iter=1000, tol=1e-6, precond=True, ranks=1, timeout=600, trace=True):
+              max_iter=1000, tol=1e-6, precond=True, ranks=1, timeout=600, trace=True, fdn=6, coeff=None):
     """Run the reference solver (one of aar, pgr, pl2r, cg).  Returns
     (x, dict(iterations, relres_printed, trace, stdout, converged_line,
     warning_line, seconds)).  trace=False: no residual recording (an empty
@@ -453,7 +461,7 @@ def ref_solve(solver, shape, rhs, x0, a, sigma=0.0, omega=0.6, beta=0.6, m=7, p=
     tmp = tempfile.mkdtemp(prefix="aarref_")
     try:
         prob = os.path.join(tmp, "p.bin")
-        write_problem(prob, shape, rhs, x0, a, sigma, omega, beta, m, p, max_iter, tol, precond)
+        write_problem(prob, shape, rhs, x0, a, sigma, omega, beta, m, p, max_iter, tol, precond, fdn, coeff)
         xo, to = os.path.join(tmp, "x.bin"), os.path.join(tmp, "t.bin")
         cmd = [REF_HARNESS, "solve", prob, solver, xo, to if trace else "-"]
         if ranks > 1:
@@ -477,11 +485,11 @@ def ref_solve(solver, shape, rhs, x0, a, sigma=0.0, omega=0.6, beta=0.6, m=7, p=
         shutil.rmtree(tmp, ignore_errors=True)
 
 
-def ref_matvec(shape, x, a, sigma=0.0, ranks=1):
+def ref_matvec(shape, x, a, sigma=0.0, ranks=1, fdn=6, coeff=None):
     tmp = tempfile.mkdtemp(prefix="aarref_")
     try:
         prob = os.path.join(tmp, "p.bin")
-        write_problem(prob, shape, np.zeros(int(np.prod(shape))), x, a, sigma)
+        write_problem(prob, shape, np.zeros(int(np.prod(shape))), x, a, sigma, fdn=fdn, coeff=coeff)
         yo = os.path.join(tmp, "y.bin")
         cmd = [REF_HARNESS, "matvec", prob, yo]
         if ranks > 1:

@@ -67,6 +67,13 @@ static void rec_precond(double diag, double *r, int np)
     for (int i = 0; i < np; i++) r[i] /= d;
 }
 
+/* "AARPRB02": the same, followed by int32 ncoef and float64 coeff[ncoef]:
+ * a caller's stencil weights that replace the system's coeff_lap[0..FDn]
+ * (the reference reads its weights only from there, system.c:102-104,
+ * AAR.c:94) */
+static int g_ncoef = 0;
+static double g_coef[32];
+
 static int read_problem(const char *path, prob_hdr *h, double **rhs, double **x0)
 {
     FILE *fp = fopen(path, "rb");
@@ -74,7 +81,9 @@ static int read_problem(const char *path, prob_hdr *h, double **rhs, double **x0
     char magic[8];
     int iv[8];
     double dv[5];
-    if (fread(magic, 1, 8, fp) != 8 || memcmp(magic, "AARPRB01", 8) != 0) { fclose(fp); return -2; }
+    if (fread(magic, 1, 8, fp) != 8) { fclose(fp); return -2; }
+    const int v2 = memcmp(magic, "AARPRB02", 8) == 0;
+    if (!v2 && memcmp(magic, "AARPRB01", 8) != 0) { fclose(fp); return -2; }
     if (fread(iv, sizeof(int), 8, fp) != 8 || fread(dv, sizeof(double), 5, fp) != 5) { fclose(fp); return -3; }
     h->n[0] = iv[0]; h->n[1] = iv[1]; h->n[2] = iv[2]; h->fdn = iv[3];
     h->m = iv[4]; h->p = iv[5]; h->max_iter = iv[6]; h->precond = iv[7];
@@ -85,6 +94,14 @@ static int read_problem(const char *path, prob_hdr *h, double **rhs, double **x0
     if (fread(*rhs, sizeof(double), n, fp) != (size_t)n || fread(*x0, sizeof(double), n, fp) != (size_t)n) {
         fclose(fp);
         return -4;
+    }
+    g_ncoef = 0;
+    if (v2) {
+        if (fread(&g_ncoef, sizeof(int), 1, fp) != 1 || g_ncoef < 1 || g_ncoef > 32 ||
+            fread(g_coef, sizeof(double), g_ncoef, fp) != (size_t)g_ncoef) {
+            fclose(fp);
+            return -5;
+        }
     }
     fclose(fp);
     return 0;
@@ -102,6 +119,7 @@ static POISSON *setup_system(const int n[3], int fdn)
     Comm_topologies(sys, sys->FDn, sys->psize, sys->coords, sys->rem, sys->np, sys->cart, &sys->comm_laplacian,
                     sys->send_neighs, sys->rec_neighs, sys->send_counts, sys->rec_counts,
                     sys->send_layers, sys->rec_layers, &sys->n_in, &sys->n_out);
+    for (int o = 0; o < g_ncoef && o <= fdn; o++) sys->coeff_lap[o] = g_coef[o];
     return sys;
 }
 

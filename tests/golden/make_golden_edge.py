@@ -60,6 +60,15 @@ def nonfinite_cases():
     return shape, out
 
 
+def user_weights():
+    """A caller's own radius-3 stencil: the mean of the FD2 and FD6 weights
+    (a convex combination of two Laplacians, so the operator stays negative
+    semi-definite with a zero row sum) -- not one of the reference's orders."""
+    c1 = O.lap_coefficient(1)
+    c3 = O.lap_coefficient(3)
+    return [0.5 * (c3[o] + (c1[o] if o <= 1 else 0.0)) for o in range(4)]
+
+
 def main():
     if not O.ref_available() or not os.path.exists(REF_MAIN):
         raise SystemExit("build the reference first: make -C oracle ref ref_main")
@@ -106,6 +115,32 @@ def main():
             print("nonfinite", name, sv, ent[sv]["iterations"], ent[sv]["x_nan_count"], repr(ent[sv]["stdout"]),
                   flush=True)
         meta["nonfinite"][name] = ent
+
+    # FD orders 2..12 (FDn = 1..6, system.c:480-498) and a caller's own
+    # weights in coeff_lap (the reference's extension point, README:25): the
+    # reference's stencil on a ragged grid, and its solvers on 16^3
+    rng2 = np.random.default_rng(91)
+    mshape = (13, 10, 14)
+    xm = rng2.standard_normal(int(np.prod(mshape)))
+    arrays["fdn_matvec_x"] = xm
+    user = user_weights()
+    arrays["user_coeff"] = np.array(user)
+    meta["fdn"] = {"matvec_shape": list(mshape), "solve_shape": [16, 16, 16], "user_fdn": len(user) - 1,
+                   "solves": {}}
+    for fdn in range(1, 7):
+        arrays[f"fdn{fdn}_matvec_y"] = O.ref_matvec(mshape, xm, -4.0, fdn=fdn)
+    arrays["user_matvec_y"] = O.ref_matvec(mshape, xm, -4.0, fdn=len(user) - 1, coeff=user)
+    sshape = (16, 16, 16)
+    bs = P.rhs_uniform(sshape, 1001)
+    arrays["fdn_solve_b"] = bs
+    for label, fdn, coeff in (("fdn2", 2, None), ("fdn4", 4, None), ("user", len(user) - 1, user)):
+        for sv in ("aar", "pgr", "pl2r", "cg"):
+            x, d = O.ref_solve(sv, sshape, bs, np.zeros(bs.size), -4.0, 0.0, 0.6, 0.6, 7, 6, 1000, 1e-8, True,
+                               fdn=fdn, coeff=coeff)
+            meta["fdn"]["solves"][f"{label}_{sv}"] = {"iterations": d["iterations"],
+                                                      "converged_line": d["converged_line"]}
+            arrays[f"{label}_{sv}_x"] = x
+            print("fdn", label, sv, d["iterations"], flush=True)
 
     with tempfile.TemporaryDirectory() as tmp:
         for args in ([], ["1e-6"], ["1e-6", "7", "6", "0.6"], ["1e-6", "7", "6", "0.6", "0.6", "1"]):

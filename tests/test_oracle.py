@@ -556,3 +556,54 @@ def test_vc_pgr_cg_solution_dense(oracle, solver):
         x, d = oracle.pgr_vc(np.zeros(n), b, shape, a, c, 0.6, 7, 6, 3000, 1e-12, l2=solver == "pl2r")
     assert d["converged"]
     assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-9
+
+
+def _edge():
+    import json
+    import os
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    with open(os.path.join(here, "golden_edge.json")) as fp:
+        meta = json.load(fp)
+    return meta, np.load(os.path.join(here, "golden_edge.npz"), allow_pickle=False)
+
+
+@pytest.mark.parametrize("fdn", [1, 2, 3, 4, 5, 6])
+def test_oracle_matvec_fdn_matches_reference(oracle, fdn):
+    """The oracle's stencil at every FD order the reference computes
+    weights for (FDn = 1..6, system.c:480-498, loop of system.c:195-201) is
+    bit-identical to the reference build's."""
+    meta, g = _edge()
+    shape = tuple(meta["fdn"]["matvec_shape"])
+    y = oracle.matvec(g["fdn_matvec_x"], shape, -4.0, fdn=fdn)
+    assert np.array_equal(y, g[f"fdn{fdn}_matvec_y"])
+
+
+def test_oracle_matvec_user_weights_match_reference(oracle):
+    """A caller's own weights in coeff_lap (the reference's extension point):
+    the oracle's stencil with them equals the reference build's."""
+    meta, g = _edge()
+    shape = tuple(meta["fdn"]["matvec_shape"])
+    c = g["user_coeff"]
+    y = oracle.matvec(g["fdn_matvec_x"], shape, -4.0, fdn=c.size - 1, coeff=c)
+    assert np.array_equal(y, g["user_matvec_y"])
+
+
+@pytest.mark.parametrize("label,fdn", [("fdn2", 2), ("fdn4", 4)])
+def test_oracle_solvers_fdn_match_reference(oracle, label, fdn):
+    """AAR, PGR, PL2R and CG at FD orders 4 and 8: the reference's iteration
+    counts and solutions (within rounding)."""
+    meta, g = _edge()
+    shape = tuple(meta["fdn"]["solve_shape"])
+    b = g["fdn_solve_b"]
+    x0 = np.zeros(b.size)
+    for sv in ("aar", "pgr", "pl2r", "cg"):
+        ref = meta["fdn"]["solves"][f"{label}_{sv}"]
+        if sv == "aar":
+            x, d = oracle.aar(x0, b, shape, -4.0, 0.6, 0.6, 7, 6, 1000, 1e-8, fdn=fdn, trace=False)
+        elif sv == "cg":
+            x, d = oracle.cg(x0, b, shape, -4.0, 1000, 1e-8, fdn=fdn, trace=False)
+        else:
+            x, d = oracle.pgr(x0, b, shape, -4.0, 0.6, 7, 6, 1000, 1e-8, l2=sv == "pl2r", fdn=fdn, trace=False)
+        assert d["iterations"] == ref["iterations"], sv
+        xr = g[f"{label}_{sv}_x"]
+        assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-10, sv
