@@ -171,6 +171,14 @@ int aar_hip_gmres(const aar_hip_grid *g, const double *c, double *x, const doubl
                   int max_iter, double tol, int precond, aar_hip_stats *st);
 int aar_hip_bicg(const aar_hip_grid *g, const double *c, double *x, const double *b, int max_iter,
                  double tol, int precond, aar_hip_stats *st);
+/* The CG of the PETSc variant, PETSC_REAL/main.c:152-175 (KSPCG with the
+ * preconditioner of its pc: precond 1 = PCJACOBI, 2 = PCBJACOBI -> ILU(0)
+ * as above, 0 = none): PETSc's KSPSolve_CG with the preconditioned residual
+ * norm, KSPConvergedDefault (ttol = tol*||B b|| for the nonzero guess), its
+ * beta = 0 (converged), indefinite-preconditioner and indefinite-matrix
+ * exits (reasons 3, -8, -10).  stats as aar_hip_gmres. */
+int aar_hip_kspcg(const aar_hip_grid *g, const double *c, double *x, const double *b, int max_iter, double tol,
+                  int precond, aar_hip_stats *st);
 /* PCApply / PCApplyTranspose of that ILU(0) preconditioner: factors the
  * matrix of g (+ c, may be NULL) and sets z = M^{-1} r (trans = 0) or
  * M^{-T} r (trans = 1); device pointers, z may equal r.  Level-scheduled
@@ -184,17 +192,19 @@ int aar_hip_ilu0_apply(const aar_hip_grid *g, const double *c, const double *r, 
  * matrix), 0 = none.  Its semantics where they differ from AAR.c: the
  * Anderson step runs whenever iter % p == 0 (iter = 1 included), the
  * update is x += sum_i (-s_i)(beta DF_i + DX_i) column by column, then
- * x += beta f.  Its dgelsd least squares is replaced by the tools.c
- * pseudo-inverse.  m <= 16; ||b - A x|| is checked every p iterations
+ * x += beta f.  Its LAPACKE_dgelsd least squares (rcond = -1,
+ * AAR_Real.c:223) is the minimum-norm solve over the singular values above
+ * 2^-53 s_max, by the Golub-Reinsch SVD of the other solvers.  m <= 16; ||b - A x|| is checked every p iterations
  * (aar_hip_history); iterations = iter - 1 as AAR_Real.c:162-166 prints. */
 int aar_hip_aar_petsc(const aar_hip_grid *g, const double *c, double *x, const double *b, double omega,
                       double beta, int m, int p, int max_iter, double tol, int precond, aar_hip_stats *st);
 
 /* PETSC_REAL/PGR_Real.c:32-195 (l2 = 0) and PL2R_Real.c:32-197 (l2 = 1) --
  * the variant's PGR/PL2R with the same preconditioner choice (PCICC for
- * pc = 1, PGR_Real.c:55-62), the DX^T DF (DF^T DF) system solved by the
- * tools.c pseudo-inverse in place of dgelsd (whose column-major call takes
- * the transpose, equal in exact arithmetic for these symmetric operators). */
+ * pc = 1, PGR_Real.c:55-62), the DX^T DF (DF^T DF) system solved with
+ * dgelsd's rcond = -1 cutoff as in aar_hip_aar_petsc (dgelsd's column-major
+ * call takes the transpose, equal in exact arithmetic for these symmetric
+ * operators). */
 int aar_hip_pgr_petsc(const aar_hip_grid *g, const double *c, double *x, const double *b, double omega, int m,
                       int p, int max_iter, double tol, int precond, int l2, aar_hip_stats *st);
 

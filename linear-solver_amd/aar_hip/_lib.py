@@ -66,7 +66,7 @@ EXTENSION_SYMBOLS = [
     "aar_hip_autotune", "aar_hip_last_system", "aar_hip_matvec_vc", "aar_hip_aar_vc", "aar_hip_set_coefficient",
     "aar_hip_pgr_vc", "aar_hip_cg_vc", "aar_hip_gmres", "aar_hip_bicg", "aar_hip_ilu0_apply", "aar_hip_aar_petsc",
     "aar_hip_pgr_petsc", "aar_hip_helm_gmres", "aar_hip_helm_bicg", "aar_hip_helm_ilu0_apply",
-    "aar_hip_stream_bandwidth", "aar_hip_pinv_noalloc", "aar_hip_set_stencil",
+    "aar_hip_stream_bandwidth", "aar_hip_pinv_noalloc", "aar_hip_set_stencil", "aar_hip_kspcg",
 ]
 
 
@@ -133,6 +133,7 @@ def load(require_gpu=True):
         "aar_hip_stream_bandwidth": [I, P, P, ctypes.c_long, I, ctypes.POINTER(D)],
         "aar_hip_pinv_noalloc": [P, P, P, I, P, P, P],
         "aar_hip_set_stencil": [P, I],
+        "aar_hip_kspcg": [G, P, P, P, I, D, I, S],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

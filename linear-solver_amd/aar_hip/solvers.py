@@ -330,6 +330,24 @@ def BiCG(x, b, shape, a, max_iter=2000, tol=1e-6, sigma=0.0, precond=True, c=Non
     return st.as_dict()
 
 
+def KSPCG(x, b, shape, a, max_iter=2000, tol=1e-6, sigma=0.0, precond=True, c=None, fdn=6):
+    """PETSc's KSPCG as PETSC_REAL/main.c:152-175 runs it; precond as in
+    GMRES (2: ILU(0), its pc = 1)."""
+    import torch
+    _check_len(x, shape, "x")
+    _check_len(b, shape, "b")
+    g = _grid(shape, a, sigma, fdn)
+    st = Stats()
+    _bind_stream()
+    cp = None
+    if c is not None:
+        _check_len(c, shape, "c")
+        cp = _dev(c, torch.float64, "c")
+    check(_L().aar_hip_kspcg(ctypes.byref(g), cp, _dev(x, torch.float64, "x"), _dev(b, torch.float64, "b"),
+                             max_iter, tol, _pc(precond), ctypes.byref(st)))
+    return st.as_dict()
+
+
 def AAR_refined(x, rhs, shape, a, omega, beta, m, p, tol, inner_tol=1e-5, max_outer=30,
                 max_inner=100000, sigma=0.0, precond=True):
     """FP64 residual refinement around FP32 AAR (no reference counterpart)."""

@@ -100,11 +100,13 @@ template <typename T>
 hipError_t launch_gram(const GramArgs<T> &a, hipStream_t st);
 
 // m x m pseudo-inverse solve on device, one wavefront: s = V W+ V^T g with
-// the Golub-Reinsch SVD of tools.c:52-352 (bit-faithful), or with its
-// textbook left reflector when complex_embed = 1 (the complex solver's
-// 2m x 2m real embedding).  Reads ctrl->G (allredvec layout), writes
-// ctrl->s, ctrl->w.
-hipError_t launch_ls(Ctrl *ctrl, int m, int complex_embed, hipStream_t st);
+// the Golub-Reinsch SVD of tools.c:52-352 (bit-faithful).  Reads ctrl->G
+// (allredvec layout), writes ctrl->s, ctrl->w.  flags: LS_TEXTBOOK -- the
+// textbook left reflector (the complex solver's 2m x 2m real embedding);
+// LS_GELSD -- LAPACK dgelsd's rcond = -1 cutoff of the PETSc variant
+// (s_j <= 2^-53 s_max dropped) instead of tools.c's (s_j < m s_max eps).
+constexpr int LS_TEXTBOOK = 1, LS_GELSD = 2;
+hipError_t launch_ls(Ctrl *ctrl, int m, int flags, hipStream_t st);
 // the same for `count` independent solves, ctrls a device array of pointers
 hipError_t launch_ls_batch(Ctrl *const *ctrls, int count, int m, hipStream_t st);
 

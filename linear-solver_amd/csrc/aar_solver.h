@@ -123,6 +123,7 @@ int cg_solve(const Op &op, double *x, const double *b, int max_iter, double tol,
 int gmres_solve(const Op &op, double *x, const double *b, int restart, int max_it, double tol, int precond,
                 aar_hip_stats *st);
 int bicg_solve(const Op &op, double *x, const double *b, int max_it, double tol, int precond, aar_hip_stats *st);
+int kspcg_solve(const Op &op, double *x, const double *b, int max_it, double tol, int precond, aar_hip_stats *st);
 int ilu0_apply(const Op &op, const double *r, double *z, int trans);
 int aar_petsc_solve(const Op &op, double *x, const double *b, double omega, double beta, int m, int p,
                     int max_iter, double tol, int precond, aar_hip_stats *st);

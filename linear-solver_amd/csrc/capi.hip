@@ -849,6 +849,14 @@ int aar_hip_bicg(const aar_hip_grid *g, const double *c, double *x, const double
     return bicg_solve(op, x, b, max_iter, tol, precond, st);
 }
 
+int aar_hip_kspcg(const aar_hip_grid *g, const double *c, double *x, const double *b, int max_iter, double tol,
+                  int precond, aar_hip_stats *st)
+{
+    Op op;
+    if (int rc = c ? make_op_vc(g, c, op) : make_op(g, nullptr, op)) return rc;
+    return kspcg_solve(op, x, b, max_iter, tol, precond, st);
+}
+
 int aar_hip_aar_petsc(const aar_hip_grid *g, const double *c, double *x, const double *b, double omega, double beta,
                       int m, int p, int max_iter, double tol, int precond, aar_hip_stats *st)
 {

@@ -391,7 +391,7 @@ struct WGrQr<M, -1> {
 };
 
 template <int M>
-__device__ __forceinline__ void ls_body(Ctrl *c, const int n, const int textbook)
+__device__ __forceinline__ void ls_body(Ctrl *c, const int n, const int flags)
 {
     __shared__ double A[M][M];
     __shared__ double V[M][M];
@@ -441,7 +441,7 @@ __device__ __forceinline__ void ls_body(Ctrl *c, const int n, const int textbook
                     const double fac = t / h;
                     for (int k = i; k < n; k++) A[k][j] += fac * A[k][i];
                 };
-                if (!textbook) {
+                if (!(flags & LS_TEXTBOOK)) {
                     if (lane >= 1 && lane < i && lane < n) colupd(lane);
                     __syncthreads();
                     if (i >= 1 && lane == i) colupd(i);
@@ -532,15 +532,24 @@ __device__ __forceinline__ void ls_body(Ctrl *c, const int n, const int textbook
     }
     __syncthreads();
 
-    // truncation (tools.c:331-335) and x = V diag(1/w) V^T b (tools.c:76-94)
+    // truncation (tools.c:331-335: w < n w_max 2.22044605e-16; with
+    // LS_GELSD, LAPACK dgelsd's rcond = -1 rule of the PETSc variant:
+    // w <= 2^-53 w_max) and x = V diag(1/w) V^T b (tools.c:76-94)
     double wmax = 0.0;
 #pragma unroll
     for (int j = 0; j < M; j++)
         if (j < n && w[j] > wmax) wmax = w[j];
-    const double wmin = n * wmax * (2.22044605e-16);
+    if (flags & LS_GELSD) {
+        const double tol = 1.1102230246251565e-16 * wmax;
 #pragma unroll
-    for (int j = 0; j < M; j++)
-        if (j < n && w[j] < wmin) w[j] = 0.0;
+        for (int j = 0; j < M; j++)
+            if (j < n && w[j] <= tol) w[j] = 0.0;
+    } else {
+        const double wmin = n * wmax * (2.22044605e-16);
+#pragma unroll
+        for (int j = 0; j < M; j++)
+            if (j < n && w[j] < wmin) w[j] = 0.0;
+    }
     if (lane < n) {
         double wl = 0.0;
 #pragma unroll
@@ -573,9 +582,9 @@ __device__ __forceinline__ void ls_body(Ctrl *c, const int n, const int textbook
 }
 
 template <int M>
-__global__ __launch_bounds__(64) void ls_kernel(Ctrl *c, int n, int textbook)
+__global__ __launch_bounds__(64) void ls_kernel(Ctrl *c, int n, int flags)
 {
-    ls_body<M>(c, n, textbook);
+    ls_body<M>(c, n, flags);
 }
 
 // one wavefront per solve of a batch
@@ -593,12 +602,12 @@ hipError_t launch_ls_batch(Ctrl *const *ctrls, int count, int n, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_ls(Ctrl *ctrl, int n, int complex_embed, hipStream_t st)
+hipError_t launch_ls(Ctrl *ctrl, int n, int flags, hipStream_t st)
 {
     if (n < 1 || n > MAXM_LS) return hipErrorInvalidValue;
-    if (n <= 8) hipLaunchKernelGGL((ls_kernel<8>), dim3(1), dim3(64), 0, st, ctrl, n, complex_embed);
-    else if (n <= 16) hipLaunchKernelGGL((ls_kernel<16>), dim3(1), dim3(64), 0, st, ctrl, n, complex_embed);
-    else hipLaunchKernelGGL((ls_kernel<32>), dim3(1), dim3(64), 0, st, ctrl, n, complex_embed);
+    if (n <= 8) hipLaunchKernelGGL((ls_kernel<8>), dim3(1), dim3(64), 0, st, ctrl, n, flags);
+    else if (n <= 16) hipLaunchKernelGGL((ls_kernel<16>), dim3(1), dim3(64), 0, st, ctrl, n, flags);
+    else hipLaunchKernelGGL((ls_kernel<32>), dim3(1), dim3(64), 0, st, ctrl, n, flags);
     return hipGetLastError();
 }
 

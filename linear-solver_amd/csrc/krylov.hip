@@ -60,7 +60,9 @@ enum KReason : int {
     K_DIVERGED_DTOL = -4,
     K_DIVERGED_BREAKDOWN = -5,
     K_DIVERGED_BREAKDOWN_BICG = -6,
+    K_DIVERGED_INDEFINITE_PC = -8,
     K_DIVERGED_NANORINF = -9,
+    K_DIVERGED_INDEFINITE_MAT = -10,
 };
 constexpr double K_ATOL = 1e-50, K_DTOL = 1e5, K_HAPTOL = 1e-30;
 
@@ -84,6 +86,7 @@ struct KCtrl {
     int max_it, restart, first, zero;   // zero: never set (a skip flag that never skips)
     double rtol, ttol, rnorm0, bpnorm, rhs_norm, scale, relres;
     double beta, betaold, bcoef, alpha;
+    double dpi;                          // CG: p.A p of the last step
     double h[KMAX + 1];
     double HH[(KMAX + 2) * (KMAX + 1)];   // column-major, columns of KMAX + 2
     double cc[KMAX + 1], ss[KMAX + 1], grs[KMAX + 2], nrs[KMAX + 1];
@@ -748,6 +751,93 @@ __global__ __launch_bounds__(KB) void k_bi_step_kernel(KCtrl *k, const double *p
     k->beta = s[1];
     k->bcoef = k->beta / k->betaold;
     k->first = 0;
+}
+
+// ---------------------------------------------------------------------------
+// CG of the PETSc variant (PETSC_REAL/main.c:152-175: KSPCG, PETSc's
+// KSPSolve_CG with the preconditioned norm).  The vector kernels are
+// BiCG's (with the shadow sequence equal to the primal one, BiCG's
+// recurrences are CG's); these scalar steps add CG's exits: beta = 0
+// (converged), beta * betaold < 0 (indefinite preconditioner) and a zero
+// or sign-changing p.A p (indefinite matrix), with PETSc's its = i + 1.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(KB) void k_cg_start_kernel(KCtrl *k, const double *part, int nb, double *hist)
+{
+    __shared__ double s[KP];
+    k_reduce(part, nb, 2, s);
+    if (threadIdx.x != 0) return;
+    const double dp = sqrt(s[0]);
+    k->beta = s[1];
+    k->first = 1;
+    k->dpi = 0.0;
+    k_push(k, hist, dp);
+    int reason = k_converged(k, 0, dp);
+    if (!reason && k->max_it > 0 && k->beta == 0.0) {
+        k->its = 1;
+        reason = K_CONVERGED_ATOL;
+    }
+    k->reason = reason;
+    if (reason) k_stop(k);
+}
+
+__global__ __launch_bounds__(KB) void k_cg_alpha_kernel(KCtrl *k, const double *part, int nb)
+{
+    if (k->c.done) return;
+    __shared__ double s[KP];
+    k_reduce(part, nb, 1, s);
+    if (threadIdx.x != 0) return;
+    const double dpiold = k->dpi, dpi = s[0];
+    k->dpi = dpi;
+    if (dpi == 0.0 || (!k->first && dpi * dpiold <= 0.0)) {
+        k->its++;
+        k->reason = K_DIVERGED_INDEFINITE_MAT;
+        k_stop(k);
+        return;
+    }
+    k->alpha = k->beta / dpi;
+}
+
+__global__ __launch_bounds__(KB) void k_cg_step_kernel(KCtrl *k, const double *part, int nb, double *hist)
+{
+    if (k->c.done) return;
+    __shared__ double s[KP];
+    k_reduce(part, nb, 2, s);
+    if (threadIdx.x != 0) return;
+    const double dp = sqrt(s[0]);
+    k->its++;
+    k_push(k, hist, dp);
+    int reason = k_converged(k, k->its, dp);
+    if (!reason && k->its >= k->max_it) reason = K_DIVERGED_ITS;
+    if (!reason) {
+        // the checks at the top of PETSc's next step (its = i + 1 there)
+        const double beta = s[1];
+        if (beta == 0.0) {
+            k->its++;
+            reason = K_CONVERGED_ATOL;
+        } else if (beta * k->beta < 0.0) {
+            k->its++;
+            reason = K_DIVERGED_INDEFINITE_PC;
+        }
+        k->betaold = k->beta;
+        k->beta = beta;
+        k->bcoef = k->beta / k->betaold;
+        k->first = 0;
+    }
+    k->reason = reason;
+    if (reason) k_stop(k);
+}
+
+// x += a p; r -= a q (CG with ILU(0): z = M^{-1} r follows)
+__global__ __launch_bounds__(KB) void k_cg_upd_kernel(long n, double *x, double *r, const double *p, const double *q,
+                                                     const KCtrl *k)
+{
+    if (k->c.done) return;
+    const double al = k->alpha, ma = -k->alpha;
+    KLOOP(i)
+    {
+        x[i] = x[i] + al * p[i];
+        r[i] = r[i] + ma * q[i];
+    }
 }
 
 __global__ __launch_bounds__(KB) void k_final_kernel(KCtrl *k, const double *part, int nb)
@@ -1421,7 +1511,7 @@ int aar_petsc_solve(const Op &op, double *x, const double *b, double omega, doub
             g.counter = nullptr;
             g.ctrl = &R.k->c;
             HC(launch_gram<double>(g, C.stream));
-            HC(launch_ls(&R.k->c, m, 0, C.stream));
+            HC(launch_ls(&R.k->c, m, LS_GELSD, C.stream));   // dgelsd, rcond = -1
             if (m <= 8) hipLaunchKernelGGL((k_aa_upd_kernel<8>), dim3(R.nb), dim3(KB), 0, C.stream, n, x,
                                            (const double *)f, (const double *)W, ld, m, beta, (const KCtrl *)R.k);
             else hipLaunchKernelGGL((k_aa_upd_kernel<16>), dim3(R.nb), dim3(KB), 0, C.stream, n, x,
@@ -1561,7 +1651,7 @@ int pgr_petsc_solve(const Op &op, double *x, const double *b, double omega, int 
             g.counter = nullptr;
             g.ctrl = &R.k->c;
             HC(launch_gram<double>(g, C.stream));
-            HC(launch_ls(&R.k->c, m, 0, C.stream));
+            HC(launch_ls(&R.k->c, m, LS_GELSD, C.stream));   // dgelsd, rcond = -1
             if (m <= 8)
                 hipLaunchKernelGGL((k_pg_upd_kernel<8>), dim3(R.nb), dim3(KB), 0, C.stream, n, x,
                                    (const double *)x_old, Ax, (const double *)Ax_prev, res, b, X, Y, ld, m, kk, R.part,
@@ -1714,6 +1804,81 @@ int bicg_solve(const Op &op, double *x, const double *b, int max_it, double tol,
         prof_end(9 * s + cb);
         HC(hipGetLastError());
         C.bytes_acc += 12 * s + cb;
+        if ((i + 1) % 8 == 0) {
+            if (int rc = poll_record(&R.k->c, cyc)) return rc;
+            if (poll_check(cyc - 2)) stopped = true;
+            cyc++;
+        }
+    }
+    return k_finish(op, x, b, q, R, st);
+}
+
+int kspcg_solve(const Op &op, double *x, const double *b, int max_it, double tol, int precond, aar_hip_stats *st)
+{
+    Context &C = ctx();
+    if (int rc = context_init()) return rc;
+    if (!x || !b) return fail(AAR_HIP_ERR_ARG, "null vector");
+    if (max_it < 0) return fail(AAR_HIP_ERR_ARG, "max_iter must be >= 0");
+    if (precond < 0 || precond > 2) return fail(AAR_HIP_ERR_ARG, "precond must be 0 (none), 1 (Jacobi) or 2 (ILU(0))");
+    const bool ilu = precond == 2;
+    if (ilu && op.fdn != R) return fail(AAR_HIP_ERR_ARG, "ILU(0) is built for the full FD12 stencil (fdn = 6)");
+    if (ilu)
+        if (int rc = check_ilu_grid(op)) return rc;
+    const long n = op.n;
+    const long ld = ring_ld(n, sizeof(double));
+    HC(ensure(C.kvec, sizeof(double) * (size_t)ld * 4));
+    if (ilu) HC(ensure(C.kilu, sizeof(double) * (size_t)n * IFS));
+    double *r = (double *)C.kvec.p, *z = r + ld, *p = r + 2 * ld, *q = r + 3 * ld;
+    const int hcap = max_it + 8;
+    C.bytes_acc = 0.0;
+    HC(hipEventRecord(C.t0, C.stream));
+    KrylovRun R;
+    if (ilu) {
+        if (int rc = k_setup(op, 2, tol, max_it, 0, hcap, b, R, (double *)C.kilu.p, z)) return rc;
+    } else {
+        if (int rc = k_setup(op, precond, tol, max_it, 0, hcap, b, R)) return rc;
+    }
+    const double s = 8.0 * n, ib = ilu ? ILU_APPLY_BYTES * n : 0.0;
+    const double cb = op.c ? s : 0.0;
+    if (int rc = k_matvec(op, x, q, nullptr)) return rc;
+    if (ilu) {
+        hipLaunchKernelGGL(k_bi2_init_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, b, (const double *)q, r, p);
+        HC(ilu_apply(R.ilu, r, z, false, &R.k->zero, C.stream));
+        hipLaunchKernelGGL(k_bi2_norms_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, (const double *)z,
+                           (const double *)r, R.part, (const KCtrl *)R.k);
+    } else {
+        hipLaunchKernelGGL(k_bi_init_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, b, (const double *)q, r, z, R.dg,
+                           R.part);
+    }
+    hipLaunchKernelGGL(k_cg_start_kernel, dim3(1), dim3(KB), 0, C.stream, R.k, (const double *)R.part, R.nb, R.hist);
+    HC(hipGetLastError());
+    C.bytes_acc += 4 * s + cb + ib;
+    bool stopped = false;
+    const int loops = std::max(max_it, 1);
+    int cyc = 0;
+    for (int i = 0; i < loops && !stopped; i++) {
+        hipLaunchKernelGGL(k_bi_p_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, (const double *)z, p,
+                           (const KCtrl *)R.k);
+        if (int rc = k_matvec(op, p, q, &R.k->c)) return rc;
+        hipLaunchKernelGGL(k_bi_dot_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, (const double *)q,
+                           (const double *)p, R.part, (const KCtrl *)R.k);
+        hipLaunchKernelGGL(k_cg_alpha_kernel, dim3(1), dim3(KB), 0, C.stream, R.k, (const double *)R.part, R.nb);
+        if (ilu) {
+            hipLaunchKernelGGL(k_cg_upd_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, x, r, (const double *)p,
+                               (const double *)q, (const KCtrl *)R.k);
+            prof_begin("krylov_ilu_apply");
+            HC(ilu_apply(R.ilu, r, z, false, &R.k->c.done, C.stream));
+            prof_end(ib);
+            hipLaunchKernelGGL(k_bi2_norms_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, (const double *)z,
+                               (const double *)r, R.part, (const KCtrl *)R.k);
+        } else {
+            hipLaunchKernelGGL(k_bi_upd_kernel, dim3(R.nb), dim3(KB), 0, C.stream, n, x, r, z, (const double *)p,
+                               (const double *)q, R.dg, R.part, (const KCtrl *)R.k);
+        }
+        hipLaunchKernelGGL(k_cg_step_kernel, dim3(1), dim3(KB), 0, C.stream, R.k, (const double *)R.part, R.nb,
+                           R.hist);
+        HC(hipGetLastError());
+        C.bytes_acc += (ilu ? 13 * s + ib : 12 * s) + cb;
         if ((i + 1) % 8 == 0) {
             if (int rc = poll_record(&R.k->c, cyc)) return rc;
             if (poll_check(cyc - 2)) stopped = true;

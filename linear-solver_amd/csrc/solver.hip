@@ -1317,7 +1317,7 @@ int helm_solve(const HOp &op, double *u, const double *b, double omega, double b
             C.bytes_acc += sv * (nv + 1);
         }
         prof_begin("ls");
-        HC(launch_ls(ctrl, 2 * m, C.complex_ref_order ? 0 : 1, C.stream));
+        HC(launch_ls(ctrl, 2 * m, C.complex_ref_order ? 0 : LS_TEXTBOOK, C.stream));
         prof_end(0);
         {
             HAncArgs hu;

@@ -146,7 +146,7 @@ static int gr_svd_wv(double *a, int n, double *w, double *v, int textbook)
                 g = -sign_of(sqrt(s), f);
                 h = f * g - s;
                 A(i, i) = f - g;
-                for (int j = textbook ? l : 1; j < n; j++) {   /* tools.c:137 starts at 1 */
+                for (int j = (textbook & 1) ? l : 1; j < n; j++) {   /* tools.c:137 starts at 1 */
                     double t = 0.0;
                     for (int k = i; k < m; k++) t += A(k, i) * A(k, j);
                     double fac = t / h;
@@ -279,11 +279,18 @@ static int gr_svd_wv(double *a, int n, double *w, double *v, int textbook)
     }
     free(rv1);
 
-    /* tools.c:331-335: zero w_j < n * w_max * 2.22044605e-16 */
     double wmax = 0.0;
     for (int j = 0; j < n; j++) if (w[j] > wmax) wmax = w[j];
-    double wmin = n * wmax * (2.22044605e-16);
-    for (int j = 0; j < n; j++) if (w[j] < wmin) w[j] = 0.0;
+    if (textbook & 2) {
+        /* LAPACK dgelsd with rcond < 0 (PETSC_REAL/AAR_Real.c:223): singular
+         * values s_j <= eps * s_max are dropped, eps = dlamch('E') = 2^-53 */
+        const double tol = 1.1102230246251565e-16 * wmax;
+        for (int j = 0; j < n; j++) if (w[j] <= tol) w[j] = 0.0;
+    } else {
+        /* tools.c:331-335: zero w_j < n * w_max * 2.22044605e-16 */
+        double wmin = n * wmax * (2.22044605e-16);
+        for (int j = 0; j < n; j++) if (w[j] < wmin) w[j] = 0.0;
+    }
     return status;
 #undef A
 #undef V
@@ -302,6 +309,18 @@ int orc_pinv_times_vec_textbook(const double *Ain, const double *b, double *x, i
                                 double *w_out, double *v_out)
 {
     return pinv_times_vec(Ain, b, x, m, w_out, v_out, 1);
+}
+
+/* The least-squares solve of the PETSc variant (LAPACKE_dgelsd with
+ * rcond = -1, PETSC_REAL/AAR_Real.c:223, PGR_Real.c:216, PL2R_Real.c:220)
+ * on its symmetric positive semi-definite normal matrices: the minimum-norm
+ * solution V diag(1/s) V^T b over the singular values above eps * s_max,
+ * computed by the same Golub-Reinsch SVD (dgelsd's own bidiagonal
+ * divide-and-conquer is restated only through its cutoff). */
+int orc_pinv_times_vec_gelsd(const double *Ain, const double *b, double *x, int m,
+                             double *w_out, double *v_out)
+{
+    return pinv_times_vec(Ain, b, x, m, w_out, v_out, 2);
 }
 
 static int pinv_times_vec(const double *Ain, const double *b, double *x, int m, double *w_out, double *v_out,
@@ -1173,6 +1192,97 @@ static int bicg_impl(int nx, int ny, int nz, int fdn, double a, double sigma, co
     return 0;
 }
 
+/* KSPCG (PETSc 3.11 KSPSolve_CG, KSP_CG_SYMMETRIC, preconditioned norm),
+ * the CG of PETSC_REAL/main.c:152-175.  r = b - A x, z = B r, dp = ||z||;
+ * converged at 0?  beta = z.r; then per step i: its = i + 1; beta = 0 ->
+ * CONVERGED_ATOL, beta * betaold < 0 -> DIVERGED_INDEFINITE_PC; p = z
+ * (i = 0) or z + (beta/betaold) p; w = A p, dpi = p.w (0, or a sign change
+ * of dpi -> DIVERGED_INDEFINITE_MAT); a = beta/dpi; x += a p; r -= a w;
+ * z = B r; dp = ||z||; convergence test; beta = z.r. */
+static int kspcg_impl(int nx, int ny, int nz, int fdn, double a, double sigma, const double *cv,
+                      double *x, const double *b, int max_it, double rtol, int precond,
+                      double *trace, int trace_cap, orc_stats *st)
+{
+    const long n = (long)nx * ny * nz;
+    double coeff[32] = {0};
+    orc_lap_coefficient(coeff, fdn);
+    double diag = a * (3 * coeff[0]);
+    if (sigma != 0.0) diag += sigma;
+    double *dinv, *F;
+    ksp_pc pc;
+    if (pc_setup(&pc, precond, nx, ny, nz, fdn, a, sigma, cv, diag, &dinv, &F)) {
+        free(dinv); free(F);
+        return -2;
+    }
+    double *R_ = (double *)malloc(sizeof(double) * n), *Z = (double *)malloc(sizeof(double) * n);
+    double *P = (double *)malloc(sizeof(double) * n), *W = (double *)malloc(sizeof(double) * n);
+    int tlen = 0;
+    ksp_conv cv_ = {0.0, 0.0, rtol, 0.0};
+    pc_apply(&pc, b, Z, n, 0);
+    cv_.bpnorm = orc_norm2(Z, n);
+
+    double t0 = now_seconds();
+    OPV(x, R_);
+    for (long i = 0; i < n; i++) R_[i] = b[i] + (-1.0) * R_[i];     /* VecAYPX(R, -1, b) */
+    pc_apply(&pc, R_, Z, n, 0);
+    double dp = orc_norm2(Z, n);
+    if (trace && tlen < trace_cap) trace[tlen++] = dp;
+    int its = 0;
+    int reason = ksp_converged(&cv_, 0, dp);
+    if (!reason) {
+        double beta = orc_dot(Z, R_, n), betaold = 0.0, dpi = 0.0, dpiold;
+        int i = 0;
+        do {
+            its = i + 1;
+            if (beta == 0.0) {
+                reason = ORC_KSP_CONVERGED_ATOL;
+                break;
+            } else if (i > 0 && beta * betaold < 0.0) {
+                reason = ORC_KSP_DIVERGED_INDEFINITE_PC;
+                break;
+            }
+            if (!i) {
+                memcpy(P, Z, sizeof(double) * n);
+            } else {
+                const double bb = beta / betaold;
+                for (long k = 0; k < n; k++) P[k] = Z[k] + bb * P[k];  /* VecAYPX(P, b, Z) */
+            }
+            betaold = beta;
+            dpiold = dpi;
+            OPV(P, W);
+            dpi = orc_dot(P, W, n);
+            if (dpi == 0.0 || (i > 0 && dpi * dpiold <= 0.0)) {
+                reason = ORC_KSP_DIVERGED_INDEFINITE_MAT;
+                break;
+            }
+            const double al = beta / dpi;
+            for (long k = 0; k < n; k++) {
+                x[k] = x[k] + al * P[k];
+                R_[k] = R_[k] + (-al) * W[k];
+            }
+            pc_apply(&pc, R_, Z, n, 0);
+            dp = orc_norm2(Z, n);
+            if (trace && tlen < trace_cap) trace[tlen++] = dp;
+            reason = ksp_converged(&cv_, i + 1, dp);
+            if (reason) break;
+            beta = orc_dot(Z, R_, n);
+            i++;
+        } while (i < max_it);
+        if (i >= max_it && !reason) reason = ORC_KSP_DIVERGED_ITS;
+    }
+    double t1 = now_seconds();
+    ksp_finish(st, its, reason, x, b, W, nx, ny, nz, fdn, a, sigma, cv, coeff, tlen, t1 - t0);
+    free(dinv); free(F); free(R_); free(Z); free(P); free(W);
+    return 0;
+}
+
+int orc_kspcg(int nx, int ny, int nz, int fdn, double a, double sigma, const double *c,
+              double *x, const double *b, int max_it, double rtol, int precond,
+              double *trace, int trace_cap, orc_stats *st)
+{
+    return kspcg_impl(nx, ny, nz, fdn, a, sigma, c, x, b, max_it, rtol, precond, trace, trace_cap, st);
+}
+
 int orc_bicg(int nx, int ny, int nz, int fdn, double a, double sigma, const double *c,
              double *x, const double *b, int max_it, double rtol, int precond,
              double *trace, int trace_cap, orc_stats *st, double *shadow_gap)
@@ -1401,7 +1511,7 @@ int orc_aar_petsc(int nx, int ny, int nz, int fdn, double a, double sigma, const
                 }
                 g[i] = orc_dot(DF + (long)i * n, f, n);
             }
-            if (orc_pinv_times_vec(G, g, s, m, NULL, NULL)) status = 1;
+            if (orc_pinv_times_vec_gelsd(G, g, s, m, NULL, NULL)) status = 1;
             for (int q = 0; q < m; q++)
                 for (long i = 0; i < n; i++)
                     x[i] = x[i] + (-s[q]) * (beta * DF[(long)q * n + i] + DX[(long)q * n + i]);
@@ -1487,7 +1597,7 @@ int orc_pgr_petsc(int nx, int ny, int nz, int fdn, double a, double sigma, const
                 for (int j = 0; j < m; j++) G[i * m + j] = orc_dot(L + (long)i * n, DF + (long)j * n, n);
                 g[i] = orc_dot(L + (long)i * n, res, n);
             }
-            if (orc_pinv_times_vec(G, g, s, m, NULL, NULL)) status = 1;
+            if (orc_pinv_times_vec_gelsd(G, g, s, m, NULL, NULL)) status = 1;
             for (int q = 0; q < m; q++)
                 for (long i = 0; i < n; i++) x[i] = x[i] + s[q] * DX[(long)q * n + i];
             for (long i = 0; i < n; i++) DX[(long)k * n + i] = (-1.0) * x_old[i] + x[i];

@@ -50,6 +50,11 @@ double orc_dot(const double *u, const double *v, long n);
  * A is m*m row-major (A[i*m+j] is the reference's A[i][j]).  Returns 0, or
  * 1 when the QR sweep exceeds 250 iterations (the reference exits(1)).
  * w_out, v_out (optional) receive the truncated singular values and V. */
+/* The same with LAPACK dgelsd's rcond = -1 cutoff (singular values
+ * s_j <= 2^-53 * s_max dropped), the least squares of the PETSc variant
+ * (PETSC_REAL/AAR_Real.c:223, PGR_Real.c:216, PL2R_Real.c:220). */
+int orc_pinv_times_vec_gelsd(const double *Ain, const double *b, double *x, int m,
+                             double *w_out, double *v_out);
 /* The same with the textbook Golub-Reinsch left reflector (columns
  * i+1..n-1); the complex solver's m x m solves use it. */
 int orc_pinv_times_vec_textbook(const double *A, const double *b, double *x, int m,
@@ -150,7 +155,9 @@ void orc_matvec_complex(int nx, int ny, int nz, int fdn, const double *coeff,
 #define ORC_KSP_DIVERGED_DTOL (-4)
 #define ORC_KSP_DIVERGED_BREAKDOWN (-5)
 #define ORC_KSP_DIVERGED_BREAKDOWN_BICG (-6)
+#define ORC_KSP_DIVERGED_INDEFINITE_PC (-8)
 #define ORC_KSP_DIVERGED_NANORINF (-9)
+#define ORC_KSP_DIVERGED_INDEFINITE_MAT (-10)
 /* precond for orc_gmres / orc_bicg: 0 none, 1 Jacobi, 2 ILU(0) (below). */
 
 /* ILU(0) of the periodic FD12 star matrix a*Lap + sigma (+ c(x)), the
@@ -167,8 +174,8 @@ int orc_ilu0_factor(int nx, int ny, int nz, int fdn, double a, double sigma, con
 void orc_ilu0_apply(int nx, int ny, int nz, const double *F, const double *in, double *out, int trans);
 /* AAR of the PETSc variant, PETSC_REAL/AAR_Real.c:32-199 (its pc = 0/1:
  * precond 1 Jacobi by the reciprocal diagonal, 2 ILU(0) standing in for
- * PCICC, 0 none), with the tools.c pseudo-inverse standing in for
- * dgelsd.  trace: ||b - A x|| at every check (iter % p == 0); stats as
+ * PCICC, 0 none); its dgelsd least squares restated with dgelsd's rcond =
+ * -1 cutoff on the Golub-Reinsch SVD (orc_pinv_times_vec_gelsd).  trace: ||b - A x|| at every check (iter % p == 0); stats as
  * orc_aar (iterations = iter - 1). */
 int orc_aar_petsc(int nx, int ny, int nz, int fdn, double a, double sigma, const double *c,
                   double *x, const double *b, double omega, double beta, int m, int p, int max_iter,
@@ -200,6 +207,14 @@ int orc_ilu0_factor_complex(int nx, int ny, int nz, int fdn, double a, const dou
 void orc_ilu0_apply_complex(int nx, int ny, int nz, const double *F, const double *in, double *out, int trans);
 int orc_gmres(int nx, int ny, int nz, int fdn, double a, double sigma, const double *c,
               double *x, const double *b, int restart, int max_it, double rtol, int precond,
+              double *trace, int trace_cap, orc_stats *st);
+/* CG of the PETSc variant, PETSC_REAL/main.c:152-175 (KSPCG with the
+ * preconditioner of its pc: 1 Jacobi, 2 ILU(0), 0 none; rtol, max_it
+ * 2000, nonzero initial guess): PETSc's KSPSolve_CG with the preconditioned
+ * residual norm and KSPConvergedDefault, its beta = 0, indefinite-PC and
+ * indefinite-matrix exits; trace and stats as orc_bicg. */
+int orc_kspcg(int nx, int ny, int nz, int fdn, double a, double sigma, const double *c,
+              double *x, const double *b, int max_it, double rtol, int precond,
               double *trace, int trace_cap, orc_stats *st);
 int orc_bicg(int nx, int ny, int nz, int fdn, double a, double sigma, const double *c,
              double *x, const double *b, int max_it, double rtol, int precond,

@@ -72,6 +72,8 @@ def lib():
         L.orc_pinv_times_vec.argtypes = [_D, _D, _D, I, _D, _D]
         L.orc_pinv_times_vec.restype = I
         L.orc_pinv_times_vec_textbook.argtypes = [_D, _D, _D, I, _D, _D]
+        L.orc_pinv_times_vec_gelsd.argtypes = [_D, _D, _D, I, _D, _D]
+        L.orc_pinv_times_vec_gelsd.restype = I
         L.orc_pinv_times_vec_textbook.restype = I
         L.orc_pythag.argtypes = [Dd, Dd]
         L.orc_pythag.restype = Dd
@@ -93,6 +95,7 @@ def lib():
         L.orc_cg_vc.argtypes = [I, I, I, I, Dd, _D, _D, _D, I, Dd, I, _D, I, ctypes.POINTER(OrcStats)]
         L.orc_gmres.argtypes = [I, I, I, I, Dd, Dd, _D, _D, _D, I, I, Dd, I, _D, I, ctypes.POINTER(OrcStats)]
         L.orc_bicg.argtypes = [I, I, I, I, Dd, Dd, _D, _D, _D, I, Dd, I, _D, I, ctypes.POINTER(OrcStats), _D]
+        L.orc_kspcg.argtypes = [I, I, I, I, Dd, Dd, _D, _D, _D, I, Dd, I, _D, I, ctypes.POINTER(OrcStats)]
         L.orc_ilu0_factor.argtypes = [I, I, I, I, Dd, Dd, _D, _D]
         L.orc_aar_petsc.argtypes = [I, I, I, I, Dd, Dd, _D, _D, _D, Dd, Dd, I, I, I, Dd, I, _D, I,
                                     ctypes.POINTER(OrcStats)]
@@ -146,14 +149,17 @@ def norm2(v):
     return lib().orc_norm2(_p(v), v.size)
 
 
-def pinv_times_vec(A, b, textbook=False):
+def pinv_times_vec(A, b, textbook=False, gelsd=False):
+    """tools.c:52-105 (textbook: the textbook left reflector; gelsd: the
+    dgelsd rcond = -1 cutoff of the PETSc variant)."""
     A = _f64(A)
     m = A.shape[0]
     b = _f64(b)
     x = np.zeros(m)
     w = np.zeros(m)
     V = np.zeros((m, m))
-    fn = lib().orc_pinv_times_vec_textbook if textbook else lib().orc_pinv_times_vec
+    fn = (lib().orc_pinv_times_vec_gelsd if gelsd else
+          lib().orc_pinv_times_vec_textbook if textbook else lib().orc_pinv_times_vec)
     st = fn(_p(A), _p(b), _p(x), m, _p(w), _p(V))
     if st:
         raise RuntimeError("SVD did not converge in 250 sweeps")
@@ -372,6 +378,25 @@ def bicg_complex(u0, rhs, shape, a, c, c0, eta, max_iter, tol, precond=True, fdn
     d = _run(lib().orc_bicg_complex, (nx, ny, nz, fdn, a, _p(cc), c0, eta, _p(u.view(np.float64)),
                                       _p(b.view(np.float64)), max_iter, tol, int(precond)), trace, max_iter + 8)
     return u, d
+
+
+def kspcg(x0, rhs, shape, a, max_iter, tol, sigma=0.0, c=None, precond=True, fdn=6, trace=True):
+    """PETSc KSPCG as PETSC_REAL/main.c:152-175 runs it (restatement);
+    precond 1 Jacobi, 2 ILU(0) (its pc = 1), 0 none."""
+    nx, ny, nz = shape
+    x = _f64(x0).reshape(-1).copy()
+    b = _f64(rhs).reshape(-1)
+    cv = None if c is None else _f64(c).reshape(-1)
+    st = OrcStats()
+    cap = max_iter + 8
+    tr = np.zeros(cap) if trace else None
+    rc = lib().orc_kspcg(nx, ny, nz, fdn, a, sigma, _p(cv), _p(x), _p(b), max_iter, tol, int(precond),
+                         _p(tr), cap if trace else 0, ctypes.byref(st))
+    d = st.as_dict()
+    d["rc"] = rc
+    if trace:
+        d["trace"] = tr[: st.trace_len].copy()
+    return x, d
 
 
 def bicg(x0, rhs, shape, a, max_iter, tol, sigma=0.0, c=None, precond=True, fdn=6, trace=True):

@@ -539,3 +539,59 @@ def test_petsc_variant_max_iter_zero(gpu, oracle):
     xg = dev(x0)
     st = gpu.PGR_petsc(xg, dev(b), shape, a, 0.6, 7, 6, 0, 1e-6, precond=True)
     assert st["iterations"] == d["iterations"] == 0 and np.array_equal(xg.cpu().numpy(), x0)
+
+
+@pytest.mark.parametrize("case", BI_CASES, ids=_bid)
+def test_kspcg_parity(gpu, oracle, case):
+    """The PETSc variant's CG (KSPCG, PETSC_REAL/main.c:152-175) with
+    Jacobi or none against the oracle's restatement of KSPSolve_CG."""
+    shape, h, sigma, max_it, tol, precond = case
+    say("kspcg", _bid(case))
+    a = P.poisson_a(h)
+    b = rhs(shape, 1001)
+    x0 = np.ones(b.size)
+    xo, d = oracle.kspcg(x0, b, shape, a, max_it, tol, sigma=sigma, precond=precond)
+    xg = dev(x0)
+    st = gpu.KSPCG(xg, dev(b), shape, a, max_it, tol, sigma=sigma, precond=precond)
+    check_close(gpu, st, d, xg.cpu().numpy(), xo, b, op_norm(oracle, a, sigma))
+
+
+@pytest.mark.parametrize("case", ILU_CASES, ids=_iid)
+def test_kspcg_ilu0_parity(gpu, oracle, case):
+    """KSPCG with the variant's pc = 1 (PCBJACOBI -> ILU(0) on one block)."""
+    shape, sigma, max_it, tol = case
+    say("kspcg ilu0", _iid(case))
+    a = P.poisson_a(0.5)
+    b = rhs(shape, 1001)
+    x0 = np.ones(b.size)
+    xo, d = oracle.kspcg(x0, b, shape, a, max_it, tol, sigma=sigma, precond=2)
+    xg = dev(x0)
+    st = gpu.KSPCG(xg, dev(b), shape, a, max_it, tol, sigma=sigma, precond=gpu.ILU0)
+    check_close(gpu, st, d, xg.cpu().numpy(), xo, b, op_norm(oracle, a, sigma))
+
+
+def test_kspcg_variable_coefficient_and_exits(gpu, oracle):
+    """KSPCG on a*Lap + c(x) (Jacobi and ILU(0)); a zero right-hand side
+    converges on atol with no step; an indefinite screened operator (sigma
+    = -60) ends with the oracle's divergence reason and count."""
+    shape = (20, 20, 20)
+    a = P.poisson_a(0.5)
+    b = rhs(shape, 1005)
+    c = P.helmholtz_coefficient(shape, 7)
+    x0 = np.ones(b.size)
+    for pc in (1, 2):
+        say("kspcg vc", pc)
+        xo, d = oracle.kspcg(x0, b, shape, a, 2000, 1e-8, c=c, precond=pc)
+        xg = dev(x0)
+        st = gpu.KSPCG(xg, dev(b), shape, a, 2000, 1e-8, c=dev(c), precond=pc if pc == 1 else gpu.ILU0)
+        check_close(gpu, st, d, xg.cpu().numpy(), xo, b, op_norm(oracle, a, c=c))
+    n = b.size
+    xg = dev(np.zeros(n))
+    st = gpu.KSPCG(xg, dev(np.zeros(n)), shape, a, 100, 1e-6)
+    assert st["reason"] == 3 and st["iterations"] == 0
+    xo, d = oracle.kspcg(x0, b, shape, a, 500, 1e-8, sigma=-60.0)
+    xg = dev(x0)
+    st = gpu.KSPCG(xg, dev(b), shape, a, 500, 1e-8, sigma=-60.0)
+    say("kspcg indefinite: oracle", d["iterations"], d["status"], "gpu", st["iterations"], st["reason"])
+    assert d["status"] < 0
+    assert st["reason"] == d["status"] and st["iterations"] == d["iterations"]

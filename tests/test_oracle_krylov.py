@@ -16,6 +16,8 @@ rounding; they are pinned to what the mathematics fixes:
 import numpy as np
 import pytest
 
+from aar_hip import problems as P
+
 
 def fft_solve(b, shape, a, coeff, sigma):
     """Spectral solve of a*Lap(x) + sigma x = b on the periodic grid
@@ -716,3 +718,70 @@ def test_krylov_checks_catch_planted_mistakes(oracle):
     _, st = oracle.bicg(np.ones(b.size), b, shape, a, 8, 1e-14, sigma=sigma)
     np.testing.assert_allclose(st["trace"], pcg_norms(False), rtol=1e-8)
     assert np.max(np.abs(st["trace"] - pcg_norms(True)) / st["trace"]) > 1e-3
+
+
+def test_gelsd_cutoff_restatement(oracle):
+    """The PETSc variant's least squares (LAPACKE_dgelsd with rcond = -1,
+    PETSC_REAL/AAR_Real.c:223) drops singular values s <= 2^-53 s_max, where
+    tools.c drops s < m s_max 2.22e-16.  On diagonal systems (whose SVD is
+    exact) a singular value between the two cutoffs is kept by the dgelsd
+    restatement and by scipy's gelsd driver, and dropped by tools.c; one at
+    exactly 2^-53 s_max is dropped by both.  On a well-conditioned SPD system
+    the restatement equals scipy's gelsd to rounding."""
+    import scipy.linalg as sl
+    b = np.array([1.0, 1.0, 1.0])
+    A = np.diag([1.0, 0.5, 5e-16])
+    xg, wg, _ = oracle.pinv_times_vec(A, b, gelsd=True)
+    xt, wt, _ = oracle.pinv_times_vec(A, b)
+    xs = sl.lstsq(A, b, cond=-1, lapack_driver="gelsd")[0]
+    assert abs(xg[2] / 2e15 - 1.0) < 1e-14 and xt[2] == 0.0
+    assert np.allclose(xg, xs, rtol=1e-14)
+    A2 = np.diag([1.0, 2.0 ** -53, 0.25])
+    xg2, _, _ = oracle.pinv_times_vec(A2, b, gelsd=True)
+    assert xg2[1] == 0.0 and np.allclose(xg2[[0, 2]], [1.0, 4.0], rtol=1e-15)
+    rng = np.random.default_rng(5)
+    F = rng.standard_normal((40, 7))
+    G = F.T @ F
+    g = rng.standard_normal(7)
+    xg3, _, _ = oracle.pinv_times_vec(G, g, gelsd=True)
+    assert np.allclose(xg3, sl.lstsq(G, g, cond=-1, lapack_driver="gelsd")[0], rtol=1e-11, atol=0)
+
+
+@pytest.mark.parametrize("precond", [0, 1, 2])
+def test_kspcg_restatement_pins(oracle, precond):
+    """KSPCG (PETSc's KSPSolve_CG) on these symmetric operators with a
+    symmetric preconditioner is BiCG with its shadow sequence equal to the
+    primal one: the oracle's KSPCG reproduces its KSPBICG restatement's
+    iterates and counts (bit for bit with Jacobi or none; ILU(0)'s M^{-T}
+    differs from M^{-1} in rounding), and converges to the dense solution.
+    A planted error in either recurrence would break the equality."""
+    shape = (14, 13, 13)
+    a = P.poisson_a(0.5)
+    b = P.rhs_uniform(shape, 1001)
+    x0 = np.ones(b.size)
+    x, d = oracle.kspcg(x0, b, shape, a, 2000, 1e-10, sigma=0.3, precond=precond)
+    xb, db = oracle.bicg(x0, b, shape, a, 2000, 1e-10, sigma=0.3, precond=precond)
+    assert d["iterations"] == db["iterations"] and d["status"] == db["status"] == 2
+    if precond == 2:
+        assert np.linalg.norm(x - xb) <= 1e-12 * np.linalg.norm(xb)
+    else:
+        assert np.array_equal(x, xb)
+    n = b.size
+    M = np.zeros((n, n))
+    e = np.zeros(n)
+    for j in range(n):
+        e[j] = 1.0
+        M[:, j] = oracle.matvec(e, shape, a, 0.3)
+        e[j] = 0.0
+    xs = np.linalg.solve(M, b)
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-8
+
+
+def test_kspcg_indefinite_exit(oracle):
+    """An indefinite operator (screened with sigma = -60) stops KSPCG with
+    a divergence reason (indefinite matrix or preconditioner), not a
+    convergence."""
+    shape = (12, 12, 12)
+    b = P.rhs_uniform(shape, 3)
+    _, d = oracle.kspcg(np.ones(b.size), b, shape, P.poisson_a(0.5), 500, 1e-8, sigma=-60.0)
+    assert d["status"] in (-8, -10)
