@@ -199,6 +199,170 @@ __global__ __launch_bounds__(BLOCK) void gram_finish_kernel(const GramArgs<T> a)
     if (threadIdx.x == 0) gram_layout<T, MM, KIND>(a, sums);
 }
 
+// ---------------------------------------------------------------------------
+// PGR's normal equations at the deep histories (m > 8, e.g. the reference
+// READMEs' m = 13): m^2 + m products per point are too many FP64
+// accumulators for one thread.  Each of the block's four
+// wavefronts owns a quarter of the products and walks the same points (the
+// block's chunks, from the caches after the first wavefront's loads), so
+// HBM reads each ring slot once and no accumulator leaves the registers.
+// ---------------------------------------------------------------------------
+template <int MM, int KIND>
+struct GramIdx {
+    static constexpr int NG = (KIND == 1) ? MM * MM : MM * (MM + 1) / 2;
+    static constexpr int NA = NG + MM;
+    // product t: column pair (j, q) of G, or (j, -1) for col_j . f
+    static constexpr int j_of(int t)
+    {
+        if (t >= NG) return t - NG;
+        if (KIND == 1) return t / MM;
+        int j = 0;
+        while ((j + 1) * (j + 2) / 2 <= t) j++;
+        return j;
+    }
+    static constexpr int q_of(int t)
+    {
+        if (t >= NG) return -1;
+        if (KIND == 1) return t % MM;
+        const int j = j_of(t);
+        return t - j * (j + 1) / 2;
+    }
+};
+
+template <typename T, int MM, int KIND, int W>
+__device__ __forceinline__ void gram_split_part(const GramArgs<T> &a)
+{
+    using Ix = GramIdx<MM, KIND>;
+    constexpr int NA = Ix::NA;
+    constexpr int NWV = BLOCK / 64;
+    constexpr int PER = (NA + NWV - 1) / NWV;
+    constexpr int T0 = W * PER < NA ? W * PER : NA;
+    constexpr int T1 = (W + 1) * PER < NA ? (W + 1) * PER : NA;
+    constexpr int NT = T1 > T0 ? T1 - T0 : 1;
+    constexpr int NS = MM + 1;
+    const int m = a.m, nv = a.nvalid, dir = a.direct;
+    (void)m;
+    const T *D = (KIND == 0) ? a.F : a.Y;
+    const T *pd[NS], *px[NS];
+    bool use[NS];
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+        use[k] = dir ? (k < nv) : (k <= nv);
+        const size_t off = (size_t)(use[k] ? a.slot[k] : 0) * a.ld;
+        pd[k] = D + off;
+        px[k] = a.F + off;
+    }
+    const T *pf = (KIND == 0 ? a.F : a.Y) + (size_t)a.fslot * a.ld;
+    const bool f_is_d0 = (KIND == 0) && !dir && a.fslot == a.slot[0];
+    double acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[t] = 0.0;
+
+    // this wavefront's products at one point
+    auto point = [&](T fv, const T (&vd)[NS], const T (&vx)[NS]) {
+        double d[MM], e[MM];
+#pragma unroll
+        for (int k = 0; k < MM; k++) {
+            d[k] = 0.0;
+            e[k] = 0.0;
+            if (k < nv) {
+                d[k] = dir ? (double)vd[k] : (double)(vd[k] - vd[k + 1]);        // DF = f - f_old
+                if constexpr (KIND == 1)
+                    e[k] = dir ? (double)vx[k] : (double)(vx[k] - vx[k + 1]);    // DX = x - x_old
+            }
+        }
+        const double fd = (double)fv;
+#pragma unroll
+        for (int t = T0; t < T1; t++) {
+            constexpr int dummy = 0;
+            (void)dummy;
+            const int j = Ix::j_of(t), q = Ix::q_of(t);
+            const double u = (q < 0) ? fd : (KIND == 1 ? e[q < 0 ? 0 : q] : d[q < 0 ? 0 : q]);
+            const double v = (q < 0 && KIND == 1) ? e[j] : d[j];
+            acc[t - T0] += u * v;
+        }
+    };
+
+    const int lane = threadIdx.x & 63;
+    constexpr int VW = 16 / sizeof(T);
+    using VT = T __attribute__((ext_vector_type(VW)));
+    const long n = a.n;
+    const bool vec_ok = (n % VW) == 0 && (KIND == 0 || ((uintptr_t)a.b % 16) == 0);
+    const long nq = vec_ok ? n / VW : n;
+    for (long c0 = (long)blockIdx.x * CHUNK_V; c0 < nq; c0 += (long)gridDim.x * CHUNK_V) {
+        const long c1 = c0 + CHUNK_V < nq ? c0 + CHUNK_V : nq;
+        if (vec_ok) {
+            for (long iq = c0 + lane; iq < c1; iq += 64) {
+                const long i = VW * iq;
+                VT vd[NS], vx[NS];
+#pragma unroll
+                for (int k = 0; k < NS; k++) {
+                    vd[k] = (VT)(T)0;
+                    vx[k] = (VT)(T)0;
+                    if (use[k]) {
+                        vd[k] = *reinterpret_cast<const VT *>(pd[k] + i);
+                        if constexpr (KIND == 1) vx[k] = *reinterpret_cast<const VT *>(px[k] + i);
+                    }
+                }
+                VT fv = f_is_d0 ? vd[0] : *reinterpret_cast<const VT *>(pf + i);
+                if constexpr (KIND != 0) fv = *reinterpret_cast<const VT *>(a.b + i) - fv;   // PGR.c:102-103
+#pragma unroll
+                for (int e = 0; e < VW; e++) {
+                    T sd[NS], sx[NS];
+#pragma unroll
+                    for (int k = 0; k < NS; k++) {
+                        sd[k] = vd[k][e];
+                        sx[k] = vx[k][e];
+                    }
+                    point(fv[e], sd, sx);
+                }
+            }
+        } else {
+            for (long i = c0 + lane; i < c1; i += 64) {
+                T vd[NS], vx[NS];
+#pragma unroll
+                for (int k = 0; k < NS; k++) {
+                    vd[k] = use[k] ? pd[k][i] : (T)0;
+                    vx[k] = (KIND == 1 && use[k]) ? px[k][i] : (T)0;
+                }
+                T f = pf[i];
+                if constexpr (KIND != 0) f = a.b[i] - f;
+                point(f, vd, vx);
+            }
+        }
+    }
+    // this wavefront owns products T0..T1-1: one value-major partial each
+#pragma unroll
+    for (int t = T0; t < T1; t++) {
+        const double v = wave_sum(acc[t - T0]);
+        if (lane == 0) a.partials[(size_t)t * a.nblocks + blockIdx.x] = v;
+    }
+}
+
+template <typename T, int MM, int KIND>
+__global__ __launch_bounds__(BLOCK) void gram_split_kernel(const GramArgs<T> a)
+{
+    static_assert(BLOCK == 256, "four wavefronts share the products");
+    constexpr int NA = GramIdx<MM, KIND>::NA;
+    static_assert(NA <= GRAM_NA_MAX, "partials buffers hold GRAM_NA_MAX per block");
+    __shared__ double sums[NA];
+    if (a.ctrl->done) return;
+    switch (threadIdx.x >> 6) {
+    case 0: gram_split_part<T, MM, KIND, 0>(a); break;
+    case 1: gram_split_part<T, MM, KIND, 1>(a); break;
+    case 2: gram_split_part<T, MM, KIND, 2>(a); break;
+    default: gram_split_part<T, MM, KIND, 3>(a); break;
+    }
+    if (!a.counter) return;                     // two-pass: gram_finish_kernel follows
+    if (last_block_arrive(a.counter, gridDim.x)) {
+        final_sum_vm(a.partials, a.nblocks, NA, sums);
+        if (threadIdx.x == 0) {
+            gram_layout<T, MM, KIND>(a, sums);
+            *a.counter = 0u;
+        }
+    }
+}
+
 template <typename T>
 hipError_t launch_gram(const GramArgs<T> &a, hipStream_t st)
 {
@@ -212,9 +376,7 @@ hipError_t launch_gram(const GramArgs<T> &a, hipStream_t st)
         return hipGetLastError();
     }
     if (a.kind < 0 || a.kind > 2) return hipErrorInvalidValue;
-    // m <= 8: every accumulator in registers; up to 16 (the reference's
-    // documented m = 13): the same kernel with 152 accumulators, at lower
-    // occupancy
+    // m <= 8: every accumulator in registers
     auto go = [&](auto mm) {
         constexpr int M = decltype(mm)::value;
         switch (a.kind) {
@@ -230,7 +392,21 @@ hipError_t launch_gram(const GramArgs<T> &a, hipStream_t st)
             }
         }
     };
+    // m > 8.  AAR's and PL2R's triangles (m(m+1)/2 + m sums) still fit one
+    // thread's registers at the exact m = 13 of the READMEs (2.9 ms against
+    // 4.4 ms padded to 16 at 512^3); PGR's square m^2 + m (182 at m = 13)
+    // does not, so its products are shared out over the wavefronts (10.9 ms
+    // against 32 ms unsplit)
+    auto go_split = [&](auto mm) {
+        constexpr int M = decltype(mm)::value;
+        hipLaunchKernelGGL((gram_split_kernel<T, M, 1>), grid, block, 0, st, a);
+        if (!a.counter) hipLaunchKernelGGL((gram_finish_kernel<T, M, 1>), dim3(1), block, 0, st, a);
+    };
     if (a.m <= MAXM_FAST) go(std::integral_constant<int, MAXM_FAST>());
+    else if (a.kind == 1) {
+        if (a.m == 13) go_split(std::integral_constant<int, 13>());
+        else go_split(std::integral_constant<int, MAXM>());
+    } else if (a.m == 13) go(std::integral_constant<int, 13>());
     else go(std::integral_constant<int, MAXM>());
     return hipGetLastError();
 }
