@@ -173,14 +173,21 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
 #pragma unroll
     for (int o = 0; o <= R; o++) cf[o] = (T)a.cf[o];
     const double trace_scale = a.precond ? a.diag : 1.0;
-    // norm terms of points past the grid edge are multiplied by 0 (exact:
-    // x * 1 = x), so the accumulation needs no select per point
+    // FP64 sweeps: norm terms of points past the grid edge are multiplied by
+    // 0 (exact: x * 1 = x), so the accumulation needs no select per point.
+    // The FP32 sweeps (four waves per SIMD at <= 128 VGPRs) keep the select:
+    // the factors would take them past 128.
+    constexpr bool MF = sizeof(T) == 8;
     double okf[PY], okts[PY];
 #pragma unroll
     for (int p = 0; p < PY; p++) {
         okf[p] = c_ok[p] ? 1.0 : 0.0;
         okts[p] = c_ok[p] ? trace_scale : 0.0;
     }
+    auto nmask = [&](int p, double v) -> double {
+        if constexpr (MF) return v * okf[p];
+        else return c_ok[p] ? v : 0.0;
+    };
     const T cgb = (MODE == SMode::CG_Q && a.cg_beta) ? (T)(*a.cg_beta) : (T)0;
 
     // fields at in-plane offset `off` of the plane starting at element zo
@@ -320,8 +327,15 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
                     if (pp >= 0 && pp < PY) return q[pp < 0 ? 0 : (pp < PY ? pp : PY - 1)][R];
                     return lds[buf][lr + d][lc];
                 };
-                auto xl = [&](int o) -> T { return xrow[R - o]; };
-                auto xr = [&](int o) -> T { return xrow[R + o]; };
+                // (FP32: indexed from lc, which keeps it within 128 VGPRs)
+                auto xl = [&](int o) -> T {
+                    if constexpr (MF) return xrow[R - o];
+                    else return lds[buf][lr][lc - o];
+                };
+                auto xr = [&](int o) -> T {
+                    if constexpr (MF) return xrow[R + o];
+                    else return lds[buf][lr][lc + o];
+                };
                 // system.c:195-201 order: centre, then o = 1..6 with
                 // (x-, x+, y-, y+, z-, z+) summed left to right
                 T y;
@@ -402,21 +416,26 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
                 const unsigned g = c_off[p];
                 // outputs are next read a whole sweep later (beyond the
                 // caches at the sizes where it matters): non-temporal stores
-                // (2% faster for the recurrence sweep at 512^3)
+                // in the FP64 sweeps (2% faster for the recurrence sweep at
+                // 512^3); the FP32 sweeps keep the plane-relative store, whose
+                // 32-bit offsets hold them within 128 VGPRs
                 auto put = [&](auto *base, auto v) {
-                    if (USTORE || ok) __builtin_nontemporal_store(v, base + zoff + g);
+                    if (USTORE || ok) {
+                        if constexpr (MF) __builtin_nontemporal_store(v, base + zoff + g);
+                        else st_pl<SA>(base, zoff, g, v);
+                    }
                 };
                 if constexpr (MODE == SMode::MATVEC) {
                     put(a.out0, y);
                 } else if constexpr (MODE == SMode::MATVEC_RES) {
                     // y = A x and ||b - A x||^2 (PGR.c:79-81 start-up)
                     put(a.out0, y);
-                    acc[0] += A::sq(A::sub(cur.en[p][0], y)) * okf[p];
+                    acc[0] += nmask(p, A::sq(A::sub(cur.en[p][0], y)));
                 } else if constexpr (MODE == SMode::AAR_TRUE) {
                     // AAR.c:80-82/93-94/119-123: f = (rhs - A x)/diag, stored in
                     // T or (mixed precision) rounded to FP32; ||rhs - A x||^2
                     const T r = A::sub(cur.en[p][0], y);
-                    acc[0] += A::sq(r) * okf[p];
+                    acc[0] += nmask(p, A::sq(r));
                     T f = r;
                     if constexpr (VC) {
                         if (a.precond) f = A::div(f, A::add(dg, cur.vv[p]));
@@ -433,15 +452,19 @@ __device__ __forceinline__ void stencil_body(const SArgs<T> &a, const int item)
                         // ||rhs - A x||: D g' pointwise (VC: D = omega/h, by
                         // the hardware reciprocal -- the trace is a record,
                         // convergence is decided on the AAR_TRUE sweep)
-                        double sc = okts[p];
+                        double sv;
                         if constexpr (VC)
-                            sc = a.precond ? okf[p] * (a.omega * __builtin_amdgcn_rcp((double)cur.vv[p])) : okf[p];
-                        const double sv = (double)y * sc;
+                            sv = (double)y * (a.precond ? a.omega * __builtin_amdgcn_rcp((double)cur.vv[p]) : 1.0);
+                        else if constexpr (MF)
+                            sv = (double)y * okts[p];
+                        else
+                            sv = (double)y * trace_scale;
+                        if constexpr (VC || !MF) sv = nmask(p, sv);
                         acc[0] = fma(sv, sv, acc[0]);
                     }
                 } else if constexpr (MODE == SMode::RESID) {
                     const T r = A::sub(cur.en[p][0], y);
-                    acc[0] += A::sq(r) * okf[p];
+                    acc[0] += nmask(p, A::sq(r));
                     if (a.out32) put(a.out32, (float)r);
                     if (a.out0) put(a.out0, r);
                 } else if constexpr (MODE == SMode::PGR_RICH) {

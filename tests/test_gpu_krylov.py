@@ -245,23 +245,46 @@ def test_krylov_ilu0_variable_coefficient_and_refusal(gpu, oracle):
         gpu.BiCG(dev(np.ones(bs.size)), dev(bs), small, a, 10, 1e-6, precond=gpu.ILU0)
 
 
-def test_krylov_ilu0_at_scale(gpu):
-    """128^3 Poisson (h = 0.25) with ILU(0): converged, true relative
-    residual near the tolerance, repeat runs identical."""
+# bounds on the true relative residual at 128^3 with ILU(0) (GMRES(30),
+# BiCG): measured 3.0e-7 and 5.7e-7 against rtol 1e-6 on the preconditioned
+# norm
+TRUE_RES_128 = (2e-6, 2e-6)
+
+
+def test_krylov_ilu0_at_scale(gpu, oracle):
+    """ILU(0) GMRES(30) and BiCG, Poisson h = 0.25 with x0 = 1.  At 64^3 the
+    oracle's iteration count and KSPConvergedReason (the ILU sweeps are
+    bit-exact against it).  At 128^3: converged, the true relative residual
+    ||b - A x|| / ||b|| of the returned x (bit-exact matvec) near the
+    tolerance, repeat runs identical."""
     import torch
-    shape = (128, 128, 128)
-    say("krylov ilu0 128^3")
     a = P.poisson_a(0.25)
+    shape = (64, 64, 64)
     b = P.rhs_uniform_torch(shape, 1002, device="cuda")
-    for fn in (gpu.GMRES, gpu.BiCG):
+    bh = b.cpu().numpy()
+    for fn, ofn in ((gpu.GMRES, oracle.gmres), (gpu.BiCG, oracle.bicg)):
+        say("krylov ilu0 64^3", fn.__name__)
+        x = torch.ones_like(b)
+        st = fn(x, b, shape, a, 2000, 1e-6, precond=gpu.ILU0)
+        xo, d = ofn(np.ones(bh.size), bh, shape, a, 2000, 1e-6, precond=2, trace=False)
+        say(fn.__name__, "gpu", st["iterations"], st["reason"], "oracle", d["iterations"], d["status"])
+        assert st["iterations"] == d["iterations"] and st["reason"] == d["status"] == 2
+        assert np.linalg.norm(x.cpu().numpy() - xo) <= 1e-8 * np.linalg.norm(xo)
+    shape = (128, 128, 128)
+    b = P.rhs_uniform_torch(shape, 1002, device="cuda")
+    for fn, bound in ((gpu.GMRES, TRUE_RES_128[0]), (gpu.BiCG, TRUE_RES_128[1])):
+        say("krylov ilu0 128^3", fn.__name__)
         x1 = torch.ones_like(b)
         s1 = fn(x1, b, shape, a, 2000, 1e-6, precond=gpu.ILU0)
         x2 = torch.ones_like(b)
         s2 = fn(x2, b, shape, a, 2000, 1e-6, precond=gpu.ILU0)
         assert s1["reason"] == 2 and s1["iterations"] == s2["iterations"]
         assert torch.equal(x1, x2)
-        assert s1["relres"] < 1e-4
-        say(fn.__name__, "ilu0", s1["iterations"], s1["relres"], s1["solve_ms"])
+        r = b - gpu.Lap_Vec_mult(x1, shape, a)
+        true_rel = float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(b))
+        say(fn.__name__, "ilu0", s1["iterations"], s1["relres"], "true", true_rel, s1["solve_ms"])
+        assert s1["relres"] <= 1e-6 and true_rel <= bound
+
 
 
 # ---------------------------------------------------------------------------
