@@ -162,6 +162,7 @@ int aar_hip_cg(const aar_hip_grid *g, double *x, const double *b, int max_iter, 
  * (main.c:112-118), reason = the KSPConvergedReason; aar_hip_history()
  * returns the norm of every convergence test. */
 #define AAR_HIP_GMRES_MAXK 32
+#define AAR_HIP_LGMRES_MAXAUG 8
 /* precond = 2 (both solvers): the pc = 1 preconditioner of the PETSc
  * variant on one rank, PCBJACOBI -> PCILU at 0 levels (main.c:88-91) = the
  * ILU(0) factors of the whole periodic matrix, computed on the device in
@@ -248,6 +249,18 @@ int aar_hip_helm_aar(const aar_hip_helm *h, double *u, const double *b, double o
  * interleaved (device).  stats as aar_hip_gmres. */
 int aar_hip_helm_gmres(const aar_hip_helm *h, double *u, const double *b, int restart, int max_iter, double tol,
                        int precond, aar_hip_stats *st);
+
+/* LGMRES of the complex PETSc variant, PETSC_COMPLEX/main.c:120-147
+ * (KSPLGMRES with PCJACOBI or, precond = 2, the complex ILU(0); PETSc's
+ * defaults restart = 30, aug = 2): each cycle takes restart - aug Arnoldi
+ * steps, then one step per stored error approximation z (newest first,
+ * its vector the stored B A z); every step is an iteration and is tested.
+ * The cycle's correction becomes the newest z (the oldest is dropped once
+ * aug are stored), with B A z = V H y from the unrotated Hessenberg, both
+ * scaled by 1/||z||.  1 <= restart <= 32, 0 <= aug < restart, aug <= 8;
+ * aug = 0 is aar_hip_helm_gmres.  Stats as aar_hip_helm_gmres. */
+int aar_hip_helm_lgmres(const aar_hip_helm *h, double *u, const double *b, int restart, int aug, int max_iter,
+                        double tol, int precond, aar_hip_stats *st);
 
 /* BiCG of the complex PETSc variant, PETSC_COMPLEX/main.c:150-175 (KSPBICG
  * with PCJACOBI): the shadow sequence with A^H (eta -> -eta; the operator

@@ -5,5 +5,5 @@ from .solvers import (AAR, CG, GMRES, ILU0, KSPCG, PGR, PL2R, AAR_batch, AAR_pet
                       Lap_Vec_mult_vc, autotune, history, last_system, pinv, profile, profile_query, profile_reset,
                       release, set_coefficient, set_option, set_stencil, set_tuning, stream_bandwidth, FDN_USER)
 from . import problems  # noqa: F401
-from .helmholtz import (HelmholtzSolve, helm_aar, helm_bicg, helm_gmres, helm_grad_c, helm_ilu0_apply,  # noqa: F401
+from .helmholtz import (HelmholtzSolve, helm_aar, helm_bicg, helm_gmres, helm_grad_c, helm_ilu0_apply, helm_lgmres,  # noqa: F401
                         helm_matvec)

@@ -65,7 +65,7 @@ EXTENSION_SYMBOLS = [
     "aar_hip_helm_matvec", "aar_hip_helm_aar", "aar_hip_helm_grad_c", "aar_hip_aar_batch",
     "aar_hip_autotune", "aar_hip_last_system", "aar_hip_matvec_vc", "aar_hip_aar_vc", "aar_hip_set_coefficient",
     "aar_hip_pgr_vc", "aar_hip_cg_vc", "aar_hip_gmres", "aar_hip_bicg", "aar_hip_ilu0_apply", "aar_hip_aar_petsc",
-    "aar_hip_pgr_petsc", "aar_hip_helm_gmres", "aar_hip_helm_bicg", "aar_hip_helm_ilu0_apply",
+    "aar_hip_pgr_petsc", "aar_hip_helm_gmres", "aar_hip_helm_lgmres", "aar_hip_helm_bicg", "aar_hip_helm_ilu0_apply",
     "aar_hip_stream_bandwidth", "aar_hip_pinv_noalloc", "aar_hip_set_stencil", "aar_hip_kspcg",
 ]
 
@@ -114,6 +114,7 @@ def load(require_gpu=True):
         "aar_hip_helm_aar": [P, P, P, D, D, I, I, I, D, I, S],
         "aar_hip_helm_grad_c": [ctypes.c_long, P, P, P],
         "aar_hip_helm_gmres": [P, P, P, I, I, D, I, S],
+        "aar_hip_helm_lgmres": [P, P, P, I, I, I, D, I, S],
         "aar_hip_helm_bicg": [P, P, P, I, D, I, S],
         "aar_hip_helm_ilu0_apply": [P, P, P, I],
         "aar_hip_aar_batch": [ctypes.POINTER(Instance), I, D, D, I, I, I, D, I, I, S],

@@ -111,6 +111,18 @@ def helm_gmres(u, b, shape, a, c=None, c0=0.0, eta=0.0, max_iter=2000, tol=1e-6,
     return st.as_dict()
 
 
+def helm_lgmres(u, b, shape, a, c=None, c0=0.0, eta=0.0, max_iter=2000, tol=1e-6, restart=30, aug=2,
+                precond=True):
+    """LGMRES of the complex PETSc variant (PETSC_COMPLEX/main.c:120-147,
+    KSPLGMRES with PETSc's defaults restart = 30, aug = 2) on A u = b."""
+    h = _helm(shape, a, c, c0, eta)
+    st = Stats()
+    _bind_stream()
+    check(_L().aar_hip_helm_lgmres(ctypes.byref(h), _cplx(u, "u"), _cplx(b, "b"), int(restart), int(aug),
+                                   int(max_iter), float(tol), _pcz(precond), ctypes.byref(st)))
+    return st.as_dict()
+
+
 def helm_bicg(u, b, shape, a, c=None, c0=0.0, eta=0.0, max_iter=2000, tol=1e-6, precond=True):
     """BiCG of the complex PETSc variant (PETSC_COMPLEX/main.c:150-175)."""
     h = _helm(shape, a, c, c0, eta)

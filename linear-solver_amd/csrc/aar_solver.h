@@ -151,6 +151,8 @@ int helm_matvec(const HOp &op, const double *u, double *y);
 int helm_matvec_async(const HOp &op, const double *u, double *y, Ctrl *gate);
 int helm_gmres_solve(const HOp &op, double *u, const double *b, int restart, int max_it, double tol, int precond,
                      aar_hip_stats *st);
+int helm_lgmres_solve(const HOp &op, double *u, const double *b, int restart, int aug, int max_it, double tol,
+                      int precond, aar_hip_stats *st);
 int helm_bicg_solve(const HOp &op, double *u, const double *b, int max_it, double tol, int precond,
                     aar_hip_stats *st);
 int helm_ilu0_apply(const HOp &op, const double *r, double *z, int trans);

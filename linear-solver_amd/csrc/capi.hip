@@ -968,6 +968,14 @@ int aar_hip_helm_gmres(const aar_hip_helm *h, double *u, const double *b, int re
     return helm_gmres_solve(op, u, b, restart, max_iter, tol, precond, st);
 }
 
+int aar_hip_helm_lgmres(const aar_hip_helm *h, double *u, const double *b, int restart, int aug, int max_iter,
+                        double tol, int precond, aar_hip_stats *st)
+{
+    HOp op;
+    if (int rc = make_hop(h, op)) return rc;
+    return helm_lgmres_solve(op, u, b, restart, aug, max_iter, tol, precond, st);
+}
+
 int aar_hip_helm_bicg(const aar_hip_helm *h, double *u, const double *b, int max_iter, double tol, int precond,
                       aar_hip_stats *st)
 {

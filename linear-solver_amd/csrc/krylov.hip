@@ -1943,9 +1943,17 @@ struct KCtrlZ {
     double2 cc[KMAX + 1], ss[KMAX + 1], grs[KMAX + 2], nrs[KMAX + 1];
     double2 beta, betaold, bcoef, alpha;   // BiCG
     int first, pad2;
+    // LGMRES: the unrotated Hessenberg, H y and 1/||z|| of the new error
+    // approximation
+    double2 HES[(KMAX + 2) * (KMAX + 1)];
+    double2 hw[KMAX + 2];
+    double ascale;
+    int lg;          // LGMRES: the last vector of a cycle is normalised too (V H y uses it)
+    int scale_nv;    // the basis slot the pending step's scale applies to
 };
 
 __device__ __forceinline__ double2 &ZHe(KCtrlZ *k, int r, int c) { return k->HH[c * (KMAX + 2) + r]; }
+__device__ __forceinline__ double2 &ZSe(KCtrlZ *k, int r, int c) { return k->HES[c * (KMAX + 2) + r]; }
 
 __device__ int kz_converged(KCtrlZ *k, int n, double rnorm)
 {
@@ -2090,9 +2098,11 @@ __global__ __launch_bounds__(KB) void kz_orth_kernel(long n, const double2 *t, d
     kz_block_sums<1>(acc, 1, part);
 }
 
-__global__ __launch_bounds__(KB) void kz_scale_kernel(long n, double2 *v, const KCtrlZ *k, int gated)
+// v *= scale (VecNormalize) unless the control says otherwise; a step's
+// vector (gated, basis slot nv) only when that step ran and produced it
+__global__ __launch_bounds__(KB) void kz_scale_kernel(long n, double2 *v, const KCtrlZ *k, int gated, int nv)
 {
-    if (k->noscale || (gated && k->gate.done)) return;
+    if (k->noscale || (gated && k->scale_nv != nv)) return;
     const double s = k->scale;
     KLOOP(i) v[i] = make_double2(s * v[i].x, s * v[i].y);
 }
@@ -2116,11 +2126,12 @@ __global__ __launch_bounds__(KB) void kz_soln_kernel(long n, double2 *x, const d
 }
 
 __global__ __launch_bounds__(KB) void kz_ctl_init_kernel(KCtrlZ *k, double rtol, int max_it, int restart, int hcap,
-                                                        const double *part, int nb)
+                                                        const double *part, int nb, int lg)
 {
     __shared__ double s[KPZ];
     kz_reduce(part, nb, 2, s);
     if (threadIdx.x != 0) return;
+    k->lg = lg;
     k->rtol = rtol;
     k->max_it = max_it;
     k->restart = restart;
@@ -2132,13 +2143,17 @@ __global__ __launch_bounds__(KB) void kz_ctl_init_kernel(KCtrlZ *k, double rtol,
     k->noscale = 1;
 }
 
-__global__ __launch_bounds__(KB) void kz_start_kernel(KCtrlZ *k, const double *part, int nb, double *hist)
+// itot: the steps of this cycle (GMRES: restart; LGMRES: the Arnoldi steps
+// plus one per stored error approximation)
+__global__ __launch_bounds__(KB) void kz_start_kernel(KCtrlZ *k, const double *part, int nb, double *hist, int itot)
 {
     if (k->c.done) return;
     __shared__ double s[KPZ];
     kz_reduce(part, nb, 1, s);
     if (threadIdx.x != 0) return;
     const double res = sqrt(s[0]);
+    k->restart = itot;
+    k->scale_nv = 0;
     k->open = 1;
     k->it = 0;
     k->grs[0] = make_double2(res, 0.0);
@@ -2166,8 +2181,8 @@ __global__ __launch_bounds__(KB) void kz_arnoldi_kernel(KCtrlZ *k, const double 
     if (threadIdx.x != 0) return;
     const int it = k->it;
     const double tt = sqrt(s[0]);
-    for (int j = 0; j <= it; j++) ZHe(k, j, it) = k->h[j];
-    ZHe(k, it + 1, it) = make_double2(tt, 0.0);
+    for (int j = 0; j <= it; j++) ZHe(k, j, it) = ZSe(k, j, it) = k->h[j];
+    ZHe(k, it + 1, it) = ZSe(k, it + 1, it) = make_double2(tt, 0.0);
     k->scale = 1.0 / tt;
     double hapbnd = tt / zabs(k->grs[it]);
     if (hapbnd > K_HAPTOL) hapbnd = K_HAPTOL;
@@ -2205,7 +2220,9 @@ __global__ __launch_bounds__(KB) void kz_arnoldi_kernel(KCtrlZ *k, const double 
     k->reason = reason;
     const bool active = !reason && !hapend && k->it < k->restart && k->its < k->max_it;
     k->gate.done = active ? 0 : 1;
-    k->noscale = (!active || tt == 0.0 || tt == 1.0);
+    // GMRES never reads the last vector of a cycle; LGMRES's B A z does
+    k->scale_nv = it + 1;
+    k->noscale = ((!active && !k->lg) || tt == 0.0 || tt == 1.0);
 }
 
 __global__ void kz_soln_ctl_kernel(KCtrlZ *k)
@@ -2223,6 +2240,87 @@ __global__ void kz_soln_ctl_kernel(KCtrlZ *k)
         k->nrs[r] = zdiv(tt, ZHe(k, r, r));
     }
     k->nosol = 0;
+}
+
+// --- LGMRES (PETSC_COMPLEX/main.c:120-147): the cycle's correction and its
+// error approximation.  za.z[q] is the stored approximation of order q + 1
+// (newest first), the basis of step za.ita + q.
+constexpr int ZAUG_MAX = AAR_HIP_LGMRES_MAXAUG;
+struct ZAug {
+    const double2 *z[ZAUG_MAX];
+    int ita, nz;
+};
+
+// c = sum_j y_j W_j over the cycle's steps (W_j = v_j for the Arnoldi
+// steps, z of the augmentation steps); x += c; znew = c; ||c||^2
+template <int NV>
+__global__ __launch_bounds__(KB) void kz_lsoln_kernel(long n, double2 *x, const double2 *V, long ld, ZAug za,
+                                                     double2 *znew, const KCtrlZ *k, double *part)
+{
+    if (k->nosol) return;
+    const int nv = k->it;
+    const int na = nv < za.ita ? nv : za.ita;
+    double2 c[NV], ca[ZAUG_MAX];
+#pragma unroll
+    for (int j = 0; j < NV; j++) c[j] = j < na ? k->nrs[j] : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int q = 0; q < ZAUG_MAX; q++)
+        ca[q] = (q < za.nz && za.ita + q < nv) ? k->nrs[za.ita + q] : make_double2(0.0, 0.0);
+    double acc[1] = {0.0};
+    KLOOP(i)
+    {
+        double2 s = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int j = 0; j < NV; j++)
+            if (j < na) s = zadd(s, zmul(c[j], V[(size_t)j * ld + i]));
+#pragma unroll
+        for (int q = 0; q < ZAUG_MAX; q++)
+            if (q < za.nz && za.ita + q < nv) s = zadd(s, zmul(ca[q], za.z[q][i]));
+        znew[i] = s;
+        x[i] = zadd(x[i], s);
+        acc[0] += s.x * s.x + s.y * s.y;
+    }
+    kz_block_sums<1>(acc, 1, part);
+}
+
+// 1/||c|| and H y (H the unrotated Hessenberg): B A c = V_{it+1} H y
+__global__ __launch_bounds__(KB) void kz_aug_ctl_kernel(KCtrlZ *k, const double *part, int nb)
+{
+    if (k->nosol) return;
+    __shared__ double s[KPZ];
+    kz_reduce(part, nb, 1, s);
+    if (threadIdx.x != 0) return;
+    k->ascale = 1.0 / sqrt(s[0]);
+    const int it = k->it;
+    for (int r = 0; r <= it; r++) {
+        double2 acc = make_double2(0.0, 0.0);
+        for (int j = 0; j < it; j++)
+            if (r <= j + 1) acc = zadd(acc, zmul(ZSe(k, r, j), k->nrs[j]));
+        k->hw[r] = acc;
+    }
+}
+
+// the new approximation, normalised: az = (V H y)/||c||, z = c/||c||
+template <int NV>
+__global__ __launch_bounds__(KB) void kz_aug_kernel(long n, const double2 *V, long ld, double2 *z, double2 *az,
+                                                   const KCtrlZ *k)
+{
+    if (k->nosol) return;
+    const int nv = k->it;
+    const double sc = k->ascale;
+    double2 hw[NV];
+#pragma unroll
+    for (int j = 0; j < NV; j++) hw[j] = j <= nv ? k->hw[j] : make_double2(0.0, 0.0);
+    KLOOP(i)
+    {
+        double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int j = 0; j < NV; j++)
+            if (j <= nv) acc = zadd(acc, zmul(hw[j], V[(size_t)j * ld + i]));
+        az[i] = make_double2(sc * acc.x, sc * acc.y);
+        const double2 c = z[i];
+        z[i] = make_double2(sc * c.x, sc * c.y);
+    }
 }
 
 __global__ void kz_end_kernel(KCtrlZ *k)
@@ -2451,6 +2549,20 @@ hipError_t launch_gz_soln(double2 *x, const double2 *V, long ld, long n, int nb,
     hipLaunchKernelGGL((kz_soln_kernel<NV>), dim3(nb), dim3(KB), 0, ctx().stream, n, x, V, ld, (const KCtrlZ *)k);
     return hipGetLastError();
 }
+template <int NV>
+hipError_t launch_gz_lsoln(double2 *x, const double2 *V, long ld, const ZAug &za, double2 *znew, long n, int nb,
+                           KCtrlZ *k, double *part)
+{
+    hipLaunchKernelGGL((kz_lsoln_kernel<NV>), dim3(nb), dim3(KB), 0, ctx().stream, n, x, V, ld, za, znew,
+                       (const KCtrlZ *)k, part);
+    return hipGetLastError();
+}
+template <int NV>
+hipError_t launch_gz_aug(const double2 *V, long ld, double2 *z, double2 *az, long n, int nb, KCtrlZ *k)
+{
+    hipLaunchKernelGGL((kz_aug_kernel<NV>), dim3(nb), dim3(KB), 0, ctx().stream, n, V, ld, z, az, (const KCtrlZ *)k);
+    return hipGetLastError();
+}
 }  // namespace
 
 static IluArgsT<double2> helm_ilu_args(const HOp &op, double2 *F)
@@ -2497,14 +2609,28 @@ int helm_ilu0_apply(const HOp &op, const double *r, double *z, int trans)
     return 0;
 }
 
-int helm_gmres_solve(const HOp &op, double *u, const double *b, int restart, int max_it, double tol, int precond,
-                     aar_hip_stats *st)
+// GMRES(restart) (aug = 0) and LGMRES(restart - aug, aug) of the complex
+// variant.  LGMRES (PETSC_COMPLEX/main.c:120-147, KSPLGMRES; the algorithm
+// of Baker, Jessup and Manteuffel with PETSc's defaults restart = 30,
+// aug = 2, the space not held constant): each cycle takes restart - aug
+// Arnoldi steps, then one step per stored error approximation z (newest
+// first) whose vector is the stored B A z; every step is an iteration.
+// The cycle's correction becomes the newest z, and B A z = V H y from the
+// unrotated Hessenberg (no operator application), both scaled by 1/||z||.
+// The approximations take aug + 1 slots: the new one is written to the free
+// slot, and the oldest slot is freed once aug are stored.  The schedule
+// (steps per cycle, slot of each order) depends only on the cycle count,
+// so the host enqueues it ahead of the convergence flag.
+static int helm_gmres_impl(const HOp &op, double *u, const double *b, int restart, int aug, int max_it, double tol,
+                           int precond, aar_hip_stats *st)
 {
     Context &C = ctx();
     if (int rc = context_init()) return rc;
     if (!u || !b) return fail(AAR_HIP_ERR_ARG, "null vector");
     if (max_it < 0) return fail(AAR_HIP_ERR_ARG, "max_iter must be >= 0");
     if (restart < 1 || restart > KMAX) return fail(AAR_HIP_ERR_ARG, "restart must be in 1..32");
+    if (aug < 0 || aug > ZAUG_MAX || aug >= restart)
+        return fail(AAR_HIP_ERR_ARG, "aug must be in 0..min(8, restart - 1)");
     if (precond < 0 || precond > 2) return fail(AAR_HIP_ERR_ARG, "precond must be 0 (none), 1 (Jacobi) or 2 (ILU(0))");
     const bool ilu = precond == 2;
     if (ilu && op.fdn != R)
@@ -2513,10 +2639,15 @@ int helm_gmres_solve(const HOp &op, double *u, const double *b, int restart, int
         return fail(AAR_HIP_ERR_ARG, "the ILU(0) preconditioner needs every grid side >= 13");
     const long n = op.n;
     const long ld = ring_ld(n, sizeof(double2));
-    HC(ensure(C.kvec, sizeof(double2) * (size_t)ld * (restart + 2)));
+    // basis v_0 .. v_restart, t, then aug + 1 slots of z and of B A z
+    const int nslot = aug ? aug + 1 : 0;
+    HC(ensure(C.kvec, sizeof(double2) * (size_t)ld * (restart + 2 + 2 * nslot)));
     double2 *V = (double2 *)C.kvec.p, *t = V + (size_t)ld * (restart + 1);
+    double2 *Zs = t + ld, *AZs = Zs + (size_t)ld * nslot;
+    int ord[ZAUG_MAX + 1] = {0}, aug_ct = 0;   // ord[slot]: 1 = newest, 0 = free
+    const int ita = restart - aug;             // Arnoldi steps per cycle
     const int nb = std::min(nblocks_for(n), 1024);
-    const int hcap = max_it + max_it / restart + 8;
+    const int hcap = max_it + max_it / ita + 8;
     HC(ensure(C.kctl, sizeof(KCtrlZ)));
     HC(ensure(C.hist0, sizeof(double) * (size_t)hcap));
     HC(ensure(C.part0, sizeof(double) * (size_t)nb * KPZ));
@@ -2528,6 +2659,8 @@ int helm_gmres_solve(const HOp &op, double *u, const double *b, int restart, int
     dg.c0 = op.c0;
     dg.eta = op.eta;
     dg.precond = precond == 1;
+    ZDiag dg0 = dg;            // the augmentation steps' vectors are already B A z
+    dg0.precond = 0;
     const double2 *bz = (const double2 *)b;
     double2 *x = (double2 *)u;
     C.bytes_acc = 0.0;
@@ -2547,7 +2680,7 @@ int helm_gmres_solve(const HOp &op, double *u, const double *b, int restart, int
         hipLaunchKernelGGL(kz_bnorm_kernel, dim3(nb), dim3(KB), 0, C.stream, n, bz, dg, part);
     }
     hipLaunchKernelGGL(kz_ctl_init_kernel, dim3(1), dim3(KB), 0, C.stream, k, tol, max_it, restart, hcap,
-                       (const double *)part, nb);
+                       (const double *)part, nb, aug > 0 ? 1 : 0);
     HC(hipGetLastError());
     C.bytes_acc += s + cb;
     bool stopped = false;
@@ -2561,40 +2694,77 @@ int helm_gmres_solve(const HOp &op, double *u, const double *b, int restart, int
                                (const int *)&k->c.done);
             C.bytes_acc += 2 * ILU_APPLY_BYTES * n;
         }
-        hipLaunchKernelGGL(kz_start_kernel, dim3(1), dim3(KB), 0, C.stream, k, (const double *)part, nb, hist);
-        hipLaunchKernelGGL(kz_scale_kernel, dim3(nb), dim3(KB), 0, C.stream, n, V, (const KCtrlZ *)k, 0);
+        // slot of the approximation of each order (1 = newest)
+        int slot_of[ZAUG_MAX + 1] = {0};
+        for (int q = 0; q < nslot; q++)
+            if (ord[q]) slot_of[ord[q]] = q;
+        const int itot = ita + aug_ct;
+        hipLaunchKernelGGL(kz_start_kernel, dim3(1), dim3(KB), 0, C.stream, k, (const double *)part, nb, hist, itot);
+        hipLaunchKernelGGL(kz_scale_kernel, dim3(nb), dim3(KB), 0, C.stream, n, V, (const KCtrlZ *)k, 0, 0);
         HC(hipGetLastError());
         C.bytes_acc += 5 * s + 2 * cb;
-        for (int it = 0; it < restart; it++) {
+        for (int it = 0; it < itot; it++) {
             const int nv = it + 1;
-            if (int rc = helm_matvec_async(op, (const double *)(V + (size_t)it * ld), (double *)t, &k->gate))
-                return rc;
             const double2 *w = t;
-            if (ilu) {
-                HC(ilu_apply(za, (const double2 *)t, V + (size_t)nv * ld, false, &k->gate.done, C.stream));
-                w = V + (size_t)nv * ld;
-                C.bytes_acc += 2 * ILU_APPLY_BYTES * n;
+            const bool arnoldi = it < ita;
+            if (arnoldi) {
+                if (int rc = helm_matvec_async(op, (const double *)(V + (size_t)it * ld), (double *)t, &k->gate))
+                    return rc;
+                if (ilu) {
+                    HC(ilu_apply(za, (const double2 *)t, V + (size_t)nv * ld, false, &k->gate.done, C.stream));
+                    w = V + (size_t)nv * ld;
+                    C.bytes_acc += 2 * ILU_APPLY_BYTES * n;
+                }
+            } else {
+                w = AZs + (size_t)ld * slot_of[it - ita + 1];
             }
+            const ZDiag &dw = (arnoldi && !ilu) ? dg : dg0;
             prof_begin("krylov_gmres_orth");
-            hipError_t e = nv <= 8 ? launch_gz_step<8>(w, V, ld, nv, n, nb, dg, k, part)
-                         : nv <= 16 ? launch_gz_step<16>(w, V, ld, nv, n, nb, dg, k, part)
-                                    : launch_gz_step<32>(w, V, ld, nv, n, nb, dg, k, part);
+            hipError_t e = nv <= 8 ? launch_gz_step<8>(w, V, ld, nv, n, nb, dw, k, part)
+                         : nv <= 16 ? launch_gz_step<16>(w, V, ld, nv, n, nb, dw, k, part)
+                                    : launch_gz_step<32>(w, V, ld, nv, n, nb, dw, k, part);
             prof_end((2.0 * nv + 3) * s + 2 * cb);
             if (e != hipSuccess) return fail(AAR_HIP_ERR_HIP, std::string("gmres step: ") + hipGetErrorString(e));
             hipLaunchKernelGGL(kz_arnoldi_kernel, dim3(1), dim3(KB), 0, C.stream, k, (const double *)part, nb, hist);
             hipLaunchKernelGGL(kz_scale_kernel, dim3(nb), dim3(KB), 0, C.stream, n, V + (size_t)nv * ld,
-                               (const KCtrlZ *)k, 1);
+                               (const KCtrlZ *)k, 1, nv);
             HC(hipGetLastError());
             C.bytes_acc += (2.0 * nv + 5) * s + 2 * cb;
         }
         hipLaunchKernelGGL(kz_soln_ctl_kernel, dim3(1), dim3(1), 0, C.stream, k);
-        hipError_t e = restart <= 8 ? launch_gz_soln<8>(x, V, ld, n, nb, k)
-                     : restart <= 16 ? launch_gz_soln<16>(x, V, ld, n, nb, k)
-                                     : launch_gz_soln<32>(x, V, ld, n, nb, k);
-        if (e != hipSuccess) return fail(AAR_HIP_ERR_HIP, std::string("gmres solution: ") + hipGetErrorString(e));
+        if (!aug) {
+            hipError_t e = restart <= 8 ? launch_gz_soln<8>(x, V, ld, n, nb, k)
+                         : restart <= 16 ? launch_gz_soln<16>(x, V, ld, n, nb, k)
+                                         : launch_gz_soln<32>(x, V, ld, n, nb, k);
+            if (e != hipSuccess) return fail(AAR_HIP_ERR_HIP, std::string("gmres solution: ") + hipGetErrorString(e));
+            C.bytes_acc += (restart + 2.0) * s;
+        } else {
+            // the new approximation goes to the free slot
+            int fs = 0;
+            while (ord[fs]) fs++;
+            double2 *zn = Zs + (size_t)ld * fs, *azn = AZs + (size_t)ld * fs;
+            ZAug zq{};
+            zq.ita = ita;
+            zq.nz = aug_ct;
+            for (int q = 0; q < aug_ct; q++) zq.z[q] = Zs + (size_t)ld * slot_of[q + 1];
+            hipError_t e = ita <= 8 ? launch_gz_lsoln<8>(x, V, ld, zq, zn, n, nb, k, part)
+                         : ita <= 16 ? launch_gz_lsoln<16>(x, V, ld, zq, zn, n, nb, k, part)
+                                     : launch_gz_lsoln<32>(x, V, ld, zq, zn, n, nb, k, part);
+            if (e != hipSuccess) return fail(AAR_HIP_ERR_HIP, std::string("lgmres solution: ") + hipGetErrorString(e));
+            hipLaunchKernelGGL(kz_aug_ctl_kernel, dim3(1), dim3(KB), 0, C.stream, k, (const double *)part, nb);
+            e = itot < 16 ? launch_gz_aug<16>((const double2 *)V, ld, zn, azn, n, nb, k)
+                          : launch_gz_aug<KMAX + 1>((const double2 *)V, ld, zn, azn, n, nb, k);
+            if (e != hipSuccess) return fail(AAR_HIP_ERR_HIP, std::string("lgmres approximation: ") + hipGetErrorString(e));
+            C.bytes_acc += (itot + 3.0) * s + (itot + 4.0) * s;
+            // the order of the stored approximations after this cycle
+            if (aug_ct == aug) ord[slot_of[aug]] = 0;
+            else aug_ct++;
+            for (int q = 0; q < nslot; q++)
+                if (ord[q]) ord[q]++;
+            ord[fs] = 1;
+        }
         hipLaunchKernelGGL(kz_end_kernel, dim3(1), dim3(1), 0, C.stream, k);
         HC(hipGetLastError());
-        C.bytes_acc += (restart + 2.0) * s;
         if (int rc = poll_record(&k->c, cyc)) return rc;
         if (poll_check(cyc - 2)) stopped = true;
     }
@@ -2630,6 +2800,18 @@ int helm_gmres_solve(const HOp &op, double *u, const double *b, int restart, int
     if (st) *st = sres;
     if (hk.reason == K_DIVERGED_NANORINF) return fail(AAR_HIP_ERR_NONFINITE, "residual is not finite");
     return 0;
+}
+
+int helm_gmres_solve(const HOp &op, double *u, const double *b, int restart, int max_it, double tol, int precond,
+                     aar_hip_stats *st)
+{
+    return helm_gmres_impl(op, u, b, restart, 0, max_it, tol, precond, st);
+}
+
+int helm_lgmres_solve(const HOp &op, double *u, const double *b, int restart, int aug, int max_it, double tol,
+                      int precond, aar_hip_stats *st)
+{
+    return helm_gmres_impl(op, u, b, restart, aug, max_it, tol, precond, st);
 }
 
 // BiCG of the complex PETSc variant (PETSC_COMPLEX/main.c:150-175)
@@ -2696,7 +2878,7 @@ int helm_bicg_solve(const HOp &op, double *u, const double *b, int max_it, doubl
         hipLaunchKernelGGL(kz_bnorm_kernel, dim3(nb), dim3(KB), 0, C.stream, n, bz, dg, part);
     }
     hipLaunchKernelGGL(kz_ctl_init_kernel, dim3(1), dim3(KB), 0, C.stream, k, tol, max_it, 0, hcap,
-                       (const double *)part, nb);
+                       (const double *)part, nb, 0);
     HC(hipGetLastError());
     const double s = 16.0 * n, cb = op.c ? 8.0 * n : 0.0;
     if (int rc = helm_matvec_async(op, u, (double *)q, nullptr)) return rc;

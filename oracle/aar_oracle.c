@@ -1649,11 +1649,26 @@ static inline zc zdiv(zc a, zc b)
 }
 static inline double zabs(zc a) { return sqrt(a.re * a.re + a.im * a.im); }
 
-int orc_gmres_complex(int nx, int ny, int nz, int fdn, double a, const double *c, double c0, double eta,
-                      double *u, const double *rhs, int restart, int max_it, double rtol, int precond,
-                      double *trace, int trace_cap, orc_stats *st)
+/* LGMRES(m, k) (PETSC_COMPLEX/main.c:120-147: KSPLGMRES, the same       */
+/* tolerances and preconditioners) in the same loop, aug = k > 0: the     */
+/* published algorithm of Baker, Jessup and Manteuffel (SIAM J. Matrix    */
+/* Anal. Appl. 26, 2005) with PETSc's defaults, restart = m + k = 30 and  */
+/* k = 2 error approximations, the space not held constant:               */
+/*  * every cycle takes restart - aug Arnoldi steps, then one step per    */
+/*    stored approximation z, newest first, with w = B A z (kept from    */
+/*    when z was made) orthogonalised like an Arnoldi vector; every step  */
+/*    counts as an iteration and is tested;                               */
+/*  * the cycle's correction is sum_j y_j W_j (W_j the Arnoldi vector or  */
+/*    the z of step j); it becomes the new z, and B A z = V H y by the     */
+/*    unrotated Hessenberg H, with no operator application; both scaled  */
+/*    by 1/||z||; the oldest z is replaced once k are stored.             */
+/* aug = 0 is GMRES(restart), operation for operation.                   */
+static int gmres_complex_impl(int nx, int ny, int nz, int fdn, double a, const double *c, double c0, double eta,
+                              double *u, const double *rhs, int restart, int aug, int max_it, double rtol,
+                              int precond, double *trace, int trace_cap, orc_stats *st)
 {
     if (restart < 1 || restart > ORC_GMRES_MAXK) return -1;
+    if (aug < 0 || aug > ORC_LGMRES_MAXAUG || aug >= restart) return -1;
     const long n = (long)nx * ny * nz;
     const int K = restart;
     double coeff[32] = {0};
@@ -1673,10 +1688,16 @@ int orc_gmres_complex(int nx, int ny, int nz, int fdn, double a, const double *c
             return -2;
         }
     }
-    zc HH[(ORC_GMRES_MAXK + 2) * (ORC_GMRES_MAXK + 1)];
+    zc HH[(ORC_GMRES_MAXK + 2) * (ORC_GMRES_MAXK + 1)], HES[(ORC_GMRES_MAXK + 2) * (ORC_GMRES_MAXK + 1)];
     zc cc[ORC_GMRES_MAXK + 1], ss[ORC_GMRES_MAXK + 1], grs[ORC_GMRES_MAXK + 2], nrs[ORC_GMRES_MAXK + 1];
-    zc h[ORC_GMRES_MAXK + 1];
+    zc h[ORC_GMRES_MAXK + 1], hw[ORC_GMRES_MAXK + 2];
+    /* the error approximations z and B A z, slot order[s] (1 = newest) */
+    zc *Z = aug ? (zc *)malloc(sizeof(zc) * n * aug) : NULL, *AZ = aug ? (zc *)malloc(sizeof(zc) * n * aug) : NULL;
+    zc *corr = aug ? (zc *)malloc(sizeof(zc) * n) : NULL;
+    int order[ORC_LGMRES_MAXAUG] = {0}, aug_ct = 0;
+    const int ita = K - aug;       /* Arnoldi steps per cycle */
 #define ZH(r, col) HH[(col) * (K + 2) + (r)]
+#define ZS(r, col) HES[(col) * (K + 2) + (r)]
     for (long i = 0; i < n; i++) {
         if (precond) {
             const double dr = dlap + (c ? c[i] : c0);
@@ -1725,13 +1746,22 @@ int orc_gmres_complex(int nx, int ny, int nz, int fdn, double a, const double *c
         if (trace && tlen < trace_cap) trace[tlen++] = res;
         if (res == 0.0) { reason = ORC_KSP_CONVERGED_ATOL; break; }
         reason = ksp_converged(&cv_, its, res);
-        while (!reason && it < K && its < max_it) {
+        const int itot = ita + aug_ct;
+        while (!reason && it < itot && its < max_it) {
             zc *vn = V + (long)(it + 1) * n;
-            orc_matvec_complex(nx, ny, nz, fdn, coeff, a, c, c0, eta, (const double *)(V + (long)it * n),
-                               (double *)t);
-            if (ilu) orc_ilu0_apply_complex(nx, ny, nz, (const double *)Fz, (const double *)t, (double *)vn, 0);
-            else
-                for (long i = 0; i < n; i++) vn[i] = precond ? zmul(dinv[i], t[i]) : t[i];
+            if (it < ita) {
+                orc_matvec_complex(nx, ny, nz, fdn, coeff, a, c, c0, eta, (const double *)(V + (long)it * n),
+                                   (double *)t);
+                if (ilu) orc_ilu0_apply_complex(nx, ny, nz, (const double *)Fz, (const double *)t, (double *)vn, 0);
+                else
+                    for (long i = 0; i < n; i++) vn[i] = precond ? zmul(dinv[i], t[i]) : t[i];
+            } else {
+                /* augmentation step: the approximation of order it - ita + 1 */
+                int spot = 0;
+                for (int q = 0; q < aug_ct; q++)
+                    if (order[q] == it - ita + 1) spot = q;
+                memcpy(vn, AZ + (long)spot * n, sizeof(zc) * n);
+            }
             for (int j = 0; j <= it; j++) {
                 zc sj = {0.0, 0.0};
                 const zc *vj = V + (long)j * n;
@@ -1748,7 +1778,7 @@ int orc_gmres_complex(int nx, int ny, int nz, int fdn, double a, const double *c
                 vn[i] = w;
                 s2 = s2 + (w.re * w.re + w.im * w.im);
             }
-            for (int j = 0; j <= it; j++) ZH(j, it) = h[j];
+            for (int j = 0; j <= it; j++) ZH(j, it) = ZS(j, it) = h[j];
             const double tt = sqrt(s2);
             if (tt != 0.0 && tt != 1.0) {
                 const double sc = 1.0 / tt;
@@ -1756,6 +1786,7 @@ int orc_gmres_complex(int nx, int ny, int nz, int fdn, double a, const double *c
             }
             ZH(it + 1, it).re = tt;
             ZH(it + 1, it).im = 0.0;
+            ZS(it + 1, it) = ZH(it + 1, it);
             double hapbnd = tt / zabs(grs[it]);
             if (hapbnd > ORC_GMRES_HAPTOL) hapbnd = ORC_GMRES_HAPTOL;
             const int hapend = tt < hapbnd;
@@ -1803,10 +1834,56 @@ int orc_gmres_complex(int nx, int ny, int nz, int fdn, double a, const double *c
                 for (int j = kk + 1; j <= k; j++) tt = zsub(tt, zmul(ZH(kk, j), nrs[j]));
                 nrs[kk] = zdiv(tt, ZH(kk, kk));
             }
-            for (long i = 0; i < n; i++) {
-                zc sacc = {0.0, 0.0};
-                for (int j = 0; j <= k; j++) sacc = zadd(sacc, zmul(nrs[j], V[(long)j * n + i]));
-                x[i] = zadd(x[i], sacc);
+            if (!aug) {
+                for (long i = 0; i < n; i++) {
+                    zc sacc = {0.0, 0.0};
+                    for (int j = 0; j <= k; j++) sacc = zadd(sacc, zmul(nrs[j], V[(long)j * n + i]));
+                    x[i] = zadd(x[i], sacc);
+                }
+            } else {
+                const zc *W[ORC_GMRES_MAXK];
+                for (int j = 0; j <= k; j++) {
+                    W[j] = V + (long)j * n;
+                    if (j >= ita)
+                        for (int q = 0; q < aug_ct; q++)
+                            if (order[q] == j - ita + 1) W[j] = Z + (long)q * n;
+                }
+                double zn2 = 0.0;
+                for (long i = 0; i < n; i++) {
+                    zc sacc = {0.0, 0.0};
+                    for (int j = 0; j <= k; j++) sacc = zadd(sacc, zmul(nrs[j], W[j][i]));
+                    corr[i] = sacc;
+                    x[i] = zadd(x[i], sacc);
+                    zn2 = zn2 + (sacc.re * sacc.re + sacc.im * sacc.im);
+                }
+                /* B A z = V_{it+1} (H y), H unrotated */
+                for (int r = 0; r <= it; r++) {
+                    zc acc = {0.0, 0.0};
+                    for (int j = 0; j <= k; j++)
+                        if (r <= j + 1) acc = zadd(acc, zmul(ZS(r, j), nrs[j]));
+                    hw[r] = acc;
+                }
+                int spot;
+                if (aug_ct < aug) {
+                    spot = aug_ct++;
+                } else {
+                    spot = 0;
+                    for (int q = 0; q < aug; q++)
+                        if (order[q] == aug) spot = q;
+                }
+                for (int q = 0; q < aug_ct; q++)
+                    if (q != spot) order[q]++;
+                order[spot] = 1;
+                const double sc = 1.0 / sqrt(zn2);
+                zc *zs = Z + (long)spot * n, *azs = AZ + (long)spot * n;
+                for (long i = 0; i < n; i++) {
+                    zc acc = {0.0, 0.0};
+                    for (int r = 0; r <= it; r++) acc = zadd(acc, zmul(hw[r], V[(long)r * n + i]));
+                    azs[i].re = sc * acc.re;
+                    azs[i].im = sc * acc.im;
+                    zs[i].re = sc * corr[i].re;
+                    zs[i].im = sc * corr[i].im;
+                }
             }
         }
         if (its >= max_it) {
@@ -1815,6 +1892,7 @@ int orc_gmres_complex(int nx, int ny, int nz, int fdn, double a, const double *c
         }
     }
 #undef ZH
+#undef ZS
     double t1 = now_seconds();
     if (st) {
         orc_matvec_complex(nx, ny, nz, fdn, coeff, a, c, c0, eta, (const double *)x, (double *)t);
@@ -1834,8 +1912,24 @@ int orc_gmres_complex(int nx, int ny, int nz, int fdn, double a, const double *c
         st->seconds = t1 - t0;
         st->status = reason;
     }
-    free(dinv); free(V); free(t); free(Fz);
+    free(dinv); free(V); free(t); free(Fz); free(Z); free(AZ); free(corr);
     return 0;
+}
+
+int orc_gmres_complex(int nx, int ny, int nz, int fdn, double a, const double *c, double c0, double eta,
+                      double *u, const double *rhs, int restart, int max_it, double rtol, int precond,
+                      double *trace, int trace_cap, orc_stats *st)
+{
+    return gmres_complex_impl(nx, ny, nz, fdn, a, c, c0, eta, u, rhs, restart, 0, max_it, rtol, precond, trace,
+                              trace_cap, st);
+}
+
+int orc_lgmres_complex(int nx, int ny, int nz, int fdn, double a, const double *c, double c0, double eta,
+                       double *u, const double *rhs, int restart, int aug, int max_it, double rtol, int precond,
+                       double *trace, int trace_cap, orc_stats *st)
+{
+    return gmres_complex_impl(nx, ny, nz, fdn, a, c, c0, eta, u, rhs, restart, aug, max_it, rtol, precond, trace,
+                              trace_cap, st);
 }
 
 /* BiCG of the complex PETSc variant (PETSC_COMPLEX/main.c:150-175,

@@ -146,6 +146,7 @@ void orc_matvec_complex(int nx, int ny, int nz, int fdn, const double *coeff,
  * operator. */
 #define ORC_GMRES_MAXK 32
 #define ORC_GMRES_HAPTOL 1e-30
+#define ORC_LGMRES_MAXAUG 8
 #define ORC_KSP_ATOL 1e-50
 #define ORC_KSP_DTOL 1e5
 #define ORC_KSP_CONVERGED_RTOL 2
@@ -193,6 +194,14 @@ int orc_pgr_petsc(int nx, int ny, int nz, int fdn, double a, double sigma, const
 int orc_gmres_complex(int nx, int ny, int nz, int fdn, double a, const double *c, double c0, double eta,
                       double *u, const double *rhs, int restart, int max_it, double rtol, int precond,
                       double *trace, int trace_cap, orc_stats *st);
+/* LGMRES of the complex PETSc variant (PETSC_COMPLEX/main.c:120-147,
+ * KSPLGMRES; PETSc's defaults restart = 30, aug = 2): restart - aug
+ * Arnoldi steps and one step per stored error approximation each cycle
+ * (aar_oracle.c); 1 <= restart <= 32, 0 <= aug < restart, aug <= 8; aug
+ * = 0 is orc_gmres_complex. */
+int orc_lgmres_complex(int nx, int ny, int nz, int fdn, double a, const double *c, double c0, double eta,
+                       double *u, const double *rhs, int restart, int aug, int max_it, double rtol, int precond,
+                       double *trace, int trace_cap, orc_stats *st);
 /* BiCG of the complex PETSc variant (PETSC_COMPLEX/main.c:150-175); the
  * shadow sequence with A^H (eta -> -eta) and B^H = conj(1/D). */
 int orc_bicg_complex(int nx, int ny, int nz, int fdn, double a, const double *c, double c0, double eta,

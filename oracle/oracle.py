@@ -103,6 +103,8 @@ def lib():
                                         ctypes.POINTER(OrcStats)]
         L.orc_bicg_complex.argtypes = [I, I, I, I, Dd, _D, Dd, Dd, _D, _D, I, Dd, I, _D, I,
                                        ctypes.POINTER(OrcStats)]
+        L.orc_lgmres_complex.argtypes = [I, I, I, I, Dd, _D, Dd, Dd, _D, _D, I, I, I, Dd, I, _D, I,
+                                         ctypes.POINTER(OrcStats)]
         L.orc_ilu0_factor_complex.argtypes = [I, I, I, I, Dd, _D, Dd, Dd, _D]
         L.orc_ilu0_factor_complex.restype = I
         L.orc_ilu0_apply_complex.argtypes = [I, I, I, _D, _D, _D, I]
@@ -113,7 +115,7 @@ def lib():
         L.orc_ilu0_apply.restype = None
         for fn in ("orc_aar", "orc_pgr", "orc_cg", "orc_aar_complex", "orc_aar_vc", "orc_pgr_vc", "orc_cg_vc",
                    "orc_gmres", "orc_bicg", "orc_ilu0_factor", "orc_aar_petsc", "orc_pgr_petsc",
-                   "orc_gmres_complex", "orc_bicg_complex"):
+                   "orc_gmres_complex", "orc_bicg_complex", "orc_lgmres_complex"):
             getattr(L, fn).restype = I
         _lib = L
     return _lib
@@ -366,6 +368,21 @@ def gmres_complex(u0, rhs, shape, a, c, c0, eta, max_iter, tol, restart=30, prec
     cap = max_iter + max_iter // max(1, restart) + 8
     d = _run(lib().orc_gmres_complex, (nx, ny, nz, fdn, a, _p(cc), c0, eta, _p(u.view(np.float64)),
                                        _p(b.view(np.float64)), restart, max_iter, tol, int(precond)), trace, cap)
+    return u, d
+
+
+def lgmres_complex(u0, rhs, shape, a, c, c0, eta, max_iter, tol, restart=30, aug=2, precond=True, fdn=6,
+                   trace=True):
+    """LGMRES of the complex PETSc variant (PETSC_COMPLEX/main.c:120-147):
+    restart - aug Arnoldi steps and one step per stored error approximation
+    each cycle; aug = 0 is gmres_complex."""
+    nx, ny, nz = shape
+    u = np.ascontiguousarray(u0, dtype=np.complex128).reshape(-1).copy()
+    b = np.ascontiguousarray(rhs, dtype=np.complex128).reshape(-1)
+    cc = None if c is None else _f64(c).reshape(-1)
+    cap = max_iter + max_iter // max(1, restart - aug) + 8
+    d = _run(lib().orc_lgmres_complex, (nx, ny, nz, fdn, a, _p(cc), c0, eta, _p(u.view(np.float64)),
+                                        _p(b.view(np.float64)), restart, aug, max_iter, tol, int(precond)), trace, cap)
     return u, d
 
 

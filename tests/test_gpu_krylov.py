@@ -479,6 +479,72 @@ def test_helm_gmres_parity(gpu, oracle, case):
     assert np.abs(ug - uo).max() <= 1e-8 * np.abs(uo).max()
 
 
+LG_CASES = [
+    # shape, variable c, c0, eta, restart, aug, max_it, tol, precond (2 = ILU(0))
+    ((16, 16, 16), True, 0.0, 0.5, 30, 2, 2000, 1e-6, 1),
+    ((16, 16, 16), True, 0.0, 0.5, 30, 2, 2000, 1e-6, 0),
+    ((24, 20, 16), False, 0.3, 0.5, 7, 3, 2000, 1e-8, 1),
+    ((16, 16, 16), True, 0.0, 0.5, 10, 2, 26, 1e-12, 1),
+    ((16, 16, 16), True, 0.0, 0.5, 30, 2, 2000, 1e-8, 2),
+    ((36, 36, 36), False, -0.134992, 0.070225, 30, 2, 2000, 1e-6, 1),
+    ((36, 36, 36), False, -0.134992, 0.070225, 30, 2, 2000, 1e-6, 2),
+]
+
+
+def _lid(c):
+    return "x".join(map(str, c[0])) + ("_vc" if c[1] else f"_c{c[2]}") + f"_k{c[4]}a{c[5]}_i{c[6]}_pc{c[8]}"
+
+
+@pytest.mark.parametrize("case", LG_CASES, ids=_lid)
+def test_helm_lgmres_parity(gpu, oracle, case):
+    """LGMRES of the complex variant (PETSC_COMPLEX/main.c:120-147):
+    counts, reasons, every convergence-test norm (1e-6 relative or 1e-8 of
+    the first) and the solution (1e-8) equal to the oracle's.  Covers
+    PETSc's defaults (30, 2), a short restart with three approximations
+    (their order rotating over many cycles), a max_it stop at the first
+    augmentation step of a cycle, ILU(0), and PETSC_COMPLEX's own 36^3
+    problem."""
+    shape, vc, c0, eta, restart, aug, max_it, tol, pc = case
+    say("helm lgmres", _lid(case))
+    n = int(np.prod(shape))
+    a = P.poisson_a(0.5) if shape[0] != 36 else -1.0 / (4 * np.pi)
+    c = P.helmholtz_coefficient(shape, 7) if vc else None
+    b = P.rhs_complex_normal(shape, 1005)
+    u0 = np.ones(n, dtype=complex)
+    uo, d = oracle.lgmres_complex(u0, b, shape, a, c, c0, eta, max_it, tol, restart=restart, aug=aug, precond=pc)
+    ug = cdev(u0)
+    st = gpu.helm_lgmres(ug, cdev(b), shape, a, c=None if c is None else dev(c), c0=c0, eta=eta, max_iter=max_it,
+                         tol=tol, restart=restart, aug=aug, precond=pc)
+    say("gpu", st["iterations"], st["reason"], "oracle", d["iterations"], d["status"])
+    assert st["iterations"] == d["iterations"] and st["reason"] == d["status"]
+    h = gpu.history()
+    assert len(h) == len(d["trace"])
+    np.testing.assert_allclose(h, d["trace"], rtol=1e-6, atol=1e-8 * d["trace"][0])
+    ug = ug.cpu().numpy()
+    assert np.abs(ug - uo).max() <= 1e-8 * np.abs(uo).max()
+
+
+def test_helm_lgmres_aug0_is_gmres(gpu):
+    """aug = 0 runs GMRES's kernels in GMRES's order: the same bits; aug out
+    of range is refused."""
+    import torch
+    from aar_hip._lib import AarHipError
+    shape = (16, 16, 16)
+    n = int(np.prod(shape))
+    a, c0, eta = P.poisson_a(0.5), 0.3, 0.5
+    b = cdev(P.rhs_complex_normal(shape, 1005))
+    u1 = cdev(np.ones(n, dtype=complex))
+    u2 = cdev(np.ones(n, dtype=complex))
+    s1 = gpu.helm_gmres(u1, b, shape, a, c0=c0, eta=eta, max_iter=300, tol=1e-9, restart=12)
+    h1 = gpu.history()
+    s2 = gpu.helm_lgmres(u2, b, shape, a, c0=c0, eta=eta, max_iter=300, tol=1e-9, restart=12, aug=0)
+    assert s1["iterations"] == s2["iterations"] and torch.equal(u1, u2)
+    assert np.array_equal(h1, gpu.history())
+    for aug in (-1, 9, 12):
+        with pytest.raises(AarHipError):
+            gpu.helm_lgmres(u2, b, shape, a, c0=c0, eta=eta, max_iter=10, restart=12, aug=aug)
+
+
 @pytest.mark.parametrize("case", [c for c in HZ_CASES if c[4] == 30], ids=_hid)
 def test_helm_bicg_parity(gpu, oracle, case):
     """BiCG of the complex variant: counts, reasons, norms (1e-6 relative or

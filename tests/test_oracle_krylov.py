@@ -526,6 +526,94 @@ def test_bicg_complex_reduces_to_real_and_solves(oracle):
     assert np.abs(u - ref).max() <= 1e-7 * np.abs(ref).max()
 
 
+# ---------------------------------------------------------------------------
+# LGMRES of the complex PETSc variant (PETSC_COMPLEX/main.c:120-147)
+# ---------------------------------------------------------------------------
+def _helm_dense(oracle, shape, a, c, c0, eta):
+    n = int(np.prod(shape))
+    A = np.empty((n, n), dtype=complex)
+    e = np.zeros(n, dtype=complex)
+    for j in range(n):
+        e[:] = 0
+        e[j] = 1
+        A[:, j] = oracle.matvec_complex(e, shape, a, c, c0, eta)
+    D = a * 3 * oracle.lap_coefficient(6)[0] + (c0 if c is None else c) + 1j * eta
+    return A, D
+
+
+def _krylov(M, r, k):
+    Q = [r / np.linalg.norm(r)]
+    while len(Q) < k:
+        v = M @ Q[-1]
+        for _ in range(2):
+            for q in Q:
+                v = v - np.vdot(q, v) * q
+        Q.append(v / np.linalg.norm(v))
+    return np.array(Q).T
+
+
+def test_lgmres_complex_cycles_minimise_over_the_augmented_space(oracle):
+    """restart 6, aug 2: 4 Arnoldi steps per cycle, then one step per stored
+    error approximation (newest first).  Stopping after a whole cycle (max_it
+    = 4, 9, 15) or after the first augmentation step (14), the iterate is
+    x_prev + W y with y minimising ||M (x_prev + W y) - B b||, W = [K_4(M,
+    r_prev), z newest, ...] and z the earlier cycles' corrections; the last
+    trace entry is that minimum.  A wrong B A z, a dropped or misordered z
+    or a wrong combination fails it."""
+    shape = (8, 8, 8)
+    n = 512
+    a, c0, eta = -4.0, 0.3, 0.5
+    rng = np.random.default_rng(71)
+    c = rng.uniform(0.5, 1.5, n)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    u0 = np.ones(n, dtype=complex)
+    A, D = _helm_dense(oracle, shape, a, c, c0, eta)
+    M = A / D[:, None]
+
+    def run(its):
+        u, d = oracle.lgmres_complex(u0, b, shape, a, c, c0, eta, its, 1e-14, restart=6, aug=2)
+        assert d["iterations"] == its and d["status"] == oracle.KSP_DIVERGED_ITS
+        return u, d
+
+    xs = [u0] + [run(k)[0] for k in (4, 9)]
+    for k, prev, zs in ((4, 0, []), (9, 1, [0]), (15, 2, [1, 0]), (14, 2, [1])):
+        x, d = run(k)
+        xp = xs[prev]
+        r = (b - A @ xp) / D
+        W = _krylov(M, r, 4)
+        for j in zs:
+            W = np.column_stack([W, xs[j + 1] - xs[j]])
+        y, *_ = np.linalg.lstsq(M @ W, r, rcond=None)
+        pred = xp + W @ y
+        assert np.linalg.norm(x - pred) <= 1e-8 * np.linalg.norm(pred), k
+        assert d["trace"][-1] == pytest.approx(np.linalg.norm(r - M @ W @ y), rel=1e-7), k
+    # one trace entry per cycle start and per step: 1+4, 1+5, 1+6
+    assert len(run(15)[1]["trace"]) == 18
+
+
+def test_lgmres_complex_aug0_is_gmres_and_solves(oracle):
+    """aug = 0 is GMRES(restart) bit for bit; with PETSc's defaults (30, 2)
+    LGMRES converges to the FFT solution with Jacobi and with ILU(0)."""
+    shape = (14, 14, 14)
+    n = int(np.prod(shape))
+    a, c0, eta = -4.0, 0.3, 0.5
+    bc = np.random.default_rng(72).standard_normal(n) + 1j * np.random.default_rng(73).standard_normal(n)
+    ug, dg = oracle.gmres_complex(np.ones(n, dtype=complex), bc, shape, a, None, c0, eta, 300, 1e-9, restart=12)
+    ul, dl = oracle.lgmres_complex(np.ones(n, dtype=complex), bc, shape, a, None, c0, eta, 300, 1e-9, restart=12,
+                                   aug=0)
+    assert np.array_equal(ug, ul) and np.array_equal(dg["trace"], dl["trace"])
+    cf = oracle.lap_coefficient(6)
+    def s1(m):
+        th = 2 * np.pi * np.arange(m) / m
+        return cf[0] + 2 * sum(cf[q] * np.cos(q * th) for q in range(1, 7))
+    lam = a * (s1(14)[:, None, None] + s1(14)[None, :, None] + s1(14)[None, None, :]) + c0 + 1j * eta
+    ref = np.fft.ifftn(np.fft.fftn(bc.reshape(14, 14, 14)) / lam).reshape(-1)
+    for pc in (1, 2):
+        u, d = oracle.lgmres_complex(np.ones(n, dtype=complex), bc, shape, a, None, c0, eta, 2000, 1e-10, precond=pc)
+        assert d["status"] == oracle.KSP_CONVERGED_RTOL
+        assert np.abs(u - ref).max() <= 1e-7 * np.abs(ref).max()
+
+
 def generic_ilu0_complex(rows):
     """generic_ilu0 for complex entries; the pivot reciprocal by the naive
     formula (re - i im) / (re^2 + im^2), the oracle's."""
