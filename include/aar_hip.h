@@ -67,8 +67,10 @@ int aar_hip_matvec(const aar_hip_grid *g, const double *x, double *y);
 /* same in FP32 (the arithmetic of the refined solver's inner sweeps) */
 int aar_hip_matvec_f32(const aar_hip_grid *g, const float *x, float *y);
 
-/* AAR.c:37-177 -- x: initial guess in, solution out; history m <= 16 (also
- * for PGR/PL2R and the stand-alone Anderson/Galerkin/L2 steps) */
+/* AAR.c:37-177 -- x: initial guess in, solution out; history m <= 32 (also
+ * for PGR/PL2R, their FP32/mixed forms and the stand-alone
+ * Anderson/Galerkin/L2 steps; the reference takes any m, AAR.c:51-73, and
+ * 32 is the largest m x m system the one-wavefront SVD solves) */
 int aar_hip_aar(const aar_hip_grid *g, double *x, const double *b, double omega, double beta,
                 int m, int p, int max_iter, double tol, int precond, aar_hip_stats *st);
 
@@ -195,7 +197,7 @@ int aar_hip_ilu0_apply(const aar_hip_grid *g, const double *c, const double *r, 
  * update is x += sum_i (-s_i)(beta DF_i + DX_i) column by column, then
  * x += beta f.  Its LAPACKE_dgelsd least squares (rcond = -1,
  * AAR_Real.c:223) is the minimum-norm solve over the singular values above
- * 2^-53 s_max, by the Golub-Reinsch SVD of the other solvers.  m <= 16; ||b - A x|| is checked every p iterations
+ * 2^-53 s_max, by the Golub-Reinsch SVD of the other solvers.  m <= 32; ||b - A x|| is checked every p iterations
  * (aar_hip_history); iterations = iter - 1 as AAR_Real.c:162-166 prints. */
 int aar_hip_aar_petsc(const aar_hip_grid *g, const double *c, double *x, const double *b, double omega,
                       double beta, int m, int p, int max_iter, double tol, int precond, aar_hip_stats *st);
@@ -282,7 +284,7 @@ int aar_hip_helm_ilu0_apply(const aar_hip_helm *h, const double *r, double *z, i
 int aar_hip_helm_grad_c(long n, const double *lambda, const double *u, double *dc);
 
 /* tools.c:52-105 on the GPU: x = pinv(A) b as the reference defines it,
- * A row-major m x m (A[i*m+j] = A[i][j]), m <= 16.  HOST pointers.
+ * A row-major m x m (A[i*m+j] = A[i][j]), m <= 32.  HOST pointers.
  * w_out, v_out may be NULL. */
 int aar_hip_pinv(const double *A, const double *b, double *x, int m, double *w_out, double *v_out);
 

@@ -14,7 +14,8 @@ namespace aar {
 constexpr int R = 6;                 // stencil radius: FD12 (FDn = 6, MPI_REAL/main.c:35)
 constexpr int TX = 64;               // tile width in x: one wavefront
 constexpr int BLOCK = 256;           // threads per block of the streaming (history/dot) kernels
-constexpr int MAXM = 16;             // largest Anderson history m of the real solvers (m x m LS)
+constexpr int MAXM = 32;             // largest Anderson history m of the real solvers (m x m LS)
+constexpr int MAXM_MID = 16;         // histories up to this run the single-pass Gram and the middle update forms
 constexpr int MAXM_FAST = 8;         // histories up to this use the register-resident Gram instantiations
 constexpr int MAXM_CPLX = 16;        // complex solver: m <= 16 (its LS is the 2m x 2m real embedding)
 constexpr int MAXM_BATCH = 8;        // batched solves: m <= 8

@@ -723,7 +723,7 @@ void CG(POISSON *system, aar_op_cb Lap, aar_pc_cb Pre, double a, double *x, doub
 void AndersonExtrapolation(double **DX, double **DF, double *f, double beta, int m, int N, double *am_vec,
                            double **FtF, double *allredvec, double *Ftf, double *svec, aar_mpi_comm)
 {
-    if (m < 1 || m > MAXM) { std::printf("AndersonExtrapolation: libaar_hip supports m <= 16\n"); return; }
+    if (m < 1 || m > MAXM) { std::printf("AndersonExtrapolation: libaar_hip supports m <= 32\n"); return; }
     StepBufs sb;
     if (int rc = step_upload(DX, DF, f, m, N, sb)) { report_failure("AndersonExtrapolation", rc); return; }
     if (int rc = step_gram_ls(0, sb, m, N)) { report_failure("AndersonExtrapolation", rc); return; }
@@ -757,7 +757,7 @@ void AndersonExtrapolation(double **DX, double **DF, double *f, double beta, int
 void Galerkin_Richardson(double **DX, double **DF, double *f, int m, int Np, double **XtF, double *allredvec,
                          double *Xtf, double *svec, aar_mpi_comm)
 {
-    if (m < 1 || m > MAXM) { std::printf("Galerkin_Richardson: libaar_hip supports m <= 16\n"); return; }
+    if (m < 1 || m > MAXM) { std::printf("Galerkin_Richardson: libaar_hip supports m <= 32\n"); return; }
     StepBufs sb;
     if (int rc = step_upload(DX, DF, f, m, Np, sb)) { report_failure("Galerkin_Richardson", rc); return; }
     if (int rc = step_gram_ls(1, sb, m, Np)) { report_failure("Galerkin_Richardson", rc); return; }
@@ -768,7 +768,7 @@ void Galerkin_Richardson(double **DX, double **DF, double *f, int m, int Np, dou
 void L2_Richardson(double **DF, double *f, int m, int Np, double **FtF, double *allredvec, double *Ftf,
                    double *svec, aar_mpi_comm)
 {
-    if (m < 1 || m > MAXM) { std::printf("L2_Richardson: libaar_hip supports m <= 16\n"); return; }
+    if (m < 1 || m > MAXM) { std::printf("L2_Richardson: libaar_hip supports m <= 32\n"); return; }
     StepBufs sb;
     if (int rc = step_upload(nullptr, DF, f, m, Np, sb)) { report_failure("L2_Richardson", rc); return; }
     if (int rc = step_gram_ls(2, sb, m, Np)) { report_failure("L2_Richardson", rc); return; }
@@ -933,7 +933,7 @@ int aar_hip_aar_refined(const aar_hip_grid *g, double *x, const double *b, doubl
 {
     Op op;
     if (int rc = make_op(g, nullptr, op)) return rc;
-    if (m < 1 || m > MAXM) return fail(AAR_HIP_ERR_ARG, "history m must be in 1..16");
+    if (m < 1 || m > MAXM) return fail(AAR_HIP_ERR_ARG, "history m must be in 1..32");
     return aar_refined(op, x, b, omega, beta, m, p, max_outer, max_inner, tol, inner_tol, precond, st);
 }
 

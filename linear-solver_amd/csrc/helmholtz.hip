@@ -448,7 +448,8 @@ __global__ __launch_bounds__(BLOCK) void hanchor_update_kernel(const HAncArgs a)
 
 hipError_t launch_hanchor_update(const HAncArgs &a, hipStream_t st)
 {
-    if (a.ng < 1 || a.ng > NGMAX || a.nc < 0 || a.nc > NCMAX || a.ncol < 0 || a.ncol > MAXM)
+    constexpr int HNG = MAXM_CPLX + 2, HNC = MAXM_CPLX + 1;   // the complex solver's m <= 16
+    if (a.ng < 1 || a.ng > HNG || a.nc < 0 || a.nc > HNC || a.ncol < 0 || a.ncol > MAXM_CPLX)
         return hipErrorInvalidValue;
     if (a.xlo < 0 || a.xhi > a.ng || !a.G || !a.x) return hipErrorInvalidValue;
     if ((a.nc > 0 || a.cout >= 0) && !a.C) return hipErrorInvalidValue;
@@ -465,7 +466,7 @@ hipError_t launch_hanchor_update(const HAncArgs &a, hipStream_t st)
         hipLaunchKernelGGL((hanchor_update_kernel<MAXM_FAST + 2, MAXM_FAST + 1>), dim3((unsigned)nb), dim3(BLOCK), 0,
                            st, a);
     else
-        hipLaunchKernelGGL((hanchor_update_kernel<NGMAX, NCMAX>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a);
+        hipLaunchKernelGGL((hanchor_update_kernel<HNG, HNC>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a);
     return hipGetLastError();
 }
 

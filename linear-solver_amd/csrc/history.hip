@@ -200,12 +200,13 @@ __global__ __launch_bounds__(BLOCK) void gram_finish_kernel(const GramArgs<T> a)
 }
 
 // ---------------------------------------------------------------------------
-// PGR's normal equations at the deep histories (m > 8, e.g. the reference
-// READMEs' m = 13): m^2 + m products per point are too many FP64
-// accumulators for one thread.  Each of the block's four
-// wavefronts owns a quarter of the products and walks the same points (the
-// block's chunks, from the caches after the first wavefront's loads), so
-// HBM reads each ring slot once and no accumulator leaves the registers.
+// Normal equations with more products than one thread's registers hold:
+// PGR's m^2 + m at the deep histories (m > 8, e.g. the reference READMEs'
+// m = 13), and every kind beyond m = 16.  The products are cut into NSL
+// slices of consecutive products; each of a block's four wavefronts owns one
+// slice and walks the block's chunks (the later wavefronts' loads come from
+// the caches).  One launch (pass) covers four slices, so HBM reads each
+// needed ring slot once per pass, and no accumulator leaves the registers.
 // ---------------------------------------------------------------------------
 template <int MM, int KIND>
 struct GramIdx {
@@ -229,15 +230,15 @@ struct GramIdx {
     }
 };
 
-template <typename T, int MM, int KIND, int W>
+template <typename T, int MM, int KIND, int W, int NSL>
 __device__ __forceinline__ void gram_split_part(const GramArgs<T> &a)
 {
     using Ix = GramIdx<MM, KIND>;
     constexpr int NA = Ix::NA;
-    constexpr int NWV = BLOCK / 64;
-    constexpr int PER = (NA + NWV - 1) / NWV;
+    constexpr int PER = (NA + NSL - 1) / NSL;
     constexpr int T0 = W * PER < NA ? W * PER : NA;
     constexpr int T1 = (W + 1) * PER < NA ? (W + 1) * PER : NA;
+    if constexpr (T1 <= T0) return;     // a slice past the last product
     constexpr int NT = T1 > T0 ? T1 - T0 : 1;
     constexpr int NS = MM + 1;
     const int m = a.m, nv = a.nvalid, dir = a.direct;
@@ -339,21 +340,24 @@ __device__ __forceinline__ void gram_split_part(const GramArgs<T> &a)
     }
 }
 
-template <typename T, int MM, int KIND>
+// pass PASS of NPASS: wavefront w owns slice 4 PASS + w of 4 NPASS
+template <typename T, int MM, int KIND, int PASS = 0, int NPASS = 1>
 __global__ __launch_bounds__(BLOCK) void gram_split_kernel(const GramArgs<T> a)
 {
     static_assert(BLOCK == 256, "four wavefronts share the products");
     constexpr int NA = GramIdx<MM, KIND>::NA;
+    constexpr int NSL = 4 * NPASS;
     static_assert(NA <= GRAM_NA_MAX, "partials buffers hold GRAM_NA_MAX per block");
     __shared__ double sums[NA];
     if (a.ctrl->done) return;
     switch (threadIdx.x >> 6) {
-    case 0: gram_split_part<T, MM, KIND, 0>(a); break;
-    case 1: gram_split_part<T, MM, KIND, 1>(a); break;
-    case 2: gram_split_part<T, MM, KIND, 2>(a); break;
-    default: gram_split_part<T, MM, KIND, 3>(a); break;
+    case 0: gram_split_part<T, MM, KIND, 4 * PASS + 0, NSL>(a); break;
+    case 1: gram_split_part<T, MM, KIND, 4 * PASS + 1, NSL>(a); break;
+    case 2: gram_split_part<T, MM, KIND, 4 * PASS + 2, NSL>(a); break;
+    default: gram_split_part<T, MM, KIND, 4 * PASS + 3, NSL>(a); break;
     }
-    if (!a.counter) return;                     // two-pass: gram_finish_kernel follows
+    // two-pass (gram_finish_kernel follows), always so for several passes
+    if (NPASS > 1 || !a.counter) return;
     if (last_block_arrive(a.counter, gridDim.x)) {
         final_sum_vm(a.partials, a.nblocks, NA, sums);
         if (threadIdx.x == 0) {
@@ -363,13 +367,20 @@ __global__ __launch_bounds__(BLOCK) void gram_split_kernel(const GramArgs<T> a)
     }
 }
 
+template <typename T, int MM, int KIND, int NPASS, int PASS = 0>
+void launch_split_passes(const GramArgs<T> &a, dim3 grid, dim3 block, hipStream_t st)
+{
+    hipLaunchKernelGGL((gram_split_kernel<T, MM, KIND, PASS, NPASS>), grid, block, 0, st, a);
+    if constexpr (PASS + 1 < NPASS) launch_split_passes<T, MM, KIND, NPASS, PASS + 1>(a, grid, block, st);
+}
+
 template <typename T>
 hipError_t launch_gram(const GramArgs<T> &a, hipStream_t st)
 {
     if (a.m < 1 || a.m > MAXM || a.nvalid < 0 || a.nvalid > a.m) return hipErrorInvalidValue;
     dim3 grid(a.nblocks), block(BLOCK);
     // the reference's default history m = 7 gets its own instantiation: 35
-    // accumulators and 8 loads per point instead of MAXM's 44 and 9
+    // accumulators and 8 loads per point instead of MAXM_FAST's 44 and 9
     if (a.kind == 0 && a.m == 7) {
         hipLaunchKernelGGL((gram_kernel<T, 7, 0>), grid, block, 0, st, a);
         if (!a.counter) hipLaunchKernelGGL((gram_finish_kernel<T, 7, 0>), dim3(1), block, 0, st, a);
@@ -403,11 +414,26 @@ hipError_t launch_gram(const GramArgs<T> &a, hipStream_t st)
         if (!a.counter) hipLaunchKernelGGL((gram_finish_kernel<T, M, 1>), dim3(1), block, 0, st, a);
     };
     if (a.m <= MAXM_FAST) go(std::integral_constant<int, MAXM_FAST>());
-    else if (a.kind == 1) {
-        if (a.m == 13) go_split(std::integral_constant<int, 13>());
-        else go_split(std::integral_constant<int, MAXM>());
-    } else if (a.m == 13) go(std::integral_constant<int, 13>());
-    else go(std::integral_constant<int, MAXM>());
+    else if (a.m <= MAXM_MID) {
+        if (a.kind == 1) {
+            if (a.m == 13) go_split(std::integral_constant<int, 13>());
+            else go_split(std::integral_constant<int, MAXM_MID>());
+        } else if (a.m == 13) go(std::integral_constant<int, 13>());
+        else go(std::integral_constant<int, MAXM_MID>());
+    } else {
+        // 16 < m <= 32: 560 (AAR, PL2R) or 1056 (PGR) products, four and
+        // eight passes of four slices, then the fixed-order finish
+        switch (a.kind) {
+        case 0: launch_split_passes<T, MAXM, 0, 4>(a, grid, block, st); break;
+        case 1: launch_split_passes<T, MAXM, 1, 8>(a, grid, block, st); break;
+        default: launch_split_passes<T, MAXM, 2, 4>(a, grid, block, st); break;
+        }
+        switch (a.kind) {
+        case 0: hipLaunchKernelGGL((gram_finish_kernel<T, MAXM, 0>), dim3(1), block, 0, st, a); break;
+        case 1: hipLaunchKernelGGL((gram_finish_kernel<T, MAXM, 1>), dim3(1), block, 0, st, a); break;
+        default: hipLaunchKernelGGL((gram_finish_kernel<T, MAXM, 2>), dim3(1), block, 0, st, a); break;
+        }
+    }
     return hipGetLastError();
 }
 
@@ -885,6 +911,7 @@ hipError_t launch_aar_update(const UpdArgs<T> &a, hipStream_t st)
     long nb = (a.n / 2 + BLOCK - 1) / BLOCK;
     nb = nb < 1 ? 1 : (nb > 4096 ? 4096 : nb);
     if (a.m <= MAXM_FAST) hipLaunchKernelGGL((aar_update_kernel<T, MAXM_FAST>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a);
+    else if (a.m <= MAXM_MID) hipLaunchKernelGGL((aar_update_kernel<T, MAXM_MID>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a);
     else hipLaunchKernelGGL((aar_update_kernel<T, MAXM>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a);
     return hipGetLastError();
 }
@@ -1012,6 +1039,9 @@ hipError_t launch_anchor_update(const AncArgs<Tx, Tg> &a, hipStream_t st)
     else if (a.ng <= MAXM_FAST + 2 && a.nc <= MAXM_FAST + 1)
         hipLaunchKernelGGL((anchor_update_kernel<Tx, Tg, MAXM_FAST + 2, MAXM_FAST + 1>), dim3((unsigned)nb), dim3(BLOCK),
                            0, st, a);
+    else if (a.ng <= MAXM_MID + 2 && a.nc <= MAXM_MID + 1)
+        hipLaunchKernelGGL((anchor_update_kernel<Tx, Tg, MAXM_MID + 2, MAXM_MID + 1>), dim3((unsigned)nb), dim3(BLOCK),
+                           0, st, a);
     else
         hipLaunchKernelGGL((anchor_update_kernel<Tx, Tg, NGMAX, NCMAX>), dim3((unsigned)nb), dim3(BLOCK), 0,
                            st, a);
@@ -1123,6 +1153,7 @@ hipError_t launch_pgr_update(const PUpdArgs<T> &a, hipStream_t st)
 {
     if (a.m < 1 || a.m > MAXM || a.nvalid < 0 || a.nvalid > a.m) return hipErrorInvalidValue;
     if (a.m <= MAXM_FAST) hipLaunchKernelGGL((pgr_update_kernel<T, MAXM_FAST>), dim3(a.nblocks), dim3(BLOCK), 0, st, a);
+    else if (a.m <= MAXM_MID) hipLaunchKernelGGL((pgr_update_kernel<T, MAXM_MID>), dim3(a.nblocks), dim3(BLOCK), 0, st, a);
     else hipLaunchKernelGGL((pgr_update_kernel<T, MAXM>), dim3(a.nblocks), dim3(BLOCK), 0, st, a);
     return hipGetLastError();
 }

@@ -511,7 +511,7 @@ int aar_rec(const Op &op, Tx *x, const Tx *b, double omega, double beta, int m, 
     Context &C = ctx();
     if (int rc = context_init()) return rc;
     if (!x || !b) return fail(AAR_HIP_ERR_ARG, "null vector");
-    if (m < 1 || m > MAXM) return fail(AAR_HIP_ERR_ARG, "history m must be in 1..16");
+    if (m < 1 || m > MAXM) return fail(AAR_HIP_ERR_ARG, "history m must be in 1..32");
     if (p < 1) return fail(AAR_HIP_ERR_ARG, "p must be >= 1");
     if (max_iter < 0) return fail(AAR_HIP_ERR_ARG, "max_iter must be >= 0");
     if (op.c && sizeof(Tx) != 8)
@@ -792,7 +792,7 @@ int pgr_solve(const Op &op, double *x, const double *b, double omega, int m, int
     Context &C = ctx();
     if (int rc = context_init()) return rc;
     if (!x || !b) return fail(AAR_HIP_ERR_ARG, "null vector");
-    if (m < 1 || m > MAXM) return fail(AAR_HIP_ERR_ARG, "history m must be in 1..16");
+    if (m < 1 || m > MAXM) return fail(AAR_HIP_ERR_ARG, "history m must be in 1..32");
     if (p < 1) return fail(AAR_HIP_ERR_ARG, "p must be >= 1");
     if (max_iter < 0) return fail(AAR_HIP_ERR_ARG, "max_iter must be >= 0");
     const long n = op.n;

@@ -25,6 +25,15 @@ def golden():
 
 
 @pytest.fixture(scope="session")
+def golden_deep():
+    """The reference solvers at the deep histories 16 < m <= 32
+    (tests/golden/make_golden_deep.py)."""
+    with open(os.path.join(GOLDEN_DIR, "golden_deep.json")) as fp:
+        meta = json.load(fp)
+    return meta, np.load(os.path.join(GOLDEN_DIR, "golden_deep.npz"), allow_pickle=False)
+
+
+@pytest.fixture(scope="session")
 def oracle():
     from oracle import oracle as O
     if not os.path.exists(O.LIB_PATH):

@@ -250,6 +250,37 @@ def test_oracle_bitwise_vs_reference(oracle, golden, name, solver):
         assert float(f"{d['relres']:g}") == ref["relres_printed"]
 
 
+DEEP_CASES = ["m20p6_24", "m32p4_24", "m20p24_24", "m32p11_28x24x20"]
+
+
+@pytest.mark.parametrize("name", DEEP_CASES)
+@pytest.mark.parametrize("solver", ["aar", "pgr", "pl2r"])
+def test_oracle_bitwise_vs_reference_deep(oracle, golden_deep, name, solver):
+    """The deep histories (m = 20, 32; p below and above m + 1): counts,
+    traces and solutions bit-identical to the reference build's."""
+    meta, g = golden_deep
+    c = meta["cases"][name]["params"]
+    shape = tuple(c["shape"])
+    b = P.rhs_uniform(shape, c["seed"])
+    x, d = run_oracle(oracle, solver, c, shape, b, np.zeros(b.size))
+    ref = meta["cases"][name]["solvers"][solver]
+    assert d["iterations"] == ref["iterations"] and d["iterations"] > c["m"]
+    assert bool(d["printed_converged"]) == ref["converged_line"]
+    assert np.array_equal(d["trace"], g[f"{name}__{solver}__trace"])
+    assert np.array_equal(x, g[f"{name}__{solver}__x"])
+
+
+def test_pinv_deep_golden_bitwise(oracle, golden_deep):
+    _, g = golden_deep
+    ia = ib = 0
+    for m in g["pinv_m"]:
+        A = g["pinv_A"][ia:ia + m * m].reshape(m, m)
+        x, _, _ = oracle.pinv_times_vec(A, g["pinv_b"][ib:ib + m])
+        assert np.array_equal(x, g["pinv_x"][ib:ib + m]), m
+        ia += m * m
+        ib += m
+
+
 # ---------------------------------------------------------------------------
 # closed forms and invariants of the converged solutions
 # ---------------------------------------------------------------------------
