@@ -138,7 +138,7 @@ typedef struct {
  * max_iter and tol, run together: every batched kernel covers all
  * instances through a work table and one Anderson cycle is replayed as a
  * captured hipGraph.  mixed = 0: FP64; mixed = 1: the mixed-precision
- * iteration of aar_hip_aar_mixed.  Needs 1 <= p <= m+1 and m <= 8.  stats: `count`
+ * iteration of aar_hip_aar_mixed.  m <= 16, any p >= 1.  stats: `count`
  * entries (solve_ms is the whole batch's).  The batch runs on the library's
  * own stream, ordered with the stream set by aar_hip_set_stream. */
 int aar_hip_aar_batch(const aar_hip_instance *inst, int count, double omega, double beta, int m, int p,

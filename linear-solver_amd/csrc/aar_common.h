@@ -18,7 +18,7 @@ constexpr int MAXM = 32;             // largest Anderson history m of the real s
 constexpr int MAXM_MID = 16;         // histories up to this run the single-pass Gram and the middle update forms
 constexpr int MAXM_FAST = 8;         // histories up to this use the register-resident Gram instantiations
 constexpr int MAXM_CPLX = 16;        // complex solver: m <= 16 (its LS is the 2m x 2m real embedding)
-constexpr int MAXM_BATCH = 8;        // batched solves: m <= 8
+constexpr int MAXM_BATCH = 16;       // batched solves: m <= 16
 constexpr int MAXM_LS = 32;          // largest LS system (complex path embeds m -> 2m)
 // per-block partial sums of the largest Gram instantiations, which every
 // buffer handed to a Gram kernel must hold per block: PGR's full DX^T DF

@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "aar_solver.h"
@@ -214,10 +215,12 @@ __global__ __launch_bounds__(BLOCK) void bgram_finish_kernel(const BInst<double,
 }
 
 // -------------------------------------------------------------------------
-// anchor update at the Anderson iteration k = *clock + jrel (mode 0), or the
+// anchor update at the Anderson iteration k = *clock + jrel (mode 0), the
 // closing x += omega sum_{first <= j <= k} g_j of a solve that ends on
-// Richardson steps (mode 1); p <= m+1, so no residual is folded earlier and
-// the Richardson terms start at the last anchor.
+// Richardson steps (mode 1), or (mode 2, p > m+1 only) x += omega g_j of a
+// Richardson residual j = *clock + jrel that no Anderson window reads: it
+// is folded into the anchor before its ring slot is reused, so the
+// Richardson terms of the next update start after the folded ones.
 // -------------------------------------------------------------------------
 template <typename Tx, typename Tg>
 __global__ __launch_bounds__(BLOCK) void banchor_kernel(const BInst<Tx, Tg> *inst, const WorkItem *work,
@@ -236,9 +239,20 @@ __global__ __launch_bounds__(BLOCK) void banchor_kernel(const BInst<Tx, Tg> *ins
         for (int c = 0; c < NCMAX; c++) s_cc[c] = 0.0;
         // first Richardson increment not yet in the anchor: the iterate after
         // the last Anderson step; iteration 1 is a Richardson step even for
-        // p = 1 (AAR.c:109), whose cycles then start at base = 1
-        const long first = (p == 1 && base == 1) ? 1 : base + 1;
-        if (mode == 0) {
+        // p = 1 (AAR.c:109), whose cycles then start at base = 1.  With
+        // p > m+1 the cycle's residuals base+1 .. base+p-m-1 are folded in
+        // (mode 2) right after the sweep that follows them.
+        const int nfold_cycle = p > m + 1 ? p - m - 1 : 0;
+        const int nfold = mode == 0 ? nfold_cycle : (mode == 1 ? min(jrel, nfold_cycle) : 0);
+        const long first = ((p == 1 && base == 1) ? 1 : base + 1) + nfold;
+        if (mode == 2) {
+            s_gs[0] = (int)(k % S);
+            s_ng = 1;
+            s_xlo = 0;
+            s_xhi = 1;
+            s_nc = 0;
+            s_cout = -1;
+        } else if (mode == 0) {
             const int nv = (int)min((long)m, k - 1);
             const long jlo = min(first, k - nv);
             const int ng = (int)(k - jlo + 1);
@@ -323,8 +337,8 @@ int batch_solve(const aar_hip_instance *ins, int count, double omega, double bet
     Context &C = ctx();
     if (int rc0 = context_init()) return rc0;
     if (!ins || count < 1) return fail(AAR_HIP_ERR_ARG, "empty batch");
-    if (m < 1 || m > MAXM_BATCH) return fail(AAR_HIP_ERR_ARG, "batched solves: history m must be in 1..8");
-    if (p < 1 || p > m + 1) return fail(AAR_HIP_ERR_ARG, "batched solves need 1 <= p <= m+1");
+    if (m < 1 || m > MAXM_BATCH) return fail(AAR_HIP_ERR_ARG, "batched solves: history m must be in 1..16");
+    if (p < 1) return fail(AAR_HIP_ERR_ARG, "p must be >= 1");
     if (max_iter < 0) return fail(AAR_HIP_ERR_ARG, "max_iter must be >= 0");
     int rc = 0;
     std::vector<void *> owned;
@@ -342,7 +356,9 @@ int batch_solve(const aar_hip_instance *ins, int count, double omega, double bet
     hipGraphExec_t exec = nullptr;
     const int S = m + 2;
     const int NC = (p <= m) ? m / p + 1 : 0;
-    const int NA = (m == 7) ? 35 : (MAXM_BATCH * (MAXM_BATCH + 1) / 2 + MAXM_BATCH);
+    // Gram partials per chunk: the accumulators of the instantiation run
+    const int MMI = m == 7 ? 7 : (m <= MAXM_FAST ? MAXM_FAST : MAXM_BATCH);
+    const int NA = MMI * (MMI + 1) / 2 + MMI;
     BShared sh = {m, p, m, omega, beta};
     // graphs cannot be captured on the legacy default stream that framework
     // callers bind: the batch runs on the library's own stream, ordered
@@ -438,20 +454,23 @@ int batch_solve(const aar_hip_instance *ins, int count, double omega, double bet
         auto rec = [&](int jrel) {
             hipLaunchKernelGGL((bstencil_kernel<Tg, Tx, Tg, SMode::AAR_REC>), dim3((unsigned)wrc.size()),
                                dim3(64 * BWY), 0, st, dI, drc, (int)wrc.size(), clock, jrel);
+            // p > m+1: the residual of iteration base + jrel is read by no
+            // Anderson window (AAR.c:97-102 keeps the last m columns)
+            if (p > m + 1 && jrel < p - m)
+                hipLaunchKernelGGL((banchor_kernel<Tx, Tg>), dim3((unsigned)wch.size()), dim3(BLOCK), 0, st, dI, dch,
+                                   clock, jrel, 2, sh);
         };
         auto anderson = [&](int jrel) {
-            if (m == 7)
-                hipLaunchKernelGGL((bgram_kernel<Tg, 7>), dim3((unsigned)wch.size()), dim3(BLOCK), 0, st, ID, IF, dch,
+            auto gram = [&](auto mm) {
+                constexpr int M = decltype(mm)::value;
+                hipLaunchKernelGGL((bgram_kernel<Tg, M>), dim3((unsigned)wch.size()), dim3(BLOCK), 0, st, ID, IF, dch,
                                    clock, jrel, sh);
-            else
-                hipLaunchKernelGGL((bgram_kernel<Tg, MAXM_BATCH>), dim3((unsigned)wch.size()), dim3(BLOCK), 0, st, ID, IF,
-                                   dch, clock, jrel, sh);
-            if (m == 7)
-                hipLaunchKernelGGL((bgram_finish_kernel<Tg, 7>), dim3(count), dim3(BLOCK), 0, st, ID, IF, clock, jrel,
+                hipLaunchKernelGGL((bgram_finish_kernel<Tg, M>), dim3(count), dim3(BLOCK), 0, st, ID, IF, clock, jrel,
                                    sh);
-            else
-                hipLaunchKernelGGL((bgram_finish_kernel<Tg, MAXM_BATCH>), dim3(count), dim3(BLOCK), 0, st, ID, IF, clock,
-                                   jrel, sh);
+            };
+            if (m == 7) gram(std::integral_constant<int, 7>());
+            else if (m <= MAXM_FAST) gram(std::integral_constant<int, MAXM_FAST>());
+            else gram(std::integral_constant<int, MAXM_BATCH>());
             (void)launch_ls_batch(dctrl, count, m, st);
             hipLaunchKernelGGL((banchor_kernel<Tx, Tg>), dim3((unsigned)wch.size()), dim3(BLOCK), 0, st, dI, dch, clock,
                                jrel, 0, sh);

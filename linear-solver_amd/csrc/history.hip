@@ -799,8 +799,9 @@ __global__ __launch_bounds__(64) void ls_batch_kernel(Ctrl *const *ctrls, int n)
 hipError_t launch_ls_batch(Ctrl *const *ctrls, int count, int n, hipStream_t st)
 {
     if (n < 1 || n > MAXM_LS || count < 1) return hipErrorInvalidValue;
-    if (n > 8) return hipErrorInvalidValue;           // batched solves: m <= 8
-    hipLaunchKernelGGL((ls_batch_kernel<8>), dim3(count), dim3(64), 0, st, ctrls, n);
+    if (n > MAXM_BATCH) return hipErrorInvalidValue;
+    if (n <= 8) hipLaunchKernelGGL((ls_batch_kernel<8>), dim3(count), dim3(64), 0, st, ctrls, n);
+    else hipLaunchKernelGGL((ls_batch_kernel<16>), dim3(count), dim3(64), 0, st, ctrls, n);
     return hipGetLastError();
 }
 

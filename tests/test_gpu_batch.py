@@ -2,9 +2,15 @@
 sides solved together (one graph-captured Anderson cycle per replay).
 
 Every instance must give the reference's iteration count (the oracle's,
-bit-faithful to MPI_REAL/AAR.c) and its solution within 1e-8, in FP64 and
-in mixed precision, and the FP64 batch must agree with the single-solve
-path on the same instance.
+bit-faithful to MPI_REAL/AAR.c), in FP64 and in mixed precision.  FP64
+solutions agree with the oracle's within 1e-8 when converged (1e-6 at a
+max_iter stop).  A mixed solution stores the residual history in FP32: with
+a deep history on a slowly converging instance it moves up to ~2e-7 from
+the FP64 iterate, as the single-solve mixed path does on the same
+instance.  Its bar: its true residual meets the tolerance, and it is no
+farther from the FP64 oracle's iterate than the two residuals allow,
+(||r|| + ||r_64||)/lambda_min.  The FP64 batch must agree with the
+single-solve path on the same instance.
 """
 import numpy as np
 import pytest
@@ -40,6 +46,13 @@ SCHEDULES = [  # m, p, max_iter, tol
     (4, 1, 1000, 1e-8),
     (3, 4, 1000, 1e-8),
     (7, 6, 13, 1e-8),
+    # the READMEs' deep history, m = 16, and p > m + 1 (residuals folded
+    # into the anchor between Anderson steps), with a max_iter stop inside
+    # a cycle's folds
+    (13, 11, 1000, 1e-8),
+    (16, 5, 1000, 1e-8),
+    (3, 8, 1000, 1e-8),
+    (4, 9, 40, 1e-12),
 ]
 
 
@@ -60,8 +73,26 @@ def test_batch_vs_oracle(gpu, oracle, sched, mixed):
         assert sts[i]["iterations"] == dr["iterations"]
         assert bool(sts[i]["printed_converged"]) == bool(dr["printed_converged"])
         x = xs[i].cpu().numpy()
-        lim = 1e-8 if dr["printed_converged"] else 1e-6
-        assert np.linalg.norm(x - xr) / np.linalg.norm(xr) <= lim
+        if not mixed:
+            lim = 1e-8 if dr["printed_converged"] else 1e-6
+            assert np.linalg.norm(x - xr) / np.linalg.norm(xr) <= lim
+        elif not dr["printed_converged"]:
+            # a max_iter stop: the same iteration's FP64 iterate (the
+            # single-solve mixed path measures 1.8e-6 on the m = 3, p = 8 stop)
+            assert np.linalg.norm(x - xr) / np.linalg.norm(xr) <= 1e-5
+        else:
+            # two converged iterates of this symmetric operator differ by at
+            # most what their true residuals allow: (||r|| + ||r64||)/lambda_min
+            a = P.poisson_a(h)
+            r = np.linalg.norm(b - oracle.matvec(x, shape, a, sigma=sig))
+            r64 = np.linalg.norm(b - oracle.matvec(xr, shape, a, sigma=sig))
+            cf = oracle.lap_coefficient(6)
+            th = 2 * np.pi / max(shape)
+            lmin = sig if sig > 0 else abs(a * (cf[0] + 2 * sum(cf[q] * np.cos(q * th) for q in range(1, 7))))
+            d = np.linalg.norm((x - x.mean()) - (xr - xr.mean())) if sig == 0 else np.linalg.norm(x - xr)
+            say(f"    mixed vs fp64 {d / np.linalg.norm(xr):.2e}, residual bound {(r + r64) / lmin / np.linalg.norm(xr):.2e}")
+            assert r <= tol * np.linalg.norm(b) * (1 + 1e-12)
+            assert d <= (r + r64) / lmin
 
 
 def test_batch_matches_single_solves(gpu):
