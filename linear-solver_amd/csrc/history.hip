@@ -200,9 +200,8 @@ __global__ __launch_bounds__(BLOCK) void gram_finish_kernel(const GramArgs<T> a)
 }
 
 // ---------------------------------------------------------------------------
-// Normal equations with more products than one thread's registers hold:
-// PGR's m^2 + m at the deep histories (m > 8, e.g. the reference READMEs'
-// m = 13), and every kind beyond m = 16.  The products are cut into NSL
+// Normal equations beyond m = 16 (560 or 1056 products: more than one pass
+// of four wavefronts holds in registers).  The products are cut into NSL
 // slices of consecutive products; each of a block's four wavefronts owns one
 // slice and walks the block's chunks (the later wavefronts' loads come from
 // the caches).  One launch (pass) covers four slices, so HBM reads each
@@ -367,6 +366,132 @@ __global__ __launch_bounds__(BLOCK) void gram_split_kernel(const GramArgs<T> a)
     }
 }
 
+// ---------------------------------------------------------------------------
+// LDS-staged normal equations for the deep histories (8 < m <= 16).  The
+// block loads each chunk of BLOCK points once (one point per thread,
+// coalesced, non-temporal), forms the difference columns DF (and DX for
+// PGR) and f there, and parks them in LDS; each of the four wavefronts then
+// accumulates its quarter of the products over the chunk from LDS.  The
+// next chunk's loads are issued before the current chunk's products, so
+// HBM streams while they run, and HBM reads each needed ring slot once
+// (the wavefront split of gram_split_kernel reads every slot once per
+// wavefront, through L2).
+// ---------------------------------------------------------------------------
+template <typename T, int MM, int KIND>
+struct GramLds {
+    static constexpr int NC = (KIND == 1) ? 2 * MM + 1 : MM + 1;   // parked columns: DF, (DX), f
+    static constexpr int LP = BLOCK;                             // points per chunk
+};
+
+template <typename T, int MM, int KIND, int W>
+__device__ __forceinline__ void gram_lds_part(const GramArgs<T> &a, double (*sv)[BLOCK])
+{
+    using Ix = GramIdx<MM, KIND>;
+    constexpr int NA = Ix::NA;
+    constexpr int NS = MM + 1;
+    constexpr int NC = GramLds<T, MM, KIND>::NC;
+    constexpr int LP = GramLds<T, MM, KIND>::LP;
+    constexpr int PER = (NA + BLOCK / 64 - 1) / (BLOCK / 64);
+    constexpr int T0 = W * PER < NA ? W * PER : NA;
+    constexpr int T1 = (W + 1) * PER < NA ? (W + 1) * PER : NA;
+    constexpr int NT = T1 > T0 ? T1 - T0 : 1;
+    const int nv = a.nvalid, dir = a.direct;
+    const T *D = (KIND == 0) ? a.F : a.Y;
+    const T *pd[NS], *px[NS];
+    bool use[NS];
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+        use[k] = dir ? (k < nv) : (k <= nv);
+        const size_t off = (size_t)(use[k] ? a.slot[k] : 0) * a.ld;
+        pd[k] = D + off;
+        px[k] = a.F + off;
+    }
+    const T *pf = (KIND == 0 ? a.F : a.Y) + (size_t)a.fslot * a.ld;
+    const bool f_is_d0 = (KIND == 0) && !dir && a.fslot == a.slot[0];
+    const long n = a.n;
+    const long nch = (n + LP - 1) / LP;
+    const int lane = threadIdx.x & 63;
+
+    T rd[NS], rx[NS], rf = (T)0, rb = (T)0;
+    auto load = [&](long c) {
+        const long i = c * LP + threadIdx.x;
+        const bool in = i < n;
+#pragma unroll
+        for (int k = 0; k < NS; k++) {
+            rd[k] = (in && use[k]) ? __builtin_nontemporal_load(pd[k] + i) : (T)0;
+            if constexpr (KIND == 1) rx[k] = (in && use[k]) ? __builtin_nontemporal_load(px[k] + i) : (T)0;
+        }
+        rf = in ? (f_is_d0 ? rd[0] : pf[i]) : (T)0;
+        if constexpr (KIND != 0) rb = in ? a.b[i] : (T)0;
+    };
+    // the chunk's columns: DF = f - f_old (DX = x - x_old) by age, and f
+    // (PGR.c:102-103: b - Ax), formed in T as the other Gram kernels do
+    auto park = [&]() {
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int k = 0; k < MM; k++) {
+            sv[k][t] = k < nv ? (dir ? (double)rd[k] : (double)(rd[k] - rd[k + 1])) : 0.0;
+            if constexpr (KIND == 1) sv[MM + k][t] = k < nv ? (dir ? (double)rx[k] : (double)(rx[k] - rx[k + 1])) : 0.0;
+        }
+        if constexpr (KIND != 0) sv[NC - 1][t] = (double)(rb - rf);
+        else sv[NC - 1][t] = (double)rf;
+    };
+
+    double acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[t] = 0.0;
+    long c = blockIdx.x;
+    if (c < nch) load(c);
+    for (; c < nch; c += gridDim.x) {
+        park();
+        __syncthreads();
+        if (c + gridDim.x < nch) load(c + gridDim.x);
+        if constexpr (T1 > T0) {
+            for (int pp = lane; pp < LP; pp += 64) {
+#pragma unroll
+                for (int t = T0; t < T1; t++) {
+                    const int j = Ix::j_of(t), q = Ix::q_of(t);
+                    const double u = (q < 0) ? sv[NC - 1][pp] : (KIND == 1 ? sv[MM + (q < 0 ? 0 : q)][pp]
+                                                                           : sv[q < 0 ? 0 : q][pp]);
+                    const double v = (q < 0 && KIND == 1) ? sv[MM + j][pp] : sv[j][pp];
+                    acc[t - T0] += u * v;
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int t = T0; t < T1; t++) {
+        const double v = wave_sum(acc[t - T0]);
+        if (lane == 0) a.partials[(size_t)t * a.nblocks + blockIdx.x] = v;
+    }
+}
+
+template <typename T, int MM, int KIND>
+__global__ __launch_bounds__(BLOCK) void gram_lds_kernel(const GramArgs<T> a)
+{
+    static_assert(BLOCK == 256, "four wavefronts share the products");
+    constexpr int NA = GramIdx<MM, KIND>::NA;
+    static_assert(NA <= GRAM_NA_MAX, "partials buffers hold GRAM_NA_MAX per block");
+    __shared__ double sv[GramLds<T, MM, KIND>::NC][BLOCK];
+    __shared__ double sums[NA];
+    if (a.ctrl->done) return;
+    switch (threadIdx.x >> 6) {
+    case 0: gram_lds_part<T, MM, KIND, 0>(a, sv); break;
+    case 1: gram_lds_part<T, MM, KIND, 1>(a, sv); break;
+    case 2: gram_lds_part<T, MM, KIND, 2>(a, sv); break;
+    default: gram_lds_part<T, MM, KIND, 3>(a, sv); break;
+    }
+    if (!a.counter) return;                     // two-pass: gram_finish_kernel follows
+    if (last_block_arrive(a.counter, gridDim.x)) {
+        final_sum_vm(a.partials, a.nblocks, NA, sums);
+        if (threadIdx.x == 0) {
+            gram_layout<T, MM, KIND>(a, sums);
+            *a.counter = 0u;
+        }
+    }
+}
+
 template <typename T, int MM, int KIND, int NPASS, int PASS = 0>
 void launch_split_passes(const GramArgs<T> &a, dim3 grid, dim3 block, hipStream_t st)
 {
@@ -403,23 +528,29 @@ hipError_t launch_gram(const GramArgs<T> &a, hipStream_t st)
             }
         }
     };
-    // m > 8.  AAR's and PL2R's triangles (m(m+1)/2 + m sums) still fit one
-    // thread's registers at the exact m = 13 of the READMEs (2.9 ms against
-    // 4.4 ms padded to 16 at 512^3); PGR's square m^2 + m (182 at m = 13)
-    // does not, so its products are shared out over the wavefronts (10.9 ms
-    // against 32 ms unsplit)
-    auto go_split = [&](auto mm) {
+    auto go_lds = [&](auto mm) {
         constexpr int M = decltype(mm)::value;
-        hipLaunchKernelGGL((gram_split_kernel<T, M, 1>), grid, block, 0, st, a);
-        if (!a.counter) hipLaunchKernelGGL((gram_finish_kernel<T, M, 1>), dim3(1), block, 0, st, a);
+        switch (a.kind) {
+        case 0: hipLaunchKernelGGL((gram_lds_kernel<T, M, 0>), grid, block, 0, st, a); break;
+        case 1: hipLaunchKernelGGL((gram_lds_kernel<T, M, 1>), grid, block, 0, st, a); break;
+        default: hipLaunchKernelGGL((gram_lds_kernel<T, M, 2>), grid, block, 0, st, a); break;
+        }
+        if (!a.counter) {
+            switch (a.kind) {
+            case 0: hipLaunchKernelGGL((gram_finish_kernel<T, M, 0>), dim3(1), block, 0, st, a); break;
+            case 1: hipLaunchKernelGGL((gram_finish_kernel<T, M, 1>), dim3(1), block, 0, st, a); break;
+            default: hipLaunchKernelGGL((gram_finish_kernel<T, M, 2>), dim3(1), block, 0, st, a); break;
+            }
+        }
     };
+    // 8 < m <= 16: the LDS-staged form, with an exact instantiation for the
+    // READMEs' m = 13.  At 512^3, m = 13: AAR 2.63 ms against 2.91 ms with
+    // every product in one thread's registers; PGR's square Gram 5.7 ms
+    // against 10.9 ms split over wavefronts reading through L2
     if (a.m <= MAXM_FAST) go(std::integral_constant<int, MAXM_FAST>());
     else if (a.m <= MAXM_MID) {
-        if (a.kind == 1) {
-            if (a.m == 13) go_split(std::integral_constant<int, 13>());
-            else go_split(std::integral_constant<int, MAXM_MID>());
-        } else if (a.m == 13) go(std::integral_constant<int, 13>());
-        else go(std::integral_constant<int, MAXM_MID>());
+        if (a.m == 13) go_lds(std::integral_constant<int, 13>());
+        else go_lds(std::integral_constant<int, MAXM_MID>());
     } else {
         // 16 < m <= 32: 560 (AAR, PL2R) or 1056 (PGR) products, four and
         // eight passes of four slices, then the fixed-order finish
