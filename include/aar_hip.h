@@ -362,11 +362,13 @@ int aar_hip_profile_query(const char *name, int *count, double *total_ms, double
 int aar_hip_last_system(int n, double *G, double *w);
 
 /* Extension (no reference counterpart): the attainable HBM bandwidth, by
- * a plain streaming kernel with 16-byte non-temporal accesses over `bytes`
- * (a positive multiple of 16; 16-byte aligned device buffers): kind 0
- * reads src, 1 writes dst, 2 copies src to dst.  One warm-up launch, then
- * `reps` launches timed with hipEvents on the library stream; *gbps =
- * bytes moved (a copy counts its read and its write) / time, in 1e9 B/s. */
+ * plain streaming kernels with 16-byte accesses over `bytes` (a positive
+ * multiple of 16; 16-byte aligned device buffers): kind 0 reads src, 1
+ * writes dst, 2 copies src to dst.  Three launch shapes (block size,
+ * accesses per thread, chunk per block, store form) are each given one
+ * warm-up launch and `reps` launches timed with hipEvents on the library
+ * stream; *gbps = the best shape's bytes moved (a copy counts its read and
+ * its write) / time, in 1e9 B/s. */
 int aar_hip_stream_bandwidth(int kind, void *dst, const void *src, long bytes, int reps, double *gbps);
 
 /* Release cached device workspace. */

@@ -432,9 +432,9 @@ def profile_query(name):
 
 
 def stream_bandwidth(kind, nbytes=4 << 30, reps=10, dst=None, src=None):
-    """Attainable HBM bandwidth (GB/s) of a streaming kernel with 16-byte
-    non-temporal accesses: kind "read", "write" or "copy" over nbytes
-    (buffers allocated here unless given as device tensors)."""
+    """Attainable HBM bandwidth (GB/s): the best of three launch shapes of a
+    streaming kernel with 16-byte accesses, kind "read", "write" or "copy"
+    over nbytes (buffers allocated here unless given as device tensors)."""
     import torch
     k = {"read": 0, "write": 1, "copy": 2}[kind]
     nbytes = int(nbytes) // 16 * 16
