@@ -184,6 +184,9 @@ struct PUpdArgs {
 };
 template <typename T>
 hipError_t launch_pgr_update(const PUpdArgs<T> &a, hipStream_t st);
+// grid (= partial count) of the PGR update on n points with history m, given
+// the streaming kernels' block count nb
+int pgr_update_blocks(long n, int m, int nb);
 
 // ||v||^2 (or v.w) with a finaliser, accumulated in FP64
 template <typename T>

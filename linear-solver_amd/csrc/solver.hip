@@ -899,7 +899,8 @@ int pgr_solve(const Op &op, double *x, const double *b, double omega, int m, int
             u.Y = Y;
             u.b = b;
             u.ld = ld;
-            u.nblocks = nb;
+            u.nblocks = pgr_update_blocks(n, m, nb);
+            if ((size_t)u.nblocks > need) return fail(AAR_HIP_ERR_ARG, "PGR: grid too large for the partials");
             u.partials = part;
             u.counter = cnt + 3;
             u.ctrl = ctrl;
