@@ -218,6 +218,8 @@ struct CgUpdArgs {
     FinArgs fin = {};
 };
 hipError_t launch_cg_update(const CgUpdArgs &a, hipStream_t st);
+// grid (= number of partial pairs) of the CG update on n points
+int cg_update_blocks(long n);
 
 // ---------------------------------------------------------------------------
 // complex Helmholtz (no MPI_REAL counterpart): A u = a*Lap(u) + (c(x) + i eta) u,

@@ -379,7 +379,9 @@ hipError_t launch_hgram(const HGramArgs &a, hipStream_t st)
 }
 
 // ---------------------------------------------------------------------------
-// complex anchor update (see HAncArgs / AncArgs)
+// complex anchor update (see HAncArgs / AncArgs): one element per thread and
+// one block per BLOCK elements, x read and written non-temporally (the FP64
+// real update's form; a grid stride over 4096 blocks before)
 // ---------------------------------------------------------------------------
 template <int NGM, int NCM>
 __global__ __launch_bounds__(BLOCK) void hanchor_update_kernel(const HAncArgs a)
@@ -409,8 +411,8 @@ __global__ __launch_bounds__(BLOCK) void hanchor_update_kernel(const HAncArgs a)
     __syncthreads();
     const int ng = a.ng, nc = a.nc;
     double2 *co = (a.cout >= 0) ? a.C + (size_t)a.cout * a.ld : nullptr;
-    const long stride = (long)gridDim.x * BLOCK;
-    for (long e = (long)blockIdx.x * BLOCK + threadIdx.x; e < a.n; e += stride) {
+    const long e = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (e < a.n) {
         double sxr = 0.0, sxi = 0.0, scr = 0.0, sci = 0.0;
         // every stream's load issued before the sums (static slot indices)
         double2 gv[NGM], cv[NCM];
@@ -440,8 +442,8 @@ __global__ __launch_bounds__(BLOCK) void hanchor_update_kernel(const HAncArgs a)
                 scr += cr * v.x - ci * v.y;
                 sci += cr * v.y + ci * v.x;
             }
-        const double2 xv = a.x[e];
-        a.x[e] = make_double2(xv.x + (a.omega * sxr + scr), xv.y + (a.omega * sxi + sci));
+        const double2 xv = ld_nt2<double2>(reinterpret_cast<const double *>(a.x + e));
+        st_nt2(reinterpret_cast<double *>(a.x + e), make_double2(xv.x + (a.omega * sxr + scr), xv.y + (a.omega * sxi + sci)));
         if (co) st_nt2(reinterpret_cast<double *>(co + e), make_double2(scr, sci));
     }
 }
@@ -459,7 +461,8 @@ hipError_t launch_hanchor_update(const HAncArgs &a, hipStream_t st)
             a.col_ref[t] >= a.m)
             return hipErrorInvalidValue;
     long nb = (a.n + BLOCK - 1) / BLOCK;
-    nb = nb < 1 ? 1 : (nb > 4096 ? 4096 : nb);
+    if (nb < 1) nb = 1;
+    if (nb > 0x7fffffffL) return hipErrorInvalidValue;
     // histories up to 8 (an (8, 1) form with half the registers measured 4%
     // slower at 256^3), and the deeper ones
     if (a.ng <= MAXM_FAST + 2 && a.nc <= MAXM_FAST + 1)

@@ -1031,7 +1031,8 @@ int cg_solve(const Op &op, double *x, const double *b, int max_iter, double tol,
             u.c = op.c;
             u.precond = precond;
             u.update_r = restart ? 0 : 1;
-            u.nblocks = nb;
+            u.nblocks = cg_update_blocks(n);
+            if ((size_t)u.nblocks * 2 > need) return fail(AAR_HIP_ERR_ARG, "CG: grid too large for the partials");
             u.partials = part;
             u.counter = cnt + 3;
             u.ctrl = ctrl;
