@@ -1063,8 +1063,11 @@ template hipError_t launch_aar_update<float>(const UpdArgs<float> &, hipStream_t
 // Otherwise (FP32 history, 8-byte pair loads): chunks of CHUNK_V pairs,
 // block b taking chunks b, b + grid, ..., over at most 4096 blocks; the
 // one-pair form halves the bytes a block keeps in flight there and measured
-// 2.40 against 1.35 ms.
-template <typename Tx, typename Tg, int NGM, int NCM, int NT, bool ONE>
+// 2.40 against 1.35 ms.  QUAD (FP32 history, FP64 x; the host checked
+// n % 4 == 0 and a 16-byte aligned x): four points per thread, 16-byte
+// history loads, one block per NT quads, every stream non-temporal.  The
+// arithmetic of each point is the pair form's.
+template <typename Tx, typename Tg, int NGM, int NCM, int NT, bool ONE, bool QUAD = false>
 __global__ __launch_bounds__(NT) void anchor_update_kernel(const AncArgs<Tx, Tg> a)
 {
     if (a.ctrl && a.ctrl->done) return;
@@ -1092,6 +1095,57 @@ __global__ __launch_bounds__(NT) void anchor_update_kernel(const AncArgs<Tx, Tg>
     Tg *co = (a.cout >= 0) ? a.C + (size_t)a.cout * a.ld : nullptr;
 
     const long n = a.n;
+    if constexpr (QUAD) {
+        static_assert(sizeof(Tg) == 4 && sizeof(Tx) == 8, "four-point form: FP32 history, FP64 x");
+        typedef float F4 __attribute__((ext_vector_type(4)));
+        typedef double D2 __attribute__((ext_vector_type(2)));
+        const long iq = (long)blockIdx.x * NT + threadIdx.x;
+        if (iq >= (n >> 2)) return;
+        const long e = 4 * iq;
+        F4 v[NGM];
+#pragma unroll
+        for (int i = 0; i < NGM; i++)
+            if (i < ng) v[i] = __builtin_nontemporal_load(reinterpret_cast<const F4 *>(a.G + (size_t)a.gslot[i] * a.ld + e));
+        F4 w[NCM > 0 ? NCM : 1];
+#pragma unroll
+        for (int c = 0; c < NCM; c++)
+            if (c < nc) w[c] = __builtin_nontemporal_load(reinterpret_cast<const F4 *>(a.C + (size_t)a.cslot[c] * a.ld + e));
+        const D2 x01 = __builtin_nontemporal_load(reinterpret_cast<const D2 *>(a.x + e));
+        const D2 x23 = __builtin_nontemporal_load(reinterpret_cast<const D2 *>(a.x + e + 2));
+        double sx[4] = {0.0, 0.0, 0.0, 0.0}, sc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int i = 0; i < NGM; i++)
+            if (i < ng) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const double g = (double)v[i][q];
+                    if (i >= xlo && i < xhi) sx[q] += g;
+                    sc[q] += cg[i] * g;
+                }
+            }
+#pragma unroll
+        for (int c = 0; c < NCM; c++)
+            if (c < nc) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) sc[q] += cc[c] * (double)w[c][q];
+            }
+        D2 o01, o23;
+        o01.x = x01.x + (om * sx[0] + sc[0]);
+        o01.y = x01.y + (om * sx[1] + sc[1]);
+        o23.x = x23.x + (om * sx[2] + sc[2]);
+        o23.y = x23.y + (om * sx[3] + sc[3]);
+        __builtin_nontemporal_store(o01, reinterpret_cast<D2 *>(a.x + e));
+        __builtin_nontemporal_store(o23, reinterpret_cast<D2 *>(a.x + e + 2));
+        if (co) {
+            F4 c4;
+            c4.x = (float)sc[0];
+            c4.y = (float)sc[1];
+            c4.z = (float)sc[2];
+            c4.w = (float)sc[3];
+            __builtin_nontemporal_store(c4, reinterpret_cast<F4 *>(co + e));
+        }
+        return;
+    }
     const long stride = (long)gridDim.x * NT;
     // pairs when n is even and x is aligned to a pair (ring slots always are)
     if ((n & 1) == 0 && ((uintptr_t)a.x % (2 * sizeof(Tx))) == 0) {
@@ -1187,6 +1241,16 @@ hipError_t launch_anchor_update(const AncArgs<Tx, Tg> &a, hipStream_t st)
     };
     if (np > 0x7fffffffL * 256L) return hipErrorInvalidValue;
     constexpr int NT8 = ONE ? 1024 : BLOCK;
+    if constexpr (sizeof(Tg) == 4 && sizeof(Tx) == 8) {
+        // FP32 history, FP64 x: four points per thread where n and x allow
+        // (ring slots are 256-byte aligned)
+        if (a.ng <= 8 && a.nc <= 1 && a.n % 4 == 0 && (uintptr_t)a.x % 16 == 0) {
+            const long nq = a.n / 4;
+            hipLaunchKernelGGL((anchor_update_kernel<Tx, Tg, 8, 1, 1024, false, true>),
+                               dim3((unsigned)((nq + 1023) / 1024)), dim3(1024), 0, st, a);
+            return hipGetLastError();
+        }
+    }
     if (a.ng <= 8 && a.nc <= 1)
         hipLaunchKernelGGL((anchor_update_kernel<Tx, Tg, 8, 1, NT8, ONE>), dim3(blocks(NT8)), dim3(NT8), 0, st, a);
     else if (a.ng <= MAXM_FAST + 2 && a.nc <= MAXM_FAST + 1)
