@@ -1227,9 +1227,11 @@ hipError_t launch_anchor_update(const AncArgs<Tx, Tg> &a, hipStream_t st)
             return hipErrorInvalidValue;
     // FP64 history: one block per NT pairs, 1024-thread blocks where the
     // terms are at most 8 residuals and one correction (m = 7, p = 6 among
-    // them), 256 where the term registers are many.  FP32 history: chunk
-    // walk over at most 4096 blocks.  The scalar fallback (odd n, x off a
-    // pair) walks a grid stride over the same grid.
+    // them), 256 where the term registers are many.  FP32 history with FP64
+    // x: four points per thread in that common form when n % 4 == 0 and x
+    // is 16-byte aligned; otherwise a chunk walk over at most 4096 blocks.
+    // The scalar fallback (odd n, x off a pair) walks a grid stride over
+    // the same grid.
     constexpr bool ONE = sizeof(Tg) == 8;
     const long np = (a.n + 1) / 2;
     auto blocks = [np](long nt) {
