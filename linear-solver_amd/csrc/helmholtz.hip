@@ -379,11 +379,13 @@ hipError_t launch_hgram(const HGramArgs &a, hipStream_t st)
 }
 
 // ---------------------------------------------------------------------------
-// complex anchor update (see HAncArgs / AncArgs): one element per thread and
-// one block per BLOCK elements, x read and written non-temporally (the FP64
-// real update's form; a grid stride over 4096 blocks before)
+// complex anchor update (see HAncArgs / AncArgs).  ONE (histories up to 8):
+// one element per thread and one block per BLOCK elements, x read and
+// written non-temporally, as the FP64 real update.  The deeper form keeps a
+// grid stride over at most 4096 blocks (its registers leave too few waves
+// for one element per thread, as measured for the real update).
 // ---------------------------------------------------------------------------
-template <int NGM, int NCM>
+template <int NGM, int NCM, bool ONE>
 __global__ __launch_bounds__(BLOCK) void hanchor_update_kernel(const HAncArgs a)
 {
     __shared__ double s_cg[2 * NGMAX], s_cc[2 * NCMAX];
@@ -411,8 +413,8 @@ __global__ __launch_bounds__(BLOCK) void hanchor_update_kernel(const HAncArgs a)
     __syncthreads();
     const int ng = a.ng, nc = a.nc;
     double2 *co = (a.cout >= 0) ? a.C + (size_t)a.cout * a.ld : nullptr;
-    const long e = (long)blockIdx.x * BLOCK + threadIdx.x;
-    if (e < a.n) {
+    const long stride = ONE ? a.n : (long)gridDim.x * BLOCK;
+    for (long e = (long)blockIdx.x * BLOCK + threadIdx.x; e < a.n; e += stride) {
         double sxr = 0.0, sxi = 0.0, scr = 0.0, sci = 0.0;
         // every stream's load issued before the sums (static slot indices)
         double2 gv[NGM], cv[NCM];
@@ -442,8 +444,14 @@ __global__ __launch_bounds__(BLOCK) void hanchor_update_kernel(const HAncArgs a)
                 scr += cr * v.x - ci * v.y;
                 sci += cr * v.y + ci * v.x;
             }
-        const double2 xv = ld_nt2<double2>(reinterpret_cast<const double *>(a.x + e));
-        st_nt2(reinterpret_cast<double *>(a.x + e), make_double2(xv.x + (a.omega * sxr + scr), xv.y + (a.omega * sxi + sci)));
+        if (ONE) {
+            const double2 xv = ld_nt2<double2>(reinterpret_cast<const double *>(a.x + e));
+            st_nt2(reinterpret_cast<double *>(a.x + e),
+                   make_double2(xv.x + (a.omega * sxr + scr), xv.y + (a.omega * sxi + sci)));
+        } else {
+            const double2 xv = a.x[e];
+            a.x[e] = make_double2(xv.x + (a.omega * sxr + scr), xv.y + (a.omega * sxi + sci));
+        }
         if (co) st_nt2(reinterpret_cast<double *>(co + e), make_double2(scr, sci));
     }
 }
@@ -463,13 +471,14 @@ hipError_t launch_hanchor_update(const HAncArgs &a, hipStream_t st)
     long nb = (a.n + BLOCK - 1) / BLOCK;
     if (nb < 1) nb = 1;
     if (nb > 0x7fffffffL) return hipErrorInvalidValue;
+    const long nstride = std::min(nb, 4096L);
     // histories up to 8 (an (8, 1) form with half the registers measured 4%
     // slower at 256^3), and the deeper ones
     if (a.ng <= MAXM_FAST + 2 && a.nc <= MAXM_FAST + 1)
-        hipLaunchKernelGGL((hanchor_update_kernel<MAXM_FAST + 2, MAXM_FAST + 1>), dim3((unsigned)nb), dim3(BLOCK), 0,
-                           st, a);
+        hipLaunchKernelGGL((hanchor_update_kernel<MAXM_FAST + 2, MAXM_FAST + 1, true>), dim3((unsigned)nb), dim3(BLOCK),
+                           0, st, a);
     else
-        hipLaunchKernelGGL((hanchor_update_kernel<HNG, HNC>), dim3((unsigned)nb), dim3(BLOCK), 0, st, a);
+        hipLaunchKernelGGL((hanchor_update_kernel<HNG, HNC, false>), dim3((unsigned)nstride), dim3(BLOCK), 0, st, a);
     return hipGetLastError();
 }
 

@@ -1057,7 +1057,8 @@ template hipError_t launch_aar_update<float>(const UpdArgs<float> &, hipStream_t
 // arguments, ctrl->s) with static indices only, so they stay in registers
 // without spills; NGM/NCM bound the term counts at compile time.
 // ---------------------------------------------------------------------------
-// Walks.  ONE (FP64 history): one pair per thread and one block per NT pairs,
+// Walks.  ONE (FP64 history, at most 8 residual terms and one correction):
+// one pair per thread and one block per NT pairs,
 // x read and written non-temporally (it is next read a whole cycle later);
 // at 512^3 with m = 7, p = 6 this took the update from 2.46 to 2.17 ms.
 // Otherwise (FP32 history, 8-byte pair loads): chunks of CHUNK_V pairs,
@@ -1225,9 +1226,8 @@ hipError_t launch_anchor_update(const AncArgs<Tx, Tg> &a, hipStream_t st)
         if (a.col_j[t] < 1 || a.col_j[t] >= a.ng || a.col_c[t] >= a.nc || a.col_ref[t] < 0 ||
             a.col_ref[t] >= MAXM)
             return hipErrorInvalidValue;
-    // FP64 history: one block per NT pairs, 1024-thread blocks where the
-    // terms are at most 8 residuals and one correction (m = 7, p = 6 among
-    // them), 256 where the term registers are many.  FP32 history with FP64
+    // FP64 history: one block per 1024 pairs where the terms are at most 8
+    // residuals and one correction (m = 7, p = 6 among them).  FP32 history with FP64
     // x: four points per thread in that common form when n % 4 == 0 and x
     // is 16-byte aligned; otherwise a chunk walk over at most 4096 blocks.
     // The scalar fallback (odd n, x off a pair) walks a grid stride over
@@ -1253,17 +1253,21 @@ hipError_t launch_anchor_update(const AncArgs<Tx, Tg> &a, hipStream_t st)
             return hipGetLastError();
         }
     }
+    // the wider forms keep the chunk walk: at 100-256 VGPRs (2-3 waves per
+    // SIMD) one pair per thread keeps too few bytes in flight (m = 13,
+    // p = 11 at 512^3: 7.6 ms against 2.0 ms for the walk)
+    const long nwalk = std::max(1L, std::min(4096L, (np + CHUNK_V - 1) / CHUNK_V));
     if (a.ng <= 8 && a.nc <= 1)
         hipLaunchKernelGGL((anchor_update_kernel<Tx, Tg, 8, 1, NT8, ONE>), dim3(blocks(NT8)), dim3(NT8), 0, st, a);
     else if (a.ng <= MAXM_FAST + 2 && a.nc <= MAXM_FAST + 1)
-        hipLaunchKernelGGL((anchor_update_kernel<Tx, Tg, MAXM_FAST + 2, MAXM_FAST + 1, BLOCK, ONE>),
-                           dim3(blocks(BLOCK)), dim3(BLOCK), 0, st, a);
+        hipLaunchKernelGGL((anchor_update_kernel<Tx, Tg, MAXM_FAST + 2, MAXM_FAST + 1, BLOCK, false>),
+                           dim3((unsigned)nwalk), dim3(BLOCK), 0, st, a);
     else if (a.ng <= MAXM_MID + 2 && a.nc <= MAXM_MID + 1)
-        hipLaunchKernelGGL((anchor_update_kernel<Tx, Tg, MAXM_MID + 2, MAXM_MID + 1, BLOCK, ONE>),
-                           dim3(blocks(BLOCK)), dim3(BLOCK), 0, st, a);
+        hipLaunchKernelGGL((anchor_update_kernel<Tx, Tg, MAXM_MID + 2, MAXM_MID + 1, BLOCK, false>),
+                           dim3((unsigned)nwalk), dim3(BLOCK), 0, st, a);
     else
-        hipLaunchKernelGGL((anchor_update_kernel<Tx, Tg, NGMAX, NCMAX, BLOCK, ONE>), dim3(blocks(BLOCK)), dim3(BLOCK),
-                           0, st, a);
+        hipLaunchKernelGGL((anchor_update_kernel<Tx, Tg, NGMAX, NCMAX, BLOCK, false>), dim3((unsigned)nwalk),
+                           dim3(BLOCK), 0, st, a);
     return hipGetLastError();
 }
 template hipError_t launch_anchor_update<double, double>(const AncArgs<double, double> &, hipStream_t);
