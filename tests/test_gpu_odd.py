@@ -5,7 +5,9 @@ element-by-element walk instead.  Both are solved here against the oracle
 (AAR.c:37-177, PGR.c:35-218, PL2R.c, CG.c:33-175 restated): the oracle's
 iteration counts and printed conditions, solutions within 1e-8 (CG 1e-7,
 the reference's own rank spread at these sizes), residual traces within
-1e-6.
+1e-6.  The mixed-precision anchor update's three walks (four points, pairs,
+single points) against the FP64 oracle: its count within one Anderson
+period, its solution within 1e-7 at tol 1e-10.
 """
 import numpy as np
 import pytest
@@ -96,3 +98,30 @@ def test_cg_variable_coefficient_walks(gpu, oracle, offset):
     st = gpu.CG(x, _dev_offset(b, 0), shape, a, 3000, 1e-8, c=_dev_offset(c, 0))
     _check(st, dr, x.cpu().numpy(), xr, "cg")
     _trace(gpu, dr, "cg")
+
+
+# mixed precision: the anchor update streams four points per thread when the
+# point count is a multiple of 4 and x is on a 16-byte boundary; otherwise
+# pairs (n % 4 == 2) or single points (odd n, x off the boundary)
+MIXED_CASES = [((16, 14, 12), 0, "quads"), ((14, 13, 13), 0, "pairs_n_mod4_2"), (ODD, 0, "odd_n"),
+               ((16, 14, 12), 1, "x_off_16B")]
+
+
+@pytest.mark.parametrize("shape,offset,walk", MIXED_CASES, ids=[c[2] for c in MIXED_CASES])
+def test_mixed_anchor_walks_vs_oracle(gpu, oracle, shape, offset, walk):
+    """FP32 sweeps and history with FP64 true residuals, on each walk of the
+    anchor update: the FP64 oracle's count within one Anderson period p,
+    and its solution within 1e-7 at tol 1e-10 (the bar of the deep-history
+    mixed test)."""
+    say("elementwise mixed", walk, shape, "offset", offset)
+    n = int(np.prod(shape))
+    a = P.poisson_a(0.5)
+    b = P.rhs_normal(shape, 2029)
+    b = b - b.mean()
+    x = _dev_offset(np.zeros(n), offset)
+    xr, dr = oracle.aar(np.zeros(n), b, shape, a, 0.6, 0.6, 7, 6, 3000, 1e-10, trace=False)
+    st = gpu.AAR_mixed(x, _dev_offset(b, 0), shape, a, 0.6, 0.6, 7, 6, 3000, 1e-10)
+    rel = np.linalg.norm(x.cpu().numpy() - xr) / np.linalg.norm(xr)
+    say(f"  mixed {st['iterations']} oracle {dr['iterations']} rel diff {rel:.2e}")
+    assert st["converged"] and abs(st["iterations"] - dr["iterations"]) <= 6
+    assert rel <= 1e-7
