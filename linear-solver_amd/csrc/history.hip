@@ -1057,17 +1057,21 @@ template hipError_t launch_aar_update<float>(const UpdArgs<float> &, hipStream_t
 // arguments, ctrl->s) with static indices only, so they stay in registers
 // without spills; NGM/NCM bound the term counts at compile time.
 // ---------------------------------------------------------------------------
-// Walks.  ONE (FP64 history, at most 8 residual terms and one correction):
-// one pair per thread and one block per NT pairs,
-// x read and written non-temporally (it is next read a whole cycle later);
-// at 512^3 with m = 7, p = 6 this took the update from 2.46 to 2.17 ms.
-// Otherwise (FP32 history, 8-byte pair loads): chunks of CHUNK_V pairs,
-// block b taking chunks b, b + grid, ..., over at most 4096 blocks; the
-// one-pair form halves the bytes a block keeps in flight there and measured
-// 2.40 against 1.35 ms.  QUAD (FP32 history, FP64 x; the host checked
-// n % 4 == 0 and a 16-byte aligned x): four points per thread, 16-byte
-// history loads, one block per NT quads, every stream non-temporal.  The
-// arithmetic of each point is the pair form's.
+// Walks.  The arithmetic of each point is the same in every walk.
+// * ONE (FP64 history, at most 8 residual terms and one correction): one
+//   pair per thread and one block per NT pairs, x read and written
+//   non-temporally (it is next read a whole cycle later); at 512^3 with
+//   m = 7, p = 6 this took the update from 2.46 to 2.17 ms.
+// * QUAD (FP32 history, FP64 x, the same term counts; the host checked
+//   n % 4 == 0 and a 16-byte aligned x): four points per thread, 16-byte
+//   history loads, one block per NT quads, every stream non-temporal
+//   (1.35 -> 1.29 ms).  One pair per thread there halves the bytes a block
+//   keeps in flight: 2.40 ms.
+// * Otherwise (the wider forms, and FP32 history off those conditions):
+//   chunks of CHUNK_V pairs, block b taking chunks b, b + grid, ..., over at
+//   most 4096 blocks.  At 100-256 VGPRs one pair per thread measured 7.6
+//   against 3.6 ms (m = 13, p = 11).
+// * Odd n, or x off a pair: a grid stride of single points.
 template <typename Tx, typename Tg, int NGM, int NCM, int NT, bool ONE, bool QUAD = false>
 __global__ __launch_bounds__(NT) void anchor_update_kernel(const AncArgs<Tx, Tg> a)
 {
