@@ -580,8 +580,9 @@ def main():
 
     # the whole FP64 solve against the roofline: the library's algorithmic
     # byte count of the solve (per Anderson cycle at m = 7, p = 6: 5
-    # recurrence sweeps x 2 vectors, the Gram 8, the anchor update 13, the
-    # true residual 3; 34 vectors) over the solve's wall time
+    # recurrence sweeps x 2 vectors, the Gram 8, the anchor update 12 (x read
+    # and written, 8 residual slots, one correction read and one written),
+    # the true residual 3; 33 vectors) over the solve's wall time
     solve_gbs = st64["bytes"] / t64 / 1e9
 
     # GPU FP64 AAR at the CPU baseline's sizes (same problem scheme)
@@ -663,7 +664,7 @@ def main():
         "solve_roofline": {"algorithmic_bytes": st64["bytes"], "seconds": t64, "GBps": solve_gbs,
                            "frac": solve_gbs / HBM_PEAK_GBS,
                            "byte_model": "per Anderson cycle (m=7, p=6): 5 recurrence sweeps x 2 vectors, "
-                                         "Gram 8, anchor update 13, true residual 3 = 34 vectors of 8 B/pt"},
+                                         "Gram 8, anchor update 12, true residual 3 = 33 vectors of 8 B/pt"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
