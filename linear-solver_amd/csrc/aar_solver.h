@@ -96,6 +96,8 @@ hipError_t ensure(DevBuf &b, size_t bytes);
 int auto_lz(const Op &op, int wy, int py, size_t es);
 long ring_ld(long n, size_t es);
 int nblocks_for(long n);
+// grid of the register Gram (m <= 8) on n points
+int gram_blocks(long n, int m);
 template <typename T>
 SArgs<T> base_sargs(const Op &op);
 template <typename T>
