@@ -1432,8 +1432,10 @@ int ilu0_apply(const Op &op, const double *r, double *z, int trans)
 // arithmetic on this symmetric matrix) or none (0).  A plain iteration on
 // x: each step applies B to r = b - A x, updates the history columns,
 // takes the Richardson or Anderson step and recomputes r; the Anderson
-// least squares uses the Gram and pseudo-inverse kernels of the MPI_REAL
-// solver (the tools.c SVD in place of the variant's dgelsd).
+// least squares uses the Gram and one-wavefront SVD kernels of the
+// MPI_REAL solver with dgelsd's rcond = -1 cutoff (singular values at or
+// below eps * s_max dropped, LS_GELSD), as the variant's LAPACKE_dgelsd
+// call does (PETSC_REAL/AAR_Real.c:223).
 int aar_petsc_solve(const Op &op, double *x, const double *b, double omega, double beta, int m, int p,
                     int max_iter, double tol, int precond, aar_hip_stats *st)
 {
@@ -1566,7 +1568,8 @@ int aar_petsc_solve(const Op &op, double *x, const double *b, double omega, doub
 // PGR (l2 = 0) / PL2R (l2 = 1) of the PETSc variant, PGR_Real.c:32-195 /
 // PL2R_Real.c:32-197, preconditioner as aar_petsc_solve.  The normal
 // equations come from the Gram kernel (kind 1: DX^T DF, kind 2: DF^T DF,
-// with res = b - Ax formed in its loads) and the tools.c pseudo-inverse.
+// with res = b - Ax formed in its loads) and the one-wavefront SVD with
+// dgelsd's rcond = -1 cutoff (LS_GELSD; PGR_Real.c:216, PL2R_Real.c:220).
 int pgr_petsc_solve(const Op &op, double *x, const double *b, double omega, int m, int p, int max_iter, double tol,
                     int precond, int l2, aar_hip_stats *st)
 {

@@ -1456,8 +1456,9 @@ void orc_ilu0_apply(int nx, int ny, int nz, const double *F, const double *in, d
 /* (AAR_Real.c:116); the update is x += sum_i (-s_i)(beta DF_i + DX_i) */
 /* one column at a time, then x += beta f (AAR_Real.c:132-137); the    */
 /* least-squares solve is MKL's dgelsd (AAR_Real.c:223), absent here:  */
-/* the tools.c pseudo-inverse stands in for it (the two agree to       */
-/* rounding on a full-rank Gram; parity with dgelsd is unpinned).      */
+/* restated as orc_pinv_times_vec_gelsd, the minimum-norm solution     */
+/* over singular values above eps * s_max (rcond = -1); parity with    */
+/* dgelsd's own arithmetic is unpinned.                                */
 /* ------------------------------------------------------------------ */
 int orc_aar_petsc(int nx, int ny, int nz, int fdn, double a, double sigma, const double *cv,
                   double *x, const double *b, double omega, double beta, int m, int p, int max_iter,
@@ -1541,7 +1542,8 @@ int orc_aar_petsc(int nx, int ny, int nz, int fdn, double a, double sigma, const
 /* s_i DX_i, DX_k = x - x_old, Ax += sum s_i DF_i, DF_k = Ax - Ax_prev;*/
 /* res = b - Ax, its norm checked every p.  The m x m system is        */
 /* A[i][j] = DX_i . DF_j (PGR; PL2R: DF_i . DF_j), rhs DX_i . res (DF_i */
-/* . res), solved by the tools.c pseudo-inverse in place of dgelsd.    */
+/* . res), solved with dgelsd's rcond = -1 cutoff (eps * s_max,       */
+/* orc_pinv_times_vec_gelsd).                                          */
 /* The variant's column-major dgelsd call receives the transpose of    */
 /* PGR's matrix; for these symmetric operators DF = A DX makes the two */
 /* equal in exact arithmetic.                                          */
