@@ -320,7 +320,8 @@ int aar_hip_set_stream(void *stream);
  * 8 with wy = 4);
  * a block owns a 64 x (wy*py) column.  Applies to subsequent calls and to
  * every grid; (0, 8, 4) returns to the automatic shapes, tuned ones where
- * a grid has been tuned. */
+ * a grid has been tuned.  Batched solves (aar_hip_aar_batch) take lz,
+ * clamped to each instance's nz, and keep their 8 x 4 launch shape. */
 int aar_hip_set_tuning(int lz, int wy, int py);
 
 /* Autotuner (extension): times the recurrence stencil sweep on an nx x ny x

@@ -400,8 +400,12 @@ int batch_solve(const aar_hip_instance *ins, int count, double omega, double bet
         if (NC > 0) BHC(bsp_alloc(&I.C, (size_t)I.ld * NC, owned));
         BHC(bsp_alloc(&I.hist, (size_t)I.hist_cap, owned));
         // stencil args (auto z-chunk for this grid and the batch launch shape)
-        // slab depth: tuned for the batched 8 x 4 launch where available
+        // slab depth: the one fixed by aar_hip_set_tuning (its wy, py do not
+        // apply: the batch launch shape is 8 x 4), else tuned for the batched
+        // launch where available, else automatic; stencil_geometry clamps
+        // it to the instance's nz
         auto batch_lz = [&](size_t es) {
+            if (C.tuning_set && C.lz > 0) return C.lz;
             auto t = C.tuned_batch.find({op.nx, op.ny, op.nz, (int)es});
             return t != C.tuned_batch.end() ? t->second.lz : auto_lz(op, BWY, BPY, es);
         };

@@ -134,3 +134,26 @@ def test_autotune_shapes_keep_parity(gpu, oracle):
     xb = dev(np.zeros(b.size))
     sts = gpu.AAR_batch([(xb, dev(b), shape, -4.0, 0.0)], 0.6, 0.6, 7, 6, 1000, 1e-8)
     assert sts[0]["iterations"] == dr["iterations"]
+
+
+@pytest.mark.parametrize("lz", [8, 20, 64], ids=lambda v: f"lz{v}")
+def test_batch_fixed_slab_depth_vs_oracle(gpu, oracle, lz):
+    """A slab depth fixed by set_tuning reaches the batch (clamped to each
+    instance's nz; 64 is one slab per column for every instance here): the
+    reference's iteration count and the oracle's iterate within 1e-8."""
+    inst = instances()
+    xs = [dev(np.zeros(b.size)) for (_, _, _, b) in inst]
+    bs = [dev(b) for (_, _, _, b) in inst]
+    args = [(xs[i], bs[i], shape, P.poisson_a(h), sig) for i, (shape, h, sig, _) in enumerate(inst)]
+    say("batch with lz", lz)
+    gpu.set_tuning(lz, 8, 4)
+    try:
+        sts = gpu.AAR_batch(args, 0.6, 0.6, 7, 6, 1000, 1e-8)
+    finally:
+        gpu.set_tuning(0, 8, 4)
+    for i, (shape, h, sig, b) in enumerate(inst):
+        xr, dr = oracle.aar(np.zeros(b.size), b, shape, P.poisson_a(h), 0.6, 0.6, 7, 6, 1000, 1e-8,
+                            sigma=sig, trace=False)
+        assert sts[i]["iterations"] == dr["iterations"], (i, shape)
+        x = xs[i].cpu().numpy()
+        assert np.linalg.norm(x - xr) / np.linalg.norm(xr) <= 1e-8, (i, shape)
