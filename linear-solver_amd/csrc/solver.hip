@@ -216,7 +216,9 @@ int nblocks_for(long n)
 // Grid of the register Gram (m <= 8): two blocks per CU unless the caller
 // set "hist_blocks".  At 512^3, m = 7 the FP64 Gram took 1.22 ms on 512
 // blocks against 1.28 ms on 2048 (FP32 history 0.71 against 0.75 ms; 256
-// blocks 1.39 ms).  The deeper Grams keep the streaming kernels' grid.
+// blocks 1.39 ms).  Between: 384 blocks 1.22 ms (FP32 history 0.82 ms),
+// 768 blocks 1.25 ms (0.74 ms), 1024 blocks 1.26 ms (0.83 ms).  The deeper
+// Grams keep the streaming kernels' grid.
 int gram_blocks(long n, int m)
 {
     const int nb = nblocks_for(n);
