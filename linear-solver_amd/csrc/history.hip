@@ -122,11 +122,12 @@ __global__ __launch_bounds__(BLOCK) void gram_kernel(const GramArgs<T> a)
     // vectors, each contiguous (measured 13% faster at 512^3 than a grid
     // stride of one vector per thread: every CU streams contiguous memory).
     // VW consecutive points per 16-byte load when N is a multiple of VW
-    // (every ring slot is 256-byte aligned).  For the AAR ring the newest
-    // slot is both the age-0 column end and f: one load.
+    // (every ring slot is 256-byte aligned).  The newest slot is both the
+    // age-0 column end and f (the residual ring of AAR; the A x ring of
+    // PGR/PL2R, f = b - A x): one load.
     constexpr int VW = 16 / sizeof(T);
     using VT = T __attribute__((ext_vector_type(VW)));
-    const bool f_is_d0 = (KIND == 0) && !dir && a.fslot == a.slot[0];
+    const bool f_is_d0 = !dir && a.fslot == a.slot[0];
     const long n = a.n;
     const long stride = (long)gridDim.x * BLOCK;
     const bool vec_ok = (n % VW) == 0 && (KIND == 0 || ((uintptr_t)a.b % 16) == 0);
@@ -253,7 +254,7 @@ __device__ __forceinline__ void gram_split_part(const GramArgs<T> &a)
         px[k] = a.F + off;
     }
     const T *pf = (KIND == 0 ? a.F : a.Y) + (size_t)a.fslot * a.ld;
-    const bool f_is_d0 = (KIND == 0) && !dir && a.fslot == a.slot[0];
+    const bool f_is_d0 = !dir && a.fslot == a.slot[0];
     double acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; t++) acc[t] = 0.0;
@@ -407,7 +408,7 @@ __device__ __forceinline__ void gram_lds_part(const GramArgs<T> &a, double (*sv)
         px[k] = a.F + off;
     }
     const T *pf = (KIND == 0 ? a.F : a.Y) + (size_t)a.fslot * a.ld;
-    const bool f_is_d0 = (KIND == 0) && !dir && a.fslot == a.slot[0];
+    const bool f_is_d0 = !dir && a.fslot == a.slot[0];
     const long n = a.n;
     const long nch = (n + LP - 1) / LP;
     const int lane = threadIdx.x & 63;
